@@ -1,0 +1,117 @@
+// Shared device-side definitions for the gfx950 GARM-LS kernels.
+//
+// Layout: every field lives in HBM as structure-of-arrays, one contiguous
+// x-fastest array per scalar component (the reference stores vectors and
+// matrices per cell, GridBasedData.h:22-24; the index order is Grid::index,
+// Grid.h:40-44).  Sampling clamps the coordinate into the grid exactly like
+// GridBasedData::at (GridBasedData.h:47).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace garm {
+
+// One regular grid (Grid.h:10-64): sizes, origin, spacing.  Passed by value.
+struct GridDev {
+	int n[3];
+	double o[3];
+	double dx, inv;
+};
+
+constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
+constexpr int kWave = 64;
+
+template <int D>
+__device__ __forceinline__ size_t gidx(const GridDev &g, int i, int j, int k)
+{
+	if constexpr (D == 2) return size_t(i) + size_t(g.n[0]) * size_t(j);
+	else return size_t(i) + size_t(g.n[0]) * (size_t(j) + size_t(g.n[1]) * size_t(k));
+}
+
+__device__ __forceinline__ int clampi(int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); }
+
+// GridBasedData::at: index of the clamped coordinate.
+template <int D>
+__device__ __forceinline__ size_t gat(const GridDev &g, int i, int j, int k)
+{
+	if constexpr (D == 2) return gidx<2>(g, clampi(i, g.n[0] - 1), clampi(j, g.n[1] - 1), 0);
+	else return gidx<3>(g, clampi(i, g.n[0] - 1), clampi(j, g.n[1] - 1), clampi(k, g.n[2] - 1));
+}
+
+// Cell/face coordinate of this thread in a (64, 4, 1) x (.., .., n2) launch.
+struct Coord { int i, j, k; };
+
+__device__ __forceinline__ Coord threadCoord()
+{
+	return Coord{int(blockIdx.x * blockDim.x + threadIdx.x), int(blockIdx.y * blockDim.y + threadIdx.y), int(blockIdx.z)};
+}
+
+template <int D>
+__device__ __forceinline__ bool inGrid(const GridDev &g, const Coord &c)
+{
+	return c.i < g.n[0] && c.j < g.n[1] && (D == 2 || c.k < g.n[2]);
+}
+
+// Grid::getLinearLower and Grid::getLowerFrac (Grid.h:52,55).
+template <int D>
+__device__ __forceinline__ void lowerFrac(const GridDev &g, const double *p, int *l, double *f)
+{
+#pragma unroll
+	for (int a = 0; a < D; a++) {
+		l[a] = int(floor((p[a] - g.o[a]) * g.inv));
+		f[a] = (p[a] - g.o[a] - l[a] * g.dx) * g.inv;
+	}
+}
+
+// Launch geometry for one grid: x is the fastest axis and one wave64 covers
+// 64 consecutive x-cells so every component array is read in 512-byte runs.
+inline dim3 gridBlock() { return dim3(64, 4, 1); }
+inline dim3 gridDims(const GridDev &g, int D)
+{
+	return dim3((g.n[0] + 63) / 64, (g.n[1] + 3) / 4, D == 3 ? g.n[2] : 1);
+}
+
+// ---- deterministic reductions ---------------------------------------------
+// Block sum of a double: wave64 shuffle tree, then the 4 wave partials in a
+// fixed order.  No atomics, so FP64 results repeat bit for bit.
+__device__ __forceinline__ double waveSum(double v)
+{
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+	return v;
+}
+
+__device__ __forceinline__ double waveMax(double v)
+{
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, kWave));
+	return v;
+}
+
+template <int NV>
+__device__ __forceinline__ void blockSum(double (&v)[NV], double *out /* block's NV partials */)
+{
+	__shared__ double red[NV][kBlock / kWave];
+	const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+	const int lane = tid & (kWave - 1), wid = tid / kWave;
+#pragma unroll
+	for (int q = 0; q < NV; q++) {
+		const double s = waveSum(v[q]);
+		if (lane == 0) red[q][wid] = s;
+	}
+	__syncthreads();
+	if (tid == 0) {
+#pragma unroll
+		for (int q = 0; q < NV; q++) {
+			double s = 0;
+			for (int w = 0; w < kBlock / kWave; w++) s += red[q][w];
+			out[q] = s;
+		}
+	}
+}
+
+__device__ __forceinline__ int flatBlock() { return int(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)); }
+
+} // namespace garm

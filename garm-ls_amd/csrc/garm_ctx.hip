@@ -1,0 +1,602 @@
+// Context and C-ABI of libgarm_gpu (declared in include/garm_gpu.h).
+//
+// A context owns every field of one LevelSetLiquid<Dim> in HBM and one HIP
+// stream.  Each entry point enqueues the kernels of one reference operator;
+// garm_step sequences them as LevelSetLiquid::advance does
+// (LevelSetLiquid.cpp:608-698).
+#include "garm_internal.hpp"
+#include "garm_project.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+using namespace garm;
+
+struct garm_ctx {
+	Fields F;
+	Projection P;
+	hipStream_t s = nullptr;
+	int device = 0;
+	std::string err;
+	std::vector<void *> allocs;
+	hipEvent_t ev[8] = {};
+	double *host_scalars = nullptr;   // pinned
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+int fail(garm_ctx *c, int code, const std::string &msg)
+{
+	c->err = msg;
+	return code;
+}
+
+int hipCheck(garm_ctx *c, hipError_t e, const char *what)
+{
+	if (e == hipSuccess) return GARM_OK;
+	return fail(c, GARM_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Kernel launches report configuration errors lazily; collect them here.
+int launched(garm_ctx *c, const char *what)
+{
+	return hipCheck(c, hipGetLastError(), what);
+}
+
+template <typename T>
+bool alloc(garm_ctx *c, T *&p, size_t n)
+{
+	void *q = nullptr;
+	if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return false;
+	c->allocs.push_back(q);
+	p = static_cast<T *>(q);
+	return true;
+}
+
+GridDev makeGrid(int D, const int *n, const double *o, double dx)
+{
+	GridDev g{};
+	for (int a = 0; a < 3; a++) { g.n[a] = a < D ? n[a] : 1; g.o[a] = a < D ? o[a] : 0; }
+	g.dx = dx;
+	g.inv = 1 / dx;
+	return g;
+}
+
+size_t fieldCount(const Fields &F, int f)
+{
+	const size_t N = F.N, D = size_t(F.D);
+	const size_t NF = F.Nf[0] + F.Nf[1] + (F.D == 3 ? F.Nf[2] : 0);
+	switch (f) {
+	case GARM_PHI: case GARM_PHI_TMP: case GARM_PHI_ORI: case GARM_PHI_REF: case GARM_ADV_REGION: case GARM_CELL_DIST: case GARM_PRESSURE:
+		return N;
+	case GARM_PHI_GRAD: case GARM_REFMAP: case GARM_PHI_REF_GRAD: return N * D;
+	case GARM_REFMAP_GRAD: return N * D * D;
+	case GARM_VELOCITY: case GARM_FRACTION: case GARM_SOLID_VELOCITY: case GARM_NORMAL: return NF;
+	case GARM_NODE_DIST: return F.Nn;
+	default: return 0;
+	}
+}
+
+// Component arrays of a field in the reference's AoS order.
+int components(Fields &F, int f, double **out)
+{
+	const int D = F.D;
+	switch (f) {
+	case GARM_PHI: out[0] = F.phi; return 1;
+	case GARM_PHI_TMP: out[0] = F.phi_tmp; return 1;
+	case GARM_PHI_ORI: out[0] = F.phi_ori; return 1;
+	case GARM_PHI_REF: out[0] = F.phi_ref; return 1;
+	case GARM_CELL_DIST: out[0] = F.cell_dist; return 1;
+	case GARM_NODE_DIST: out[0] = F.node_dist; return 1;
+	case GARM_PHI_GRAD: for (int a = 0; a < D; a++) out[a] = F.phi_grad[a]; return D;
+	case GARM_REFMAP: for (int a = 0; a < D; a++) out[a] = F.xi[a]; return D;
+	case GARM_PHI_REF_GRAD: for (int a = 0; a < D; a++) out[a] = F.phi_ref_grad[a]; return D;
+	case GARM_REFMAP_GRAD: for (int e = 0; e < D * D; e++) out[e] = F.J[e]; return D * D;
+	default: return 0;
+	}
+}
+
+double **faceArrays(Fields &F, int f)
+{
+	switch (f) {
+	case GARM_VELOCITY: return F.u;
+	case GARM_FRACTION: return F.frac;
+	case GARM_SOLID_VELOCITY: return F.bvel;
+	case GARM_NORMAL: return F.bnormal;
+	default: return nullptr;
+	}
+}
+
+void swapFaces(Fields &F)
+{
+	for (int a = 0; a < F.D; a++) std::swap(F.u[a], F.u_new[a]);
+}
+
+void swapRefMap(Fields &F)
+{
+	for (int a = 0; a < F.D; a++) std::swap(F.xi[a], F.xi_new[a]);
+	for (int e = 0; e < F.D * F.D; e++) std::swap(F.J[e], F.J_new[e]);
+}
+
+int readScalars(garm_ctx *c, int n)
+{
+	int rc = hipCheck(c, hipMemcpyAsync(c->host_scalars, c->F.dscalars, n * sizeof(double), hipMemcpyDeviceToHost, c->s), "scalar readback");
+	if (rc) return rc;
+	return hipCheck(c, hipStreamSynchronize(c->s), "stream synchronize");
+}
+
+int doProject(garm_ctx *c, const garm_pcg_opts &o, garm_pcg_stats *st)
+{
+	garm_pcg_stats local{};
+	if (!st) st = &local;
+	if (o.mode != GARM_PCG_FP64) return fail(c, GARM_EINVAL, "only the FP64 PCG mode is available in this build");
+	if (!(o.tolerance > 0) || o.max_iterations < 0) return fail(c, GARM_EINVAL, "bad PCG tolerance or iteration limit");
+	std::string err;
+	const int rc = runProjection(c->F, c->P, o, st, c->s, err);
+	if (rc != GARM_OK) return fail(c, rc, err);
+	return launched(c, "projection");
+}
+
+int doReinit(garm_ctx *c, int iters, double cfl)
+{
+	if (iters < 0 || !(cfl > 0)) return fail(c, GARM_EINVAL, "bad reinitialisation arguments");
+	launchReinit(c->F, iters, cfl, c->s);
+	return launched(c, "reinitialisation");
+}
+
+int doCheck(garm_ctx *c, double threshold, double band, double ratio, int *ok)
+{
+	launchCheckRefMap(c->F, threshold, band * c->F.cell.dx, c->s);
+	int rc = launched(c, "reference-map check");
+	if (rc) return rc;
+	if ((rc = readScalars(c, 3))) return rc;
+	const double tot = c->host_scalars[1], cnt = c->host_scalars[2];
+	*ok = 1.0 * cnt <= ratio * tot ? 1 : 0;
+	return GARM_OK;
+}
+
+void doRestart(garm_ctx *c)
+{
+	Fields &F = c->F;
+	launchCopy(F.phi_ref, F.phi, F.N, c->s);
+	launchGradient4(F, F.phi_ref, F.phi_ref_grad, c->s);
+	launchRefLevelSetMixed(F, c->s);
+	launchResetRefMap(F, c->s);
+}
+
+void doEnforceBoundary(garm_ctx *c)
+{
+	launchEnforceBoundary(c->F, c->s);
+	swapFaces(c->F);
+}
+
+} // namespace
+
+// =========================================================================
+extern "C" {
+
+const char *garm_create_error(void) { return g_create_error.c_str(); }
+const char *garm_last_error(const garm_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int garm_create(garm_ctx **out, int dim, const int res[3], double dx, const double origin[3], int bw)
+{
+	if (!out) return GARM_EINVAL;
+	*out = nullptr;
+	g_create_error.clear();
+	if ((dim != 2 && dim != 3) || !res || !origin || !(dx > 0) || bw < 0) {
+		g_create_error = "invalid grid arguments";
+		return GARM_EINVAL;
+	}
+	for (int a = 0; a < dim; a++)
+		if (res[a] < 2 || res[a] > 4096) { g_create_error = "resolution out of range"; return GARM_EINVAL; }
+	int ndev = 0;
+	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { g_create_error = "no HIP device"; return GARM_ENODEV; }
+	int dev = 0;
+	(void)hipGetDevice(&dev);
+	hipDeviceProp_t prop;
+	if (hipGetDeviceProperties(&prop, dev) != hipSuccess || std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+		g_create_error = std::string("device is not gfx950: ") + prop.gcnArchName;
+		return GARM_ENODEV;
+	}
+
+	auto *c = new garm_ctx;
+	c->device = dev;
+	Fields &F = c->F;
+	F.D = dim;
+	F.bw = bw;
+	for (int a = 0; a < 3; a++) F.res[a] = a < dim ? res[a] : 1;
+	// StaggeredGrid (StaggeredGrid.h:27-36, 104-120) with an explicit origin
+	double co[3], no[3];
+	for (int a = 0; a < 3; a++) { co[a] = origin[a] + dx / 2; no[a] = origin[a]; }
+	int nn[3] = {res[0] + 1, res[1] + 1, dim == 3 ? res[2] + 1 : 1};
+	F.cell = makeGrid(dim, F.res, co, dx);
+	F.node = makeGrid(dim, nn, no, dx);
+	for (int ax = 0; ax < dim; ax++) {
+		int fn[3]; double fo[3];
+		for (int a = 0; a < 3; a++) {
+			fn[a] = F.res[a] + (a == ax ? 1 : 0);
+			fo[a] = origin[a] + ((a < dim && a != ax) ? dx / 2 : 0);
+		}
+		F.face[ax] = makeGrid(dim, fn, fo, dx);
+	}
+	F.N = size_t(F.res[0]) * F.res[1] * F.res[2];
+	F.Nn = size_t(nn[0]) * nn[1] * nn[2];
+	for (int ax = 0; ax < dim; ax++) F.Nf[ax] = size_t(F.face[ax].n[0]) * F.face[ax].n[1] * F.face[ax].n[2];
+
+	bool ok = hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) == hipSuccess;
+	const size_t N = F.N;
+	ok = ok && alloc(c, F.phi, N) && alloc(c, F.phi_tmp, N) && alloc(c, F.phi_ori, N) && alloc(c, F.phi_ref, N)
+		&& alloc(c, F.cell_dist, N) && alloc(c, F.node_dist, F.Nn) && alloc(c, F.adv_region, N) && alloc(c, F.layer, N)
+		&& alloc(c, F.scratch[0], N) && alloc(c, F.scratch[1], N);
+	for (int a = 0; a < dim && ok; a++)
+		ok = alloc(c, F.phi_grad[a], N) && alloc(c, F.xi[a], N) && alloc(c, F.xi_new[a], N) && alloc(c, F.phi_ref_grad[a], N)
+			&& alloc(c, F.u[a], F.Nf[a]) && alloc(c, F.u_new[a], F.Nf[a]) && alloc(c, F.frac[a], F.Nf[a])
+			&& alloc(c, F.bvel[a], F.Nf[a]) && alloc(c, F.bnormal[a], F.Nf[a]) && alloc(c, F.layer_f[a], F.Nf[a]);
+	for (int e = 0; e < dim * dim && ok; e++) ok = alloc(c, F.J[e], N) && alloc(c, F.J_new[e], N);
+	for (int m = 0; m < (dim == 3 ? 4 : 1) && ok; m++) ok = alloc(c, F.phi_ref_mixed[m], N);
+	F.partials_cap = 1 << 16;
+	ok = ok && alloc(c, F.partials, F.partials_cap) && alloc(c, F.dscalars, 64);
+	ok = ok && hipHostMalloc(&c->host_scalars, 64 * sizeof(double)) == hipSuccess;
+	for (int i = 0; i < 8 && ok; i++) ok = hipEventCreate(&c->ev[i]) == hipSuccess;
+	if (!ok) {
+		g_create_error = "device allocation failed";
+		garm_destroy(c);
+		return GARM_ENOMEM;
+	}
+
+	// Boundary (domain box) on the host, then uploaded.
+	size_t NF = 0;
+	for (int a = 0; a < dim; a++) NF += F.Nf[a];
+	std::vector<double> nodeDist(F.Nn), cellDist(N), frac(NF), normal(NF);
+	buildDomainBoundary(dim, F.res, dx, origin, bw, nodeDist.data(), cellDist.data(), frac.data(), normal.data());
+	ok = hipMemcpy(F.node_dist, nodeDist.data(), F.Nn * sizeof(double), hipMemcpyHostToDevice) == hipSuccess
+		&& hipMemcpy(F.cell_dist, cellDist.data(), N * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+	size_t off = 0;
+	for (int a = 0; a < dim && ok; a++) {
+		ok = hipMemcpy(F.frac[a], frac.data() + off, F.Nf[a] * sizeof(double), hipMemcpyHostToDevice) == hipSuccess
+			&& hipMemcpy(F.bnormal[a], normal.data() + off, F.Nf[a] * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+		off += F.Nf[a];
+	}
+	// initial state (LevelSetLiquid.cpp:47-83): phi = +inf, everything else 0
+	const double inf = std::numeric_limits<double>::infinity();
+	launchFill(F.phi, inf, N, c->s);
+	for (double *p : {F.phi_tmp, F.phi_ori, F.phi_ref, F.scratch[0], F.scratch[1]}) launchFill(p, 0.0, N, c->s);
+	for (int a = 0; a < dim; a++) {
+		for (double *p : {F.phi_grad[a], F.xi[a], F.xi_new[a], F.phi_ref_grad[a]}) launchFill(p, 0.0, N, c->s);
+		for (double *p : {F.u[a], F.u_new[a], F.bvel[a]}) launchFill(p, 0.0, F.Nf[a], c->s);
+	}
+	for (int e = 0; e < dim * dim; e++) { launchFill(F.J[e], 0.0, N, c->s); launchFill(F.J_new[e], 0.0, N, c->s); }
+	for (int m = 0; m < (dim == 3 ? 4 : 1); m++) launchFill(F.phi_ref_mixed[m], 0.0, N, c->s);
+	launchFillU8(F.adv_region, 0, N, c->s);
+	ok = ok && hipGetLastError() == hipSuccess && hipStreamSynchronize(c->s) == hipSuccess;
+	if (!ok) {
+		g_create_error = "initial upload failed";
+		garm_destroy(c);
+		return GARM_EHIP;
+	}
+	*out = c;
+	return GARM_OK;
+}
+
+void garm_destroy(garm_ctx *c)
+{
+	if (!c) return;
+	if (c->s) (void)hipStreamSynchronize(c->s);
+	c->P.release();
+	for (void *p : c->allocs) (void)hipFree(p);
+	for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
+	if (c->host_scalars) (void)hipHostFree(c->host_scalars);
+	if (c->s) (void)hipStreamDestroy(c->s);
+	delete c;
+}
+
+size_t garm_field_count(const garm_ctx *c, int f) { return c ? fieldCount(c->F, f) : 0; }
+
+int garm_upload(garm_ctx *c, int f, const void *host, size_t n)
+{
+	if (!c || !host) return GARM_EINVAL;
+	Fields &F = c->F;
+	if (n != fieldCount(F, f) || n == 0) return fail(c, GARM_EINVAL, "field id or element count does not match");
+	if (f == GARM_PRESSURE) return fail(c, GARM_EINVAL, "pressure is an output");
+	int rc;
+	if ((rc = hipCheck(c, hipStreamSynchronize(c->s), "synchronize"))) return rc;
+	if (f == GARM_ADV_REGION) {
+		if ((rc = hipCheck(c, hipMemcpyAsync(F.adv_region, host, n, hipMemcpyHostToDevice, c->s), "upload"))) return rc;
+		return hipCheck(c, hipStreamSynchronize(c->s), "upload");
+	}
+	if (double **fa = faceArrays(F, f)) {
+		size_t off = 0;
+		for (int a = 0; a < F.D; a++) {
+			if ((rc = hipCheck(c, hipMemcpyAsync(fa[a], (const double *)host + off, F.Nf[a] * sizeof(double), hipMemcpyHostToDevice, c->s), "upload")))
+				return rc;
+			off += F.Nf[a];
+		}
+		return hipCheck(c, hipStreamSynchronize(c->s), "upload");
+	}
+	double *comp[9];
+	const int nc = components(F, f, comp);
+	const size_t N = n / size_t(nc);
+	std::vector<double> tmp(N);
+	const double *h = (const double *)host;
+	for (int q = 0; q < nc; q++) {
+		for (size_t i = 0; i < N; i++) tmp[i] = h[i * nc + q];
+		if ((rc = hipCheck(c, hipMemcpy(comp[q], tmp.data(), N * sizeof(double), hipMemcpyHostToDevice), "upload"))) return rc;
+	}
+	if (f == GARM_PHI_REF_GRAD) {
+		launchRefLevelSetMixed(F, c->s);
+		if ((rc = launched(c, "mixed derivatives"))) return rc;
+		return hipCheck(c, hipStreamSynchronize(c->s), "upload");
+	}
+	return GARM_OK;
+}
+
+int garm_download(garm_ctx *c, int f, void *host, size_t n)
+{
+	if (!c || !host) return GARM_EINVAL;
+	Fields &F = c->F;
+	if (n != fieldCount(F, f) || n == 0) return fail(c, GARM_EINVAL, "field id or element count does not match");
+	int rc;
+	if ((rc = hipCheck(c, hipStreamSynchronize(c->s), "synchronize"))) return rc;
+	if (f == GARM_ADV_REGION) return hipCheck(c, hipMemcpy(host, F.adv_region, n, hipMemcpyDeviceToHost), "download");
+	if (f == GARM_PRESSURE) {
+		scatterPressure(F, c->P, F.scratch[0], c->s);
+		if ((rc = launched(c, "pressure scatter")) || (rc = hipCheck(c, hipStreamSynchronize(c->s), "synchronize"))) return rc;
+		return hipCheck(c, hipMemcpy(host, F.scratch[0], n * sizeof(double), hipMemcpyDeviceToHost), "download");
+	}
+	if (double **fa = faceArrays(F, f)) {
+		size_t off = 0;
+		for (int a = 0; a < F.D; a++) {
+			if ((rc = hipCheck(c, hipMemcpy((double *)host + off, fa[a], F.Nf[a] * sizeof(double), hipMemcpyDeviceToHost), "download"))) return rc;
+			off += F.Nf[a];
+		}
+		return GARM_OK;
+	}
+	double *comp[9];
+	const int nc = components(F, f, comp);
+	const size_t N = n / size_t(nc);
+	std::vector<double> tmp(N);
+	double *h = (double *)host;
+	for (int q = 0; q < nc; q++) {
+		if ((rc = hipCheck(c, hipMemcpy(tmp.data(), comp[q], N * sizeof(double), hipMemcpyDeviceToHost), "download"))) return rc;
+		for (size_t i = 0; i < N; i++) h[i * nc + q] = tmp[i];
+	}
+	return GARM_OK;
+}
+
+int garm_union_box(garm_ctx *c, const double mn[3], const double len[3])
+{
+	if (!c || !mn || !len) return GARM_EINVAL;
+	double ce[3] = {0, 0, 0}, hl[3] = {0, 0, 0};
+	for (int a = 0; a < c->F.D; a++) { ce[a] = mn[a] + len[a] / 2; hl[a] = len[a] / 2; }   // ImplicitSurface.h:37
+	launchUnionBox(c->F, ce, hl, c->s);
+	return launched(c, "union box");
+}
+
+int garm_union_sphere(garm_ctx *c, const double ce[3], double r)
+{
+	if (!c || !ce) return GARM_EINVAL;
+	double q[3] = {ce[0], ce[1], c->F.D == 3 ? ce[2] : 0};
+	launchUnionSphere(c->F, q, r, c->s);
+	return launched(c, "union sphere");
+}
+
+int garm_union_plane(garm_ctx *c, const double p[3], const double n[3])
+{
+	if (!c || !p || !n) return GARM_EINVAL;
+	const int D = c->F.D;
+	double s = 0, nn[3] = {0, 0, 0}, pp[3] = {p[0], p[1], D == 3 ? p[2] : 0};
+	for (int a = 0; a < D; a++) s += n[a] * n[a];
+	s = std::sqrt(s);
+	for (int a = 0; a < D; a++) nn[a] = s > 0 ? n[a] / s : n[a];   // ImplicitPlane ctor normalises
+	launchUnionPlane(c->F, pp, nn, c->s);
+	return launched(c, "union plane");
+}
+
+int garm_advect_refmap(garm_ctx *c, double dt)
+{
+	if (!c) return GARM_EINVAL;
+	launchAdvectRefMap(c->F, dt, c->s);
+	swapRefMap(c->F);
+	return launched(c, "reference-map advection");
+}
+
+int garm_advect_velocity(garm_ctx *c, double dt)
+{
+	if (!c) return GARM_EINVAL;
+	launchAdvectVelocity(c->F, dt, c->s);
+	swapFaces(c->F);
+	return launched(c, "velocity advection");
+}
+
+int garm_compose_levelset(garm_ctx *c)
+{
+	if (!c) return GARM_EINVAL;
+	launchCompose(c->F, c->s);
+	return launched(c, "composition");
+}
+
+int garm_reinit_weno5(garm_ctx *c, int iters, double cfl) { return c ? doReinit(c, iters, cfl) : GARM_EINVAL; }
+
+int garm_extrapolate_collider(garm_ctx *c)
+{
+	if (!c) return GARM_EINVAL;
+	launchExtrapolateCollider(c->F, c->s);
+	return launched(c, "collider extrapolation");
+}
+
+int garm_extrapolate_air(garm_ctx *c)
+{
+	if (!c) return GARM_EINVAL;
+	launchExtrapolateAir(c->F, c->s);
+	return launched(c, "air extrapolation");
+}
+
+int garm_enforce_boundary(garm_ctx *c)
+{
+	if (!c) return GARM_EINVAL;
+	doEnforceBoundary(c);
+	return launched(c, "boundary enforcement");
+}
+
+int garm_apply_gravity(garm_ctx *c, double dt)
+{
+	if (!c) return GARM_EINVAL;
+	launchGravity(c->F, dt, c->s);
+	return launched(c, "gravity");
+}
+
+int garm_check_refmap(garm_ctx *c, double threshold, double band, double ratio, int *ok)
+{
+	if (!c || !ok) return GARM_EINVAL;
+	return doCheck(c, threshold, band, ratio, ok);
+}
+
+int garm_restart(garm_ctx *c)
+{
+	if (!c) return GARM_EINVAL;
+	doRestart(c);
+	return launched(c, "restart");
+}
+
+int garm_project(garm_ctx *c, const garm_pcg_opts *o, garm_pcg_stats *st)
+{
+	if (!c || !o) return GARM_EINVAL;
+	return doProject(c, *o, st);
+}
+
+int garm_max_speed(garm_ctx *c, double *out)
+{
+	if (!c || !out) return GARM_EINVAL;
+	launchMaxSpeed(c->F, c->s);
+	int rc = launched(c, "max speed");
+	if (rc || (rc = readScalars(c, 1))) return rc;
+	*out = c->host_scalars[0];
+	return GARM_OK;
+}
+
+int garm_synchronize(garm_ctx *c)
+{
+	if (!c) return GARM_EINVAL;
+	return hipCheck(c, hipStreamSynchronize(c->s), "synchronize");
+}
+
+int garm_initialize(garm_ctx *c, const garm_step_opts *o)
+{
+	if (!c || !o) return GARM_EINVAL;
+	Fields &F = c->F;
+	launchEnforceLevelSet(F, F.phi, 0.0, c->s);
+	launchGradient4(F, F.phi, F.phi_grad, c->s);        // Newton build: gradient before reinit
+	int rc = doReinit(c, o->reinit_iterations, o->reinit_cfl);
+	if (rc) return rc;
+	launchCopy(F.phi_tmp, F.phi, F.N, c->s);
+	launchCopy(F.phi_ref, F.phi, F.N, c->s);
+	launchCopy(F.phi_ori, F.phi, F.N, c->s);
+	for (int a = 0; a < F.D; a++) launchCopy(F.phi_ref_grad[a], F.phi_grad[a], F.N, c->s);
+	launchRefLevelSetMixed(F, c->s);
+	launchResetRefMap(F, c->s);
+	launchFillU8(F.adv_region, 1, F.N, c->s);
+	if ((rc = launched(c, "initialize"))) return rc;
+	if ((rc = doProject(c, o->pcg, nullptr))) return rc;
+	launchExtrapolateAir(F, c->s);
+	doEnforceBoundary(c);
+	if ((rc = launched(c, "initialize"))) return rc;
+	return hipCheck(c, hipStreamSynchronize(c->s), "initialize");
+}
+
+int garm_step(garm_ctx *c, double dt, int step_index, const garm_step_opts *o, garm_step_stats *st)
+{
+	if (!c || !o) return GARM_EINVAL;
+	garm_step_stats local{};
+	if (!st) st = &local;
+	*st = garm_step_stats{};
+	Fields &F = c->F;
+	hipEvent_t *ev = c->ev;
+	const int k = o->reinit_interval > 0 ? o->reinit_interval : 1;
+	int rc;
+
+	(void)hipEventRecord(ev[0], c->s);
+	launchAdvectRefMap(F, dt, c->s);
+	swapRefMap(F);
+	launchAdvectVelocity(F, dt, c->s);
+	swapFaces(F);
+	launchCompose(F, c->s);
+	(void)hipEventRecord(ev[1], c->s);
+	st->reinitialized = step_index % k == 0;
+	if (st->reinitialized && (rc = doReinit(c, o->reinit_iterations, o->reinit_cfl))) return rc;
+	(void)hipEventRecord(ev[2], c->s);
+	launchExtrapolateCollider(F, c->s);
+	(void)hipEventRecord(ev[3], c->s);
+	int ok = 1;
+	if ((rc = doCheck(c, 0.883022221559489, 4, 1e-5, &ok))) return rc;   // LevelSetLiquid.cpp:631
+	if (!ok) {
+		doRestart(c);
+		launchCompose(F, c->s);
+		launchExtrapolateCollider(F, c->s);
+		st->restarted = 1;
+	}
+	(void)hipEventRecord(ev[4], c->s);
+	launchGravity(F, dt, c->s);
+	if ((rc = launched(c, "step"))) return rc;
+	rc = doProject(c, o->pcg, &st->pcg);
+	if (rc) return rc;
+	(void)hipEventRecord(ev[5], c->s);
+	launchExtrapolateAir(F, c->s);
+	doEnforceBoundary(c);
+	(void)hipEventRecord(ev[6], c->s);
+	if ((rc = launched(c, "step"))) return rc;
+	if ((rc = hipCheck(c, hipEventSynchronize(ev[6]), "step"))) return rc;
+	float t[7] = {};
+	for (int i = 1; i <= 6; i++) (void)hipEventElapsedTime(&t[i], ev[i - 1], ev[i]);
+	st->t_advect_ms = t[1];
+	st->t_restart_ms = t[2] + t[4];
+	st->t_extrap_ms = t[3] + t[6];
+	st->t_project_ms = t[5];
+	float tot = 0;
+	(void)hipEventElapsedTime(&tot, ev[0], ev[6]);
+	st->t_total_ms = tot;
+	return GARM_OK;
+}
+
+int garm_time_kernel(garm_ctx *c, int which, int reps, double *avg_ms, double *bytes)
+{
+	if (!c || !avg_ms || !bytes || reps < 1 || which < 0 || which >= GARM_NUM_KERNELS) return GARM_EINVAL;
+	Fields &F = c->F;
+	const int D = F.D;
+	int rc;
+	if ((rc = hipCheck(c, hipStreamSynchronize(c->s), "synchronize"))) return rc;
+	size_t NF = 0;
+	for (int a = 0; a < D; a++) NF += F.Nf[a];
+	const double N = double(F.N);
+	float ms = 0;
+	if (which == GARM_K_PCG_SPMV || which == GARM_K_PCG_UPDATE) {
+		rc = timePcgKernel(D, c->P, which == GARM_K_PCG_SPMV ? 0 : 1, reps, c->s, c->ev[0], c->ev[1], &ms, bytes);
+		if (rc) return fail(c, rc, "PCG kernels need a preceding projection");
+	} else {
+		const double dt = 0.5 * F.cell.dx;   // backtrace length of half a cell
+		(void)hipEventRecord(c->ev[0], c->s);
+		for (int q = 0; q < reps; q++) {
+			if (which == GARM_K_ADVECT_REFMAP) launchAdvectRefMap(F, dt, c->s);           // writes xi_new, J_new only
+			else if (which == GARM_K_ADVECT_VELOCITY) launchAdvectVelocity(F, dt, c->s);  // writes u_new only
+			else if (which == GARM_K_COMPOSE) launchCompose(F, c->s);
+			else launchReinitStage(F, 0.9 * F.cell.dx, c->s);                            // writes scratch only
+		}
+		(void)hipEventRecord(c->ev[1], c->s);
+		if ((rc = launched(c, "timed kernel")) || (rc = hipCheck(c, hipEventSynchronize(c->ev[1]), "timed kernel"))) return rc;
+		(void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+		const int M = D == 3 ? 4 : 1;
+		if (which == GARM_K_ADVECT_REFMAP) *bytes = NF * 8 + N * (D + D * D) * 8 * 2;
+		else if (which == GARM_K_ADVECT_VELOCITY) *bytes = 2 * NF * 8;
+		else if (which == GARM_K_COMPOSE) *bytes = N * (8 * (D + D * D + 1 + D + M + 1) + 1 + 8 * (3 + D));
+		else *bytes = N * 16;
+	}
+	*avg_ms = double(ms) / reps;
+	return GARM_OK;
+}
+
+} // extern "C"

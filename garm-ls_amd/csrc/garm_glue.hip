@@ -1,0 +1,473 @@
+// Per-step glue of LevelSetLiquid::advance on gfx950: scene level sets,
+// extrapolations, boundary enforcement, gravity, |u|max and the
+// reference-map distortion check.
+//
+// Extrapolation (Extrapolation.h:44-218) advances a front one neighbour layer
+// per iteration, each new value the mean of the already-reached neighbours.
+// Here every element carries the iteration at which it is reached (0 for
+// fixed, 255 for not yet); one launch per layer writes only elements still
+// at 255 and reads only neighbours below the current layer, so a single
+// in-place array replaces the reference's double-buffered copies with no race.
+#include "garm_internal.hpp"
+
+namespace garm {
+
+constexpr uint8_t kUnreached = 255;
+
+// Nine component arrays passed by value (J = d xi / dx, column-major).
+struct Mat9 { double *p[9]; };
+
+// ----------------------------------------------------------------- scene
+template <int D>
+__global__ __launch_bounds__(kBlock) void kUnionBox(GridDev g, double *phi, double c0, double c1, double c2, double h0, double h1, double h2)
+{
+	const Coord c = threadCoord();
+	if (!inGrid<D>(g, c)) return;
+	const size_t id = gidx<D>(g, c.i, c.j, c.k);
+	const double p[3] = {g.o[0] + c.i * g.dx, g.o[1] + c.j * g.dx, g.o[2] + c.k * g.dx};
+	const double ce[3] = {c0, c1, c2}, hl[3] = {h0, h1, h2};
+	// ImplicitBox::signedDistance (ImplicitSurface.h:51-56)
+	double ph[3];
+	bool allIn = true;
+#pragma unroll
+	for (int a = 0; a < D; a++) { ph[a] = fabs(p[a] - ce[a]) - hl[a]; if (!(ph[a] <= 0)) allIn = false; }
+	double sd;
+	if (allIn) { sd = ph[0]; for (int a = 1; a < D; a++) sd = fmax(sd, ph[a]); }
+	else { double s = 0; for (int a = 0; a < D; a++) { const double m = fmax(ph[a], 0.0); s += m * m; } sd = sqrt(s); }
+	phi[id] = fmin(phi[id], sd);
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void kUnionSphere(GridDev g, double *phi, double c0, double c1, double c2, double r)
+{
+	const Coord c = threadCoord();
+	if (!inGrid<D>(g, c)) return;
+	const size_t id = gidx<D>(g, c.i, c.j, c.k);
+	const double p[3] = {g.o[0] + c.i * g.dx, g.o[1] + c.j * g.dx, g.o[2] + c.k * g.dx};
+	const double ce[3] = {c0, c1, c2};
+	double s = 0;
+#pragma unroll
+	for (int a = 0; a < D; a++) s += (p[a] - ce[a]) * (p[a] - ce[a]);
+	phi[id] = fmin(phi[id], sqrt(s) - r);   // ImplicitSphere (ImplicitSurface.h:23)
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void kUnionPlane(GridDev g, double *phi, double p0, double p1, double p2, double n0, double n1, double n2)
+{
+	const Coord c = threadCoord();
+	if (!inGrid<D>(g, c)) return;
+	const size_t id = gidx<D>(g, c.i, c.j, c.k);
+	const double p[3] = {g.o[0] + c.i * g.dx, g.o[1] + c.j * g.dx, g.o[2] + c.k * g.dx};
+	const double q[3] = {p0, p1, p2}, n[3] = {n0, n1, n2};
+	double d = 0;
+#pragma unroll
+	for (int a = 0; a < D; a++) d += (p[a] - q[a]) * n[a];
+	phi[id] = fmin(phi[id], d);   // ImplicitPlane (ImplicitSurface.h:132)
+}
+
+// _refMap = cell centre, _refMapGrad = I (LevelSetLiquid.cpp:563-569, 1345-1351)
+template <int D>
+__global__ __launch_bounds__(kBlock) void kResetRefMap(GridDev g, double *x0, double *x1, double *x2, Mat9 J)
+{
+	const Coord c = threadCoord();
+	if (!inGrid<D>(g, c)) return;
+	const size_t id = gidx<D>(g, c.i, c.j, c.k);
+	x0[id] = g.o[0] + c.i * g.dx;
+	x1[id] = g.o[1] + c.j * g.dx;
+	if constexpr (D == 3) x2[id] = g.o[2] + c.k * g.dx;
+#pragma unroll
+	for (int e = 0; e < D * D; e++) J.p[e][id] = (e % (D + 1) == 0) ? 1.0 : 0.0;
+}
+
+__global__ void kFill(double *p, double v, size_t n)
+{
+	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = v;
+}
+__global__ void kFillU8(uint8_t *p, uint8_t v, size_t n)
+{
+	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = v;
+}
+__global__ void kCopy(double *dst, const double *src, size_t n)
+{
+	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) dst[i] = src[i];
+}
+
+static unsigned linearBlocks(size_t n)
+{
+	const size_t b = (n + kBlock - 1) / kBlock;
+	return unsigned(b < 8192 ? (b ? b : 1) : 8192);
+}
+
+// enforceBoundaryLevelSet(sdf, tol) (LevelSetLiquid.cpp:966-975)
+__global__ void kEnforceLevelSet(double *sdf, const double *cellDist, double tol, size_t n)
+{
+	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+		if (sdf[i] < -cellDist[i] - tol) sdf[i] = -cellDist[i] - tol;
+}
+
+// applyBodyForces (LevelSetLiquid.cpp:837-841), kGravity = 9.8 (Constants.h:9)
+__global__ void kGravity(double *uy, double dv, size_t n)
+{
+	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) uy[i] -= dv;
+}
+
+// ------------------------------------------------------------ extrapolation
+enum CellMask { kMaskCellDistPositive = 0, kMaskAdvRegion = 1, kMaskPhiOriNonPositive = 2 };
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void kLayerInitCells(GridDev g, uint8_t *layer, int mask, const double *cellDist,
+	const uint8_t *adv, const double *phiOri)
+{
+	const Coord c = threadCoord();
+	if (!inGrid<D>(g, c)) return;
+	const size_t id = gidx<D>(g, c.i, c.j, c.k);
+	bool fixed;
+	if (mask == kMaskCellDistPositive) fixed = cellDist[id] > 0;
+	else if (mask == kMaskAdvRegion) fixed = adv[id] > 0;
+	else fixed = phiOri[id] <= 0;
+	layer[id] = fixed ? 0 : kUnreached;
+}
+
+// One iteration of Extrapolation::solve over cells (Extrapolation.h:101-124,
+// 143-166, 191-214): nv value arrays averaged over reached neighbours in
+// the reference's ordinal order (-x, +x, -y, +y, -z, +z).
+template <int D>
+__global__ __launch_bounds__(kBlock) void kLayerStepCells(GridDev g, uint8_t *layer, int L, int nv, double *v0, double *v1, double *v2)
+{
+	const Coord c = threadCoord();
+	if (!inGrid<D>(g, c)) return;
+	const size_t id = gidx<D>(g, c.i, c.j, c.k);
+	if (layer[id] != kUnreached) return;
+	int cnt = 0;
+	double s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+	for (int o = 0; o < 2 * D; o++) {
+		const int ax = o >> 1, sd = (o & 1) ? 1 : -1;
+		const int qi = c.i + (ax == 0 ? sd : 0), qj = c.j + (ax == 1 ? sd : 0), qk = c.k + (ax == 2 ? sd : 0);
+		if (qi < 0 || qi >= g.n[0] || qj < 0 || qj >= g.n[1] || (D == 3 && (qk < 0 || qk >= g.n[2]))) continue;
+		const size_t q = gidx<D>(g, qi, qj, qk);
+		if (layer[q] < L) {
+			if (nv > 0) s0 += v0[q];
+			if (nv > 1) s1 += v1[q];
+			if (nv > 2) s2 += v2[q];
+			cnt++;
+		}
+	}
+	if (cnt > 0) {
+		if (nv > 0) v0[id] = s0 / cnt;
+		if (nv > 1) v1[id] = s1 / cnt;
+		if (nv > 2) v2[id] = s2 / cnt;
+		layer[id] = uint8_t(L);
+	}
+}
+
+// Reference map not reached: reset to the cell centre (Extrapolation.h:210-212).
+template <int D>
+__global__ __launch_bounds__(kBlock) void kLayerFinishRefMap(GridDev g, const uint8_t *layer, double *x0, double *x1, double *x2)
+{
+	const Coord c = threadCoord();
+	if (!inGrid<D>(g, c)) return;
+	const size_t id = gidx<D>(g, c.i, c.j, c.k);
+	if (layer[id] != kUnreached) return;
+	x0[id] = g.o[0] + c.i * g.dx;
+	x1[id] = g.o[1] + c.j * g.dx;
+	if constexpr (D == 3) x2[id] = g.o[2] + c.k * g.dx;
+}
+
+// _advRegion = reached within the radius (Extrapolation.h:130-175 applied to
+// the indicator of phi_ori <= 0, LevelSetLiquid.cpp:1064-1070).
+__global__ void kLayerToMask(const uint8_t *layer, uint8_t *adv, size_t n)
+{
+	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+		adv[i] = layer[i] != kUnreached ? 1 : 0;
+}
+
+// Velocity faces: fixed when fraction < 1 and an adjacent cell is liquid in
+// phi_ori (LevelSetLiquid.cpp:1072-1083).
+template <int D>
+__global__ __launch_bounds__(kBlock) void kLayerInitFaces(GridDev g, GridDev cg, int axis, uint8_t *layer, const double *frac, const double *phiOri)
+{
+	const Coord c = threadCoord();
+	if (!inGrid<D>(g, c)) return;
+	const size_t id = gidx<D>(g, c.i, c.j, c.k);
+	bool fixed = false;
+	const int i0 = c.i - (axis == 0), j0 = c.j - (axis == 1), k0 = c.k - (axis == 2);
+	// Both adjacent cells exist for every face whose fraction is below 1 (the
+	// outermost faces are boundary faces, fraction 1); the guard keeps an
+	// uploaded fraction field from reading outside the cell grid.
+	const bool inside = i0 >= 0 && j0 >= 0 && k0 >= 0 && c.i < cg.n[0] && c.j < cg.n[1] && (D == 2 || c.k < cg.n[2]);
+	if (frac[id] < 1 && inside)
+		fixed = phiOri[gidx<D>(cg, i0, j0, k0)] <= 0 || phiOri[gidx<D>(cg, c.i, c.j, c.k)] <= 0;
+	layer[id] = fixed ? 0 : kUnreached;
+}
+
+// Faces never reached are zeroed ("for velocity only", Extrapolation.h:76-78).
+__global__ void kLayerFinishFaces(const uint8_t *layer, double *u, size_t n)
+{
+	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+		if (layer[i] == kUnreached) u[i] = 0;
+}
+
+// ---------------------------------------------------------- enforce (u)
+template <int D>
+__device__ __forceinline__ double lerpFace(const GridDev &g, const double *a, const double *p)
+{
+	int l[3]; double f[3];
+	lowerFrac<D>(g, p, l, f);
+	double w[2][3];
+#pragma unroll
+	for (int b = 0; b < D; b++) { w[0][b] = 1 - f[b]; w[1][b] = f[b]; }
+	double val = 0;
+#pragma unroll
+	for (int k = 0; k < (D == 3 ? 2 : 1); k++)
+#pragma unroll
+		for (int j = 0; j < 2; j++)
+#pragma unroll
+			for (int i = 0; i < 2; i++) {
+				const double wt = D == 2 ? w[i][0] * w[j][1] : w[i][0] * w[j][1] * w[k][2];
+				val += a[gat<D>(g, l[0] + i, l[1] + j, D == 3 ? l[2] + k : 0)] * wt;
+			}
+	return val;
+}
+
+template <int D>
+struct EnforceArgs {
+	GridDev g[3];
+	const double *u[3], *bvel[3], *bnormal[3], *frac[3];
+	double *out[3];
+};
+
+// Boundary::enforce(velocity) (Boundary.cpp:89-120); its `fraction == 1`
+// branch (:115) is unreachable and omitted.
+template <int D>
+__global__ __launch_bounds__(kBlock) void kEnforceBoundary(EnforceArgs<D> A, int axis)
+{
+	const GridDev &g = A.g[axis];
+	const Coord c = threadCoord();
+	if (!inGrid<D>(g, c)) return;
+	const size_t id = gidx<D>(g, c.i, c.j, c.k);
+	const double fr = A.frac[axis][id];
+	double out = A.u[axis][id];
+	if (fr > 0.1 && fr <= 1) {
+		const double p[3] = {g.o[0] + c.i * g.dx, g.o[1] + c.j * g.dx, g.o[2] + c.k * g.dx};
+		double n[3];
+#pragma unroll
+		for (int a = 0; a < D; a++) n[a] = lerpFace<D>(A.g[a], A.bnormal[a], p);
+		double s = 0;
+#pragma unroll
+		for (int a = 0; a < D; a++) s += n[a] * n[a];
+		if (s > 0) { const double r = sqrt(s); for (int a = 0; a < D; a++) n[a] /= r; }
+		bool any = false;
+#pragma unroll
+		for (int a = 0; a < D; a++) any = any || n[a] != 0;
+		if (any) {
+			double dv[3], vel_ax = 0, vn = 0;
+#pragma unroll
+			for (int a = 0; a < D; a++) {
+				const double vel = lerpFace<D>(A.g[a], A.u[a], p);
+				const double vel0 = lerpFace<D>(A.g[a], A.bvel[a], p);
+				dv[a] = vel - vel0;
+				if (a == axis) vel_ax = vel;
+				vn += dv[a] * n[a];
+			}
+			if (vn < 0) {
+				const double vt = dv[axis] - vn * n[axis];
+				out = (vel_ax - vn * n[axis]) - 0.8 * vt;
+			}
+		}
+	}
+	A.out[axis][id] = out;
+}
+
+// ------------------------------------------------------------- reductions
+__global__ __launch_bounds__(kBlock) void kMaxAbsPartial(const double *a, size_t n, double *partials, int slot0)
+{
+	double m = 0;
+	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) m = fmax(m, fabs(a[i]));
+	__shared__ double red[kBlock / kWave];
+	m = waveMax(m);
+	const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+	if (lane == 0) red[wid] = m;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double r = 0;
+		for (int w = 0; w < kBlock / kWave; w++) r = fmax(r, red[w]);
+		partials[slot0 + blockIdx.x] = r;
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void kMaxFinal(const double *partials, int n, double *out)
+{
+	double m = 0;
+	for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, partials[i]);
+	__shared__ double red[kBlock / kWave];
+	m = waveMax(m);
+	const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+	if (lane == 0) red[wid] = m;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double r = 0;
+		for (int w = 0; w < kBlock / kWave; w++) r = fmax(r, red[w]);
+		*out = r;
+	}
+}
+
+// checkReferenceMap (LevelSetLiquid.cpp:1246-1283): per-block (tot, cnt).
+template <int D>
+__global__ __launch_bounds__(kBlock) void kCheckRefMap(GridDev g, const double *phiOri, const double *cellDist, Mat9 J,
+	double threshold, double band, double *partials)
+{
+	const Coord c = threadCoord();
+	double v[2] = {0, 0};
+	if (inGrid<D>(g, c)) {
+		const size_t id = gidx<D>(g, c.i, c.j, c.k);
+		if (fabs(phiOri[id]) < band && cellDist[id] > 0) {
+			v[0] = 1;
+			double m[9];
+#pragma unroll
+			for (int e = 0; e < D * D; e++) m[e] = J.p[e][id];
+			auto dot = [&](int a, int b) { double s = 0; for (int r = 0; r < D; r++) s += m[a * D + r] * m[b * D + r]; return s; };
+			if constexpr (D == 2) {
+				const double vab = dot(0, 1) * dot(0, 1) / dot(0, 0) / dot(1, 1);
+				if (vab > threshold) v[1] = 1;
+			} else {
+				const double ab = dot(0, 1), ac = dot(0, 2), bc = dot(1, 2);
+				const double vab = ab * ab / dot(0, 0) / dot(1, 1);
+				const double vac = ac * ac / dot(0, 0) / dot(2, 2);
+				const double vbc = bc * bc / dot(1, 1) / dot(2, 2);
+				if (vab > threshold || vac > threshold || vbc > threshold) v[1] = 1;
+			}
+		}
+	}
+	double outp[2];
+	blockSum<2>(v, outp);
+	const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+	if (tid == 0) {
+		const int b = flatBlock();
+		partials[2 * b] = outp[0];
+		partials[2 * b + 1] = outp[1];
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void kSumFinal2(const double *partials, int nblocks, double *out)
+{
+	double v[2] = {0, 0};
+	for (int i = threadIdx.x; i < nblocks; i += blockDim.x) { v[0] += partials[2 * i]; v[1] += partials[2 * i + 1]; }
+	double o[2];
+	blockSum<2>(v, o);
+	if (threadIdx.x == 0) { out[0] = o[0]; out[1] = o[1]; }
+}
+
+// --------------------------------------------------------------- launchers
+template <typename K, typename... Args>
+static void launchCells(int D, const GridDev &g, hipStream_t s, K k2, K k3, Args... args)
+{
+	if (D == 2) hipLaunchKernelGGL(k2, gridDims(g, 2), gridBlock(), 0, s, args...);
+	else hipLaunchKernelGGL(k3, gridDims(g, 3), gridBlock(), 0, s, args...);
+}
+
+void launchUnionBox(const Fields &F, const double *ce, const double *hl, hipStream_t s)
+{
+	launchCells(F.D, F.cell, s, kUnionBox<2>, kUnionBox<3>, F.cell, F.phi, ce[0], ce[1], ce[2], hl[0], hl[1], hl[2]);
+}
+void launchUnionSphere(const Fields &F, const double *ce, double r, hipStream_t s)
+{
+	launchCells(F.D, F.cell, s, kUnionSphere<2>, kUnionSphere<3>, F.cell, F.phi, ce[0], ce[1], ce[2], r);
+}
+void launchUnionPlane(const Fields &F, const double *p, const double *n, hipStream_t s)
+{
+	launchCells(F.D, F.cell, s, kUnionPlane<2>, kUnionPlane<3>, F.cell, F.phi, p[0], p[1], p[2], n[0], n[1], n[2]);
+}
+
+void launchResetRefMap(const Fields &F, hipStream_t s)
+{
+	Mat9 J{};
+	for (int e = 0; e < F.D * F.D; e++) J.p[e] = F.J[e];
+	launchCells(F.D, F.cell, s, kResetRefMap<2>, kResetRefMap<3>, F.cell, F.xi[0], F.xi[1], F.xi[2], J);
+}
+
+void launchFill(double *p, double v, size_t n, hipStream_t s) { hipLaunchKernelGGL(kFill, dim3(linearBlocks(n)), dim3(kBlock), 0, s, p, v, n); }
+void launchFillU8(uint8_t *p, uint8_t v, size_t n, hipStream_t s) { hipLaunchKernelGGL(kFillU8, dim3(linearBlocks(n)), dim3(kBlock), 0, s, p, v, n); }
+void launchCopy(double *d, const double *src, size_t n, hipStream_t s) { hipLaunchKernelGGL(kCopy, dim3(linearBlocks(n)), dim3(kBlock), 0, s, d, src, n); }
+
+void launchEnforceLevelSet(const Fields &F, double *sdf, double tol, hipStream_t s)
+{
+	hipLaunchKernelGGL(kEnforceLevelSet, dim3(linearBlocks(F.N)), dim3(kBlock), 0, s, sdf, (const double *)F.cell_dist, tol, F.N);
+}
+
+void launchGravity(const Fields &F, double dt, hipStream_t s)
+{
+	hipLaunchKernelGGL(kGravity, dim3(linearBlocks(F.Nf[1])), dim3(kBlock), 0, s, F.u[1], 9.8 * dt, F.Nf[1]);
+}
+
+static void cellLayers(const Fields &F, int mask, int steps, int nv, double *v0, double *v1, double *v2, hipStream_t s)
+{
+	launchCells(F.D, F.cell, s, kLayerInitCells<2>, kLayerInitCells<3>, F.cell, F.layer, mask, (const double *)F.cell_dist,
+		(const uint8_t *)F.adv_region, (const double *)F.phi_ori);
+	for (int L = 1; L <= steps; L++)
+		launchCells(F.D, F.cell, s, kLayerStepCells<2>, kLayerStepCells<3>, F.cell, F.layer, L, nv, v0, v1, v2);
+}
+
+// extrapolateFieldsToCollider (LevelSetLiquid.cpp:1086-1137): phi_ori and phi,
+// 6 layers (_boundaryExtrapolationRadius, LevelSetLiquid.h:92), fixed where
+// cellDist > 0.  Both arrays share the mask, so one sweep moves both.
+void launchExtrapolateCollider(const Fields &F, hipStream_t s)
+{
+	cellLayers(F, kMaskCellDistPositive, 6, 2, F.phi_ori, F.phi, nullptr, s);
+}
+
+// extrapolateFieldsToAir (LevelSetLiquid.cpp:1059-1084): xi (15 layers from
+// the old advRegion), advRegion rebuilt from phi_ori <= 0 (15 layers), u
+// (17 layers per axis).
+void launchExtrapolateAir(const Fields &F, hipStream_t s)
+{
+	cellLayers(F, kMaskAdvRegion, 15, F.D, F.xi[0], F.xi[1], F.D == 3 ? F.xi[2] : nullptr, s);
+	launchCells(F.D, F.cell, s, kLayerFinishRefMap<2>, kLayerFinishRefMap<3>, F.cell, (const uint8_t *)F.layer, F.xi[0], F.xi[1], F.xi[2]);
+	cellLayers(F, kMaskPhiOriNonPositive, 15, 0, nullptr, nullptr, nullptr, s);
+	hipLaunchKernelGGL(kLayerToMask, dim3(linearBlocks(F.N)), dim3(kBlock), 0, s, (const uint8_t *)F.layer, F.adv_region, F.N);
+	for (int a = 0; a < F.D; a++) {
+		launchCells(F.D, F.face[a], s, kLayerInitFaces<2>, kLayerInitFaces<3>, F.face[a], F.cell, a, F.layer_f[a],
+			(const double *)F.frac[a], (const double *)F.phi_ori);
+		for (int L = 1; L <= 17; L++)
+			launchCells(F.D, F.face[a], s, kLayerStepCells<2>, kLayerStepCells<3>, F.face[a], F.layer_f[a], L, 1, F.u[a],
+				(double *)nullptr, (double *)nullptr);
+		hipLaunchKernelGGL(kLayerFinishFaces, dim3(linearBlocks(F.Nf[a])), dim3(kBlock), 0, s, (const uint8_t *)F.layer_f[a], F.u[a], F.Nf[a]);
+	}
+}
+
+void launchEnforceBoundary(const Fields &F, hipStream_t s)
+{
+	if (F.D == 2) {
+		EnforceArgs<2> A{};
+		for (int a = 0; a < 2; a++) { A.g[a] = F.face[a]; A.u[a] = F.u[a]; A.bvel[a] = F.bvel[a]; A.bnormal[a] = F.bnormal[a]; A.frac[a] = F.frac[a]; A.out[a] = F.u_new[a]; }
+		for (int a = 0; a < 2; a++) hipLaunchKernelGGL(kEnforceBoundary<2>, gridDims(F.face[a], 2), gridBlock(), 0, s, A, a);
+	} else {
+		EnforceArgs<3> A{};
+		for (int a = 0; a < 3; a++) { A.g[a] = F.face[a]; A.u[a] = F.u[a]; A.bvel[a] = F.bvel[a]; A.bnormal[a] = F.bnormal[a]; A.frac[a] = F.frac[a]; A.out[a] = F.u_new[a]; }
+		for (int a = 0; a < 3; a++) hipLaunchKernelGGL(kEnforceBoundary<3>, gridDims(F.face[a], 3), gridBlock(), 0, s, A, a);
+	}
+}
+
+void launchMaxSpeed(const Fields &F, hipStream_t s)
+{
+	int slot = 0;
+	for (int a = 0; a < F.D; a++) {
+		const unsigned nb = 1024;
+		hipLaunchKernelGGL(kMaxAbsPartial, dim3(nb), dim3(kBlock), 0, s, (const double *)F.u[a], F.Nf[a], F.partials, slot);
+		slot += nb;
+	}
+	hipLaunchKernelGGL(kMaxFinal, dim3(1), dim3(kBlock), 0, s, (const double *)F.partials, slot, F.dscalars);
+}
+
+void launchCheckRefMap(const Fields &F, double threshold, double band, hipStream_t s)
+{
+	const dim3 gd = gridDims(F.cell, F.D);
+	const int nb = int(gd.x * gd.y * gd.z);
+	Mat9 J{};
+	for (int e = 0; e < F.D * F.D; e++) J.p[e] = F.J[e];
+	launchCells(F.D, F.cell, s, kCheckRefMap<2>, kCheckRefMap<3>, F.cell, (const double *)F.phi_ori, (const double *)F.cell_dist,
+		J, threshold, band, F.partials);
+	hipLaunchKernelGGL(kSumFinal2, dim3(1), dim3(kBlock), 0, s, (const double *)F.partials, nb, F.dscalars + 1);
+}
+
+} // namespace garm

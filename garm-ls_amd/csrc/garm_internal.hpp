@@ -1,0 +1,72 @@
+// Internal declarations shared by the HIP translation units of libgarm_gpu.
+#pragma once
+
+#include "garm_common.hpp"
+#include "garm_boundary.hpp"
+
+#include <cstdint>
+#include <string>
+
+namespace garm {
+
+// Face-centred velocity: one array per axis on its own face grid
+// (SGridBasedData<Dim,double>, StaggeredGrid.h:104-120).
+struct FaceSet {
+	const double *a[3];
+	GridDev g[3];
+};
+
+// Every device-resident field of one LevelSetLiquid<Dim>
+// (LevelSetLiquid.h:27-121), structure-of-arrays.
+struct Fields {
+	int D = 3;
+	int bw = 2;
+	int res[3] = {1, 1, 1};
+	GridDev cell{}, node{}, face[3]{};
+	size_t N = 0, Nn = 0, Nf[3] = {0, 0, 0};
+
+	double *u[3] = {}, *u_new[3] = {};               // _velocity (+ double buffer)
+	double *phi = nullptr, *phi_grad[3] = {};         // _levelSet, _levelSetGrad
+	double *xi[3] = {}, *J[9] = {};                   // _refMap, _refMapGrad: J[c*D + r] = d xi_r / d x_c
+	double *xi_new[3] = {}, *J_new[9] = {};
+	double *phi_tmp = nullptr, *phi_ori = nullptr;    // _tmpLevelSet, _oriLevelSet
+	double *phi_ref = nullptr, *phi_ref_grad[3] = {}; // _refLevelSet, _refLevelSetGrad
+	double *phi_ref_mixed[4] = {};                    // _refLevelSetView's Hermite mixed derivatives
+	uint8_t *adv_region = nullptr;                    // _advRegion
+	double *cell_dist = nullptr, *node_dist = nullptr;// _boundary.cellDist / nodeDist
+	double *frac[3] = {}, *bvel[3] = {}, *bnormal[3] = {};   // _boundary.fraction / velocity / normal
+	double *scratch[2] = {};                          // two cell arrays (reinit stages)
+	uint8_t *layer = nullptr;                         // extrapolation layer index per cell
+	uint8_t *layer_f[3] = {};                         // ... per face
+	double *partials = nullptr;                       // reduction partials
+	size_t partials_cap = 0;
+	double *dscalars = nullptr;                       // small device scalars
+};
+
+// ---- launchers (one per kernel family; all enqueue on `s`) ---------------
+void launchAdvectRefMap(const Fields &F, double dt, hipStream_t s);
+void launchAdvectVelocity(const Fields &F, double dt, hipStream_t s);
+void launchCompose(const Fields &F, hipStream_t s);
+void launchRefLevelSetMixed(const Fields &F, hipStream_t s);
+void launchGradient4(const Fields &F, const double *src, double *const *dst, hipStream_t s);
+void launchReinit(const Fields &F, int iterations, double cfl, hipStream_t s);
+// one Runge-Kutta stage L(phi) -> scratch[0], for timing
+void launchReinitStage(const Fields &F, double dtau, hipStream_t s);
+
+void launchUnionBox(const Fields &F, const double *center, const double *half, hipStream_t s);
+void launchUnionSphere(const Fields &F, const double *center, double r, hipStream_t s);
+void launchUnionPlane(const Fields &F, const double *pos, const double *nrm, hipStream_t s);
+void launchResetRefMap(const Fields &F, hipStream_t s);
+void launchFill(double *p, double v, size_t n, hipStream_t s);
+void launchFillU8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
+void launchCopy(double *dst, const double *src, size_t n, hipStream_t s);
+void launchEnforceLevelSet(const Fields &F, double *sdf, double tol, hipStream_t s);
+void launchGravity(const Fields &F, double dt, hipStream_t s);
+void launchExtrapolateCollider(const Fields &F, hipStream_t s);
+void launchExtrapolateAir(const Fields &F, hipStream_t s);
+void launchEnforceBoundary(const Fields &F, hipStream_t s);
+// results land in F.dscalars: [0] = max |u|; [1] = tot, [2] = cnt (as doubles)
+void launchMaxSpeed(const Fields &F, hipStream_t s);
+void launchCheckRefMap(const Fields &F, double threshold, double band, hipStream_t s);
+
+} // namespace garm

@@ -1,0 +1,256 @@
+"""ctypes binding of libgarm_gpu.so (the C-ABI declared in include/garm_gpu.h).
+
+The shared library is built in-tree (garm-ls_amd/lib/).  There is no CPU
+fallback: if the library or a gfx950 device is missing, every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "lib", "libgarm_gpu.so")
+HEADER_PATH = os.path.join(os.path.dirname(_PKG), "include", "garm_gpu.h")
+
+# garm_field
+PHI, PHI_GRAD, REFMAP, REFMAP_GRAD, PHI_TMP, PHI_ORI, PHI_REF, PHI_REF_GRAD, ADV_REGION, VELOCITY, \
+    CELL_DIST, NODE_DIST, FRACTION, SOLID_VELOCITY, NORMAL, PRESSURE = range(16)
+FIELD_NAMES = ["phi", "phi_grad", "refmap", "refmap_grad", "phi_tmp", "phi_ori", "phi_ref", "phi_ref_grad",
+               "adv_region", "velocity", "cell_dist", "node_dist", "fraction", "solid_velocity", "normal", "pressure"]
+
+# garm_kernel (instrumentation)
+K_ADVECT_REFMAP, K_ADVECT_VELOCITY, K_COMPOSE, K_REINIT_STAGE, K_PCG_SPMV, K_PCG_UPDATE = range(6)
+KERNEL_NAMES = ["advect_refmap", "advect_velocity", "compose", "reinit_stage", "pcg_spmv", "pcg_update"]
+
+# garm_pcg_mode
+PCG_FP64, PCG_MIXED, PCG_LEARNED = 0, 1, 2
+FP64, FP32 = 0, 1
+
+GARM_OK, GARM_EINVAL, GARM_EHIP, GARM_ENOCONV, GARM_ENOMEM, GARM_ENODEV = 0, -1, -2, -3, -4, -5
+STATUS_NAMES = {0: "GARM_OK", -1: "GARM_EINVAL", -2: "GARM_EHIP", -3: "GARM_ENOCONV", -4: "GARM_ENOMEM",
+                -5: "GARM_ENODEV"}
+
+
+class PcgOpts(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int), ("tolerance", ctypes.c_double), ("max_iterations", ctypes.c_int),
+                ("check_interval", ctypes.c_int)]
+
+
+class PcgStats(ctypes.Structure):
+    _fields_ = [("unknowns", ctypes.c_int), ("iterations", ctypes.c_int), ("outer_iterations", ctypes.c_int),
+                ("fell_back", ctypes.c_int), ("relative_residual", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class StepOpts(ctypes.Structure):
+    _fields_ = [("reinit_interval", ctypes.c_int), ("reinit_iterations", ctypes.c_int), ("reinit_cfl", ctypes.c_double),
+                ("advect_precision", ctypes.c_int), ("pcg", PcgOpts)]
+
+
+class StepStats(ctypes.Structure):
+    _fields_ = [("t_advect_ms", ctypes.c_double), ("t_restart_ms", ctypes.c_double), ("t_project_ms", ctypes.c_double),
+                ("t_extrap_ms", ctypes.c_double), ("t_total_ms", ctypes.c_double), ("restarted", ctypes.c_int),
+                ("reinitialized", ctypes.c_int), ("pcg", PcgStats)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "pcg"}
+        d["pcg"] = self.pcg.as_dict()
+        return d
+
+
+class GarmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libgarm_gpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libgarm_gpu.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, d, i, sz = ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_size_t
+        dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+        sig = {
+            "garm_create": (i, [ctypes.POINTER(vp), i, ip, d, dp, i]),
+            "garm_destroy": (None, [vp]),
+            "garm_last_error": (ctypes.c_char_p, [vp]),
+            "garm_create_error": (ctypes.c_char_p, []),
+            "garm_field_count": (sz, [vp, i]),
+            "garm_upload": (i, [vp, i, vp, sz]),
+            "garm_download": (i, [vp, i, vp, sz]),
+            "garm_union_box": (i, [vp, dp, dp]),
+            "garm_union_sphere": (i, [vp, dp, d]),
+            "garm_union_plane": (i, [vp, dp, dp]),
+            "garm_initialize": (i, [vp, ctypes.POINTER(StepOpts)]),
+            "garm_advect_refmap": (i, [vp, d]),
+            "garm_advect_velocity": (i, [vp, d]),
+            "garm_compose_levelset": (i, [vp]),
+            "garm_reinit_weno5": (i, [vp, i, d]),
+            "garm_extrapolate_collider": (i, [vp]),
+            "garm_extrapolate_air": (i, [vp]),
+            "garm_enforce_boundary": (i, [vp]),
+            "garm_apply_gravity": (i, [vp, d]),
+            "garm_check_refmap": (i, [vp, d, d, d, ip]),
+            "garm_restart": (i, [vp]),
+            "garm_project": (i, [vp, ctypes.POINTER(PcgOpts), ctypes.POINTER(PcgStats)]),
+            "garm_step": (i, [vp, d, i, ctypes.POINTER(StepOpts), ctypes.POINTER(StepStats)]),
+            "garm_max_speed": (i, [vp, dp]),
+            "garm_synchronize": (i, [vp]),
+            "garm_time_kernel": (i, [vp, i, i, dp, dp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def declared_symbols(header=HEADER_PATH):
+    """Names of the functions declared in include/garm_gpu.h."""
+    import re
+    text = open(header).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_]+\s*\**\s*(garm_[a-z0-9_]+)\s*\(", text, flags=re.M)))
+
+
+def _d3(v):
+    v = list(v) + [0.0] * (3 - len(v))
+    return (ctypes.c_double * 3)(*v)
+
+
+def step_opts(reinit_interval=1, reinit_iterations=30, reinit_cfl=0.9, precision=FP64, mode=PCG_FP64, tol=1e-5,
+              max_iterations=2000, check_interval=8):
+    o = StepOpts()
+    o.reinit_interval = reinit_interval
+    o.reinit_iterations = reinit_iterations
+    o.reinit_cfl = reinit_cfl
+    o.advect_precision = precision
+    o.pcg.mode = mode
+    o.pcg.tolerance = tol
+    o.pcg.max_iterations = max_iterations
+    o.pcg.check_interval = check_interval
+    return o
+
+
+class Liquid:
+    """One device-resident LevelSetLiquid<Dim> (LevelSetLiquid.h:17-121)."""
+
+    def __init__(self, dim, res, dx, origin, bw=2):
+        L = lib()
+        r = list(res) + [1] * (3 - len(res))
+        h = ctypes.c_void_p()
+        rc = L.garm_create(ctypes.byref(h), dim, (ctypes.c_int * 3)(*r), dx, _d3(origin), bw)
+        if rc != GARM_OK:
+            raise GarmError(rc, L.garm_create_error().decode())
+        self.h = h
+        self.dim = dim
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().garm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != GARM_OK:
+            raise GarmError(rc, lib().garm_last_error(self.h).decode())
+        return rc
+
+    def count(self, field):
+        return lib().garm_field_count(self.h, field)
+
+    def get(self, field):
+        n = self.count(field)
+        a = np.empty(n, dtype=np.uint8 if field == ADV_REGION else np.float64)
+        self._check(lib().garm_download(self.h, field, a.ctypes.data, n))
+        return a
+
+    def set(self, field, arr):
+        a = np.ascontiguousarray(arr, dtype=np.uint8 if field == ADV_REGION else np.float64).ravel()
+        self._check(lib().garm_upload(self.h, field, a.ctypes.data, a.size))
+
+    def union_box(self, mn, lengths):
+        self._check(lib().garm_union_box(self.h, _d3(mn), _d3(lengths)))
+
+    def union_sphere(self, c, r):
+        self._check(lib().garm_union_sphere(self.h, _d3(c), r))
+
+    def union_plane(self, p, n):
+        self._check(lib().garm_union_plane(self.h, _d3(p), _d3(n)))
+
+    def initialize(self, opts):
+        self._check(lib().garm_initialize(self.h, ctypes.byref(opts)))
+
+    def advect_refmap(self, dt):
+        self._check(lib().garm_advect_refmap(self.h, dt))
+
+    def advect_velocity(self, dt):
+        self._check(lib().garm_advect_velocity(self.h, dt))
+
+    def compose(self):
+        self._check(lib().garm_compose_levelset(self.h))
+
+    def reinit(self, iters=30, cfl=0.9):
+        self._check(lib().garm_reinit_weno5(self.h, iters, cfl))
+
+    def extrapolate_collider(self):
+        self._check(lib().garm_extrapolate_collider(self.h))
+
+    def extrapolate_air(self):
+        self._check(lib().garm_extrapolate_air(self.h))
+
+    def enforce_boundary(self):
+        self._check(lib().garm_enforce_boundary(self.h))
+
+    def gravity(self, dt):
+        self._check(lib().garm_apply_gravity(self.h, dt))
+
+    def check_refmap(self, threshold=0.883022221559489, band=4.0, ratio=1e-5):
+        ok = ctypes.c_int()
+        self._check(lib().garm_check_refmap(self.h, threshold, band, ratio, ctypes.byref(ok)))
+        return bool(ok.value)
+
+    def restart(self):
+        self._check(lib().garm_restart(self.h))
+
+    def project(self, mode=PCG_FP64, tol=1e-5, max_iterations=2000, check_interval=8, raise_on_noconv=True):
+        o = PcgOpts(mode, tol, max_iterations, check_interval)
+        st = PcgStats()
+        rc = lib().garm_project(self.h, ctypes.byref(o), ctypes.byref(st))
+        if rc != GARM_OK and (raise_on_noconv or rc != GARM_ENOCONV):
+            self._check(rc)
+        d = st.as_dict()
+        d["rc"] = rc
+        return d
+
+    def step(self, dt, step_index, opts):
+        st = StepStats()
+        self._check(lib().garm_step(self.h, dt, step_index, ctypes.byref(opts), ctypes.byref(st)))
+        return st.as_dict()
+
+    def max_speed(self):
+        v = ctypes.c_double()
+        self._check(lib().garm_max_speed(self.h, ctypes.byref(v)))
+        return v.value
+
+    def time_kernel(self, which, reps=10):
+        """(mean ms per launch, algorithmic HBM bytes per launch)."""
+        ms, by = ctypes.c_double(), ctypes.c_double()
+        self._check(lib().garm_time_kernel(self.h, which, reps, ctypes.byref(ms), ctypes.byref(by)))
+        return ms.value, by.value
+
+    def synchronize(self):
+        self._check(lib().garm_synchronize(self.h))

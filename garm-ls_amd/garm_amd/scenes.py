@@ -1,0 +1,62 @@
+"""Grid convention and scenes of the benchmark configurations.
+
+Grid: n array cells per axis including a solid band of `bw` cells on each
+side, as the reference's builder lays out StaggeredGrid(bw, L/(n - 2 bw), n)
+(LevelSetLiquidBuilder.cpp:49,74); the origin is chosen so the liquid domain
+is [0, 1]^D (domainOrigin = 0, StaggeredGrid.h:40).
+
+A scene acts on any object with union_box / union_sphere / union_plane /
+get / set / count (the device Liquid, or the CPU oracle in tests).
+"""
+import numpy as np
+
+
+def grid(n, dim, bw=2):
+    dx = 1.0 / (n - 2 * bw)
+    return dict(dim=dim, res=[n] * dim, dx=dx, origin=[-bw * dx] * dim, bw=bw)
+
+
+def face_positions(g, axis):
+    """Positions of the faces of `axis` (StaggeredGrid::makeFaceGrids,
+    StaggeredGrid.h:104-120), x-fastest, shape (count, dim)."""
+    dim, dx = g["dim"], g["dx"]
+    n = [r + (1 if a == axis else 0) for a, r in enumerate(g["res"])]
+    o = [g["origin"][a] + (0.0 if a == axis else dx / 2) for a in range(dim)]
+    axes = [o[a] + np.arange(n[a]) * dx for a in range(dim)]
+    mesh = np.meshgrid(*axes[::-1], indexing="ij")[::-1]
+    return np.stack([m.ravel() for m in mesh], axis=1)
+
+
+def cell_positions(g):
+    dim, dx = g["dim"], g["dx"]
+    axes = [g["origin"][a] + dx / 2 + np.arange(g["res"][a]) * dx for a in range(dim)]
+    mesh = np.meshgrid(*axes[::-1], indexing="ij")[::-1]
+    return np.stack([m.ravel() for m in mesh], axis=1)
+
+
+def dam_break(sim, g, extents=(0.4, 0.6, 0.5)):
+    """Liquid box x < e0, y < e1 (, z < e2) anchored at the domain origin,
+    as buildCase1 anchors its box (LevelSetLiquidBuilder.cpp:79)."""
+    dim = g["dim"]
+    sim.union_box([0.0] * dim, list(extents[:dim]))
+
+
+def droplet(sim, g, seed=0, radius=0.08, pool=0.3, height=0.7, speed=-2.0):
+    """Pool phi = y - pool union a sphere at (x, height, z), x, z ~ U[0.3, 0.7]
+    (seeded), the drop moving at `speed` along y (buildCase0 pattern,
+    LevelSetLiquidBuilder.cpp:57-62)."""
+    dim = g["dim"]
+    rng = np.random.default_rng(seed)
+    c = [float(rng.uniform(0.3, 0.7)), height] + ([float(rng.uniform(0.3, 0.7))] if dim == 3 else [])
+    sim.union_plane([0.0, pool, 0.0][:dim], [0.0, 1.0, 0.0][:dim])
+    sim.union_sphere(c, radius)
+    u = np.zeros(sim.count(9))
+    off = 0
+    for a in range(dim):
+        pos = face_positions(g, a)
+        if a == 1:
+            inside = np.linalg.norm(pos - np.array(c)[None, :], axis=1) < radius
+            u[off:off + len(pos)][inside] = speed
+        off += len(pos)
+    sim.set(9, u)
+    return c

@@ -1,0 +1,201 @@
+/*
+ * garm_gpu.h — C-ABI of the MI355X (gfx950) implementation of the GARM-LS
+ * per-timestep grid path.
+ *
+ * The reference composes its step from compile-time C++ templates called by
+ * LevelSetLiquid<Dim> (Source/Core/LevelSetLiquid.cpp:608-698) behind the
+ * virtual Simulation interface (Common/Util/Simulation.h:21-33).  It has no
+ * FFI.  This header is the seam a host program binds instead: one opaque
+ * context owns every field on the device, and each entry point below replaces
+ * one reference operator (cited per entry).  Host buffers cross the boundary
+ * in the reference's own array-of-structures layout (the layout of its .sav
+ * files, GridBasedData.h:102-103, SGridBasedData.h:54-55); the device keeps
+ * structure-of-arrays.
+ *
+ * Conventions
+ *  - Every call returns GARM_OK (0) or a negative garm_status; the reference
+ *    calls std::exit(-1) instead (LinearSystem.h:26-37).  The message of the
+ *    last failure is available from garm_last_error().
+ *  - A context is bound to one HIP device and one HIP stream and is not
+ *    thread-safe (the reference is single-threaded at this level too).
+ *  - Cell arrays are x-fastest (Grid.h:40-44).  Face arrays of all axes are
+ *    concatenated axis 0..Dim-1 (SGridBasedData.h:54-55).
+ */
+#ifndef GARM_GPU_H
+#define GARM_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct garm_ctx garm_ctx;
+
+typedef enum garm_status {
+	GARM_OK = 0,
+	GARM_EINVAL = -1,    /* bad argument, size or field id */
+	GARM_EHIP = -2,      /* a HIP runtime call failed */
+	GARM_ENOCONV = -3,   /* the pressure solve did not reach its tolerance */
+	GARM_ENOMEM = -4,    /* device allocation failed */
+	GARM_ENODEV = -5     /* no usable gfx950 device */
+} garm_status;
+
+/* Field identifiers for garm_upload / garm_download.  Element type is double
+ * unless stated; "D" is the dimension. */
+typedef enum garm_field {
+	GARM_PHI = 0,            /* _levelSet, 1/cell                              */
+	GARM_PHI_GRAD = 1,       /* _levelSetGrad, D/cell                          */
+	GARM_REFMAP = 2,         /* _refMap (xi), D/cell                           */
+	GARM_REFMAP_GRAD = 3,    /* _refMapGrad (J = d xi / dx), D*D/cell, col-major */
+	GARM_PHI_TMP = 4,        /* _tmpLevelSet, 1/cell                           */
+	GARM_PHI_ORI = 5,        /* _oriLevelSet, 1/cell                           */
+	GARM_PHI_REF = 6,        /* _refLevelSet, 1/cell                           */
+	GARM_PHI_REF_GRAD = 7,   /* _refLevelSetGrad, D/cell                       */
+	GARM_ADV_REGION = 8,     /* _advRegion, uint8 1/cell                       */
+	GARM_VELOCITY = 9,       /* _velocity, faces                               */
+	GARM_CELL_DIST = 10,     /* _boundary.cellDist, 1/cell                     */
+	GARM_NODE_DIST = 11,     /* _boundary.nodeDist, 1/node                     */
+	GARM_FRACTION = 12,      /* _boundary.fraction, faces                      */
+	GARM_SOLID_VELOCITY = 13,/* _boundary.velocity, faces                      */
+	GARM_NORMAL = 14,        /* _boundary.normal, faces                        */
+	GARM_PRESSURE = 15,      /* Projector::_rdP scattered to cells (0 elsewhere), 1/cell */
+	GARM_NUM_FIELDS = 16
+} garm_field;
+
+/* Pressure solver selection for garm_project. */
+typedef enum garm_pcg_mode {
+	GARM_PCG_FP64 = 0,       /* FP64 Jacobi-preconditioned CG                         */
+	GARM_PCG_MIXED = 1,      /* FP32 Jacobi-PCG inner solves inside FP64 iterative
+	                            refinement (FP64 residual b - Ax and FP64 reductions) */
+	GARM_PCG_LEARNED = 2     /* flexible PCG preconditioned by the learned U-Net,
+	                            falling back to GARM_PCG_FP64 when the residual stalls */
+} garm_pcg_mode;
+
+typedef struct garm_pcg_opts {
+	int mode;                /* garm_pcg_mode */
+	double tolerance;        /* relative residual ||b-Ax||/||b|| (reference: 1e-5, Projector.cpp:161) */
+	int max_iterations;      /* reference: 2000 */
+	int check_interval;      /* iterations between host convergence checks (>=1) */
+} garm_pcg_opts;
+
+typedef struct garm_pcg_stats {
+	int unknowns;            /* number of pressure unknowns (Projector::setUnknowns) */
+	int iterations;          /* CG iterations, counted as the reference's solver does */
+	int outer_iterations;    /* refinement sweeps (mixed) */
+	int fell_back;           /* 1 if the learned solve fell back to plain PCG */
+	double relative_residual;/* final ||b-Ax||/||b|| computed in FP64 */
+} garm_pcg_stats;
+
+/* Precision of advection and reinitialisation. */
+typedef enum garm_precision { GARM_FP64 = 0, GARM_FP32 = 1 } garm_precision;
+
+typedef struct garm_step_opts {
+	int reinit_interval;     /* WENO5 reinitialisation every k steps (reference: every step, k=1) */
+	int reinit_iterations;   /* reference: 30 (LevelSetLiquid.cpp:1156) */
+	double reinit_cfl;       /* reference: 0.9 */
+	int advect_precision;    /* garm_precision for advection + reinit */
+	garm_pcg_opts pcg;
+} garm_step_opts;
+
+typedef struct garm_step_stats {
+	double t_advect_ms;      /* reference bucket tadvect  (LevelSetLiquid.h:103-109) */
+	double t_restart_ms;     /* trestart: reinit + restart check + restart */
+	double t_project_ms;     /* tproject */
+	double t_extrap_ms;      /* textrap */
+	double t_total_ms;
+	int restarted;           /* 1 if the reference map was restarted this step */
+	int reinitialized;
+	garm_pcg_stats pcg;
+} garm_step_stats;
+
+/* ---- context ---------------------------------------------------------- */
+
+/* Replaces the StaggeredGrid + LevelSetLiquid constructors
+ * (StaggeredGrid.h:27-36, LevelSetLiquid.cpp:47-83) and the domain-box
+ * Boundary constructor + finish (Boundary.cpp:7-35, 79-87).  res[] holds the
+ * cell counts (res[2] ignored for dim 2), origin[] the grid's lower corner. */
+int garm_create(garm_ctx **out, int dim, const int res[3], double dx, const double origin[3], int boundary_width);
+void garm_destroy(garm_ctx *ctx);
+const char *garm_last_error(const garm_ctx *ctx);
+/* Last error of a failed garm_create (no context exists then). */
+const char *garm_create_error(void);
+
+/* Number of elements (not bytes) of a field. */
+size_t garm_field_count(const garm_ctx *ctx, int field);
+/* Host -> device / device -> host in the reference's AoS layout.  n is the
+ * element count and must equal garm_field_count. */
+int garm_upload(garm_ctx *ctx, int field, const void *host, size_t n);
+int garm_download(garm_ctx *ctx, int field, void *host, size_t n);
+
+/* LevelSetLiquid::unionLevelSet with ImplicitBox / ImplicitSphere /
+ * ImplicitPlane (LevelSetLiquid.cpp:700-707, ImplicitSurface.h:7-59,117-133). */
+int garm_union_box(garm_ctx *ctx, const double min_corner[3], const double lengths[3]);
+int garm_union_sphere(garm_ctx *ctx, const double center[3], double radius);
+int garm_union_plane(garm_ctx *ctx, const double position[3], const double normal[3]);
+
+/* ---- operators (each replaces the cited reference function) ------------ */
+
+/* LevelSetLiquid::initialize (LevelSetLiquid.cpp:528-598) with WENO5
+ * reinitialisation. */
+int garm_initialize(garm_ctx *ctx, const garm_step_opts *opts);
+/* advectFields part 1: dxi, J by RK2 + Hermite (LevelSetLiquid.cpp:739-748,
+ * Advection.h:24-31,59-82,147-160, HermiteIntrpl.h). */
+int garm_advect_refmap(garm_ctx *ctx, double dt);
+/* advectFields part 2: cubic semi-Lagrangian velocity (LevelSetLiquid.cpp:825,
+ * Advection.h:15-22,33-43,137-145). */
+int garm_advect_velocity(garm_ctx *ctx, double dt);
+/* generateCurrentFields (LevelSetLiquid.cpp:1218-1244). */
+int garm_compose_levelset(garm_ctx *ctx);
+/* Reinitialization::solve<5> on _levelSet (Reinitialization.h:20-80). */
+int garm_reinit_weno5(garm_ctx *ctx, int iterations, double cfl);
+/* extrapolateFieldsToCollider (LevelSetLiquid.cpp:1086-1137). */
+int garm_extrapolate_collider(garm_ctx *ctx);
+/* extrapolateFieldsToAir (LevelSetLiquid.cpp:1059-1084). */
+int garm_extrapolate_air(garm_ctx *ctx);
+/* Boundary::enforce(velocity) (Boundary.cpp:89-120). */
+int garm_enforce_boundary(garm_ctx *ctx);
+/* applyBodyForces, gravity only (LevelSetLiquid.cpp:834-841). */
+int garm_apply_gravity(garm_ctx *ctx, double dt);
+/* checkReferenceMap (LevelSetLiquid.cpp:1246-1283); *ok = its return value. */
+int garm_check_refmap(garm_ctx *ctx, double threshold, double band_steps, double ratio, int *ok);
+/* restart without Hybrid Newton (LevelSetLiquid.cpp:1331-1352). */
+int garm_restart(garm_ctx *ctx);
+/* Projector::project (Projector.cpp:13-20) with matrix-free PCG in place of
+ * Eigen's IC-preconditioned CG (LinearSystem.h:23-39). */
+int garm_project(garm_ctx *ctx, const garm_pcg_opts *opts, garm_pcg_stats *stats);
+/* LevelSetLiquid::advance (LevelSetLiquid.cpp:608-698); step_index selects
+ * the reinitialisation cadence (reinit when step_index % reinit_interval == 0). */
+int garm_step(garm_ctx *ctx, double dt, int step_index, const garm_step_opts *opts, garm_step_stats *stats);
+/* SGridBasedData::normMax of the velocity (SGridBasedData.h:48-52), used by
+ * getTimeStep (LevelSetLiquid.h:129). */
+int garm_max_speed(garm_ctx *ctx, double *out);
+/* Blocks until all work queued on the context's stream is done. */
+int garm_synchronize(garm_ctx *ctx);
+
+/* ---- instrumentation (no counterpart in the reference) ----------------- */
+
+/* Kernels that garm_time_kernel can time in isolation. */
+typedef enum garm_kernel {
+	GARM_K_ADVECT_REFMAP = 0,   /* GARM xi/J advection (one launch)            */
+	GARM_K_ADVECT_VELOCITY = 1, /* cubic velocity advection (Dim launches)     */
+	GARM_K_COMPOSE = 2,         /* phi / grad-phi composition                  */
+	GARM_K_REINIT_STAGE = 3,    /* one WENO5 Runge-Kutta stage                 */
+	GARM_K_PCG_SPMV = 4,        /* matrix-free Poisson operator + p.Ap partials */
+	GARM_K_PCG_UPDATE = 5,      /* x, r update + |r|^2, r.z partials           */
+	GARM_NUM_KERNELS = 6
+} garm_kernel;
+
+/* Launches kernel `which` `reps` times on the context's stream on the current
+ * state, bracketed by HIP events on that stream, and returns the mean launch
+ * time and the kernel's algorithmic HBM bytes per launch (each array it must
+ * touch read or written once).  The state is left as a normal step would
+ * leave it.  The PCG kernels need a preceding garm_project / garm_step. */
+int garm_time_kernel(garm_ctx *ctx, int which, int reps, double *avg_ms, double *bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,179 @@
+"""ctypes binding of the CPU oracle (oracle/lib/libgarm_oracle.so).
+
+ORACLE — test infrastructure only.  Imported by tests/, __graft_entry__.smoke()
+and bench.py's CPU-baseline leg, never by the product package.  Method names
+follow the reference operators they restate (see garm_oracle.cpp).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libgarm_oracle.so")
+
+# field ids (same numbering as include/garm_gpu.h)
+PHI, PHI_GRAD, REFMAP, REFMAP_GRAD, PHI_TMP, PHI_ORI, PHI_REF, PHI_REF_GRAD, ADV_REGION, VELOCITY, \
+    CELL_DIST, NODE_DIST, FRACTION, SOLID_VELOCITY, NORMAL, PRESSURE = range(16)
+
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_IC0 = 0, 1, 2
+
+_lib = None
+
+
+def build():
+    """Compile the oracle with its Makefile (g++ + OpenMP)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, d, i, sz = ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_size_t
+        ip, dp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+        llp = ctypes.POINTER(ctypes.c_longlong)
+        sig = {
+            "ora_create": (vp, [i, ip, d, dp, i]),
+            "ora_destroy": (None, [vp]),
+            "ora_set_threads": (None, [i]),
+            "ora_get_threads": (i, []),
+            "ora_field_count": (sz, [vp, i]),
+            "ora_set": (i, [vp, i, vp, sz]),
+            "ora_get": (i, [vp, i, vp, sz]),
+            "ora_union_box": (None, [vp, dp, dp]),
+            "ora_union_sphere": (None, [vp, dp, d]),
+            "ora_union_plane": (None, [vp, dp, dp]),
+            "ora_initialize": (i, [vp, i, d, i, d, i]),
+            "ora_advect_refmap": (None, [vp, d]),
+            "ora_advect_velocity": (None, [vp, d]),
+            "ora_compose": (None, [vp]),
+            "ora_reinit": (None, [vp, i, d]),
+            "ora_extrapolate_collider": (None, [vp]),
+            "ora_extrapolate_air": (None, [vp]),
+            "ora_enforce_boundary": (None, [vp]),
+            "ora_gravity": (None, [vp, d]),
+            "ora_check_refmap": (i, [vp, d, d, d, llp, llp]),
+            "ora_restart": (None, [vp]),
+            "ora_project": (i, [vp, i, d, i, ip, dp, ip]),
+            "ora_step": (i, [vp, d, i, i, d, i, d, i, ip, ip, dp, ip]),
+            "ora_max_speed": (d, [vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _d3(v):
+    v = list(v) + [0.0] * (3 - len(v))
+    return (ctypes.c_double * 3)(*v)
+
+
+class Oracle:
+    """One CPU LevelSetLiquid<Dim> (LevelSetLiquid.h:17-121) restated."""
+
+    def __init__(self, dim, res, dx, origin, bw=2):
+        L = lib()
+        r = list(res) + [1] * (3 - len(res))
+        self.dim = dim
+        self.h = L.ora_create(dim, (ctypes.c_int * 3)(*r), dx, _d3(origin), bw)
+        if not self.h:
+            raise ValueError("ora_create failed")
+
+    def close(self):
+        if self.h:
+            lib().ora_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def set_threads(n):
+        lib().ora_set_threads(int(n))
+
+    @staticmethod
+    def threads():
+        return lib().ora_get_threads()
+
+    def count(self, field):
+        return lib().ora_field_count(self.h, field)
+
+    def get(self, field):
+        n = self.count(field)
+        a = np.empty(n, dtype=np.uint8 if field == ADV_REGION else np.float64)
+        if lib().ora_get(self.h, field, a.ctypes.data, n) != 0:
+            raise ValueError(f"ora_get({field}) failed")
+        return a
+
+    def set(self, field, arr):
+        a = np.ascontiguousarray(arr, dtype=np.uint8 if field == ADV_REGION else np.float64).ravel()
+        if lib().ora_set(self.h, field, a.ctypes.data, a.size) != 0:
+            raise ValueError(f"ora_set({field}) size {a.size} != {self.count(field)}")
+
+    def union_box(self, mn, lengths):
+        lib().ora_union_box(self.h, _d3(mn), _d3(lengths))
+
+    def union_sphere(self, c, r):
+        lib().ora_union_sphere(self.h, _d3(c), r)
+
+    def union_plane(self, p, n):
+        lib().ora_union_plane(self.h, _d3(p), _d3(n))
+
+    def initialize(self, reinit_iters=30, cfl=0.9, precond=PRECOND_JACOBI, tol=1e-10, max_it=20000):
+        return lib().ora_initialize(self.h, reinit_iters, cfl, precond, tol, max_it)
+
+    def advect_refmap(self, dt):
+        lib().ora_advect_refmap(self.h, dt)
+
+    def advect_velocity(self, dt):
+        lib().ora_advect_velocity(self.h, dt)
+
+    def compose(self):
+        lib().ora_compose(self.h)
+
+    def reinit(self, iters=30, cfl=0.9):
+        lib().ora_reinit(self.h, iters, cfl)
+
+    def extrapolate_collider(self):
+        lib().ora_extrapolate_collider(self.h)
+
+    def extrapolate_air(self):
+        lib().ora_extrapolate_air(self.h)
+
+    def enforce_boundary(self):
+        lib().ora_enforce_boundary(self.h)
+
+    def gravity(self, dt):
+        lib().ora_gravity(self.h, dt)
+
+    def check_refmap(self, threshold=0.883022221559489, band=4.0, ratio=1e-5):
+        tot, cnt = ctypes.c_longlong(), ctypes.c_longlong()
+        ok = lib().ora_check_refmap(self.h, threshold, band, ratio, ctypes.byref(tot), ctypes.byref(cnt))
+        return bool(ok), tot.value, cnt.value
+
+    def restart(self):
+        lib().ora_restart(self.h)
+
+    def project(self, precond=PRECOND_JACOBI, tol=1e-10, max_it=20000):
+        it, unk, err = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        rc = lib().ora_project(self.h, precond, tol, max_it, ctypes.byref(it), ctypes.byref(err), ctypes.byref(unk))
+        return dict(rc=rc, iterations=it.value, residual=err.value, unknowns=unk.value)
+
+    def step(self, dt, do_reinit=True, reinit_iters=30, cfl=0.9, precond=PRECOND_JACOBI, tol=1e-10, max_it=20000):
+        rs, it, unk, err = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        rc = lib().ora_step(self.h, dt, int(do_reinit), reinit_iters, cfl, precond, tol, max_it,
+                            ctypes.byref(rs), ctypes.byref(it), ctypes.byref(err), ctypes.byref(unk))
+        return dict(rc=rc, restarted=rs.value, iterations=it.value, residual=err.value, unknowns=unk.value)
+
+    def max_speed(self):
+        return lib().ora_max_speed(self.h)

@@ -1,0 +1,197 @@
+"""Parity of the HIP path against the CPU oracle, through the C-ABI.
+
+Each operator is applied on the device and on the oracle to the same state
+(an oracle dam break advanced a few steps, uploaded to the device) and the
+outputs are compared in relative L-infinity.  The bar is 1e-10 for every
+floating-point field (FP64 path); masks and counts must be identical.
+"""
+import numpy as np
+import pytest
+
+from garm_amd import capi, scenes
+from garm_amd.driver import FrameClock
+from helpers import copy_state, evolved_oracle, make_gpu, make_oracle, rel_linf, say
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+CASES = [(2, 48), (3, 24)]
+IDS = ["2d-48", "3d-24"]
+
+
+def _pair(dim, n, steps=4):
+    g = scenes.grid(n, dim)
+    o = evolved_oracle(g, steps=steps)
+    d = make_gpu(g)
+    copy_state(o, d)
+    return g, o, d
+
+
+def _cmp(o, d, fields, tol=TOL, exact=()):
+    for f in fields:
+        a, b = d.get(f), o.get(f)
+        if f in exact or f == capi.ADV_REGION:
+            assert np.array_equal(a, b), f"{capi.FIELD_NAMES[f]} differs in {np.count_nonzero(a != b)} entries"
+        else:
+            e = rel_linf(a, b)
+            assert e <= tol, f"{capi.FIELD_NAMES[f]}: rel Linf {e:.3e} > {tol:.0e}"
+
+
+@pytest.mark.parametrize("dim,n", CASES, ids=IDS)
+def test_boundary_identical(dim, n):
+    g = scenes.grid(n, dim)
+    say(f"boundary {dim}d n={n}")
+    o, d = make_oracle(g), make_gpu(g)
+    for f in (capi.CELL_DIST, capi.NODE_DIST, capi.FRACTION, capi.NORMAL, capi.SOLID_VELOCITY):
+        assert np.array_equal(d.get(f), o.get(f)), capi.FIELD_NAMES[f]
+
+
+@pytest.mark.parametrize("dim,n", CASES, ids=IDS)
+def test_scene_levelsets(dim, n):
+    g = scenes.grid(n, dim)
+    say(f"scene {dim}d n={n}")
+    o, d = make_oracle(g), make_gpu(g)
+    for s in (o, d):
+        scenes.droplet(s, g, seed=3)
+        scenes.dam_break(s, g)
+    assert rel_linf(d.get(capi.PHI), o.get(capi.PHI)) <= 1e-15
+    assert np.array_equal(d.get(capi.VELOCITY), o.get(capi.VELOCITY))
+
+
+@pytest.mark.parametrize("dim,n", CASES, ids=IDS)
+def test_initialize(dim, n):
+    g = scenes.grid(n, dim)
+    say(f"initialize {dim}d n={n}")
+    o, d = make_oracle(g), make_gpu(g)
+    for s in (o, d):
+        scenes.dam_break(s, g)
+    assert o.initialize(tol=1e-13) == 0
+    d.initialize(capi.step_opts(tol=1e-13, max_iterations=20000))
+    _cmp(o, d, [capi.PHI, capi.PHI_GRAD, capi.PHI_REF, capi.PHI_REF_GRAD, capi.REFMAP, capi.REFMAP_GRAD,
+                capi.ADV_REGION, capi.VELOCITY, capi.PRESSURE])
+
+
+@pytest.mark.parametrize("dim,n", CASES, ids=IDS)
+def test_advect_refmap(dim, n):
+    g, o, d = _pair(dim, n)
+    dt = 0.5 * g["dx"] / max(o.max_speed(), 1e-12)
+    say(f"advect_refmap {dim}d n={n} dt={dt:.3e}")
+    o.advect_refmap(dt)
+    d.advect_refmap(dt)
+    _cmp(o, d, [capi.REFMAP, capi.REFMAP_GRAD])
+
+
+@pytest.mark.parametrize("dim,n", CASES, ids=IDS)
+def test_advect_velocity(dim, n):
+    g, o, d = _pair(dim, n)
+    dt = 0.8 * g["dx"] / max(o.max_speed(), 1e-12)
+    say(f"advect_velocity {dim}d n={n} dt={dt:.3e}")
+    o.advect_velocity(dt)
+    d.advect_velocity(dt)
+    _cmp(o, d, [capi.VELOCITY])
+
+
+@pytest.mark.parametrize("dim,n", CASES, ids=IDS)
+def test_compose(dim, n):
+    g, o, d = _pair(dim, n)
+    say(f"compose {dim}d n={n}")
+    o.compose()
+    d.compose()
+    _cmp(o, d, [capi.PHI, capi.PHI_GRAD, capi.PHI_TMP, capi.PHI_ORI])
+
+
+@pytest.mark.parametrize("dim,n", CASES, ids=IDS)
+def test_reinit(dim, n):
+    g, o, d = _pair(dim, n)
+    say(f"reinit {dim}d n={n}")
+    o.reinit(30, 0.9)
+    d.reinit(30, 0.9)
+    _cmp(o, d, [capi.PHI])
+
+
+@pytest.mark.parametrize("dim,n", CASES, ids=IDS)
+def test_extrapolations_and_enforce(dim, n):
+    g, o, d = _pair(dim, n)
+    say(f"extrapolate {dim}d n={n}")
+    o.extrapolate_collider()
+    d.extrapolate_collider()
+    _cmp(o, d, [capi.PHI, capi.PHI_ORI])
+    o.extrapolate_air()
+    d.extrapolate_air()
+    _cmp(o, d, [capi.REFMAP, capi.ADV_REGION, capi.VELOCITY])
+    o.enforce_boundary()
+    d.enforce_boundary()
+    _cmp(o, d, [capi.VELOCITY])
+    o.gravity(0.01)
+    d.gravity(0.01)
+    _cmp(o, d, [capi.VELOCITY])
+
+
+@pytest.mark.parametrize("dim,n", CASES, ids=IDS)
+def test_check_refmap_and_restart(dim, n):
+    g, o, d = _pair(dim, n, steps=6)
+    say(f"check/restart {dim}d n={n}")
+    ok_o, tot, cnt = o.check_refmap()
+    assert d.check_refmap() == ok_o
+    # a threshold of 0 flags every band cell that is not exactly orthogonal
+    ok0, tot0, cnt0 = o.check_refmap(threshold=0.0)
+    assert d.check_refmap(threshold=0.0) == ok0
+    o.restart()
+    d.restart()
+    _cmp(o, d, [capi.PHI_REF, capi.PHI_REF_GRAD, capi.REFMAP, capi.REFMAP_GRAD])
+
+
+@pytest.mark.parametrize("dim,n", CASES, ids=IDS)
+def test_project(dim, n):
+    g, o, d = _pair(dim, n)
+    say(f"project {dim}d n={n}")
+    o.gravity(0.02)
+    d.gravity(0.02)
+    ro = o.project(tol=1e-13, max_it=50000)
+    rd = d.project(tol=1e-13, max_iterations=50000)
+    assert ro["rc"] == 0 and rd["rc"] == 0
+    assert rd["unknowns"] == ro["unknowns"]
+    assert abs(rd["iterations"] - ro["iterations"]) <= max(3, ro["iterations"] // 50)
+    _cmp(o, d, [capi.PRESSURE, capi.VELOCITY])
+
+
+@pytest.mark.parametrize("dim,n", CASES, ids=IDS)
+def test_full_step_from_identical_inputs(dim, n):
+    g, o, d = _pair(dim, n)
+    opts = capi.step_opts(reinit_interval=1, tol=1e-13, max_iterations=50000)
+    clk = FrameClock(g["dx"], 1.0)
+    for s in range(3):
+        copy_state(o, d)
+        dt = clk.next_dt(o.max_speed())
+        say(f"step {s} {dim}d n={n} dt={dt:.3e}")
+        ro = o.step(dt, do_reinit=True, tol=1e-13, max_it=50000)
+        rd = d.step(dt, 0, opts)
+        assert ro["rc"] == 0
+        assert rd["restarted"] == ro["restarted"]
+        assert rd["pcg"]["unknowns"] == ro["unknowns"]
+        _cmp(o, d, [capi.PHI, capi.PHI_GRAD, capi.VELOCITY, capi.PRESSURE, capi.REFMAP, capi.REFMAP_GRAD,
+                    capi.PHI_TMP, capi.PHI_ORI, capi.ADV_REGION])
+
+
+@pytest.mark.parametrize("dim,n", CASES, ids=IDS)
+def test_free_trajectory(dim, n):
+    """20 steps from the same initial state with no re-synchronisation."""
+    g = scenes.grid(n, dim)
+    o, d = make_oracle(g), make_gpu(g)
+    for s in (o, d):
+        scenes.dam_break(s, g)
+    assert o.initialize(tol=1e-12) == 0
+    opts = capi.step_opts(reinit_interval=5, tol=1e-12, max_iterations=50000)
+    d.initialize(opts)
+    clk = FrameClock(g["dx"], 1.0)
+    for s in range(20):
+        vo, vd = o.max_speed(), d.max_speed()
+        assert abs(vd - vo) <= 1e-9 * max(vo, 1e-30)
+        dt = clk.next_dt(vo)
+        say(f"trajectory step {s} {dim}d n={n}")
+        ro = o.step(dt, do_reinit=(s % 5 == 0), tol=1e-12, max_it=50000)
+        d.step(dt, s, opts)
+        assert ro["rc"] == 0
+    for f in (capi.PHI, capi.VELOCITY, capi.PRESSURE):
+        e = rel_linf(d.get(f), o.get(f))
+        assert e <= 1e-8, f"{capi.FIELD_NAMES[f]} drifted {e:.2e} after 20 steps"
