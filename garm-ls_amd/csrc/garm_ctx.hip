@@ -240,7 +240,12 @@ int garm_create(garm_ctx **out, int dim, const int res[3], double dx, const doub
 			&& alloc(c, F.bvel[a], F.Nf[a]) && alloc(c, F.bnormal[a], F.Nf[a]) && alloc(c, F.layer_f[a], F.Nf[a]);
 	for (int e = 0; e < dim * dim && ok; e++) ok = alloc(c, F.J[e], N) && alloc(c, F.J_new[e], N);
 	for (int m = 0; m < (dim == 3 ? 4 : 1) && ok; m++) ok = alloc(c, F.phi_ref_mixed[m], N);
-	F.partials_cap = 1 << 16;
+	// reduction partials: two per workgroup of a cell-grid launch, or three
+	// times 1024 for the |u|max reduction, whichever is larger
+	{
+		const dim3 gd = gridDims(F.cell, dim);
+		F.partials_cap = std::max<size_t>(size_t(2) * gd.x * gd.y * gd.z + 64, 4096);
+	}
 	ok = ok && alloc(c, F.partials, F.partials_cap) && alloc(c, F.dscalars, 64);
 	ok = ok && hipHostMalloc(&c->host_scalars, 64 * sizeof(double)) == hipSuccess;
 	for (int i = 0; i < 8 && ok; i++) ok = hipEventCreate(&c->ev[i]) == hipSuccess;

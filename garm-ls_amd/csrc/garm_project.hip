@@ -70,7 +70,11 @@ __global__ __launch_bounds__(kBlock) void kAssemble(GridDev g, FaceFracs fr, Fac
 		const double w = 1 - fr.a[ax][fid];
 		int nbIdx = -1;
 		double nbW = 0;
-		if (w > 0) {
+		// an unknown always has its 2D neighbours inside the grid when the
+		// outermost faces are solid (fraction 1); an uploaded fraction field
+		// that opens an outer face is treated as closed there
+		const bool nbInside = q[ax] >= 0 && q[ax] < g.n[ax];
+		if (w > 0 && nbInside) {
 			const size_t qid = gidx<D>(g, q[0], q[1], q[2]);
 			const int cc = grid2mat[qid];
 			if (cc >= 0) {
@@ -84,7 +88,7 @@ __global__ __launch_bounds__(kBlock) void kAssemble(GridDev g, FaceFracs fr, Fac
 			}
 			div += side * w * uf.a[ax][fid];
 		}
-		if (w < 1) div += side * (1 - w) * bv.a[ax][fid];
+		if (w < 1 || !nbInside) div += side * (w < 1 ? 1 - w : 1.0) * bv.a[ax][fid];
 		P.nb[o][r] = nbIdx;
 		P.w[o][r] = nbW;
 	}
