@@ -33,16 +33,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(args):
+def run_gpu(args, mode_name):
     g = scenes.grid(args.n, 3)
-    mode = {"fp64": capi.PCG_FP64, "mixed": capi.PCG_MIXED, "learned": capi.PCG_LEARNED}[args.mode]
+    mode = {"fp64": capi.PCG_FP64, "mixed": capi.PCG_MIXED, "learned": capi.PCG_LEARNED}[mode_name]
     opts = capi.step_opts(reinit_interval=args.reinit_interval, tol=args.tol, max_iterations=args.max_iterations,
                           check_interval=args.check_interval, mode=mode)
     t0 = time.time()
     sim = capi.Liquid(3, g["res"], g["dx"], g["origin"], g["bw"])
     scenes.dam_break(sim, g)
     sim.initialize(opts)
-    log(f"[bench] {args.n}^3 initialised in {time.time() - t0:.1f}s")
+    log(f"[bench] {mode_name}: {args.n}^3 initialised in {time.time() - t0:.1f}s")
     clk = FrameClock(g["dx"], args.cfl)
     step = 0
     for _ in range(args.warmup):
@@ -65,12 +65,16 @@ def run_gpu(args):
 
 def roofline(sim, stats, args):
     K = len(stats)
-    iters = sum(s["pcg"]["iterations"] + 1 for s in stats) / K    # SpMV launches per step
+    iters = sum(s["pcg"]["iterations"] + s["pcg"]["outer_iterations"] for s in stats) / K    # SpMV launches per step
+    mixed = args.mode != "fp64"
     reinit_stages = 3 * 30 * sum(s["reinitialized"] for s in stats) / K
     per_step = {capi.K_ADVECT_REFMAP: 1, capi.K_ADVECT_VELOCITY: 1, capi.K_COMPOSE: 1 + sum(s["restarted"] for s in stats) / K,
-                capi.K_REINIT_STAGE: reinit_stages, capi.K_PCG_SPMV: iters, capi.K_PCG_UPDATE: iters}
+                capi.K_REINIT_STAGE: reinit_stages, capi.K_PCG_SPMV: 0 if mixed else iters, capi.K_PCG_UPDATE: 0 if mixed else iters,
+                capi.K_PCG32_SPMV: iters if mixed else 0, capi.K_PCG32_UPDATE: iters if mixed else 0}
     kernels = {}
-    for k in range(capi.K_PCG_UPDATE + 1):
+    for k in per_step:
+        if per_step[k] == 0:
+            continue
         ms, by = sim.time_kernel(k, reps=args.kernel_reps)
         kernels[capi.KERNEL_NAMES[k]] = dict(avg_ms=ms, bytes=by, gbs=by / ms / 1e6, per_step=per_step[k],
                                             ms_per_step=ms * per_step[k])
@@ -125,7 +129,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--mode", choices=["fp64", "mixed", "learned"], default="fp64")
+    ap.add_argument("--mode", choices=["fp64", "mixed", "learned"], default="mixed")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 run beside a mixed/learned one")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iterations", type=int, default=20000)
     ap.add_argument("--check-interval", type=int, default=16)
@@ -141,9 +146,17 @@ def main():
               "but not built", file=sys.stderr)
         sys.exit(2)
 
-    g, sim, stats, elapsed = run_gpu(args)
-    K = len(stats)
     cells = args.n ** 3
+    plain = None
+    if args.mode != "fp64" and not args.no_plain:
+        g, sim, stats, elapsed = run_gpu(args, "fp64")
+        sim.close()
+        K = len(stats)
+        plain = {"value": round(cells * K / elapsed, 1), "ms_per_step": round(1000 * elapsed / K, 3),
+                 "pcg_iterations_mean": sum(s["pcg"]["iterations"] for s in stats) / K,
+                 "phases_ms": {k: round(sum(s[k] for s in stats) / K, 3) for k in ("t_advect_ms", "t_restart_ms", "t_project_ms", "t_extrap_ms")}}
+    g, sim, stats, elapsed = run_gpu(args, args.mode)
+    K = len(stats)
     value = cells * K / elapsed
     phases = {k: round(sum(s[k] for s in stats) / K, 3) for k in ("t_advect_ms", "t_restart_ms", "t_project_ms", "t_extrap_ms", "t_total_ms")}
     pcg_iters = [s["pcg"]["iterations"] for s in stats]
@@ -158,7 +171,11 @@ def main():
                    "pcg": args.mode, "tolerance": args.tol, "reinit_interval": args.reinit_interval,
                    "cells": cells},
         "phases_ms": phases,
-        "pcg_iterations": {"mean": sum(pcg_iters) / K, "min": min(pcg_iters), "max": max(pcg_iters)},
+        "pcg_iterations": {"mean": sum(pcg_iters) / K, "min": min(pcg_iters), "max": max(pcg_iters),
+                           "outer_mean": sum(s["pcg"]["outer_iterations"] for s in stats) / K},
+        "pcg_residual_max": max(s["pcg"]["relative_residual"] for s in stats),
+        "plain_fp64": plain,
+        "speedup_vs_plain_fp64": round(value / plain["value"], 3) if plain else None,
         "restarts": sum(s["restarted"] for s in stats),
         "kernels": {k: {kk: round(vv, 5) if isinstance(vv, float) else vv for kk, vv in v.items()} for k, v in kernels.items()},
         "roofline": roof,

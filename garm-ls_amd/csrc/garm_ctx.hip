@@ -135,7 +135,7 @@ int doProject(garm_ctx *c, const garm_pcg_opts &o, garm_pcg_stats *st)
 {
 	garm_pcg_stats local{};
 	if (!st) st = &local;
-	if (o.mode != GARM_PCG_FP64) return fail(c, GARM_EINVAL, "only the FP64 PCG mode is available in this build");
+	if (o.mode != GARM_PCG_FP64 && o.mode != GARM_PCG_MIXED) return fail(c, GARM_EINVAL, "unknown PCG mode");
 	if (!(o.tolerance > 0) || o.max_iterations < 0) return fail(c, GARM_EINVAL, "bad PCG tolerance or iteration limit");
 	std::string err;
 	const int rc = runProjection(c->F, c->P, o, st, c->s, err);
@@ -579,9 +579,10 @@ int garm_time_kernel(garm_ctx *c, int which, int reps, double *avg_ms, double *b
 	for (int a = 0; a < D; a++) NF += F.Nf[a];
 	const double N = double(F.N);
 	float ms = 0;
-	if (which == GARM_K_PCG_SPMV || which == GARM_K_PCG_UPDATE) {
-		rc = timePcgKernel(D, c->P, which == GARM_K_PCG_SPMV ? 0 : 1, reps, c->s, c->ev[0], c->ev[1], &ms, bytes);
-		if (rc) return fail(c, rc, "PCG kernels need a preceding projection");
+	if (which >= GARM_K_PCG_SPMV) {
+		const int k = which == GARM_K_PCG_SPMV ? 0 : which == GARM_K_PCG_UPDATE ? 1 : which == GARM_K_PCG32_SPMV ? 2 : 3;
+		rc = timeSolverKernel(D, c->P.rows, c->P.rows32, c->P.n, k, reps, c->P.K, c->s, c->ev[0], c->ev[1], &ms, bytes);
+		if (rc) return fail(c, rc, "solver kernels need a preceding projection (FP32 ones a mixed-precision one)");
 	} else {
 		const double dt = 0.5 * F.cell.dx;   // backtrace length of half a cell
 		(void)hipEventRecord(c->ev[0], c->s);

@@ -1,0 +1,59 @@
+// Krylov solvers for the ghost-fluid Poisson system on gfx950.
+#pragma once
+
+#include "garm_common.hpp"
+#include "../../include/garm_gpu.h"
+
+#include <string>
+
+namespace garm {
+
+// The Poisson operator of Projector::buildLinearSystem (Projector.cpp:30-68)
+// as rows: (A p)_r = diag_r p_r - sum_o w[o]_r p[nb[o]_r], nb = -1 where the
+// neighbour is not an unknown (its ghost-fluid weight is folded into diag).
+template <typename T>
+struct Rows {
+	int *nb[6];
+	T *w[6];
+	T *diag, *invdiag;
+};
+
+// device scalar slots (double) and state slots (int)
+enum {
+	kScAbsNew = 0, kScAlpha = 1, kScBeta = 2, kScRn2 = 3, kScThreshold = 4, kScBnorm2 = 5, kScTol = 6,
+	kScOuterRn2 = 7, kScOuterThreshold = 8, kScInnerTol = 9
+};
+enum { kStDone = 0, kStIter = 1, kStMaxIter = 2, kStOuterDone = 3, kStSweep = 4, kStCounter = 5 };
+
+// Device-resident solver workspace; vectors sized for `cap` unknowns.
+struct Krylov {
+	size_t cap = 0;
+	double *x = nullptr, *r = nullptr, *p = nullptr, *y = nullptr;    // FP64 vectors
+	float *r32 = nullptr, *p32 = nullptr, *y32 = nullptr, *e32 = nullptr;   // FP32 inner vectors
+	double *partials = nullptr;
+	double *sc = nullptr;
+	int *state = nullptr;
+	int *host_state = nullptr;   // pinned
+	double *host_sc = nullptr;   // pinned
+
+	~Krylov();
+	void release();
+	bool reserve(size_t n, bool mixed, std::string &err);
+};
+
+// FP64 Jacobi-preconditioned CG, Eigen's conjugate_gradient restated
+// (zero initial guess, ||r||^2 < tol^2 ||b||^2, LinearSystem.h:23-39).
+int solvePcg64(int D, const Rows<double> &A, const double *b, int n, const garm_pcg_opts &o, Krylov &K,
+	hipStream_t s, garm_pcg_stats *st, std::string &err);
+
+// FP64 iterative refinement around FP32 Jacobi-PCG inner solves: the outer
+// residual r = b - A x, the solution x and every reduction stay FP64.
+int solveMixed(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const garm_pcg_opts &o,
+	Krylov &K, hipStream_t s, garm_pcg_stats *st, std::string &err);
+
+// Launch one solver kernel `reps` times for timing (which: 0 FP64 SpMV,
+// 1 FP64 x/r update, 2 FP32 SpMV, 3 FP32 update); x and r are left unchanged.
+int timeSolverKernel(int D, const Rows<double> &A64, const Rows<float> &A32, int n, int which, int reps, Krylov &K,
+	hipStream_t s, hipEvent_t e0, hipEvent_t e1, float *ms, double *bytes);
+
+} // namespace garm

@@ -49,7 +49,7 @@ __global__ void kCompactUnknowns(const int *flags, int *grid2mat, int *mat2grid,
 // Projector::buildLinearSystem (Projector.cpp:30-68), one unknown per thread.
 template <int D>
 __global__ __launch_bounds__(kBlock) void kAssemble(GridDev g, FaceFracs fr, FaceFracs uf, FaceFracs bv, const double *phi,
-	const int *grid2mat, const int *mat2grid, int n, PoissonRows P)
+	const int *grid2mat, const int *mat2grid, int n, Rows<double> P, Rows<float> P32, double *rhs)
 {
 	const int r = int(blockIdx.x * blockDim.x + threadIdx.x);
 	if (r >= n) return;
@@ -91,10 +91,13 @@ __global__ __launch_bounds__(kBlock) void kAssemble(GridDev g, FaceFracs fr, Fac
 		if (w < 1 || !nbInside) div += side * (w < 1 ? 1 - w : 1.0) * bv.a[ax][fid];
 		P.nb[o][r] = nbIdx;
 		P.w[o][r] = nbW;
+		if (P32.w[0]) P32.w[o][r] = float(nbW);
 	}
-	P.rhs[r] = div;
+	rhs[r] = div;
 	P.diag[r] = diag;
-	P.invdiag[r] = diag != 0 ? 1 / diag : 1.0;   // DiagonalPreconditioner (Eigen)
+	const double inv = diag != 0 ? 1 / diag : 1.0;   // DiagonalPreconditioner (Eigen)
+	P.invdiag[r] = inv;
+	if (P32.w[0]) { P32.diag[r] = float(diag); P32.invdiag[r] = float(inv); }
 }
 
 // Projector::applyPressureGradient (Projector.cpp:91-123), no pressure jump.
@@ -131,134 +134,6 @@ __global__ void kScatterPressure(const int *grid2mat, const double *x, double *o
 	}
 }
 
-// ------------------------------------------------------------------- PCG
-// Device scalar slots (PcgScalars): see garm_project.hpp.
-
-// y = A p (diag p - sum w p_nb) and the block partial of p.y.
-template <int D>
-__global__ __launch_bounds__(kBlock) void kSpmvDot(PoissonRows P, int n, const double *p, double *y, double *partials, const int *state)
-{
-	if (state[kStDone]) return;
-	const int r = int(blockIdx.x * blockDim.x + threadIdx.x);
-	double v[1] = {0};
-	if (r < n) {
-		const double pr = p[r];
-		double s = P.diag[r] * pr;
-#pragma unroll
-		for (int o = 0; o < 2 * D; o++) {
-			const int c = P.nb[o][r];
-			if (c >= 0) s += -P.w[o][r] * p[c];
-		}
-		y[r] = s;
-		v[0] = pr * s;
-	}
-	double out[1];
-	blockSum<1>(v, out);
-	if (threadIdx.x == 0) partials[blockIdx.x] = out[0];
-}
-
-// alpha = (r.z) / (p.Ap)
-__global__ __launch_bounds__(kBlock) void kAlpha(const double *partials, int nb, double *sc, const int *state)
-{
-	if (state[kStDone]) return;
-	double v[1] = {0};
-	for (int i = threadIdx.x; i < nb; i += blockDim.x) v[0] += partials[i];
-	double o[1];
-	blockSum<1>(v, o);
-	if (threadIdx.x == 0) sc[kScAlpha] = sc[kScAbsNew] / o[0];
-}
-
-// x += alpha p; r -= alpha Ap; partials of r.r and r.(M^-1 r)
-__global__ __launch_bounds__(kBlock) void kUpdateXR(int n, double *x, double *r, const double *p, const double *y, const double *invdiag,
-	double *partials, const double *sc, const int *state)
-{
-	if (state[kStDone]) return;
-	const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
-	double v[2] = {0, 0};
-	if (i < n) {
-		const double alpha = sc[kScAlpha];
-		x[i] += alpha * p[i];
-		const double ri = r[i] - alpha * y[i];
-		r[i] = ri;
-		v[0] = ri * ri;
-		v[1] = ri * (invdiag[i] * ri);
-	}
-	double o[2];
-	blockSum<2>(v, o);
-	if (threadIdx.x == 0) { partials[2 * blockIdx.x] = o[0]; partials[2 * blockIdx.x + 1] = o[1]; }
-}
-
-// Convergence test ||r||^2 < threshold, then beta = (r.z)_new / (r.z)_old.
-__global__ __launch_bounds__(kBlock) void kBeta(const double *partials, int nb, double *sc, int *state)
-{
-	if (state[kStDone]) return;
-	double v[2] = {0, 0};
-	for (int i = threadIdx.x; i < nb; i += blockDim.x) { v[0] += partials[2 * i]; v[1] += partials[2 * i + 1]; }
-	double o[2];
-	blockSum<2>(v, o);
-	if (threadIdx.x == 0) {
-		sc[kScRn2] = o[0];
-		if (o[0] < sc[kScThreshold]) { state[kStDone] = 1; return; }
-		const double absOld = sc[kScAbsNew];
-		sc[kScAbsNew] = o[1];
-		sc[kScBeta] = o[1] / absOld;
-		const int it = state[kStIter] + 1;
-		state[kStIter] = it;
-		if (it >= state[kStMaxIter]) state[kStDone] = 1;
-	}
-}
-
-// p = M^-1 r + beta p
-__global__ void kUpdateP(int n, double *p, const double *r, const double *invdiag, const double *sc, const int *state)
-{
-	if (state[kStDone]) return;
-	const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
-	if (i < n) p[i] = invdiag[i] * r[i] + sc[kScBeta] * p[i];
-}
-
-// Initial state: x = 0, r = b, p = M^-1 r; partials of b.b and r.p.
-__global__ __launch_bounds__(kBlock) void kPcgStart(int n, const double *b, const double *invdiag, double *x, double *r, double *p, double *partials)
-{
-	const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
-	double v[2] = {0, 0};
-	if (i < n) {
-		const double bi = b[i];
-		x[i] = 0;
-		r[i] = bi;
-		const double z = invdiag[i] * bi;
-		p[i] = z;
-		v[0] = bi * bi;
-		v[1] = bi * z;
-	}
-	double o[2];
-	blockSum<2>(v, o);
-	if (threadIdx.x == 0) { partials[2 * blockIdx.x] = o[0]; partials[2 * blockIdx.x + 1] = o[1]; }
-}
-
-// Tolerance handling of LinearSystem::solve (LinearSystem.h:31) and the
-// start-up tests of Eigen's conjugate_gradient.
-__global__ __launch_bounds__(kBlock) void kPcgStartFinal(const double *partials, int nb, double tol, int maxIt, double *sc, int *state)
-{
-	double v[2] = {0, 0};
-	for (int i = threadIdx.x; i < nb; i += blockDim.x) { v[0] += partials[2 * i]; v[1] += partials[2 * i + 1]; }
-	double o[2];
-	blockSum<2>(v, o);
-	if (threadIdx.x == 0) {
-		const double b2 = o[0];
-		sc[kScBnorm2] = b2;
-		sc[kScRn2] = b2;
-		sc[kScAbsNew] = o[1];
-		state[kStIter] = 0;
-		state[kStMaxIter] = maxIt;
-		state[kStDone] = 0;
-		if (b2 == 0) { state[kStDone] = 1; sc[kScThreshold] = 0; sc[kScTol] = tol; return; }
-		const double t = fmax(tol, DBL_EPSILON / sqrt(b2));
-		sc[kScTol] = t;
-		sc[kScThreshold] = fmax(t * t * b2, DBL_MIN);
-		if (b2 < sc[kScThreshold] || maxIt <= 0) state[kStDone] = 1;
-	}
-}
-
 // ---------------------------------------------------------------- driver
 static unsigned linBlocks(size_t n)
 {
@@ -272,10 +147,10 @@ void Projection::release()
 {
 	auto fr = [](auto *&p) { if (p) { (void)hipFree(p); p = nullptr; } };
 	fr(flags); fr(grid2mat); fr(mat2grid); fr(scan_tmp);
-	for (int o = 0; o < 6; o++) { fr(rows.nb[o]); fr(rows.w[o]); }
-	fr(rows.diag); fr(rows.invdiag); fr(rows.rhs);
-	fr(x); fr(r); fr(p); fr(y); fr(partials); fr(sc); fr(state);
-	if (host_state) { (void)hipHostFree(host_state); host_state = nullptr; }
+	for (int o = 0; o < 6; o++) { fr(rows.nb[o]); fr(rows.w[o]); fr(rows32.w[o]); }
+	fr(rows.diag); fr(rows.invdiag); fr(rhs); fr(rows32.diag); fr(rows32.invdiag);
+	for (int o = 0; o < 6; o++) rows32.nb[o] = nullptr;
+	K.release();
 	cap = 0;
 }
 
@@ -290,29 +165,32 @@ bool Projection::reserveGrid(size_t N, std::string &err)
 	if (hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, flags, grid2mat, int(N)) != hipSuccess) { err = "scan sizing failed"; return false; }
 	scan_bytes = bytes;
 	if (hipMalloc(&scan_tmp, bytes ? bytes : 16) != hipSuccess) { err = "hipMalloc failed for scan storage"; return false; }
-	if (hipMalloc(&sc, 32 * sizeof(double)) != hipSuccess || hipMalloc(&state, 8 * sizeof(int)) != hipSuccess) { err = "hipMalloc failed"; return false; }
-	if (hipHostMalloc(&host_state, 8 * sizeof(int)) != hipSuccess) { err = "hipHostMalloc failed"; return false; }
 	return true;
 }
 
-bool Projection::reserveRows(int D, size_t n, std::string &err)
+bool Projection::reserveRows(int D, size_t n, bool mixed, std::string &err)
 {
-	if (n <= cap && mat2grid) return true;
-	size_t want = std::max<size_t>(n + n / 4, 1024);
 	auto fr = [](auto *&p) { if (p) { (void)hipFree(p); p = nullptr; } };
-	fr(mat2grid);
-	for (int o = 0; o < 6; o++) { fr(rows.nb[o]); fr(rows.w[o]); }
-	fr(rows.diag); fr(rows.invdiag); fr(rows.rhs); fr(x); fr(r); fr(p); fr(y); fr(partials);
-	bool ok = hipMalloc(&mat2grid, want * sizeof(int)) == hipSuccess;
-	for (int o = 0; o < 2 * D && ok; o++)
-		ok = hipMalloc(&rows.nb[o], want * sizeof(int)) == hipSuccess && hipMalloc(&rows.w[o], want * sizeof(double)) == hipSuccess;
-	for (double **q : {&rows.diag, &rows.invdiag, &rows.rhs, &x, &r, &p, &y})
-		ok = ok && hipMalloc(q, want * sizeof(double)) == hipSuccess;
-	const size_t nb = (want + kBlock - 1) / kBlock + 1;
-	ok = ok && hipMalloc(&partials, 2 * nb * sizeof(double)) == hipSuccess;
-	if (!ok) { err = "hipMalloc failed for the pressure system"; return false; }
-	cap = want;
-	return true;
+	if (n > cap || !mat2grid) {
+		const size_t want = std::max<size_t>(n + n / 4, 1024);
+		fr(mat2grid);
+		for (int o = 0; o < 6; o++) { fr(rows.nb[o]); fr(rows.w[o]); fr(rows32.w[o]); }
+		fr(rows.diag); fr(rows.invdiag); fr(rhs); fr(rows32.diag); fr(rows32.invdiag);
+		bool ok = hipMalloc(&mat2grid, want * sizeof(int)) == hipSuccess;
+		for (int o = 0; o < 2 * D && ok; o++)
+			ok = hipMalloc(&rows.nb[o], want * sizeof(int)) == hipSuccess && hipMalloc(&rows.w[o], want * sizeof(double)) == hipSuccess;
+		for (double **q : {&rows.diag, &rows.invdiag, &rhs}) ok = ok && hipMalloc(q, want * sizeof(double)) == hipSuccess;
+		if (!ok) { err = "hipMalloc failed for the pressure system"; return false; }
+		cap = want;
+	}
+	if (mixed && !rows32.diag) {
+		bool ok = true;
+		for (int o = 0; o < 2 * D && ok; o++) ok = hipMalloc(&rows32.w[o], cap * sizeof(float)) == hipSuccess;
+		ok = ok && hipMalloc(&rows32.diag, cap * sizeof(float)) == hipSuccess && hipMalloc(&rows32.invdiag, cap * sizeof(float)) == hipSuccess;
+		if (!ok) { err = "hipMalloc failed for the FP32 pressure system"; return false; }
+	}
+	for (int o = 0; o < 6; o++) rows32.nb[o] = rows.nb[o];
+	return K.reserve(n, mixed, err);
 }
 
 static FaceFracs faceArrays(const Fields &F, double *const *a)
@@ -323,7 +201,7 @@ static FaceFracs faceArrays(const Fields &F, double *const *a)
 }
 
 template <int D>
-static int numberUnknowns(const Fields &F, Projection &P, hipStream_t s, std::string &err)
+static int numberUnknowns(const Fields &F, Projection &P, bool mixed, hipStream_t s, std::string &err)
 {
 	hipLaunchKernelGGL(kUnknownFlags<D>, gridDims(F.cell, D), gridBlock(), 0, s, F.cell, faceArrays(F, F.frac), (const double *)F.phi, P.flags);
 	size_t bytes = P.scan_bytes;
@@ -333,61 +211,27 @@ static int numberUnknowns(const Fields &F, Projection &P, hipStream_t s, std::st
 		hipMemcpyAsync(&last[1], P.flags + F.N - 1, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
 		hipStreamSynchronize(s) != hipSuccess) { err = "reading the unknown count failed"; return -1; }
 	const int n = last[0] + last[1];
-	if (!P.reserveRows(D, size_t(n), err)) return -1;
+	if (!P.reserveRows(D, size_t(n), mixed, err)) return -1;
 	hipLaunchKernelGGL(kCompactUnknowns, dim3(linBlocks(F.N)), dim3(kBlock), 0, s, (const int *)P.flags, P.grid2mat, P.mat2grid, F.N);
 	return n;
 }
 
 template <int D>
-static void assemble(const Fields &F, Projection &P, int n, hipStream_t s)
+static void assemble(const Fields &F, Projection &P, int n, bool mixed, hipStream_t s)
 {
 	if (n == 0) return;
+	Rows<float> r32 = P.rows32;
+	if (!mixed) r32.w[0] = nullptr;
 	hipLaunchKernelGGL(kAssemble<D>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, F.cell, faceArrays(F, F.frac),
-		faceArrays(F, F.u), faceArrays(F, F.bvel), (const double *)F.phi, (const int *)P.grid2mat, (const int *)P.mat2grid, n, P.rows);
-}
-
-template <int D>
-static int pcg64(Projection &P, int n, double tol, int maxIt, int check, hipStream_t s, garm_pcg_stats *st, std::string &err)
-{
-	const int nb = (n + kBlock - 1) / kBlock;
-	const dim3 g1(nb > 0 ? nb : 1), b1(kBlock);
-	hipLaunchKernelGGL(kPcgStart, g1, b1, 0, s, n, (const double *)P.rows.rhs, (const double *)P.rows.invdiag, P.x, P.r, P.p, P.partials);
-	hipLaunchKernelGGL(kPcgStartFinal, dim3(1), b1, 0, s, (const double *)P.partials, nb, tol, maxIt, P.sc, P.state);
-	int launched = 0;
-	for (;;) {
-		if (hipMemcpyAsync(P.host_state, P.state, 4 * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
-			hipStreamSynchronize(s) != hipSuccess) { err = "PCG state readback failed"; return GARM_EHIP; }
-		if (P.host_state[kStDone] || launched >= maxIt) break;
-		const int k = std::min(check, maxIt - launched);
-		for (int q = 0; q < k; q++) {
-			hipLaunchKernelGGL(kSpmvDot<D>, g1, b1, 0, s, P.rows, n, (const double *)P.p, P.y, P.partials, (const int *)P.state);
-			hipLaunchKernelGGL(kAlpha, dim3(1), b1, 0, s, (const double *)P.partials, nb, P.sc, (const int *)P.state);
-			hipLaunchKernelGGL(kUpdateXR, g1, b1, 0, s, n, P.x, P.r, (const double *)P.p, (const double *)P.y, (const double *)P.rows.invdiag,
-				P.partials, (const double *)P.sc, (const int *)P.state);
-			hipLaunchKernelGGL(kBeta, dim3(1), b1, 0, s, (const double *)P.partials, nb, P.sc, P.state);
-			hipLaunchKernelGGL(kUpdateP, g1, b1, 0, s, n, P.p, (const double *)P.r, (const double *)P.rows.invdiag, (const double *)P.sc,
-				(const int *)P.state);
-		}
-		launched += k;
-	}
-	double hs[8];
-	if (hipMemcpyAsync(hs, P.sc, 8 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
-		hipMemcpyAsync(P.host_state, P.state, 4 * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
-		hipStreamSynchronize(s) != hipSuccess) { err = "PCG result readback failed"; return GARM_EHIP; }
-	st->iterations = P.host_state[kStIter];
-	st->relative_residual = hs[kScBnorm2] > 0 ? std::sqrt(hs[kScRn2] / hs[kScBnorm2]) : 0.0;
-	if (st->relative_residual > hs[kScTol]) {
-		err = "PCG did not reach the requested tolerance";
-		return GARM_ENOCONV;
-	}
-	return GARM_OK;
+		faceArrays(F, F.u), faceArrays(F, F.bvel), (const double *)F.phi, (const int *)P.grid2mat, (const int *)P.mat2grid, n, P.rows, r32, P.rhs);
 }
 
 template <int D>
 static int projectD(Fields &F, Projection &P, const garm_pcg_opts &o, garm_pcg_stats *st, hipStream_t s, std::string &err)
 {
+	const bool mixed = o.mode == GARM_PCG_MIXED;
 	if (!P.reserveGrid(F.N, err)) return GARM_ENOMEM;
-	const int n = numberUnknowns<D>(F, P, s, err);
+	const int n = numberUnknowns<D>(F, P, mixed, s, err);
 	if (n < 0) return GARM_EHIP;
 	P.n = n;
 	st->unknowns = n;
@@ -397,14 +241,15 @@ static int projectD(Fields &F, Projection &P, const garm_pcg_opts &o, garm_pcg_s
 	st->relative_residual = 0;
 	int rc = GARM_OK;
 	if (n > 0) {
-		assemble<D>(F, P, n, s);
-		rc = pcg64<D>(P, n, o.tolerance, o.max_iterations, std::max(1, o.check_interval), s, st, err);
+		assemble<D>(F, P, n, mixed, s);
+		if (mixed) rc = solveMixed(D, P.rows, P.rows32, P.rhs, n, o, P.K, s, st, err);
+		else rc = solvePcg64(D, P.rows, P.rhs, n, o, P.K, s, st, err);
 	}
-	// the velocity is updated with whatever the solver reached, as the
-	// reference does before deciding (it exits instead)
+	// the velocity is updated with whatever the solver reached (the reference
+	// exits instead when its solver fails, LinearSystem.h:34-37)
 	for (int a = 0; a < D && n > 0; a++)
 		hipLaunchKernelGGL(kApplyPressureGradient<D>, gridDims(F.face[a], D), gridBlock(), 0, s, F.face[a], F.cell, a, F.u[a],
-			(const double *)F.phi, (const int *)P.grid2mat, (const double *)P.x);
+			(const double *)F.phi, (const int *)P.grid2mat, (const double *)P.K.x);
 	return rc;
 }
 
@@ -413,49 +258,10 @@ int runProjection(Fields &F, Projection &P, const garm_pcg_opts &o, garm_pcg_sta
 	return F.D == 2 ? projectD<2>(F, P, o, st, s, err) : projectD<3>(F, P, o, st, s, err);
 }
 
-// Launch one PCG kernel `reps` times for timing with alpha forced to 0 so x
-// and r keep their values; the solver scalars are restored afterwards.
-int timePcgKernel(int D, Projection &P, int which, int reps, hipStream_t s, hipEvent_t e0, hipEvent_t e1, float *ms, double *bytes)
-{
-	const int n = P.n;
-	if (n <= 0 || !P.sc) return GARM_EINVAL;
-	double sc[32];
-	int st[8];
-	if (hipMemcpyAsync(sc, P.sc, sizeof(sc), hipMemcpyDeviceToHost, s) != hipSuccess ||
-		hipMemcpyAsync(st, P.state, sizeof(st), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
-		return GARM_EHIP;
-	double sc0[32];
-	std::copy(sc, sc + 32, sc0);
-	sc0[kScAlpha] = 0;
-	int st0[8];
-	std::copy(st, st + 8, st0);
-	st0[kStDone] = 0;
-	if (hipMemcpyAsync(P.sc, sc0, sizeof(sc0), hipMemcpyHostToDevice, s) != hipSuccess ||
-		hipMemcpyAsync(P.state, st0, sizeof(st0), hipMemcpyHostToDevice, s) != hipSuccess) return GARM_EHIP;
-	const int nb = (n + kBlock - 1) / kBlock;
-	(void)hipEventRecord(e0, s);
-	for (int q = 0; q < reps; q++) {
-		if (which == 0) {
-			if (D == 2) hipLaunchKernelGGL(kSpmvDot<2>, dim3(nb), dim3(kBlock), 0, s, P.rows, n, (const double *)P.p, P.y, P.partials, (const int *)P.state);
-			else hipLaunchKernelGGL(kSpmvDot<3>, dim3(nb), dim3(kBlock), 0, s, P.rows, n, (const double *)P.p, P.y, P.partials, (const int *)P.state);
-		} else {
-			hipLaunchKernelGGL(kUpdateXR, dim3(nb), dim3(kBlock), 0, s, n, P.x, P.r, (const double *)P.p, (const double *)P.y,
-				(const double *)P.rows.invdiag, P.partials, (const double *)P.sc, (const int *)P.state);
-		}
-	}
-	(void)hipEventRecord(e1, s);
-	if (hipMemcpyAsync(P.sc, sc, sizeof(sc), hipMemcpyHostToDevice, s) != hipSuccess ||
-		hipMemcpyAsync(P.state, st, sizeof(st), hipMemcpyHostToDevice, s) != hipSuccess ||
-		hipStreamSynchronize(s) != hipSuccess) return GARM_EHIP;
-	(void)hipEventElapsedTime(ms, e0, e1);
-	*bytes = which == 0 ? double(n) * (8 + 2 * D * (4 + 8) + 8 + 8) : double(n) * (5 * 8 + 2 * 8);
-	return GARM_OK;
-}
-
 void scatterPressure(const Fields &F, const Projection &P, double *out, hipStream_t s)
 {
 	if (!P.grid2mat || P.n == 0) { launchFill(out, 0.0, F.N, s); return; }
-	hipLaunchKernelGGL(kScatterPressure, dim3(linBlocks(F.N)), dim3(kBlock), 0, s, (const int *)P.grid2mat, (const double *)P.x, out, F.N);
+	hipLaunchKernelGGL(kScatterPressure, dim3(linBlocks(F.N)), dim3(kBlock), 0, s, (const int *)P.grid2mat, (const double *)P.K.x, out, F.N);
 }
 
 } // namespace garm

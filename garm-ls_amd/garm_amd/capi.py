@@ -19,8 +19,9 @@ FIELD_NAMES = ["phi", "phi_grad", "refmap", "refmap_grad", "phi_tmp", "phi_ori",
                "adv_region", "velocity", "cell_dist", "node_dist", "fraction", "solid_velocity", "normal", "pressure"]
 
 # garm_kernel (instrumentation)
-K_ADVECT_REFMAP, K_ADVECT_VELOCITY, K_COMPOSE, K_REINIT_STAGE, K_PCG_SPMV, K_PCG_UPDATE = range(6)
-KERNEL_NAMES = ["advect_refmap", "advect_velocity", "compose", "reinit_stage", "pcg_spmv", "pcg_update"]
+K_ADVECT_REFMAP, K_ADVECT_VELOCITY, K_COMPOSE, K_REINIT_STAGE, K_PCG_SPMV, K_PCG_UPDATE, K_PCG32_SPMV, K_PCG32_UPDATE = range(8)
+KERNEL_NAMES = ["advect_refmap", "advect_velocity", "compose", "reinit_stage", "pcg_spmv", "pcg_update", "pcg32_spmv",
+                "pcg32_update"]
 
 # garm_pcg_mode
 PCG_FP64, PCG_MIXED, PCG_LEARNED = 0, 1, 2
@@ -33,7 +34,7 @@ STATUS_NAMES = {0: "GARM_OK", -1: "GARM_EINVAL", -2: "GARM_EHIP", -3: "GARM_ENOC
 
 class PcgOpts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("tolerance", ctypes.c_double), ("max_iterations", ctypes.c_int),
-                ("check_interval", ctypes.c_int)]
+                ("check_interval", ctypes.c_int), ("inner_tolerance", ctypes.c_double)]
 
 
 class PcgStats(ctypes.Structure):
@@ -127,7 +128,7 @@ def _d3(v):
 
 
 def step_opts(reinit_interval=1, reinit_iterations=30, reinit_cfl=0.9, precision=FP64, mode=PCG_FP64, tol=1e-5,
-              max_iterations=2000, check_interval=8):
+              max_iterations=2000, check_interval=8, inner_tolerance=0.0):
     o = StepOpts()
     o.reinit_interval = reinit_interval
     o.reinit_iterations = reinit_iterations
@@ -137,6 +138,7 @@ def step_opts(reinit_interval=1, reinit_iterations=30, reinit_cfl=0.9, precision
     o.pcg.tolerance = tol
     o.pcg.max_iterations = max_iterations
     o.pcg.check_interval = check_interval
+    o.pcg.inner_tolerance = inner_tolerance
     return o
 
 
@@ -226,8 +228,9 @@ class Liquid:
     def restart(self):
         self._check(lib().garm_restart(self.h))
 
-    def project(self, mode=PCG_FP64, tol=1e-5, max_iterations=2000, check_interval=8, raise_on_noconv=True):
-        o = PcgOpts(mode, tol, max_iterations, check_interval)
+    def project(self, mode=PCG_FP64, tol=1e-5, max_iterations=2000, check_interval=8, raise_on_noconv=True,
+                inner_tolerance=0.0):
+        o = PcgOpts(mode, tol, max_iterations, check_interval, inner_tolerance)
         st = PcgStats()
         rc = lib().garm_project(self.h, ctypes.byref(o), ctypes.byref(st))
         if rc != GARM_OK and (raise_on_noconv or rc != GARM_ENOCONV):

@@ -78,6 +78,8 @@ typedef struct garm_pcg_opts {
 	double tolerance;        /* relative residual ||b-Ax||/||b|| (reference: 1e-5, Projector.cpp:161) */
 	int max_iterations;      /* reference: 2000 */
 	int check_interval;      /* iterations between host convergence checks (>=1) */
+	double inner_tolerance;  /* GARM_PCG_MIXED: relative residual reduction of each FP32
+	                            inner solve (<= 0 selects 1e-3) */
 } garm_pcg_opts;
 
 typedef struct garm_pcg_stats {
@@ -184,7 +186,9 @@ typedef enum garm_kernel {
 	GARM_K_REINIT_STAGE = 3,    /* one WENO5 Runge-Kutta stage                 */
 	GARM_K_PCG_SPMV = 4,        /* matrix-free Poisson operator + p.Ap partials */
 	GARM_K_PCG_UPDATE = 5,      /* x, r update + |r|^2, r.z partials           */
-	GARM_NUM_KERNELS = 6
+	GARM_K_PCG32_SPMV = 6,      /* the same two in FP32 (mixed-precision inner) */
+	GARM_K_PCG32_UPDATE = 7,
+	GARM_NUM_KERNELS = 8
 } garm_kernel;
 
 /* Launches kernel `which` `reps` times on the context's stream on the current

@@ -195,3 +195,41 @@ def test_free_trajectory(dim, n):
     for f in (capi.PHI, capi.VELOCITY, capi.PRESSURE):
         e = rel_linf(d.get(f), o.get(f))
         assert e <= 1e-8, f"{capi.FIELD_NAMES[f]} drifted {e:.2e} after 20 steps"
+
+
+@pytest.mark.parametrize("dim,n", CASES, ids=IDS)
+def test_project_mixed_precision_reaches_fp64_tolerance(dim, n):
+    """FP32 inner PCG inside FP64 iterative refinement: the FP64 residual is
+    driven below 1e-11, so pressure and velocity match the FP64 oracle solved
+    to 1e-13 within the conditioning of the system."""
+    g, o, d = _pair(dim, n)
+    say(f"project mixed {dim}d n={n}")
+    o.gravity(0.02)
+    d.gravity(0.02)
+    ro = o.project(tol=1e-13, max_it=50000)
+    rd = d.project(mode=capi.PCG_MIXED, tol=1e-11, max_iterations=50000, inner_tolerance=1e-3)
+    assert ro["rc"] == 0 and rd["rc"] == 0
+    assert rd["relative_residual"] <= 1e-11
+    assert rd["outer_iterations"] >= 2
+    assert rd["unknowns"] == ro["unknowns"]
+    for f in (capi.PRESSURE, capi.VELOCITY):
+        e = rel_linf(d.get(f), o.get(f))
+        assert e <= 1e-7, f"{capi.FIELD_NAMES[f]}: {e:.2e}"
+
+
+@pytest.mark.parametrize("dim,n", CASES, ids=IDS)
+def test_mixed_step_error_against_fp64(dim, n):
+    """A full step with the mixed solver at the benchmark tolerance (1e-6)
+    stays within that tolerance's reach of the FP64 oracle step."""
+    g, o, d = _pair(dim, n)
+    copy_state(o, d)
+    dt = FrameClock(g["dx"], 1.0).next_dt(o.max_speed())
+    opts = capi.step_opts(reinit_interval=1, tol=1e-6, max_iterations=50000, mode=capi.PCG_MIXED)
+    say(f"mixed step {dim}d n={n}")
+    ro = o.step(dt, do_reinit=True, tol=1e-13, max_it=50000)
+    rd = d.step(dt, 0, opts)
+    assert ro["rc"] == 0 and rd["pcg"]["relative_residual"] <= 1e-6
+    _cmp(o, d, [capi.PHI, capi.PHI_GRAD, capi.REFMAP, capi.REFMAP_GRAD])   # untouched by the solver
+    for f, bound in ((capi.PRESSURE, 1e-3), (capi.VELOCITY, 1e-3)):
+        e = rel_linf(d.get(f), o.get(f))
+        assert e <= bound, f"{capi.FIELD_NAMES[f]}: {e:.2e}"
