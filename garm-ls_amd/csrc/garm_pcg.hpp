@@ -20,8 +20,8 @@ struct Rows {
 
 // device scalar slots (double) and state slots (int)
 enum {
-	kScAbsNew = 0, kScAlpha = 1, kScBeta = 2, kScRn2 = 3, kScThreshold = 4, kScBnorm2 = 5, kScTol = 6,
-	kScOuterRn2 = 7, kScOuterThreshold = 8, kScInnerTol = 9
+	kScRz0 = 0, kScRz1 = 1, kScAlpha = 2, kScBeta = 3, kScRn2 = 4, kScThreshold = 5, kScBnorm2 = 6, kScTol = 7,
+	kScOuterRn2 = 8, kScOuterThreshold = 9
 };
 enum { kStDone = 0, kStIter = 1, kStMaxIter = 2, kStOuterDone = 3, kStSweep = 4, kStCounter = 5 };
 
@@ -29,8 +29,8 @@ enum { kStDone = 0, kStIter = 1, kStMaxIter = 2, kStOuterDone = 3, kStSweep = 4,
 struct Krylov {
 	size_t cap = 0;
 	double *x = nullptr, *r = nullptr, *p = nullptr, *y = nullptr;    // FP64 vectors
-	float *r32 = nullptr, *p32 = nullptr, *y32 = nullptr, *e32 = nullptr;   // FP32 inner vectors
-	double *partials = nullptr;
+	float *r32 = nullptr, *p32 = nullptr, *y32 = nullptr;   // FP32 vectors of the mixed solver
+	double *pa = nullptr, *pb = nullptr;                     // per-workgroup partials
 	double *sc = nullptr;
 	int *state = nullptr;
 	int *host_state = nullptr;   // pinned
@@ -46,8 +46,9 @@ struct Krylov {
 int solvePcg64(int D, const Rows<double> &A, const double *b, int n, const garm_pcg_opts &o, Krylov &K,
 	hipStream_t s, garm_pcg_stats *st, std::string &err);
 
-// FP64 iterative refinement around FP32 Jacobi-PCG inner solves: the outer
-// residual r = b - A x, the solution x and every reduction stay FP64.
+// FP32 Jacobi-PCG with the solution accumulated in FP64 and the residual
+// replaced by the FP64 residual b - A x at every 1e3 reduction; every
+// reduction stays FP64 and convergence is judged on the FP64 residual.
 int solveMixed(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const garm_pcg_opts &o,
 	Krylov &K, hipStream_t s, garm_pcg_stats *st, std::string &err);
 

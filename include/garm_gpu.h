@@ -67,8 +67,9 @@ typedef enum garm_field {
 /* Pressure solver selection for garm_project. */
 typedef enum garm_pcg_mode {
 	GARM_PCG_FP64 = 0,       /* FP64 Jacobi-preconditioned CG                         */
-	GARM_PCG_MIXED = 1,      /* FP32 Jacobi-PCG inner solves inside FP64 iterative
-	                            refinement (FP64 residual b - Ax and FP64 reductions) */
+	GARM_PCG_MIXED = 1,      /* FP32 Jacobi-PCG refined in FP64: solution accumulated in
+	                            FP64, residual replaced by the FP64 b - Ax at every 1e3
+	                            reduction, FP64 reductions, FP64 convergence test */
 	GARM_PCG_LEARNED = 2     /* flexible PCG preconditioned by the learned U-Net,
 	                            falling back to GARM_PCG_FP64 when the residual stalls */
 } garm_pcg_mode;
@@ -78,14 +79,13 @@ typedef struct garm_pcg_opts {
 	double tolerance;        /* relative residual ||b-Ax||/||b|| (reference: 1e-5, Projector.cpp:161) */
 	int max_iterations;      /* reference: 2000 */
 	int check_interval;      /* iterations between host convergence checks (>=1) */
-	double inner_tolerance;  /* GARM_PCG_MIXED: relative residual reduction of each FP32
-	                            inner solve (<= 0 selects 1e-3) */
+	double inner_tolerance;  /* reserved (0) */
 } garm_pcg_opts;
 
 typedef struct garm_pcg_stats {
 	int unknowns;            /* number of pressure unknowns (Projector::setUnknowns) */
 	int iterations;          /* CG iterations, counted as the reference's solver does */
-	int outer_iterations;    /* refinement sweeps (mixed) */
+	int outer_iterations;    /* FP64 residual replacements (mixed) */
 	int fell_back;           /* 1 if the learned solve fell back to plain PCG */
 	double relative_residual;/* final ||b-Ax||/||b|| computed in FP64 */
 } garm_pcg_stats;

@@ -199,18 +199,18 @@ def test_free_trajectory(dim, n):
 
 @pytest.mark.parametrize("dim,n", CASES, ids=IDS)
 def test_project_mixed_precision_reaches_fp64_tolerance(dim, n):
-    """FP32 inner PCG inside FP64 iterative refinement: the FP64 residual is
-    driven below 1e-11, so pressure and velocity match the FP64 oracle solved
-    to 1e-13 within the conditioning of the system."""
+    """FP32 PCG refined in FP64 (FP64 solution, FP64 residual replacement):
+    the FP64 residual is driven below 1e-11, so pressure and velocity match
+    the FP64 oracle solved to 1e-13 within the conditioning of the system."""
     g, o, d = _pair(dim, n)
     say(f"project mixed {dim}d n={n}")
     o.gravity(0.02)
     d.gravity(0.02)
     ro = o.project(tol=1e-13, max_it=50000)
-    rd = d.project(mode=capi.PCG_MIXED, tol=1e-11, max_iterations=50000, inner_tolerance=1e-3)
+    rd = d.project(mode=capi.PCG_MIXED, tol=1e-11, max_iterations=50000)
     assert ro["rc"] == 0 and rd["rc"] == 0
     assert rd["relative_residual"] <= 1e-11
-    assert rd["outer_iterations"] >= 2
+    assert rd["outer_iterations"] >= 3          # 1e-11 needs at least three 1e-3 replacements
     assert rd["unknowns"] == ro["unknowns"]
     for f in (capi.PRESSURE, capi.VELOCITY):
         e = rel_linf(d.get(f), o.get(f))
