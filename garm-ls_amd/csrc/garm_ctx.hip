@@ -238,7 +238,8 @@ int garm_create(garm_ctx **out, int dim, const int res[3], double dx, const doub
 		ok = alloc(c, F.phi_grad[a], N) && alloc(c, F.xi[a], N) && alloc(c, F.xi_new[a], N) && alloc(c, F.phi_ref_grad[a], N)
 			&& alloc(c, F.u[a], F.Nf[a]) && alloc(c, F.u_new[a], F.Nf[a]) && alloc(c, F.frac[a], F.Nf[a])
 			&& alloc(c, F.bvel[a], F.Nf[a]) && alloc(c, F.bnormal[a], F.Nf[a]) && alloc(c, F.layer_f[a], F.Nf[a]);
-	for (int e = 0; e < dim * dim && ok; e++) ok = alloc(c, F.J[e], N) && alloc(c, F.J_new[e], N);
+	for (int e = 0; e < dim * dim && ok; e++) ok = alloc(c, F.J[e], N) && alloc(c, F.J_new[e], N) && alloc(c, F.trace_r[e], N);
+	for (int a = 0; a < dim && ok; a++) ok = alloc(c, F.trace_x[a], N);
 	for (int m = 0; m < (dim == 3 ? 4 : 1) && ok; m++) ok = alloc(c, F.phi_ref_mixed[m], N);
 	// reduction partials: two per workgroup of a cell-grid launch, or three
 	// times 1024 for the |u|max reduction, whichever is larger
@@ -596,7 +597,7 @@ int garm_time_kernel(garm_ctx *c, int which, int reps, double *avg_ms, double *b
 		if ((rc = launched(c, "timed kernel")) || (rc = hipCheck(c, hipEventSynchronize(c->ev[1]), "timed kernel"))) return rc;
 		(void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
 		const int M = D == 3 ? 4 : 1;
-		if (which == GARM_K_ADVECT_REFMAP) *bytes = NF * 8 + N * (D + D * D) * 8 * 2;
+		if (which == GARM_K_ADVECT_REFMAP) *bytes = NF * 8 + N * (D + D * D) * 8 * 2;   // trace scratch not counted
 		else if (which == GARM_K_ADVECT_VELOCITY) *bytes = 2 * NF * 8;
 		else if (which == GARM_K_COMPOSE) *bytes = N * (8 * (D + D * D + 1 + D + M + 1) + 1 + 8 * (3 + D));
 		else *bytes = N * 16;
