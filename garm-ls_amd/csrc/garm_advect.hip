@@ -4,10 +4,8 @@
 // One thread per cell (or face); the velocity gradient and the Hermite mixed
 // derivatives of the reference map are formed on the fly from the clamped
 // neighbours instead of being materialised as the reference does
-// (LevelSetLiquid.cpp:739-742, HermiteIntrpl.h:17-30).  The reference-map
-// advection runs as two launches, the RK2 backtrace (position + 3x3 Jacobian
-// to scratch) and the Hermite stage, so neither needs more than half the
-// register file and each keeps several waves per SIMD in flight.
+// (LevelSetLiquid.cpp:739-742, HermiteIntrpl.h:17-30), so each launch reads
+// u, xi and J once from HBM and writes xi', J' once.
 #include "garm_internal.hpp"
 
 namespace garm {
@@ -108,38 +106,25 @@ __device__ __forceinline__ void hermiteRefMap(const GridDev &g, const double *co
 	for (int e = 0; e < D * D; e++) Gh[e] = 0;
 	const double dx = g.dx, inv = g.inv;
 
-#pragma unroll 1
+#pragma unroll
 	for (int cn = 0; cn < (1 << D); cn++) {
 		const int vx = cn & 1, vy = cn >> 1 & 1, vz = D == 3 ? cn >> 2 : 0;
 		const int qi = clampi(l[0] + vx, g.n[0] - 1), qj = clampi(l[1] + vy, g.n[1] - 1);
 		const int qk = D == 3 ? clampi(l[2] + vz, g.n[2] - 1) : 0;
 		const size_t q = gidx<D>(g, qi, qj, qk);
-		// this corner's 1-D factors (value / slope basis and derivatives)
-		double cb[3][2], cg[3][2];
-#pragma unroll
-		for (int o = 0; o < 2; o++) {
-			cb[0][o] = vx ? B[0][1][o] : B[0][0][o];
-			cg[0][o] = vx ? Bg[0][1][o] : Bg[0][0][o];
-			cb[1][o] = vy ? B[1][1][o] : B[1][0][o];
-			cg[1][o] = vy ? Bg[1][1][o] : Bg[1][0][o];
-			if constexpr (D == 3) {
-				cb[2][o] = vz ? B[2][1][o] : B[2][0][o];
-				cg[2][o] = vz ? Bg[2][1][o] : Bg[2][0][o];
-			}
-		}
 #pragma unroll
 		for (int ord = 0; ord < (1 << D); ord++) {
 			const int ox = ord & 1, oy = ord >> 1 & 1, oz = D == 3 ? ord >> 2 : 0;
-			double wt = cb[0][ox] * cb[1][oy];
+			double wt = B[0][vx][ox] * B[1][vy][oy];
 			double wg[3];
 			if constexpr (D == 2) {
-				wg[0] = cg[0][ox] * cb[1][oy];
-				wg[1] = cb[0][ox] * cg[1][oy];
+				wg[0] = Bg[0][vx][ox] * B[1][vy][oy];
+				wg[1] = B[0][vx][ox] * Bg[1][vy][oy];
 			} else {
-				wt = wt * cb[2][oz];
-				wg[0] = cg[0][ox] * cb[1][oy] * cb[2][oz];
-				wg[1] = cb[0][ox] * cg[1][oy] * cb[2][oz];
-				wg[2] = cb[0][ox] * cb[1][oy] * cg[2][oz];
+				wt = wt * B[2][vz][oz];
+				wg[0] = Bg[0][vx][ox] * B[1][vy][oy] * B[2][vz][oz];
+				wg[1] = B[0][vx][ox] * Bg[1][vy][oy] * B[2][vz][oz];
+				wg[2] = B[0][vx][ox] * B[1][vy][oy] * Bg[2][vz][oz];
 			}
 #pragma unroll
 			for (int r = 0; r < D; r++) {
@@ -176,27 +161,28 @@ __device__ __forceinline__ void hermiteRefMap(const GridDev &g, const double *co
 
 // ------------------------------------------------------------------ kernels
 template <int D>
-struct TraceArgs {
+struct RefMapArgs {
 	GridDev cell;
 	FaceSet U;
-	double *X[3];    // backtraced positions
-	double *R[9];    // RK2 Jacobian of the backtrace, row-major
-	double h;        // -dt
+	const double *xi[3];
+	const double *J[9];
+	double *xi_new[3];
+	double *J_new[9];
+	double h;   // -dt
 };
 
-// Advection::traceDrv<2> (Advection.h:71-82) from every cell centre through
-// the multilinear velocity and its face gradient.  The position and the
-// 3x3 matrix go to scratch so that the Hermite stage runs at a register
-// budget that keeps several waves per SIMD resident.
+// SemiLagrangian<D, 2, HermiteIntrpl<D, VectorDd>>::advect (drv overload,
+// Advection.h:147-160) with Advection::traceDrv<2> (Advection.h:71-82).
 template <int D>
-__global__ __launch_bounds__(kBlock) void kTraceRefMap(TraceArgs<D> A)
+__global__ __launch_bounds__(kBlock) void kAdvectRefMap(RefMapArgs<D> A)
 {
 	const Coord c = threadCoord();
 	if (!inGrid<D>(A.cell, c)) return;
 	const size_t id = gidx<D>(A.cell, c.i, c.j, c.k);
 	const double h = A.h;
 	double x[3] = {A.cell.o[0] + c.i * A.cell.dx, A.cell.o[1] + c.j * A.cell.dx, A.cell.o[2] + c.k * A.cell.dx};
-	double v0[3], G0[9], p1[3], R1[9], v1[3], G1[9];
+
+	double v0[3], G0[9], p1[3], R1[9], v1[3], G1[9], X[3], R[9];
 	sampleVelGrad<D>(A.U, x, v0, G0);
 #pragma unroll
 	for (int a = 0; a < D; a++) p1[a] = x[a] + v0[a] * h;
@@ -206,7 +192,7 @@ __global__ __launch_bounds__(kBlock) void kTraceRefMap(TraceArgs<D> A)
 		for (int b = 0; b < D; b++) R1[a * D + b] = (a == b ? 1.0 : 0.0) + G0[a * D + b] * h;
 	sampleVelGrad<D>(A.U, p1, v1, G1);
 #pragma unroll
-	for (int a = 0; a < D; a++) A.X[a][id] = x[a] + (v0[a] + v1[a]) * h / 2;
+	for (int a = 0; a < D; a++) X[a] = x[a] + (v0[a] + v1[a]) * h / 2;
 #pragma unroll
 	for (int a = 0; a < D; a++)
 #pragma unroll
@@ -214,32 +200,9 @@ __global__ __launch_bounds__(kBlock) void kTraceRefMap(TraceArgs<D> A)
 			double gr = 0;
 #pragma unroll
 			for (int k = 0; k < D; k++) gr += G1[a * D + k] * R1[k * D + b];
-			A.R[a * D + b][id] = (a == b ? 1.0 : 0.0) + (G0[a * D + b] + gr) * h / 2;
+			R[a * D + b] = (a == b ? 1.0 : 0.0) + (G0[a * D + b] + gr) * h / 2;
 		}
-}
 
-template <int D>
-struct RefMapArgs {
-	GridDev cell;
-	const double *X[3];
-	const double *R[9];
-	const double *xi[3];
-	const double *J[9];
-	double *xi_new[3];
-	double *J_new[9];
-};
-
-// SemiLagrangian<D, 2, HermiteIntrpl<D, VectorDd>>::advect (drv overload,
-// Advection.h:147-160): xi' = H[xi](X), J' = grad H[xi](X) R.
-template <int D>
-__global__ __launch_bounds__(kBlock) void kHermiteRefMap(RefMapArgs<D> A)
-{
-	const Coord c = threadCoord();
-	if (!inGrid<D>(A.cell, c)) return;
-	const size_t id = gidx<D>(A.cell, c.i, c.j, c.k);
-	double X[3];
-#pragma unroll
-	for (int a = 0; a < D; a++) X[a] = A.X[a][id];
 	double val[3], Gh[9];
 	hermiteRefMap<D>(A.cell, A.xi, A.J, X, val, Gh);
 #pragma unroll
@@ -250,7 +213,7 @@ __global__ __launch_bounds__(kBlock) void kHermiteRefMap(RefMapArgs<D> A)
 		for (int col = 0; col < D; col++) {
 			double s = 0;
 #pragma unroll
-			for (int k = 0; k < D; k++) s += Gh[k * D + r] * A.R[k * D + col][id];
+			for (int k = 0; k < D; k++) s += Gh[k * D + r] * R[k * D + col];
 			A.J_new[col * D + r][id] = s;
 		}
 }
@@ -439,19 +402,13 @@ static FaceSet faceSet(const Fields &F)
 template <int D>
 static void advectRefMapD(const Fields &F, double dt, hipStream_t s)
 {
-	// backtrace (X, R) to the trace arrays, then the Hermite stage
-	TraceArgs<D> T{};
-	T.cell = F.cell;
-	T.U = faceSet(F);
-	for (int a = 0; a < D; a++) T.X[a] = F.trace_x[a];
-	for (int e = 0; e < D * D; e++) T.R[e] = F.trace_r[e];
-	T.h = -dt;
-	hipLaunchKernelGGL(kTraceRefMap<D>, gridDims(F.cell, D), gridBlock(), 0, s, T);
 	RefMapArgs<D> A{};
 	A.cell = F.cell;
-	for (int a = 0; a < D; a++) { A.X[a] = F.trace_x[a]; A.xi[a] = F.xi[a]; A.xi_new[a] = F.xi_new[a]; }
-	for (int e = 0; e < D * D; e++) { A.R[e] = F.trace_r[e]; A.J[e] = F.J[e]; A.J_new[e] = F.J_new[e]; }
-	hipLaunchKernelGGL(kHermiteRefMap<D>, gridDims(F.cell, D), gridBlock(), 0, s, A);
+	A.U = faceSet(F);
+	for (int a = 0; a < D; a++) { A.xi[a] = F.xi[a]; A.xi_new[a] = F.xi_new[a]; }
+	for (int e = 0; e < D * D; e++) { A.J[e] = F.J[e]; A.J_new[e] = F.J_new[e]; }
+	A.h = -dt;
+	hipLaunchKernelGGL(kAdvectRefMap<D>, gridDims(F.cell, D), gridBlock(), 0, s, A);
 }
 
 void launchAdvectRefMap(const Fields &F, double dt, hipStream_t s)

@@ -232,14 +232,21 @@ int garm_create(garm_ctx **out, int dim, const int res[3], double dx, const doub
 	bool ok = hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) == hipSuccess;
 	const size_t N = F.N;
 	ok = ok && alloc(c, F.phi, N) && alloc(c, F.phi_tmp, N) && alloc(c, F.phi_ori, N) && alloc(c, F.phi_ref, N)
-		&& alloc(c, F.cell_dist, N) && alloc(c, F.node_dist, F.Nn) && alloc(c, F.adv_region, N) && alloc(c, F.layer, N)
+		&& alloc(c, F.cell_dist, N) && alloc(c, F.node_dist, F.Nn) && alloc(c, F.adv_region, N) && alloc(c, F.layer, N + 8)
 		&& alloc(c, F.scratch[0], N) && alloc(c, F.scratch[1], N);
 	for (int a = 0; a < dim && ok; a++)
 		ok = alloc(c, F.phi_grad[a], N) && alloc(c, F.xi[a], N) && alloc(c, F.xi_new[a], N) && alloc(c, F.phi_ref_grad[a], N)
 			&& alloc(c, F.u[a], F.Nf[a]) && alloc(c, F.u_new[a], F.Nf[a]) && alloc(c, F.frac[a], F.Nf[a])
-			&& alloc(c, F.bvel[a], F.Nf[a]) && alloc(c, F.bnormal[a], F.Nf[a]) && alloc(c, F.layer_f[a], F.Nf[a]);
-	for (int e = 0; e < dim * dim && ok; e++) ok = alloc(c, F.J[e], N) && alloc(c, F.J_new[e], N) && alloc(c, F.trace_r[e], N);
-	for (int a = 0; a < dim && ok; a++) ok = alloc(c, F.trace_x[a], N);
+			&& alloc(c, F.bvel[a], F.Nf[a]) && alloc(c, F.bnormal[a], F.Nf[a]) && alloc(c, F.layer_f[a], F.Nf[a] + 8);
+	{
+		// a front never lists an element twice, so the larger of the cell and
+		// face counts bounds every list
+		size_t NFall = 0;
+		for (int a = 0; a < dim; a++) NFall += F.Nf[a];
+		const size_t cap = std::max(N, NFall);
+		ok = ok && alloc(c, F.front_list[0], cap) && alloc(c, F.front_list[1], cap) && alloc(c, F.front_cnt, 4);
+	}
+	for (int e = 0; e < dim * dim && ok; e++) ok = alloc(c, F.J[e], N) && alloc(c, F.J_new[e], N);
 	for (int m = 0; m < (dim == 3 ? 4 : 1) && ok; m++) ok = alloc(c, F.phi_ref_mixed[m], N);
 	// reduction partials: two per workgroup of a cell-grid launch, or three
 	// times 1024 for the |u|max reduction, whichever is larger

@@ -1,0 +1,272 @@
+// Layered extrapolation (Extrapolation.h:44-218) by frontier lists on gfx950.
+//
+// The reference sweeps the whole grid once per layer, each not-yet-reached
+// element taking the mean of its reached neighbours.  Here each element
+// carries the layer at which it is reached (0 fixed, 255 not yet).  One
+// full-grid pass marks the fixed elements and lists those with an unfixed
+// neighbour; every later layer L walks only the list of layer L-1, claims
+// each unreached neighbour with a compare-and-swap on its byte, gives it the
+// mean of its neighbours below layer L (in the reference's ordinal order
+// -x, +x, -y, +y, -z, +z, so the value is the reference's exactly) and lists
+// it for layer L+1.  Work per layer is proportional to the front, not to the
+// grid.  A byte compare-and-swap retries at most once per byte of its word
+// that changes in the same launch, so every claim loop ends.
+#include "garm_internal.hpp"
+
+namespace garm {
+
+constexpr uint8_t kFar = 255;
+constexpr int kFrontGrid = 1024;
+
+// One to three grids whose elements are numbered consecutively.
+struct LayerSpace {
+	int ng;
+	GridDev g[3];
+	uint8_t *layer[3];
+	double *v[3][3];       // v[grid][component]
+	int nv;                // components carried per element
+	int off[4];
+};
+
+enum FixRule { kFixCellDistPositive = 0, kFixAdvRegion = 1, kFixPhiOriNonPositive = 2, kFixLiquidFace = 3 };
+
+struct FixArgs {
+	int rule;
+	GridDev cell;
+	const double *cellDist;
+	const uint8_t *adv;
+	const double *phiOri;
+	const double *frac[3];
+};
+
+template <int D>
+__device__ __forceinline__ bool validIn(const GridDev &g, int i, int j, int k)
+{
+	return i >= 0 && i < g.n[0] && j >= 0 && j < g.n[1] && (D == 2 || (k >= 0 && k < g.n[2]));
+}
+
+template <int D>
+__device__ __forceinline__ bool isFixed(const FixArgs &P, int a, const GridDev &g, int i, int j, int k)
+{
+	const size_t id = gidx<D>(g, i, j, k);
+	switch (P.rule) {
+	case kFixCellDistPositive: return P.cellDist[id] > 0;                 // LevelSetLiquid.cpp:1090-1101
+	case kFixAdvRegion: return P.adv[id] > 0;                              // LevelSetLiquid.cpp:1062-1064
+	case kFixPhiOriNonPositive: return P.phiOri[id] <= 0;                  // LevelSetLiquid.cpp:1068-1070
+	default: {                                                             // LevelSetLiquid.cpp:1072-1083
+		if (!(P.frac[a][id] < 1)) return false;
+		const int i0 = i - (a == 0), j0 = j - (a == 1), k0 = k - (a == 2);
+		if (!validIn<D>(P.cell, i0, j0, k0) || !validIn<D>(P.cell, i, j, k)) return false;
+		return P.phiOri[gidx<D>(P.cell, i0, j0, k0)] <= 0 || P.phiOri[gidx<D>(P.cell, i, j, k)] <= 0;
+	}
+	}
+}
+
+__device__ __forceinline__ void decode(const LayerSpace &S, int gid, int &a, int &i, int &j, int &k)
+{
+	a = 0;
+	while (a + 1 < S.ng && gid >= S.off[a + 1]) a++;
+	const int loc = gid - S.off[a];
+	const GridDev &g = S.g[a];
+	i = loc % g.n[0];
+	const int rest = loc / g.n[0];
+	j = rest % g.n[1];
+	k = rest / g.n[1];
+}
+
+// Full pass over grid `a`: layer 0 for fixed elements, 255 otherwise; fixed
+// elements with an unfixed neighbour start the front.
+template <int D>
+__global__ __launch_bounds__(kBlock) void kFrontInit(LayerSpace S, int a, FixArgs P, int *list, int *cnt)
+{
+	const GridDev &g = S.g[a];
+	const Coord c = threadCoord();
+	if (!inGrid<D>(g, c)) return;
+	const size_t id = gidx<D>(g, c.i, c.j, c.k);
+	const bool fixed = isFixed<D>(P, a, g, c.i, c.j, c.k);
+	S.layer[a][id] = fixed ? 0 : kFar;
+	if (!fixed) return;
+	bool edge = false;
+#pragma unroll
+	for (int o = 0; o < 2 * D; o++) {
+		const int ax = o >> 1, sd = (o & 1) ? 1 : -1;
+		const int qi = c.i + (ax == 0 ? sd : 0), qj = c.j + (ax == 1 ? sd : 0), qk = c.k + (ax == 2 ? sd : 0);
+		if (validIn<D>(g, qi, qj, qk) && !isFixed<D>(P, a, g, qi, qj, qk)) edge = true;
+	}
+	if (edge) list[atomicAdd(cnt, 1)] = S.off[a] + int(id);
+}
+
+__device__ __forceinline__ bool claimByte(uint8_t *base, size_t idx, uint8_t L)
+{
+	const uintptr_t addr = reinterpret_cast<uintptr_t>(base + idx);
+	unsigned int *word = reinterpret_cast<unsigned int *>(addr & ~uintptr_t(3));
+	const int shift = int(addr & 3) * 8;
+	unsigned int old = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	for (;;) {
+		if (((old >> shift) & 0xffu) != kFar) return false;
+		const unsigned int nw = (old & ~(0xffu << shift)) | (unsigned(L) << shift);
+		const unsigned int prev = atomicCAS(word, old, nw);
+		if (prev == old) return true;
+		old = prev;
+	}
+}
+
+// Layer L: claim the unreached neighbours of the layer L-1 front.
+template <int D>
+__global__ __launch_bounds__(kBlock) void kFrontStep(LayerSpace S, int L, const int *listIn, const int *cntIn, int *listOut, int *cntOut, int *cntZero)
+{
+	if (blockIdx.x == 0 && threadIdx.x == 0) *cntZero = 0;
+	const int n = *cntIn;
+	for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+		int a, i, j, k;
+		decode(S, listIn[t], a, i, j, k);
+		const GridDev &g = S.g[a];
+		uint8_t *lay = S.layer[a];
+		for (int o = 0; o < 2 * D; o++) {
+			const int ax = o >> 1, sd = (o & 1) ? 1 : -1;
+			const int qi = i + (ax == 0 ? sd : 0), qj = j + (ax == 1 ? sd : 0), qk = k + (ax == 2 ? sd : 0);
+			if (!validIn<D>(g, qi, qj, qk)) continue;
+			const size_t q = gidx<D>(g, qi, qj, qk);
+			if (lay[q] != kFar || !claimByte(lay, q, uint8_t(L))) continue;
+			// mean over q's neighbours reached before layer L (Extrapolation.h:101-124)
+			int cnt = 0;
+			double s0 = 0, s1 = 0, s2 = 0;
+			for (int p = 0; p < 2 * D; p++) {
+				const int bx = p >> 1, bs = (p & 1) ? 1 : -1;
+				const int ri = qi + (bx == 0 ? bs : 0), rj = qj + (bx == 1 ? bs : 0), rk = qk + (bx == 2 ? bs : 0);
+				if (!validIn<D>(g, ri, rj, rk)) continue;
+				const size_t rr = gidx<D>(g, ri, rj, rk);
+				if (lay[rr] < L) {
+					if (S.nv > 0) s0 += S.v[a][0][rr];
+					if (S.nv > 1) s1 += S.v[a][1][rr];
+					if (S.nv > 2) s2 += S.v[a][2][rr];
+					cnt++;
+				}
+			}
+			if (S.nv > 0) S.v[a][0][q] = s0 / cnt;
+			if (S.nv > 1) S.v[a][1][q] = s1 / cnt;
+			if (S.nv > 2) S.v[a][2][q] = s2 / cnt;
+			listOut[atomicAdd(cntOut, 1)] = S.off[a] + int(q);
+		}
+	}
+}
+
+__global__ void kZeroInts(int *p, int n)
+{
+	if (int(threadIdx.x) < n) p[threadIdx.x] = 0;
+}
+
+// Reference map not reached: the cell centre (Extrapolation.h:210-212).
+template <int D>
+__global__ __launch_bounds__(kBlock) void kFrontFinishRefMap(GridDev g, const uint8_t *layer, double *x0, double *x1, double *x2)
+{
+	const Coord c = threadCoord();
+	if (!inGrid<D>(g, c)) return;
+	const size_t id = gidx<D>(g, c.i, c.j, c.k);
+	if (layer[id] != kFar) return;
+	x0[id] = g.o[0] + c.i * g.dx;
+	x1[id] = g.o[1] + c.j * g.dx;
+	if constexpr (D == 3) x2[id] = g.o[2] + c.k * g.dx;
+}
+
+// _advRegion = reached (Extrapolation.h:130-175 on the phi_ori <= 0 indicator).
+__global__ void kFrontToMask(const uint8_t *layer, uint8_t *adv, size_t n)
+{
+	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+		adv[i] = layer[i] != kFar ? 1 : 0;
+}
+
+// Faces never reached are zeroed ("for velocity only", Extrapolation.h:76-78).
+__global__ void kFrontFinishFaces(const uint8_t *layer, double *u, size_t n)
+{
+	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+		if (layer[i] == kFar) u[i] = 0;
+}
+
+static unsigned linBlocks(size_t n)
+{
+	const size_t b = (n + kBlock - 1) / kBlock;
+	return unsigned(b < 8192 ? (b ? b : 1) : 8192);
+}
+
+// Run `steps` layers over S with fixed elements given by P.
+static void runFront(const Fields &F, const LayerSpace &S, const FixArgs &P, int steps, hipStream_t s)
+{
+	int *cnt = F.front_cnt;
+	hipLaunchKernelGGL(kZeroInts, dim3(1), dim3(64), 0, s, cnt, 3);
+	for (int a = 0; a < S.ng; a++) {
+		if (F.D == 2) hipLaunchKernelGGL(kFrontInit<2>, gridDims(S.g[a], 2), gridBlock(), 0, s, S, a, P, F.front_list[0], cnt);
+		else hipLaunchKernelGGL(kFrontInit<3>, gridDims(S.g[a], 3), gridBlock(), 0, s, S, a, P, F.front_list[0], cnt);
+	}
+	for (int L = 1; L <= steps; L++) {
+		const int *in = F.front_list[(L - 1) & 1];
+		int *out = F.front_list[L & 1];
+		const int *cin = cnt + (L - 1) % 3;
+		int *cout = cnt + L % 3, *cz = cnt + (L + 1) % 3;
+		if (F.D == 2) hipLaunchKernelGGL(kFrontStep<2>, dim3(kFrontGrid), dim3(kBlock), 0, s, S, L, in, cin, out, cout, cz);
+		else hipLaunchKernelGGL(kFrontStep<3>, dim3(kFrontGrid), dim3(kBlock), 0, s, S, L, in, cin, out, cout, cz);
+	}
+}
+
+static LayerSpace cellSpace(const Fields &F, int nv, double *v0, double *v1, double *v2)
+{
+	LayerSpace S{};
+	S.ng = 1;
+	S.g[0] = F.cell;
+	S.layer[0] = F.layer;
+	S.v[0][0] = v0; S.v[0][1] = v1; S.v[0][2] = v2;
+	S.nv = nv;
+	S.off[0] = 0;
+	S.off[1] = int(F.N);
+	return S;
+}
+
+static FixArgs fixArgs(const Fields &F, int rule)
+{
+	FixArgs P{};
+	P.rule = rule;
+	P.cell = F.cell;
+	P.cellDist = F.cell_dist;
+	P.adv = F.adv_region;
+	P.phiOri = F.phi_ori;
+	for (int a = 0; a < F.D; a++) P.frac[a] = F.frac[a];
+	return P;
+}
+
+// extrapolateFieldsToCollider (LevelSetLiquid.cpp:1086-1137): phi_ori and phi,
+// 6 layers (_boundaryExtrapolationRadius, LevelSetLiquid.h:92), fixed where
+// cellDist > 0; both share the fixed set, so one front carries both.
+void launchExtrapolateCollider(const Fields &F, hipStream_t s)
+{
+	runFront(F, cellSpace(F, 2, F.phi_ori, F.phi, nullptr), fixArgs(F, kFixCellDistPositive), 6, s);
+}
+
+// extrapolateFieldsToAir (LevelSetLiquid.cpp:1059-1084): xi from the old
+// advRegion (15 layers, unreached reset to the cell centre), advRegion
+// rebuilt from phi_ori <= 0 (15 layers), u on every face grid (17 layers,
+// unreached zeroed).
+void launchExtrapolateAir(const Fields &F, hipStream_t s)
+{
+	runFront(F, cellSpace(F, F.D, F.xi[0], F.xi[1], F.D == 3 ? F.xi[2] : nullptr), fixArgs(F, kFixAdvRegion), 15, s);
+	if (F.D == 2) hipLaunchKernelGGL(kFrontFinishRefMap<2>, gridDims(F.cell, 2), gridBlock(), 0, s, F.cell, (const uint8_t *)F.layer, F.xi[0], F.xi[1], F.xi[2]);
+	else hipLaunchKernelGGL(kFrontFinishRefMap<3>, gridDims(F.cell, 3), gridBlock(), 0, s, F.cell, (const uint8_t *)F.layer, F.xi[0], F.xi[1], F.xi[2]);
+	runFront(F, cellSpace(F, 0, nullptr, nullptr, nullptr), fixArgs(F, kFixPhiOriNonPositive), 15, s);
+	hipLaunchKernelGGL(kFrontToMask, dim3(linBlocks(F.N)), dim3(kBlock), 0, s, (const uint8_t *)F.layer, F.adv_region, F.N);
+	LayerSpace S{};
+	S.ng = F.D;
+	S.nv = 1;
+	int off = 0;
+	for (int a = 0; a < F.D; a++) {
+		S.g[a] = F.face[a];
+		S.layer[a] = F.layer_f[a];
+		S.v[a][0] = F.u[a];
+		S.off[a] = off;
+		off += int(F.Nf[a]);
+	}
+	S.off[F.D] = off;
+	runFront(F, S, fixArgs(F, kFixLiquidFace), 17, s);
+	for (int a = 0; a < F.D; a++)
+		hipLaunchKernelGGL(kFrontFinishFaces, dim3(linBlocks(F.Nf[a])), dim3(kBlock), 0, s, (const uint8_t *)F.layer_f[a], F.u[a], F.Nf[a]);
+}
+
+} // namespace garm

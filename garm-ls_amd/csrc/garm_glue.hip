@@ -1,18 +1,9 @@
 // Per-step glue of LevelSetLiquid::advance on gfx950: scene level sets,
-// extrapolations, boundary enforcement, gravity, |u|max and the
-// reference-map distortion check.
-//
-// Extrapolation (Extrapolation.h:44-218) advances a front one neighbour layer
-// per iteration, each new value the mean of the already-reached neighbours.
-// Here every element carries the iteration at which it is reached (0 for
-// fixed, 255 for not yet); one launch per layer writes only elements still
-// at 255 and reads only neighbours below the current layer, so a single
-// in-place array replaces the reference's double-buffered copies with no race.
+// boundary enforcement, gravity, |u|max and the reference-map distortion
+// check.  (Extrapolation lives in garm_extrap.hip.)
 #include "garm_internal.hpp"
 
 namespace garm {
-
-constexpr uint8_t kUnreached = 255;
 
 // Nine component arrays passed by value (J = d xi / dx, column-major).
 struct Mat9 { double *p[9]; };
@@ -109,103 +100,6 @@ __global__ void kEnforceLevelSet(double *sdf, const double *cellDist, double tol
 __global__ void kGravity(double *uy, double dv, size_t n)
 {
 	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) uy[i] -= dv;
-}
-
-// ------------------------------------------------------------ extrapolation
-enum CellMask { kMaskCellDistPositive = 0, kMaskAdvRegion = 1, kMaskPhiOriNonPositive = 2 };
-
-template <int D>
-__global__ __launch_bounds__(kBlock) void kLayerInitCells(GridDev g, uint8_t *layer, int mask, const double *cellDist,
-	const uint8_t *adv, const double *phiOri)
-{
-	const Coord c = threadCoord();
-	if (!inGrid<D>(g, c)) return;
-	const size_t id = gidx<D>(g, c.i, c.j, c.k);
-	bool fixed;
-	if (mask == kMaskCellDistPositive) fixed = cellDist[id] > 0;
-	else if (mask == kMaskAdvRegion) fixed = adv[id] > 0;
-	else fixed = phiOri[id] <= 0;
-	layer[id] = fixed ? 0 : kUnreached;
-}
-
-// One iteration of Extrapolation::solve over cells (Extrapolation.h:101-124,
-// 143-166, 191-214): nv value arrays averaged over reached neighbours in
-// the reference's ordinal order (-x, +x, -y, +y, -z, +z).
-template <int D>
-__global__ __launch_bounds__(kBlock) void kLayerStepCells(GridDev g, uint8_t *layer, int L, int nv, double *v0, double *v1, double *v2)
-{
-	const Coord c = threadCoord();
-	if (!inGrid<D>(g, c)) return;
-	const size_t id = gidx<D>(g, c.i, c.j, c.k);
-	if (layer[id] != kUnreached) return;
-	int cnt = 0;
-	double s0 = 0, s1 = 0, s2 = 0;
-#pragma unroll
-	for (int o = 0; o < 2 * D; o++) {
-		const int ax = o >> 1, sd = (o & 1) ? 1 : -1;
-		const int qi = c.i + (ax == 0 ? sd : 0), qj = c.j + (ax == 1 ? sd : 0), qk = c.k + (ax == 2 ? sd : 0);
-		if (qi < 0 || qi >= g.n[0] || qj < 0 || qj >= g.n[1] || (D == 3 && (qk < 0 || qk >= g.n[2]))) continue;
-		const size_t q = gidx<D>(g, qi, qj, qk);
-		if (layer[q] < L) {
-			if (nv > 0) s0 += v0[q];
-			if (nv > 1) s1 += v1[q];
-			if (nv > 2) s2 += v2[q];
-			cnt++;
-		}
-	}
-	if (cnt > 0) {
-		if (nv > 0) v0[id] = s0 / cnt;
-		if (nv > 1) v1[id] = s1 / cnt;
-		if (nv > 2) v2[id] = s2 / cnt;
-		layer[id] = uint8_t(L);
-	}
-}
-
-// Reference map not reached: reset to the cell centre (Extrapolation.h:210-212).
-template <int D>
-__global__ __launch_bounds__(kBlock) void kLayerFinishRefMap(GridDev g, const uint8_t *layer, double *x0, double *x1, double *x2)
-{
-	const Coord c = threadCoord();
-	if (!inGrid<D>(g, c)) return;
-	const size_t id = gidx<D>(g, c.i, c.j, c.k);
-	if (layer[id] != kUnreached) return;
-	x0[id] = g.o[0] + c.i * g.dx;
-	x1[id] = g.o[1] + c.j * g.dx;
-	if constexpr (D == 3) x2[id] = g.o[2] + c.k * g.dx;
-}
-
-// _advRegion = reached within the radius (Extrapolation.h:130-175 applied to
-// the indicator of phi_ori <= 0, LevelSetLiquid.cpp:1064-1070).
-__global__ void kLayerToMask(const uint8_t *layer, uint8_t *adv, size_t n)
-{
-	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
-		adv[i] = layer[i] != kUnreached ? 1 : 0;
-}
-
-// Velocity faces: fixed when fraction < 1 and an adjacent cell is liquid in
-// phi_ori (LevelSetLiquid.cpp:1072-1083).
-template <int D>
-__global__ __launch_bounds__(kBlock) void kLayerInitFaces(GridDev g, GridDev cg, int axis, uint8_t *layer, const double *frac, const double *phiOri)
-{
-	const Coord c = threadCoord();
-	if (!inGrid<D>(g, c)) return;
-	const size_t id = gidx<D>(g, c.i, c.j, c.k);
-	bool fixed = false;
-	const int i0 = c.i - (axis == 0), j0 = c.j - (axis == 1), k0 = c.k - (axis == 2);
-	// Both adjacent cells exist for every face whose fraction is below 1 (the
-	// outermost faces are boundary faces, fraction 1); the guard keeps an
-	// uploaded fraction field from reading outside the cell grid.
-	const bool inside = i0 >= 0 && j0 >= 0 && k0 >= 0 && c.i < cg.n[0] && c.j < cg.n[1] && (D == 2 || c.k < cg.n[2]);
-	if (frac[id] < 1 && inside)
-		fixed = phiOri[gidx<D>(cg, i0, j0, k0)] <= 0 || phiOri[gidx<D>(cg, c.i, c.j, c.k)] <= 0;
-	layer[id] = fixed ? 0 : kUnreached;
-}
-
-// Faces never reached are zeroed ("for velocity only", Extrapolation.h:76-78).
-__global__ void kLayerFinishFaces(const uint8_t *layer, double *u, size_t n)
-{
-	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
-		if (layer[i] == kUnreached) u[i] = 0;
 }
 
 // ---------------------------------------------------------- enforce (u)
@@ -398,41 +292,6 @@ void launchEnforceLevelSet(const Fields &F, double *sdf, double tol, hipStream_t
 void launchGravity(const Fields &F, double dt, hipStream_t s)
 {
 	hipLaunchKernelGGL(kGravity, dim3(linearBlocks(F.Nf[1])), dim3(kBlock), 0, s, F.u[1], 9.8 * dt, F.Nf[1]);
-}
-
-static void cellLayers(const Fields &F, int mask, int steps, int nv, double *v0, double *v1, double *v2, hipStream_t s)
-{
-	launchCells(F.D, F.cell, s, kLayerInitCells<2>, kLayerInitCells<3>, F.cell, F.layer, mask, (const double *)F.cell_dist,
-		(const uint8_t *)F.adv_region, (const double *)F.phi_ori);
-	for (int L = 1; L <= steps; L++)
-		launchCells(F.D, F.cell, s, kLayerStepCells<2>, kLayerStepCells<3>, F.cell, F.layer, L, nv, v0, v1, v2);
-}
-
-// extrapolateFieldsToCollider (LevelSetLiquid.cpp:1086-1137): phi_ori and phi,
-// 6 layers (_boundaryExtrapolationRadius, LevelSetLiquid.h:92), fixed where
-// cellDist > 0.  Both arrays share the mask, so one sweep moves both.
-void launchExtrapolateCollider(const Fields &F, hipStream_t s)
-{
-	cellLayers(F, kMaskCellDistPositive, 6, 2, F.phi_ori, F.phi, nullptr, s);
-}
-
-// extrapolateFieldsToAir (LevelSetLiquid.cpp:1059-1084): xi (15 layers from
-// the old advRegion), advRegion rebuilt from phi_ori <= 0 (15 layers), u
-// (17 layers per axis).
-void launchExtrapolateAir(const Fields &F, hipStream_t s)
-{
-	cellLayers(F, kMaskAdvRegion, 15, F.D, F.xi[0], F.xi[1], F.D == 3 ? F.xi[2] : nullptr, s);
-	launchCells(F.D, F.cell, s, kLayerFinishRefMap<2>, kLayerFinishRefMap<3>, F.cell, (const uint8_t *)F.layer, F.xi[0], F.xi[1], F.xi[2]);
-	cellLayers(F, kMaskPhiOriNonPositive, 15, 0, nullptr, nullptr, nullptr, s);
-	hipLaunchKernelGGL(kLayerToMask, dim3(linearBlocks(F.N)), dim3(kBlock), 0, s, (const uint8_t *)F.layer, F.adv_region, F.N);
-	for (int a = 0; a < F.D; a++) {
-		launchCells(F.D, F.face[a], s, kLayerInitFaces<2>, kLayerInitFaces<3>, F.face[a], F.cell, a, F.layer_f[a],
-			(const double *)F.frac[a], (const double *)F.phi_ori);
-		for (int L = 1; L <= 17; L++)
-			launchCells(F.D, F.face[a], s, kLayerStepCells<2>, kLayerStepCells<3>, F.face[a], F.layer_f[a], L, 1, F.u[a],
-				(double *)nullptr, (double *)nullptr);
-		hipLaunchKernelGGL(kLayerFinishFaces, dim3(linearBlocks(F.Nf[a])), dim3(kBlock), 0, s, (const uint8_t *)F.layer_f[a], F.u[a], F.Nf[a]);
-	}
 }
 
 void launchEnforceBoundary(const Fields &F, hipStream_t s)

@@ -36,9 +36,10 @@ struct Fields {
 	double *cell_dist = nullptr, *node_dist = nullptr;// _boundary.cellDist / nodeDist
 	double *frac[3] = {}, *bvel[3] = {}, *bnormal[3] = {};   // _boundary.fraction / velocity / normal
 	double *scratch[2] = {};                          // two cell arrays (reinit stages)
-	double *trace_x[3] = {}, *trace_r[9] = {};        // backtrace position and Jacobian per cell
 	uint8_t *layer = nullptr;                         // extrapolation layer index per cell
 	uint8_t *layer_f[3] = {};                         // ... per face
+	int *front_list[2] = {};                          // extrapolation fronts (element ids)
+	int *front_cnt = nullptr;                         // three front counters
 	double *partials = nullptr;                       // reduction partials
 	size_t partials_cap = 0;
 	double *dscalars = nullptr;                       // small device scalars

@@ -82,25 +82,13 @@ __device__ __forceinline__ T applyRow(const Rows<T> &A, const T *p, int r, T pr)
 	return s;
 }
 
-// y = A p, four rows per thread per pass so the index -> gather chains of
-// several rows are in flight together.
+// y = A p
 template <int D, typename T>
 __global__ __launch_bounds__(kBlock) void kSpmv(Rows<T> A, int n, const T *p, T *y, double *pa, const int *st)
 {
 	if (st[kStDone]) return;
 	double v[1] = {0};
-	const int stride = gridDim.x * blockDim.x;
-	int r = blockIdx.x * blockDim.x + threadIdx.x;
-	for (; r + 3 * stride < n; r += 4 * stride) {
-		T pr[4], s[4];
-#pragma unroll
-		for (int u = 0; u < 4; u++) pr[u] = p[r + u * stride];
-#pragma unroll
-		for (int u = 0; u < 4; u++) s[u] = applyRow<D, T>(A, p, r + u * stride, pr[u]);
-#pragma unroll
-		for (int u = 0; u < 4; u++) { y[r + u * stride] = s[u]; v[0] += double(pr[u]) * double(s[u]); }
-	}
-	for (; r < n; r += stride) {
+	for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
 		const T pr = p[r];
 		const T s = applyRow<D, T>(A, p, r, pr);
 		y[r] = s;
