@@ -74,6 +74,7 @@ size_t fieldCount(const Fields &F, int f)
 	const size_t NF = F.Nf[0] + F.Nf[1] + (F.D == 3 ? F.Nf[2] : 0);
 	switch (f) {
 	case GARM_PHI: case GARM_PHI_TMP: case GARM_PHI_ORI: case GARM_PHI_REF: case GARM_ADV_REGION: case GARM_CELL_DIST: case GARM_PRESSURE:
+	case GARM_RHS: case GARM_UNKNOWNS:
 		return N;
 	case GARM_PHI_GRAD: case GARM_REFMAP: case GARM_PHI_REF_GRAD: return N * D;
 	case GARM_REFMAP_GRAD: return N * D * D;
@@ -135,7 +136,7 @@ int doProject(garm_ctx *c, const garm_pcg_opts &o, garm_pcg_stats *st)
 {
 	garm_pcg_stats local{};
 	if (!st) st = &local;
-	if (o.mode != GARM_PCG_FP64 && o.mode != GARM_PCG_MIXED) return fail(c, GARM_EINVAL, "unknown PCG mode");
+	if (o.mode != GARM_PCG_FP64 && o.mode != GARM_PCG_MIXED && o.mode != GARM_PCG_LEARNED) return fail(c, GARM_EINVAL, "unknown PCG mode");
 	if (!(o.tolerance > 0) || o.max_iterations < 0) return fail(c, GARM_EINVAL, "bad PCG tolerance or iteration limit");
 	std::string err;
 	const int rc = runProjection(c->F, c->P, o, st, c->s, err);
@@ -316,7 +317,7 @@ int garm_upload(garm_ctx *c, int f, const void *host, size_t n)
 	if (!c || !host) return GARM_EINVAL;
 	Fields &F = c->F;
 	if (n != fieldCount(F, f) || n == 0) return fail(c, GARM_EINVAL, "field id or element count does not match");
-	if (f == GARM_PRESSURE) return fail(c, GARM_EINVAL, "pressure is an output");
+	if (f == GARM_PRESSURE || f == GARM_RHS || f == GARM_UNKNOWNS) return fail(c, GARM_EINVAL, "the pressure system fields are outputs");
 	int rc;
 	if ((rc = hipCheck(c, hipStreamSynchronize(c->s), "synchronize"))) return rc;
 	if (f == GARM_ADV_REGION) {
@@ -357,8 +358,8 @@ int garm_download(garm_ctx *c, int f, void *host, size_t n)
 	int rc;
 	if ((rc = hipCheck(c, hipStreamSynchronize(c->s), "synchronize"))) return rc;
 	if (f == GARM_ADV_REGION) return hipCheck(c, hipMemcpy(host, F.adv_region, n, hipMemcpyDeviceToHost), "download");
-	if (f == GARM_PRESSURE) {
-		scatterPressure(F, c->P, F.scratch[0], c->s);
+	if (f == GARM_PRESSURE || f == GARM_RHS || f == GARM_UNKNOWNS) {
+		scatterCompact(F, c->P, f == GARM_PRESSURE ? kCompactPressure : f == GARM_RHS ? kCompactRhs : kCompactIndicator, F.scratch[0], c->s);
 		if ((rc = launched(c, "pressure scatter")) || (rc = hipCheck(c, hipStreamSynchronize(c->s), "synchronize"))) return rc;
 		return hipCheck(c, hipMemcpy(host, F.scratch[0], n * sizeof(double), hipMemcpyDeviceToHost), "download");
 	}
@@ -491,6 +492,17 @@ int garm_max_speed(garm_ctx *c, double *out)
 	if (rc || (rc = readScalars(c, 1))) return rc;
 	*out = c->host_scalars[0];
 	return GARM_OK;
+}
+
+int garm_attach_preconditioner(garm_ctx *c, garm_unet *net)
+{
+	if (!c) return GARM_EINVAL;
+	if (!net) { c->P.net = nullptr; return GARM_OK; }
+	int dev = -1;
+	Unet *U = unetOf(net, &dev);
+	if (dev != c->device) return fail(c, GARM_EINVAL, "the network lives on another device");
+	c->P.net = U;
+	return hipCheck(c, hipStreamSynchronize(U->s), "network synchronize");
 }
 
 int garm_synchronize(garm_ctx *c)

@@ -1,0 +1,571 @@
+// Learned pressure solve and the network's C-ABI (no reference counterpart;
+// the reference solves with Eigen's IC-PCG, LinearSystem.h:10-39).
+//
+// Flexible conjugate gradient (Notay's FCG with one direction kept):
+//   alpha = r.p / p.Ap,  x += alpha p,  r -= alpha Ap,
+//   z = M(r),  beta = -z.Ap / p.Ap,  p = z + beta p,
+// which stays a descent method for a nonlinear preconditioner M.  M is a
+// two-level cycle with the U-Net as the coarse correction:
+//   z0 = w D^-1 r                      (damped Jacobi pre-smoothing)
+//   z1 = z0 + N(r - A z0)              (network on the smoothed residual)
+//   z  = z1 + w D^-1 (r - A z1)        (post-smoothing)
+// with w = 2/3 and N the U-Net applied on the bounding box of the unknowns
+// (two cells of context, sizes padded to multiples of four) to its input
+// scaled to unit RMS, the output scaled back.  The smoother removes the rough
+// error components a network trained on pressures does not resolve.  When the
+// residual stalls (less than a halving over eight iterations) or the
+// iteration budget runs out, the solve falls back to the mixed-precision
+// Jacobi PCG on the current residual; the FP64 residual b - A x judges the
+// result either way.
+#include "garm_learned.hpp"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+namespace garm {
+
+constexpr int kLGrid = 1024;
+
+static int lblocks(int n) { return std::max(1, std::min(kLGrid, (n + kBlock - 1) / kBlock)); }
+
+template <int NV>
+__device__ __forceinline__ void sumPart(const double *partials, int nb, double (&tot)[NV])
+{
+	double s[NV];
+#pragma unroll
+	for (int q = 0; q < NV; q++) s[q] = 0;
+	for (int i = threadIdx.x; i < nb; i += blockDim.x)
+#pragma unroll
+		for (int q = 0; q < NV; q++) s[q] += partials[NV * i + q];
+	blockSum<NV>(s, tot);
+}
+
+template <int NV>
+__device__ __forceinline__ void bcast(double (&v)[NV])
+{
+	__shared__ double b[NV];
+	__syncthreads();
+	if (threadIdx.x == 0)
+#pragma unroll
+		for (int q = 0; q < NV; q++) b[q] = v[q];
+	__syncthreads();
+#pragma unroll
+	for (int q = 0; q < NV; q++) v[q] = b[q];
+}
+
+template <int NV>
+__device__ __forceinline__ void putPart(double (&v)[NV], double *partials)
+{
+	double blk[NV];
+	blockSum<NV>(v, blk);
+	if (threadIdx.x == 0)
+#pragma unroll
+		for (int q = 0; q < NV; q++) partials[NV * blockIdx.x + q] = blk[q];
+}
+
+enum { kLsPq = 0, kLsRr = 1, kLsBb = 2, kLsTrue = 3, kLsSmooth = 4 };
+
+constexpr double kOmega = 2.0 / 3.0;
+
+// z = w D^-1 r
+__global__ void kJacobi(const double *invdiag, int n, const double *r, double *z)
+{
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) z[i] = kOmega * invdiag[i] * r[i];
+}
+
+// out = z + w D^-1 (r - A z)
+template <int D>
+__global__ __launch_bounds__(kBlock) void kPostSmooth(Rows<double> A, int n, const double *r, const double *z, double *out)
+{
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+		const double zi = z[i];
+		double s = A.diag[i] * zi;
+#pragma unroll
+		for (int o = 0; o < 2 * D; o++) {
+			const int c = A.nb[o][i];
+			if (c >= 0) s += -A.w[o][i] * z[c];
+		}
+		out[i] = zi + kOmega * A.invdiag[i] * (r[i] - s);
+	}
+}
+
+// q = A p; partials of p.q and r.p
+template <int D>
+__global__ __launch_bounds__(kBlock) void kFcgSpmv(Rows<double> A, int n, const double *p, double *q, const double *r, double *pa)
+{
+	double v[2] = {0, 0};
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+		const double pi = p[i];
+		double s = A.diag[i] * pi;
+#pragma unroll
+		for (int o = 0; o < 2 * D; o++) {
+			const int c = A.nb[o][i];
+			if (c >= 0) s += -A.w[o][i] * p[c];
+		}
+		q[i] = s;
+		v[0] += pi * s;
+		v[1] += r[i] * pi;
+	}
+	putPart<2>(v, pa);
+}
+
+// alpha = (r.p) / (p.q); x += alpha p; r -= alpha q; partial |r|^2
+__global__ __launch_bounds__(kBlock) void kFcgUpdate(int n, double *x, double *r, const double *p, const double *q, const double *pa, double *pb, double *ls)
+{
+	double t[2];
+	sumPart<2>(pa, gridDim.x, t);
+	double a[1] = {0};
+	if (threadIdx.x == 0) a[0] = t[0] != 0 ? t[1] / t[0] : 0.0;
+	bcast<1>(a);
+	if (blockIdx.x == 0 && threadIdx.x == 0) ls[kLsPq] = t[0];
+	double v[1] = {0};
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+		x[i] += a[0] * p[i];
+		const double ri = r[i] - a[0] * q[i];
+		r[i] = ri;
+		v[0] += ri * ri;
+	}
+	putPart<1>(v, pb);
+}
+
+__global__ __launch_bounds__(kBlock) void kFcgReduce1(const double *pb, int nb, double *dst)
+{
+	double t[1];
+	sumPart<1>(pb, nb, t);
+	if (threadIdx.x == 0) *dst = t[0];
+}
+
+// partial z.q
+__global__ __launch_bounds__(kBlock) void kFcgDot(int n, const double *z, const double *q, double *pa)
+{
+	double v[1] = {0};
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v[0] += z[i] * q[i];
+	putPart<1>(v, pa);
+}
+
+// beta = -(z.q) / (p.q); p = z + beta p
+__global__ __launch_bounds__(kBlock) void kFcgP(int n, double *p, const double *z, const double *pa, const double *ls)
+{
+	double t[1];
+	sumPart<1>(pa, gridDim.x, t);
+	double b[1] = {0};
+	if (threadIdx.x == 0) b[0] = ls[kLsPq] != 0 ? -t[0] / ls[kLsPq] : 0.0;
+	bcast<1>(b);
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = z[i] + b[0] * p[i];
+}
+
+__global__ void kCopyD(double *dst, const double *src, int n)
+{
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
+__global__ void kZeroD(double *dst, int n)
+{
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = 0;
+}
+
+__global__ void kAddD(double *dst, const double *src, int n)
+{
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] += src[i];
+}
+
+// rt = b - A x (FP64), partial |rt|^2
+template <int D>
+__global__ __launch_bounds__(kBlock) void kTrueResidual(Rows<double> A, int n, const double *b, const double *x, double *rt, double *pb)
+{
+	double v[1] = {0};
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+		double s = A.diag[i] * x[i];
+#pragma unroll
+		for (int o = 0; o < 2 * D; o++) {
+			const int c = A.nb[o][i];
+			if (c >= 0) s += -A.w[o][i] * x[c];
+		}
+		const double ri = b[i] - s;
+		rt[i] = ri;
+		v[0] += ri * ri;
+	}
+	putPart<1>(v, pb);
+}
+
+// bounding box of the unknowns' cells
+__global__ void kBBox(const int *mat2grid, int n, int nx, int ny, int *box)
+{
+	int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+		const int c = mat2grid[i];
+		const int x = c % nx, y = (c / nx) % ny, z = c / nx / ny;
+		mn[0] = min(mn[0], x); mn[1] = min(mn[1], y); mn[2] = min(mn[2], z);
+		mx[0] = max(mx[0], x); mx[1] = max(mx[1], y); mx[2] = max(mx[2], z);
+	}
+	for (int a = 0; a < 3; a++) { atomicMin(&box[a], mn[a]); atomicMax(&box[3 + a], mx[a]); }
+}
+
+__global__ void kBBoxInit(int *box)
+{
+	if (threadIdx.x < 3) box[threadIdx.x] = INT_MAX;
+	else if (threadIdx.x < 6) box[threadIdx.x] = INT_MIN;
+}
+
+// ------------------------------------------------------------------ work
+LearnedWork::~LearnedWork()
+{
+	auto fr = [](auto *&q) { if (q) { (void)hipFree(q); q = nullptr; } };
+	fr(x); fr(r); fr(p); fr(q); fr(z); fr(t); fr(b2); fr(pa); fr(pb); fr(ls); fr(box);
+	if (host) { (void)hipHostFree(host); host = nullptr; }
+}
+
+bool LearnedWork::reserve(size_t n, std::string &err)
+{
+	bool ok = true;
+	if (!ls) {
+		ok = hipMalloc(&pa, 2 * kLGrid * sizeof(double)) == hipSuccess && hipMalloc(&pb, kLGrid * sizeof(double)) == hipSuccess
+			&& hipMalloc(&ls, 8 * sizeof(double)) == hipSuccess && hipMalloc(&box, 8 * sizeof(int)) == hipSuccess
+			&& hipHostMalloc(&host, 16 * sizeof(double)) == hipSuccess;
+		if (!ok) { err = "learned-solver scalar allocation failed"; return false; }
+	}
+	if (n > cap) {
+		auto fr = [](auto *&q) { if (q) { (void)hipFree(q); q = nullptr; } };
+		fr(x); fr(r); fr(p); fr(q); fr(z); fr(t); fr(b2);
+		const size_t want = std::max<size_t>(n + n / 4, 1024);
+		for (double **v : {&x, &r, &p, &q, &z, &t, &b2}) ok = ok && hipMalloc(v, want * sizeof(double)) == hipSuccess;
+		if (!ok) { err = "learned-solver vector allocation failed"; cap = 0; return false; }
+		cap = want;
+	}
+	return true;
+}
+
+static int readHost(LearnedWork &W, const double *src, int count, hipStream_t s, std::string &err)
+{
+	if (hipMemcpyAsync(W.host, src, count * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+		err = "learned-solver readback failed";
+		return GARM_EHIP;
+	}
+	return GARM_OK;
+}
+
+template <int D>
+static int learned(const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *grid2mat, const int *mat2grid,
+	const GridDev &cell, const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Unet &U, hipStream_t s, garm_pcg_stats *st, std::string &err)
+{
+	if (!W.reserve(size_t(n), err)) return GARM_ENOMEM;
+	const int nb = lblocks(n);
+	const dim3 g1(nb), b1(kBlock);
+	int rc;
+	// network box: bounding box of the unknowns, two cells of context, multiples of four
+	hipLaunchKernelGGL(kBBoxInit, dim3(1), dim3(64), 0, s, W.box);
+	hipLaunchKernelGGL(kBBox, g1, b1, 0, s, mat2grid, n, cell.n[0], cell.n[1], W.box);
+	int hb[6];
+	if (hipMemcpyAsync(hb, W.box, sizeof(hb), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+		err = "bounding-box readback failed";
+		return GARM_EHIP;
+	}
+	UnetBox B{};
+	for (int a = 0; a < 3; a++) {
+		const bool live = a < D;
+		const int lo = live ? hb[a] - 2 : 0, hi = live ? hb[3 + a] + 2 : 0;
+		B.org[a] = lo;
+		B.n[a] = live ? (hi - lo + 1 + 3) / 4 * 4 : 4;
+		B.gn[a] = cell.n[a];
+	}
+	const size_t boxVox = size_t(B.n[0]) * B.n[1] * B.n[2];
+	if (!U.buf.reserve(std::max(boxVox + boxVox / 4, U.buf.cap_vox), U.buf.has_grad, err) || !U.buf.setDims(B.n[0], B.n[1], B.n[2], err))
+		return GARM_ENOMEM;
+
+	// x = 0, r = b, |b|^2
+	hipLaunchKernelGGL(kZeroD, g1, b1, 0, s, W.x, n);
+	hipLaunchKernelGGL(kCopyD, g1, b1, 0, s, W.r, b, n);
+	hipLaunchKernelGGL(kFcgDot, g1, b1, 0, s, n, b, b, W.pb);
+	hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nb, W.ls + kLsBb);
+	if ((rc = readHost(W, W.ls, 4, s, err))) return rc;
+	const double bb = W.host[kLsBb];
+	st->iterations = 0;
+	st->outer_iterations = 0;
+	st->fell_back = 0;
+	if (bb == 0) { st->relative_residual = 0; return GARM_OK; }
+	const double tol = std::max(o.tolerance, 2.220446049250313e-16 / std::sqrt(bb));
+	const double thr = tol * tol * bb;
+	// z = M(r): two-level cycle, network on the pre-smoothed residual
+	auto precondition = [&]() {
+		hipLaunchKernelGGL(kJacobi, g1, b1, 0, s, (const double *)A64.invdiag, n, (const double *)W.r, W.z);
+		hipLaunchKernelGGL(kTrueResidual<D>, g1, b1, 0, s, A64, n, (const double *)W.r, (const double *)W.z, W.t, W.pb);
+		hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nb, W.ls + kLsSmooth);
+		unetApplyCompact(U, B, grid2mat, W.t, W.ls + kLsSmooth, n, W.z, s);
+		hipLaunchKernelGGL(kPostSmooth<D>, g1, b1, 0, s, A64, n, (const double *)W.r, (const double *)W.z, W.t);
+		std::swap(W.z, W.t);
+	};
+	precondition();
+	hipLaunchKernelGGL(kCopyD, g1, b1, 0, s, W.p, (const double *)W.z, n);
+	std::vector<double> hist;
+	hist.push_back(std::sqrt(bb));
+	const int budget = std::min(o.max_iterations, 400);
+	bool converged = false, stalled = false;
+	int it = 0;
+	for (; it < budget; it++) {
+		hipLaunchKernelGGL(kFcgSpmv<D>, g1, b1, 0, s, A64, n, (const double *)W.p, W.q, (const double *)W.r, W.pa);
+		hipLaunchKernelGGL(kFcgUpdate, g1, b1, 0, s, n, W.x, W.r, (const double *)W.p, (const double *)W.q, (const double *)W.pa, W.pb, W.ls);
+		hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nb, W.ls + kLsRr);
+		if ((rc = readHost(W, W.ls, 4, s, err))) return rc;
+		const double rr = W.host[kLsRr];
+		hist.push_back(std::sqrt(rr));
+		if (!std::isfinite(rr)) { stalled = true; break; }
+		if (rr < thr) { converged = true; it++; break; }
+		if (hist.size() > 9 && hist.back() > 0.5 * hist[hist.size() - 9]) { stalled = true; it++; break; }
+		precondition();
+		hipLaunchKernelGGL(kFcgDot, g1, b1, 0, s, n, (const double *)W.z, (const double *)W.q, W.pa);
+		hipLaunchKernelGGL(kFcgP, g1, b1, 0, s, n, W.p, (const double *)W.z, (const double *)W.pa, (const double *)W.ls);
+	}
+	st->iterations = it;
+	(void)converged;
+	(void)stalled;
+	// the FP64 residual judges the result; fall back to plain PCG on what is left
+	if (!std::isfinite(W.host[kLsRr])) hipLaunchKernelGGL(kZeroD, g1, b1, 0, s, W.x, n);
+	hipLaunchKernelGGL(kTrueResidual<D>, g1, b1, 0, s, A64, n, b, (const double *)W.x, W.b2, W.pb);
+	hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nb, W.ls + kLsTrue);
+	if ((rc = readHost(W, W.ls, 4, s, err))) return rc;
+	double rt = W.host[kLsTrue];
+	if (!(rt < thr)) {
+		st->fell_back = 1;
+		garm_pcg_opts fo = o;
+		fo.tolerance = std::sqrt(thr / rt) * 0.999;
+		fo.max_iterations = std::max(1, o.max_iterations - it);
+		garm_pcg_stats fs{};
+		rc = solveMixed(D, A64, A32, W.b2, n, fo, K, s, &fs, err);
+		if (rc != GARM_OK && rc != GARM_ENOCONV) return rc;
+		hipLaunchKernelGGL(kAddD, g1, b1, 0, s, W.x, (const double *)K.x, n);
+		st->iterations += fs.iterations;
+		st->outer_iterations = fs.outer_iterations;
+		hipLaunchKernelGGL(kTrueResidual<D>, g1, b1, 0, s, A64, n, b, (const double *)W.x, W.b2, W.pb);
+		hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nb, W.ls + kLsTrue);
+		if ((rc = readHost(W, W.ls, 4, s, err))) return rc;
+		rt = W.host[kLsTrue];
+	}
+	st->relative_residual = std::sqrt(rt / bb);
+	if (!(rt < thr) && st->relative_residual > tol) { err = "learned solve and its fallback did not reach the tolerance"; return GARM_ENOCONV; }
+	return GARM_OK;
+}
+
+int solveLearned(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *grid2mat, const int *mat2grid,
+	const GridDev &cell, const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Unet &U, hipStream_t s, garm_pcg_stats *st, std::string &err)
+{
+	return D == 2 ? learned<2>(A64, A32, b, n, grid2mat, mat2grid, cell, o, K, W, U, s, st, err)
+	              : learned<3>(A64, A32, b, n, grid2mat, mat2grid, cell, o, K, W, U, s, st, err);
+}
+
+// ------------------------------------------------------------- training
+__global__ __launch_bounds__(kBlock) void kLossGrad(const float *y, const float *target, const uint8_t *mask, float scale, float invCount,
+	float *gy, size_t vox, float *lossPart)
+{
+	float acc = 0;
+	for (size_t v = blockIdx.x * size_t(blockDim.x) + threadIdx.x; v < vox; v += size_t(gridDim.x) * blockDim.x) {
+		const float d = mask[v] ? y[v] - target[v] * scale : 0.0f;
+		gy[v] = 2.0f * d * invCount;
+		acc += d * d;
+	}
+	double a[1] = {acc};
+	double o[1];
+	blockSum<1>(a, o);
+	if (threadIdx.x == 0) lossPart[blockIdx.x] = float(o[0]);
+}
+
+__global__ void kLossReduce(const float *part, int nb, float invCount, float *dst)
+{
+	if (threadIdx.x != 0) return;
+	float s = 0;
+	for (int i = 0; i < nb; i++) s += part[i];
+	*dst = s * invCount;
+}
+
+__global__ void kLoadScaled(const float *r, const uint8_t *mask, float scale, bf16 *in, size_t vox)
+{
+	for (size_t v = blockIdx.x * size_t(blockDim.x) + threadIdx.x; v < vox; v += size_t(gridDim.x) * blockDim.x) {
+		const bool m = mask[v] != 0;
+		in[v * 4 + 0] = f2bf(m ? r[v] * scale : 0.0f);
+		in[v * 4 + 1] = f2bf(m ? 1.0f : 0.0f);
+		in[v * 4 + 2] = 0;
+		in[v * 4 + 3] = 0;
+	}
+}
+
+} // namespace garm
+
+// ==================================================================== C-ABI
+using namespace garm;
+
+struct garm_unet {
+	Unet U;
+	hipStream_t s = nullptr;
+	int device = 0;
+	std::string err;
+};
+
+Unet *garm::unetOf(garm_unet *net, int *device)
+{
+	*device = net->device;
+	return &net->U;
+}
+
+namespace {
+thread_local std::string g_unet_create_error;
+
+int ufail(garm_unet *u, int code, const std::string &m)
+{
+	u->err = m;
+	return code;
+}
+
+template <typename T>
+struct DevBuf {
+	T *p = nullptr;
+	explicit DevBuf(size_t n) { if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) p = nullptr; }
+	~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+bool dims3(const int *d, size_t &vox)
+{
+	if (!d || d[0] < 4 || d[1] < 4 || d[2] < 4 || d[0] % 4 || d[1] % 4 || d[2] % 4 || d[0] > 1024 || d[1] > 1024 || d[2] > 1024) return false;
+	vox = size_t(d[0]) * d[1] * d[2];
+	return true;
+}
+} // namespace
+
+extern "C" {
+
+const char *garm_unet_create_error(void) { return g_unet_create_error.c_str(); }
+const char *garm_unet_last_error(const garm_unet *u) { return u ? u->err.c_str() : "null network"; }
+size_t garm_unet_param_count(void) { return unetParamCount(); }
+
+int garm_unet_create(garm_unet **out, unsigned long long seed)
+{
+	if (!out) return GARM_EINVAL;
+	*out = nullptr;
+	int ndev = 0;
+	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { g_unet_create_error = "no HIP device"; return GARM_ENODEV; }
+	int dev = 0;
+	hipDeviceProp_t prop{};
+	if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) { g_unet_create_error = "device query failed"; return GARM_ENODEV; }
+	if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) { g_unet_create_error = std::string("device is not gfx950: ") + prop.gcnArchName; return GARM_ENODEV; }
+	auto *u = new garm_unet;
+	u->device = dev;
+	std::string err;
+	if (hipStreamCreateWithFlags(&u->s, hipStreamNonBlocking) != hipSuccess || !unetCreate(u->U, seed, u->s, err)) {
+		g_unet_create_error = err.empty() ? "stream creation failed" : err;
+		garm_unet_destroy(u);
+		return GARM_ENOMEM;
+	}
+	*out = u;
+	return GARM_OK;
+}
+
+void garm_unet_destroy(garm_unet *u)
+{
+	if (!u) return;
+	if (u->s) (void)hipStreamSynchronize(u->s);
+	hipStream_t s = u->s;
+	delete u;
+	if (s) (void)hipStreamDestroy(s);
+}
+
+int garm_unet_get_params(garm_unet *u, float *out, size_t n)
+{
+	if (!u || !out || n != unetParamCount()) return GARM_EINVAL;
+	return unetGetParams(u->U, out, false, u->err) ? GARM_OK : GARM_EHIP;
+}
+
+int garm_unet_set_params(garm_unet *u, const float *in, size_t n)
+{
+	if (!u || !in || n != unetParamCount()) return GARM_EINVAL;
+	return unetSetParams(u->U, in, u->err) ? GARM_OK : GARM_EHIP;
+}
+
+int garm_unet_forward(garm_unet *u, const int dims[3], const float *r, const uint8_t *mask, float *y)
+{
+	size_t vox;
+	if (!u || !r || !mask || !y || !dims3(dims, vox)) return u ? ufail(u, GARM_EINVAL, "bad network grid or buffers") : GARM_EINVAL;
+	if (!u->U.buf.reserve(vox, u->U.buf.has_grad, u->err) || !u->U.buf.setDims(dims[0], dims[1], dims[2], u->err)) return GARM_ENOMEM;
+	DevBuf<float> dr(vox);
+	DevBuf<uint8_t> dm(vox);
+	if (!dr.p || !dm.p) return ufail(u, GARM_ENOMEM, "device allocation failed");
+	if (hipMemcpy(dr.p, r, vox * sizeof(float), hipMemcpyHostToDevice) != hipSuccess || hipMemcpy(dm.p, mask, vox, hipMemcpyHostToDevice) != hipSuccess)
+		return ufail(u, GARM_EHIP, "upload failed");
+	unetLoadInput(u->U, dr.p, dm.p, u->s);
+	unetForward(u->U, u->s);
+	if (hipGetLastError() != hipSuccess || hipStreamSynchronize(u->s) != hipSuccess) return ufail(u, GARM_EHIP, "network forward failed");
+	return hipMemcpy(y, u->U.buf.y, vox * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess ? GARM_OK : ufail(u, GARM_EHIP, "download failed");
+}
+
+int garm_unet_loss_grad(garm_unet *u, const int dims[3], const float *r, const uint8_t *mask, const float *target, double *loss, float *grad, size_t n)
+{
+	size_t vox;
+	if (!u || !r || !mask || !target || !loss || !grad || n != unetParamCount() || !dims3(dims, vox)) return GARM_EINVAL;
+	if (!u->U.buf.reserve(vox, true, u->err) || !u->U.buf.setDims(dims[0], dims[1], dims[2], u->err)) return GARM_ENOMEM;
+	DevBuf<float> dr(vox), dt(vox), part(1024), dl(1);
+	DevBuf<uint8_t> dm(vox);
+	if (!dr.p || !dt.p || !dm.p || !part.p || !dl.p) return ufail(u, GARM_ENOMEM, "device allocation failed");
+	size_t m = 0;
+	for (size_t v = 0; v < vox; v++) m += mask[v] != 0;
+	if (m == 0) return ufail(u, GARM_EINVAL, "empty mask");
+	if (hipMemcpy(dr.p, r, vox * sizeof(float), hipMemcpyHostToDevice) != hipSuccess || hipMemcpy(dt.p, target, vox * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+		hipMemcpy(dm.p, mask, vox, hipMemcpyHostToDevice) != hipSuccess) return ufail(u, GARM_EHIP, "upload failed");
+	hipLaunchKernelGGL(kLoadScaled, dim3(1024), dim3(kBlock), 0, u->s, (const float *)dr.p, (const uint8_t *)dm.p, 1.0f, u->U.buf.in, vox);
+	unetForward(u->U, u->s);
+	const int nb = 1024;
+	hipLaunchKernelGGL(kLossGrad, dim3(nb), dim3(kBlock), 0, u->s, (const float *)u->U.buf.y, (const float *)dt.p, (const uint8_t *)dm.p, 1.0f,
+		float(1.0 / double(m)), u->U.buf.gy, vox, part.p);
+	hipLaunchKernelGGL(kLossReduce, dim3(1), dim3(64), 0, u->s, (const float *)part.p, nb, float(1.0 / double(m)), dl.p);
+	if (!unetBackward(u->U, u->s, u->err)) return GARM_ENOMEM;
+	float hl = 0;
+	if (hipGetLastError() != hipSuccess || hipStreamSynchronize(u->s) != hipSuccess || hipMemcpy(&hl, dl.p, sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+		return ufail(u, GARM_EHIP, "network gradient failed");
+	*loss = hl;
+	return unetGetParams(u->U, grad, true, u->err) ? GARM_OK : GARM_EHIP;
+}
+
+int garm_unet_train(garm_unet *u, const int dims[3], int n_pairs, const float *rhs, const float *pressure, const uint8_t *mask, int steps,
+	float lr, unsigned long long seed, float *loss_history)
+{
+	size_t vox;
+	if (!u || !rhs || !pressure || !mask || n_pairs < 1 || steps < 0 || !(lr > 0) || !dims3(dims, vox)) return GARM_EINVAL;
+	if (!u->U.buf.reserve(vox, true, u->err) || !u->U.buf.setDims(dims[0], dims[1], dims[2], u->err)) return GARM_ENOMEM;
+	const size_t total = vox * size_t(n_pairs);
+	DevBuf<float> dr(total), dp(total), part(1024), dl(std::max(steps, 1));
+	DevBuf<uint8_t> dm(total);
+	if (!dr.p || !dp.p || !dm.p || !part.p || !dl.p) return ufail(u, GARM_ENOMEM, "training set does not fit");
+	if (hipMemcpy(dr.p, rhs, total * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+		hipMemcpy(dp.p, pressure, total * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+		hipMemcpy(dm.p, mask, total, hipMemcpyHostToDevice) != hipSuccess) return ufail(u, GARM_EHIP, "upload failed");
+	// per pair: unit-RMS scale of the right-hand side over its unknowns
+	std::vector<float> scale(n_pairs);
+	std::vector<double> invCount(n_pairs);
+	for (int i = 0; i < n_pairs; i++) {
+		double ss = 0;
+		size_t m = 0;
+		for (size_t v = 0; v < vox; v++)
+			if (mask[i * vox + v]) { ss += double(rhs[i * vox + v]) * rhs[i * vox + v]; m++; }
+		scale[i] = (ss > 0 && m > 0) ? float(std::sqrt(double(m) / ss)) : 1.0f;
+		invCount[i] = m > 0 ? 1.0 / double(m) : 0.0;
+	}
+	std::mt19937_64 rng(seed);
+	const int nb = 1024;
+	for (int k = 0; k < steps; k++) {
+		const int i = int(rng() % uint64_t(n_pairs));
+		const size_t off = size_t(i) * vox;
+		hipLaunchKernelGGL(kLoadScaled, dim3(1024), dim3(kBlock), 0, u->s, (const float *)(dr.p + off), (const uint8_t *)(dm.p + off), scale[i], u->U.buf.in, vox);
+		unetForward(u->U, u->s);
+		hipLaunchKernelGGL(kLossGrad, dim3(nb), dim3(kBlock), 0, u->s, (const float *)u->U.buf.y, (const float *)(dp.p + off), (const uint8_t *)(dm.p + off),
+			scale[i], float(invCount[i]), u->U.buf.gy, vox, part.p);
+		hipLaunchKernelGGL(kLossReduce, dim3(1), dim3(64), 0, u->s, (const float *)part.p, nb, float(invCount[i]), dl.p + k);
+		if (!unetBackward(u->U, u->s, u->err)) return GARM_ENOMEM;
+		unetAdam(u->U, lr, 0.9f, 0.999f, 1e-8f, u->s);
+	}
+	if (hipGetLastError() != hipSuccess || hipStreamSynchronize(u->s) != hipSuccess) return ufail(u, GARM_EHIP, "training failed");
+	if (loss_history && steps > 0 && hipMemcpy(loss_history, dl.p, steps * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+		return ufail(u, GARM_EHIP, "download failed");
+	return GARM_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,32 @@
+// Flexible PCG preconditioned by the U-Net, with plain PCG as the fallback.
+#pragma once
+
+#include "garm_common.hpp"
+#include "garm_pcg.hpp"
+#include "garm_unet.hpp"
+#include "../../include/garm_gpu.h"
+
+#include <string>
+
+namespace garm {
+
+// FP64 vectors and scalars of the flexible solve.
+struct LearnedWork {
+	double *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *t = nullptr, *b2 = nullptr;
+	double *pa = nullptr, *pb = nullptr, *ls = nullptr;
+	int *box = nullptr;
+	double *host = nullptr;   // pinned
+	size_t cap = 0;
+	~LearnedWork();
+	bool reserve(size_t n, std::string &err);
+};
+
+// Solves A x = b; the solution is left in W.x.  grid2mat / mat2grid map the
+// cell grid `cell` to the compact unknowns.
+int solveLearned(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *grid2mat, const int *mat2grid,
+	const GridDev &cell, const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Unet &U, hipStream_t s, garm_pcg_stats *st, std::string &err);
+
+// The network behind a C-ABI handle and the device it was created on.
+Unet *unetOf(garm_unet *net, int *device);
+
+} // namespace garm

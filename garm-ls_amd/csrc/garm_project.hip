@@ -126,11 +126,12 @@ __global__ __launch_bounds__(kBlock) void kApplyPressureGradient(GridDev fg, Gri
 	}
 }
 
-__global__ void kScatterPressure(const int *grid2mat, const double *x, double *out, size_t n)
+// x == nullptr writes the unknown indicator instead
+__global__ void kScatterCompact(const int *grid2mat, const double *x, double *out, size_t n)
 {
 	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
 		const int m = grid2mat[i];
-		out[i] = m >= 0 ? x[m] : 0.0;
+		out[i] = m >= 0 ? (x ? x[m] : 1.0) : 0.0;
 	}
 }
 
@@ -229,7 +230,9 @@ static void assemble(const Fields &F, Projection &P, int n, bool mixed, hipStrea
 template <int D>
 static int projectD(Fields &F, Projection &P, const garm_pcg_opts &o, garm_pcg_stats *st, hipStream_t s, std::string &err)
 {
-	const bool mixed = o.mode == GARM_PCG_MIXED;
+	const bool learned = o.mode == GARM_PCG_LEARNED;
+	const bool mixed = o.mode == GARM_PCG_MIXED || learned;
+	if (learned && !P.net) { err = "no preconditioner network is attached"; return GARM_EINVAL; }
 	if (!P.reserveGrid(F.N, err)) return GARM_ENOMEM;
 	const int n = numberUnknowns<D>(F, P, mixed, s, err);
 	if (n < 0) return GARM_EHIP;
@@ -242,7 +245,13 @@ static int projectD(Fields &F, Projection &P, const garm_pcg_opts &o, garm_pcg_s
 	int rc = GARM_OK;
 	if (n > 0) {
 		assemble<D>(F, P, n, mixed, s);
-		if (mixed) rc = solveMixed(D, P.rows, P.rows32, P.rhs, n, o, P.K, s, st, err);
+		if (learned) {
+			rc = solveLearned(D, P.rows, P.rows32, P.rhs, n, P.grid2mat, P.mat2grid, F.cell, o, P.K, P.LW, *P.net, s, st, err);
+			if ((rc == GARM_OK || rc == GARM_ENOCONV) && hipMemcpyAsync(P.K.x, P.LW.x, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, s) != hipSuccess) {
+				err = "copying the learned solution failed";
+				rc = GARM_EHIP;
+			}
+		} else if (mixed) rc = solveMixed(D, P.rows, P.rows32, P.rhs, n, o, P.K, s, st, err);
 		else rc = solvePcg64(D, P.rows, P.rhs, n, o, P.K, s, st, err);
 	}
 	// the velocity is updated with whatever the solver reached (the reference
@@ -258,10 +267,11 @@ int runProjection(Fields &F, Projection &P, const garm_pcg_opts &o, garm_pcg_sta
 	return F.D == 2 ? projectD<2>(F, P, o, st, s, err) : projectD<3>(F, P, o, st, s, err);
 }
 
-void scatterPressure(const Fields &F, const Projection &P, double *out, hipStream_t s)
+void scatterCompact(const Fields &F, const Projection &P, int which, double *out, hipStream_t s)
 {
 	if (!P.grid2mat || P.n == 0) { launchFill(out, 0.0, F.N, s); return; }
-	hipLaunchKernelGGL(kScatterPressure, dim3(linBlocks(F.N)), dim3(kBlock), 0, s, (const int *)P.grid2mat, (const double *)P.K.x, out, F.N);
+	const double *x = which == kCompactPressure ? P.K.x : which == kCompactRhs ? P.rhs : nullptr;
+	hipLaunchKernelGGL(kScatterCompact, dim3(linBlocks(F.N)), dim3(kBlock), 0, s, (const int *)P.grid2mat, x, out, F.N);
 }
 
 } // namespace garm

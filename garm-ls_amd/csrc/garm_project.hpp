@@ -3,6 +3,7 @@
 
 #include "garm_internal.hpp"
 #include "garm_pcg.hpp"
+#include "garm_learned.hpp"
 #include "../../include/garm_gpu.h"
 
 #include <string>
@@ -23,6 +24,8 @@ struct Projection {
 	Rows<float> rows32{};
 	double *rhs = nullptr;
 	Krylov K;
+	Unet *net = nullptr;      // preconditioner of learned solves (not owned)
+	LearnedWork LW;
 	size_t cap = 0;
 	int n = 0;
 
@@ -33,6 +36,8 @@ struct Projection {
 };
 
 int runProjection(Fields &F, Projection &P, const garm_pcg_opts &o, garm_pcg_stats *st, hipStream_t s, std::string &err);
-void scatterPressure(const Fields &F, const Projection &P, double *out, hipStream_t s);
+// What scatterCompact writes to every cell (0 where there is no unknown).
+enum CompactField { kCompactPressure = 0, kCompactRhs = 1, kCompactIndicator = 2 };
+void scatterCompact(const Fields &F, const Projection &P, int which, double *out, hipStream_t s);
 
 } // namespace garm

@@ -1,0 +1,777 @@
+// Three-level 3D U-Net (16/32/64 channels) on gfx950: BF16 channels-last
+// activations and weights, 3x3x3 convolutions as implicit GEMM on
+// v_mfma_f32_16x16x16_bf16 with FP32 accumulation, FP32 master weights and
+// Adam.  The network has no counterpart in the reference; its layer order,
+// layouts and gradients follow oracle/unet_oracle.py, which the tests hold it
+// against.
+//
+// Convolution tile: 16 (x) x 4 (y) x 4 (z) output voxels per 256-thread
+// workgroup, each wave64 one z-slice of four 16-voxel M-fragments, output
+// channels in groups of 32 (two N-fragments).  The input tile with a one-voxel
+// halo (18 x 6 x 6 voxels, 16 channels at a time) and the matching weights are
+// staged in LDS (48 KB), so every input voxel is fetched from HBM once per
+// 16-channel chunk and group instead of 27 times.
+//
+// MFMA 16x16x16 BF16 lane layout (wave64): lane l holds A[l%16][4(l/16)..+3],
+// B[4(l/16)..+3][l%16], and D[4(l/16)+i][l%16] for i = 0..3.
+#include "garm_unet.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+
+namespace garm {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int TX = 16, TY = 4, TZ = 4, HX = TX + 2, HY = TY + 2, HZ = TZ + 2;
+constexpr int kUB = 256;
+
+static int roundUp16(int c) { return (c + 15) / 16 * 16; }
+
+// ------------------------------------------------------------- convolution
+struct ConvArgs {
+	const bf16 *in;
+	int inStride, Cin;
+	const bf16 *w;          // [27][coutP][cinP]
+	int cinP, coutP;
+	const float *bias;
+	bf16 *out;
+	int outStride, outOff, Cout;
+	int nx, ny, nz;
+	int relu;
+};
+
+__global__ __launch_bounds__(kUB) void kConv3(ConvArgs A)
+{
+	__shared__ __align__(16) bf16 tin[HZ * HY * HX * 16];
+	__shared__ __align__(16) bf16 tw[27 * 32 * 16];
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int kg = lane >> 4, li = lane & 15;
+	const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+	const int nzt = (A.nz + TZ - 1) / TZ;
+	const int zt = blockIdx.z % nzt, grp = blockIdx.z / nzt;
+	const int z0 = zt * TZ, co0 = grp * 32;
+	const int NF = min(32, A.coutP - co0) / 16;
+	f4 acc[4][2];
+#pragma unroll
+	for (int f = 0; f < 4; f++)
+#pragma unroll
+		for (int q = 0; q < 2; q++) acc[f][q] = f4{0, 0, 0, 0};
+
+	for (int ci0 = 0; ci0 < A.Cin; ci0 += 16) {
+		for (int q = tid; q < HZ * HY * HX * 4; q += kUB) {
+			const int quad = q & 3, v = q >> 2;
+			const int hx = v % HX, hy = (v / HX) % HY, hz = v / (HX * HY);
+			const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+			const int ch = ci0 + quad * 4;
+			uint2 val = make_uint2(0, 0);
+			if (gx >= 0 && gx < A.nx && gy >= 0 && gy < A.ny && gz >= 0 && gz < A.nz && ch < A.Cin)
+				val = *reinterpret_cast<const uint2 *>(A.in + (size_t(gz * A.ny + gy) * A.nx + gx) * A.inStride + ch);
+			*reinterpret_cast<uint2 *>(tin + v * 16 + quad * 4) = val;
+		}
+		for (int q = tid; q < 27 * NF * 16 * 4; q += kUB) {
+			const int quad = q & 3, rr = q >> 2;
+			const int co = rr % (NF * 16), t = rr / (NF * 16);
+			const uint2 val = *reinterpret_cast<const uint2 *>(A.w + (size_t(t) * A.coutP + co0 + co) * A.cinP + ci0 + quad * 4);
+			*reinterpret_cast<uint2 *>(tw + (t * 32 + co) * 16 + quad * 4) = val;
+		}
+		__syncthreads();
+		for (int t = 0; t < 27; t++) {
+			const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
+			s4 b[2];
+			b[0] = *reinterpret_cast<const s4 *>(tw + (t * 32 + li) * 16 + kg * 4);
+			b[1] = NF > 1 ? *reinterpret_cast<const s4 *>(tw + (t * 32 + 16 + li) * 16 + kg * 4) : b[0];
+#pragma unroll
+			for (int f = 0; f < 4; f++) {
+				const s4 a = *reinterpret_cast<const s4 *>(tin + (((wave + dz) * HY + f + dy) * HX + li + dx) * 16 + kg * 4);
+				acc[f][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[0], acc[f][0], 0, 0, 0);
+				if (NF > 1) acc[f][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[1], acc[f][1], 0, 0, 0);
+			}
+		}
+		__syncthreads();
+	}
+	const int z = z0 + wave;
+#pragma unroll
+	for (int f = 0; f < 4; f++) {
+		const int y = y0 + f;
+#pragma unroll
+		for (int q = 0; q < 2; q++) {
+			if (q >= NF) continue;
+			const int co = co0 + q * 16 + li;
+			if (co >= A.Cout || y >= A.ny || z >= A.nz) continue;
+			const float bb = A.bias ? A.bias[co] : 0.0f;
+#pragma unroll
+			for (int i = 0; i < 4; i++) {
+				const int x = x0 + kg * 4 + i;
+				if (x >= A.nx) continue;
+				float v = acc[f][q][i] + bb;
+				if (A.relu) v = fmaxf(v, 0.0f);
+				A.out[(size_t(z * A.ny + y) * A.nx + x) * A.outStride + A.outOff + co] = f2bf(v);
+			}
+		}
+	}
+}
+
+// Weight gradient partials: for each tap t, dW_t = X_t^T Gz over this
+// workgroup's 256 voxels (K = voxels on the MFMA), and the bias partial.
+struct WGradArgs {
+	const bf16 *x;
+	int xStride, Cin;
+	const bf16 *gz;
+	int gStride, gOff, Cout;
+	int cinP, coutP;
+	int nx, ny, nz;
+	float *partial;    // [block][27][cinP][coutP]
+	float *partialB;   // [block][coutP]
+};
+
+__global__ __launch_bounds__(kUB) void kConv3WGrad(WGradArgs A)
+{
+	__shared__ __align__(16) bf16 tin[HZ * HY * HX * 16];
+	__shared__ __align__(16) bf16 tg[TZ * TY * TX * 32];
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int kg = lane >> 4, li = lane & 15;
+	const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+	const int nzt = (A.nz + TZ - 1) / TZ;
+	const int zt = blockIdx.z % nzt, grp = blockIdx.z / nzt;
+	const int z0 = zt * TZ, co0 = grp * 32;
+	const int NF = min(32, A.coutP - co0) / 16;
+	const int block = (zt * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+	// output-gradient tile (256 voxels x up to 32 channels), zero outside
+	for (int q = tid; q < TZ * TY * TX * 32; q += kUB) {
+		const int c = q & 31, v = q >> 5;
+		const int vx = v % TX, vy = (v / TX) % TY, vz = v / (TX * TY);
+		const int gx = x0 + vx, gy = y0 + vy, gzz = z0 + vz;
+		bf16 val = 0;
+		if (c < NF * 16 && co0 + c < A.Cout && gx < A.nx && gy < A.ny && gzz < A.nz)
+			val = A.gz[(size_t(gzz * A.ny + gy) * A.nx + gx) * A.gStride + A.gOff + co0 + c];
+		tg[v * 32 + c] = val;
+	}
+	__syncthreads();
+	if (tid < NF * 16) {
+		float sb = 0;
+		for (int v = 0; v < TZ * TY * TX; v++) sb += bf2f(tg[v * 32 + tid]);
+		A.partialB[size_t(block) * A.coutP + co0 + tid] = sb;
+	}
+	for (int ci0 = 0; ci0 < A.cinP; ci0 += 16) {
+		__syncthreads();
+		for (int q = tid; q < HZ * HY * HX * 4; q += kUB) {
+			const int quad = q & 3, v = q >> 2;
+			const int hx = v % HX, hy = (v / HX) % HY, hz = v / (HX * HY);
+			const int gx = x0 + hx - 1, gy = y0 + hy - 1, gzz = z0 + hz - 1;
+			const int ch = ci0 + quad * 4;
+			uint2 val = make_uint2(0, 0);
+			if (gx >= 0 && gx < A.nx && gy >= 0 && gy < A.ny && gzz >= 0 && gzz < A.nz && ch < A.Cin)
+				val = *reinterpret_cast<const uint2 *>(A.x + (size_t(gzz * A.ny + gy) * A.nx + gx) * A.xStride + ch);
+			*reinterpret_cast<uint2 *>(tin + v * 16 + quad * 4) = val;
+		}
+		__syncthreads();
+		for (int t = wave; t < 27; t += 4) {
+			const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
+			f4 acc[2] = {f4{0, 0, 0, 0}, f4{0, 0, 0, 0}};
+			for (int s = 0; s < TZ * TY * TX / 16; s++) {
+				s4 a, b0, b1;
+#pragma unroll
+				for (int j = 0; j < 4; j++) {
+					const int vk = s * 16 + kg * 4 + j;
+					const int vx = vk % TX, vy = (vk / TX) % TY, vz = vk / (TX * TY);
+					a[j] = short(tin[(((vz + dz) * HY + vy + dy) * HX + vx + dx) * 16 + li]);
+					b0[j] = short(tg[vk * 32 + li]);
+					b1[j] = short(tg[vk * 32 + 16 + li]);
+				}
+				acc[0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b0, acc[0], 0, 0, 0);
+				if (NF > 1) acc[1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b1, acc[1], 0, 0, 0);
+			}
+			float *dst = A.partial + (size_t(block) * 27 + t) * A.cinP * A.coutP;
+			for (int q = 0; q < NF; q++)
+#pragma unroll
+				for (int i = 0; i < 4; i++) dst[size_t(ci0 + kg * 4 + i) * A.coutP + co0 + q * 16 + li] = acc[q][i];
+		}
+	}
+}
+
+// Deterministic sum of the partials into the FP32 gradients [27][cin][cout].
+__global__ void kReduceWGrad(const float *partial, const float *partialB, int nblocks, int cin, int cout, int cinP, int coutP, float *gw, float *gb)
+{
+	const int n = 27 * cin * cout;
+	for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n + cout; e += gridDim.x * blockDim.x) {
+		if (e < n) {
+			const int co = e % cout, ci = (e / cout) % cin, t = e / (cout * cin);
+			const size_t off = (size_t(t) * cinP + ci) * coutP + co;
+			float s = 0;
+			for (int b = 0; b < nblocks; b++) s += partial[size_t(b) * 27 * cinP * coutP + off];
+			gw[e] = s;
+		} else {
+			const int co = e - n;
+			float s = 0;
+			for (int b = 0; b < nblocks; b++) s += partialB[size_t(b) * coutP + co];
+			gb[co] = s;
+		}
+	}
+}
+
+// ------------------------------------------------------ elementwise pieces
+__device__ __forceinline__ void coords(size_t v, int nx, int ny, int &x, int &y, int &z)
+{
+	x = int(v % nx);
+	y = int((v / nx) % ny);
+	z = int(v / (size_t(nx) * ny));
+}
+
+// 2x2x2 mean (child grid n = 2 * parent)
+__global__ void kPool(const bf16 *in, int inStride, int inOff, int C, bf16 *out, int outStride, int nx, int ny, int nz)
+{
+	const int px = nx / 2, py = ny / 2, pz = nz / 2;
+	const size_t total = size_t(px) * py * pz * C;
+	for (size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x; e < total; e += size_t(gridDim.x) * blockDim.x) {
+		const int c = int(e % C);
+		int x, y, z;
+		coords(e / C, px, py, x, y, z);
+		float s = 0;
+		for (int k = 0; k < 8; k++) {
+			const int cx = 2 * x + (k & 1), cy = 2 * y + (k >> 1 & 1), cz = 2 * z + (k >> 2);
+			s += bf2f(in[(size_t(cz * ny + cy) * nx + cx) * inStride + inOff + c]);
+		}
+		out[(size_t(z * py + y) * px + x) * outStride + c] = f2bf(s * 0.125f);
+	}
+}
+
+// gradient of the mean: child += parent / 8
+__global__ void kPoolBack(const bf16 *gp, int pStride, int C, bf16 *gc, int cStride, int cOff, int nx, int ny, int nz)
+{
+	const size_t total = size_t(nx) * ny * nz * C;
+	for (size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x; e < total; e += size_t(gridDim.x) * blockDim.x) {
+		const int c = int(e % C);
+		int x, y, z;
+		coords(e / C, nx, ny, x, y, z);
+		const float g = bf2f(gp[(size_t((z / 2) * (ny / 2) + y / 2) * (nx / 2) + x / 2) * pStride + c]) * 0.125f;
+		bf16 &dst = gc[(size_t(z * ny + y) * nx + x) * cStride + cOff + c];
+		dst = f2bf(bf2f(dst) + g);
+	}
+}
+
+// nearest-neighbour x2 into channels [0, C) of a wider tensor (child grid n)
+__global__ void kUp(const bf16 *in, int inStride, int C, bf16 *out, int outStride, int nx, int ny, int nz)
+{
+	const size_t total = size_t(nx) * ny * nz * C;
+	for (size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x; e < total; e += size_t(gridDim.x) * blockDim.x) {
+		const int c = int(e % C);
+		int x, y, z;
+		coords(e / C, nx, ny, x, y, z);
+		out[(size_t(z * ny + y) * nx + x) * outStride + c] = in[(size_t((z / 2) * (ny / 2) + y / 2) * (nx / 2) + x / 2) * inStride + c];
+	}
+}
+
+// gradient of the copy: parent = sum of its eight children
+__global__ void kUpBack(const bf16 *gc, int cStride, int C, bf16 *gp, int pStride, int nx, int ny, int nz)
+{
+	const int px = nx / 2, py = ny / 2, pz = nz / 2;
+	const size_t total = size_t(px) * py * pz * C;
+	for (size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x; e < total; e += size_t(gridDim.x) * blockDim.x) {
+		const int c = int(e % C);
+		int x, y, z;
+		coords(e / C, px, py, x, y, z);
+		float s = 0;
+		for (int k = 0; k < 8; k++) {
+			const int cx = 2 * x + (k & 1), cy = 2 * y + (k >> 1 & 1), cz = 2 * z + (k >> 2);
+			s += bf2f(gc[(size_t(cz * ny + cy) * nx + cx) * cStride + c]);
+		}
+		gp[(size_t(z * py + y) * px + x) * pStride + c] = f2bf(s);
+	}
+}
+
+// ReLU backward in place: g *= (a > 0)
+__global__ void kReluBack(bf16 *g, const bf16 *a, int stride, int off, int C, size_t vox)
+{
+	const size_t total = vox * C;
+	for (size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x; e < total; e += size_t(gridDim.x) * blockDim.x) {
+		const size_t idx = (e / C) * stride + off + e % C;
+		if (!(bf2f(a[idx]) > 0.0f)) g[idx] = 0;
+	}
+}
+
+// 1x1 output convolution 16 -> 1 times the mask (input channel 1)
+__global__ void kOut(const bf16 *d1, const bf16 *in, const float *w, const float *b, float *y, size_t vox)
+{
+	for (size_t v = blockIdx.x * size_t(blockDim.x) + threadIdx.x; v < vox; v += size_t(gridDim.x) * blockDim.x) {
+		float s = b[0];
+		for (int c = 0; c < 16; c++) s += bf2f(d1[v * 16 + c]) * w[c];
+		y[v] = s * bf2f(in[v * 4 + 1]);
+	}
+}
+
+__global__ __launch_bounds__(kUB) void kOutBack(const float *gy, const bf16 *d1, const bf16 *in, const float *w, bf16 *gd1, float *part, size_t vox)
+{
+	float acc[17];
+	for (int c = 0; c < 17; c++) acc[c] = 0;
+	for (size_t v = blockIdx.x * size_t(blockDim.x) + threadIdx.x; v < vox; v += size_t(gridDim.x) * blockDim.x) {
+		const float g = gy[v] * bf2f(in[v * 4 + 1]);
+		for (int c = 0; c < 16; c++) {
+			acc[c] += g * bf2f(d1[v * 16 + c]);
+			gd1[v * 16 + c] = f2bf(g * w[c]);
+		}
+		acc[16] += g;
+	}
+	__shared__ float red[17][kUB];
+	for (int c = 0; c < 17; c++) red[c][threadIdx.x] = acc[c];
+	__syncthreads();
+	if (threadIdx.x < 17) {
+		float s = 0;
+		for (int i = 0; i < kUB; i++) s += red[threadIdx.x][i];
+		part[blockIdx.x * 17 + threadIdx.x] = s;
+	}
+}
+
+__global__ void kOutBackReduce(const float *part, int nb, float *gw, float *gb)
+{
+	const int c = threadIdx.x;
+	if (c >= 17) return;
+	float s = 0;
+	for (int b = 0; b < nb; b++) s += part[b * 17 + c];
+	if (c < 16) gw[c] = s; else gb[0] = s;
+}
+
+__global__ void kAdam(float *w, const float *g, float *m, float *v, int n, float lr, float b1, float b2, float eps, float c1, float c2)
+{
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+		const float gi = g[i];
+		const float mi = b1 * m[i] + (1 - b1) * gi;
+		const float vi = b2 * v[i] + (1 - b2) * gi * gi;
+		m[i] = mi;
+		v[i] = vi;
+		w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+	}
+}
+
+// forward pack [27][coutP][cinP] and data-gradient pack [27][cinP][coutP]
+// (taps mirrored: the transpose of a cross-correlation)
+__global__ void kPack(const float *w, int cin, int cout, int cinP, int coutP, bf16 *wf, bf16 *wd)
+{
+	const int n = 27 * cinP * coutP;
+	for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+		const int ci = e % cinP, co = (e / cinP) % coutP, t = e / (cinP * coutP);
+		const float v = (ci < cin && co < cout) ? w[(size_t(t) * cin + ci) * cout + co] : 0.0f;
+		wf[(size_t(t) * coutP + co) * cinP + ci] = f2bf(v);
+		const float vd = (ci < cin && co < cout) ? w[(size_t(26 - t) * cin + ci) * cout + co] : 0.0f;
+		wd[(size_t(t) * cinP + ci) * coutP + co] = f2bf(vd);
+	}
+}
+
+__global__ void kZeroBf(bf16 *p, size_t n)
+{
+	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = 0;
+}
+
+// input tensor (r, mask, 0, 0) from FP32 grid arrays
+__global__ void kLoadInput(const float *r, const uint8_t *mask, bf16 *in, size_t vox)
+{
+	for (size_t v = blockIdx.x * size_t(blockDim.x) + threadIdx.x; v < vox; v += size_t(gridDim.x) * blockDim.x) {
+		const bool m = mask[v] != 0;
+		in[v * 4 + 0] = f2bf(m ? r[v] : 0.0f);
+		in[v * 4 + 1] = f2bf(m ? 1.0f : 0.0f);
+		in[v * 4 + 2] = 0;
+		in[v * 4 + 3] = 0;
+	}
+}
+
+// compact residual -> box input tensor, scaled to unit RMS over the n
+// unknowns (rr = |r|^2 on the device)
+__global__ void kGatherBox(UnetBox B, const int *grid2mat, const double *r, const double *rr, int n, bf16 *in)
+{
+	const double scale = rr[0] > 0 ? sqrt(double(n) / rr[0]) : 1.0;
+	const size_t vox = size_t(B.n[0]) * B.n[1] * B.n[2];
+	for (size_t v = blockIdx.x * size_t(blockDim.x) + threadIdx.x; v < vox; v += size_t(gridDim.x) * blockDim.x) {
+		int x, y, z;
+		coords(v, B.n[0], B.n[1], x, y, z);
+		const int gx = B.org[0] + x, gy = B.org[1] + y, gz = B.org[2] + z;
+		int m = -1;
+		if (gx >= 0 && gx < B.gn[0] && gy >= 0 && gy < B.gn[1] && gz >= 0 && gz < B.gn[2])
+			m = grid2mat[(size_t(gz) * B.gn[1] + gy) * B.gn[0] + gx];
+		in[v * 4 + 0] = f2bf(m >= 0 ? float(r[m] * scale) : 0.0f);
+		in[v * 4 + 1] = f2bf(m >= 0 ? 1.0f : 0.0f);
+		in[v * 4 + 2] = 0;
+		in[v * 4 + 3] = 0;
+	}
+}
+
+// box output -> compact unknowns: z += y / scale
+__global__ void kScatterBox(UnetBox B, const int *grid2mat, const float *y, const double *rr, int n, double *z)
+{
+	const double inv = rr[0] > 0 ? sqrt(rr[0] / double(n)) : 1.0;
+	const size_t vox = size_t(B.n[0]) * B.n[1] * B.n[2];
+	for (size_t v = blockIdx.x * size_t(blockDim.x) + threadIdx.x; v < vox; v += size_t(gridDim.x) * blockDim.x) {
+		int x, yy, zz;
+		coords(v, B.n[0], B.n[1], x, yy, zz);
+		const int gx = B.org[0] + x, gy = B.org[1] + yy, gz = B.org[2] + zz;
+		if (gx < 0 || gx >= B.gn[0] || gy < 0 || gy >= B.gn[1] || gz < 0 || gz >= B.gn[2]) continue;
+		const int m = grid2mat[(size_t(gz) * B.gn[1] + gy) * B.gn[0] + gx];
+		if (m >= 0) z[m] += double(y[v]) * inv;
+	}
+}
+
+// ------------------------------------------------------------------ host
+static unsigned blocksFor(size_t n) { return unsigned(std::min<size_t>((n + kUB - 1) / kUB, 8192)); }
+
+struct LayerIO {
+	const bf16 *in;
+	int inStride;
+	bf16 *out;
+	int outStride, outOff;
+	int level;
+};
+
+static const int kCin[kUnetLayers] = {4, 16, 16, 32, 32, 64, 96, 32, 48, 16};
+static const int kCout[kUnetLayers] = {16, 16, 32, 32, 64, 64, 32, 32, 16, 16};
+static const int kLevel[kUnetLayers] = {0, 0, 1, 1, 2, 2, 1, 1, 0, 0};
+
+UnetBuffers::~UnetBuffers()
+{
+	for (void *p : owned) (void)hipFree(p);
+}
+
+bool UnetBuffers::reserve(size_t maxVox, bool train, std::string &err)
+{
+	if (maxVox <= cap_vox && (!train || has_grad)) return true;
+	for (void *p : owned) (void)hipFree(p);
+	owned.clear();
+	maxVox = std::max(maxVox, cap_vox);
+	const size_t v0 = maxVox, v1 = maxVox / 8 + 1, v2 = maxVox / 64 + 1;
+	bool ok = true;
+	auto A = [&](auto *&p, size_t count) {
+		void *q = nullptr;
+		if (ok && hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(*p)) == hipSuccess) { owned.push_back(q); p = static_cast<std::remove_reference_t<decltype(p)>>(q); }
+		else ok = false;
+	};
+	A(in, v0 * 4); A(a1, v0 * 16); A(c1, v0 * 48); A(a5, v0 * 16); A(d1, v0 * 16);
+	A(p1, v1 * 16); A(a2, v1 * 32); A(c2, v1 * 96); A(a4, v1 * 32); A(d2, v1 * 32);
+	A(p2, v2 * 32); A(a3, v2 * 64); A(b3, v2 * 64);
+	A(y, v0);
+	if (train) {
+		A(gy, v0);
+		A(g_a1, v0 * 16); A(g_c1, v0 * 48); A(g_a5, v0 * 16); A(g_d1, v0 * 16);
+		A(g_p1, v1 * 16); A(g_a2, v1 * 32); A(g_c2, v1 * 96); A(g_a4, v1 * 32); A(g_d2, v1 * 32);
+		A(g_p2, v2 * 32); A(g_a3, v2 * 64); A(g_b3, v2 * 64);
+		// weight-gradient partials: one per 256-voxel tile (tiles of a level
+		// hold at most vox / 256 + faces of partial tiles; bound generously)
+		size_t cap = 0;
+		for (int i = 0; i < kUnetLayers; i++) {
+			const size_t vl = kLevel[i] == 0 ? v0 : kLevel[i] == 1 ? v1 : v2;
+			const size_t blocks = vl / 64 + 64;
+			cap = std::max(cap, blocks * (27 * roundUp16(kCin[i]) * roundUp16(kCout[i]) + roundUp16(kCout[i])));
+		}
+		A(partial, cap);
+		partial_cap = cap;
+		A(red, 17 * 1024);
+	} else {
+		gy = nullptr; partial = nullptr; red = nullptr; partial_cap = 0;
+		g_a1 = g_c1 = g_a5 = g_d1 = g_p1 = g_a2 = g_c2 = g_a4 = g_d2 = g_p2 = g_a3 = g_b3 = nullptr;
+	}
+	if (!ok) { err = "network buffer allocation failed"; cap_vox = 0; has_grad = false; return false; }
+	cap_vox = maxVox;
+	has_grad = train;
+	return true;
+}
+
+bool UnetBuffers::setDims(int nx, int ny, int nz, std::string &err)
+{
+	if (nx % 4 || ny % 4 || nz % 4 || nx < 4 || ny < 4 || nz < 4) { err = "network grid sizes must be positive multiples of 4"; return false; }
+	if (size_t(nx) * ny * nz > cap_vox) { err = "network grid exceeds the reserved capacity"; return false; }
+	for (int l = 0; l < 3; l++) {
+		n[l][0] = nx >> l; n[l][1] = ny >> l; n[l][2] = nz >> l;
+		vox[l] = size_t(n[l][0]) * n[l][1] * n[l][2];
+	}
+	return true;
+}
+
+Unet::~Unet()
+{
+	for (void *p : owned) (void)hipFree(p);
+}
+
+size_t unetParamCount()
+{
+	size_t n = 0;
+	for (int i = 0; i < kUnetLayers; i++) n += size_t(27) * kCin[i] * kCout[i] + kCout[i];
+	return n + 16 + 1;
+}
+
+bool unetCreate(Unet &U, unsigned long long seed, hipStream_t s, std::string &err)
+{
+	U.s = s;
+	bool ok = true;
+	auto A = [&](float *&p, size_t count) {
+		void *q = nullptr;
+		if (ok && hipMalloc(&q, count * sizeof(float)) == hipSuccess) { U.owned.push_back(q); p = static_cast<float *>(q); (void)hipMemset(q, 0, count * sizeof(float)); }
+		else ok = false;
+	};
+	auto Ab = [&](bf16 *&p, size_t count) {
+		void *q = nullptr;
+		if (ok && hipMalloc(&q, count * sizeof(bf16)) == hipSuccess) { U.owned.push_back(q); p = static_cast<bf16 *>(q); }
+		else ok = false;
+	};
+	for (int i = 0; i < kUnetLayers; i++) {
+		UnetLayer &L = U.L[i];
+		L.cin = kCin[i]; L.cout = kCout[i]; L.level = kLevel[i];
+		const size_t nw = size_t(27) * L.cin * L.cout;
+		A(L.w, nw); A(L.gw, nw); A(L.mw, nw); A(L.vw, nw);
+		A(L.b, L.cout); A(L.gb, L.cout); A(L.mb, L.cout); A(L.vb, L.cout);
+		const size_t np = size_t(27) * roundUp16(L.cin) * roundUp16(L.cout);
+		Ab(L.wf, np); Ab(L.wd, np);
+	}
+	A(U.ow, 16); A(U.gow, 16); A(U.mow, 16); A(U.vow, 16);
+	A(U.ob, 1); A(U.gob, 1); A(U.mob, 1); A(U.vob, 1);
+	if (!ok) { err = "network parameter allocation failed"; return false; }
+	// He-normal initialisation (std sqrt(2 / fan_in)), zero biases, seeded
+	std::mt19937_64 rng(seed);
+	std::vector<float> flat(unetParamCount(), 0.0f);
+	size_t off = 0;
+	for (int i = 0; i < kUnetLayers; i++) {
+		const int fanIn = 27 * (i == 0 ? 2 : kCin[i]);
+		std::normal_distribution<double> nd(0.0, std::sqrt(2.0 / fanIn));
+		for (int t = 0; t < 27; t++)
+			for (int ci = 0; ci < kCin[i]; ci++)
+				for (int co = 0; co < kCout[i]; co++) {
+					const double v = nd(rng);
+					flat[off + (size_t(t) * kCin[i] + ci) * kCout[i] + co] = (i == 0 && ci >= 2) ? 0.0f : float(v);
+				}
+		off += size_t(27) * kCin[i] * kCout[i] + kCout[i];
+	}
+	std::normal_distribution<double> nd(0.0, std::sqrt(2.0 / 16));
+	for (int c = 0; c < 16; c++) flat[off + c] = float(nd(rng));
+	return unetSetParams(U, flat.data(), err);
+}
+
+// flat order: per layer w [27][cin][cout] then b [cout]; then out.w [16], out.b [1]
+bool unetSetParams(Unet &U, const float *flat, std::string &err)
+{
+	size_t off = 0;
+	for (int i = 0; i < kUnetLayers; i++) {
+		UnetLayer &L = U.L[i];
+		const size_t nw = size_t(27) * L.cin * L.cout;
+		if (hipMemcpy(L.w, flat + off, nw * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+			hipMemcpy(L.b, flat + off + nw, L.cout * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) { err = "parameter upload failed"; return false; }
+		off += nw + L.cout;
+	}
+	if (hipMemcpy(U.ow, flat + off, 16 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+		hipMemcpy(U.ob, flat + off + 16, sizeof(float), hipMemcpyHostToDevice) != hipSuccess) { err = "parameter upload failed"; return false; }
+	U.adam_t = 0;
+	unetPackWeights(U, U.s);
+	return hipStreamSynchronize(U.s) == hipSuccess;
+}
+
+bool unetGetParams(Unet &U, float *flat, bool grads, std::string &err)
+{
+	if (hipStreamSynchronize(U.s) != hipSuccess) { err = "synchronize failed"; return false; }
+	size_t off = 0;
+	for (int i = 0; i < kUnetLayers; i++) {
+		UnetLayer &L = U.L[i];
+		const size_t nw = size_t(27) * L.cin * L.cout;
+		if (hipMemcpy(flat + off, grads ? L.gw : L.w, nw * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess ||
+			hipMemcpy(flat + off + nw, grads ? L.gb : L.b, L.cout * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) { err = "parameter download failed"; return false; }
+		off += nw + L.cout;
+	}
+	if (hipMemcpy(flat + off, grads ? U.gow : U.ow, 16 * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess ||
+		hipMemcpy(flat + off + 16, grads ? U.gob : U.ob, sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) { err = "parameter download failed"; return false; }
+	return true;
+}
+
+void unetPackWeights(Unet &U, hipStream_t s)
+{
+	for (int i = 0; i < kUnetLayers; i++) {
+		UnetLayer &L = U.L[i];
+		hipLaunchKernelGGL(kPack, dim3(blocksFor(size_t(27) * roundUp16(L.cin) * roundUp16(L.cout))), dim3(kUB), 0, s,
+			(const float *)L.w, L.cin, L.cout, roundUp16(L.cin), roundUp16(L.cout), L.wf, L.wd);
+	}
+}
+
+static void conv(const UnetBuffers &B, const UnetLayer &L, int level, const bf16 *in, int inStride, int cin, const bf16 *w, int cinP, int coutP,
+	const float *bias, bf16 *out, int outStride, int outOff, int cout, bool relu, hipStream_t s)
+{
+	ConvArgs A{};
+	A.in = in; A.inStride = inStride; A.Cin = cin;
+	A.w = w; A.cinP = cinP; A.coutP = coutP;
+	A.bias = bias;
+	A.out = out; A.outStride = outStride; A.outOff = outOff; A.Cout = cout;
+	A.nx = B.n[level][0]; A.ny = B.n[level][1]; A.nz = B.n[level][2];
+	A.relu = relu ? 1 : 0;
+	const int groups = (coutP + 31) / 32;
+	const dim3 g((A.nx + TX - 1) / TX, (A.ny + TY - 1) / TY, ((A.nz + TZ - 1) / TZ) * groups);
+	hipLaunchKernelGGL(kConv3, g, dim3(kUB), 0, s, A);
+	(void)L;
+}
+
+// forward layer table: input tensor / stride, output tensor / stride / channel offset
+static void layerIO(UnetBuffers &B, int i, LayerIO &io)
+{
+	switch (i) {
+	case 0: io = {B.in, 4, B.a1, 16, 0, 0}; break;
+	case 1: io = {B.a1, 16, B.c1, 48, 32, 0}; break;
+	case 2: io = {B.p1, 16, B.a2, 32, 0, 1}; break;
+	case 3: io = {B.a2, 32, B.c2, 96, 64, 1}; break;
+	case 4: io = {B.p2, 32, B.a3, 64, 0, 2}; break;
+	case 5: io = {B.a3, 64, B.b3, 64, 0, 2}; break;
+	case 6: io = {B.c2, 96, B.a4, 32, 0, 1}; break;
+	case 7: io = {B.a4, 32, B.d2, 32, 0, 1}; break;
+	case 8: io = {B.c1, 48, B.a5, 16, 0, 0}; break;
+	default: io = {B.a5, 16, B.d1, 16, 0, 0}; break;
+	}
+}
+
+// gradient tensors mirroring layerIO: gradient of the layer output and of its input
+static void layerGrad(UnetBuffers &B, int i, bf16 *&gOut, bf16 *&gIn)
+{
+	switch (i) {
+	case 0: gOut = B.g_a1; gIn = nullptr; break;
+	case 1: gOut = B.g_c1; gIn = B.g_a1; break;
+	case 2: gOut = B.g_a2; gIn = B.g_p1; break;
+	case 3: gOut = B.g_c2; gIn = B.g_a2; break;
+	case 4: gOut = B.g_a3; gIn = B.g_p2; break;
+	case 5: gOut = B.g_b3; gIn = B.g_a3; break;
+	case 6: gOut = B.g_a4; gIn = B.g_c2; break;
+	case 7: gOut = B.g_d2; gIn = B.g_a4; break;
+	case 8: gOut = B.g_a5; gIn = B.g_c1; break;
+	default: gOut = B.g_d1; gIn = B.g_a5; break;
+	}
+}
+
+void unetForward(Unet &U, hipStream_t s)
+{
+	UnetBuffers &B = U.buf;
+	auto run = [&](int i) {
+		LayerIO io;
+		layerIO(B, i, io);
+		const UnetLayer &L = U.L[i];
+		conv(B, L, io.level, io.in, io.inStride, L.cin, L.wf, roundUp16(L.cin), roundUp16(L.cout), L.b, io.out, io.outStride, io.outOff, L.cout, true, s);
+	};
+	const int *n0 = B.n[0], *n1 = B.n[1];
+	run(0);
+	run(1);
+	hipLaunchKernelGGL(kPool, dim3(blocksFor(B.vox[1] * 16)), dim3(kUB), 0, s, (const bf16 *)B.c1, 48, 32, 16, B.p1, 16, n0[0], n0[1], n0[2]);
+	run(2);
+	run(3);
+	hipLaunchKernelGGL(kPool, dim3(blocksFor(B.vox[2] * 32)), dim3(kUB), 0, s, (const bf16 *)B.c2, 96, 64, 32, B.p2, 32, n1[0], n1[1], n1[2]);
+	run(4);
+	run(5);
+	hipLaunchKernelGGL(kUp, dim3(blocksFor(B.vox[1] * 64)), dim3(kUB), 0, s, (const bf16 *)B.b3, 64, 64, B.c2, 96, n1[0], n1[1], n1[2]);
+	run(6);
+	run(7);
+	hipLaunchKernelGGL(kUp, dim3(blocksFor(B.vox[0] * 32)), dim3(kUB), 0, s, (const bf16 *)B.d2, 32, 32, B.c1, 48, n0[0], n0[1], n0[2]);
+	run(8);
+	run(9);
+	hipLaunchKernelGGL(kOut, dim3(blocksFor(B.vox[0])), dim3(kUB), 0, s, (const bf16 *)B.d1, (const bf16 *)B.in, (const float *)U.ow,
+		(const float *)U.ob, B.y, B.vox[0]);
+}
+
+// exact size of the weight-gradient partials for the current dimensions
+static size_t partialNeed(const UnetBuffers &B)
+{
+	size_t need = 0;
+	for (int i = 0; i < kUnetLayers; i++) {
+		const int *n = B.n[kLevel[i]];
+		const size_t tiles = size_t((n[0] + TX - 1) / TX) * ((n[1] + TY - 1) / TY) * ((n[2] + TZ - 1) / TZ);
+		need = std::max(need, tiles * (27 * roundUp16(kCin[i]) * roundUp16(kCout[i]) + roundUp16(kCout[i])));
+	}
+	return need;
+}
+
+bool unetBackward(Unet &U, hipStream_t s, std::string &err)
+{
+	UnetBuffers &B = U.buf;
+	if (!B.has_grad) { err = "network buffers were reserved without gradients"; return false; }
+	const size_t need = partialNeed(B);
+	if (need > B.partial_cap) {
+		(void)hipStreamSynchronize(s);
+		auto it = std::find(B.owned.begin(), B.owned.end(), static_cast<void *>(B.partial));
+		if (it != B.owned.end()) { (void)hipFree(*it); B.owned.erase(it); }
+		void *q = nullptr;
+		if (hipMalloc(&q, need * sizeof(float)) != hipSuccess) { err = "gradient partial allocation failed"; B.partial = nullptr; B.partial_cap = 0; return false; }
+		B.owned.push_back(q);
+		B.partial = static_cast<float *>(q);
+		B.partial_cap = need;
+	}
+	const int *n0 = B.n[0], *n1 = B.n[1];
+	// output layer
+	const int nbOut = int(std::min<size_t>((B.vox[0] + kUB - 1) / kUB, 1024));
+	hipLaunchKernelGGL(kOutBack, dim3(nbOut), dim3(kUB), 0, s, (const float *)B.gy, (const bf16 *)B.d1, (const bf16 *)B.in,
+		(const float *)U.ow, B.g_d1, B.red, B.vox[0]);
+	hipLaunchKernelGGL(kOutBackReduce, dim3(1), dim3(32), 0, s, (const float *)B.red, nbOut, U.gow, U.gob);
+
+	auto back = [&](int i) {
+		const UnetLayer &L = U.L[i];
+		LayerIO io;
+		layerIO(B, i, io);
+		bf16 *gOut, *gIn;
+		layerGrad(B, i, gOut, gIn);
+		const int lv = io.level;
+		// dL/dz = dL/da * (a > 0), in place
+		hipLaunchKernelGGL(kReluBack, dim3(blocksFor(B.vox[lv] * L.cout)), dim3(kUB), 0, s, gOut, (const bf16 *)io.out, io.outStride, io.outOff,
+			L.cout, B.vox[lv]);
+		// weight and bias gradients
+		WGradArgs W{};
+		W.x = io.in; W.xStride = io.inStride; W.Cin = L.cin;
+		W.gz = gOut; W.gStride = io.outStride; W.gOff = io.outOff; W.Cout = L.cout;
+		W.cinP = roundUp16(L.cin); W.coutP = roundUp16(L.cout);
+		W.nx = B.n[lv][0]; W.ny = B.n[lv][1]; W.nz = B.n[lv][2];
+		const int tx = (W.nx + TX - 1) / TX, ty = (W.ny + TY - 1) / TY, tz = (W.nz + TZ - 1) / TZ;
+		const int nblocks = tx * ty * tz;
+		W.partial = B.partial;
+		W.partialB = B.partial + size_t(nblocks) * 27 * W.cinP * W.coutP;
+		const int groups = (W.coutP + 31) / 32;
+		hipLaunchKernelGGL(kConv3WGrad, dim3(tx, ty, tz * groups), dim3(kUB), 0, s, W);
+		hipLaunchKernelGGL(kReduceWGrad, dim3(blocksFor(size_t(27) * L.cin * L.cout + L.cout)), dim3(kUB), 0, s, (const float *)W.partial,
+			(const float *)W.partialB, nblocks, L.cin, L.cout, W.cinP, W.coutP, L.gw, L.gb);
+		// input gradient: the transposed convolution (mirrored taps, swapped channels)
+		if (gIn) {
+			const bf16 *gz = gOut + io.outOff;
+			conv(B, L, lv, gz, io.outStride, L.cout, L.wd, roundUp16(L.cout), roundUp16(L.cin), nullptr, gIn, io.inStride, 0, L.cin, false, s);
+		}
+	};
+	back(9);
+	back(8);
+	hipLaunchKernelGGL(kUpBack, dim3(blocksFor(B.vox[1] * 32)), dim3(kUB), 0, s, (const bf16 *)B.g_c1, 48, 32, B.g_d2, 32, n0[0], n0[1], n0[2]);
+	back(7);
+	back(6);
+	hipLaunchKernelGGL(kUpBack, dim3(blocksFor(B.vox[2] * 64)), dim3(kUB), 0, s, (const bf16 *)B.g_c2, 96, 64, B.g_b3, 64, n1[0], n1[1], n1[2]);
+	back(5);
+	back(4);
+	hipLaunchKernelGGL(kPoolBack, dim3(blocksFor(B.vox[1] * 32)), dim3(kUB), 0, s, (const bf16 *)B.g_p2, 32, 32, B.g_c2, 96, 64, n1[0], n1[1], n1[2]);
+	back(3);
+	back(2);
+	hipLaunchKernelGGL(kPoolBack, dim3(blocksFor(B.vox[0] * 16)), dim3(kUB), 0, s, (const bf16 *)B.g_p1, 16, 16, B.g_c1, 48, 32, n0[0], n0[1], n0[2]);
+	back(1);
+	back(0);
+	return true;
+}
+
+void unetAdam(Unet &U, float lr, float b1, float b2, float eps, hipStream_t s)
+{
+	U.adam_t++;
+	const float c1 = 1 - std::pow(b1, float(U.adam_t)), c2 = 1 - std::pow(b2, float(U.adam_t));
+	for (int i = 0; i < kUnetLayers; i++) {
+		UnetLayer &L = U.L[i];
+		const int nw = 27 * L.cin * L.cout;
+		hipLaunchKernelGGL(kAdam, dim3(blocksFor(nw)), dim3(kUB), 0, s, L.w, (const float *)L.gw, L.mw, L.vw, nw, lr, b1, b2, eps, c1, c2);
+		hipLaunchKernelGGL(kAdam, dim3(1), dim3(kUB), 0, s, L.b, (const float *)L.gb, L.mb, L.vb, L.cout, lr, b1, b2, eps, c1, c2);
+	}
+	hipLaunchKernelGGL(kAdam, dim3(1), dim3(kUB), 0, s, U.ow, (const float *)U.gow, U.mow, U.vow, 16, lr, b1, b2, eps, c1, c2);
+	hipLaunchKernelGGL(kAdam, dim3(1), dim3(kUB), 0, s, U.ob, (const float *)U.gob, U.mob, U.vob, 1, lr, b1, b2, eps, c1, c2);
+	unetPackWeights(U, s);
+}
+
+void unetLoadInput(Unet &U, const float *r, const uint8_t *mask, hipStream_t s)
+{
+	hipLaunchKernelGGL(kLoadInput, dim3(blocksFor(U.buf.vox[0])), dim3(kUB), 0, s, r, mask, U.buf.in, U.buf.vox[0]);
+}
+
+void unetApplyCompact(Unet &U, const UnetBox &B, const int *grid2mat, const double *r, const double *rr, int n, double *z, hipStream_t s)
+{
+	const size_t vox = size_t(B.n[0]) * B.n[1] * B.n[2];
+	hipLaunchKernelGGL(kGatherBox, dim3(blocksFor(vox)), dim3(kUB), 0, s, B, grid2mat, r, rr, n, U.buf.in);
+	unetForward(U, s);
+	hipLaunchKernelGGL(kScatterBox, dim3(blocksFor(vox)), dim3(kUB), 0, s, B, grid2mat, (const float *)U.buf.y, rr, n, z);
+}
+
+} // namespace garm

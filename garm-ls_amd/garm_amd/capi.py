@@ -14,9 +14,10 @@ HEADER_PATH = os.path.join(os.path.dirname(_PKG), "include", "garm_gpu.h")
 
 # garm_field
 PHI, PHI_GRAD, REFMAP, REFMAP_GRAD, PHI_TMP, PHI_ORI, PHI_REF, PHI_REF_GRAD, ADV_REGION, VELOCITY, \
-    CELL_DIST, NODE_DIST, FRACTION, SOLID_VELOCITY, NORMAL, PRESSURE = range(16)
+    CELL_DIST, NODE_DIST, FRACTION, SOLID_VELOCITY, NORMAL, PRESSURE, RHS, UNKNOWNS = range(18)
 FIELD_NAMES = ["phi", "phi_grad", "refmap", "refmap_grad", "phi_tmp", "phi_ori", "phi_ref", "phi_ref_grad",
-               "adv_region", "velocity", "cell_dist", "node_dist", "fraction", "solid_velocity", "normal", "pressure"]
+               "adv_region", "velocity", "cell_dist", "node_dist", "fraction", "solid_velocity", "normal", "pressure",
+               "rhs", "unknowns"]
 
 # garm_kernel (instrumentation)
 K_ADVECT_REFMAP, K_ADVECT_VELOCITY, K_COMPOSE, K_REINIT_STAGE, K_PCG_SPMV, K_PCG_UPDATE, K_PCG32_SPMV, K_PCG32_UPDATE = range(8)
@@ -106,6 +107,17 @@ def lib():
             "garm_max_speed": (i, [vp, dp]),
             "garm_synchronize": (i, [vp]),
             "garm_time_kernel": (i, [vp, i, i, dp, dp]),
+            "garm_unet_create": (i, [ctypes.POINTER(vp), ctypes.c_ulonglong]),
+            "garm_unet_destroy": (None, [vp]),
+            "garm_unet_last_error": (ctypes.c_char_p, [vp]),
+            "garm_unet_create_error": (ctypes.c_char_p, []),
+            "garm_unet_param_count": (sz, []),
+            "garm_unet_get_params": (i, [vp, vp, sz]),
+            "garm_unet_set_params": (i, [vp, vp, sz]),
+            "garm_unet_forward": (i, [vp, ip, vp, vp, vp]),
+            "garm_unet_loss_grad": (i, [vp, ip, vp, vp, vp, dp, vp, sz]),
+            "garm_unet_train": (i, [vp, ip, i, vp, vp, vp, i, ctypes.c_float, ctypes.c_ulonglong, vp]),
+            "garm_attach_preconditioner": (i, [vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -257,3 +269,83 @@ class Liquid:
 
     def synchronize(self):
         self._check(lib().garm_synchronize(self.h))
+
+    def attach_preconditioner(self, net):
+        """Use `net` (a Unet, or None) for PCG_LEARNED solves."""
+        self._check(lib().garm_attach_preconditioner(self.h, net.h if net is not None else None))
+        self._net = net   # keep the network alive while attached
+
+
+class Unet:
+    """The learned pressure preconditioner (garm_unet_* in include/garm_gpu.h).
+
+    Grids are numpy arrays of shape (nz, ny, nx), each size a multiple of 4.
+    """
+
+    def __init__(self, seed=0):
+        L = lib()
+        h = ctypes.c_void_p()
+        rc = L.garm_unet_create(ctypes.byref(h), seed)
+        if rc != GARM_OK:
+            raise GarmError(rc, L.garm_unet_create_error().decode())
+        self.h = h
+
+    @staticmethod
+    def param_count():
+        return lib().garm_unet_param_count()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().garm_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != GARM_OK:
+            raise GarmError(rc, lib().garm_unet_last_error(self.h).decode())
+
+    def get_params(self):
+        a = np.empty(self.param_count(), dtype=np.float32)
+        self._check(lib().garm_unet_get_params(self.h, a.ctypes.data, a.size))
+        return a
+
+    def set_params(self, flat):
+        a = np.ascontiguousarray(flat, dtype=np.float32).ravel()
+        self._check(lib().garm_unet_set_params(self.h, a.ctypes.data, a.size))
+
+    @staticmethod
+    def _dims(shape):
+        nz, ny, nx = shape
+        return (ctypes.c_int * 3)(nx, ny, nz)
+
+    def forward(self, r, mask):
+        r = np.ascontiguousarray(r, dtype=np.float32)
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        y = np.empty_like(r)
+        self._check(lib().garm_unet_forward(self.h, self._dims(r.shape), r.ctypes.data, m.ctypes.data, y.ctypes.data))
+        return y
+
+    def loss_grad(self, r, mask, target):
+        r = np.ascontiguousarray(r, dtype=np.float32)
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        g = np.empty(self.param_count(), dtype=np.float32)
+        loss = ctypes.c_double()
+        self._check(lib().garm_unet_loss_grad(self.h, self._dims(r.shape), r.ctypes.data, m.ctypes.data, t.ctypes.data,
+                                              ctypes.byref(loss), g.ctypes.data, g.size))
+        return loss.value, g
+
+    def train(self, rhs, pressure, mask, steps, lr=1e-3, seed=0):
+        """rhs, pressure, mask: (pairs, nz, ny, nx).  Returns the per-step losses."""
+        r = np.ascontiguousarray(rhs, dtype=np.float32)
+        p = np.ascontiguousarray(pressure, dtype=np.float32)
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        hist = np.zeros(max(steps, 1), dtype=np.float32)
+        self._check(lib().garm_unet_train(self.h, self._dims(r.shape[1:]), r.shape[0], r.ctypes.data, p.ctypes.data,
+                                          m.ctypes.data, steps, lr, seed, hist.ctypes.data))
+        return hist[:steps]

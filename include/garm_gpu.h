@@ -32,6 +32,7 @@ extern "C" {
 #endif
 
 typedef struct garm_ctx garm_ctx;
+typedef struct garm_unet garm_unet;
 
 typedef enum garm_status {
 	GARM_OK = 0,
@@ -61,7 +62,11 @@ typedef enum garm_field {
 	GARM_SOLID_VELOCITY = 13,/* _boundary.velocity, faces                      */
 	GARM_NORMAL = 14,        /* _boundary.normal, faces                        */
 	GARM_PRESSURE = 15,      /* Projector::_rdP scattered to cells (0 elsewhere), 1/cell */
-	GARM_NUM_FIELDS = 16
+	GARM_RHS = 16,           /* Projector::_rhs of the last solve scattered to cells
+	                            (0 elsewhere), 1/cell, download only */
+	GARM_UNKNOWNS = 17,      /* 1 where the last solve had an unknown, else 0, 1/cell,
+	                            download only (Projector::setUnknowns) */
+	GARM_NUM_FIELDS = 18
 } garm_field;
 
 /* Pressure solver selection for garm_project. */
@@ -70,8 +75,10 @@ typedef enum garm_pcg_mode {
 	GARM_PCG_MIXED = 1,      /* FP32 Jacobi-PCG refined in FP64: solution accumulated in
 	                            FP64, residual replaced by the FP64 b - Ax at every 1e3
 	                            reduction, FP64 reductions, FP64 convergence test */
-	GARM_PCG_LEARNED = 2     /* flexible PCG preconditioned by the learned U-Net,
-	                            falling back to GARM_PCG_FP64 when the residual stalls */
+	GARM_PCG_LEARNED = 2     /* flexible CG preconditioned by the U-Net attached with
+	                            garm_attach_preconditioner, falling back to
+	                            GARM_PCG_MIXED on the remaining residual when the
+	                            residual stalls; judged by the FP64 b - Ax */
 } garm_pcg_mode;
 
 typedef struct garm_pcg_opts {
@@ -197,6 +204,39 @@ typedef enum garm_kernel {
  * touch read or written once).  The state is left as a normal step would
  * leave it.  The PCG kernels need a preceding garm_project / garm_step. */
 int garm_time_kernel(garm_ctx *ctx, int which, int reps, double *avg_ms, double *bytes_per_launch);
+
+/* ---- learned preconditioner (no counterpart in the reference) ----------
+ *
+ * A three-level 3D U-Net (16/32/64 channels, 3x3x3 convolutions, BF16
+ * weights and activations on MFMA, FP32 accumulation, FP32 master weights)
+ * mapping a residual r, scaled to unit RMS over its unknowns, to an
+ * approximate solution of A z = r.  Network grids are nx*ny*nz voxels,
+ * x-fastest, each size a multiple of 4 in [4, 1024].  The network is bound to
+ * the device current at creation and owns one HIP stream. */
+int garm_unet_create(garm_unet **out, unsigned long long seed);
+void garm_unet_destroy(garm_unet *net);
+const char *garm_unet_last_error(const garm_unet *net);
+const char *garm_unet_create_error(void);
+/* Parameters as FP32, per layer w[27][cin][cout] then b[cout] (layers
+ * L1a L1b L2a L2b L3a L3b L4a L4b L5a L5b), then the 1x1 output w[16], b[1]. */
+size_t garm_unet_param_count(void);
+int garm_unet_get_params(garm_unet *net, float *out, size_t n);
+int garm_unet_set_params(garm_unet *net, const float *in, size_t n);
+/* y = net(r, mask) on one grid (inputs used as given, no scaling). */
+int garm_unet_forward(garm_unet *net, const int dims[3], const float *r, const uint8_t *mask, float *y);
+/* Mean squared error over the mask of net(r, mask) against target and its
+ * gradient with respect to every parameter (layout of garm_unet_get_params). */
+int garm_unet_loss_grad(garm_unet *net, const int dims[3], const float *r, const uint8_t *mask, const float *target,
+	double *loss, float *grad, size_t n);
+/* Adam training on n_pairs (rhs, pressure, mask) grids stored one after the
+ * other: each step draws a pair (seeded), scales it so that rhs has unit RMS
+ * over its mask, and takes one Adam step (beta 0.9 / 0.999) on the masked
+ * squared error.  loss_history (optional) receives `steps` losses. */
+int garm_unet_train(garm_unet *net, const int dims[3], int n_pairs, const float *rhs, const float *pressure, const uint8_t *mask,
+	int steps, float lr, unsigned long long seed, float *loss_history);
+/* Uses `net` for GARM_PCG_LEARNED solves of this context (NULL detaches).
+ * The network must outlive the attachment and live on the same device. */
+int garm_attach_preconditioner(garm_ctx *ctx, garm_unet *net);
 
 #ifdef __cplusplus
 }

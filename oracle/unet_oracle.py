@@ -14,6 +14,11 @@ Architecture (channels-last tensors of shape (nz, ny, nx, C)):
   out  1x1 conv 16->1 (+bias), times mask
 conv3 = 3x3x3 cross-correlation with zero padding; weights W[t][ci][co],
 t = (dz+1)*9 + (dy+1)*3 + (dx+1).
+
+forward() and backward() take an optional rounding function `q` applied
+where the GPU network stores a BF16 tensor (bf16() below): the input, the
+convolution weights, every activation, pooled and gradient tensor.  With the
+default identity they are the exact FP64 network.
 """
 import numpy as np
 
@@ -25,6 +30,18 @@ LAYERS = [  # name, cin, cout
     ("L5a", 48, 16), ("L5b", 16, 16),
 ]
 IN_CHANNELS = 2   # live input channels (residual, mask); the rest are zero
+
+
+def ident(x):
+    return x
+
+
+def bf16(x):
+    """Round to the nearest BF16 value (ties to even), returned as float64."""
+    f = np.ascontiguousarray(x, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
 def he_init(seed):
@@ -98,22 +115,22 @@ def make_input(r, mask):
     return x
 
 
-def forward(params, x, mask, keep=False):
+def forward(params, x, mask, keep=False, q=ident):
     """x: (nz, ny, nx, 4); returns (nz, ny, nx) output and the tape."""
     P = params
     tape = {}
 
     def cr(name, inp):
-        z = conv3(inp, P[name + ".w"], P[name + ".b"])
-        a = np.maximum(z, 0.0)
+        z = conv3(inp, q(P[name + ".w"]), P[name + ".b"])
+        a = q(np.maximum(z, 0.0))
         tape[name] = (inp, z)
         return a
 
-    a = cr("L1a", x)
+    a = cr("L1a", q(x))
     e1 = cr("L1b", a)
-    a = cr("L2a", pool(e1))
+    a = cr("L2a", q(pool(e1)))
     e2 = cr("L2b", a)
-    a = cr("L3a", pool(e2))
+    a = cr("L3a", q(pool(e2)))
     b3 = cr("L3b", a)
     c2 = np.concatenate([up(b3), e2], axis=-1)
     a = cr("L4a", c2)
@@ -126,7 +143,7 @@ def forward(params, x, mask, keep=False):
     return (y, tape) if keep else y
 
 
-def backward(params, tape, mask, gy):
+def backward(params, tape, mask, gy, q=ident):
     """Gradient of sum(gy * y) with respect to every parameter."""
     P = params
     g = {}
@@ -134,41 +151,62 @@ def backward(params, tape, mask, gy):
     gyo = gy * mask
     g["out.w"] = (d1 * gyo[..., None]).sum(axis=(0, 1, 2))
     g["out.b"] = np.array([gyo.sum()])
-    gd1 = gyo[..., None] * P["out.w"][None, None, None, :]
+    gd1 = q(gyo[..., None] * P["out.w"][None, None, None, :])
 
     def cr_b(name, ga):
         inp, z = tape[name]
         gz = ga * (z > 0)
-        gx, gw, gb = conv3_backward(inp, P[name + ".w"], gz)
+        gx, gw, gb = conv3_backward(inp, q(P[name + ".w"]), gz)
         g[name + ".w"] = gw
         g[name + ".b"] = gb
-        return gx
+        return q(gx)
 
     ga = cr_b("L5b", gd1)
     gc1 = cr_b("L5a", ga)
-    gd2 = up_backward(gc1[..., :32])
+    gd2 = q(up_backward(gc1[..., :32]))
     ge1 = gc1[..., 32:]
     ga = cr_b("L4b", gd2)
     gc2 = cr_b("L4a", ga)
-    gb3 = up_backward(gc2[..., :64])
+    gb3 = q(up_backward(gc2[..., :64]))
     ge2 = gc2[..., 64:]
     ga = cr_b("L3b", gb3)
     gp2 = cr_b("L3a", ga)
-    ge2 = ge2 + pool_backward(gp2)
+    ge2 = q(ge2 + q(pool_backward(gp2)))
     ga = cr_b("L2b", ge2)
     gp1 = cr_b("L2a", ga)
-    ge1 = ge1 + pool_backward(gp1)
+    ge1 = q(ge1 + q(pool_backward(gp1)))
     ga = cr_b("L1b", ge1)
     cr_b("L1a", ga)
     g["L1a.w"][:, IN_CHANNELS:, :] = 0.0
     return g
 
 
-def loss_and_grad(params, x, mask, target):
+def loss_and_grad(params, x, mask, target, q=ident):
     """Mean squared error over the masked cells and its parameter gradient."""
-    y, tape = forward(params, x, mask, keep=True)
+    y, tape = forward(params, x, mask, keep=True, q=q)
     m = mask.sum()
     diff = (y - target) * mask
     loss = (diff ** 2).sum() / m
-    g = backward(params, tape, mask, 2.0 * diff / m)
+    g = backward(params, tape, mask, 2.0 * diff / m, q=q)
     return loss, g
+
+
+def flatten(params):
+    """Parameters in the order and layout of garm_unet_get_params."""
+    parts = []
+    for name, _, _ in LAYERS:
+        parts += [params[name + ".w"].ravel(), params[name + ".b"].ravel()]
+    parts += [params["out.w"].ravel(), params["out.b"].ravel()]
+    return np.concatenate(parts)
+
+
+def unflatten(flat):
+    params, off = {}, 0
+    for name, cin, cout in LAYERS:
+        n = 27 * cin * cout
+        params[name + ".w"] = np.asarray(flat[off:off + n], dtype=np.float64).reshape(27, cin, cout)
+        params[name + ".b"] = np.asarray(flat[off + n:off + n + cout], dtype=np.float64)
+        off += n + cout
+    params["out.w"] = np.asarray(flat[off:off + 16], dtype=np.float64)
+    params["out.b"] = np.asarray(flat[off + 16:off + 17], dtype=np.float64)
+    return params
