@@ -484,6 +484,19 @@ int garm_project(garm_ctx *c, const garm_pcg_opts *o, garm_pcg_stats *st)
 	return doProject(c, *o, st);
 }
 
+int garm_apply_pressure_system(garm_ctx *c, const double *x, double *b)
+{
+	if (!c || !x || !b) return GARM_EINVAL;
+	Fields &F = c->F;
+	if (!c->P.rows.diag) return fail(c, GARM_EINVAL, "no pressure system has been assembled yet");
+	int rc;
+	if ((rc = hipCheck(c, hipStreamSynchronize(c->s), "synchronize"))) return rc;
+	if ((rc = hipCheck(c, hipMemcpy(F.scratch[0], x, F.N * sizeof(double), hipMemcpyHostToDevice), "upload"))) return rc;
+	applySystemCells(F, c->P, F.scratch[0], F.scratch[1], c->s);
+	if ((rc = launched(c, "pressure operator")) || (rc = hipCheck(c, hipStreamSynchronize(c->s), "synchronize"))) return rc;
+	return hipCheck(c, hipMemcpy(b, F.scratch[1], F.N * sizeof(double), hipMemcpyDeviceToHost), "download");
+}
+
 int garm_max_speed(garm_ctx *c, double *out)
 {
 	if (!c || !out) return GARM_EINVAL;

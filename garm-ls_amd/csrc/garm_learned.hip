@@ -13,10 +13,10 @@
 // (two cells of context, sizes padded to multiples of four) to its input
 // scaled to unit RMS, the output scaled back.  The smoother removes the rough
 // error components a network trained on pressures does not resolve.  When the
-// residual stalls (less than a halving over eight iterations) or the
-// iteration budget runs out, the solve falls back to the mixed-precision
-// Jacobi PCG on the current residual; the FP64 residual b - A x judges the
-// result either way.
+// residual stalls (an application that does not halve it) or the iteration
+// budget runs out, the solve falls back to the mixed-precision Jacobi PCG on
+// the current residual; the FP64 residual b - A x judges the result either
+// way.
 #include "garm_learned.hpp"
 
 #include <algorithm>
@@ -70,6 +70,8 @@ __device__ __forceinline__ void putPart(double (&v)[NV], double *partials)
 enum { kLsPq = 0, kLsRr = 1, kLsBb = 2, kLsTrue = 3, kLsSmooth = 4 };
 
 constexpr double kOmega = 2.0 / 3.0;
+// each network application must reduce the residual norm at least this much
+constexpr double kMinReduction = 0.5;
 
 // z = w D^-1 r
 __global__ void kJacobi(const double *invdiag, int n, const double *r, double *z)
@@ -286,6 +288,7 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 	st->iterations = 0;
 	st->outer_iterations = 0;
 	st->fell_back = 0;
+	st->preconditioner_applications = 0;
 	if (bb == 0) { st->relative_residual = 0; return GARM_OK; }
 	const double tol = std::max(o.tolerance, 2.220446049250313e-16 / std::sqrt(bb));
 	const double thr = tol * tol * bb;
@@ -300,10 +303,8 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 	};
 	precondition();
 	hipLaunchKernelGGL(kCopyD, g1, b1, 0, s, W.p, (const double *)W.z, n);
-	std::vector<double> hist;
-	hist.push_back(std::sqrt(bb));
+	double rrPrev = bb;
 	const int budget = std::min(o.max_iterations, 400);
-	bool converged = false, stalled = false;
 	int it = 0;
 	for (; it < budget; it++) {
 		hipLaunchKernelGGL(kFcgSpmv<D>, g1, b1, 0, s, A64, n, (const double *)W.p, W.q, (const double *)W.r, W.pa);
@@ -311,17 +312,17 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 		hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nb, W.ls + kLsRr);
 		if ((rc = readHost(W, W.ls, 4, s, err))) return rc;
 		const double rr = W.host[kLsRr];
-		hist.push_back(std::sqrt(rr));
-		if (!std::isfinite(rr)) { stalled = true; break; }
-		if (rr < thr) { converged = true; it++; break; }
-		if (hist.size() > 9 && hist.back() > 0.5 * hist[hist.size() - 9]) { stalled = true; it++; break; }
+		if (!std::isfinite(rr)) break;
+		if (rr < thr) { it++; break; }
+		// stalled: this application did not halve the residual
+		if (rr > kMinReduction * kMinReduction * rrPrev) { it++; break; }
+		rrPrev = rr;
 		precondition();
 		hipLaunchKernelGGL(kFcgDot, g1, b1, 0, s, n, (const double *)W.z, (const double *)W.q, W.pa);
 		hipLaunchKernelGGL(kFcgP, g1, b1, 0, s, n, W.p, (const double *)W.z, (const double *)W.pa, (const double *)W.ls);
 	}
 	st->iterations = it;
-	(void)converged;
-	(void)stalled;
+	st->preconditioner_applications = it;
 	// the FP64 residual judges the result; fall back to plain PCG on what is left
 	if (!std::isfinite(W.host[kLsRr])) hipLaunchKernelGGL(kZeroD, g1, b1, 0, s, W.x, n);
 	hipLaunchKernelGGL(kTrueResidual<D>, g1, b1, 0, s, A64, n, b, (const double *)W.x, W.b2, W.pb);
@@ -552,12 +553,16 @@ int garm_unet_train(garm_unet *u, const int dims[3], int n_pairs, const float *r
 	std::mt19937_64 rng(seed);
 	const int nb = 1024;
 	for (int k = 0; k < steps; k++) {
-		const int i = int(rng() % uint64_t(n_pairs));
+		const uint64_t draw = rng();
+		const int i = int((draw >> 1) % uint64_t(n_pairs));
+		// the pair or its negative: the preconditioner must follow the sign of
+		// its input rather than predict a typical pressure from the mask
+		const float sc = (draw & 1) ? -scale[i] : scale[i];
 		const size_t off = size_t(i) * vox;
-		hipLaunchKernelGGL(kLoadScaled, dim3(1024), dim3(kBlock), 0, u->s, (const float *)(dr.p + off), (const uint8_t *)(dm.p + off), scale[i], u->U.buf.in, vox);
+		hipLaunchKernelGGL(kLoadScaled, dim3(1024), dim3(kBlock), 0, u->s, (const float *)(dr.p + off), (const uint8_t *)(dm.p + off), sc, u->U.buf.in, vox);
 		unetForward(u->U, u->s);
 		hipLaunchKernelGGL(kLossGrad, dim3(nb), dim3(kBlock), 0, u->s, (const float *)u->U.buf.y, (const float *)(dp.p + off), (const uint8_t *)(dm.p + off),
-			scale[i], float(invCount[i]), u->U.buf.gy, vox, part.p);
+			sc, float(invCount[i]), u->U.buf.gy, vox, part.p);
 		hipLaunchKernelGGL(kLossReduce, dim3(1), dim3(64), 0, u->s, (const float *)part.p, nb, float(invCount[i]), dl.p + k);
 		if (!unetBackward(u->U, u->s, u->err)) return GARM_ENOMEM;
 		unetAdam(u->U, lr, 0.9f, 0.999f, 1e-8f, u->s);

@@ -126,6 +126,21 @@ __global__ __launch_bounds__(kBlock) void kApplyPressureGradient(GridDev fg, Gri
 	}
 }
 
+// b = A x on cell fields for the unknowns of the last assembly (0 elsewhere)
+template <int D>
+__global__ __launch_bounds__(kBlock) void kApplySystemCells(Rows<double> A, int n, const int *mat2grid, const double *x, double *b)
+{
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+		double s = A.diag[i] * x[mat2grid[i]];
+#pragma unroll
+		for (int o = 0; o < 2 * D; o++) {
+			const int c = A.nb[o][i];
+			if (c >= 0) s += -A.w[o][i] * x[mat2grid[c]];
+		}
+		b[mat2grid[i]] = s;
+	}
+}
+
 // x == nullptr writes the unknown indicator instead
 __global__ void kScatterCompact(const int *grid2mat, const double *x, double *out, size_t n)
 {
@@ -241,6 +256,7 @@ static int projectD(Fields &F, Projection &P, const garm_pcg_opts &o, garm_pcg_s
 	st->iterations = 0;
 	st->outer_iterations = 0;
 	st->fell_back = 0;
+	st->preconditioner_applications = 0;
 	st->relative_residual = 0;
 	int rc = GARM_OK;
 	if (n > 0) {
@@ -265,6 +281,15 @@ static int projectD(Fields &F, Projection &P, const garm_pcg_opts &o, garm_pcg_s
 int runProjection(Fields &F, Projection &P, const garm_pcg_opts &o, garm_pcg_stats *st, hipStream_t s, std::string &err)
 {
 	return F.D == 2 ? projectD<2>(F, P, o, st, s, err) : projectD<3>(F, P, o, st, s, err);
+}
+
+void applySystemCells(const Fields &F, const Projection &P, const double *x, double *b, hipStream_t s)
+{
+	launchFill(b, 0.0, F.N, s);
+	if (!P.grid2mat || P.n == 0) return;
+	const dim3 g(std::min(8192, (P.n + kBlock - 1) / kBlock));
+	if (F.D == 2) hipLaunchKernelGGL(kApplySystemCells<2>, g, dim3(kBlock), 0, s, P.rows, P.n, (const int *)P.mat2grid, x, b);
+	else hipLaunchKernelGGL(kApplySystemCells<3>, g, dim3(kBlock), 0, s, P.rows, P.n, (const int *)P.mat2grid, x, b);
 }
 
 void scatterCompact(const Fields &F, const Projection &P, int which, double *out, hipStream_t s)

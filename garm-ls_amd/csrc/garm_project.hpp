@@ -39,5 +39,7 @@ int runProjection(Fields &F, Projection &P, const garm_pcg_opts &o, garm_pcg_sta
 // What scatterCompact writes to every cell (0 where there is no unknown).
 enum CompactField { kCompactPressure = 0, kCompactRhs = 1, kCompactIndicator = 2 };
 void scatterCompact(const Fields &F, const Projection &P, int which, double *out, hipStream_t s);
+// b = A x over whole cell fields with the operator of the last assembly
+void applySystemCells(const Fields &F, const Projection &P, const double *x, double *b, hipStream_t s);
 
 } // namespace garm

@@ -40,7 +40,8 @@ class PcgOpts(ctypes.Structure):
 
 class PcgStats(ctypes.Structure):
     _fields_ = [("unknowns", ctypes.c_int), ("iterations", ctypes.c_int), ("outer_iterations", ctypes.c_int),
-                ("fell_back", ctypes.c_int), ("relative_residual", ctypes.c_double)]
+                ("fell_back", ctypes.c_int), ("relative_residual", ctypes.c_double),
+                ("preconditioner_applications", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -105,6 +106,7 @@ def lib():
             "garm_project": (i, [vp, ctypes.POINTER(PcgOpts), ctypes.POINTER(PcgStats)]),
             "garm_step": (i, [vp, d, i, ctypes.POINTER(StepOpts), ctypes.POINTER(StepStats)]),
             "garm_max_speed": (i, [vp, dp]),
+            "garm_apply_pressure_system": (i, [vp, vp, vp]),
             "garm_synchronize": (i, [vp]),
             "garm_time_kernel": (i, [vp, i, i, dp, dp]),
             "garm_unet_create": (i, [ctypes.POINTER(vp), ctypes.c_ulonglong]),
@@ -255,6 +257,15 @@ class Liquid:
         st = StepStats()
         self._check(lib().garm_step(self.h, dt, step_index, ctypes.byref(opts), ctypes.byref(st)))
         return st.as_dict()
+
+    def apply_pressure_system(self, x):
+        """b = A x with the operator of the last projection (cell fields)."""
+        xx = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        b = np.empty_like(xx)
+        if xx.size != self.count(PHI):
+            raise ValueError("x must hold one value per cell")
+        self._check(lib().garm_apply_pressure_system(self.h, xx.ctypes.data, b.ctypes.data))
+        return b
 
     def max_speed(self):
         v = ctypes.c_double()

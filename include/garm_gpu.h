@@ -95,6 +95,8 @@ typedef struct garm_pcg_stats {
 	int outer_iterations;    /* FP64 residual replacements (mixed) */
 	int fell_back;           /* 1 if the learned solve fell back to plain PCG */
 	double relative_residual;/* final ||b-Ax||/||b|| computed in FP64 */
+	int preconditioner_applications; /* learned: flexible-CG iterations with the network
+	                            (included in iterations) */
 } garm_pcg_stats;
 
 /* Precision of advection and reinitialisation. */
@@ -177,6 +179,11 @@ int garm_project(garm_ctx *ctx, const garm_pcg_opts *opts, garm_pcg_stats *stats
 /* LevelSetLiquid::advance (LevelSetLiquid.cpp:608-698); step_index selects
  * the reinitialisation cadence (reinit when step_index % reinit_interval == 0). */
 int garm_step(garm_ctx *ctx, double dt, int step_index, const garm_step_opts *opts, garm_step_stats *stats);
+/* b = A x with the pressure operator of the last garm_project / garm_step
+ * (Projector::buildLinearSystem's matrix, Projector.cpp:30-68) applied to a
+ * cell field x (N doubles, values at non-unknown cells ignored); b is 0 at
+ * non-unknown cells.  Used to make (A x, x) training pairs. */
+int garm_apply_pressure_system(garm_ctx *ctx, const double *x, double *b);
 /* SGridBasedData::normMax of the velocity (SGridBasedData.h:48-52), used by
  * getTimeStep (LevelSetLiquid.h:129). */
 int garm_max_speed(garm_ctx *ctx, double *out);
@@ -229,9 +236,10 @@ int garm_unet_forward(garm_unet *net, const int dims[3], const float *r, const u
 int garm_unet_loss_grad(garm_unet *net, const int dims[3], const float *r, const uint8_t *mask, const float *target,
 	double *loss, float *grad, size_t n);
 /* Adam training on n_pairs (rhs, pressure, mask) grids stored one after the
- * other: each step draws a pair (seeded), scales it so that rhs has unit RMS
- * over its mask, and takes one Adam step (beta 0.9 / 0.999) on the masked
- * squared error.  loss_history (optional) receives `steps` losses. */
+ * other: each step draws a pair and a sign (seeded), scales the pair by the
+ * sign and by the factor that gives rhs unit RMS over its mask, and takes one
+ * Adam step (beta 0.9 / 0.999) on the masked squared error.  loss_history
+ * (optional) receives `steps` losses. */
 int garm_unet_train(garm_unet *net, const int dims[3], int n_pairs, const float *rhs, const float *pressure, const uint8_t *mask,
 	int steps, float lr, unsigned long long seed, float *loss_history);
 /* Uses `net` for GARM_PCG_LEARNED solves of this context (NULL detaches).

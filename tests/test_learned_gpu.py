@@ -1,0 +1,129 @@
+"""Learned preconditioner inside the projection: training on the simulator's
+own (divergence, pressure) pairs, the flexible solve with its fallback and
+the FP64 residual as judge, and what the trained network gains.
+
+The gain is measured on one application, with the public operator
+(garm_apply_pressure_system) and the FP64 solution p of the same system: a
+step x = alpha z along a direction z, with the alpha that conjugate gradients
+would take, leaves the A-norm error
+    |p - alpha z|_A / |p|_A = sqrt(1 - (b.z)^2 / ((z.Az)(b.p))).
+The error removed by the trained network's direction, 1 - that ratio, must
+be at least ten times what the untrained network's and the plain residual
+direction z = b remove.
+"""
+import numpy as np
+import pytest
+
+from garm_amd import capi, scenes, training
+from garm_amd.driver import FrameClock
+from helpers import rel_linf, say
+
+pytestmark = pytest.mark.gpu
+
+N = 48
+STATE = [capi.PHI, capi.PHI_GRAD, capi.REFMAP, capi.REFMAP_GRAD, capi.PHI_TMP, capi.PHI_ORI,
+         capi.PHI_REF, capi.PHI_REF_GRAD, capi.ADV_REGION, capi.VELOCITY]
+
+
+@pytest.fixture(scope="module")
+def trained():
+    g = scenes.grid(N, 3)
+    sim = capi.Liquid(3, g["res"], g["dx"], g["origin"], g["bw"])
+    scenes.dam_break(sim, g)
+    sim.initialize(capi.step_opts(mode=capi.PCG_FP64, tol=1e-8))
+    R, P, M = training.collect_pairs(sim, g, 16, tol=1e-10)
+    say(f"learned: {len(R)} pairs at {N}^3")
+    net = capi.Unet(seed=3)
+    hist = net.train(R, P, M, steps=1500, lr=1e-3, seed=7)
+    k = len(hist) // 10
+    say(f"  training loss {hist[:k].mean():.3e} -> {hist[-k:].mean():.3e}")
+    assert hist[-k:].mean() < 0.3 * hist[:k].mean()
+    # a held-out system: the projection of the next step, set up operator by operator
+    clk = FrameClock(g["dx"], 1.0)
+    opts = capi.step_opts(mode=capi.PCG_FP64, tol=1e-10, max_iterations=20000)
+    for s in range(16, 19):
+        sim.step(clk.next_dt(sim.max_speed()), s, opts)
+    state = {f: sim.get(f) for f in STATE}
+    dt = clk.next_dt(sim.max_speed())
+    return g, net, state, dt
+
+
+def _prepared(g, state, dt):
+    d = capi.Liquid(3, g["res"], g["dx"], g["origin"], g["bw"])
+    for f in STATE:
+        d.set(f, state[f])
+    d.advect_refmap(dt)
+    d.advect_velocity(dt)
+    d.compose()
+    d.extrapolate_collider()
+    d.gravity(dt)
+    return d
+
+
+def _a_norm_error_after_step(d, b, p, z):
+    az = d.apply_pressure_system(z)
+    return np.sqrt(max(0.0, 1.0 - np.dot(b, z) ** 2 / (np.dot(z, az) * np.dot(b, p))))
+
+
+def test_learned_solve_is_judged_and_matches_fp64(trained):
+    g, net, state, dt = trained
+    ref = _prepared(g, state, dt)
+    st64 = ref.project(mode=capi.PCG_FP64, tol=1e-12, max_iterations=20000)
+    d = _prepared(g, state, dt)
+    d.attach_preconditioner(net)
+    st = d.project(mode=capi.PCG_LEARNED, tol=1e-6, max_iterations=5000)
+    say(f"  learned: {st['iterations']} iterations ({st['preconditioner_applications']} with the network), "
+        f"fell back {st['fell_back']}, rel residual {st['relative_residual']:.2e}; fp64 {st64['iterations']}")
+    assert st["rc"] == 0 and st["relative_residual"] <= 1e-6
+    assert st["preconditioner_applications"] >= 1
+    # a 1e-6 solve moves the velocity by far less than 1e-4 of its range
+    assert rel_linf(d.get(capi.VELOCITY), ref.get(capi.VELOCITY)) <= 1e-4
+    assert rel_linf(d.get(capi.PRESSURE), ref.get(capi.PRESSURE)) <= 1e-4
+
+
+def test_useless_network_falls_back_and_still_converges(trained):
+    g, _, state, dt = trained
+    zero = capi.Unet(seed=0)
+    zero.set_params(np.zeros(capi.Unet.param_count(), np.float32))
+    d = _prepared(g, state, dt)
+    d.attach_preconditioner(zero)
+    st = d.project(mode=capi.PCG_LEARNED, tol=1e-6, max_iterations=5000)
+    say(f"  zero network: {st['iterations']} iterations, fell back {st['fell_back']}, rel residual {st['relative_residual']:.2e}")
+    assert st["fell_back"] == 1
+    assert st["rc"] == 0 and st["relative_residual"] <= 1e-6
+
+
+def test_learned_mode_without_a_network_is_an_error(trained):
+    g, _, state, dt = trained
+    d = _prepared(g, state, dt)
+    with pytest.raises(capi.GarmError):
+        d.project(mode=capi.PCG_LEARNED, tol=1e-6)
+
+
+def test_trained_network_gains_over_untrained_and_plain_direction(trained):
+    g, net, state, dt = trained
+    d = _prepared(g, state, dt)
+    d.project(mode=capi.PCG_FP64, tol=1e-12, max_iterations=20000)   # assembles and solves the system
+    b = d.get(capi.RHS)
+    p = d.get(capi.PRESSURE)
+    unknowns = d.get(capi.UNKNOWNS)
+    res = g["res"]
+    shape = training.network_shape(res)
+    nz, ny, nx = res[2], res[1], res[0]
+
+    def direction(network):
+        bg = np.zeros(shape, np.float32)
+        mg = np.zeros(shape, np.uint8)
+        bg[:nz, :ny, :nx] = b.reshape(nz, ny, nx)
+        mg[:nz, :ny, :nx] = unknowns.reshape(nz, ny, nx)
+        scale = np.sqrt(unknowns.sum() / np.sum(b ** 2))
+        y = network.forward(bg * scale, mg)[:nz, :ny, :nx].ravel().astype(np.float64) / scale
+        return y * unknowns
+
+    untrained = capi.Unet(seed=3)
+    rho_trained = _a_norm_error_after_step(d, b, p, direction(net))
+    rho_untrained = _a_norm_error_after_step(d, b, p, direction(untrained))
+    rho_plain = _a_norm_error_after_step(d, b, p, b)
+    say(f"  A-norm error after one step: trained {rho_trained:.3f}, untrained {rho_untrained:.3f}, z = b {rho_plain:.3f}")
+    assert rho_trained <= 0.8
+    assert 1.0 - rho_trained >= 10.0 * max(1.0 - rho_untrained, 1.0 - rho_plain)
