@@ -8,32 +8,34 @@
 // u, xi and J once from HBM and writes xi', J' once.
 #include "garm_internal.hpp"
 
+#include <type_traits>
+
 namespace garm {
 
 // ---------------------------------------------------------------- velocity
 // VectorFieldWrapper<SGrid, Linear>::operator() (VectorField.h:66-72):
 // per axis, the multilinear sample of u_a on face grid a (LagrangeIntrpl.h:40-72).
-template <int D>
-__device__ __forceinline__ void sampleVel(const FaceSet &U, const double *p, double *v)
+template <int D, typename T>
+__device__ __forceinline__ void sampleVel(const FaceSet &U, const T *p, T *v)
 {
 #pragma unroll
 	for (int a = 0; a < D; a++) {
 		const GridDev &g = U.g[a];
 		const double *ua = U.a[a];
-		int l[3]; double f[3];
-		lowerFrac<D>(g, p, l, f);
-		double w[2][3];
+		int l[3]; T f[3];
+		lowerFracT<D, T>(g, p, l, f);
+		T w[2][3];
 #pragma unroll
 		for (int b = 0; b < D; b++) { w[0][b] = 1 - f[b]; w[1][b] = f[b]; }
-		double val = 0;
+		T val = 0;
 #pragma unroll
 		for (int k = 0; k < (D == 3 ? 2 : 1); k++)
 #pragma unroll
 			for (int j = 0; j < 2; j++)
 #pragma unroll
 				for (int i = 0; i < 2; i++) {
-					const double wt = D == 2 ? w[i][0] * w[j][1] : w[i][0] * w[j][1] * w[k][2];
-					val += ua[gat<D>(g, l[0] + i, l[1] + j, D == 3 ? l[2] + k : 0)] * wt;
+					const T wt = D == 2 ? w[i][0] * w[j][1] : w[i][0] * w[j][1] * w[k][2];
+					val += T(ua[gat<D>(g, l[0] + i, l[1] + j, D == 3 ? l[2] + k : 0)]) * wt;
 				}
 		v[a] = val;
 	}
@@ -42,33 +44,35 @@ __device__ __forceinline__ void sampleVel(const FaceSet &U, const double *p, dou
 // DrvVectorFieldWrapper<SGrid, Linear> (VectorField.h:106-138) with the
 // face-gradient of Derivatives::computeFirst<2> (Derivatives.h:47,51-57,98-105)
 // evaluated at each clamped corner: row a of G is grad u_a.  G row-major.
-template <int D>
-__device__ __forceinline__ void sampleVelGrad(const FaceSet &U, const double *p, double *v, double *G)
+template <int D, typename T>
+__device__ __forceinline__ void sampleVelGrad(const FaceSet &U, const T *p, T *v, T *G)
 {
 #pragma unroll
 	for (int a = 0; a < D; a++) {
 		const GridDev &g = U.g[a];
 		const double *ua = U.a[a];
-		int l[3]; double f[3];
-		lowerFrac<D>(g, p, l, f);
-		double w[2][3];
+		int l[3]; T f[3];
+		lowerFracT<D, T>(g, p, l, f);
+		T w[2][3];
 #pragma unroll
 		for (int b = 0; b < D; b++) { w[0][b] = 1 - f[b]; w[1][b] = f[b]; }
-		double val = 0, gr[3] = {0, 0, 0};
+		const T inv = T(g.inv);
+		T val = 0, gr[3] = {0, 0, 0};
 #pragma unroll
 		for (int k = 0; k < (D == 3 ? 2 : 1); k++)
 #pragma unroll
 			for (int j = 0; j < 2; j++)
 #pragma unroll
 				for (int i = 0; i < 2; i++) {
-					const double wt = D == 2 ? w[i][0] * w[j][1] : w[i][0] * w[j][1] * w[k][2];
+					const T wt = D == 2 ? w[i][0] * w[j][1] : w[i][0] * w[j][1] * w[k][2];
 					const int qi = clampi(l[0] + i, g.n[0] - 1), qj = clampi(l[1] + j, g.n[1] - 1);
 					const int qk = D == 3 ? clampi(l[2] + k, g.n[2] - 1) : 0;
-					val += ua[gidx<D>(g, qi, qj, qk)] * wt;
-					gr[0] += ((ua[gat<D>(g, qi + 1, qj, qk)] - ua[gat<D>(g, qi - 1, qj, qk)]) / 2 * g.inv) * wt;
-					gr[1] += ((ua[gat<D>(g, qi, qj + 1, qk)] - ua[gat<D>(g, qi, qj - 1, qk)]) / 2 * g.inv) * wt;
+					val += T(ua[gidx<D>(g, qi, qj, qk)]) * wt;
+					// neighbour differences taken on the FP64 data, then rounded
+					gr[0] += (T(ua[gat<D>(g, qi + 1, qj, qk)] - ua[gat<D>(g, qi - 1, qj, qk)]) / 2 * inv) * wt;
+					gr[1] += (T(ua[gat<D>(g, qi, qj + 1, qk)] - ua[gat<D>(g, qi, qj - 1, qk)]) / 2 * inv) * wt;
 					if constexpr (D == 3)
-						gr[2] += ((ua[gat<D>(g, qi, qj, qk + 1)] - ua[gat<D>(g, qi, qj, qk - 1)]) / 2 * g.inv) * wt;
+						gr[2] += (T(ua[gat<D>(g, qi, qj, qk + 1)] - ua[gat<D>(g, qi, qj, qk - 1)]) / 2 * inv) * wt;
 				}
 		v[a] = val;
 #pragma unroll
@@ -79,7 +83,8 @@ __device__ __forceinline__ void sampleVelGrad(const FaceSet &U, const double *p,
 // ----------------------------------------------------------------- Hermite
 // 1-D cubic Hermite basis (HermiteIntrpl.h:89-125) for one axis:
 // B[vtx][ord] value weights, Bg[vtx][ord] their derivatives.
-__device__ __forceinline__ void hermiteBasis(double x, double B[2][2], double Bg[2][2])
+template <typename T>
+__device__ __forceinline__ void hermiteBasis(T x, T B[2][2], T Bg[2][2])
 {
 	B[0][0] = (x + x + 1) * (x - 1) * (x - 1); Bg[0][0] = 6 * x * (x - 1);
 	B[1][0] = x * x * (3 - x - x);             Bg[1][0] = 6 * x * (1 - x);
@@ -91,20 +96,26 @@ __device__ __forceinline__ void hermiteBasis(double x, double B[2][2], double Bg
 // (HermiteIntrpl.h:32-58, getValues :62-87) of the reference map with the
 // mixed derivatives of its constructor (:17-30) formed from J at the clamped
 // neighbours of each clamped corner.  Gh is column-major: Gh[c*D + r].
-template <int D>
+// In float the map is interpolated as the displacement xi - x: the data at a
+// corner become xi - x_corner (x_corner of the unclamped corner) and J - I,
+// taken in FP64 before rounding, and X and I are added back.  Cubic Hermite
+// reproduces the identity map exactly, so this is the same interpolant
+// without the cancellation of positions of size 1 in gradients of size 1/dx.
+template <int D, typename T>
 __device__ __forceinline__ void hermiteRefMap(const GridDev &g, const double *const *xi, const double *const *J,
-	const double *X, double *val, double *Gh)
+	const T *X, T *val, T *Gh)
 {
-	int l[3]; double f[3];
-	lowerFrac<D>(g, X, l, f);
-	double B[3][2][2], Bg[3][2][2];
+	int l[3] = {0, 0, 0}; T f[3];
+	lowerFracT<D, T>(g, X, l, f);
+	T B[3][2][2], Bg[3][2][2];
 #pragma unroll
 	for (int a = 0; a < D; a++) hermiteBasis(f[a], B[a], Bg[a]);
 #pragma unroll
 	for (int r = 0; r < D; r++) val[r] = 0;
 #pragma unroll
 	for (int e = 0; e < D * D; e++) Gh[e] = 0;
-	const double dx = g.dx, inv = g.inv;
+	const T dx = T(g.dx), inv = T(g.inv);
+	constexpr bool kDisp = !std::is_same<T, double>::value;
 
 #pragma unroll
 	for (int cn = 0; cn < (1 << D); cn++) {
@@ -115,8 +126,8 @@ __device__ __forceinline__ void hermiteRefMap(const GridDev &g, const double *co
 #pragma unroll
 		for (int ord = 0; ord < (1 << D); ord++) {
 			const int ox = ord & 1, oy = ord >> 1 & 1, oz = D == 3 ? ord >> 2 : 0;
-			double wt = B[0][vx][ox] * B[1][vy][oy];
-			double wg[3];
+			T wt = B[0][vx][ox] * B[1][vy][oy];
+			T wg[3];
 			if constexpr (D == 2) {
 				wg[0] = Bg[0][vx][ox] * B[1][vy][oy];
 				wg[1] = B[0][vx][ox] * Bg[1][vy][oy];
@@ -128,24 +139,32 @@ __device__ __forceinline__ void hermiteRefMap(const GridDev &g, const double *co
 			}
 #pragma unroll
 			for (int r = 0; r < D; r++) {
-				double v;
+				T v;
 				// getValues ordering: 0 value, 1 d/dx, 2 d/dy, 3 dxy, 4 d/dz, 5 dxz, 6 dyz, 7 dxyz
-				if (ord == 0) v = xi[r][q];
-				else if (ord == 1) v = J[0 * D + r][q] * dx;
-				else if (ord == 2) v = J[1 * D + r][q] * dx;
-				else if (ord == 4) v = J[2 * D + r][q] * dx;
-				else if (ord == 3) {
-					const double m = (J[1 * D + r][gat<D>(g, qi + 1, qj, qk)] - J[1 * D + r][gat<D>(g, qi - 1, qj, qk)]) / 2 * inv;
+				if constexpr (kDisp) {
+					const int lv[3] = {l[0] + vx, l[1] + vy, l[2] + vz};
+					if (ord == 0) v = T(xi[r][q] - (g.o[r] + lv[r] * g.dx));
+					else if (ord == 1) v = T(J[0 * D + r][q] - (r == 0)) * dx;
+					else if (ord == 2) v = T(J[1 * D + r][q] - (r == 1)) * dx;
+					else if (ord == 4) v = T(J[2 * D + r][q] - (r == 2)) * dx;
+				} else {
+					if (ord == 0) v = T(xi[r][q]);
+					else if (ord == 1) v = T(J[0 * D + r][q]) * dx;
+					else if (ord == 2) v = T(J[1 * D + r][q]) * dx;
+					else if (ord == 4) v = T(J[2 * D + r][q]) * dx;
+				}
+				if (ord == 3) {
+					const T m = T(J[1 * D + r][gat<D>(g, qi + 1, qj, qk)] - J[1 * D + r][gat<D>(g, qi - 1, qj, qk)]) / 2 * inv;
 					v = m * dx * dx;
 				} else if (ord == 5) {
-					const double m = (J[2 * D + r][gat<D>(g, qi + 1, qj, qk)] - J[2 * D + r][gat<D>(g, qi - 1, qj, qk)]) / 2 * inv;
+					const T m = T(J[2 * D + r][gat<D>(g, qi + 1, qj, qk)] - J[2 * D + r][gat<D>(g, qi - 1, qj, qk)]) / 2 * inv;
 					v = m * dx * dx;
 				} else if (ord == 6) {
-					const double m = (J[2 * D + r][gat<D>(g, qi, qj + 1, qk)] - J[2 * D + r][gat<D>(g, qi, qj - 1, qk)]) / 2 * inv;
+					const T m = T(J[2 * D + r][gat<D>(g, qi, qj + 1, qk)] - J[2 * D + r][gat<D>(g, qi, qj - 1, qk)]) / 2 * inv;
 					v = m * dx * dx;
-				} else {
+				} else if (ord == 7) {
 					const double *c2 = J[2 * D + r];
-					const double m = (c2[gat<D>(g, qi + 1, qj + 1, qk)] - c2[gat<D>(g, qi - 1, qj + 1, qk)]
+					const T m = T(c2[gat<D>(g, qi + 1, qj + 1, qk)] - c2[gat<D>(g, qi - 1, qj + 1, qk)]
 						- c2[gat<D>(g, qi + 1, qj - 1, qk)] + c2[gat<D>(g, qi - 1, qj - 1, qk)]) / 4 * inv * inv;
 					v = m * dx * dx * dx;
 				}
@@ -157,6 +176,10 @@ __device__ __forceinline__ void hermiteRefMap(const GridDev &g, const double *co
 	}
 #pragma unroll
 	for (int e = 0; e < D * D; e++) Gh[e] *= inv;
+	if constexpr (kDisp) {
+#pragma unroll
+		for (int r = 0; r < D; r++) { val[r] += X[r]; Gh[r * D + r] += T(1); }
+	}
 }
 
 // ------------------------------------------------------------------ kernels
@@ -173,45 +196,45 @@ struct RefMapArgs {
 
 // SemiLagrangian<D, 2, HermiteIntrpl<D, VectorDd>>::advect (drv overload,
 // Advection.h:147-160) with Advection::traceDrv<2> (Advection.h:71-82).
-template <int D>
+template <int D, typename T>
 __global__ __launch_bounds__(kBlock) void kAdvectRefMap(RefMapArgs<D> A)
 {
 	const Coord c = threadCoord();
 	if (!inGrid<D>(A.cell, c)) return;
 	const size_t id = gidx<D>(A.cell, c.i, c.j, c.k);
-	const double h = A.h;
-	double x[3] = {A.cell.o[0] + c.i * A.cell.dx, A.cell.o[1] + c.j * A.cell.dx, A.cell.o[2] + c.k * A.cell.dx};
+	const T h = T(A.h);
+	T x[3] = {T(A.cell.o[0] + c.i * A.cell.dx), T(A.cell.o[1] + c.j * A.cell.dx), T(A.cell.o[2] + c.k * A.cell.dx)};
 
-	double v0[3], G0[9], p1[3], R1[9], v1[3], G1[9], X[3], R[9];
-	sampleVelGrad<D>(A.U, x, v0, G0);
+	T v0[3], G0[9], p1[3], R1[9], v1[3], G1[9], X[3], R[9];
+	sampleVelGrad<D, T>(A.U, x, v0, G0);
 #pragma unroll
 	for (int a = 0; a < D; a++) p1[a] = x[a] + v0[a] * h;
 #pragma unroll
 	for (int a = 0; a < D; a++)
 #pragma unroll
-		for (int b = 0; b < D; b++) R1[a * D + b] = (a == b ? 1.0 : 0.0) + G0[a * D + b] * h;
-	sampleVelGrad<D>(A.U, p1, v1, G1);
+		for (int b = 0; b < D; b++) R1[a * D + b] = (a == b ? T(1) : T(0)) + G0[a * D + b] * h;
+	sampleVelGrad<D, T>(A.U, p1, v1, G1);
 #pragma unroll
 	for (int a = 0; a < D; a++) X[a] = x[a] + (v0[a] + v1[a]) * h / 2;
 #pragma unroll
 	for (int a = 0; a < D; a++)
 #pragma unroll
 		for (int b = 0; b < D; b++) {
-			double gr = 0;
+			T gr = 0;
 #pragma unroll
 			for (int k = 0; k < D; k++) gr += G1[a * D + k] * R1[k * D + b];
-			R[a * D + b] = (a == b ? 1.0 : 0.0) + (G0[a * D + b] + gr) * h / 2;
+			R[a * D + b] = (a == b ? T(1) : T(0)) + (G0[a * D + b] + gr) * h / 2;
 		}
 
-	double val[3], Gh[9];
-	hermiteRefMap<D>(A.cell, A.xi, A.J, X, val, Gh);
+	T val[3], Gh[9];
+	hermiteRefMap<D, T>(A.cell, A.xi, A.J, X, val, Gh);
 #pragma unroll
 	for (int r = 0; r < D; r++) A.xi_new[r][id] = val[r];
 #pragma unroll
 	for (int r = 0; r < D; r++)
 #pragma unroll
 		for (int col = 0; col < D; col++) {
-			double s = 0;
+			T s = 0;
 #pragma unroll
 			for (int k = 0; k < D; k++) s += Gh[k * D + r] * R[k * D + col];
 			A.J_new[col * D + r][id] = s;
@@ -220,46 +243,48 @@ __global__ __launch_bounds__(kBlock) void kAdvectRefMap(RefMapArgs<D> A)
 
 // SemiLagrangian<D, 2, CubicIntrpl<D>>::advect on face grid `axis`
 // (Advection.h:15-22, 37-43, 137-145; CubicIntrpl LagrangeIntrpl.h:128-171).
-template <int D>
-__global__ __launch_bounds__(kBlock) void kAdvectVelocity(FaceSet U, int axis, double *out, double h)
+template <int D, typename T>
+__global__ __launch_bounds__(kBlock) void kAdvectVelocity(FaceSet U, int axis, double *out, double hd)
 {
 	const GridDev &g = U.g[axis];
 	const Coord c = threadCoord();
 	if (!inGrid<D>(g, c)) return;
-	double x[3] = {g.o[0] + c.i * g.dx, g.o[1] + c.j * g.dx, g.o[2] + c.k * g.dx};
-	double v0[3], p1[3], v1[3], X[3];
-	sampleVel<D>(U, x, v0);
+	const T h = T(hd);
+	T x[3] = {T(g.o[0] + c.i * g.dx), T(g.o[1] + c.j * g.dx), T(g.o[2] + c.k * g.dx)};
+	T v0[3], p1[3], v1[3], X[3];
+	sampleVel<D, T>(U, x, v0);
 #pragma unroll
 	for (int a = 0; a < D; a++) p1[a] = x[a] + v0[a] * h;
-	sampleVel<D>(U, p1, v1);
+	sampleVel<D, T>(U, p1, v1);
 #pragma unroll
 	for (int a = 0; a < D; a++) X[a] = x[a] + (v0[a] + v1[a]) * h / 2;
 
-	int l[3]; double f[3];
+	int l[3]; T f[3];
 #pragma unroll
 	for (int a = 0; a < D; a++) {
-		l[a] = int(floor((X[a] - g.o[a]) * g.inv)) - 1;   // getCubicLower (Grid.h:54)
-		f[a] = (X[a] - g.o[a] - l[a] * g.dx) * g.inv;
+		const T o = T(g.o[a]), inv = T(g.inv);
+		l[a] = int(floor((X[a] - o) * inv)) - 1;   // getCubicLower (Grid.h:54)
+		f[a] = (X[a] - o - l[a] * T(g.dx)) * inv;
 	}
-	double w[4][3];
+	T w[4][3];
 #pragma unroll
 	for (int a = 0; a < D; a++) {
-		const double t = f[a];
+		const T t = f[a];
 		w[0][a] = (t - 1) * (t - 2) * (t - 3) / -6;
 		w[1][a] = t * (t - 2) * (t - 3) / 2;
 		w[2][a] = t * (t - 1) * (t - 3) / -2;
 		w[3][a] = t * (t - 1) * (t - 2) / 6;
 	}
 	const double *ua = U.a[axis];
-	double val = 0;
+	T val = 0;
 #pragma unroll
 	for (int k = 0; k < (D == 3 ? 4 : 1); k++)
 #pragma unroll
 		for (int j = 0; j < 4; j++)
 #pragma unroll
 			for (int i = 0; i < 4; i++) {
-				const double wt = D == 2 ? w[i][0] * w[j][1] : w[i][0] * w[j][1] * w[k][2];
-				val += ua[gat<D>(g, l[0] + i, l[1] + j, D == 3 ? l[2] + k : 0)] * wt;
+				const T wt = D == 2 ? w[i][0] * w[j][1] : w[i][0] * w[j][1] * w[k][2];
+				val += T(ua[gat<D>(g, l[0] + i, l[1] + j, D == 3 ? l[2] + k : 0)]) * wt;
 			}
 	out[gidx<D>(g, c.i, c.j, c.k)] = val;
 }
@@ -283,43 +308,43 @@ struct ComposeArgs {
 // grad phi = J^T grad H[phi_ref](xi) (DrvScalarFieldWrapper with
 // HermiteIntrpl<D, double>, ScalarField.h:50-69), air clamp,
 // enforceBoundaryLevelSet at 3 dx and 0 (:966-975), phi_tmp, phi_ori.
-template <int D>
+template <int D, typename T>
 __global__ __launch_bounds__(kBlock) void kCompose(ComposeArgs<D> A)
 {
 	const GridDev &g = A.cell;
 	const Coord c = threadCoord();
 	if (!inGrid<D>(g, c)) return;
 	const size_t id = gidx<D>(g, c.i, c.j, c.k);
-	double X[3];
+	T X[3];
 #pragma unroll
-	for (int a = 0; a < D; a++) X[a] = A.xi[a][id];
-	int l[3]; double f[3];
-	lowerFrac<D>(g, X, l, f);
-	double B[3][2][2], Bg[3][2][2];
+	for (int a = 0; a < D; a++) X[a] = T(A.xi[a][id]);
+	int l[3]; T f[3];
+	lowerFracT<D, T>(g, X, l, f);
+	T B[3][2][2], Bg[3][2][2];
 #pragma unroll
 	for (int a = 0; a < D; a++) hermiteBasis(f[a], B[a], Bg[a]);
-	const double dx = g.dx;
-	double val = 0, gr[3] = {0, 0, 0};
+	const T dx = T(g.dx);
+	T val = 0, gr[3] = {0, 0, 0};
 #pragma unroll
 	for (int cn = 0; cn < (1 << D); cn++) {
 		const int vx = cn & 1, vy = cn >> 1 & 1, vz = D == 3 ? cn >> 2 : 0;
 		const size_t q = gat<D>(g, l[0] + vx, l[1] + vy, D == 3 ? l[2] + vz : 0);
-		double v[8];
-		v[0] = A.phi_ref[q];
-		v[1] = A.phi_ref_grad[0][q] * dx;
-		v[2] = A.phi_ref_grad[1][q] * dx;
-		v[3] = A.mixed[0][q] * dx * dx;
+		T v[8];
+		v[0] = T(A.phi_ref[q]);
+		v[1] = T(A.phi_ref_grad[0][q]) * dx;
+		v[2] = T(A.phi_ref_grad[1][q]) * dx;
+		v[3] = T(A.mixed[0][q]) * dx * dx;
 		if constexpr (D == 3) {
-			v[4] = A.phi_ref_grad[2][q] * dx;
-			v[5] = A.mixed[1][q] * dx * dx;
-			v[6] = A.mixed[2][q] * dx * dx;
-			v[7] = A.mixed[3][q] * dx * dx * dx;
+			v[4] = T(A.phi_ref_grad[2][q]) * dx;
+			v[5] = T(A.mixed[1][q]) * dx * dx;
+			v[6] = T(A.mixed[2][q]) * dx * dx;
+			v[7] = T(A.mixed[3][q]) * dx * dx * dx;
 		}
 #pragma unroll
 		for (int ord = 0; ord < (1 << D); ord++) {
 			const int ox = ord & 1, oy = ord >> 1 & 1, oz = D == 3 ? ord >> 2 : 0;
-			double wt = B[0][vx][ox] * B[1][vy][oy];
-			double wg[3];
+			T wt = B[0][vx][ox] * B[1][vy][oy];
+			T wg[3];
 			if constexpr (D == 2) {
 				wg[0] = Bg[0][vx][ox] * B[1][vy][oy];
 				wg[1] = B[0][vx][ox] * Bg[1][vy][oy];
@@ -335,12 +360,12 @@ __global__ __launch_bounds__(kBlock) void kCompose(ComposeArgs<D> A)
 		}
 	}
 #pragma unroll
-	for (int a = 0; a < D; a++) gr[a] *= g.inv;
+	for (int a = 0; a < D; a++) gr[a] *= T(g.inv);
 #pragma unroll
 	for (int col = 0; col < D; col++) {
-		double s = 0;
+		T s = 0;
 #pragma unroll
-		for (int r = 0; r < D; r++) s += A.J[col * D + r][id] * gr[r];
+		for (int r = 0; r < D; r++) s += T(A.J[col * D + r][id]) * gr[r];
 		A.phi_grad[col][id] = s;
 	}
 	double phi = val;
@@ -400,7 +425,7 @@ static FaceSet faceSet(const Fields &F)
 }
 
 template <int D>
-static void advectRefMapD(const Fields &F, double dt, hipStream_t s)
+static void advectRefMapD(const Fields &F, double dt, bool fp32, hipStream_t s)
 {
 	RefMapArgs<D> A{};
 	A.cell = F.cell;
@@ -408,25 +433,30 @@ static void advectRefMapD(const Fields &F, double dt, hipStream_t s)
 	for (int a = 0; a < D; a++) { A.xi[a] = F.xi[a]; A.xi_new[a] = F.xi_new[a]; }
 	for (int e = 0; e < D * D; e++) { A.J[e] = F.J[e]; A.J_new[e] = F.J_new[e]; }
 	A.h = -dt;
-	hipLaunchKernelGGL(kAdvectRefMap<D>, gridDims(F.cell, D), gridBlock(), 0, s, A);
+	if (fp32) hipLaunchKernelGGL((kAdvectRefMap<D, float>), gridDims(F.cell, D), gridBlock(), 0, s, A);
+	else hipLaunchKernelGGL((kAdvectRefMap<D, double>), gridDims(F.cell, D), gridBlock(), 0, s, A);
 }
 
-void launchAdvectRefMap(const Fields &F, double dt, hipStream_t s)
+void launchAdvectRefMap(const Fields &F, double dt, hipStream_t s, bool fp32)
 {
-	if (F.D == 2) advectRefMapD<2>(F, dt, s); else advectRefMapD<3>(F, dt, s);
+	if (F.D == 2) advectRefMapD<2>(F, dt, fp32, s); else advectRefMapD<3>(F, dt, fp32, s);
 }
 
-void launchAdvectVelocity(const Fields &F, double dt, hipStream_t s)
+template <int D, typename T>
+static void advectVelocityD(const Fields &F, double dt, hipStream_t s)
 {
 	const FaceSet U = faceSet(F);
-	for (int a = 0; a < F.D; a++) {
-		if (F.D == 2) hipLaunchKernelGGL(kAdvectVelocity<2>, gridDims(F.face[a], 2), gridBlock(), 0, s, U, a, F.u_new[a], -dt);
-		else hipLaunchKernelGGL(kAdvectVelocity<3>, gridDims(F.face[a], 3), gridBlock(), 0, s, U, a, F.u_new[a], -dt);
-	}
+	for (int a = 0; a < D; a++) hipLaunchKernelGGL((kAdvectVelocity<D, T>), gridDims(F.face[a], D), gridBlock(), 0, s, U, a, F.u_new[a], -dt);
+}
+
+void launchAdvectVelocity(const Fields &F, double dt, hipStream_t s, bool fp32)
+{
+	if (F.D == 2) { if (fp32) advectVelocityD<2, float>(F, dt, s); else advectVelocityD<2, double>(F, dt, s); }
+	else { if (fp32) advectVelocityD<3, float>(F, dt, s); else advectVelocityD<3, double>(F, dt, s); }
 }
 
 template <int D>
-static void composeD(const Fields &F, hipStream_t s)
+static void composeD(const Fields &F, bool fp32, hipStream_t s)
 {
 	ComposeArgs<D> A{};
 	A.cell = F.cell;
@@ -439,12 +469,13 @@ static void composeD(const Fields &F, hipStream_t s)
 	A.phi = F.phi; A.phi_tmp = F.phi_tmp; A.phi_ori = F.phi_ori;
 	A.band = 15.0 * F.cell.dx;
 	A.tol = 3.0 * F.cell.dx;
-	hipLaunchKernelGGL(kCompose<D>, gridDims(F.cell, D), gridBlock(), 0, s, A);
+	if (fp32) hipLaunchKernelGGL((kCompose<D, float>), gridDims(F.cell, D), gridBlock(), 0, s, A);
+	else hipLaunchKernelGGL((kCompose<D, double>), gridDims(F.cell, D), gridBlock(), 0, s, A);
 }
 
-void launchCompose(const Fields &F, hipStream_t s)
+void launchCompose(const Fields &F, hipStream_t s, bool fp32)
 {
-	if (F.D == 2) composeD<2>(F, s); else composeD<3>(F, s);
+	if (F.D == 2) composeD<2>(F, fp32, s); else composeD<3>(F, fp32, s);
 }
 
 void launchRefLevelSetMixed(const Fields &F, hipStream_t s)

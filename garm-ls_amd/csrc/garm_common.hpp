@@ -65,6 +65,18 @@ __device__ __forceinline__ void lowerFrac(const GridDev &g, const double *p, int
 	}
 }
 
+// The same in compute type T (double reproduces lowerFrac exactly).
+template <int D, typename T>
+__device__ __forceinline__ void lowerFracT(const GridDev &g, const T *p, int *l, T *f)
+{
+#pragma unroll
+	for (int a = 0; a < D; a++) {
+		const T o = T(g.o[a]), inv = T(g.inv);
+		l[a] = int(floor((p[a] - o) * inv));
+		f[a] = (p[a] - o - l[a] * T(g.dx)) * inv;
+	}
+}
+
 // Launch geometry for one grid: x is the fastest axis and one wave64 covers
 // 64 consecutive x-cells so every component array is read in 512-byte runs.
 inline dim3 gridBlock() { return dim3(64, 4, 1); }

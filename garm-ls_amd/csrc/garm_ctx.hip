@@ -25,6 +25,7 @@ struct garm_ctx {
 	std::vector<void *> allocs;
 	hipEvent_t ev[8] = {};
 	double *host_scalars = nullptr;   // pinned
+	bool fp32 = false;                // compute precision of the per-operator advection / reinit calls
 };
 
 namespace {
@@ -144,10 +145,10 @@ int doProject(garm_ctx *c, const garm_pcg_opts &o, garm_pcg_stats *st)
 	return launched(c, "projection");
 }
 
-int doReinit(garm_ctx *c, int iters, double cfl)
+int doReinit(garm_ctx *c, int iters, double cfl, bool fp32)
 {
 	if (iters < 0 || !(cfl > 0)) return fail(c, GARM_EINVAL, "bad reinitialisation arguments");
-	launchReinit(c->F, iters, cfl, c->s);
+	launchReinit(c->F, iters, cfl, c->s, fp32);
 	return launched(c, "reinitialisation");
 }
 
@@ -415,7 +416,7 @@ int garm_union_plane(garm_ctx *c, const double p[3], const double n[3])
 int garm_advect_refmap(garm_ctx *c, double dt)
 {
 	if (!c) return GARM_EINVAL;
-	launchAdvectRefMap(c->F, dt, c->s);
+	launchAdvectRefMap(c->F, dt, c->s, c->fp32);
 	swapRefMap(c->F);
 	return launched(c, "reference-map advection");
 }
@@ -423,7 +424,7 @@ int garm_advect_refmap(garm_ctx *c, double dt)
 int garm_advect_velocity(garm_ctx *c, double dt)
 {
 	if (!c) return GARM_EINVAL;
-	launchAdvectVelocity(c->F, dt, c->s);
+	launchAdvectVelocity(c->F, dt, c->s, c->fp32);
 	swapFaces(c->F);
 	return launched(c, "velocity advection");
 }
@@ -431,11 +432,19 @@ int garm_advect_velocity(garm_ctx *c, double dt)
 int garm_compose_levelset(garm_ctx *c)
 {
 	if (!c) return GARM_EINVAL;
-	launchCompose(c->F, c->s);
+	launchCompose(c->F, c->s, c->fp32);
 	return launched(c, "composition");
 }
 
-int garm_reinit_weno5(garm_ctx *c, int iters, double cfl) { return c ? doReinit(c, iters, cfl) : GARM_EINVAL; }
+int garm_reinit_weno5(garm_ctx *c, int iters, double cfl) { return c ? doReinit(c, iters, cfl, c->fp32) : GARM_EINVAL; }
+
+int garm_set_precision(garm_ctx *c, int precision)
+{
+	if (!c) return GARM_EINVAL;
+	if (precision != GARM_FP64 && precision != GARM_FP32) return fail(c, GARM_EINVAL, "unknown precision");
+	c->fp32 = precision == GARM_FP32;
+	return GARM_OK;
+}
 
 int garm_extrapolate_collider(garm_ctx *c)
 {
@@ -484,6 +493,16 @@ int garm_project(garm_ctx *c, const garm_pcg_opts *o, garm_pcg_stats *st)
 	return doProject(c, *o, st);
 }
 
+int garm_volume(garm_ctx *c, int order, double *out)
+{
+	if (!c || !out || (order != GARM_VOLUME_REFERENCE && order != GARM_VOLUME_CORRECTED)) return c ? fail(c, GARM_EINVAL, "bad volume arguments") : GARM_EINVAL;
+	launchVolume(c->F, order == GARM_VOLUME_CORRECTED, c->s);
+	int rc;
+	if ((rc = launched(c, "volume")) || (rc = readScalars(c, 4))) return rc;
+	*out = c->host_scalars[3];
+	return GARM_OK;
+}
+
 int garm_apply_pressure_system(garm_ctx *c, const double *x, double *b)
 {
 	if (!c || !x || !b) return GARM_EINVAL;
@@ -530,7 +549,7 @@ int garm_initialize(garm_ctx *c, const garm_step_opts *o)
 	Fields &F = c->F;
 	launchEnforceLevelSet(F, F.phi, 0.0, c->s);
 	launchGradient4(F, F.phi, F.phi_grad, c->s);        // Newton build: gradient before reinit
-	int rc = doReinit(c, o->reinit_iterations, o->reinit_cfl);
+	int rc = doReinit(c, o->reinit_iterations, o->reinit_cfl, false);
 	if (rc) return rc;
 	launchCopy(F.phi_tmp, F.phi, F.N, c->s);
 	launchCopy(F.phi_ref, F.phi, F.N, c->s);
@@ -558,15 +577,17 @@ int garm_step(garm_ctx *c, double dt, int step_index, const garm_step_opts *o, g
 	const int k = o->reinit_interval > 0 ? o->reinit_interval : 1;
 	int rc;
 
+	if (o->advect_precision != GARM_FP64 && o->advect_precision != GARM_FP32) return fail(c, GARM_EINVAL, "unknown precision");
+	const bool fp32 = o->advect_precision == GARM_FP32;
 	(void)hipEventRecord(ev[0], c->s);
-	launchAdvectRefMap(F, dt, c->s);
+	launchAdvectRefMap(F, dt, c->s, fp32);
 	swapRefMap(F);
-	launchAdvectVelocity(F, dt, c->s);
+	launchAdvectVelocity(F, dt, c->s, fp32);
 	swapFaces(F);
-	launchCompose(F, c->s);
+	launchCompose(F, c->s, fp32);
 	(void)hipEventRecord(ev[1], c->s);
 	st->reinitialized = step_index % k == 0;
-	if (st->reinitialized && (rc = doReinit(c, o->reinit_iterations, o->reinit_cfl))) return rc;
+	if (st->reinitialized && (rc = doReinit(c, o->reinit_iterations, o->reinit_cfl, fp32))) return rc;
 	(void)hipEventRecord(ev[2], c->s);
 	launchExtrapolateCollider(F, c->s);
 	(void)hipEventRecord(ev[3], c->s);
@@ -574,7 +595,7 @@ int garm_step(garm_ctx *c, double dt, int step_index, const garm_step_opts *o, g
 	if ((rc = doCheck(c, 0.883022221559489, 4, 1e-5, &ok))) return rc;   // LevelSetLiquid.cpp:631
 	if (!ok) {
 		doRestart(c);
-		launchCompose(F, c->s);
+		launchCompose(F, c->s, fp32);
 		launchExtrapolateCollider(F, c->s);
 		st->restarted = 1;
 	}
@@ -620,10 +641,10 @@ int garm_time_kernel(garm_ctx *c, int which, int reps, double *avg_ms, double *b
 		const double dt = 0.5 * F.cell.dx;   // backtrace length of half a cell
 		(void)hipEventRecord(c->ev[0], c->s);
 		for (int q = 0; q < reps; q++) {
-			if (which == GARM_K_ADVECT_REFMAP) launchAdvectRefMap(F, dt, c->s);           // writes xi_new, J_new only
-			else if (which == GARM_K_ADVECT_VELOCITY) launchAdvectVelocity(F, dt, c->s);  // writes u_new only
-			else if (which == GARM_K_COMPOSE) launchCompose(F, c->s);
-			else launchReinitStage(F, 0.9 * F.cell.dx, c->s);                            // writes scratch only
+			if (which == GARM_K_ADVECT_REFMAP) launchAdvectRefMap(F, dt, c->s, c->fp32);           // writes xi_new, J_new only
+			else if (which == GARM_K_ADVECT_VELOCITY) launchAdvectVelocity(F, dt, c->s, c->fp32);  // writes u_new only
+			else if (which == GARM_K_COMPOSE) launchCompose(F, c->s, c->fp32);
+			else launchReinitStage(F, 0.9 * F.cell.dx, c->s, c->fp32);                            // writes scratch only
 		}
 		(void)hipEventRecord(c->ev[1], c->s);
 		if ((rc = launched(c, "timed kernel")) || (rc = hipCheck(c, hipEventSynchronize(c->ev[1]), "timed kernel"))) return rc;
@@ -632,7 +653,7 @@ int garm_time_kernel(garm_ctx *c, int which, int reps, double *avg_ms, double *b
 		if (which == GARM_K_ADVECT_REFMAP) *bytes = NF * 8 + N * (D + D * D) * 8 * 2;   // trace scratch not counted
 		else if (which == GARM_K_ADVECT_VELOCITY) *bytes = 2 * NF * 8;
 		else if (which == GARM_K_COMPOSE) *bytes = N * (8 * (D + D * D + 1 + D + M + 1) + 1 + 8 * (3 + D));
-		else *bytes = N * 16;
+		else *bytes = N * (8 + (c->fp32 ? 4 : 8));
 	}
 	*avg_ms = double(ms) / reps;
 	return GARM_OK;

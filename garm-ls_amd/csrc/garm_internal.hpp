@@ -46,19 +46,21 @@ struct Fields {
 };
 
 // ---- launchers (one per kernel family; all enqueue on `s`) ---------------
-void launchAdvectRefMap(const Fields &F, double dt, hipStream_t s);
-void launchAdvectVelocity(const Fields &F, double dt, hipStream_t s);
-void launchCompose(const Fields &F, hipStream_t s);
+void launchAdvectRefMap(const Fields &F, double dt, hipStream_t s, bool fp32 = false);
+void launchAdvectVelocity(const Fields &F, double dt, hipStream_t s, bool fp32 = false);
+void launchCompose(const Fields &F, hipStream_t s, bool fp32 = false);
 void launchRefLevelSetMixed(const Fields &F, hipStream_t s);
 void launchGradient4(const Fields &F, const double *src, double *const *dst, hipStream_t s);
-void launchReinit(const Fields &F, int iterations, double cfl, hipStream_t s);
+void launchReinit(const Fields &F, int iterations, double cfl, hipStream_t s, bool fp32 = false);
 // one Runge-Kutta stage L(phi) -> scratch[0], for timing
-void launchReinitStage(const Fields &F, double dtau, hipStream_t s);
+void launchReinitStage(const Fields &F, double dtau, hipStream_t s, bool fp32 = false);
 
 void launchUnionBox(const Fields &F, const double *center, const double *half, hipStream_t s);
 void launchUnionSphere(const Fields &F, const double *center, double r, hipStream_t s);
 void launchUnionPlane(const Fields &F, const double *pos, const double *nrm, hipStream_t s);
 void launchResetRefMap(const Fields &F, hipStream_t s);
+// volume of phi_tmp into F.dscalars[3] (corrected: circular corner order)
+void launchVolume(const Fields &F, bool corrected, hipStream_t s);
 void launchFill(double *p, double v, size_t n, hipStream_t s);
 void launchFillU8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
 void launchCopy(double *dst, const double *src, size_t n, hipStream_t s);

@@ -24,6 +24,9 @@ K_ADVECT_REFMAP, K_ADVECT_VELOCITY, K_COMPOSE, K_REINIT_STAGE, K_PCG_SPMV, K_PCG
 KERNEL_NAMES = ["advect_refmap", "advect_velocity", "compose", "reinit_stage", "pcg_spmv", "pcg_update", "pcg32_spmv",
                 "pcg32_update"]
 
+# garm_volume_order
+VOLUME_REFERENCE, VOLUME_CORRECTED = 0, 1
+
 # garm_pcg_mode
 PCG_FP64, PCG_MIXED, PCG_LEARNED = 0, 1, 2
 FP64, FP32 = 0, 1
@@ -107,6 +110,8 @@ def lib():
             "garm_step": (i, [vp, d, i, ctypes.POINTER(StepOpts), ctypes.POINTER(StepStats)]),
             "garm_max_speed": (i, [vp, dp]),
             "garm_apply_pressure_system": (i, [vp, vp, vp]),
+            "garm_volume": (i, [vp, i, dp]),
+            "garm_set_precision": (i, [vp, i]),
             "garm_synchronize": (i, [vp]),
             "garm_time_kernel": (i, [vp, i, i, dp, dp]),
             "garm_unet_create": (i, [ctypes.POINTER(vp), ctypes.c_ulonglong]),
@@ -257,6 +262,17 @@ class Liquid:
         st = StepStats()
         self._check(lib().garm_step(self.h, dt, step_index, ctypes.byref(opts), ctypes.byref(st)))
         return st.as_dict()
+
+    def set_precision(self, precision):
+        """FP64 or FP32 compute for the per-operator advection / reinit calls."""
+        self._check(lib().garm_set_precision(self.h, precision))
+
+    def volume(self, corrected=False):
+        """Liquid volume of phi_tmp (3D: MC-style fractions in the reference's
+        vertex order unless corrected; 2D: filled-contour area)."""
+        v = ctypes.c_double()
+        self._check(lib().garm_volume(self.h, 1 if corrected else 0, ctypes.byref(v)))
+        return v.value
 
     def apply_pressure_system(self, x):
         """b = A x with the operator of the last projection (cell fields)."""

@@ -106,7 +106,9 @@ typedef struct garm_step_opts {
 	int reinit_interval;     /* WENO5 reinitialisation every k steps (reference: every step, k=1) */
 	int reinit_iterations;   /* reference: 30 (LevelSetLiquid.cpp:1156) */
 	double reinit_cfl;       /* reference: 0.9 */
-	int advect_precision;    /* garm_precision for advection + reinit */
+	int advect_precision;    /* garm_precision of advection, composition and reinit
+	                            (computed in FP32, stored in FP64; garm_initialize's
+	                            reinitialisation is always FP64) */
 	garm_pcg_opts pcg;
 } garm_step_opts;
 
@@ -161,6 +163,11 @@ int garm_advect_velocity(garm_ctx *ctx, double dt);
 int garm_compose_levelset(garm_ctx *ctx);
 /* Reinitialization::solve<5> on _levelSet (Reinitialization.h:20-80). */
 int garm_reinit_weno5(garm_ctx *ctx, int iterations, double cfl);
+/* Compute precision (garm_precision) of the four calls above and of
+ * garm_time_kernel's advection / reinit kernels; FP64 by default.  Fields
+ * stay FP64 in memory either way; garm_step takes its own setting from
+ * garm_step_opts.advect_precision.  No counterpart in the reference. */
+int garm_set_precision(garm_ctx *ctx, int precision);
 /* extrapolateFieldsToCollider (LevelSetLiquid.cpp:1086-1137). */
 int garm_extrapolate_collider(garm_ctx *ctx);
 /* extrapolateFieldsToAir (LevelSetLiquid.cpp:1059-1084). */
@@ -187,6 +194,18 @@ int garm_apply_pressure_system(garm_ctx *ctx, const double *x, double *b);
 /* SGridBasedData::normMax of the velocity (SGridBasedData.h:48-52), used by
  * getTimeStep (LevelSetLiquid.h:129). */
 int garm_max_speed(garm_ctx *ctx, double *out);
+/* Vertex order of the 3D volume evaluator. */
+typedef enum garm_volume_order {
+	GARM_VOLUME_REFERENCE = 0,   /* lexicographic stencil passed to the circular-order
+	                                fraction, as LevelSetLiquid.cpp:1184-1185 does */
+	GARM_VOLUME_CORRECTED = 1    /* stencil permuted to the fraction's corner order */
+} garm_volume_order;
+
+/* LevelSetLiquid::calculateVolume (LevelSetLiquid.cpp:1162-1190) on phi_tmp,
+ * in FP64: 3D MC-style volume fractions (Fraction::get_mc_vol,
+ * MC-style-vol-eval/fraction.hpp:510-771), 2D filled-contour area
+ * (Contourer<2, true>, Contourer.cpp:9-96); `order` applies in 3D only. */
+int garm_volume(garm_ctx *ctx, int order, double *out);
 /* Blocks until all work queued on the context's stream is done. */
 int garm_synchronize(garm_ctx *ctx);
 

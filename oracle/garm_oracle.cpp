@@ -1359,4 +1359,47 @@ int ora_step(void *p, double dt, int doReinit, int reinitIters, double cfl, int 
 }
 double ora_max_speed(void *p) { auto *h = (Handle *)p; double r = 0; DISPATCH(h, r = L.maxSpeed()); return r; }
 
+
+double ora_mc_fraction(const double *c);
+double ora_filled_area(const double *p);
+
+// LevelSetLiquid::calculateVolume (LevelSetLiquid.cpp:1162-1190) on phi_tmp,
+// summed serially in the reference's x-fastest cell order.  corrected = 0
+// passes the lexicographic stencil as the reference does; 1 reorders it to
+// the fraction's circular corner order first.
+double ora_volume(void *p, int corrected)
+{
+	auto *h = (Handle *)p;
+	double res = 0.0;
+	if (h->dim == 2) {
+		const auto &L = *h->l2;
+		const int nx = L.res[0], ny = L.res[1];
+		for (int j = 0; j + 1 < ny; j++)
+			for (int i = 0; i + 1 < nx; i++) {
+				const size_t b = size_t(j) * nx + i;
+				const double v[4] = {L.phiTmp[b], L.phiTmp[b + 1], L.phiTmp[b + nx], L.phiTmp[b + nx + 1]};
+				res += ora_filled_area(v) * L.dx * L.dx;
+			}
+		return res;
+	}
+	const auto &L = *h->l3;
+	const int nx = L.res[0], ny = L.res[1], nz = L.res[2];
+	const double dx3 = L.dx * L.dx * L.dx;
+	const size_t sy = size_t(nx), sz = size_t(nx) * ny;
+	for (int k = 0; k + 1 < nz; k++)
+		for (int j = 0; j + 1 < ny; j++)
+			for (int i = 0; i + 1 < nx; i++) {
+				const size_t b = (size_t(k) * ny + j) * nx + i;
+				const double *f = L.phiTmp.data();
+				const double lex[8] = {f[b], f[b + 1], f[b + sy], f[b + sy + 1], f[b + sz], f[b + sz + 1], f[b + sz + sy], f[b + sz + sy + 1]};
+				if (corrected) {
+					const double c[8] = {lex[0], lex[1], lex[3], lex[2], lex[4], lex[5], lex[7], lex[6]};
+					res += ora_mc_fraction(c) * dx3;
+				} else {
+					res += ora_mc_fraction(lex) * dx3;
+				}
+			}
+	return res;
+}
+
 } // extern "C"

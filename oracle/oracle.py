@@ -61,6 +61,9 @@ def lib():
             "ora_project": (i, [vp, i, d, i, ip, dp, ip]),
             "ora_step": (i, [vp, d, i, i, d, i, d, i, ip, ip, dp, ip]),
             "ora_max_speed": (d, [vp]),
+            "ora_volume": (d, [vp, ctypes.c_int]),
+            "ora_mc_fraction": (d, [ctypes.POINTER(ctypes.c_double)]),
+            "ora_filled_area": (d, [ctypes.POINTER(ctypes.c_double)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -177,3 +180,20 @@ class Oracle:
 
     def max_speed(self):
         return lib().ora_max_speed(self.h)
+
+    def volume(self, corrected=False):
+        """LevelSetLiquid::calculateVolume on phi_tmp (reference vertex order
+        unless corrected)."""
+        return lib().ora_volume(self.h, 1 if corrected else 0)
+
+
+def mc_fraction(v):
+    """Fraction::get_mc_vol of 8 corner values in the fraction's own order."""
+    a = (ctypes.c_double * 8)(*[float(x) for x in v])
+    return lib().ora_mc_fraction(a)
+
+
+def filled_area(v):
+    """Filled-contour area of one cell, nodes (0,0), (1,0), (0,1), (1,1)."""
+    a = (ctypes.c_double * 4)(*[float(x) for x in v])
+    return lib().ora_filled_area(a)
