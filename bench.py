@@ -4,9 +4,19 @@
 Workload: 3D dam break, n^3 MAC grid (default 256^3 cells including a 2-cell
 solid band; liquid box x < 0.4, y < 0.6, z < 0.5 of the unit domain), gravity
 -9.8, CFL 1, frame rate 25 (the reference's dt rule), WENO5 reinitialisation
-(30 iterations) every 5 steps, pressure PCG to relative residual 1e-6.
+(30 iterations) every 5 steps, pressure solve to relative residual 1e-6.
 A step is one LevelSetLiquid::advance (LevelSetLiquid.cpp:608-698) with every
 field resident in HBM.
+
+Three configurations of the same workload run one after the other:
+  fp64     plain path: FP64 advection / reinit, FP64 Jacobi PCG;
+  mixed    FP32 advection / reinit, FP32 Jacobi PCG refined in FP64;
+  learned  FP32 advection / reinit, flexible CG preconditioned by the U-Net
+           (falling back to the mixed solver), the network trained first,
+           untimed, on (divergence, pressure) pairs of 128^3 dam breaks with
+           seeded box extents.
+The headline value is --mode (default learned); the others are reported
+beside it with their speed-up over fp64 and their volume drift against it.
 
 Prints one JSON line: cell-updates/s over the timed steps, per-phase times,
 the roofline position of the dominant kernel (timed live with HIP events on
@@ -33,15 +43,56 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(args, mode_name):
+MODES = {"fp64": (capi.PCG_FP64, capi.FP64), "mixed": (capi.PCG_MIXED, capi.FP32), "learned": (capi.PCG_LEARNED, capi.FP32)}
+
+
+def train_network(args):
+    """Network trained on the simulator's own pressure solves (untimed)."""
+    import numpy as np
+    from garm_amd import training
+    rng = np.random.default_rng(args.train_seed)
+    t0 = time.time()
+    R, P, M = [], [], []
+    for run in range(args.train_runs):
+        ext = tuple(float(x) for x in rng.uniform(0.2, 0.8, size=3))
+        g = scenes.grid(args.train_n, 3)
+        sim = capi.Liquid(3, g["res"], g["dx"], g["origin"], g["bw"])
+        scenes.dam_break(sim, g, ext)
+        sim.initialize(capi.step_opts(mode=capi.PCG_FP64, tol=1e-8, max_iterations=args.max_iterations))
+        r, p, m = training.collect_pairs(sim, g, args.train_pairs, tol=1e-8, reinit_interval=args.reinit_interval)
+        R.append(r)
+        P.append(p)
+        M.append(m)
+        sim.close()
+        log(f"[bench] training run {run + 1}/{args.train_runs}: box {tuple(round(e, 3) for e in ext)}, {len(r)} pairs")
+    R, P, M = np.concatenate(R), np.concatenate(P), np.concatenate(M)
+    t_collect = time.time() - t0
+    net = capi.Unet(seed=args.train_seed)
+    t0 = time.time()
+    hist = net.train(R, P, M, steps=args.train_steps, lr=args.train_lr, seed=args.train_seed)
+    t_train = time.time() - t0
+    k = max(1, len(hist) // 10)
+    info = {"pairs": int(len(R)), "grid": f"{args.train_n}^3", "runs": args.train_runs, "steps": args.train_steps,
+            "lr": args.train_lr, "loss_first": float(hist[:k].mean()), "loss_last": float(hist[-k:].mean()),
+            "collect_s": round(t_collect, 1), "train_s": round(t_train, 1)}
+    log(f"[bench] network trained: {info}")
+    return net, info
+
+
+def run_gpu(args, mode_name, net=None):
     g = scenes.grid(args.n, 3)
-    mode = {"fp64": capi.PCG_FP64, "mixed": capi.PCG_MIXED, "learned": capi.PCG_LEARNED}[mode_name]
+    mode, precision = MODES[mode_name]
     opts = capi.step_opts(reinit_interval=args.reinit_interval, tol=args.tol, max_iterations=args.max_iterations,
-                          check_interval=args.check_interval, mode=mode)
+                          check_interval=args.check_interval, mode=mode, precision=precision)
     t0 = time.time()
     sim = capi.Liquid(3, g["res"], g["dx"], g["origin"], g["bw"])
+    if net is not None:
+        sim.attach_preconditioner(net)
     scenes.dam_break(sim, g)
-    sim.initialize(opts)
+    init = capi.step_opts(reinit_interval=args.reinit_interval, tol=args.tol, max_iterations=args.max_iterations,
+                          check_interval=args.check_interval, mode=capi.PCG_FP64)
+    sim.initialize(init)
+    v0 = sim.volume()
     log(f"[bench] {mode_name}: {args.n}^3 initialised in {time.time() - t0:.1f}s")
     clk = FrameClock(g["dx"], args.cfl)
     step = 0
@@ -60,13 +111,29 @@ def run_gpu(args, mode_name):
             log(f"[bench] timed step {k + 1}/{args.steps}")
     sim.synchronize()
     elapsed = time.perf_counter() - t0
-    return g, sim, stats, elapsed
+    v1 = sim.volume()
+    sim.set_precision(precision)   # the kernels timed for the roofline are the ones this mode ran
+    return g, sim, stats, elapsed, (v1 - v0) / v0
 
 
-def roofline(sim, stats, args):
+def summary(stats, elapsed, cells, drift):
+    K = len(stats)
+    it = [s["pcg"]["iterations"] for s in stats]
+    return {"value": round(cells * K / elapsed, 1), "ms_per_step": round(1000 * elapsed / K, 3),
+            "phases_ms": {k: round(sum(s[k] for s in stats) / K, 3) for k in ("t_advect_ms", "t_restart_ms", "t_project_ms", "t_extrap_ms", "t_total_ms")},
+            "pcg_iterations": {"mean": sum(it) / K, "min": min(it), "max": max(it),
+                               "outer_mean": sum(s["pcg"]["outer_iterations"] for s in stats) / K,
+                               "network_applications_mean": sum(s["pcg"]["preconditioner_applications"] for s in stats) / K,
+                               "fell_back": sum(s["pcg"]["fell_back"] for s in stats)},
+            "pcg_residual_max": max(s["pcg"]["relative_residual"] for s in stats),
+            "restarts": sum(s["restarted"] for s in stats),
+            "volume_drift": drift}
+
+
+def roofline(sim, stats, args, mode_name):
     K = len(stats)
     iters = sum(s["pcg"]["iterations"] + s["pcg"]["outer_iterations"] for s in stats) / K    # SpMV launches per step
-    mixed = args.mode != "fp64"
+    mixed = mode_name != "fp64"
     reinit_stages = 3 * 30 * sum(s["reinitialized"] for s in stats) / K
     per_step = {capi.K_ADVECT_REFMAP: 1, capi.K_ADVECT_VELOCITY: 1, capi.K_COMPOSE: 1 + sum(s["restarted"] for s in stats) / K,
                 capi.K_REINIT_STAGE: reinit_stages, capi.K_PCG_SPMV: 0 if mixed else iters, capi.K_PCG_UPDATE: 0 if mixed else iters,
@@ -129,14 +196,20 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--mode", choices=["fp64", "mixed", "learned"], default="mixed")
-    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 run beside a mixed/learned one")
+    ap.add_argument("--mode", choices=list(MODES), default="learned", help="headline configuration")
+    ap.add_argument("--also", default="fp64,mixed", help="configurations run beside the headline one ('' for none)")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iterations", type=int, default=20000)
     ap.add_argument("--check-interval", type=int, default=16)
     ap.add_argument("--reinit-interval", type=int, default=5)
     ap.add_argument("--cfl", type=float, default=1.0)
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--train-n", type=int, default=128)
+    ap.add_argument("--train-runs", type=int, default=2)
+    ap.add_argument("--train-pairs", type=int, default=24)
+    ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--train-lr", type=float, default=1e-3)
+    ap.add_argument("--train-seed", type=int, default=19260817)
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -147,36 +220,36 @@ def main():
         sys.exit(2)
 
     cells = args.n ** 3
-    plain = None
-    if args.mode != "fp64" and not args.no_plain:
-        g, sim, stats, elapsed = run_gpu(args, "fp64")
+    order = [m for m in args.also.split(",") if m and m != args.mode] + [args.mode]
+    results, net, train_info = {}, None, None
+    for m in order:
+        if m == "learned" and net is None:
+            net, train_info = train_network(args)
+        g, sim, stats, elapsed, drift = run_gpu(args, m, net if m == "learned" else None)
+        results[m] = summary(stats, elapsed, cells, drift)
+        if m == args.mode:
+            dom, kernels, roof = roofline(sim, stats, args, m)
         sim.close()
-        K = len(stats)
-        plain = {"value": round(cells * K / elapsed, 1), "ms_per_step": round(1000 * elapsed / K, 3),
-                 "pcg_iterations_mean": sum(s["pcg"]["iterations"] for s in stats) / K,
-                 "phases_ms": {k: round(sum(s[k] for s in stats) / K, 3) for k in ("t_advect_ms", "t_restart_ms", "t_project_ms", "t_extrap_ms")}}
-    g, sim, stats, elapsed = run_gpu(args, args.mode)
-    K = len(stats)
-    value = cells * K / elapsed
-    phases = {k: round(sum(s[k] for s in stats) / K, 3) for k in ("t_advect_ms", "t_restart_ms", "t_project_ms", "t_extrap_ms", "t_total_ms")}
-    pcg_iters = [s["pcg"]["iterations"] for s in stats]
-    dom, kernels, roof = roofline(sim, stats, args)
-    sim.close()
+    head = results[args.mode]
+    plain = results.get("fp64")
+    for m, r in results.items():
+        if plain and m != "fp64":
+            r["speedup_vs_fp64"] = round(r["value"] / plain["value"], 3)
+            r["volume_drift_difference_vs_fp64"] = abs(r["volume_drift"] - plain["volume_drift"])
     out = {
-        "metric": METRIC, "value": round(value, 1), "unit": "cell-updates/s", "n_gpus": 1, "steps": K,
-        "warmup": args.warmup, "ms_per_step": round(1000 * elapsed / K, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.mode == "fp64" else "f64+f32",
+        "metric": METRIC, "value": head["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.mode == "fp64" else "f32+f64+bf16" if args.mode == "learned" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"3D dam break full step, {args.n}^3 MAC grid (liquid box x<0.4, y<0.6, z<0.5, gravity -9.8, CFL {args.cfl:g})",
-                   "pcg": args.mode, "tolerance": args.tol, "reinit_interval": args.reinit_interval,
-                   "cells": cells},
-        "phases_ms": phases,
-        "pcg_iterations": {"mean": sum(pcg_iters) / K, "min": min(pcg_iters), "max": max(pcg_iters),
-                           "outer_mean": sum(s["pcg"]["outer_iterations"] for s in stats) / K},
-        "pcg_residual_max": max(s["pcg"]["relative_residual"] for s in stats),
-        "plain_fp64": plain,
-        "speedup_vs_plain_fp64": round(value / plain["value"], 3) if plain else None,
-        "restarts": sum(s["restarted"] for s in stats),
+                   "mode": args.mode, "tolerance": args.tol, "reinit_interval": args.reinit_interval, "cells": cells},
+        "phases_ms": head["phases_ms"],
+        "pcg_iterations": head["pcg_iterations"],
+        "pcg_residual_max": head["pcg_residual_max"],
+        "speedup_vs_fp64": head.get("speedup_vs_fp64"),
+        "volume_drift_difference_vs_fp64": head.get("volume_drift_difference_vs_fp64"),
+        "modes": results,
+        "network_training": train_info,
         "kernels": {k: {kk: round(vv, 5) if isinstance(vv, float) else vv for kk, vv in v.items()} for k, v in kernels.items()},
         "roofline": roof,
         "cpu_baseline": None,
