@@ -147,9 +147,13 @@ def roofline(sim, stats, args, mode_name):
                                             ms_per_step=ms * per_step[k])
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
     d = kernels[dom]
+    # a plain FP64 copy of one cell field: the bandwidth a streaming kernel reaches here
+    ms, by = sim.time_kernel(capi.K_COPY, reps=args.kernel_reps)
+    copy = {"avg_ms": round(ms, 5), "bytes": by, "gbs": round(by / ms / 1e6, 1)}
     return dom, kernels, {"kernel": dom, "bound": "hbm", "achieved": round(d["gbs"], 1), "peak": HBM_PEAK_GBS,
-                          "unit": "GB/s", "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
-                          "avg_launch_ms": round(d["avg_ms"], 5), "bytes_per_launch": d["bytes"]}
+                          "unit": "GB/s", "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": args.traffic,
+                          "avg_launch_ms": round(d["avg_ms"], 5), "bytes_per_launch": d["bytes"],
+                          "copy_kernel": copy}
 
 
 def cpu_baseline(args):
@@ -213,6 +217,9 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic", type=float, default=None,
+                    help="HBM bytes per launch of the dominant kernel measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE "
+                         "(scaled by the copy kernel's reading), reported as roofline.traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU decomposition is designed "

@@ -633,7 +633,14 @@ int garm_time_kernel(garm_ctx *c, int which, int reps, double *avg_ms, double *b
 	for (int a = 0; a < D; a++) NF += F.Nf[a];
 	const double N = double(F.N);
 	float ms = 0;
-	if (which >= GARM_K_PCG_SPMV) {
+	if (which == GARM_K_COPY) {
+		(void)hipEventRecord(c->ev[0], c->s);
+		for (int q = 0; q < reps; q++) launchCopy(F.scratch[1], F.scratch[0], F.N, c->s);
+		(void)hipEventRecord(c->ev[1], c->s);
+		if ((rc = launched(c, "timed copy")) || (rc = hipCheck(c, hipEventSynchronize(c->ev[1]), "timed copy"))) return rc;
+		(void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+		*bytes = 16.0 * N;
+	} else if (which >= GARM_K_PCG_SPMV) {
 		const int k = which == GARM_K_PCG_SPMV ? 0 : which == GARM_K_PCG_UPDATE ? 1 : which == GARM_K_PCG32_SPMV ? 2 : 3;
 		rc = timeSolverKernel(D, c->P.rows, c->P.rows32, c->P.n, k, reps, c->P.K, c->s, c->ev[0], c->ev[1], &ms, bytes);
 		if (rc) return fail(c, rc, "solver kernels need a preceding projection (FP32 ones a mixed-precision one)");

@@ -20,9 +20,10 @@ FIELD_NAMES = ["phi", "phi_grad", "refmap", "refmap_grad", "phi_tmp", "phi_ori",
                "rhs", "unknowns"]
 
 # garm_kernel (instrumentation)
-K_ADVECT_REFMAP, K_ADVECT_VELOCITY, K_COMPOSE, K_REINIT_STAGE, K_PCG_SPMV, K_PCG_UPDATE, K_PCG32_SPMV, K_PCG32_UPDATE = range(8)
+K_ADVECT_REFMAP, K_ADVECT_VELOCITY, K_COMPOSE, K_REINIT_STAGE, K_PCG_SPMV, K_PCG_UPDATE, K_PCG32_SPMV, K_PCG32_UPDATE, \
+    K_COPY = range(9)
 KERNEL_NAMES = ["advect_refmap", "advect_velocity", "compose", "reinit_stage", "pcg_spmv", "pcg_update", "pcg32_spmv",
-                "pcg32_update"]
+                "pcg32_update", "copy"]
 
 # garm_volume_order
 VOLUME_REFERENCE, VOLUME_CORRECTED = 0, 1

@@ -221,7 +221,8 @@ typedef enum garm_kernel {
 	GARM_K_PCG_UPDATE = 5,      /* x, r update + |r|^2, r.z partials           */
 	GARM_K_PCG32_SPMV = 6,      /* the same two in FP32 (mixed-precision inner) */
 	GARM_K_PCG32_UPDATE = 7,
-	GARM_NUM_KERNELS = 8
+	GARM_K_COPY = 8,            /* plain FP64 copy of one cell field (bandwidth reference) */
+	GARM_NUM_KERNELS = 9
 } garm_kernel;
 
 /* Launches kernel `which` `reps` times on the context's stream on the current
