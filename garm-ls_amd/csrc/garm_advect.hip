@@ -417,6 +417,11 @@ __global__ __launch_bounds__(kBlock) void kGradient4(GridDev g, const double *sr
 }
 
 // ---------------------------------------------------------------- launchers
+// This file is compiled twice: once without fused multiply-add contraction
+// for the FP64 kernels (the IEEE operations of the oracle, so the plain path
+// reproduces it bit for bit) and once, with GARM_ADVECT_FP32_UNIT defined and
+// contraction on, for the FP32 kernels.  Each object launches only its own
+// instantiations; the dispatchers live in the FP64 object.
 static FaceSet faceSet(const Fields &F)
 {
 	FaceSet U{};
@@ -424,8 +429,14 @@ static FaceSet faceSet(const Fields &F)
 	return U;
 }
 
+#ifdef GARM_ADVECT_FP32_UNIT
+using Real = float;
+#else
+using Real = double;
+#endif
+
 template <int D>
-static void advectRefMapD(const Fields &F, double dt, bool fp32, hipStream_t s)
+static void advectRefMapD(const Fields &F, double dt, hipStream_t s)
 {
 	RefMapArgs<D> A{};
 	A.cell = F.cell;
@@ -433,30 +444,18 @@ static void advectRefMapD(const Fields &F, double dt, bool fp32, hipStream_t s)
 	for (int a = 0; a < D; a++) { A.xi[a] = F.xi[a]; A.xi_new[a] = F.xi_new[a]; }
 	for (int e = 0; e < D * D; e++) { A.J[e] = F.J[e]; A.J_new[e] = F.J_new[e]; }
 	A.h = -dt;
-	if (fp32) hipLaunchKernelGGL((kAdvectRefMap<D, float>), gridDims(F.cell, D), gridBlock(), 0, s, A);
-	else hipLaunchKernelGGL((kAdvectRefMap<D, double>), gridDims(F.cell, D), gridBlock(), 0, s, A);
-}
-
-void launchAdvectRefMap(const Fields &F, double dt, hipStream_t s, bool fp32)
-{
-	if (F.D == 2) advectRefMapD<2>(F, dt, fp32, s); else advectRefMapD<3>(F, dt, fp32, s);
-}
-
-template <int D, typename T>
-static void advectVelocityD(const Fields &F, double dt, hipStream_t s)
-{
-	const FaceSet U = faceSet(F);
-	for (int a = 0; a < D; a++) hipLaunchKernelGGL((kAdvectVelocity<D, T>), gridDims(F.face[a], D), gridBlock(), 0, s, U, a, F.u_new[a], -dt);
-}
-
-void launchAdvectVelocity(const Fields &F, double dt, hipStream_t s, bool fp32)
-{
-	if (F.D == 2) { if (fp32) advectVelocityD<2, float>(F, dt, s); else advectVelocityD<2, double>(F, dt, s); }
-	else { if (fp32) advectVelocityD<3, float>(F, dt, s); else advectVelocityD<3, double>(F, dt, s); }
+	hipLaunchKernelGGL((kAdvectRefMap<D, Real>), gridDims(F.cell, D), gridBlock(), 0, s, A);
 }
 
 template <int D>
-static void composeD(const Fields &F, bool fp32, hipStream_t s)
+static void advectVelocityD(const Fields &F, double dt, hipStream_t s)
+{
+	const FaceSet U = faceSet(F);
+	for (int a = 0; a < D; a++) hipLaunchKernelGGL((kAdvectVelocity<D, Real>), gridDims(F.face[a], D), gridBlock(), 0, s, U, a, F.u_new[a], -dt);
+}
+
+template <int D>
+static void composeD(const Fields &F, hipStream_t s)
 {
 	ComposeArgs<D> A{};
 	A.cell = F.cell;
@@ -469,13 +468,51 @@ static void composeD(const Fields &F, bool fp32, hipStream_t s)
 	A.phi = F.phi; A.phi_tmp = F.phi_tmp; A.phi_ori = F.phi_ori;
 	A.band = 15.0 * F.cell.dx;
 	A.tol = 3.0 * F.cell.dx;
-	if (fp32) hipLaunchKernelGGL((kCompose<D, float>), gridDims(F.cell, D), gridBlock(), 0, s, A);
-	else hipLaunchKernelGGL((kCompose<D, double>), gridDims(F.cell, D), gridBlock(), 0, s, A);
+	hipLaunchKernelGGL((kCompose<D, Real>), gridDims(F.cell, D), gridBlock(), 0, s, A);
+}
+
+#ifdef GARM_ADVECT_FP32_UNIT
+
+void launchAdvectRefMap32(const Fields &F, double dt, hipStream_t s)
+{
+	if (F.D == 2) advectRefMapD<2>(F, dt, s); else advectRefMapD<3>(F, dt, s);
+}
+
+void launchAdvectVelocity32(const Fields &F, double dt, hipStream_t s)
+{
+	if (F.D == 2) advectVelocityD<2>(F, dt, s); else advectVelocityD<3>(F, dt, s);
+}
+
+void launchCompose32(const Fields &F, hipStream_t s)
+{
+	if (F.D == 2) composeD<2>(F, s); else composeD<3>(F, s);
+}
+
+#else
+
+void launchAdvectRefMap32(const Fields &F, double dt, hipStream_t s);
+void launchAdvectVelocity32(const Fields &F, double dt, hipStream_t s);
+void launchCompose32(const Fields &F, hipStream_t s);
+
+void launchAdvectRefMap(const Fields &F, double dt, hipStream_t s, bool fp32)
+{
+	if (fp32) launchAdvectRefMap32(F, dt, s);
+	else if (F.D == 2) advectRefMapD<2>(F, dt, s);
+	else advectRefMapD<3>(F, dt, s);
+}
+
+void launchAdvectVelocity(const Fields &F, double dt, hipStream_t s, bool fp32)
+{
+	if (fp32) launchAdvectVelocity32(F, dt, s);
+	else if (F.D == 2) advectVelocityD<2>(F, dt, s);
+	else advectVelocityD<3>(F, dt, s);
 }
 
 void launchCompose(const Fields &F, hipStream_t s, bool fp32)
 {
-	if (F.D == 2) composeD<2>(F, fp32, s); else composeD<3>(F, fp32, s);
+	if (fp32) launchCompose32(F, s);
+	else if (F.D == 2) composeD<2>(F, s);
+	else composeD<3>(F, s);
 }
 
 void launchRefLevelSetMixed(const Fields &F, hipStream_t s)
@@ -494,5 +531,7 @@ void launchGradient4(const Fields &F, const double *src, double *const *dst, hip
 	if (F.D == 2) hipLaunchKernelGGL(kGradient4<2>, gridDims(F.cell, 2), gridBlock(), 0, s, F.cell, src, dst[0], dst[1], nullptr);
 	else hipLaunchKernelGGL(kGradient4<3>, gridDims(F.cell, 3), gridBlock(), 0, s, F.cell, src, dst[0], dst[1], dst[2]);
 }
+
+#endif
 
 } // namespace garm

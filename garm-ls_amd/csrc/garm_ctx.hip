@@ -516,6 +516,14 @@ int garm_apply_pressure_system(garm_ctx *c, const double *x, double *b)
 	return hipCheck(c, hipMemcpy(b, F.scratch[1], F.N * sizeof(double), hipMemcpyDeviceToHost), "download");
 }
 
+int garm_set_velocity_field(garm_ctx *c, int field, double time, double period)
+{
+	if (!c) return GARM_EINVAL;
+	if (field != GARM_VELOCITY_ENRIGHT || c->F.D != 3 || !(period > 0)) return fail(c, GARM_EINVAL, "unknown analytic field, or not a 3D context");
+	launchEnrightVelocity(c->F, std::cos(3.14159265358979323846 * time / period), c->s);
+	return launched(c, "analytic velocity");
+}
+
 int garm_max_speed(garm_ctx *c, double *out)
 {
 	if (!c || !out) return GARM_EINVAL;
@@ -594,6 +602,10 @@ int garm_step(garm_ctx *c, double dt, int step_index, const garm_step_opts *o, g
 	int ok = 1;
 	if ((rc = doCheck(c, 0.883022221559489, 4, 1e-5, &ok))) return rc;   // LevelSetLiquid.cpp:631
 	if (!ok) {
+		// the reference reinitialises every step (LevelSetLiquid.cpp:625), so a
+		// restart always captures a reinitialised level set; with a longer
+		// cadence the restart step reinitialises first
+		if (!st->reinitialized && (rc = doReinit(c, o->reinit_iterations, o->reinit_cfl, fp32))) return rc;
 		doRestart(c);
 		launchCompose(F, c->s, fp32);
 		launchExtrapolateCollider(F, c->s);

@@ -102,6 +102,29 @@ __global__ void kGravity(double *uy, double dv, size_t n)
 	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) uy[i] -= dv;
 }
 
+// Analytic velocity fields of the reference's advection tests, sampled at
+// the face positions of axis `axis`.  Enright deformation on the unit domain
+// (TestAdvect3D.cpp:152-190 with the domain mapped to [0,1]^3):
+//   u = 2 sin^2(pi x) sin(2 pi y) sin(2 pi z) c,  v = -sin^2(pi y) sin(2 pi x) sin(2 pi z) c,
+//   w = -sin^2(pi z) sin(2 pi x) sin(2 pi y) c,   c = cos(pi t / period).
+__global__ void kEnrightFaces(GridDev g, int axis, double coeff, double *u)
+{
+	const Coord c = threadCoord();
+	if (!inGrid<3>(g, c)) return;
+	const double pi = 3.14159265358979323846;
+	const double x = g.o[0] + c.i * g.dx, y = g.o[1] + c.j * g.dx, z = g.o[2] + c.k * g.dx;
+	double v;
+	if (axis == 0) v = 2 * sin(pi * x) * sin(pi * x) * sin(2 * pi * y) * sin(2 * pi * z);
+	else if (axis == 1) v = -sin(pi * y) * sin(pi * y) * sin(2 * pi * x) * sin(2 * pi * z);
+	else v = -sin(pi * z) * sin(pi * z) * sin(2 * pi * x) * sin(2 * pi * y);
+	u[gidx<3>(g, c.i, c.j, c.k)] = v * coeff;
+}
+
+void launchEnrightVelocity(const Fields &F, double coeff, hipStream_t s)
+{
+	for (int a = 0; a < 3; a++) hipLaunchKernelGGL(kEnrightFaces, gridDims(F.face[a], 3), gridBlock(), 0, s, F.face[a], a, coeff, F.u[a]);
+}
+
 // ---------------------------------------------------------- enforce (u)
 template <int D>
 __device__ __forceinline__ double lerpFace(const GridDev &g, const double *a, const double *p)

@@ -66,6 +66,8 @@ void launchFillU8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
 void launchCopy(double *dst, const double *src, size_t n, hipStream_t s);
 void launchEnforceLevelSet(const Fields &F, double *sdf, double tol, hipStream_t s);
 void launchGravity(const Fields &F, double dt, hipStream_t s);
+// Enright deformation times coeff on the face arrays (3D)
+void launchEnrightVelocity(const Fields &F, double coeff, hipStream_t s);
 void launchExtrapolateCollider(const Fields &F, hipStream_t s);
 void launchExtrapolateAir(const Fields &F, hipStream_t s);
 void launchEnforceBoundary(const Fields &F, hipStream_t s);

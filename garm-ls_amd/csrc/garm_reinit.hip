@@ -4,10 +4,14 @@
 // One launch per Runge-Kutta stage; the stage combinations of solve<5>
 // (:70-78) are fused into the stage that produces them, so an iteration is
 // three passes:  s0 = L(phi);  s1 = (3 phi + L(s0)) / 4;  phi = (phi + 2 L(s1)) / 3.
-// The FP32 form computes in float and keeps s0, s1 in float; phi stays FP64.
+// The FP32 form computes in float (constant quotients as products with
+// rounded reciprocals, fused multiply-adds) and keeps s0, s1 in float; phi
+// stays FP64.
 // Its WENO epsilon floor is 1e-18 instead of 1e-99 (which is 0 in float and
 // would give 0/0 on flat data); in FP64 it is the reference's 1e-99.
 #include "garm_internal.hpp"
+
+#include <type_traits>
 
 namespace garm {
 
@@ -49,6 +53,72 @@ __device__ __forceinline__ void weno5(const T *v, T invDx, T &dm, T &dp)
 	}
 }
 
+// The FP64 form evaluates the reference's expressions in the reference's
+// order (divisions by the constants, left to right) without contracting
+// a * b + c into fused multiply-adds, i.e. the IEEE operations of the CPU
+// code: the front where the reinitialisation meets the flat air band
+// amplifies the last-bit differences of any other order by eight orders of
+// magnitude.
+__device__ __forceinline__ double sqExact(double x)
+{
+#pragma clang fp contract(off)
+	return x * x;
+}
+
+__device__ __forceinline__ void weno5Exact(const double *v, double invDx, double &dm, double &dp)
+{
+#pragma clang fp contract(off)
+#pragma unroll
+	for (int i = 0; i < 2; i++) {
+		double dev[5];
+		if (i) { dev[0] = v[6] - v[5]; dev[1] = v[5] - v[4]; dev[2] = v[4] - v[3]; dev[3] = v[3] - v[2]; dev[4] = v[2] - v[1]; }
+		else { dev[0] = v[1] - v[0]; dev[1] = v[2] - v[1]; dev[2] = v[3] - v[2]; dev[3] = v[4] - v[3]; dev[4] = v[5] - v[4]; }
+		double eps = 0;
+#pragma unroll
+		for (int j = 0; j < 5; j++) eps = fmax(eps, dev[j] * dev[j]);
+		eps = eps * 1e-6 + 1e-99;
+		const double phix1 = dev[0] / 3 - dev[1] * 7 / 6 + dev[2] * 11 / 6;
+		const double phix2 = -dev[1] / 6 + dev[2] * 5 / 6 + dev[3] / 3;
+		const double phix3 = dev[2] / 3 + dev[3] * 5 / 6 - dev[4] / 6;
+		const double s1 = sqExact(dev[0] - 2 * dev[1] + dev[2]) * 13 / 12 + sqExact(dev[0] - 4 * dev[1] + 3 * dev[2]) / 4;
+		const double s2 = sqExact(dev[1] - 2 * dev[2] + dev[3]) * 13 / 12 + sqExact(dev[1] - dev[3]) / 4;
+		const double s3 = sqExact(dev[2] - 2 * dev[3] + dev[4]) * 13 / 12 + sqExact(3 * dev[2] - 4 * dev[3] + dev[4]) / 4;
+		const double a1 = .1 / sqExact(s1 + eps);
+		const double a2 = .6 / sqExact(s2 + eps);
+		const double a3 = .3 / sqExact(s3 + eps);
+		const double sum = a1 + a2 + a3;
+		(i ? dp : dm) = (a1 * phix1 + a2 * phix2 + a3 * phix3) / sum * invDx;
+	}
+}
+
+template <int D>
+__device__ __forceinline__ double advanceCellExact(const GridDev &g, const double *f, const Coord &c, size_t id, double dtau)
+{
+#pragma clang fp contract(off)
+	double dm[3], dp[3];
+#pragma unroll
+	for (int a = 0; a < D; a++) {
+		double v[7];
+#pragma unroll
+		for (int t = 0; t < 7; t++) {
+			const int o = t - 3;
+			v[t] = f[gat<D>(g, c.i + (a == 0 ? o : 0), c.j + (a == 1 ? o : 0), c.k + (a == 2 ? o : 0))];
+		}
+		weno5Exact(v, g.inv, dm[a], dp[a]);
+	}
+	double sum = 0;
+	if (f[id] < 0) {
+#pragma unroll
+		for (int a = 0; a < D; a++) { sum += sqExact(fmin(dm[a], 0.0)); sum += sqExact(fmax(dp[a], 0.0)); }
+	} else {
+#pragma unroll
+		for (int a = 0; a < D; a++) { sum += sqExact(fmax(dm[a], 0.0)); sum += sqExact(fmin(dp[a], 0.0)); }
+	}
+	sum = sqrt(sum);
+	const double val = f[id], sp = g.dx * sum;
+	return val + dtau * (val / sqrt(val * val + sp * sp)) * (1 - sum);
+}
+
 // Reinitialization::advance (Reinitialization.h:20-48) at one cell, in T,
 // reading the stage input from storage of type TI.
 template <int D, typename T, typename TI>
@@ -86,10 +156,18 @@ __global__ __launch_bounds__(kBlock) void kReinitStage(GridDev g, const TI *in, 
 	const Coord c = threadCoord();
 	if (!inGrid<D>(g, c)) return;
 	const size_t id = gidx<D>(g, c.i, c.j, c.k);
-	const T l = advanceCell<D, T, TI>(g, in, c, id, T(dtau));
-	if constexpr (MODE == 0) out[id] = TO(l);
-	else if constexpr (MODE == 1) out[id] = TO((T(phi[id]) * 3 + l) / 4);
-	else phi[id] = (T(phi[id]) + l * 2) / 3;
+	if constexpr (std::is_same<T, double>::value) {
+#pragma clang fp contract(off)
+		const double l = advanceCellExact<D>(g, in, c, id, dtau);
+		if constexpr (MODE == 0) out[id] = l;
+		else if constexpr (MODE == 1) out[id] = (phi[id] * 3 + l) / 4;
+		else phi[id] = (phi[id] + l * 2) / 3;
+	} else {
+		const T l = advanceCell<D, T, TI>(g, in, c, id, T(dtau));
+		if constexpr (MODE == 0) out[id] = TO(l);
+		else if constexpr (MODE == 1) out[id] = TO((T(phi[id]) * 3 + l) / 4);
+		else phi[id] = (T(phi[id]) + l * 2) / 3;
+	}
 }
 
 template <int D, typename T>

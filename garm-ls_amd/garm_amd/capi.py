@@ -25,6 +25,9 @@ K_ADVECT_REFMAP, K_ADVECT_VELOCITY, K_COMPOSE, K_REINIT_STAGE, K_PCG_SPMV, K_PCG
 KERNEL_NAMES = ["advect_refmap", "advect_velocity", "compose", "reinit_stage", "pcg_spmv", "pcg_update", "pcg32_spmv",
                 "pcg32_update", "copy"]
 
+# garm_velocity_field
+VELOCITY_ENRIGHT = 0
+
 # garm_volume_order
 VOLUME_REFERENCE, VOLUME_CORRECTED = 0, 1
 
@@ -113,6 +116,7 @@ def lib():
             "garm_apply_pressure_system": (i, [vp, vp, vp]),
             "garm_volume": (i, [vp, i, dp]),
             "garm_set_precision": (i, [vp, i]),
+            "garm_set_velocity_field": (i, [vp, i, d, d]),
             "garm_synchronize": (i, [vp]),
             "garm_time_kernel": (i, [vp, i, i, dp, dp]),
             "garm_unet_create": (i, [ctypes.POINTER(vp), ctypes.c_ulonglong]),
@@ -263,6 +267,10 @@ class Liquid:
         st = StepStats()
         self._check(lib().garm_step(self.h, dt, step_index, ctypes.byref(opts), ctypes.byref(st)))
         return st.as_dict()
+
+    def set_velocity_field(self, field, time, period):
+        """Analytic face velocity (VELOCITY_ENRIGHT) at time t with period T."""
+        self._check(lib().garm_set_velocity_field(self.h, field, time, period))
 
     def set_precision(self, precision):
         """FP64 or FP32 compute for the per-operator advection / reinit calls."""

@@ -103,7 +103,8 @@ typedef struct garm_pcg_stats {
 typedef enum garm_precision { GARM_FP64 = 0, GARM_FP32 = 1 } garm_precision;
 
 typedef struct garm_step_opts {
-	int reinit_interval;     /* WENO5 reinitialisation every k steps (reference: every step, k=1) */
+	int reinit_interval;     /* WENO5 reinitialisation every k steps (reference: every step, k=1);
+	                            a step that restarts the reference map reinitialises first */
 	int reinit_iterations;   /* reference: 30 (LevelSetLiquid.cpp:1156) */
 	double reinit_cfl;       /* reference: 0.9 */
 	int advect_precision;    /* garm_precision of advection, composition and reinit
@@ -194,6 +195,19 @@ int garm_apply_pressure_system(garm_ctx *ctx, const double *x, double *b);
 /* SGridBasedData::normMax of the velocity (SGridBasedData.h:48-52), used by
  * getTimeStep (LevelSetLiquid.h:129). */
 int garm_max_speed(garm_ctx *ctx, double *out);
+/* Analytic velocity fields of the reference's advection tests. */
+typedef enum garm_velocity_field {
+	GARM_VELOCITY_ENRIGHT = 0    /* Enright deformation on [0,1]^3 (TestAdvect3D.cpp:152-190):
+	                                u = 2 sin^2(pi x) sin(2 pi y) sin(2 pi z) cos(pi t / T),
+	                                v = -sin^2(pi y) sin(2 pi x) sin(2 pi z) cos(pi t / T),
+	                                w = -sin^2(pi z) sin(2 pi x) sin(2 pi y) cos(pi t / T) */
+} garm_velocity_field;
+
+/* Sets the face velocity of a 3D context to an analytic field at time t
+ * with period T, on the device (the reference's tests evaluate it per
+ * sample, TestAdvect3D.cpp:152-190, 390-397). */
+int garm_set_velocity_field(garm_ctx *ctx, int field, double time, double period);
+
 /* Vertex order of the 3D volume evaluator. */
 typedef enum garm_volume_order {
 	GARM_VOLUME_REFERENCE = 0,   /* lexicographic stencil passed to the circular-order

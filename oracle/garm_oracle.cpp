@@ -1156,6 +1156,10 @@ struct Liquid {
 		extrapolateCollider();
 		*restarted = 0;
 		if (!checkRefMap(0.883022221559489, 4, 1e-5)) {
+			// the reference reinitialises every step, so a restart always
+			// captures a reinitialised level set; with a longer cadence the
+			// restart step reinitialises first
+			if (!doReinit) reinit(reinitIters, cfl);
 			restart();
 			compose();
 			extrapolateCollider();

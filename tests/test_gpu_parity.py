@@ -233,3 +233,38 @@ def test_mixed_step_error_against_fp64(dim, n):
     for f, bound in ((capi.PRESSURE, 1e-3), (capi.VELOCITY, 1e-3)):
         e = rel_linf(d.get(f), o.get(f))
         assert e <= bound, f"{capi.FIELD_NAMES[f]}: {e:.2e}"
+
+
+def test_c1_dam_break_2d_128_per_step_and_volume():
+    """Config C1: 2D dam break on a 128^2 grid, 200 steps, WENO5 every 5
+    steps.  Every step starts from the oracle's state (per-step parity at
+    1e-10, the oracle's pressure solved to 1e-14), and a free GPU trajectory
+    at the configuration's tolerance (1e-10) keeps the oracle's volume drift
+    within 0.1% of the initial area."""
+    g = scenes.grid(128, 2)
+    o, d, free = make_oracle(g), make_gpu(g), make_gpu(g)
+    for s in (o,):
+        scenes.dam_break(s, g)
+    assert o.initialize(tol=1e-14, max_it=200000) == 0
+    copy_state(o, free)
+    a0o, a0f = o.volume(), free.volume()
+    step_opts = capi.step_opts(reinit_interval=5, tol=1e-13, max_iterations=200000)
+    free_opts = capi.step_opts(reinit_interval=5, tol=1e-10, max_iterations=200000)
+    clk = FrameClock(g["dx"], 1.0)
+    worst, worst_drift = {}, 0.0
+    for s in range(200):
+        copy_state(o, d)
+        dt = clk.next_dt(o.max_speed())
+        ro = o.step(dt, do_reinit=(s % 5 == 0), tol=1e-14, max_it=200000)
+        rd = d.step(dt, s, step_opts)
+        free.step(dt, s, free_opts)
+        assert ro["rc"] == 0 and rd["restarted"] == ro["restarted"]
+        for f in (capi.PHI, capi.PHI_GRAD, capi.VELOCITY, capi.PRESSURE):
+            worst[f] = max(worst.get(f, 0.0), rel_linf(d.get(f), o.get(f)))
+        worst_drift = max(worst_drift, abs((free.volume() - a0f) / a0f - (o.volume() - a0o) / a0o))
+        if s % 50 == 49:
+            say(f"C1 step {s + 1}: worst per-step rel Linf " + ", ".join(f"{capi.FIELD_NAMES[f]} {e:.1e}" for f, e in worst.items())
+                + f"; free-run area drift difference {worst_drift:.1e}")
+    for f, e in worst.items():
+        assert e <= TOL, f"{capi.FIELD_NAMES[f]}: {e:.2e}"
+    assert worst_drift <= 1e-3
