@@ -5,6 +5,8 @@ Workload: 3D dam break, n^3 MAC grid (default 256^3 cells including a 2-cell
 solid band; liquid box x < 0.4, y < 0.6, z < 0.5 of the unit domain), gravity
 -9.8, CFL 1, frame rate 25 (the reference's dt rule), WENO5 reinitialisation
 (30 iterations) every 5 steps, pressure solve to relative residual 1e-6.
+--scene droplet runs the droplet-into-pool scene instead (pool y < 0.3,
+sphere r = 0.08 at a seeded (x, 0.7, z), falling at 2).
 A step is one LevelSetLiquid::advance (LevelSetLiquid.cpp:608-698) with every
 field resident in HBM.
 
@@ -43,7 +45,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-MODES = {"fp64": (capi.PCG_FP64, capi.FP64), "mixed": (capi.PCG_MIXED, capi.FP32), "learned": (capi.PCG_LEARNED, capi.FP32)}
+MODES = {"fp64": (capi.PCG_FP64, capi.FP64), "mixed": (capi.PCG_MIXED, capi.FP32), "learned": (capi.PCG_LEARNED, capi.FP32),
+         "multigrid": (capi.PCG_MULTIGRID, capi.FP32)}
+
+
+def build_scene(sim, g, args, rng=None):
+    """The benchmark scene, or with `rng` a randomised one for training:
+    dam-break box extents U[0.2, 0.8]; droplet positions seeded."""
+    if args.scene == "droplet":
+        return scenes.droplet(sim, g, seed=args.scene_seed if rng is None else int(rng.integers(1 << 30)))
+    ext = (0.4, 0.6, 0.5) if rng is None else tuple(float(x) for x in rng.uniform(0.2, 0.8, size=3))
+    scenes.dam_break(sim, g, ext)
+    return ext
 
 
 def train_network(args):
@@ -54,17 +67,16 @@ def train_network(args):
     t0 = time.time()
     R, P, M = [], [], []
     for run in range(args.train_runs):
-        ext = tuple(float(x) for x in rng.uniform(0.2, 0.8, size=3))
         g = scenes.grid(args.train_n, 3)
         sim = capi.Liquid(3, g["res"], g["dx"], g["origin"], g["bw"])
-        scenes.dam_break(sim, g, ext)
+        ext = build_scene(sim, g, args, rng)
         sim.initialize(capi.step_opts(mode=capi.PCG_FP64, tol=1e-8, max_iterations=args.max_iterations))
         r, p, m = training.collect_pairs(sim, g, args.train_pairs, tol=1e-8, reinit_interval=args.reinit_interval)
         R.append(r)
         P.append(p)
         M.append(m)
         sim.close()
-        log(f"[bench] training run {run + 1}/{args.train_runs}: box {tuple(round(e, 3) for e in ext)}, {len(r)} pairs")
+        log(f"[bench] training run {run + 1}/{args.train_runs}: {args.scene} {ext}, {len(r)} pairs")
     R, P, M = np.concatenate(R), np.concatenate(P), np.concatenate(M)
     t_collect = time.time() - t0
     net = capi.Unet(seed=args.train_seed)
@@ -88,7 +100,7 @@ def run_gpu(args, mode_name, net=None):
     sim = capi.Liquid(3, g["res"], g["dx"], g["origin"], g["bw"])
     if net is not None:
         sim.attach_preconditioner(net)
-    scenes.dam_break(sim, g)
+    build_scene(sim, g, args)
     init = capi.step_opts(reinit_interval=args.reinit_interval, tol=args.tol, max_iterations=args.max_iterations,
                           check_interval=args.check_interval, mode=capi.PCG_FP64)
     sim.initialize(init)
@@ -200,6 +212,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--scene", choices=["dam_break", "droplet"], default="dam_break")
+    ap.add_argument("--scene-seed", type=int, default=19260817, help="droplet position seed")
     ap.add_argument("--mode", choices=list(MODES), default="learned", help="headline configuration")
     ap.add_argument("--also", default="fp64,mixed", help="configurations run beside the headline one ('' for none)")
     ap.add_argument("--tol", type=float, default=1e-6)
@@ -248,7 +262,10 @@ def main():
         "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.mode == "fp64" else "f32+f64+bf16" if args.mode == "learned" else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"3D dam break full step, {args.n}^3 MAC grid (liquid box x<0.4, y<0.6, z<0.5, gravity -9.8, CFL {args.cfl:g})",
+        "config": {"workload": (f"3D dam break full step, {args.n}^3 MAC grid (liquid box x<0.4, y<0.6, z<0.5, gravity -9.8, CFL {args.cfl:g})"
+                                if args.scene == "dam_break" else
+                                f"3D droplet into pool full step, {args.n}^3 MAC grid (pool y<0.3, sphere r=0.08 at seeded (x,0.7,z), "
+                                f"u=(0,-2,0) in the drop, gravity -9.8, CFL {args.cfl:g}, seed {args.scene_seed})"),
                    "mode": args.mode, "tolerance": args.tol, "reinit_interval": args.reinit_interval, "cells": cells},
         "phases_ms": head["phases_ms"],
         "pcg_iterations": head["pcg_iterations"],

@@ -137,7 +137,10 @@ int doProject(garm_ctx *c, const garm_pcg_opts &o, garm_pcg_stats *st)
 {
 	garm_pcg_stats local{};
 	if (!st) st = &local;
-	if (o.mode != GARM_PCG_FP64 && o.mode != GARM_PCG_MIXED && o.mode != GARM_PCG_LEARNED) return fail(c, GARM_EINVAL, "unknown PCG mode");
+	if (o.mode != GARM_PCG_FP64 && o.mode != GARM_PCG_MIXED && o.mode != GARM_PCG_LEARNED && o.mode != GARM_PCG_MULTIGRID)
+		return fail(c, GARM_EINVAL, "unknown PCG mode");
+	if (o.mode == GARM_PCG_LEARNED && o.fallback_mode != 0 && o.fallback_mode != GARM_PCG_MIXED && o.fallback_mode != GARM_PCG_MULTIGRID)
+		return fail(c, GARM_EINVAL, "unknown fallback for the learned solve");
 	if (!(o.tolerance > 0) || o.max_iterations < 0) return fail(c, GARM_EINVAL, "bad PCG tolerance or iteration limit");
 	std::string err;
 	const int rc = runProjection(c->F, c->P, o, st, c->s, err);

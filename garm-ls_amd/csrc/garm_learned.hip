@@ -18,11 +18,13 @@
 // the current residual; the FP64 residual b - A x judges the result either
 // way.
 #include "garm_learned.hpp"
+#include "garm_multigrid.hpp"
 
 #include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <random>
 #include <vector>
 
@@ -195,7 +197,7 @@ __global__ __launch_bounds__(kBlock) void kTrueResidual(Rows<double> A, int n, c
 }
 
 // bounding box of the unknowns' cells
-__global__ void kBBox(const int *mat2grid, int n, int nx, int ny, int *box)
+__global__ __launch_bounds__(kBlock) void kBBox(const int *mat2grid, int n, int nx, int ny, int *box)
 {
 	int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
 	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -204,7 +206,24 @@ __global__ void kBBox(const int *mat2grid, int n, int nx, int ny, int *box)
 		mn[0] = min(mn[0], x); mn[1] = min(mn[1], y); mn[2] = min(mn[2], z);
 		mx[0] = max(mx[0], x); mx[1] = max(mx[1], y); mx[2] = max(mx[2], z);
 	}
-	for (int a = 0; a < 3; a++) { atomicMin(&box[a], mn[a]); atomicMax(&box[3 + a], mx[a]); }
+	// reduce in the workgroup (wave shuffles, then LDS), one atomic per corner component
+	__shared__ int red[6][kBlock / kWave];
+	const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+	for (int a = 0; a < 3; a++) {
+		for (int off = kWave / 2; off > 0; off >>= 1) {
+			mn[a] = min(mn[a], __shfl_xor(mn[a], off));
+			mx[a] = max(mx[a], __shfl_xor(mx[a], off));
+		}
+		if (lane == 0) { red[a][wid] = mn[a]; red[3 + a][wid] = mx[a]; }
+	}
+	__syncthreads();
+	if (threadIdx.x < 6) {
+		int v = red[threadIdx.x][0];
+		for (int w = 1; w < kBlock / kWave; w++) v = threadIdx.x < 3 ? min(v, red[threadIdx.x][w]) : max(v, red[threadIdx.x][w]);
+		if (threadIdx.x < 3) atomicMin(&box[threadIdx.x], v);
+		else atomicMax(&box[threadIdx.x], v);
+	}
 }
 
 __global__ void kBBoxInit(int *box)
@@ -250,34 +269,38 @@ static int readHost(LearnedWork &W, const double *src, int count, hipStream_t s,
 	return GARM_OK;
 }
 
-template <int D>
-static int learned(const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *grid2mat, const int *mat2grid,
-	const GridDev &cell, const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Unet &U, hipStream_t s, garm_pcg_stats *st, std::string &err)
+// Bounding box of the unknowns' cells (inclusive corners).
+static int unknownBox(const int *mat2grid, int n, const GridDev &cell, LearnedWork &W, hipStream_t s, int lo[3], int hi[3], std::string &err)
 {
-	if (!W.reserve(size_t(n), err)) return GARM_ENOMEM;
-	const int nb = lblocks(n);
-	const dim3 g1(nb), b1(kBlock);
-	int rc;
-	// network box: bounding box of the unknowns, two cells of context, multiples of four
 	hipLaunchKernelGGL(kBBoxInit, dim3(1), dim3(64), 0, s, W.box);
-	hipLaunchKernelGGL(kBBox, g1, b1, 0, s, mat2grid, n, cell.n[0], cell.n[1], W.box);
+	hipLaunchKernelGGL(kBBox, dim3(lblocks(n)), dim3(kBlock), 0, s, mat2grid, n, cell.n[0], cell.n[1], W.box);
 	int hb[6];
 	if (hipMemcpyAsync(hb, W.box, sizeof(hb), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
 		err = "bounding-box readback failed";
 		return GARM_EHIP;
 	}
-	UnetBox B{};
-	for (int a = 0; a < 3; a++) {
-		const bool live = a < D;
-		const int lo = live ? hb[a] - 2 : 0, hi = live ? hb[3 + a] + 2 : 0;
-		B.org[a] = lo;
-		B.n[a] = live ? (hi - lo + 1 + 3) / 4 * 4 : 4;
-		B.gn[a] = cell.n[a];
-	}
-	const size_t boxVox = size_t(B.n[0]) * B.n[1] * B.n[2];
-	if (!U.buf.reserve(std::max(boxVox + boxVox / 4, U.buf.cap_vox), U.buf.has_grad, err) || !U.buf.setDims(B.n[0], B.n[1], B.n[2], err))
-		return GARM_ENOMEM;
+	for (int a = 0; a < 3; a++) { lo[a] = hb[a]; hi[a] = hb[3 + a]; }
+	return GARM_OK;
+}
 
+// Flexible CG on the FP64 system with preconditioner `precondition` (writes
+// W.z = M(W.r)).  Stall rule: with `perApplication` every application must
+// halve the residual, otherwise the residual must halve every kStallWindow
+// iterations.  After the loop the FP64 residual b - A x judges the result and
+// what is left goes to the mixed-precision Jacobi solver.
+constexpr int kStallWindow = 10;
+
+template <int D>
+static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const garm_pcg_opts &o, Krylov &K,
+	LearnedWork &W, hipStream_t s, garm_pcg_stats *st, std::string &err, const std::function<void()> &first, bool perApplication,
+	int firstBudget, const std::function<void()> &second, const char *what)
+{
+	std::function<void()> precondition = first;
+	const int budget = second ? o.max_iterations : firstBudget;
+	int switchedAt = -1;
+	const int nb = lblocks(n);
+	const dim3 g1(nb), b1(kBlock);
+	int rc;
 	// x = 0, r = b, |b|^2
 	hipLaunchKernelGGL(kZeroD, g1, b1, 0, s, W.x, n);
 	hipLaunchKernelGGL(kCopyD, g1, b1, 0, s, W.r, b, n);
@@ -292,19 +315,9 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 	if (bb == 0) { st->relative_residual = 0; return GARM_OK; }
 	const double tol = std::max(o.tolerance, 2.220446049250313e-16 / std::sqrt(bb));
 	const double thr = tol * tol * bb;
-	// z = M(r): two-level cycle, network on the pre-smoothed residual
-	auto precondition = [&]() {
-		hipLaunchKernelGGL(kJacobi, g1, b1, 0, s, (const double *)A64.invdiag, n, (const double *)W.r, W.z);
-		hipLaunchKernelGGL(kTrueResidual<D>, g1, b1, 0, s, A64, n, (const double *)W.r, (const double *)W.z, W.t, W.pb);
-		hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nb, W.ls + kLsSmooth);
-		unetApplyCompact(U, B, grid2mat, W.t, W.ls + kLsSmooth, n, W.z, s);
-		hipLaunchKernelGGL(kPostSmooth<D>, g1, b1, 0, s, A64, n, (const double *)W.r, (const double *)W.z, W.t);
-		std::swap(W.z, W.t);
-	};
 	precondition();
 	hipLaunchKernelGGL(kCopyD, g1, b1, 0, s, W.p, (const double *)W.z, n);
-	double rrPrev = bb;
-	const int budget = std::min(o.max_iterations, 400);
+	std::vector<double> hist(1, bb);
 	int it = 0;
 	for (; it < budget; it++) {
 		hipLaunchKernelGGL(kFcgSpmv<D>, g1, b1, 0, s, A64, n, (const double *)W.p, W.q, (const double *)W.r, W.pa);
@@ -313,16 +326,29 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 		if ((rc = readHost(W, W.ls, 4, s, err))) return rc;
 		const double rr = W.host[kLsRr];
 		if (!std::isfinite(rr)) break;
+		hist.push_back(rr);
 		if (rr < thr) { it++; break; }
-		// stalled: this application did not halve the residual
-		if (rr > kMinReduction * kMinReduction * rrPrev) { it++; break; }
-		rrPrev = rr;
+		const bool onFirst = switchedAt < 0;
+		bool stalled;
+		if (onFirst && perApplication) stalled = rr > kMinReduction * kMinReduction * hist[hist.size() - 2];   // did not halve
+		else {
+			const size_t from = onFirst ? 0 : size_t(switchedAt) + 1;
+			stalled = hist.size() > from + kStallWindow && rr > kMinReduction * kMinReduction * hist[hist.size() - 1 - kStallWindow];
+		}
+		if (stalled || (onFirst && it + 1 >= firstBudget)) {
+			if (!onFirst || !second) { it++; break; }
+			// the first preconditioner has stopped paying: the same iteration
+			// continues with the second one
+			precondition = second;
+			switchedAt = int(hist.size()) - 1;
+			st->fell_back = 1;
+		}
 		precondition();
 		hipLaunchKernelGGL(kFcgDot, g1, b1, 0, s, n, (const double *)W.z, (const double *)W.q, W.pa);
 		hipLaunchKernelGGL(kFcgP, g1, b1, 0, s, n, W.p, (const double *)W.z, (const double *)W.pa, (const double *)W.ls);
 	}
 	st->iterations = it;
-	st->preconditioner_applications = it;
+	st->preconditioner_applications = switchedAt < 0 ? it : switchedAt;
 	// the FP64 residual judges the result; fall back to plain PCG on what is left
 	if (!std::isfinite(W.host[kLsRr])) hipLaunchKernelGGL(kZeroD, g1, b1, 0, s, W.x, n);
 	hipLaunchKernelGGL(kTrueResidual<D>, g1, b1, 0, s, A64, n, b, (const double *)W.x, W.b2, W.pb);
@@ -332,6 +358,7 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 	if (!(rt < thr)) {
 		st->fell_back = 1;
 		garm_pcg_opts fo = o;
+		fo.mode = GARM_PCG_MIXED;
 		fo.tolerance = std::sqrt(thr / rt) * 0.999;
 		fo.max_iterations = std::max(1, o.max_iterations - it);
 		garm_pcg_stats fs{};
@@ -346,15 +373,90 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 		rt = W.host[kLsTrue];
 	}
 	st->relative_residual = std::sqrt(rt / bb);
-	if (!(rt < thr) && st->relative_residual > tol) { err = "learned solve and its fallback did not reach the tolerance"; return GARM_ENOCONV; }
+	if (!(rt < thr) && st->relative_residual > tol) { err = std::string(what) + " and its fallback did not reach the tolerance"; return GARM_ENOCONV; }
 	return GARM_OK;
 }
 
-int solveLearned(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *grid2mat, const int *mat2grid,
-	const GridDev &cell, const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Unet &U, hipStream_t s, garm_pcg_stats *st, std::string &err)
+template <int D>
+static int learned(const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *grid2mat, const int *mat2grid,
+	const GridDev &cell, const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Unet &U, Multigrid &M, hipStream_t s, garm_pcg_stats *st,
+	std::string &err)
 {
-	return D == 2 ? learned<2>(A64, A32, b, n, grid2mat, mat2grid, cell, o, K, W, U, s, st, err)
-	              : learned<3>(A64, A32, b, n, grid2mat, mat2grid, cell, o, K, W, U, s, st, err);
+	if (!W.reserve(size_t(n), err)) return GARM_ENOMEM;
+	const int nb = lblocks(n);
+	const dim3 g1(nb), b1(kBlock);
+	int rc, lo[3], hi[3];
+	if ((rc = unknownBox(mat2grid, n, cell, W, s, lo, hi, err))) return rc;
+	// network box: bounding box of the unknowns, two cells of context, multiples of four
+	UnetBox B{};
+	for (int a = 0; a < 3; a++) {
+		const bool live = a < D;
+		const int l = live ? lo[a] - 2 : 0, h = live ? hi[a] + 2 : 0;
+		B.org[a] = l;
+		B.n[a] = live ? (h - l + 1 + 3) / 4 * 4 : 4;
+		B.gn[a] = cell.n[a];
+	}
+	const size_t boxVox = size_t(B.n[0]) * B.n[1] * B.n[2];
+	if (!U.buf.reserve(std::max(boxVox + boxVox / 4, U.buf.cap_vox), U.buf.has_grad, err) || !U.buf.setDims(B.n[0], B.n[1], B.n[2], err))
+		return GARM_ENOMEM;
+	// z = M(r): two-level cycle, network on the pre-smoothed residual
+	auto precondition = [&]() {
+		hipLaunchKernelGGL(kJacobi, g1, b1, 0, s, (const double *)A64.invdiag, n, (const double *)W.r, W.z);
+		hipLaunchKernelGGL(kTrueResidual<D>, g1, b1, 0, s, A64, n, (const double *)W.r, (const double *)W.z, W.t, W.pb);
+		hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nb, W.ls + kLsSmooth);
+		unetApplyCompact(U, B, grid2mat, W.t, W.ls + kLsSmooth, n, W.z, s);
+		hipLaunchKernelGGL(kPostSmooth<D>, g1, b1, 0, s, A64, n, (const double *)W.r, (const double *)W.z, W.t);
+		std::swap(W.z, W.t);
+	};
+	std::function<void()> second;
+	if (o.fallback_mode == GARM_PCG_MULTIGRID) {
+		int org[3], dims[3], gn[3];
+		for (int a = 0; a < 3; a++) {
+			const bool live = a < D;
+			org[a] = live ? lo[a] : 0;
+			dims[a] = live ? hi[a] - lo[a] + 1 : 1;
+			gn[a] = cell.n[a];
+		}
+		if (!M.layout(D, org, dims, gn, err)) return GARM_ENOMEM;
+		mgBuild(M, A32, n, mat2grid, s);
+		second = [&]() { mgApply(M, mat2grid, n, W.r, W.z, s); };
+	}
+	return flexibleCg<D>(A64, A32, b, n, o, K, W, s, st, err, precondition, true, std::min(o.max_iterations, 400), second, "learned solve");
+}
+
+template <int D>
+static int multigrid(const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *mat2grid, const GridDev &cell,
+	const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Multigrid &M, hipStream_t s, garm_pcg_stats *st, std::string &err)
+{
+	if (!W.reserve(size_t(n), err)) return GARM_ENOMEM;
+	int rc, lo[3], hi[3];
+	if ((rc = unknownBox(mat2grid, n, cell, W, s, lo, hi, err))) return rc;
+	int org[3], dims[3], gn[3];
+	for (int a = 0; a < 3; a++) {
+		const bool live = a < D;
+		org[a] = live ? lo[a] : 0;
+		dims[a] = live ? hi[a] - lo[a] + 1 : 1;
+		gn[a] = cell.n[a];
+	}
+	if (!M.layout(D, org, dims, gn, err)) return GARM_ENOMEM;
+	mgBuild(M, A32, n, mat2grid, s);
+	auto precondition = [&]() { mgApply(M, mat2grid, n, W.r, W.z, s); };
+	return flexibleCg<D>(A64, A32, b, n, o, K, W, s, st, err, precondition, false, o.max_iterations, nullptr, "multigrid solve");
+}
+
+int solveLearned(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *grid2mat, const int *mat2grid,
+	const GridDev &cell, const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Unet &U, Multigrid &M, hipStream_t s, garm_pcg_stats *st,
+	std::string &err)
+{
+	return D == 2 ? learned<2>(A64, A32, b, n, grid2mat, mat2grid, cell, o, K, W, U, M, s, st, err)
+	              : learned<3>(A64, A32, b, n, grid2mat, mat2grid, cell, o, K, W, U, M, s, st, err);
+}
+
+int solveMultigrid(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *mat2grid,
+	const GridDev &cell, const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Multigrid &M, hipStream_t s, garm_pcg_stats *st, std::string &err)
+{
+	return D == 2 ? multigrid<2>(A64, A32, b, n, mat2grid, cell, o, K, W, M, s, st, err)
+	              : multigrid<3>(A64, A32, b, n, mat2grid, cell, o, K, W, M, s, st, err);
 }
 
 // ------------------------------------------------------------- training

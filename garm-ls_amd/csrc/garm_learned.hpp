@@ -4,6 +4,7 @@
 #include "garm_common.hpp"
 #include "garm_pcg.hpp"
 #include "garm_unet.hpp"
+#include "garm_multigrid.hpp"
 #include "../../include/garm_gpu.h"
 
 #include <string>
@@ -24,7 +25,13 @@ struct LearnedWork {
 // Solves A x = b; the solution is left in W.x.  grid2mat / mat2grid map the
 // cell grid `cell` to the compact unknowns.
 int solveLearned(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *grid2mat, const int *mat2grid,
-	const GridDev &cell, const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Unet &U, hipStream_t s, garm_pcg_stats *st, std::string &err);
+	const GridDev &cell, const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Unet &U, Multigrid &M, hipStream_t s, garm_pcg_stats *st,
+	std::string &err);
+
+// Solves A x = b by flexible CG with the multigrid V-cycle preconditioner
+// (FP32 cycle, FP64 Krylov vectors and residual); the solution is left in W.x.
+int solveMultigrid(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *mat2grid,
+	const GridDev &cell, const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Multigrid &M, hipStream_t s, garm_pcg_stats *st, std::string &err);
 
 // The network behind a C-ABI handle and the device it was created on.
 Unet *unetOf(garm_unet *net, int *device);

@@ -1,0 +1,41 @@
+// Multigrid V-cycle preconditioner for the ghost-fluid Poisson system.
+#pragma once
+
+#include "garm_common.hpp"
+#include "garm_pcg.hpp"
+
+#include <string>
+#include <vector>
+
+namespace garm {
+
+constexpr int kMgMaxLevels = 12;
+
+// Level grids over the bounding box of the unknowns.  Level 0 holds the
+// fine operator; level l+1 is its Galerkin coarsening by 2x2x2 aggregates
+// (piecewise-constant prolongation).  Per cell: diagonal and the weights
+// of its -x, -y, -z faces (operator (A x)_c = diag_c x_c - sum_faces w x_nb);
+// b, x are the level's right-hand side and iterate.  FP32 throughout.
+struct Multigrid {
+	int levels = 0;
+	int dims[kMgMaxLevels][3] = {};
+	size_t cells[kMgMaxLevels] = {};
+	float *diag[kMgMaxLevels] = {}, *wx[kMgMaxLevels] = {}, *wy[kMgMaxLevels] = {}, *wz[kMgMaxLevels] = {};
+	float *b[kMgMaxLevels] = {}, *x[kMgMaxLevels] = {};
+	int org[3] = {};          // box origin in cell coordinates
+	int gn[3] = {};           // cell-grid size
+	int D = 3;
+	float *pool = nullptr;    // one allocation for every level array
+	size_t pool_cap = 0;      // floats
+	~Multigrid();
+	// lays out the levels for a box of the given size (multiple solves reuse the pool)
+	bool layout(int D, const int box_org[3], const int box_n[3], const int grid_n[3], std::string &err);
+};
+
+// Level 0 from the FP32 rows of the n unknowns, then every coarser level.
+void mgBuild(Multigrid &M, const Rows<float> &A, int n, const int *mat2grid, hipStream_t s);
+// z = V(r): one symmetric V-cycle (red-black Gauss-Seidel, two sweeps down
+// and up) applied to the compact residual r; z is compact, FP64.
+void mgApply(Multigrid &M, const int *mat2grid, int n, const double *r, double *z, hipStream_t s);
+
+} // namespace garm

@@ -245,8 +245,8 @@ static void assemble(const Fields &F, Projection &P, int n, bool mixed, hipStrea
 template <int D>
 static int projectD(Fields &F, Projection &P, const garm_pcg_opts &o, garm_pcg_stats *st, hipStream_t s, std::string &err)
 {
-	const bool learned = o.mode == GARM_PCG_LEARNED;
-	const bool mixed = o.mode == GARM_PCG_MIXED || learned;
+	const bool learned = o.mode == GARM_PCG_LEARNED, mg = o.mode == GARM_PCG_MULTIGRID;
+	const bool mixed = o.mode == GARM_PCG_MIXED || learned || mg;
 	if (learned && !P.net) { err = "no preconditioner network is attached"; return GARM_EINVAL; }
 	if (!P.reserveGrid(F.N, err)) return GARM_ENOMEM;
 	const int n = numberUnknowns<D>(F, P, mixed, s, err);
@@ -261,8 +261,9 @@ static int projectD(Fields &F, Projection &P, const garm_pcg_opts &o, garm_pcg_s
 	int rc = GARM_OK;
 	if (n > 0) {
 		assemble<D>(F, P, n, mixed, s);
-		if (learned) {
-			rc = solveLearned(D, P.rows, P.rows32, P.rhs, n, P.grid2mat, P.mat2grid, F.cell, o, P.K, P.LW, *P.net, s, st, err);
+		if (learned || mg) {
+			rc = learned ? solveLearned(D, P.rows, P.rows32, P.rhs, n, P.grid2mat, P.mat2grid, F.cell, o, P.K, P.LW, *P.net, P.MG, s, st, err)
+			             : solveMultigrid(D, P.rows, P.rows32, P.rhs, n, P.mat2grid, F.cell, o, P.K, P.LW, P.MG, s, st, err);
 			if ((rc == GARM_OK || rc == GARM_ENOCONV) && hipMemcpyAsync(P.K.x, P.LW.x, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, s) != hipSuccess) {
 				err = "copying the learned solution failed";
 				rc = GARM_EHIP;

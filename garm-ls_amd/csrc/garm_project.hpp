@@ -26,6 +26,7 @@ struct Projection {
 	Krylov K;
 	Unet *net = nullptr;      // preconditioner of learned solves (not owned)
 	LearnedWork LW;
+	Multigrid MG;
 	size_t cap = 0;
 	int n = 0;
 

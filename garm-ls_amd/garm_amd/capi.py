@@ -32,7 +32,7 @@ VELOCITY_ENRIGHT = 0
 VOLUME_REFERENCE, VOLUME_CORRECTED = 0, 1
 
 # garm_pcg_mode
-PCG_FP64, PCG_MIXED, PCG_LEARNED = 0, 1, 2
+PCG_FP64, PCG_MIXED, PCG_LEARNED, PCG_MULTIGRID = 0, 1, 2, 3
 FP64, FP32 = 0, 1
 
 GARM_OK, GARM_EINVAL, GARM_EHIP, GARM_ENOCONV, GARM_ENOMEM, GARM_ENODEV = 0, -1, -2, -3, -4, -5
@@ -42,7 +42,7 @@ STATUS_NAMES = {0: "GARM_OK", -1: "GARM_EINVAL", -2: "GARM_EHIP", -3: "GARM_ENOC
 
 class PcgOpts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("tolerance", ctypes.c_double), ("max_iterations", ctypes.c_int),
-                ("check_interval", ctypes.c_int), ("inner_tolerance", ctypes.c_double)]
+                ("check_interval", ctypes.c_int), ("fallback_mode", ctypes.c_int)]
 
 
 class PcgStats(ctypes.Structure):
@@ -152,7 +152,7 @@ def _d3(v):
 
 
 def step_opts(reinit_interval=1, reinit_iterations=30, reinit_cfl=0.9, precision=FP64, mode=PCG_FP64, tol=1e-5,
-              max_iterations=2000, check_interval=8, inner_tolerance=0.0):
+              max_iterations=2000, check_interval=8, fallback_mode=0):
     o = StepOpts()
     o.reinit_interval = reinit_interval
     o.reinit_iterations = reinit_iterations
@@ -162,7 +162,7 @@ def step_opts(reinit_interval=1, reinit_iterations=30, reinit_cfl=0.9, precision
     o.pcg.tolerance = tol
     o.pcg.max_iterations = max_iterations
     o.pcg.check_interval = check_interval
-    o.pcg.inner_tolerance = inner_tolerance
+    o.pcg.fallback_mode = fallback_mode
     return o
 
 
@@ -253,8 +253,8 @@ class Liquid:
         self._check(lib().garm_restart(self.h))
 
     def project(self, mode=PCG_FP64, tol=1e-5, max_iterations=2000, check_interval=8, raise_on_noconv=True,
-                inner_tolerance=0.0):
-        o = PcgOpts(mode, tol, max_iterations, check_interval, inner_tolerance)
+                fallback_mode=0):
+        o = PcgOpts(mode, tol, max_iterations, check_interval, fallback_mode)
         st = PcgStats()
         rc = lib().garm_project(self.h, ctypes.byref(o), ctypes.byref(st))
         if rc != GARM_OK and (raise_on_noconv or rc != GARM_ENOCONV):

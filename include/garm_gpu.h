@@ -75,10 +75,15 @@ typedef enum garm_pcg_mode {
 	GARM_PCG_MIXED = 1,      /* FP32 Jacobi-PCG refined in FP64: solution accumulated in
 	                            FP64, residual replaced by the FP64 b - Ax at every 1e3
 	                            reduction, FP64 reductions, FP64 convergence test */
-	GARM_PCG_LEARNED = 2     /* flexible CG preconditioned by the U-Net attached with
-	                            garm_attach_preconditioner, falling back to
-	                            GARM_PCG_MIXED on the remaining residual when the
-	                            residual stalls; judged by the FP64 b - Ax */
+	GARM_PCG_LEARNED = 2,    /* flexible CG preconditioned by the U-Net attached with
+	                            garm_attach_preconditioner; when an application does not
+	                            halve the residual, fallback_mode takes over; judged by
+	                            the FP64 b - Ax */
+	GARM_PCG_MULTIGRID = 3   /* flexible CG preconditioned by an FP32 multigrid V-cycle
+	                            (Galerkin coarsening of the assembled operator on the
+	                            bounding box of the unknowns, red-black Gauss-Seidel),
+	                            FP64 Krylov vectors; falls back to GARM_PCG_MIXED if
+	                            the residual stops halving; judged by the FP64 b - Ax */
 } garm_pcg_mode;
 
 typedef struct garm_pcg_opts {
@@ -86,7 +91,10 @@ typedef struct garm_pcg_opts {
 	double tolerance;        /* relative residual ||b-Ax||/||b|| (reference: 1e-5, Projector.cpp:161) */
 	int max_iterations;      /* reference: 2000 */
 	int check_interval;      /* iterations between host convergence checks (>=1) */
-	double inner_tolerance;  /* reserved (0) */
+	int fallback_mode;       /* GARM_PCG_LEARNED only: what takes over when the network
+	                            stalls -- 0 or GARM_PCG_MIXED: the mixed Jacobi solver on
+	                            the remaining residual; GARM_PCG_MULTIGRID: the same
+	                            flexible CG continues with the multigrid V-cycle */
 } garm_pcg_opts;
 
 typedef struct garm_pcg_stats {

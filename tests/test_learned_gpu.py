@@ -127,3 +127,16 @@ def test_trained_network_gains_over_untrained_and_plain_direction(trained):
     say(f"  A-norm error after one step: trained {rho_trained:.3f}, untrained {rho_untrained:.3f}, z = b {rho_plain:.3f}")
     assert rho_trained <= 0.8
     assert 1.0 - rho_trained >= 10.0 * max(1.0 - rho_untrained, 1.0 - rho_plain)
+
+
+def test_learned_with_multigrid_fallback_converges_fast(trained):
+    g, net, state, dt = trained
+    d = _prepared(g, state, dt)
+    d.attach_preconditioner(net)
+    st = d.project(mode=capi.PCG_LEARNED, tol=1e-6, max_iterations=5000, fallback_mode=capi.PCG_MULTIGRID)
+    ref = _prepared(g, state, dt)
+    sj = ref.project(mode=capi.PCG_MIXED, tol=1e-6, max_iterations=5000)
+    say(f"  learned + multigrid: {st['iterations']} iterations ({st['preconditioner_applications']} with the network), "
+        f"mixed Jacobi {sj['iterations']}; rel residual {st['relative_residual']:.2e}")
+    assert st["rc"] == 0 and st["relative_residual"] <= 1e-6
+    assert st["iterations"] * 3 <= sj["iterations"]
