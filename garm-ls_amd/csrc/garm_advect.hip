@@ -1,11 +1,17 @@
 // Gradient-augmented reference-map advection, semi-Lagrangian velocity
 // advection and the phi / grad-phi composition for gfx950.
 //
-// One thread per cell (or face); the velocity gradient and the Hermite mixed
-// derivatives of the reference map are formed on the fly from the clamped
-// neighbours instead of being materialised as the reference does
-// (LevelSetLiquid.cpp:739-742, HermiteIntrpl.h:17-30), so each launch reads
-// u, xi and J once from HBM and writes xi', J' once.
+// One thread per cell (or face).  The interpolants gather from the 2^D
+// corners of the cell that holds the traced point, so the data each corner
+// contributes are first packed per cell / face into one contiguous record
+// (array of structures, 16-byte vector loads): the velocity with its
+// central-difference gradient per face, the Hermite data of the reference
+// map (value, first and mixed derivatives per component) per cell, and the
+// same for the reference level set.  A gather then costs one clamped index
+// and a few vector loads per corner instead of a dozen scalar loads with
+// their own clamped neighbour indices.  The packed values are the products
+// the reference's interpolants form (HermiteIntrpl.h:17-30, 62-87;
+// Derivatives.h:98-105), computed once per cell in the same operations.
 #include "garm_internal.hpp"
 
 #include <type_traits>
@@ -41,22 +47,65 @@ __device__ __forceinline__ void sampleVel(const FaceSet &U, const T *p, T *v)
 	}
 }
 
-// DrvVectorFieldWrapper<SGrid, Linear> (VectorField.h:106-138) with the
-// face-gradient of Derivatives::computeFirst<2> (Derivatives.h:47,51-57,98-105)
-// evaluated at each clamped corner: row a of G is grad u_a.  G row-major.
+// ------------------------------------------------------------------ packing
+template <typename T> struct Vec4;
+template <> struct Vec4<float> { using type = float4; };
+template <> struct Vec4<double> { using type = double4; };
+
+// S consecutive values of record q (S a multiple of four)
+template <typename T, int S>
+__device__ __forceinline__ void loadRecord(const T *base, size_t q, T (&v)[S])
+{
+	using V = typename Vec4<T>::type;
+	const V *p = reinterpret_cast<const V *>(base) + q * (S / 4);
+#pragma unroll
+	for (int i = 0; i < S / 4; i++) {
+		const V t = p[i];
+		v[4 * i] = t.x; v[4 * i + 1] = t.y; v[4 * i + 2] = t.z; v[4 * i + 3] = t.w;
+	}
+}
+
+template <typename T, int S>
+__device__ __forceinline__ void storeRecord(T *base, size_t q, const T (&v)[S])
+{
+	using V = typename Vec4<T>::type;
+	V *p = reinterpret_cast<V *>(base) + q * (S / 4);
+#pragma unroll
+	for (int i = 0; i < S / 4; i++) p[i] = V{v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]};
+}
+
+// Face record of axis a: u_a and its central differences along x, y, z
+// (0 for a missing axis) -- what sampleVelGrad reads at one corner.
 template <int D, typename T>
-__device__ __forceinline__ void sampleVelGrad(const FaceSet &U, const T *p, T *v, T *G)
+__global__ __launch_bounds__(kBlock) void kPackVelocity(GridDev g, const double *ua, T *out)
+{
+	const Coord c = threadCoord();
+	if (!inGrid<D>(g, c)) return;
+	const size_t id = gidx<D>(g, c.i, c.j, c.k);
+	const T inv = T(g.inv);
+	T v[4];
+	v[0] = T(ua[id]);
+	v[1] = T(ua[gat<D>(g, c.i + 1, c.j, c.k)] - ua[gat<D>(g, c.i - 1, c.j, c.k)]) / 2 * inv;
+	v[2] = T(ua[gat<D>(g, c.i, c.j + 1, c.k)] - ua[gat<D>(g, c.i, c.j - 1, c.k)]) / 2 * inv;
+	v[3] = D == 3 ? T(ua[gat<D>(g, c.i, c.j, c.k + 1)] - ua[gat<D>(g, c.i, c.j, c.k - 1)]) / 2 * inv : T(0);
+	storeRecord<T, 4>(out, id, v);
+}
+
+// DrvVectorFieldWrapper<SGrid, Linear> (VectorField.h:106-138): per axis a,
+// the multilinear sample of u_a and of its face gradient
+// (Derivatives::computeFirst<2>, Derivatives.h:47,51-57,98-105) taken at each
+// clamped corner, from the face records.  Row a of G is grad u_a (row-major).
+template <int D, typename T>
+__device__ __forceinline__ void sampleVelGradPacked(const FaceSet &U, const T *const *rec, const T *p, T *v, T *G)
 {
 #pragma unroll
 	for (int a = 0; a < D; a++) {
 		const GridDev &g = U.g[a];
-		const double *ua = U.a[a];
 		int l[3]; T f[3];
 		lowerFracT<D, T>(g, p, l, f);
 		T w[2][3];
 #pragma unroll
 		for (int b = 0; b < D; b++) { w[0][b] = 1 - f[b]; w[1][b] = f[b]; }
-		const T inv = T(g.inv);
 		T val = 0, gr[3] = {0, 0, 0};
 #pragma unroll
 		for (int k = 0; k < (D == 3 ? 2 : 1); k++)
@@ -65,14 +114,11 @@ __device__ __forceinline__ void sampleVelGrad(const FaceSet &U, const T *p, T *v
 #pragma unroll
 				for (int i = 0; i < 2; i++) {
 					const T wt = D == 2 ? w[i][0] * w[j][1] : w[i][0] * w[j][1] * w[k][2];
-					const int qi = clampi(l[0] + i, g.n[0] - 1), qj = clampi(l[1] + j, g.n[1] - 1);
-					const int qk = D == 3 ? clampi(l[2] + k, g.n[2] - 1) : 0;
-					val += T(ua[gidx<D>(g, qi, qj, qk)]) * wt;
-					// neighbour differences taken on the FP64 data, then rounded
-					gr[0] += (T(ua[gat<D>(g, qi + 1, qj, qk)] - ua[gat<D>(g, qi - 1, qj, qk)]) / 2 * inv) * wt;
-					gr[1] += (T(ua[gat<D>(g, qi, qj + 1, qk)] - ua[gat<D>(g, qi, qj - 1, qk)]) / 2 * inv) * wt;
-					if constexpr (D == 3)
-						gr[2] += (T(ua[gat<D>(g, qi, qj, qk + 1)] - ua[gat<D>(g, qi, qj, qk - 1)]) / 2 * inv) * wt;
+					T d[4];
+					loadRecord<T, 4>(rec[a], gat<D>(g, l[0] + i, l[1] + j, D == 3 ? l[2] + k : 0), d);
+					val += d[0] * wt;
+#pragma unroll
+					for (int b = 0; b < D; b++) gr[b] += d[1 + b] * wt;
 				}
 		v[a] = val;
 #pragma unroll
@@ -92,19 +138,78 @@ __device__ __forceinline__ void hermiteBasis(T x, T B[2][2], T Bg[2][2])
 	B[1][1] = x * x * (x - 1);                 Bg[1][1] = x * (3 * x - 2);
 }
 
-// HermiteIntrpl<D, VectorDd>::interpolate + interpolateGradient
-// (HermiteIntrpl.h:32-58, getValues :62-87) of the reference map with the
-// mixed derivatives of its constructor (:17-30) formed from J at the clamped
-// neighbours of each clamped corner.  Gh is column-major: Gh[c*D + r].
-// In float the map is interpolated as the displacement xi - x: the data at a
-// corner become xi - x_corner (x_corner of the unclamped corner) and J - I,
-// taken in FP64 before rounding, and X and I are added back.  Cubic Hermite
-// reproduces the identity map exactly, so this is the same interpolant
-// without the cancellation of positions of size 1 in gradients of size 1/dx.
+// Hermite record of the reference map at one cell (S = 2^D orders x D
+// components, rec[ord * D + r]), in the getValues order of HermiteIntrpl
+// (HermiteIntrpl.h:62-87): 0 value, 1 d/dx, 2 d/dy, 3 dxy, 4 d/dz, 5 dxz,
+// 6 dyz, 7 dxyz, each scaled by dx per derivative order; the mixed
+// derivatives are the central differences of J of the constructor
+// (:17-30).  In float the map is stored as the displacement: xi - x_cell and
+// J - I, taken in FP64 before rounding (cubic Hermite reproduces the
+// identity map exactly, so the interpolant of the displacement plus the
+// traced point is the same interpolant without the cancellation of
+// positions of size 1 against gradients of size 1/dx).
+template <int D>
+struct RefMapFields {
+	const double *xi[3];
+	const double *J[9];
+};
+
 template <int D, typename T>
-__device__ __forceinline__ void hermiteRefMap(const GridDev &g, const double *const *xi, const double *const *J,
-	const T *X, T *val, T *Gh)
+__global__ __launch_bounds__(kBlock) void kPackRefMap(GridDev g, RefMapFields<D> A, T *out)
 {
+	constexpr int S = (1 << D) * D;
+	constexpr bool kDisp = !std::is_same<T, double>::value;
+	const Coord c = threadCoord();
+	if (!inGrid<D>(g, c)) return;
+	const size_t q = gidx<D>(g, c.i, c.j, c.k);
+	const int qi = c.i, qj = c.j, qk = c.k;
+	const int pos[3] = {qi, qj, qk};
+	const T dx = T(g.dx), inv = T(g.inv);
+	T rec[S];
+#pragma unroll
+	for (int r = 0; r < D; r++) {
+		T v[8];
+		if constexpr (kDisp) {
+			v[0] = T(A.xi[r][q] - (g.o[r] + pos[r] * g.dx));
+			v[1] = T(A.J[0 * D + r][q] - (r == 0)) * dx;
+			v[2] = T(A.J[1 * D + r][q] - (r == 1)) * dx;
+		} else {
+			v[0] = T(A.xi[r][q]);
+			v[1] = T(A.J[0 * D + r][q]) * dx;
+			v[2] = T(A.J[1 * D + r][q]) * dx;
+		}
+		{
+			const double *c1 = A.J[1 * D + r];
+			const T m = T(c1[gat<D>(g, qi + 1, qj, qk)] - c1[gat<D>(g, qi - 1, qj, qk)]) / 2 * inv;
+			v[3] = m * dx * dx;
+		}
+		if constexpr (D == 3) {
+			const double *c2 = A.J[2 * D + r];
+			v[4] = kDisp ? T(c2[q] - (r == 2)) * dx : T(c2[q]) * dx;
+			T m = T(c2[gat<D>(g, qi + 1, qj, qk)] - c2[gat<D>(g, qi - 1, qj, qk)]) / 2 * inv;
+			v[5] = m * dx * dx;
+			m = T(c2[gat<D>(g, qi, qj + 1, qk)] - c2[gat<D>(g, qi, qj - 1, qk)]) / 2 * inv;
+			v[6] = m * dx * dx;
+			m = T(c2[gat<D>(g, qi + 1, qj + 1, qk)] - c2[gat<D>(g, qi - 1, qj + 1, qk)]
+				- c2[gat<D>(g, qi + 1, qj - 1, qk)] + c2[gat<D>(g, qi - 1, qj - 1, qk)]) / 4 * inv * inv;
+			v[7] = m * dx * dx * dx;
+		}
+#pragma unroll
+		for (int ord = 0; ord < (1 << D); ord++) rec[ord * D + r] = v[ord];
+	}
+	storeRecord<T, S>(out, q, rec);
+}
+
+// HermiteIntrpl<D, VectorDd>::interpolate + interpolateGradient
+// (HermiteIntrpl.h:32-58) of the reference map from the cell records.
+// Gh is column-major: Gh[c*D + r].  In float a corner clamped into the grid
+// keeps its own position in the displacement (the reference reads the
+// clamped corner's data at the unclamped corner).
+template <int D, typename T>
+__device__ __forceinline__ void hermiteRefMap(const GridDev &g, const T *recs, const T *X, T *val, T *Gh)
+{
+	constexpr int S = (1 << D) * D;
+	constexpr bool kDisp = !std::is_same<T, double>::value;
 	int l[3] = {0, 0, 0}; T f[3];
 	lowerFracT<D, T>(g, X, l, f);
 	T B[3][2][2], Bg[3][2][2];
@@ -114,15 +219,20 @@ __device__ __forceinline__ void hermiteRefMap(const GridDev &g, const double *co
 	for (int r = 0; r < D; r++) val[r] = 0;
 #pragma unroll
 	for (int e = 0; e < D * D; e++) Gh[e] = 0;
-	const T dx = T(g.dx), inv = T(g.inv);
-	constexpr bool kDisp = !std::is_same<T, double>::value;
+	const T inv = T(g.inv);
 
 #pragma unroll
 	for (int cn = 0; cn < (1 << D); cn++) {
 		const int vx = cn & 1, vy = cn >> 1 & 1, vz = D == 3 ? cn >> 2 : 0;
-		const int qi = clampi(l[0] + vx, g.n[0] - 1), qj = clampi(l[1] + vy, g.n[1] - 1);
-		const int qk = D == 3 ? clampi(l[2] + vz, g.n[2] - 1) : 0;
-		const size_t q = gidx<D>(g, qi, qj, qk);
+		const int lv[3] = {l[0] + vx, l[1] + vy, l[2] + vz};
+		const int qv[3] = {clampi(lv[0], g.n[0] - 1), clampi(lv[1], g.n[1] - 1), D == 3 ? clampi(lv[2], g.n[2] - 1) : 0};
+		T rec[S];
+		loadRecord<T, S>(recs, gidx<D>(g, qv[0], qv[1], qv[2]), rec);
+		if constexpr (kDisp) {
+#pragma unroll
+			for (int r = 0; r < D; r++)
+				if (qv[r] != lv[r]) rec[r] += T((qv[r] - lv[r]) * g.dx);
+		}
 #pragma unroll
 		for (int ord = 0; ord < (1 << D); ord++) {
 			const int ox = ord & 1, oy = ord >> 1 & 1, oz = D == 3 ? ord >> 2 : 0;
@@ -139,35 +249,7 @@ __device__ __forceinline__ void hermiteRefMap(const GridDev &g, const double *co
 			}
 #pragma unroll
 			for (int r = 0; r < D; r++) {
-				T v;
-				// getValues ordering: 0 value, 1 d/dx, 2 d/dy, 3 dxy, 4 d/dz, 5 dxz, 6 dyz, 7 dxyz
-				if constexpr (kDisp) {
-					const int lv[3] = {l[0] + vx, l[1] + vy, l[2] + vz};
-					if (ord == 0) v = T(xi[r][q] - (g.o[r] + lv[r] * g.dx));
-					else if (ord == 1) v = T(J[0 * D + r][q] - (r == 0)) * dx;
-					else if (ord == 2) v = T(J[1 * D + r][q] - (r == 1)) * dx;
-					else if (ord == 4) v = T(J[2 * D + r][q] - (r == 2)) * dx;
-				} else {
-					if (ord == 0) v = T(xi[r][q]);
-					else if (ord == 1) v = T(J[0 * D + r][q]) * dx;
-					else if (ord == 2) v = T(J[1 * D + r][q]) * dx;
-					else if (ord == 4) v = T(J[2 * D + r][q]) * dx;
-				}
-				if (ord == 3) {
-					const T m = T(J[1 * D + r][gat<D>(g, qi + 1, qj, qk)] - J[1 * D + r][gat<D>(g, qi - 1, qj, qk)]) / 2 * inv;
-					v = m * dx * dx;
-				} else if (ord == 5) {
-					const T m = T(J[2 * D + r][gat<D>(g, qi + 1, qj, qk)] - J[2 * D + r][gat<D>(g, qi - 1, qj, qk)]) / 2 * inv;
-					v = m * dx * dx;
-				} else if (ord == 6) {
-					const T m = T(J[2 * D + r][gat<D>(g, qi, qj + 1, qk)] - J[2 * D + r][gat<D>(g, qi, qj - 1, qk)]) / 2 * inv;
-					v = m * dx * dx;
-				} else if (ord == 7) {
-					const double *c2 = J[2 * D + r];
-					const T m = T(c2[gat<D>(g, qi + 1, qj + 1, qk)] - c2[gat<D>(g, qi - 1, qj + 1, qk)]
-						- c2[gat<D>(g, qi + 1, qj - 1, qk)] + c2[gat<D>(g, qi - 1, qj - 1, qk)]) / 4 * inv * inv;
-					v = m * dx * dx * dx;
-				}
+				const T v = rec[ord * D + r];
 				val[r] += v * wt;
 #pragma unroll
 				for (int c = 0; c < D; c++) Gh[c * D + r] += v * wg[c];
@@ -183,12 +265,12 @@ __device__ __forceinline__ void hermiteRefMap(const GridDev &g, const double *co
 }
 
 // ------------------------------------------------------------------ kernels
-template <int D>
+template <int D, typename T>
 struct RefMapArgs {
 	GridDev cell;
 	FaceSet U;
-	const double *xi[3];
-	const double *J[9];
+	const T *vrec[3];   // face records of u
+	const T *hrec;      // Hermite records of the reference map
 	double *xi_new[3];
 	double *J_new[9];
 	double h;   // -dt
@@ -197,7 +279,7 @@ struct RefMapArgs {
 // SemiLagrangian<D, 2, HermiteIntrpl<D, VectorDd>>::advect (drv overload,
 // Advection.h:147-160) with Advection::traceDrv<2> (Advection.h:71-82).
 template <int D, typename T>
-__global__ __launch_bounds__(kBlock) void kAdvectRefMap(RefMapArgs<D> A)
+__global__ __launch_bounds__(kBlock) void kAdvectRefMap(RefMapArgs<D, T> A)
 {
 	const Coord c = threadCoord();
 	if (!inGrid<D>(A.cell, c)) return;
@@ -206,14 +288,14 @@ __global__ __launch_bounds__(kBlock) void kAdvectRefMap(RefMapArgs<D> A)
 	T x[3] = {T(A.cell.o[0] + c.i * A.cell.dx), T(A.cell.o[1] + c.j * A.cell.dx), T(A.cell.o[2] + c.k * A.cell.dx)};
 
 	T v0[3], G0[9], p1[3], R1[9], v1[3], G1[9], X[3], R[9];
-	sampleVelGrad<D, T>(A.U, x, v0, G0);
+	sampleVelGradPacked<D, T>(A.U, A.vrec, x, v0, G0);
 #pragma unroll
 	for (int a = 0; a < D; a++) p1[a] = x[a] + v0[a] * h;
 #pragma unroll
 	for (int a = 0; a < D; a++)
 #pragma unroll
 		for (int b = 0; b < D; b++) R1[a * D + b] = (a == b ? T(1) : T(0)) + G0[a * D + b] * h;
-	sampleVelGrad<D, T>(A.U, p1, v1, G1);
+	sampleVelGradPacked<D, T>(A.U, A.vrec, p1, v1, G1);
 #pragma unroll
 	for (int a = 0; a < D; a++) X[a] = x[a] + (v0[a] + v1[a]) * h / 2;
 #pragma unroll
@@ -227,7 +309,7 @@ __global__ __launch_bounds__(kBlock) void kAdvectRefMap(RefMapArgs<D> A)
 		}
 
 	T val[3], Gh[9];
-	hermiteRefMap<D, T>(A.cell, A.xi, A.J, X, val, Gh);
+	hermiteRefMap<D, T>(A.cell, A.hrec, X, val, Gh);
 #pragma unroll
 	for (int r = 0; r < D; r++) A.xi_new[r][id] = val[r];
 #pragma unroll
@@ -289,14 +371,12 @@ __global__ __launch_bounds__(kBlock) void kAdvectVelocity(FaceSet U, int axis, d
 	out[gidx<D>(g, c.i, c.j, c.k)] = val;
 }
 
-template <int D>
+template <int D, typename T>
 struct ComposeArgs {
 	GridDev cell;
 	const double *xi[3];
 	const double *J[9];
-	const double *phi_ref;
-	const double *phi_ref_grad[3];
-	const double *mixed[4];
+	const T *rrec;         // Hermite records of the reference level set
 	const uint8_t *adv;
 	const double *cell_dist;
 	double *phi, *phi_grad[3], *phi_tmp, *phi_ori;
@@ -309,7 +389,7 @@ struct ComposeArgs {
 // HermiteIntrpl<D, double>, ScalarField.h:50-69), air clamp,
 // enforceBoundaryLevelSet at 3 dx and 0 (:966-975), phi_tmp, phi_ori.
 template <int D, typename T>
-__global__ __launch_bounds__(kBlock) void kCompose(ComposeArgs<D> A)
+__global__ __launch_bounds__(kBlock) void kCompose(ComposeArgs<D, T> A)
 {
 	const GridDev &g = A.cell;
 	const Coord c = threadCoord();
@@ -323,23 +403,12 @@ __global__ __launch_bounds__(kBlock) void kCompose(ComposeArgs<D> A)
 	T B[3][2][2], Bg[3][2][2];
 #pragma unroll
 	for (int a = 0; a < D; a++) hermiteBasis(f[a], B[a], Bg[a]);
-	const T dx = T(g.dx);
 	T val = 0, gr[3] = {0, 0, 0};
 #pragma unroll
 	for (int cn = 0; cn < (1 << D); cn++) {
 		const int vx = cn & 1, vy = cn >> 1 & 1, vz = D == 3 ? cn >> 2 : 0;
-		const size_t q = gat<D>(g, l[0] + vx, l[1] + vy, D == 3 ? l[2] + vz : 0);
-		T v[8];
-		v[0] = T(A.phi_ref[q]);
-		v[1] = T(A.phi_ref_grad[0][q]) * dx;
-		v[2] = T(A.phi_ref_grad[1][q]) * dx;
-		v[3] = T(A.mixed[0][q]) * dx * dx;
-		if constexpr (D == 3) {
-			v[4] = T(A.phi_ref_grad[2][q]) * dx;
-			v[5] = T(A.mixed[1][q]) * dx * dx;
-			v[6] = T(A.mixed[2][q]) * dx * dx;
-			v[7] = T(A.mixed[3][q]) * dx * dx * dx;
-		}
+		T v[1 << D];
+		loadRecord<T, (1 << D)>(A.rrec, gat<D>(g, l[0] + vx, l[1] + vy, D == 3 ? l[2] + vz : 0), v);
 #pragma unroll
 		for (int ord = 0; ord < (1 << D); ord++) {
 			const int ox = ord & 1, oy = ord >> 1 & 1, oz = D == 3 ? ord >> 2 : 0;
@@ -377,23 +446,44 @@ __global__ __launch_bounds__(kBlock) void kCompose(ComposeArgs<D> A)
 	A.phi_tmp[id] = phi < -cd - 0.0 ? -cd - 0.0 : phi;
 }
 
-// HermiteIntrpl<D, double> constructor (HermiteIntrpl.h:17-30) for the
-// reference level set; rebuilt whenever _refLevelSetGrad changes.
+// Hermite records of the reference level set for kCompose, in FP64 and
+// FP32: HermiteIntrpl<D, double> constructor (HermiteIntrpl.h:17-30: mixed
+// derivatives as central differences of the stored gradient) and getValues
+// (:62-87) order 0 value, 1 d/dx, 2 d/dy, 3 dxy, 4 d/dz, 5 dxz, 6 dyz, 7 dxyz,
+// scaled by dx per derivative order.  Rebuilt whenever _refLevelSet or
+// _refLevelSetGrad changes.
 template <int D>
-__global__ __launch_bounds__(kBlock) void kScalarMixed(GridDev g, const double *g0, const double *g1, const double *g2,
-	double *m0, double *m1, double *m2, double *m3)
+__global__ __launch_bounds__(kBlock) void kPackRefLevelSet(GridDev g, const double *phi_ref, const double *g0, const double *g1,
+	const double *g2, double *out64, float *out32)
 {
 	const Coord c = threadCoord();
 	if (!inGrid<D>(g, c)) return;
 	const size_t id = gidx<D>(g, c.i, c.j, c.k);
-	const double inv = g.inv;
-	m0[id] = (g1[gat<D>(g, c.i + 1, c.j, c.k)] - g1[gat<D>(g, c.i - 1, c.j, c.k)]) / 2 * inv;
+	const double inv = g.inv, dx = g.dx;
+	double m[4] = {0, 0, 0, 0};
+	m[0] = (g1[gat<D>(g, c.i + 1, c.j, c.k)] - g1[gat<D>(g, c.i - 1, c.j, c.k)]) / 2 * inv;
 	if constexpr (D == 3) {
-		m1[id] = (g2[gat<D>(g, c.i + 1, c.j, c.k)] - g2[gat<D>(g, c.i - 1, c.j, c.k)]) / 2 * inv;
-		m2[id] = (g2[gat<D>(g, c.i, c.j + 1, c.k)] - g2[gat<D>(g, c.i, c.j - 1, c.k)]) / 2 * inv;
-		m3[id] = (g2[gat<D>(g, c.i + 1, c.j + 1, c.k)] - g2[gat<D>(g, c.i - 1, c.j + 1, c.k)]
+		m[1] = (g2[gat<D>(g, c.i + 1, c.j, c.k)] - g2[gat<D>(g, c.i - 1, c.j, c.k)]) / 2 * inv;
+		m[2] = (g2[gat<D>(g, c.i, c.j + 1, c.k)] - g2[gat<D>(g, c.i, c.j - 1, c.k)]) / 2 * inv;
+		m[3] = (g2[gat<D>(g, c.i + 1, c.j + 1, c.k)] - g2[gat<D>(g, c.i - 1, c.j + 1, c.k)]
 			- g2[gat<D>(g, c.i + 1, c.j - 1, c.k)] + g2[gat<D>(g, c.i - 1, c.j - 1, c.k)]) / 4 * inv * inv;
 	}
+	constexpr int S = 1 << D;
+	double r64[S];
+	float r32[S];
+	const float fdx = float(dx);
+	r64[0] = phi_ref[id];        r32[0] = float(phi_ref[id]);
+	r64[1] = g0[id] * dx;        r32[1] = float(g0[id]) * fdx;
+	r64[2] = g1[id] * dx;        r32[2] = float(g1[id]) * fdx;
+	r64[3] = m[0] * dx * dx;     r32[3] = float(m[0]) * fdx * fdx;
+	if constexpr (D == 3) {
+		r64[4] = g2[id] * dx;        r32[4] = float(g2[id]) * fdx;
+		r64[5] = m[1] * dx * dx;     r32[5] = float(m[1]) * fdx * fdx;
+		r64[6] = m[2] * dx * dx;     r32[6] = float(m[2]) * fdx * fdx;
+		r64[7] = m[3] * dx * dx * dx; r32[7] = float(m[3]) * fdx * fdx * fdx;
+	}
+	storeRecord<double, S>(out64, id, r64);
+	storeRecord<float, S>(out32, id, r32);
 }
 
 // Derivatives::computeFirst<4> (Derivatives.h:48, 51-57, 98-105).
@@ -435,14 +525,32 @@ using Real = float;
 using Real = double;
 #endif
 
+// Face records of u, then the reference-map records, then the trace.  The
+// records live in F.adv_rec: cell records first, face records of each axis
+// after them, each part 256-byte aligned.
 template <int D>
 static void advectRefMapD(const Fields &F, double dt, hipStream_t s)
 {
-	RefMapArgs<D> A{};
+	constexpr int S = (1 << D) * D;
+	RefMapArgs<D, Real> A{};
 	A.cell = F.cell;
 	A.U = faceSet(F);
-	for (int a = 0; a < D; a++) { A.xi[a] = F.xi[a]; A.xi_new[a] = F.xi_new[a]; }
-	for (int e = 0; e < D * D; e++) { A.J[e] = F.J[e]; A.J_new[e] = F.J_new[e]; }
+	char *base = static_cast<char *>(F.adv_rec);
+	size_t off = (F.N * S * sizeof(Real) + 255) / 256 * 256;
+	for (int a = 0; a < D; a++) {
+		Real *rec = reinterpret_cast<Real *>(base + off);
+		hipLaunchKernelGGL((kPackVelocity<D, Real>), gridDims(F.face[a], D), gridBlock(), 0, s, F.face[a], (const double *)F.u[a], rec);
+		A.vrec[a] = rec;
+		off += (F.Nf[a] * 4 * sizeof(Real) + 255) / 256 * 256;
+	}
+	RefMapFields<D> R{};
+	for (int a = 0; a < D; a++) R.xi[a] = F.xi[a];
+	for (int e = 0; e < D * D; e++) R.J[e] = F.J[e];
+	Real *hrec = reinterpret_cast<Real *>(base);
+	hipLaunchKernelGGL((kPackRefMap<D, Real>), gridDims(F.cell, D), gridBlock(), 0, s, F.cell, R, hrec);
+	A.hrec = hrec;
+	for (int a = 0; a < D; a++) A.xi_new[a] = F.xi_new[a];
+	for (int e = 0; e < D * D; e++) A.J_new[e] = F.J_new[e];
 	A.h = -dt;
 	hipLaunchKernelGGL((kAdvectRefMap<D, Real>), gridDims(F.cell, D), gridBlock(), 0, s, A);
 }
@@ -457,12 +565,11 @@ static void advectVelocityD(const Fields &F, double dt, hipStream_t s)
 template <int D>
 static void composeD(const Fields &F, hipStream_t s)
 {
-	ComposeArgs<D> A{};
+	ComposeArgs<D, Real> A{};
 	A.cell = F.cell;
-	for (int a = 0; a < D; a++) { A.xi[a] = F.xi[a]; A.phi_ref_grad[a] = F.phi_ref_grad[a]; A.phi_grad[a] = F.phi_grad[a]; }
+	for (int a = 0; a < D; a++) { A.xi[a] = F.xi[a]; A.phi_grad[a] = F.phi_grad[a]; }
 	for (int e = 0; e < D * D; e++) A.J[e] = F.J[e];
-	for (int m = 0; m < 4; m++) A.mixed[m] = F.phi_ref_mixed[m];
-	A.phi_ref = F.phi_ref;
+	if constexpr (std::is_same<Real, double>::value) A.rrec = F.ref_rec64; else A.rrec = F.ref_rec32;
 	A.adv = F.adv_region;
 	A.cell_dist = F.cell_dist;
 	A.phi = F.phi; A.phi_tmp = F.phi_tmp; A.phi_ori = F.phi_ori;
@@ -515,15 +622,15 @@ void launchCompose(const Fields &F, hipStream_t s, bool fp32)
 	else composeD<3>(F, s);
 }
 
-void launchRefLevelSetMixed(const Fields &F, hipStream_t s)
+void launchRefLevelSetRecords(const Fields &F, hipStream_t s)
 {
 	const double *g = F.phi_ref_grad[0];
 	if (F.D == 2)
-		hipLaunchKernelGGL(kScalarMixed<2>, gridDims(F.cell, 2), gridBlock(), 0, s, F.cell, g, F.phi_ref_grad[1], g,
-			F.phi_ref_mixed[0], nullptr, nullptr, nullptr);
+		hipLaunchKernelGGL(kPackRefLevelSet<2>, gridDims(F.cell, 2), gridBlock(), 0, s, F.cell, (const double *)F.phi_ref, g,
+			(const double *)F.phi_ref_grad[1], g, F.ref_rec64, F.ref_rec32);
 	else
-		hipLaunchKernelGGL(kScalarMixed<3>, gridDims(F.cell, 3), gridBlock(), 0, s, F.cell, g, F.phi_ref_grad[1], F.phi_ref_grad[2],
-			F.phi_ref_mixed[0], F.phi_ref_mixed[1], F.phi_ref_mixed[2], F.phi_ref_mixed[3]);
+		hipLaunchKernelGGL(kPackRefLevelSet<3>, gridDims(F.cell, 3), gridBlock(), 0, s, F.cell, (const double *)F.phi_ref, g,
+			(const double *)F.phi_ref_grad[1], (const double *)F.phi_ref_grad[2], F.ref_rec64, F.ref_rec32);
 }
 
 void launchGradient4(const Fields &F, const double *src, double *const *dst, hipStream_t s)

@@ -25,6 +25,7 @@ struct garm_ctx {
 	std::vector<void *> allocs;
 	hipEvent_t ev[8] = {};
 	double *host_scalars = nullptr;   // pinned
+	size_t bface_rec_cap = 0;         // listed boundary faces F.bface_rec holds room for
 	bool fp32 = false;                // compute precision of the per-operator advection / reinit calls
 };
 
@@ -171,14 +172,39 @@ void doRestart(garm_ctx *c)
 	Fields &F = c->F;
 	launchCopy(F.phi_ref, F.phi, F.N, c->s);
 	launchGradient4(F, F.phi_ref, F.phi_ref_grad, c->s);
-	launchRefLevelSetMixed(F, c->s);
+	launchRefLevelSetRecords(F, c->s);
 	launchResetRefMap(F, c->s);
 }
 
 void doEnforceBoundary(garm_ctx *c)
 {
 	launchEnforceBoundary(c->F, c->s);
-	swapFaces(c->F);
+}
+
+// Lists the faces Boundary::enforce acts on and their solid normal and
+// velocity; after the boundary fields are set or replaced.
+int rebuildBoundaryFaces(garm_ctx *c)
+{
+	Fields &F = c->F;
+	int rc, n = 0;
+	if ((rc = hipCheck(c, hipMemsetAsync(F.front_cnt + 3, 0, sizeof(int), c->s), "boundary faces"))) return rc;
+	launchBoundaryFaces(F, F.bface, F.front_cnt + 3, c->s);
+	if ((rc = launched(c, "boundary faces"))) return rc;
+	if ((rc = hipCheck(c, hipMemcpyAsync(&n, F.front_cnt + 3, sizeof(int), hipMemcpyDeviceToHost, c->s), "boundary faces"))) return rc;
+	if ((rc = hipCheck(c, hipStreamSynchronize(c->s), "boundary faces"))) return rc;
+	if (size_t(n) > c->bface_rec_cap) {
+		if (F.bface_rec) (void)hipFree(F.bface_rec);
+		F.bface_rec = nullptr;
+		c->bface_rec_cap = 0;
+		void *p = nullptr;
+		if (hipMalloc(&p, size_t(n) * 8 * sizeof(double)) != hipSuccess) return fail(c, GARM_ENOMEM, "boundary face records");
+		F.bface_rec = static_cast<double *>(p);
+		c->bface_rec_cap = size_t(n);
+	}
+	F.nbface = n;
+	launchBoundaryRecords(F, c->s);
+	if ((rc = launched(c, "boundary faces"))) return rc;
+	return hipCheck(c, hipStreamSynchronize(c->s), "boundary faces");
 }
 
 } // namespace
@@ -249,10 +275,19 @@ int garm_create(garm_ctx **out, int dim, const int res[3], double dx, const doub
 		size_t NFall = 0;
 		for (int a = 0; a < dim; a++) NFall += F.Nf[a];
 		const size_t cap = std::max(N, NFall);
-		ok = ok && alloc(c, F.front_list[0], cap) && alloc(c, F.front_list[1], cap) && alloc(c, F.front_cnt, 4);
+		ok = ok && alloc(c, F.front_list[0], cap) && alloc(c, F.front_list[1], cap) && alloc(c, F.front_cnt, 4)
+			&& alloc(c, F.bface, NFall);
 	}
 	for (int e = 0; e < dim * dim && ok; e++) ok = alloc(c, F.J[e], N) && alloc(c, F.J_new[e], N);
-	for (int m = 0; m < (dim == 3 ? 4 : 1) && ok; m++) ok = alloc(c, F.phi_ref_mixed[m], N);
+	ok = ok && alloc(c, F.ref_rec64, N << dim) && alloc(c, F.ref_rec32, N << dim);
+	{
+		// advection records, sized for FP64: 2^D x D per cell, 4 per face
+		size_t bytes = (N * (size_t(dim) << dim) * sizeof(double) + 255) / 256 * 256;
+		for (int a = 0; a < dim; a++) bytes += (F.Nf[a] * 4 * sizeof(double) + 255) / 256 * 256;
+		char *rec = nullptr;
+		ok = ok && alloc(c, rec, bytes);
+		F.adv_rec = rec;
+	}
 	// reduction partials: two per workgroup of a cell-grid launch, or three
 	// times 1024 for the |u|max reduction, whichever is larger
 	{
@@ -290,9 +325,10 @@ int garm_create(garm_ctx **out, int dim, const int res[3], double dx, const doub
 		for (double *p : {F.u[a], F.u_new[a], F.bvel[a]}) launchFill(p, 0.0, F.Nf[a], c->s);
 	}
 	for (int e = 0; e < dim * dim; e++) { launchFill(F.J[e], 0.0, N, c->s); launchFill(F.J_new[e], 0.0, N, c->s); }
-	for (int m = 0; m < (dim == 3 ? 4 : 1); m++) launchFill(F.phi_ref_mixed[m], 0.0, N, c->s);
+	launchRefLevelSetRecords(F, c->s);
 	launchFillU8(F.adv_region, 0, N, c->s);
 	ok = ok && hipGetLastError() == hipSuccess && hipStreamSynchronize(c->s) == hipSuccess;
+	ok = ok && rebuildBoundaryFaces(c) == GARM_OK;
 	if (!ok) {
 		g_create_error = "initial upload failed";
 		garm_destroy(c);
@@ -308,6 +344,7 @@ void garm_destroy(garm_ctx *c)
 	if (c->s) (void)hipStreamSynchronize(c->s);
 	c->P.release();
 	for (void *p : c->allocs) (void)hipFree(p);
+	if (c->F.bface_rec) (void)hipFree(c->F.bface_rec);
 	for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
 	if (c->host_scalars) (void)hipHostFree(c->host_scalars);
 	if (c->s) (void)hipStreamDestroy(c->s);
@@ -335,7 +372,8 @@ int garm_upload(garm_ctx *c, int f, const void *host, size_t n)
 				return rc;
 			off += F.Nf[a];
 		}
-		return hipCheck(c, hipStreamSynchronize(c->s), "upload");
+		if ((rc = hipCheck(c, hipStreamSynchronize(c->s), "upload"))) return rc;
+		return f == GARM_VELOCITY ? GARM_OK : rebuildBoundaryFaces(c);
 	}
 	double *comp[9];
 	const int nc = components(F, f, comp);
@@ -346,9 +384,9 @@ int garm_upload(garm_ctx *c, int f, const void *host, size_t n)
 		for (size_t i = 0; i < N; i++) tmp[i] = h[i * nc + q];
 		if ((rc = hipCheck(c, hipMemcpy(comp[q], tmp.data(), N * sizeof(double), hipMemcpyHostToDevice), "upload"))) return rc;
 	}
-	if (f == GARM_PHI_REF_GRAD) {
-		launchRefLevelSetMixed(F, c->s);
-		if ((rc = launched(c, "mixed derivatives"))) return rc;
+	if (f == GARM_PHI_REF || f == GARM_PHI_REF_GRAD) {
+		launchRefLevelSetRecords(F, c->s);
+		if ((rc = launched(c, "reference level-set records"))) return rc;
 		return hipCheck(c, hipStreamSynchronize(c->s), "upload");
 	}
 	return GARM_OK;
@@ -566,7 +604,7 @@ int garm_initialize(garm_ctx *c, const garm_step_opts *o)
 	launchCopy(F.phi_ref, F.phi, F.N, c->s);
 	launchCopy(F.phi_ori, F.phi, F.N, c->s);
 	for (int a = 0; a < F.D; a++) launchCopy(F.phi_ref_grad[a], F.phi_grad[a], F.N, c->s);
-	launchRefLevelSetMixed(F, c->s);
+	launchRefLevelSetRecords(F, c->s);
 	launchResetRefMap(F, c->s);
 	launchFillU8(F.adv_region, 1, F.N, c->s);
 	if ((rc = launched(c, "initialize"))) return rc;

@@ -10,7 +10,11 @@
 // -x, +x, -y, +y, -z, +z, so the value is the reference's exactly) and lists
 // it for layer L+1.  Work per layer is proportional to the front, not to the
 // grid.  A byte compare-and-swap retries at most once per byte of its word
-// that changes in the same launch, so every claim loop ends.
+// that changes in the same launch, so every claim loop ends.  Lists are
+// appended a workgroup at a time (a prefix sum in LDS and one atomic on the
+// list's counter per workgroup and pass): atomics on one address serialise
+// in L2, and a front of 10^5 elements appended one by one costs more than
+// the rest of the layer.
 #include "garm_internal.hpp"
 
 namespace garm {
@@ -45,21 +49,47 @@ __device__ __forceinline__ bool validIn(const GridDev &g, int i, int j, int k)
 	return i >= 0 && i < g.n[0] && j >= 0 && j < g.n[1] && (D == 2 || (k >= 0 && k < g.n[2]));
 }
 
-template <int D>
+template <int D, int RULE>
 __device__ __forceinline__ bool isFixed(const FixArgs &P, int a, const GridDev &g, int i, int j, int k)
 {
 	const size_t id = gidx<D>(g, i, j, k);
-	switch (P.rule) {
-	case kFixCellDistPositive: return P.cellDist[id] > 0;                 // LevelSetLiquid.cpp:1090-1101
-	case kFixAdvRegion: return P.adv[id] > 0;                              // LevelSetLiquid.cpp:1062-1064
-	case kFixPhiOriNonPositive: return P.phiOri[id] <= 0;                  // LevelSetLiquid.cpp:1068-1070
-	default: {                                                             // LevelSetLiquid.cpp:1072-1083
+	if constexpr (RULE == kFixCellDistPositive) return P.cellDist[id] > 0;   // LevelSetLiquid.cpp:1090-1101
+	else if constexpr (RULE == kFixAdvRegion) return P.adv[id] > 0;           // LevelSetLiquid.cpp:1062-1064
+	else if constexpr (RULE == kFixPhiOriNonPositive) return P.phiOri[id] <= 0;   // LevelSetLiquid.cpp:1068-1070
+	else {                                                                    // LevelSetLiquid.cpp:1072-1083
 		if (!(P.frac[a][id] < 1)) return false;
 		const int i0 = i - (a == 0), j0 = j - (a == 1), k0 = k - (a == 2);
 		if (!validIn<D>(P.cell, i0, j0, k0) || !validIn<D>(P.cell, i, j, k)) return false;
 		return P.phiOri[gidx<D>(P.cell, i0, j0, k0)] <= 0 || P.phiOri[gidx<D>(P.cell, i, j, k)] <= 0;
 	}
+}
+
+// Workgroup-wide append: every thread of the (256-thread) workgroup calls
+// this with the number of entries it will write; the return value is where
+// its entries start.  One atomic on *cnt per workgroup.
+__device__ __forceinline__ int blockAppendBase(int *cnt, int count)
+{
+	__shared__ int waveOff[kBlock / kWave];
+	__shared__ int blockBase;
+	const int tid = int(threadIdx.x + blockDim.x * threadIdx.y);
+	const int lane = tid & (kWave - 1), wid = tid / kWave;
+	int x = count;   // inclusive scan over the wave
+#pragma unroll
+	for (int off = 1; off < kWave; off <<= 1) {
+		const int y = __shfl_up(x, off, kWave);
+		if (lane >= off) x += y;
 	}
+	if (lane == kWave - 1) waveOff[wid] = x;
+	__syncthreads();
+	if (tid == 0) {
+		int t = 0;
+		for (int w = 0; w < kBlock / kWave; w++) { const int v = waveOff[w]; waveOff[w] = t; t += v; }
+		blockBase = t > 0 ? atomicAdd(cnt, t) : 0;
+	}
+	__syncthreads();
+	const int r = blockBase + waveOff[wid] + x - count;
+	__syncthreads();   // the LDS words are reused by the next call
+	return r;
 }
 
 __device__ __forceinline__ void decode(const LayerSpace &S, int gid, int &a, int &i, int &j, int &k)
@@ -75,25 +105,51 @@ __device__ __forceinline__ void decode(const LayerSpace &S, int gid, int &a, int
 }
 
 // Full pass over grid `a`: layer 0 for fixed elements, 255 otherwise; fixed
-// elements with an unfixed neighbour start the front.
+// elements with an unfixed neighbour start the front.  Each thread takes
+// kInitRun elements along the slowest axis (3D: k = blockIdx.z * kInitRun + r;
+// 2D: j = blockIdx.y * 4 * kInitRun + 4 r + threadIdx.y), so a workgroup
+// appends up to 2048 elements with one atomic.
+constexpr int kInitRun = 8;
+
 template <int D>
+static dim3 frontInitDims(const GridDev &g)
+{
+	if constexpr (D == 3) return dim3((g.n[0] + 63) / 64, (g.n[1] + 3) / 4, (g.n[2] + kInitRun - 1) / kInitRun);
+	else return dim3((g.n[0] + 63) / 64, (g.n[1] + 4 * kInitRun - 1) / (4 * kInitRun), 1);
+}
+
+template <int D, int RULE>
 __global__ __launch_bounds__(kBlock) void kFrontInit(LayerSpace S, int a, FixArgs P, int *list, int *cnt)
 {
 	const GridDev &g = S.g[a];
-	const Coord c = threadCoord();
-	if (!inGrid<D>(g, c)) return;
-	const size_t id = gidx<D>(g, c.i, c.j, c.k);
-	const bool fixed = isFixed<D>(P, a, g, c.i, c.j, c.k);
-	S.layer[a][id] = fixed ? 0 : kFar;
-	if (!fixed) return;
-	bool edge = false;
+	const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+	unsigned edges = 0;
 #pragma unroll
-	for (int o = 0; o < 2 * D; o++) {
-		const int ax = o >> 1, sd = (o & 1) ? 1 : -1;
-		const int qi = c.i + (ax == 0 ? sd : 0), qj = c.j + (ax == 1 ? sd : 0), qk = c.k + (ax == 2 ? sd : 0);
-		if (validIn<D>(g, qi, qj, qk) && !isFixed<D>(P, a, g, qi, qj, qk)) edge = true;
+	for (int r = 0; r < kInitRun; r++) {
+		const int j = D == 3 ? int(blockIdx.y * blockDim.y + threadIdx.y) : int(blockIdx.y * blockDim.y * kInitRun + r * blockDim.y + threadIdx.y);
+		const int k = D == 3 ? int(blockIdx.z) * kInitRun + r : 0;
+		if (!validIn<D>(g, i, j, k)) continue;
+		const size_t id = gidx<D>(g, i, j, k);
+		const bool fixed = isFixed<D, RULE>(P, a, g, i, j, k);
+		S.layer[a][id] = fixed ? 0 : kFar;
+		if (!fixed) continue;
+		bool edge = false;
+#pragma unroll
+		for (int o = 0; o < 2 * D; o++) {
+			const int ax = o >> 1, sd = (o & 1) ? 1 : -1;
+			const int qi = i + (ax == 0 ? sd : 0), qj = j + (ax == 1 ? sd : 0), qk = k + (ax == 2 ? sd : 0);
+			if (validIn<D>(g, qi, qj, qk) && !isFixed<D, RULE>(P, a, g, qi, qj, qk)) edge = true;
+		}
+		if (edge) edges |= 1u << r;
 	}
-	if (edge) list[atomicAdd(cnt, 1)] = S.off[a] + int(id);
+	int at = blockAppendBase(cnt, __popc(edges));
+#pragma unroll
+	for (int r = 0; r < kInitRun; r++) {
+		if (!(edges >> r & 1u)) continue;
+		const int j = D == 3 ? int(blockIdx.y * blockDim.y + threadIdx.y) : int(blockIdx.y * blockDim.y * kInitRun + r * blockDim.y + threadIdx.y);
+		const int k = D == 3 ? int(blockIdx.z) * kInitRun + r : 0;
+		list[at++] = S.off[a] + int(gidx<D>(g, i, j, k));
+	}
 }
 
 __device__ __forceinline__ bool claimByte(uint8_t *base, size_t idx, uint8_t L)
@@ -111,42 +167,55 @@ __device__ __forceinline__ bool claimByte(uint8_t *base, size_t idx, uint8_t L)
 	}
 }
 
-// Layer L: claim the unreached neighbours of the layer L-1 front.
+// Layer L: claim the unreached neighbours of the layer L-1 front.  The
+// grid-stride loop runs the same trip count on every thread of a workgroup
+// (each workgroup's start and the stride are workgroup-uniform), as
+// blockAppendBase needs.
 template <int D>
 __global__ __launch_bounds__(kBlock) void kFrontStep(LayerSpace S, int L, const int *listIn, const int *cntIn, int *listOut, int *cntOut, int *cntZero)
 {
 	if (blockIdx.x == 0 && threadIdx.x == 0) *cntZero = 0;
 	const int n = *cntIn;
-	for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-		int a, i, j, k;
-		decode(S, listIn[t], a, i, j, k);
-		const GridDev &g = S.g[a];
-		uint8_t *lay = S.layer[a];
-		for (int o = 0; o < 2 * D; o++) {
-			const int ax = o >> 1, sd = (o & 1) ? 1 : -1;
-			const int qi = i + (ax == 0 ? sd : 0), qj = j + (ax == 1 ? sd : 0), qk = k + (ax == 2 ? sd : 0);
-			if (!validIn<D>(g, qi, qj, qk)) continue;
-			const size_t q = gidx<D>(g, qi, qj, qk);
-			if (lay[q] != kFar || !claimByte(lay, q, uint8_t(L))) continue;
-			// mean over q's neighbours reached before layer L (Extrapolation.h:101-124)
-			int cnt = 0;
-			double s0 = 0, s1 = 0, s2 = 0;
-			for (int p = 0; p < 2 * D; p++) {
-				const int bx = p >> 1, bs = (p & 1) ? 1 : -1;
-				const int ri = qi + (bx == 0 ? bs : 0), rj = qj + (bx == 1 ? bs : 0), rk = qk + (bx == 2 ? bs : 0);
-				if (!validIn<D>(g, ri, rj, rk)) continue;
-				const size_t rr = gidx<D>(g, ri, rj, rk);
-				if (lay[rr] < L) {
-					if (S.nv > 0) s0 += S.v[a][0][rr];
-					if (S.nv > 1) s1 += S.v[a][1][rr];
-					if (S.nv > 2) s2 += S.v[a][2][rr];
-					cnt++;
+	for (int t0 = blockIdx.x * blockDim.x; t0 < n; t0 += gridDim.x * blockDim.x) {
+		const int t = t0 + int(threadIdx.x);
+		int a = 0, i = 0, j = 0, k = 0;
+		unsigned took = 0;
+		if (t < n) {
+			decode(S, listIn[t], a, i, j, k);
+			const GridDev &g = S.g[a];
+			uint8_t *lay = S.layer[a];
+			for (int o = 0; o < 2 * D; o++) {
+				const int ax = o >> 1, sd = (o & 1) ? 1 : -1;
+				const int qi = i + (ax == 0 ? sd : 0), qj = j + (ax == 1 ? sd : 0), qk = k + (ax == 2 ? sd : 0);
+				if (!validIn<D>(g, qi, qj, qk)) continue;
+				const size_t q = gidx<D>(g, qi, qj, qk);
+				if (lay[q] != kFar || !claimByte(lay, q, uint8_t(L))) continue;
+				took |= 1u << o;
+				// mean over q's neighbours reached before layer L (Extrapolation.h:101-124)
+				int cnt = 0;
+				double s0 = 0, s1 = 0, s2 = 0;
+				for (int p = 0; p < 2 * D; p++) {
+					const int bx = p >> 1, bs = (p & 1) ? 1 : -1;
+					const int ri = qi + (bx == 0 ? bs : 0), rj = qj + (bx == 1 ? bs : 0), rk = qk + (bx == 2 ? bs : 0);
+					if (!validIn<D>(g, ri, rj, rk)) continue;
+					const size_t rr = gidx<D>(g, ri, rj, rk);
+					if (lay[rr] < L) {
+						if (S.nv > 0) s0 += S.v[a][0][rr];
+						if (S.nv > 1) s1 += S.v[a][1][rr];
+						if (S.nv > 2) s2 += S.v[a][2][rr];
+						cnt++;
+					}
 				}
+				if (S.nv > 0) S.v[a][0][q] = s0 / cnt;
+				if (S.nv > 1) S.v[a][1][q] = s1 / cnt;
+				if (S.nv > 2) S.v[a][2][q] = s2 / cnt;
 			}
-			if (S.nv > 0) S.v[a][0][q] = s0 / cnt;
-			if (S.nv > 1) S.v[a][1][q] = s1 / cnt;
-			if (S.nv > 2) S.v[a][2][q] = s2 / cnt;
-			listOut[atomicAdd(cntOut, 1)] = S.off[a] + int(q);
+		}
+		int at = blockAppendBase(cntOut, __popc(took));
+		for (int o = 0; o < 2 * D; o++) {
+			if (!(took >> o & 1u)) continue;
+			const int ax = o >> 1, sd = (o & 1) ? 1 : -1;
+			listOut[at++] = S.off[a] + int(gidx<D>(S.g[a], i + (ax == 0 ? sd : 0), j + (ax == 1 ? sd : 0), k + (ax == 2 ? sd : 0)));
 		}
 	}
 }
@@ -189,14 +258,26 @@ static unsigned linBlocks(size_t n)
 	return unsigned(b < 8192 ? (b ? b : 1) : 8192);
 }
 
+template <int D>
+static void launchFrontInit(const LayerSpace &S, int a, const FixArgs &P, int *list, int *cnt, hipStream_t s)
+{
+	const dim3 gd = frontInitDims<D>(S.g[a]), bd = gridBlock();
+	switch (P.rule) {
+	case kFixCellDistPositive: hipLaunchKernelGGL((kFrontInit<D, kFixCellDistPositive>), gd, bd, 0, s, S, a, P, list, cnt); break;
+	case kFixAdvRegion: hipLaunchKernelGGL((kFrontInit<D, kFixAdvRegion>), gd, bd, 0, s, S, a, P, list, cnt); break;
+	case kFixPhiOriNonPositive: hipLaunchKernelGGL((kFrontInit<D, kFixPhiOriNonPositive>), gd, bd, 0, s, S, a, P, list, cnt); break;
+	default: hipLaunchKernelGGL((kFrontInit<D, kFixLiquidFace>), gd, bd, 0, s, S, a, P, list, cnt); break;
+	}
+}
+
 // Run `steps` layers over S with fixed elements given by P.
 static void runFront(const Fields &F, const LayerSpace &S, const FixArgs &P, int steps, hipStream_t s)
 {
 	int *cnt = F.front_cnt;
 	hipLaunchKernelGGL(kZeroInts, dim3(1), dim3(64), 0, s, cnt, 3);
 	for (int a = 0; a < S.ng; a++) {
-		if (F.D == 2) hipLaunchKernelGGL(kFrontInit<2>, gridDims(S.g[a], 2), gridBlock(), 0, s, S, a, P, F.front_list[0], cnt);
-		else hipLaunchKernelGGL(kFrontInit<3>, gridDims(S.g[a], 3), gridBlock(), 0, s, S, a, P, F.front_list[0], cnt);
+		if (F.D == 2) launchFrontInit<2>(S, a, P, F.front_list[0], cnt, s);
+		else launchFrontInit<3>(S, a, P, F.front_list[0], cnt, s);
 	}
 	for (int L = 1; L <= steps; L++) {
 		const int *in = F.front_list[(L - 1) & 1];

@@ -154,46 +154,121 @@ struct EnforceArgs {
 	double *out[3];
 };
 
-// Boundary::enforce(velocity) (Boundary.cpp:89-120); its `fraction == 1`
-// branch (:115) is unreachable and omitted.
+// Boundary::enforce(velocity) (Boundary.cpp:89-120) touches only the faces
+// with 0.1 < fraction <= 1 (:95); its `fraction == 1` branch (:115) is
+// unreachable and omitted.  Those faces, and the solid's interpolated unit
+// normal and velocity at them (fixed unless the boundary fields are
+// replaced), are listed once; each enforce then computes the listed faces
+// into a side buffer and writes them back in place, so the rest of the
+// velocity is neither read nor copied.
 template <int D>
-__global__ __launch_bounds__(kBlock) void kEnforceBoundary(EnforceArgs<D> A, int axis)
+__device__ __forceinline__ void decodeFace(const EnforceArgs<D> &A, const int *off, int gid, int &a, int &i, int &j, int &k)
+{
+	a = 0;
+	while (a + 1 < D && gid >= off[a + 1]) a++;
+	const int loc = gid - off[a];
+	const GridDev &g = A.g[a];
+	i = loc % g.n[0];
+	const int rest = loc / g.n[0];
+	j = rest % g.n[1];
+	k = rest / g.n[1];
+}
+
+struct FaceOffsets { int off[4]; };
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void kBoundaryFaces(EnforceArgs<D> A, int axis, FaceOffsets O, int *ids, int *cnt)
 {
 	const GridDev &g = A.g[axis];
 	const Coord c = threadCoord();
-	if (!inGrid<D>(g, c)) return;
-	const size_t id = gidx<D>(g, c.i, c.j, c.k);
-	const double fr = A.frac[axis][id];
-	double out = A.u[axis][id];
-	if (fr > 0.1 && fr <= 1) {
-		const double p[3] = {g.o[0] + c.i * g.dx, g.o[1] + c.j * g.dx, g.o[2] + c.k * g.dx};
-		double n[3];
+	bool take = false;
+	int gid = 0;
+	if (inGrid<D>(g, c)) {
+		const size_t id = gidx<D>(g, c.i, c.j, c.k);
+		const double fr = A.frac[axis][id];
+		take = fr > 0.1 && fr <= 1;
+		gid = O.off[axis] + int(id);
+	}
+	const unsigned long long m = __ballot(take);
+	if (m == 0) return;
+	const int lane = int(__lane_id()), leader = __ffsll((long long)m) - 1;
+	int base = 0;
+	if (lane == leader) base = atomicAdd(cnt, __popcll(m));
+	base = __shfl(base, leader, kWave);
+	if (take) ids[base + __popcll(m & ((1ull << lane) - 1ull))] = gid;
+}
+
+__device__ __forceinline__ void facePos(const GridDev &g, int i, int j, int k, double *p)
+{
+	p[0] = g.o[0] + i * g.dx; p[1] = g.o[1] + j * g.dx; p[2] = g.o[2] + k * g.dx;
+}
+
+// record per listed face: unit normal n[0..2], solid velocity v0[0..2], out, pad
+constexpr int kFaceRec = 8;
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void kBoundaryRecords(EnforceArgs<D> A, FaceOffsets O, const int *ids, int n, double *rec)
+{
+	const int t = int(blockIdx.x * blockDim.x + threadIdx.x);
+	if (t >= n) return;
+	int ax, i, j, k;
+	decodeFace<D>(A, O.off, ids[t], ax, i, j, k);
+	double p[3];
+	facePos(A.g[ax], i, j, k, p);
+	double nr[3] = {0, 0, 0}, v0[3] = {0, 0, 0};
 #pragma unroll
-		for (int a = 0; a < D; a++) n[a] = lerpFace<D>(A.g[a], A.bnormal[a], p);
-		double s = 0;
+	for (int a = 0; a < D; a++) nr[a] = lerpFace<D>(A.g[a], A.bnormal[a], p);
+	double s = 0;
 #pragma unroll
-		for (int a = 0; a < D; a++) s += n[a] * n[a];
-		if (s > 0) { const double r = sqrt(s); for (int a = 0; a < D; a++) n[a] /= r; }
-		bool any = false;
+	for (int a = 0; a < D; a++) s += nr[a] * nr[a];
+	if (s > 0) { const double r = sqrt(s); for (int a = 0; a < D; a++) nr[a] /= r; }
 #pragma unroll
-		for (int a = 0; a < D; a++) any = any || n[a] != 0;
-		if (any) {
-			double dv[3], vel_ax = 0, vn = 0;
+	for (int a = 0; a < D; a++) v0[a] = lerpFace<D>(A.g[a], A.bvel[a], p);
+	double *q = rec + size_t(t) * kFaceRec;
 #pragma unroll
-			for (int a = 0; a < D; a++) {
-				const double vel = lerpFace<D>(A.g[a], A.u[a], p);
-				const double vel0 = lerpFace<D>(A.g[a], A.bvel[a], p);
-				dv[a] = vel - vel0;
-				if (a == axis) vel_ax = vel;
-				vn += dv[a] * n[a];
-			}
-			if (vn < 0) {
-				const double vt = dv[axis] - vn * n[axis];
-				out = (vel_ax - vn * n[axis]) - 0.8 * vt;
-			}
+	for (int a = 0; a < 3; a++) { q[a] = nr[a]; q[3 + a] = v0[a]; }
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void kEnforceFaces(EnforceArgs<D> A, FaceOffsets O, const int *ids, int n, double *rec)
+{
+	const int t = int(blockIdx.x * blockDim.x + threadIdx.x);
+	if (t >= n) return;
+	int axis, i, j, k;
+	decodeFace<D>(A, O.off, ids[t], axis, i, j, k);
+	const GridDev &g = A.g[axis];
+	double *q = rec + size_t(t) * kFaceRec;
+	double out = A.u[axis][gidx<D>(g, i, j, k)];
+	bool any = false;
+#pragma unroll
+	for (int a = 0; a < D; a++) any = any || q[a] != 0;
+	if (any) {
+		double p[3];
+		facePos(g, i, j, k, p);
+		double dv[3], vel_ax = 0, vn = 0;
+#pragma unroll
+		for (int a = 0; a < D; a++) {
+			const double vel = lerpFace<D>(A.g[a], A.u[a], p);
+			dv[a] = vel - q[3 + a];
+			if (a == axis) vel_ax = vel;
+			vn += dv[a] * q[a];
+		}
+		if (vn < 0) {
+			const double vt = dv[axis] - vn * q[axis];
+			out = (vel_ax - vn * q[axis]) - 0.8 * vt;
 		}
 	}
-	A.out[axis][id] = out;
+	q[6] = out;
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void kEnforceWrite(EnforceArgs<D> A, FaceOffsets O, const int *ids, int n, const double *rec)
+{
+	const int t = int(blockIdx.x * blockDim.x + threadIdx.x);
+	if (t >= n) return;
+	int axis, i, j, k;
+	decodeFace<D>(A, O.off, ids[t], axis, i, j, k);
+	A.out[axis][gidx<D>(A.g[axis], i, j, k)] = rec[size_t(t) * kFaceRec + 6];
 }
 
 // ------------------------------------------------------------- reductions
@@ -317,17 +392,62 @@ void launchGravity(const Fields &F, double dt, hipStream_t s)
 	hipLaunchKernelGGL(kGravity, dim3(linearBlocks(F.Nf[1])), dim3(kBlock), 0, s, F.u[1], 9.8 * dt, F.Nf[1]);
 }
 
+template <int D>
+static EnforceArgs<D> enforceArgs(const Fields &F, FaceOffsets &O)
+{
+	EnforceArgs<D> A{};
+	int off = 0;
+	for (int a = 0; a < D; a++) {
+		A.g[a] = F.face[a]; A.u[a] = F.u[a]; A.bvel[a] = F.bvel[a]; A.bnormal[a] = F.bnormal[a]; A.frac[a] = F.frac[a]; A.out[a] = F.u[a];
+		O.off[a] = off;
+		off += int(F.Nf[a]);
+	}
+	O.off[D] = off;
+	return A;
+}
+
+template <int D>
+static void boundaryFacesD(const Fields &F, int *ids, int *cnt, hipStream_t s)
+{
+	FaceOffsets O{};
+	const EnforceArgs<D> A = enforceArgs<D>(F, O);
+	for (int a = 0; a < D; a++) hipLaunchKernelGGL(kBoundaryFaces<D>, gridDims(F.face[a], D), gridBlock(), 0, s, A, a, O, ids, cnt);
+}
+
+void launchBoundaryFaces(const Fields &F, int *ids, int *cnt, hipStream_t s)
+{
+	if (F.D == 2) boundaryFacesD<2>(F, ids, cnt, s); else boundaryFacesD<3>(F, ids, cnt, s);
+}
+
+template <int D>
+static void boundaryRecordsD(const Fields &F, hipStream_t s)
+{
+	FaceOffsets O{};
+	const EnforceArgs<D> A = enforceArgs<D>(F, O);
+	const unsigned nb = unsigned((F.nbface + kBlock - 1) / kBlock);
+	hipLaunchKernelGGL(kBoundaryRecords<D>, dim3(nb), dim3(kBlock), 0, s, A, O, (const int *)F.bface, F.nbface, F.bface_rec);
+}
+
+void launchBoundaryRecords(const Fields &F, hipStream_t s)
+{
+	if (F.nbface == 0) return;
+	if (F.D == 2) boundaryRecordsD<2>(F, s); else boundaryRecordsD<3>(F, s);
+}
+
+template <int D>
+static void enforceD(const Fields &F, hipStream_t s)
+{
+	FaceOffsets O{};
+	const EnforceArgs<D> A = enforceArgs<D>(F, O);
+	const unsigned nb = unsigned((F.nbface + kBlock - 1) / kBlock);
+	hipLaunchKernelGGL(kEnforceFaces<D>, dim3(nb), dim3(kBlock), 0, s, A, O, (const int *)F.bface, F.nbface, F.bface_rec);
+	hipLaunchKernelGGL(kEnforceWrite<D>, dim3(nb), dim3(kBlock), 0, s, A, O, (const int *)F.bface, F.nbface, (const double *)F.bface_rec);
+}
+
 void launchEnforceBoundary(const Fields &F, hipStream_t s)
 {
-	if (F.D == 2) {
-		EnforceArgs<2> A{};
-		for (int a = 0; a < 2; a++) { A.g[a] = F.face[a]; A.u[a] = F.u[a]; A.bvel[a] = F.bvel[a]; A.bnormal[a] = F.bnormal[a]; A.frac[a] = F.frac[a]; A.out[a] = F.u_new[a]; }
-		for (int a = 0; a < 2; a++) hipLaunchKernelGGL(kEnforceBoundary<2>, gridDims(F.face[a], 2), gridBlock(), 0, s, A, a);
-	} else {
-		EnforceArgs<3> A{};
-		for (int a = 0; a < 3; a++) { A.g[a] = F.face[a]; A.u[a] = F.u[a]; A.bvel[a] = F.bvel[a]; A.bnormal[a] = F.bnormal[a]; A.frac[a] = F.frac[a]; A.out[a] = F.u_new[a]; }
-		for (int a = 0; a < 3; a++) hipLaunchKernelGGL(kEnforceBoundary<3>, gridDims(F.face[a], 3), gridBlock(), 0, s, A, a);
-	}
+	if (F.nbface == 0) return;
+	if (F.D == 2) enforceD<2>(F, s); else enforceD<3>(F, s);
 }
 
 void launchMaxSpeed(const Fields &F, hipStream_t s)

@@ -31,7 +31,9 @@ struct Fields {
 	double *xi_new[3] = {}, *J_new[9] = {};
 	double *phi_tmp = nullptr, *phi_ori = nullptr;    // _tmpLevelSet, _oriLevelSet
 	double *phi_ref = nullptr, *phi_ref_grad[3] = {}; // _refLevelSet, _refLevelSetGrad
-	double *phi_ref_mixed[4] = {};                    // _refLevelSetView's Hermite mixed derivatives
+	double *ref_rec64 = nullptr;                      // _refLevelSetView's Hermite data, 2^D per cell (FP64)
+	float *ref_rec32 = nullptr;                       // ... rounded to FP32
+	void *adv_rec = nullptr;                          // advection records (velocity per face, reference map per cell)
 	uint8_t *adv_region = nullptr;                    // _advRegion
 	double *cell_dist = nullptr, *node_dist = nullptr;// _boundary.cellDist / nodeDist
 	double *frac[3] = {}, *bvel[3] = {}, *bnormal[3] = {};   // _boundary.fraction / velocity / normal
@@ -40,6 +42,9 @@ struct Fields {
 	uint8_t *layer_f[3] = {};                         // ... per face
 	int *front_list[2] = {};                          // extrapolation fronts (element ids)
 	int *front_cnt = nullptr;                         // three front counters
+	int *bface = nullptr;                             // faces Boundary::enforce acts on (global face ids)
+	double *bface_rec = nullptr;                      // per listed face: unit normal, solid velocity, result
+	int nbface = 0;                                   // listed faces
 	double *partials = nullptr;                       // reduction partials
 	size_t partials_cap = 0;
 	double *dscalars = nullptr;                       // small device scalars
@@ -49,7 +54,8 @@ struct Fields {
 void launchAdvectRefMap(const Fields &F, double dt, hipStream_t s, bool fp32 = false);
 void launchAdvectVelocity(const Fields &F, double dt, hipStream_t s, bool fp32 = false);
 void launchCompose(const Fields &F, hipStream_t s, bool fp32 = false);
-void launchRefLevelSetMixed(const Fields &F, hipStream_t s);
+// Hermite records of the reference level set (after phi_ref or its gradient changes)
+void launchRefLevelSetRecords(const Fields &F, hipStream_t s);
 void launchGradient4(const Fields &F, const double *src, double *const *dst, hipStream_t s);
 void launchReinit(const Fields &F, int iterations, double cfl, hipStream_t s, bool fp32 = false);
 // one Runge-Kutta stage L(phi) -> scratch[0], for timing
@@ -70,7 +76,12 @@ void launchGravity(const Fields &F, double dt, hipStream_t s);
 void launchEnrightVelocity(const Fields &F, double coeff, hipStream_t s);
 void launchExtrapolateCollider(const Fields &F, hipStream_t s);
 void launchExtrapolateAir(const Fields &F, hipStream_t s);
+// Boundary::enforce on the listed faces, in place on F.u
 void launchEnforceBoundary(const Fields &F, hipStream_t s);
+// lists the faces with 0.1 < fraction <= 1 into ids (count in *cnt, zeroed by the caller)
+void launchBoundaryFaces(const Fields &F, int *ids, int *cnt, hipStream_t s);
+// unit normal and solid velocity of the F.nbface listed faces into F.bface_rec
+void launchBoundaryRecords(const Fields &F, hipStream_t s);
 // results land in F.dscalars: [0] = max |u|; [1] = tot, [2] = cnt (as doubles)
 void launchMaxSpeed(const Fields &F, hipStream_t s);
 void launchCheckRefMap(const Fields &F, double threshold, double band, hipStream_t s);
