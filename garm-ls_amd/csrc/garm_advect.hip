@@ -33,6 +33,8 @@ __device__ __forceinline__ void sampleVel(const FaceSet &U, const T *p, T *v)
 		T w[2][3];
 #pragma unroll
 		for (int b = 0; b < D; b++) { w[0][b] = 1 - f[b]; w[1][b] = f[b]; }
+		unsigned ox[2], oy[2], oz[2];
+		cornerOffsets<D, 2>(g, l, ox, oy, oz);
 		T val = 0;
 #pragma unroll
 		for (int k = 0; k < (D == 3 ? 2 : 1); k++)
@@ -41,7 +43,7 @@ __device__ __forceinline__ void sampleVel(const FaceSet &U, const T *p, T *v)
 #pragma unroll
 				for (int i = 0; i < 2; i++) {
 					const T wt = D == 2 ? w[i][0] * w[j][1] : w[i][0] * w[j][1] * w[k][2];
-					val += T(ua[gat<D>(g, l[0] + i, l[1] + j, D == 3 ? l[2] + k : 0)]) * wt;
+					val += T(ua[ox[i] + oy[j] + oz[k]]) * wt;
 				}
 		v[a] = val;
 	}
@@ -106,6 +108,8 @@ __device__ __forceinline__ void sampleVelGradPacked(const FaceSet &U, const T *c
 		T w[2][3];
 #pragma unroll
 		for (int b = 0; b < D; b++) { w[0][b] = 1 - f[b]; w[1][b] = f[b]; }
+		unsigned ox[2], oy[2], oz[2];
+		cornerOffsets<D, 2>(g, l, ox, oy, oz);
 		T val = 0, gr[3] = {0, 0, 0};
 #pragma unroll
 		for (int k = 0; k < (D == 3 ? 2 : 1); k++)
@@ -115,7 +119,7 @@ __device__ __forceinline__ void sampleVelGradPacked(const FaceSet &U, const T *c
 				for (int i = 0; i < 2; i++) {
 					const T wt = D == 2 ? w[i][0] * w[j][1] : w[i][0] * w[j][1] * w[k][2];
 					T d[4];
-					loadRecord<T, 4>(rec[a], gat<D>(g, l[0] + i, l[1] + j, D == 3 ? l[2] + k : 0), d);
+					loadRecord<T, 4>(rec[a], ox[i] + oy[j] + oz[k], d);
 					val += d[0] * wt;
 #pragma unroll
 					for (int b = 0; b < D; b++) gr[b] += d[1 + b] * wt;
@@ -220,6 +224,8 @@ __device__ __forceinline__ void hermiteRefMap(const GridDev &g, const T *recs, c
 #pragma unroll
 	for (int e = 0; e < D * D; e++) Gh[e] = 0;
 	const T inv = T(g.inv);
+	unsigned ox[2], oy[2], oz[2];
+	cornerOffsets<D, 2>(g, l, ox, oy, oz);
 
 #pragma unroll
 	for (int cn = 0; cn < (1 << D); cn++) {
@@ -227,7 +233,7 @@ __device__ __forceinline__ void hermiteRefMap(const GridDev &g, const T *recs, c
 		const int lv[3] = {l[0] + vx, l[1] + vy, l[2] + vz};
 		const int qv[3] = {clampi(lv[0], g.n[0] - 1), clampi(lv[1], g.n[1] - 1), D == 3 ? clampi(lv[2], g.n[2] - 1) : 0};
 		T rec[S];
-		loadRecord<T, S>(recs, gidx<D>(g, qv[0], qv[1], qv[2]), rec);
+		loadRecord<T, S>(recs, ox[vx] + oy[vy] + oz[vz], rec);
 		if constexpr (kDisp) {
 #pragma unroll
 			for (int r = 0; r < D; r++)
@@ -358,6 +364,8 @@ __global__ __launch_bounds__(kBlock) void kAdvectVelocity(FaceSet U, int axis, d
 		w[3][a] = t * (t - 1) * (t - 2) / 6;
 	}
 	const double *ua = U.a[axis];
+	unsigned ox[4], oy[4], oz[4];
+	cornerOffsets<D, 4>(g, l, ox, oy, oz);
 	T val = 0;
 #pragma unroll
 	for (int k = 0; k < (D == 3 ? 4 : 1); k++)
@@ -366,7 +374,7 @@ __global__ __launch_bounds__(kBlock) void kAdvectVelocity(FaceSet U, int axis, d
 #pragma unroll
 			for (int i = 0; i < 4; i++) {
 				const T wt = D == 2 ? w[i][0] * w[j][1] : w[i][0] * w[j][1] * w[k][2];
-				val += T(ua[gat<D>(g, l[0] + i, l[1] + j, D == 3 ? l[2] + k : 0)]) * wt;
+				val += T(ua[ox[i] + oy[j] + oz[k]]) * wt;
 			}
 	out[gidx<D>(g, c.i, c.j, c.k)] = val;
 }
@@ -403,12 +411,14 @@ __global__ __launch_bounds__(kBlock) void kCompose(ComposeArgs<D, T> A)
 	T B[3][2][2], Bg[3][2][2];
 #pragma unroll
 	for (int a = 0; a < D; a++) hermiteBasis(f[a], B[a], Bg[a]);
+	unsigned ox[2], oy[2], oz[2];
+	cornerOffsets<D, 2>(g, l, ox, oy, oz);
 	T val = 0, gr[3] = {0, 0, 0};
 #pragma unroll
 	for (int cn = 0; cn < (1 << D); cn++) {
 		const int vx = cn & 1, vy = cn >> 1 & 1, vz = D == 3 ? cn >> 2 : 0;
 		T v[1 << D];
-		loadRecord<T, (1 << D)>(A.rrec, gat<D>(g, l[0] + vx, l[1] + vy, D == 3 ? l[2] + vz : 0), v);
+		loadRecord<T, (1 << D)>(A.rrec, ox[vx] + oy[vy] + oz[vz], v);
 #pragma unroll
 		for (int ord = 0; ord < (1 << D); ord++) {
 			const int ox = ord & 1, oy = ord >> 1 & 1, oz = D == 3 ? ord >> 2 : 0;

@@ -40,6 +40,22 @@ __device__ __forceinline__ size_t gat(const GridDev &g, int i, int j, int k)
 	else return gidx<3>(g, clampi(i, g.n[0] - 1), clampi(j, g.n[1] - 1), clampi(k, g.n[2] - 1));
 }
 
+// Offsets of the clamped corner coordinates l + w, w = 0..W-1, per axis
+// (GridBasedData::at): the index of corner (i, j, k) of the W^D box is
+// ox[i] + oy[j] + oz[k], W clamps per axis instead of W^D clamped indices.
+// Grids hold fewer than 2^31 elements (checked at creation).
+template <int D, int W>
+__device__ __forceinline__ void cornerOffsets(const GridDev &g, const int *l, unsigned (&ox)[W], unsigned (&oy)[W], unsigned (&oz)[W])
+{
+	const unsigned sy = unsigned(g.n[0]), sz = unsigned(g.n[0]) * unsigned(g.n[1]);
+#pragma unroll
+	for (int w = 0; w < W; w++) {
+		ox[w] = unsigned(clampi(l[0] + w, g.n[0] - 1));
+		oy[w] = unsigned(clampi(l[1] + w, g.n[1] - 1)) * sy;
+		oz[w] = D == 3 ? unsigned(clampi(l[2] + w, g.n[2] - 1)) * sz : 0u;
+	}
+}
+
 // Cell/face coordinate of this thread in a (64, 4, 1) x (.., .., n2) launch.
 struct Coord { int i, j, k; };
 

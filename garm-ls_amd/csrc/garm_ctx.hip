@@ -226,6 +226,13 @@ int garm_create(garm_ctx **out, int dim, const int res[3], double dx, const doub
 	}
 	for (int a = 0; a < dim; a++)
 		if (res[a] < 2 || res[a] > 4096) { g_create_error = "resolution out of range"; return GARM_EINVAL; }
+	{
+		// element ids are 32-bit: every grid and all face grids together stay
+		// below 2^31 (each grid has at most (res + 1)^D elements)
+		double nodes = 1;
+		for (int a = 0; a < dim; a++) nodes *= res[a] + 1.0;
+		if (dim * nodes >= 2147483647.0) { g_create_error = "grid too large for 32-bit element ids"; return GARM_EINVAL; }
+	}
 	int ndev = 0;
 	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { g_create_error = "no HIP device"; return GARM_ENODEV; }
 	int dev = 0;

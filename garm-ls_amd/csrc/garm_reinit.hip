@@ -5,8 +5,9 @@
 // (:70-78) are fused into the stage that produces them, so an iteration is
 // three passes:  s0 = L(phi);  s1 = (3 phi + L(s0)) / 4;  phi = (phi + 2 L(s1)) / 3.
 // The FP32 form computes in float (constant quotients as products with
-// rounded reciprocals, fused multiply-adds) and keeps s0, s1 in float; phi
-// stays FP64.
+// rounded reciprocals, the weight and normalisation quotients with the
+// hardware reciprocal and reciprocal square root, fused multiply-adds) and
+// keeps s0, s1 in float; phi stays FP64.
 // Its WENO epsilon floor is 1e-18 instead of 1e-99 (which is 0 in float and
 // would give 0/0 on flat data); in FP64 it is the reference's 1e-99.
 #include "garm_internal.hpp"
@@ -17,6 +18,9 @@ namespace garm {
 
 template <typename T>
 __device__ __forceinline__ T sq(T x) { return x * x; }
+
+__device__ __forceinline__ float rcpT(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float rsqrtT(float x) { return __builtin_amdgcn_rsqf(x); }
 
 template <typename T> __device__ __forceinline__ T wenoEpsFloor();
 template <> __device__ __forceinline__ double wenoEpsFloor<double>() { return 1e-99; }
@@ -45,11 +49,12 @@ __device__ __forceinline__ void weno5(const T *v, T invDx, T &dm, T &dp)
 		const T s1 = sq(dev[0] - 2 * dev[1] + dev[2]) * k1312 + sq(dev[0] - 4 * dev[1] + 3 * dev[2]) * T(0.25);
 		const T s2 = sq(dev[1] - 2 * dev[2] + dev[3]) * k1312 + sq(dev[1] - dev[3]) * T(0.25);
 		const T s3 = sq(dev[2] - 2 * dev[3] + dev[4]) * k1312 + sq(3 * dev[2] - 4 * dev[3] + dev[4]) * T(0.25);
-		const T a1 = T(.1) / sq(s1 + eps);
-		const T a2 = T(.6) / sq(s2 + eps);
-		const T a3 = T(.3) / sq(s3 + eps);
+		// v_rcp_f32 (1 ulp); s + eps >= 1e-18 keeps every weight below 1e36
+		const T a1 = T(.1) * rcpT(sq(s1 + eps));
+		const T a2 = T(.6) * rcpT(sq(s2 + eps));
+		const T a3 = T(.3) * rcpT(sq(s3 + eps));
 		const T sum = a1 + a2 + a3;
-		(i ? dp : dm) = (a1 * phix1 + a2 * phix2 + a3 * phix3) / sum * invDx;
+		(i ? dp : dm) = (a1 * phix1 + a2 * phix2 + a3 * phix3) * rcpT(sum) * invDx;
 	}
 }
 
@@ -91,23 +96,45 @@ __device__ __forceinline__ void weno5Exact(const double *v, double invDx, double
 	}
 }
 
+// The 7-point stencils of the D axes around cell c, clamped into the grid
+// (GridBasedData::at).  Cells three or more away from every face of the grid
+// (all but the outer layers) take plain offsets from their own index.
+template <int D, typename T, typename TI>
+__device__ __forceinline__ void gatherStencils(const GridDev &g, const TI *f, const Coord &c, size_t id, T (&v)[3][7])
+{
+	const bool inner = c.i >= 3 && c.i < g.n[0] - 3 && c.j >= 3 && c.j < g.n[1] - 3 && (D == 2 || (c.k >= 3 && c.k < g.n[2] - 3));
+	if (inner) {
+		const long sx = 1, sy = g.n[0], sz = long(g.n[0]) * g.n[1];
+		const TI *p = f + id;
+#pragma unroll
+		for (int a = 0; a < D; a++) {
+			const long st = a == 0 ? sx : a == 1 ? sy : sz;
+#pragma unroll
+			for (int t = 0; t < 7; t++) v[a][t] = T(p[(t - 3) * st]);
+		}
+	} else {
+#pragma unroll
+		for (int a = 0; a < D; a++)
+#pragma unroll
+			for (int t = 0; t < 7; t++) {
+				const int o = t - 3;
+				v[a][t] = T(f[gat<D>(g, c.i + (a == 0 ? o : 0), c.j + (a == 1 ? o : 0), c.k + (a == 2 ? o : 0))]);
+			}
+	}
+}
+
 template <int D>
 __device__ __forceinline__ double advanceCellExact(const GridDev &g, const double *f, const Coord &c, size_t id, double dtau)
 {
 #pragma clang fp contract(off)
+	double v[3][7];
+	gatherStencils<D, double, double>(g, f, c, id, v);
 	double dm[3], dp[3];
 #pragma unroll
-	for (int a = 0; a < D; a++) {
-		double v[7];
-#pragma unroll
-		for (int t = 0; t < 7; t++) {
-			const int o = t - 3;
-			v[t] = f[gat<D>(g, c.i + (a == 0 ? o : 0), c.j + (a == 1 ? o : 0), c.k + (a == 2 ? o : 0))];
-		}
-		weno5Exact(v, g.inv, dm[a], dp[a]);
-	}
+	for (int a = 0; a < D; a++) weno5Exact(v[a], g.inv, dm[a], dp[a]);
 	double sum = 0;
-	if (f[id] < 0) {
+	const double val = v[0][3];
+	if (val < 0) {
 #pragma unroll
 		for (int a = 0; a < D; a++) { sum += sqExact(fmin(dm[a], 0.0)); sum += sqExact(fmax(dp[a], 0.0)); }
 	} else {
@@ -115,38 +142,32 @@ __device__ __forceinline__ double advanceCellExact(const GridDev &g, const doubl
 		for (int a = 0; a < D; a++) { sum += sqExact(fmax(dm[a], 0.0)); sum += sqExact(fmin(dp[a], 0.0)); }
 	}
 	sum = sqrt(sum);
-	const double val = f[id], sp = g.dx * sum;
+	const double sp = g.dx * sum;
 	return val + dtau * (val / sqrt(val * val + sp * sp)) * (1 - sum);
 }
 
-// Reinitialization::advance (Reinitialization.h:20-48) at one cell, in T,
+// Reinitialization::advance (Reinitialization.h:20-48) at one cell, in float,
 // reading the stage input from storage of type TI.
-template <int D, typename T, typename TI>
-__device__ __forceinline__ T advanceCell(const GridDev &g, const TI *f, const Coord &c, size_t id, T dtau)
+template <int D, typename TI>
+__device__ __forceinline__ float advanceCell(const GridDev &g, const TI *f, const Coord &c, size_t id, float dtau)
 {
-	T dm[3], dp[3];
+	float v[3][7];
+	gatherStencils<D, float, TI>(g, f, c, id, v);
+	float dm[3], dp[3];
 #pragma unroll
-	for (int a = 0; a < D; a++) {
-		T v[7];
-#pragma unroll
-		for (int t = 0; t < 7; t++) {
-			const int o = t - 3;
-			v[t] = T(f[gat<D>(g, c.i + (a == 0 ? o : 0), c.j + (a == 1 ? o : 0), c.k + (a == 2 ? o : 0))]);
-		}
-		weno5<T>(v, T(g.inv), dm[a], dp[a]);
-	}
-	const T val = T(f[id]);
-	T sum = 0;
+	for (int a = 0; a < D; a++) weno5<float>(v[a], float(g.inv), dm[a], dp[a]);
+	const float val = v[0][3];
+	float sum = 0;
 	if (val < 0) {
 #pragma unroll
-		for (int a = 0; a < D; a++) { sum += sq(fmin(dm[a], T(0))); sum += sq(fmax(dp[a], T(0))); }
+		for (int a = 0; a < D; a++) { sum += sq(fminf(dm[a], 0.f)); sum += sq(fmaxf(dp[a], 0.f)); }
 	} else {
 #pragma unroll
-		for (int a = 0; a < D; a++) { sum += sq(fmax(dm[a], T(0))); sum += sq(fmin(dp[a], T(0))); }
+		for (int a = 0; a < D; a++) { sum += sq(fmaxf(dm[a], 0.f)); sum += sq(fminf(dp[a], 0.f)); }
 	}
-	sum = sqrt(sum);
-	const T sp = T(g.dx) * sum;
-	return val + dtau * (val / sqrt(val * val + sp * sp)) * (1 - sum);
+	sum = sqrtf(sum);
+	const float sp = float(g.dx) * sum;
+	return val + dtau * (val * rsqrtT(val * val + sp * sp)) * (1 - sum);
 }
 
 // mode 0: out = L(in);  mode 1: out = (3 phi + L(in)) / 4;  mode 2: phi = (phi + 2 L(in)) / 3
@@ -163,7 +184,7 @@ __global__ __launch_bounds__(kBlock) void kReinitStage(GridDev g, const TI *in, 
 		else if constexpr (MODE == 1) out[id] = (phi[id] * 3 + l) / 4;
 		else phi[id] = (phi[id] + l * 2) / 3;
 	} else {
-		const T l = advanceCell<D, T, TI>(g, in, c, id, T(dtau));
+		const T l = advanceCell<D, TI>(g, in, c, id, T(dtau));
 		if constexpr (MODE == 0) out[id] = TO(l);
 		else if constexpr (MODE == 1) out[id] = TO((T(phi[id]) * 3 + l) / 4);
 		else phi[id] = (T(phi[id]) + l * 2) / 3;
