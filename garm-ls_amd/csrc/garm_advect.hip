@@ -14,38 +14,64 @@
 // Derivatives.h:98-105), computed once per cell in the same operations.
 #include "garm_internal.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace garm {
 
 // ---------------------------------------------------------------- velocity
+// Face velocity arrays of element type S (double, or a float copy for the
+// FP32 kernels) on their face grids.
+template <typename S>
+struct FaceArrays {
+	const S *a[3];
+	GridDev g[3];
+};
+
 // VectorFieldWrapper<SGrid, Linear>::operator() (VectorField.h:66-72):
 // per axis, the multilinear sample of u_a on face grid a (LagrangeIntrpl.h:40-72).
-template <int D, typename T>
-__device__ __forceinline__ void sampleVel(const FaceSet &U, const T *p, T *v)
+// In double the corner weights are formed and summed as the reference does;
+// in float the lerps are nested along x, y, z.
+template <int D, typename T, typename S>
+__device__ __forceinline__ void sampleVel(const FaceArrays<S> &U, const T *p, T *v)
 {
 #pragma unroll
 	for (int a = 0; a < D; a++) {
 		const GridDev &g = U.g[a];
-		const double *ua = U.a[a];
+		const S *ua = U.a[a];
 		int l[3]; T f[3];
 		lowerFracT<D, T>(g, p, l, f);
-		T w[2][3];
-#pragma unroll
-		for (int b = 0; b < D; b++) { w[0][b] = 1 - f[b]; w[1][b] = f[b]; }
 		unsigned ox[2], oy[2], oz[2];
 		cornerOffsets<D, 2>(g, l, ox, oy, oz);
-		T val = 0;
+		if constexpr (std::is_same<T, float>::value) {
+			T yz[2];
 #pragma unroll
-		for (int k = 0; k < (D == 3 ? 2 : 1); k++)
+			for (int k = 0; k < (D == 3 ? 2 : 1); k++) {
+				T row[2];
 #pragma unroll
-			for (int j = 0; j < 2; j++)
-#pragma unroll
-				for (int i = 0; i < 2; i++) {
-					const T wt = D == 2 ? w[i][0] * w[j][1] : w[i][0] * w[j][1] * w[k][2];
-					val += T(ua[ox[i] + oy[j] + oz[k]]) * wt;
+				for (int j = 0; j < 2; j++) {
+					const T u0 = T(ua[ox[0] + oy[j] + oz[k]]), u1 = T(ua[ox[1] + oy[j] + oz[k]]);
+					row[j] = u0 + (u1 - u0) * f[0];
 				}
-		v[a] = val;
+				yz[k] = row[0] + (row[1] - row[0]) * f[1];
+			}
+			v[a] = D == 3 ? yz[0] + (yz[1] - yz[0]) * f[2] : yz[0];
+		} else {
+			T w[2][3];
+#pragma unroll
+			for (int b = 0; b < D; b++) { w[0][b] = 1 - f[b]; w[1][b] = f[b]; }
+			T val = 0;
+#pragma unroll
+			for (int k = 0; k < (D == 3 ? 2 : 1); k++)
+#pragma unroll
+				for (int j = 0; j < 2; j++)
+#pragma unroll
+					for (int i = 0; i < 2; i++) {
+						const T wt = D == 2 ? w[i][0] * w[j][1] : w[i][0] * w[j][1] * w[k][2];
+						val += T(ua[ox[i] + oy[j] + oz[k]]) * wt;
+					}
+			v[a] = val;
+		}
 	}
 }
 
@@ -79,7 +105,7 @@ __device__ __forceinline__ void storeRecord(T *base, size_t q, const T (&v)[S])
 // Face record of axis a: u_a and its central differences along x, y, z
 // (0 for a missing axis) -- what sampleVelGrad reads at one corner.
 template <int D, typename T>
-__global__ __launch_bounds__(kBlock) void kPackVelocity(GridDev g, const double *ua, T *out)
+__global__ __launch_bounds__(kBlock) void kPackVelocity(GridDev g, const double *ua, T *out, float *copy)
 {
 	const Coord c = threadCoord();
 	if (!inGrid<D>(g, c)) return;
@@ -91,6 +117,7 @@ __global__ __launch_bounds__(kBlock) void kPackVelocity(GridDev g, const double 
 	v[2] = T(ua[gat<D>(g, c.i, c.j + 1, c.k)] - ua[gat<D>(g, c.i, c.j - 1, c.k)]) / 2 * inv;
 	v[3] = D == 3 ? T(ua[gat<D>(g, c.i, c.j, c.k + 1)] - ua[gat<D>(g, c.i, c.j, c.k - 1)]) / 2 * inv : T(0);
 	storeRecord<T, 4>(out, id, v);
+	if (copy) copy[id] = float(ua[id]);
 }
 
 // DrvVectorFieldWrapper<SGrid, Linear> (VectorField.h:106-138): per axis a,
@@ -163,45 +190,70 @@ __global__ __launch_bounds__(kBlock) void kPackRefMap(GridDev g, RefMapFields<D>
 {
 	constexpr int S = (1 << D) * D;
 	constexpr bool kDisp = !std::is_same<T, double>::value;
+	__shared__ T stage[kBlock / kWave][kWave * S];
 	const Coord c = threadCoord();
-	if (!inGrid<D>(g, c)) return;
-	const size_t q = gidx<D>(g, c.i, c.j, c.k);
-	const int qi = c.i, qj = c.j, qk = c.k;
-	const int pos[3] = {qi, qj, qk};
-	const T dx = T(g.dx), inv = T(g.inv);
+	const bool live = inGrid<D>(g, c);
+	const int lane = int(threadIdx.x);   // the block is one wave wide in x
 	T rec[S];
-#pragma unroll
-	for (int r = 0; r < D; r++) {
-		T v[8];
-		if constexpr (kDisp) {
-			v[0] = T(A.xi[r][q] - (g.o[r] + pos[r] * g.dx));
-			v[1] = T(A.J[0 * D + r][q] - (r == 0)) * dx;
-			v[2] = T(A.J[1 * D + r][q] - (r == 1)) * dx;
+	if (live) {
+		const size_t q = gidx<D>(g, c.i, c.j, c.k);
+		const int qi = c.i, qj = c.j, qk = c.k;
+		const int pos[3] = {qi, qj, qk};
+		const T dx = T(g.dx), inv = T(g.inv);
+		// x / y neighbours (clamped) shared by every component
+		size_t xp, xm, yp, ym, pp, mp, pm, mm;
+		if (qi >= 1 && qi + 1 < g.n[0] && qj >= 1 && qj + 1 < g.n[1]) {
+			const size_t sy = size_t(g.n[0]);
+			xp = q + 1; xm = q - 1; yp = q + sy; ym = q - sy;
+			pp = q + 1 + sy; mp = q - 1 + sy; pm = q + 1 - sy; mm = q - 1 - sy;
 		} else {
-			v[0] = T(A.xi[r][q]);
-			v[1] = T(A.J[0 * D + r][q]) * dx;
-			v[2] = T(A.J[1 * D + r][q]) * dx;
-		}
-		{
-			const double *c1 = A.J[1 * D + r];
-			const T m = T(c1[gat<D>(g, qi + 1, qj, qk)] - c1[gat<D>(g, qi - 1, qj, qk)]) / 2 * inv;
-			v[3] = m * dx * dx;
-		}
-		if constexpr (D == 3) {
-			const double *c2 = A.J[2 * D + r];
-			v[4] = kDisp ? T(c2[q] - (r == 2)) * dx : T(c2[q]) * dx;
-			T m = T(c2[gat<D>(g, qi + 1, qj, qk)] - c2[gat<D>(g, qi - 1, qj, qk)]) / 2 * inv;
-			v[5] = m * dx * dx;
-			m = T(c2[gat<D>(g, qi, qj + 1, qk)] - c2[gat<D>(g, qi, qj - 1, qk)]) / 2 * inv;
-			v[6] = m * dx * dx;
-			m = T(c2[gat<D>(g, qi + 1, qj + 1, qk)] - c2[gat<D>(g, qi - 1, qj + 1, qk)]
-				- c2[gat<D>(g, qi + 1, qj - 1, qk)] + c2[gat<D>(g, qi - 1, qj - 1, qk)]) / 4 * inv * inv;
-			v[7] = m * dx * dx * dx;
+			xp = gat<D>(g, qi + 1, qj, qk); xm = gat<D>(g, qi - 1, qj, qk);
+			yp = gat<D>(g, qi, qj + 1, qk); ym = gat<D>(g, qi, qj - 1, qk);
+			pp = gat<D>(g, qi + 1, qj + 1, qk); mp = gat<D>(g, qi - 1, qj + 1, qk);
+			pm = gat<D>(g, qi + 1, qj - 1, qk); mm = gat<D>(g, qi - 1, qj - 1, qk);
 		}
 #pragma unroll
-		for (int ord = 0; ord < (1 << D); ord++) rec[ord * D + r] = v[ord];
+		for (int r = 0; r < D; r++) {
+			T v[8];
+			if constexpr (kDisp) {
+				v[0] = T(A.xi[r][q] - (g.o[r] + pos[r] * g.dx));
+				v[1] = T(A.J[0 * D + r][q] - (r == 0)) * dx;
+				v[2] = T(A.J[1 * D + r][q] - (r == 1)) * dx;
+			} else {
+				v[0] = T(A.xi[r][q]);
+				v[1] = T(A.J[0 * D + r][q]) * dx;
+				v[2] = T(A.J[1 * D + r][q]) * dx;
+			}
+			{
+				const double *c1 = A.J[1 * D + r];
+				const T m = T(c1[xp] - c1[xm]) / 2 * inv;
+				v[3] = m * dx * dx;
+			}
+			if constexpr (D == 3) {
+				const double *c2 = A.J[2 * D + r];
+				v[4] = kDisp ? T(c2[q] - (r == 2)) * dx : T(c2[q]) * dx;
+				T m = T(c2[xp] - c2[xm]) / 2 * inv;
+				v[5] = m * dx * dx;
+				m = T(c2[yp] - c2[ym]) / 2 * inv;
+				v[6] = m * dx * dx;
+				m = T(c2[pp] - c2[mp] - c2[pm] + c2[mm]) / 4 * inv * inv;
+				v[7] = m * dx * dx * dx;
+			}
+#pragma unroll
+			for (int ord = 0; ord < (1 << D); ord++) rec[ord * D + r] = v[ord];
+		}
+#pragma unroll
+		for (int e = 0; e < S; e++) stage[threadIdx.y][lane * S + e] = rec[e];
 	}
-	storeRecord<T, S>(out, q, rec);
+	__syncthreads();
+	// the wave's records are one run in memory: write it as consecutive 16-byte words
+	const int i0 = int(blockIdx.x * blockDim.x);
+	const bool row = c.j < g.n[1] && (D == 2 || c.k < g.n[2]);
+	const int count = row ? min(kWave, g.n[0] - i0) : 0;
+	using V = typename Vec4<T>::type;
+	const V *src = reinterpret_cast<const V *>(stage[threadIdx.y]);
+	V *dst = reinterpret_cast<V *>(out + gidx<D>(g, row ? i0 : 0, row ? c.j : 0, row ? c.k : 0) * S);
+	for (int e = lane; e < count * S / 4; e += kWave) dst[e] = src[e];
 }
 
 // HermiteIntrpl<D, VectorDd>::interpolate + interpolateGradient
@@ -270,6 +322,73 @@ __device__ __forceinline__ void hermiteRefMap(const GridDev &g, const T *recs, c
 	}
 }
 
+// The same interpolant in float as a tensor product: the corners in
+// (vy, vz) pairs, each pair's two records contracted along x first (value
+// and x-derivative basis), then weighted in y and z.  About 600 operations
+// per cell instead of 1800; the rounding differs from the corner-by-corner
+// sum only in float.
+template <int D>
+__device__ __forceinline__ void hermiteRefMapFast(const GridDev &g, const float *recs, const float *X, float *val, float *Gh)
+{
+	constexpr int S = (1 << D) * D;
+	int l[3] = {0, 0, 0}; float f[3];
+	lowerFracT<D, float>(g, X, l, f);
+	float B[3][2][2], Bg[3][2][2];
+#pragma unroll
+	for (int a = 0; a < D; a++) hermiteBasis(f[a], B[a], Bg[a]);
+	unsigned ox[2], oy[2], oz[2];
+	cornerOffsets<D, 2>(g, l, ox, oy, oz);
+	// a corner clamped into the grid keeps its own position in the displacement
+	float cx[2], cy[2], cz[2];
+#pragma unroll
+	for (int w = 0; w < 2; w++) {
+		const int lx = l[0] + w, ly = l[1] + w, lz = l[2] + w;
+		cx[w] = float((clampi(lx, g.n[0] - 1) - lx) * g.dx);
+		cy[w] = float((clampi(ly, g.n[1] - 1) - ly) * g.dx);
+		cz[w] = D == 3 ? float((clampi(lz, g.n[2] - 1) - lz) * g.dx) : 0.f;
+	}
+	float v[3] = {0, 0, 0}, gx[3] = {0, 0, 0}, gy[3] = {0, 0, 0}, gz[3] = {0, 0, 0};
+#pragma unroll
+	for (int vz = 0; vz < (D == 3 ? 2 : 1); vz++)
+#pragma unroll
+		for (int vy = 0; vy < 2; vy++) {
+			float r0[S], r1[S];
+			loadRecord<float, S>(recs, ox[0] + oy[vy] + oz[vz], r0);
+			loadRecord<float, S>(recs, ox[1] + oy[vy] + oz[vz], r1);
+			r0[0] += cx[0]; r1[0] += cx[1];
+			r0[1] += cy[vy]; r1[1] += cy[vy];
+			if constexpr (D == 3) { r0[2] += cz[vz]; r1[2] += cz[vz]; }
+#pragma unroll
+			for (int wz = 0; wz < (D == 3 ? 2 : 1); wz++)
+#pragma unroll
+				for (int wy = 0; wy < 2; wy++) {
+					const int o0 = (wy << 1) | (wz << 2), o1 = o0 | 1;
+					const float by = B[1][vy][wy], bgy = Bg[1][vy][wy];
+					const float bz = D == 3 ? B[2][vz][wz] : 1.f, bgz = D == 3 ? Bg[2][vz][wz] : 0.f;
+					const float wv = by * bz, wgy = bgy * bz, wgz = by * bgz;
+#pragma unroll
+					for (int r = 0; r < D; r++) {
+						const float a0 = r0[o0 * D + r], a1 = r0[o1 * D + r], b0 = r1[o0 * D + r], b1 = r1[o1 * D + r];
+						const float tx = a0 * B[0][0][0] + a1 * B[0][0][1] + b0 * B[0][1][0] + b1 * B[0][1][1];
+						const float tgx = a0 * Bg[0][0][0] + a1 * Bg[0][0][1] + b0 * Bg[0][1][0] + b1 * Bg[0][1][1];
+						v[r] += tx * wv;
+						gx[r] += tgx * wv;
+						gy[r] += tx * wgy;
+						if constexpr (D == 3) gz[r] += tx * wgz;
+					}
+				}
+		}
+	const float inv = float(g.inv);
+#pragma unroll
+	for (int r = 0; r < D; r++) {
+		val[r] = v[r] + X[r];
+		Gh[0 * D + r] = gx[r] * inv;
+		Gh[1 * D + r] = gy[r] * inv;
+		if constexpr (D == 3) Gh[2 * D + r] = gz[r] * inv;
+		Gh[r * D + r] += 1.f;
+	}
+}
+
 // ------------------------------------------------------------------ kernels
 template <int D, typename T>
 struct RefMapArgs {
@@ -315,7 +434,8 @@ __global__ __launch_bounds__(kBlock) void kAdvectRefMap(RefMapArgs<D, T> A)
 		}
 
 	T val[3], Gh[9];
-	hermiteRefMap<D, T>(A.cell, A.hrec, X, val, Gh);
+	if constexpr (std::is_same<T, float>::value) hermiteRefMapFast<D>(A.cell, A.hrec, X, val, Gh);
+	else hermiteRefMap<D, T>(A.cell, A.hrec, X, val, Gh);
 #pragma unroll
 	for (int r = 0; r < D; r++) A.xi_new[r][id] = val[r];
 #pragma unroll
@@ -331,8 +451,8 @@ __global__ __launch_bounds__(kBlock) void kAdvectRefMap(RefMapArgs<D, T> A)
 
 // SemiLagrangian<D, 2, CubicIntrpl<D>>::advect on face grid `axis`
 // (Advection.h:15-22, 37-43, 137-145; CubicIntrpl LagrangeIntrpl.h:128-171).
-template <int D, typename T>
-__global__ __launch_bounds__(kBlock) void kAdvectVelocity(FaceSet U, int axis, double *out, double hd)
+template <int D, typename T, typename S>
+__global__ __launch_bounds__(kBlock) void kAdvectVelocity(FaceArrays<S> U, int axis, double *out, double hd)
 {
 	const GridDev &g = U.g[axis];
 	const Coord c = threadCoord();
@@ -340,10 +460,10 @@ __global__ __launch_bounds__(kBlock) void kAdvectVelocity(FaceSet U, int axis, d
 	const T h = T(hd);
 	T x[3] = {T(g.o[0] + c.i * g.dx), T(g.o[1] + c.j * g.dx), T(g.o[2] + c.k * g.dx)};
 	T v0[3], p1[3], v1[3], X[3];
-	sampleVel<D, T>(U, x, v0);
+	sampleVel<D, T, S>(U, x, v0);
 #pragma unroll
 	for (int a = 0; a < D; a++) p1[a] = x[a] + v0[a] * h;
-	sampleVel<D, T>(U, p1, v1);
+	sampleVel<D, T, S>(U, p1, v1);
 #pragma unroll
 	for (int a = 0; a < D; a++) X[a] = x[a] + (v0[a] + v1[a]) * h / 2;
 
@@ -363,20 +483,43 @@ __global__ __launch_bounds__(kBlock) void kAdvectVelocity(FaceSet U, int axis, d
 		w[2][a] = t * (t - 1) * (t - 3) / -2;
 		w[3][a] = t * (t - 1) * (t - 2) / 6;
 	}
-	const double *ua = U.a[axis];
+	const S *ua = U.a[axis];
 	unsigned ox[4], oy[4], oz[4];
 	cornerOffsets<D, 4>(g, l, ox, oy, oz);
 	T val = 0;
+	if constexpr (std::is_same<T, float>::value) {
+		// separable: rows along x, then y, then z
 #pragma unroll
-	for (int k = 0; k < (D == 3 ? 4 : 1); k++)
+		for (int k = 0; k < (D == 3 ? 4 : 1); k++) {
+			T plane = 0;
 #pragma unroll
-		for (int j = 0; j < 4; j++)
+			for (int j = 0; j < 4; j++) {
+				T row = 0;
 #pragma unroll
-			for (int i = 0; i < 4; i++) {
-				const T wt = D == 2 ? w[i][0] * w[j][1] : w[i][0] * w[j][1] * w[k][2];
-				val += T(ua[ox[i] + oy[j] + oz[k]]) * wt;
+				for (int i = 0; i < 4; i++) row += T(ua[ox[i] + oy[j] + oz[k]]) * w[i][0];
+				plane += row * w[j][1];
 			}
+			val += D == 3 ? plane * w[k][2] : plane;
+		}
+	} else {
+#pragma unroll
+		for (int k = 0; k < (D == 3 ? 4 : 1); k++)
+#pragma unroll
+			for (int j = 0; j < 4; j++)
+#pragma unroll
+				for (int i = 0; i < 4; i++) {
+					const T wt = D == 2 ? w[i][0] * w[j][1] : w[i][0] * w[j][1] * w[k][2];
+					val += T(ua[ox[i] + oy[j] + oz[k]]) * wt;
+				}
+	}
 	out[gidx<D>(g, c.i, c.j, c.k)] = val;
+}
+
+// float copy of a face array (the FP32 velocity advection's input)
+template <typename S>
+__global__ void kFaceToFloat(const double *u, S *out, size_t n)
+{
+	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) out[i] = float(u[i]);
 }
 
 template <int D, typename T>
@@ -414,6 +557,32 @@ __global__ __launch_bounds__(kBlock) void kCompose(ComposeArgs<D, T> A)
 	unsigned ox[2], oy[2], oz[2];
 	cornerOffsets<D, 2>(g, l, ox, oy, oz);
 	T val = 0, gr[3] = {0, 0, 0};
+	if constexpr (std::is_same<T, float>::value) {
+		// tensor product as in hermiteRefMapFast: x contracted per (vy, vz) pair
+#pragma unroll
+		for (int vz = 0; vz < (D == 3 ? 2 : 1); vz++)
+#pragma unroll
+			for (int vy = 0; vy < 2; vy++) {
+				T r0[1 << D], r1[1 << D];
+				loadRecord<T, (1 << D)>(A.rrec, ox[0] + oy[vy] + oz[vz], r0);
+				loadRecord<T, (1 << D)>(A.rrec, ox[1] + oy[vy] + oz[vz], r1);
+#pragma unroll
+				for (int wz = 0; wz < (D == 3 ? 2 : 1); wz++)
+#pragma unroll
+					for (int wy = 0; wy < 2; wy++) {
+						const int o0 = (wy << 1) | (wz << 2), o1 = o0 | 1;
+						const T by = B[1][vy][wy], bgy = Bg[1][vy][wy];
+						const T bz = D == 3 ? B[2][vz][wz] : T(1), bgz = D == 3 ? Bg[2][vz][wz] : T(0);
+						const T tx = r0[o0] * B[0][0][0] + r0[o1] * B[0][0][1] + r1[o0] * B[0][1][0] + r1[o1] * B[0][1][1];
+						const T tgx = r0[o0] * Bg[0][0][0] + r0[o1] * Bg[0][0][1] + r1[o0] * Bg[0][1][0] + r1[o1] * Bg[0][1][1];
+						const T wv = by * bz;
+						val += tx * wv;
+						gr[0] += tgx * wv;
+						gr[1] += tx * (bgy * bz);
+						if constexpr (D == 3) gr[2] += tx * (by * bgz);
+					}
+			}
+	} else {
 #pragma unroll
 	for (int cn = 0; cn < (1 << D); cn++) {
 		const int vx = cn & 1, vy = cn >> 1 & 1, vz = D == 3 ? cn >> 2 : 0;
@@ -437,6 +606,7 @@ __global__ __launch_bounds__(kBlock) void kCompose(ComposeArgs<D, T> A)
 #pragma unroll
 			for (int a = 0; a < D; a++) gr[a] += v[ord] * wg[a];
 		}
+	}
 	}
 #pragma unroll
 	for (int a = 0; a < D; a++) gr[a] *= T(g.inv);
@@ -535,23 +705,41 @@ using Real = float;
 using Real = double;
 #endif
 
-// Face records of u, then the reference-map records, then the trace.  The
-// records live in F.adv_rec: cell records first, face records of each axis
-// after them, each part 256-byte aligned.
+// Layout of F.adv_rec (sized for FP64 records): the cell records, the face
+// records of each axis, then (float kernels) a float copy of each face
+// array of u; each part 256-byte aligned.
+struct RecordLayout {
+	size_t face[3], copy[3];
+};
+
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+template <int D>
+static RecordLayout recordLayout(const Fields &F, size_t real)
+{
+	RecordLayout L{};
+	size_t off = align256(F.N * ((1 << D) * D) * real);
+	for (int a = 0; a < D; a++) { L.face[a] = off; off += align256(F.Nf[a] * 4 * real); }
+	for (int a = 0; a < D; a++) { L.copy[a] = off; off += align256(F.Nf[a] * sizeof(float)); }
+	return L;
+}
+
+// Face records of u (and its float copy), then the reference-map records,
+// then the trace.
 template <int D>
 static void advectRefMapD(const Fields &F, double dt, hipStream_t s)
 {
-	constexpr int S = (1 << D) * D;
 	RefMapArgs<D, Real> A{};
 	A.cell = F.cell;
 	A.U = faceSet(F);
 	char *base = static_cast<char *>(F.adv_rec);
-	size_t off = (F.N * S * sizeof(Real) + 255) / 256 * 256;
+	const RecordLayout Ly = recordLayout<D>(F, sizeof(Real));
+	constexpr bool fp32 = std::is_same<Real, float>::value;
 	for (int a = 0; a < D; a++) {
-		Real *rec = reinterpret_cast<Real *>(base + off);
-		hipLaunchKernelGGL((kPackVelocity<D, Real>), gridDims(F.face[a], D), gridBlock(), 0, s, F.face[a], (const double *)F.u[a], rec);
+		Real *rec = reinterpret_cast<Real *>(base + Ly.face[a]);
+		float *copy = fp32 ? reinterpret_cast<float *>(base + Ly.copy[a]) : nullptr;
+		hipLaunchKernelGGL((kPackVelocity<D, Real>), gridDims(F.face[a], D), gridBlock(), 0, s, F.face[a], (const double *)F.u[a], rec, copy);
 		A.vrec[a] = rec;
-		off += (F.Nf[a] * 4 * sizeof(Real) + 255) / 256 * 256;
 	}
 	RefMapFields<D> R{};
 	for (int a = 0; a < D; a++) R.xi[a] = F.xi[a];
@@ -565,11 +753,33 @@ static void advectRefMapD(const Fields &F, double dt, hipStream_t s)
 	hipLaunchKernelGGL((kAdvectRefMap<D, Real>), gridDims(F.cell, D), gridBlock(), 0, s, A);
 }
 
+// The float kernels read the float copy of u, made here unless the
+// reference-map advection of the same step made it (copiesFresh).
 template <int D>
-static void advectVelocityD(const Fields &F, double dt, hipStream_t s)
+static void advectVelocityD(const Fields &F, double dt, hipStream_t s, bool copiesFresh)
 {
-	const FaceSet U = faceSet(F);
-	for (int a = 0; a < D; a++) hipLaunchKernelGGL((kAdvectVelocity<D, Real>), gridDims(F.face[a], D), gridBlock(), 0, s, U, a, F.u_new[a], -dt);
+	if constexpr (std::is_same<Real, float>::value) {
+		FaceArrays<float> U{};
+		char *base = static_cast<char *>(F.adv_rec);
+		const RecordLayout Ly = recordLayout<D>(F, sizeof(Real));
+		for (int a = 0; a < D; a++) {
+			float *copy = reinterpret_cast<float *>(base + Ly.copy[a]);
+			if (!copiesFresh) {
+				const unsigned nb = unsigned(std::min<size_t>((F.Nf[a] + kBlock - 1) / kBlock, 8192));
+				hipLaunchKernelGGL(kFaceToFloat<float>, dim3(nb), dim3(kBlock), 0, s, (const double *)F.u[a], copy, F.Nf[a]);
+			}
+			U.a[a] = copy;
+			U.g[a] = F.face[a];
+		}
+		for (int a = 0; a < D; a++)
+			hipLaunchKernelGGL((kAdvectVelocity<D, float, float>), gridDims(F.face[a], D), gridBlock(), 0, s, U, a, F.u_new[a], -dt);
+	} else {
+		(void)copiesFresh;
+		FaceArrays<double> U{};
+		for (int a = 0; a < D; a++) { U.a[a] = F.u[a]; U.g[a] = F.face[a]; }
+		for (int a = 0; a < D; a++)
+			hipLaunchKernelGGL((kAdvectVelocity<D, double, double>), gridDims(F.face[a], D), gridBlock(), 0, s, U, a, F.u_new[a], -dt);
+	}
 }
 
 template <int D>
@@ -595,9 +805,9 @@ void launchAdvectRefMap32(const Fields &F, double dt, hipStream_t s)
 	if (F.D == 2) advectRefMapD<2>(F, dt, s); else advectRefMapD<3>(F, dt, s);
 }
 
-void launchAdvectVelocity32(const Fields &F, double dt, hipStream_t s)
+void launchAdvectVelocity32(const Fields &F, double dt, hipStream_t s, bool copiesFresh)
 {
-	if (F.D == 2) advectVelocityD<2>(F, dt, s); else advectVelocityD<3>(F, dt, s);
+	if (F.D == 2) advectVelocityD<2>(F, dt, s, copiesFresh); else advectVelocityD<3>(F, dt, s, copiesFresh);
 }
 
 void launchCompose32(const Fields &F, hipStream_t s)
@@ -608,7 +818,7 @@ void launchCompose32(const Fields &F, hipStream_t s)
 #else
 
 void launchAdvectRefMap32(const Fields &F, double dt, hipStream_t s);
-void launchAdvectVelocity32(const Fields &F, double dt, hipStream_t s);
+void launchAdvectVelocity32(const Fields &F, double dt, hipStream_t s, bool copiesFresh);
 void launchCompose32(const Fields &F, hipStream_t s);
 
 void launchAdvectRefMap(const Fields &F, double dt, hipStream_t s, bool fp32)
@@ -618,11 +828,11 @@ void launchAdvectRefMap(const Fields &F, double dt, hipStream_t s, bool fp32)
 	else advectRefMapD<3>(F, dt, s);
 }
 
-void launchAdvectVelocity(const Fields &F, double dt, hipStream_t s, bool fp32)
+void launchAdvectVelocity(const Fields &F, double dt, hipStream_t s, bool fp32, bool copiesFresh)
 {
-	if (fp32) launchAdvectVelocity32(F, dt, s);
-	else if (F.D == 2) advectVelocityD<2>(F, dt, s);
-	else advectVelocityD<3>(F, dt, s);
+	if (fp32) launchAdvectVelocity32(F, dt, s, copiesFresh);
+	else if (F.D == 2) advectVelocityD<2>(F, dt, s, false);
+	else advectVelocityD<3>(F, dt, s, false);
 }
 
 void launchCompose(const Fields &F, hipStream_t s, bool fp32)

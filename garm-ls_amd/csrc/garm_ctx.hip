@@ -638,7 +638,7 @@ int garm_step(garm_ctx *c, double dt, int step_index, const garm_step_opts *o, g
 	(void)hipEventRecord(ev[0], c->s);
 	launchAdvectRefMap(F, dt, c->s, fp32);
 	swapRefMap(F);
-	launchAdvectVelocity(F, dt, c->s, fp32);
+	launchAdvectVelocity(F, dt, c->s, fp32, true);   // u is unchanged since the reference-map advection
 	swapFaces(F);
 	launchCompose(F, c->s, fp32);
 	(void)hipEventRecord(ev[1], c->s);

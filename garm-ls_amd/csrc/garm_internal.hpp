@@ -52,7 +52,8 @@ struct Fields {
 
 // ---- launchers (one per kernel family; all enqueue on `s`) ---------------
 void launchAdvectRefMap(const Fields &F, double dt, hipStream_t s, bool fp32 = false);
-void launchAdvectVelocity(const Fields &F, double dt, hipStream_t s, bool fp32 = false);
+// copiesFresh: the float copy of u made by launchAdvectRefMap(fp32) of the same u is current
+void launchAdvectVelocity(const Fields &F, double dt, hipStream_t s, bool fp32 = false, bool copiesFresh = false);
 void launchCompose(const Fields &F, hipStream_t s, bool fp32 = false);
 // Hermite records of the reference level set (after phi_ref or its gradient changes)
 void launchRefLevelSetRecords(const Fields &F, hipStream_t s);

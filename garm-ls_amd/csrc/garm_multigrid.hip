@@ -13,9 +13,12 @@
 // Free surfaces (ghost-fluid terms in the diagonal) and solid faces (zero
 // weights) carry over to every level without special cases.  Smoothing is
 // red-black Gauss-Seidel, two sweeps down and two up in mirrored colour order
-// so the cycle is symmetric; the coarsest level (<= 512 cells) is relaxed by
-// one workgroup.  Everything is FP32 on structured grids: unit-stride,
-// coalesced neighbours, no index arrays.
+// so the cycle is symmetric; the coarsest level (<= 512 cells) is relaxed
+// 30 times.  The coarse levels that fit in 128 KiB of LDS together (from a
+// few thousand cells down) run as one workgroup-wide kernel on LDS copies:
+// each of their sweeps would otherwise be a launch of a few microseconds
+// over a grid too small to fill the chip.  Everything is FP32 on structured
+// grids: unit-stride, coalesced neighbours, no index arrays.
 #include "garm_multigrid.hpp"
 
 #include <algorithm>
@@ -172,21 +175,162 @@ __global__ __launch_bounds__(kBlock) void kMgProlong(MgLevel F, MgLevel C)
 	}
 }
 
-// the coarsest level, relaxed by one workgroup (its iterate is zero on entry)
-template <int D>
-__global__ __launch_bounds__(1024) void kMgCoarsest(MgLevel L, int sweeps)
+// ---- the coarse levels in LDS -------------------------------------------
+// Each tail level keeps its six arrays padded by one xy-plane of zeros on
+// both sides: every cell at a box face already has a zero weight on that
+// face (the box bounds the unknowns, and coarsening sums those zeros), so
+// the stencil can read its six neighbours without bounds checks, the
+// missing ones contributing 0 * finite = 0 in the same order as offSum.
+// The active cells (diag > 0) of each colour are listed once per cycle
+// (16-bit cell index, and for prolongation the coarse cell's index), so
+// the sweeps do no index arithmetic.
+constexpr int kMgTailFloats = 40448;    // 158 KiB of the 160 KiB of LDS: padded arrays and cell lists
+
+struct MgTail {
+	int first, levels;              // levels first .. levels-1 run in LDS
+	int sweeps;                     // relaxation sweeps on the coarsest level
+	MgLevel g[kMgMaxLevels];        // their global arrays
+	int off[kMgMaxLevels];          // LDS offset (floats) of each level's padded arrays
+	int list[kMgMaxLevels];         // LDS offset (words) of each level's cell list
+};
+
+// padded plane of level L
+__device__ __forceinline__ int mgPad(const MgLevel &L) { return L.n[0] * L.n[1]; }
+__device__ __forceinline__ int mgCells(const MgLevel &L) { return L.n[0] * L.n[1] * L.n[2]; }
+
+__device__ __forceinline__ MgLevel ldsLevel(float *lds, const MgTail &T, int l)
 {
-	const size_t cells = size_t(L.n[0]) * L.n[1] * L.n[2];
-	for (int s = 0; s < 2 * sweeps; s++) {
-		const int color = (s & 1) ^ (s >= sweeps ? 1 : 0);   // red-black down, black-red up
-		for (size_t q = threadIdx.x; q < cells; q += blockDim.x) {
-			const int i = int(q % L.n[0]), j = int((q / L.n[0]) % L.n[1]), k = int(q / (size_t(L.n[0]) * L.n[1]));
-			if (((i + j + k) & 1) != color) continue;
-			const float d = L.diag[q];
-			if (d > 0) L.x[q] = (L.b[q] + offSum<D>(L, q, i, j, k)) / d;
+	MgLevel V;
+	for (int a = 0; a < 3; a++) V.n[a] = T.g[l].n[a];
+	const int c = mgCells(V), pad = mgPad(V), span = c + 2 * pad;
+	float *p = lds + T.off[l] + pad;
+	V.diag = p; V.wx = p + span; V.wy = p + 2 * span; V.wz = p + 3 * span; V.b = p + 4 * span; V.x = p + 5 * span;
+	return V;
+}
+
+template <int D>
+__device__ __forceinline__ float offSumPadded(const MgLevel &L, int c)
+{
+	const int sy = L.n[0], sz = L.n[0] * L.n[1];
+	float s = 0;
+	s += L.wx[c] * L.x[c - 1];
+	s += L.wx[c + 1] * L.x[c + 1];
+	s += L.wy[c] * L.x[c - sy];
+	s += L.wy[c + sy] * L.x[c + sy];
+	if constexpr (D == 3) {
+		s += L.wz[c] * L.x[c - sz];
+		s += L.wz[c + sz] * L.x[c + sz];
+	}
+	return s;
+}
+
+// one colour of red-black Gauss-Seidel over the listed cells
+template <int D>
+__device__ __forceinline__ void tailSmooth(const MgLevel &L, const unsigned *list, int from, int to)
+{
+	for (int t = from + int(threadIdx.x); t < to; t += blockDim.x) {
+		const int c = int(list[t] & 0xffffu);
+		L.x[c] = (L.b[c] + offSumPadded<D>(L, c)) / L.diag[c];
+	}
+	__syncthreads();
+}
+
+template <int D>
+__global__ __launch_bounds__(1024) void kMgTail(MgTail T)
+{
+	__shared__ float lds[kMgTailFloats];
+	__shared__ int ncol[kMgMaxLevels][2];   // active cells per colour
+	const int tid = int(threadIdx.x);
+	if (tid < 2 * kMgMaxLevels) ncol[tid / 2][tid % 2] = 0;
+	// operators of every tail level (zero padding), the first level's right-hand side, zero iterate
+	for (int l = T.first; l < T.levels; l++) {
+		const MgLevel G = T.g[l], V = ldsLevel(lds, T, l);
+		const int c = mgCells(G), pad = mgPad(G);
+		for (int q = tid - pad; q < c + pad; q += blockDim.x) {
+			const bool in = q >= 0 && q < c;
+			V.diag[q] = in ? G.diag[q] : 0.0f;
+			V.wx[q] = in ? G.wx[q] : 0.0f;
+			V.wy[q] = in ? G.wy[q] : 0.0f;
+			V.wz[q] = in ? G.wz[q] : 0.0f;
+			V.b[q] = in && l == T.first ? G.b[q] : 0.0f;
+			V.x[q] = 0.0f;
+		}
+	}
+	__syncthreads();
+	// count the active cells of each colour, then list them (red first)
+	for (int l = T.first; l < T.levels; l++) {
+		const MgLevel G = T.g[l];
+		for (int q = tid; q < mgCells(G); q += blockDim.x) {
+			if (!(G.diag[q] > 0)) continue;
+			const int i = q % G.n[0], j = (q / G.n[0]) % G.n[1], k = q / (G.n[0] * G.n[1]);
+			atomicAdd(&ncol[l][(i + j + k) & 1], 1);
+		}
+	}
+	__syncthreads();
+	__shared__ int fill[kMgMaxLevels][2];
+	if (tid < 2 * kMgMaxLevels) fill[tid / 2][tid % 2] = tid % 2 ? ncol[tid / 2][0] : 0;
+	__syncthreads();
+	for (int l = T.first; l < T.levels; l++) {
+		const MgLevel G = T.g[l];
+		unsigned *list = reinterpret_cast<unsigned *>(lds + T.list[l]);
+		const bool coarse = l + 1 < T.levels;
+		const MgLevel C = coarse ? T.g[l + 1] : G;
+		for (int q = tid; q < mgCells(G); q += blockDim.x) {
+			if (!(G.diag[q] > 0)) continue;
+			const int i = q % G.n[0], j = (q / G.n[0]) % G.n[1], k = q / (G.n[0] * G.n[1]);
+			const unsigned cq = coarse ? unsigned(cidx(C, i / 2, j / 2, D == 3 ? k / 2 : 0)) : 0u;
+			list[atomicAdd(&fill[l][(i + j + k) & 1], 1)] = unsigned(q) | cq << 16;
+		}
+	}
+	__syncthreads();
+	const int top = T.levels - 1;
+	for (int l = T.first; l < top; l++) {
+		const MgLevel F = ldsLevel(lds, T, l), C = ldsLevel(lds, T, l + 1);
+		const unsigned *list = reinterpret_cast<const unsigned *>(lds + T.list[l]);
+		const int nr = ncol[l][0], na = nr + ncol[l][1];
+		for (int v = 0; v < kMgSweeps; v++) { tailSmooth<D>(F, list, 0, nr); tailSmooth<D>(F, list, nr, na); }
+		// kMgRestrict
+		const int cc = mgCells(C);
+		for (int q = tid; q < cc; q += blockDim.x) {
+			const int I = q % C.n[0], J = (q / C.n[0]) % C.n[1], K = q / (C.n[0] * C.n[1]);
+			float s = 0;
+			for (int cz = 0; cz < (D == 3 ? 2 : 1); cz++)
+				for (int cy = 0; cy < 2; cy++)
+					for (int cx = 0; cx < 2; cx++) {
+						const int i = 2 * I + cx, j = 2 * J + cy, k = 2 * K + cz;
+						if (i >= F.n[0] || j >= F.n[1] || k >= F.n[2]) continue;
+						const int f = int(cidx(F, i, j, k));
+						const float d = F.diag[f];
+						if (d > 0) s += F.b[f] - (d * F.x[f] - offSumPadded<D>(F, f));
+					}
+			C.b[q] = s;
 		}
 		__syncthreads();
 	}
+	{
+		// kMgCoarsest
+		const MgLevel L = ldsLevel(lds, T, top);
+		const unsigned *list = reinterpret_cast<const unsigned *>(lds + T.list[top]);
+		const int nr = ncol[top][0], na = nr + ncol[top][1];
+		for (int s = 0; s < 2 * T.sweeps; s++) {
+			const int color = (s & 1) ^ (s >= T.sweeps ? 1 : 0);
+			tailSmooth<D>(L, list, color ? nr : 0, color ? na : nr);
+		}
+	}
+	for (int l = top - 1; l >= T.first; l--) {
+		const MgLevel F = ldsLevel(lds, T, l), C = ldsLevel(lds, T, l + 1);
+		const unsigned *list = reinterpret_cast<const unsigned *>(lds + T.list[l]);
+		const int nr = ncol[l][0], na = nr + ncol[l][1];
+		// kMgProlong
+		for (int t = tid; t < na; t += blockDim.x) {
+			const unsigned e = list[t];
+			F.x[e & 0xffffu] += kMgProlongScale * C.x[e >> 16];
+		}
+		__syncthreads();
+		for (int v = 0; v < kMgSweeps; v++) { tailSmooth<D>(F, list, nr, na); tailSmooth<D>(F, list, 0, nr); }
+	}
+	const MgLevel G = T.g[T.first], V = ldsLevel(lds, T, T.first);
+	for (int q = tid; q < mgCells(G); q += blockDim.x) G.x[q] = V.x[q];
 }
 
 // compact FP64 residual -> level-0 right-hand side (the level's b and x were zeroed)
@@ -220,6 +364,18 @@ bool Multigrid::layout(int dim, const int box_org[3], const int box_n[3], const 
 		if (cells[l] <= size_t(kMgCoarsestCells) || l + 1 == kMgMaxLevels) break;
 		for (int a = 0; a < 3; a++) dims[l + 1][a] = (a < D) ? (dims[l][a] + 1) / 2 : 1;
 		levels++;
+	}
+	// the coarse levels that fit in the LDS of one workgroup together (six
+	// padded arrays and a cell list each; 16-bit cell indices)
+	auto tailWords = [&](int l) {
+		const size_t pad = size_t(dims[l][0]) * dims[l][1];
+		return 6 * (cells[l] + 2 * pad) + cells[l];
+	};
+	tail_first = levels - 1;
+	size_t words = tailWords(levels - 1);
+	while (tail_first > 0 && cells[tail_first - 1] <= 65535 && words + tailWords(tail_first - 1) <= size_t(kMgTailFloats)) {
+		tail_first--;
+		words += tailWords(tail_first);
 	}
 	const size_t need = 6 * total + 64;
 	if (need > pool_cap) {
@@ -263,8 +419,8 @@ static void vcycle(Multigrid &M, const int *mat2grid, int n, const double *r, do
 	const MgLevel L0 = level(M, 0);
 	hipLaunchKernelGGL(kMgZero, dim3(mgBlocks(2 * M.cells[0])), dim3(kBlock), 0, s, L0.b, 2 * M.cells[0]);   // b and x are adjacent
 	hipLaunchKernelGGL(kMgLoad, dim3(mgBlocks(size_t(n))), dim3(kBlock), 0, s, n, mat2grid, B, L0, r);
-	const int top = M.levels - 1;
-	for (int l = 0; l < top; l++) {
+	const int first = M.tail_first;
+	for (int l = 0; l < first; l++) {
 		const MgLevel L = level(M, l);
 		const unsigned g = mgBlocks((M.cells[l] + 1) / 2);
 		for (int v = 0; v < kMgSweeps; v++) {
@@ -273,8 +429,22 @@ static void vcycle(Multigrid &M, const int *mat2grid, int n, const double *r, do
 		}
 		hipLaunchKernelGGL(kMgRestrict<D>, dim3(mgBlocks(M.cells[l + 1])), dim3(kBlock), 0, s, L, level(M, l + 1));
 	}
-	hipLaunchKernelGGL(kMgCoarsest<D>, dim3(1), dim3(1024), 0, s, level(M, top), kMgCoarsestSweeps);
-	for (int l = top - 1; l >= 0; l--) {
+	MgTail T{};
+	T.first = first;
+	T.levels = M.levels;
+	T.sweeps = kMgCoarsestSweeps;
+	int off = 0;
+	for (int l = first; l < M.levels; l++) {
+		T.g[l] = level(M, l);
+		T.off[l] = off;
+		off += int(6 * (M.cells[l] + 2 * size_t(M.dims[l][0]) * M.dims[l][1]));
+	}
+	for (int l = first; l < M.levels; l++) {
+		T.list[l] = off;
+		off += int(M.cells[l]);
+	}
+	hipLaunchKernelGGL(kMgTail<D>, dim3(1), dim3(1024), 0, s, T);
+	for (int l = first - 1; l >= 0; l--) {
 		const MgLevel L = level(M, l);
 		hipLaunchKernelGGL(kMgProlong<D>, dim3(mgBlocks(M.cells[l])), dim3(kBlock), 0, s, L, level(M, l + 1));
 		const unsigned g = mgBlocks((M.cells[l] + 1) / 2);

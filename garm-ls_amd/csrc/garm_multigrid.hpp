@@ -25,6 +25,7 @@ struct Multigrid {
 	int org[3] = {};          // box origin in cell coordinates
 	int gn[3] = {};           // cell-grid size
 	int D = 3;
+	int tail_first = 0;       // first of the coarse levels run in one workgroup's LDS
 	float *pool = nullptr;    // one allocation for every level array
 	size_t pool_cap = 0;      // floats
 	~Multigrid();
