@@ -39,6 +39,10 @@ from garm_amd.driver import FrameClock  # noqa: E402
 
 METRIC = "cell-updates/s at 256³, mixed+learned PCG vs plain FP64 PCG; HBM GB/s; vol err"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the dominant kernel at 256^3 from two rocprofv3 --pmc passes
+# (FETCH_SIZE, WRITE_SIZE, scaled by a copy kernel's reading):
+# profiles/dam_break_256_reinit_stage_traffic.md.  Used when --traffic is not given.
+MEASURED_TRAFFIC = {("reinit_stage", 256): 3.849e8}
 
 
 def log(*a):
@@ -168,7 +172,8 @@ def roofline(sim, stats, args, mode_name):
     ms, by = sim.time_kernel(capi.K_COPY, reps=args.kernel_reps)
     copy = {"avg_ms": round(ms, 5), "bytes": by, "gbs": round(by / ms / 1e6, 1)}
     return dom, kernels, {"kernel": dom, "bound": "hbm", "achieved": round(d["gbs"], 1), "peak": HBM_PEAK_GBS,
-                          "unit": "GB/s", "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": args.traffic,
+                          "unit": "GB/s", "frac": round(d["gbs"] / HBM_PEAK_GBS, 4),
+                          "traffic": args.traffic if args.traffic is not None else MEASURED_TRAFFIC.get((dom, args.n)),
                           "avg_launch_ms": round(d["avg_ms"], 5), "bytes_per_launch": d["bytes"],
                           "copy_kernel": copy}
 
@@ -220,7 +225,7 @@ def main():
     ap.add_argument("--scene", choices=["dam_break", "droplet"], default="dam_break")
     ap.add_argument("--scene-seed", type=int, default=19260817, help="droplet position seed")
     ap.add_argument("--mode", choices=list(MODES), default="learned", help="headline configuration")
-    ap.add_argument("--also", default="fp64,mixed", help="configurations run beside the headline one ('' for none)")
+    ap.add_argument("--also", default="fp64,mixed,multigrid", help="configurations run beside the headline one ('' for none)")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iterations", type=int, default=20000)
     ap.add_argument("--check-interval", type=int, default=16)
