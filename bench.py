@@ -39,6 +39,7 @@ from garm_amd.driver import FrameClock  # noqa: E402
 
 METRIC = "cell-updates/s at 256³, mixed+learned PCG vs plain FP64 PCG; HBM GB/s; vol err"
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0   # dense BF16 MFMA
 # HBM bytes per launch of the dominant kernel at 256^3 from two rocprofv3 --pmc passes
 # (FETCH_SIZE, WRITE_SIZE, scaled by a copy kernel's reading):
 # profiles/dam_break_256_reinit_stage_traffic.md.  Used when --traffic is not given.
@@ -134,10 +135,34 @@ def run_gpu(args, mode_name, net=None):
     return g, sim, stats, elapsed, (v1 - v0) / v0
 
 
-def summary(stats, elapsed, cells, drift):
+def phase_roofline(stats, cells, mode_name):
+    """Algorithmic FP64 bytes per cell of each phase (one read and one write of
+    every array it must touch, derived quantities on the fly) and the rate
+    they imply at the measured phase times: advection a1-a7 433 B
+    (216 + 48 + 169), WENO5 64 B per iteration (30 per reinitialisation),
+    glue a13 266 B, projection set-up and gradient 160 B plus 136 B (FP64
+    Jacobi PCG) or 68 B (FP32 inner) per iteration.  The flexible solvers'
+    iterations (V-cycles, network) have no per-cell byte count here."""
+    K = len(stats)
+    it = sum(s["pcg"]["iterations"] + s["pcg"]["outer_iterations"] for s in stats) / K
+    per_it = {"fp64": 136.0, "mixed": 68.0}.get(mode_name)
+    reinit = 30 * 64.0 * sum(s["reinitialized"] + s["restarted"] * (1 - s["reinitialized"]) for s in stats) / K
+    phases = {"advect": (433.0, "t_advect_ms"), "reinit_check": (reinit, "t_restart_ms"), "extrap_enforce": (266.0, "t_extrap_ms"),
+              "project": (160.0 + per_it * it if per_it else None, "t_project_ms")}
+    out = {}
+    for name, (b, key) in phases.items():
+        ms = sum(s[key] for s in stats) / K
+        out[name] = {"ms": round(ms, 3), "bytes_per_cell": round(b, 1) if b else None,
+                     "gbs": round(b * cells / ms / 1e6, 1) if b and ms > 0 else None,
+                     "frac": round(b * cells / ms / 1e6 / HBM_PEAK_GBS, 4) if b and ms > 0 else None}
+    return out
+
+
+def summary(stats, elapsed, cells, drift, mode_name):
     K = len(stats)
     it = [s["pcg"]["iterations"] for s in stats]
     return {"value": round(cells * K / elapsed, 1), "ms_per_step": round(1000 * elapsed / K, 3),
+            "phase_roofline": phase_roofline(stats, cells, mode_name),
             "phases_ms": {k: round(sum(s[k] for s in stats) / K, 3) for k in ("t_advect_ms", "t_restart_ms", "t_project_ms", "t_extrap_ms", "t_total_ms")},
             "pcg_iterations": {"mean": sum(it) / K, "min": min(it), "max": max(it),
                                "outer_mean": sum(s["pcg"]["outer_iterations"] for s in stats) / K,
@@ -176,6 +201,23 @@ def roofline(sim, stats, args, mode_name):
                           "traffic": args.traffic if args.traffic is not None else MEASURED_TRAFFIC.get((dom, args.n)),
                           "avg_launch_ms": round(d["avg_ms"], 5), "bytes_per_launch": d["bytes"],
                           "copy_kernel": copy}
+
+
+def network_roofline(sim, net, res, reps=10):
+    """One U-Net application on the bounding box the learned solve uses (the
+    unknowns' box, two cells of context, sizes rounded up to multiples of 4),
+    timed with HIP events over synthetic input, against the dense BF16 MFMA peak."""
+    import numpy as np
+    g = sim.get(capi.UNKNOWNS).reshape(res[2], res[1], res[0]) != 0
+    dims = []
+    for axis in (0, 1, 2):   # z, y, x
+        idx = np.nonzero(g.any(axis=tuple(a for a in (0, 1, 2) if a != axis)))[0]
+        dims.append(((int(idx[-1]) - int(idx[0]) + 5) + 3) // 4 * 4)
+    ms, flops = net.time_forward(tuple(dims), reps=reps)
+    tf = flops / ms / 1e9
+    return {"box": [dims[2], dims[1], dims[0]], "ms_per_application": round(ms, 4), "gflop_per_application": round(flops / 1e9, 1),
+            "bound": "mfma", "achieved": round(tf, 1), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 4)}
 
 
 def cpu_baseline(args):
@@ -252,13 +294,15 @@ def main():
 
     cells = args.n ** 3
     order = [m for m in args.also.split(",") if m and m != args.mode] + [args.mode]
-    results, net, train_info = {}, None, None
+    results, net, train_info, network = {}, None, None, None
     for m in order:
         learned = MODES[m][0] == capi.PCG_LEARNED
         if learned and net is None:
             net, train_info = train_network(args)
         g, sim, stats, elapsed, drift = run_gpu(args, m, net if learned else None)
-        results[m] = summary(stats, elapsed, cells, drift)
+        results[m] = summary(stats, elapsed, cells, drift, m)
+        if learned and network is None:
+            network = network_roofline(sim, net, g["res"])
         if m == args.mode:
             dom, kernels, roof = roofline(sim, stats, args, m)
         sim.close()
@@ -285,6 +329,7 @@ def main():
         "volume_drift_difference_vs_fp64": head.get("volume_drift_difference_vs_fp64"),
         "modes": results,
         "network_training": train_info,
+        "network": network,
         "kernels": {k: {kk: round(vv, 5) if isinstance(vv, float) else vv for kk, vv in v.items()} for k, v in kernels.items()},
         "roofline": roof,
         "cpu_baseline": None,

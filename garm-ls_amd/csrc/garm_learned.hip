@@ -628,6 +628,50 @@ int garm_unet_loss_grad(garm_unet *u, const int dims[3], const float *r, const u
 	return unetGetParams(u->U, grad, true, u->err) ? GARM_OK : GARM_EHIP;
 }
 
+// synthetic network input for timing: a smooth residual and a full mask
+__global__ void kSyntheticInput(bf16 *in, int nx, int ny, int nz)
+{
+	const size_t vox = size_t(nx) * ny * nz;
+	for (size_t v = blockIdx.x * size_t(blockDim.x) + threadIdx.x; v < vox; v += size_t(gridDim.x) * blockDim.x) {
+		const int i = int(v % nx), j = int((v / nx) % ny), k = int(v / (size_t(nx) * ny));
+		in[v * 4 + 0] = f2bf(__sinf(0.11f * i) * __cosf(0.07f * j + 0.05f * k));
+		in[v * 4 + 1] = f2bf(1.0f);
+		in[v * 4 + 2] = 0;
+		in[v * 4 + 3] = 0;
+	}
+}
+
+int garm_unet_time_forward(garm_unet *u, const int dims[3], int reps, double *avg_ms, double *flops)
+{
+	size_t vox;
+	if (!u || !avg_ms || !flops || reps < 1 || !dims3(dims, vox)) return u ? ufail(u, GARM_EINVAL, "bad network grid or arguments") : GARM_EINVAL;
+	if (!u->U.buf.reserve(vox, u->U.buf.has_grad, u->err) || !u->U.buf.setDims(dims[0], dims[1], dims[2], u->err)) return GARM_ENOMEM;
+	hipLaunchKernelGGL(kSyntheticInput, dim3(1024), dim3(kBlock), 0, u->s, u->U.buf.in, dims[0], dims[1], dims[2]);
+	unetForward(u->U, u->s);   // warm-up
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+		if (e0) (void)hipEventDestroy(e0);
+		return ufail(u, GARM_EHIP, "event creation failed");
+	}
+	(void)hipEventRecord(e0, u->s);
+	for (int q = 0; q < reps; q++) unetForward(u->U, u->s);
+	(void)hipEventRecord(e1, u->s);
+	float ms = 0;
+	const bool ok = hipGetLastError() == hipSuccess && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	if (!ok) return ufail(u, GARM_EHIP, "network timing failed");
+	*avg_ms = double(ms) / reps;
+	// multiply-adds of the true channel counts: 2 * 27 * cin * cout per voxel of each layer's level, plus the 1x1 output
+	double f = 0;
+	for (int l = 0; l < kUnetLayers; l++) {
+		const UnetLayer &L = u->U.L[l];
+		f += 2.0 * 27 * L.cin * L.cout * double(u->U.buf.vox[L.level]);
+	}
+	*flops = f + 2.0 * 16 * double(vox);
+	return GARM_OK;
+}
+
 int garm_unet_train(garm_unet *u, const int dims[3], int n_pairs, const float *rhs, const float *pressure, const uint8_t *mask, int steps,
 	float lr, unsigned long long seed, float *loss_history)
 {

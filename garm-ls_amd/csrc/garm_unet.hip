@@ -1,7 +1,7 @@
 // Three-level 3D U-Net (16/32/64 channels) on gfx950: BF16 channels-last
 // activations and weights, 3x3x3 convolutions as implicit GEMM on
-// v_mfma_f32_16x16x16_bf16 with FP32 accumulation, FP32 master weights and
-// Adam.  The network has no counterpart in the reference; its layer order,
+// v_mfma_f32_16x16x32_bf16 (v_mfma_f32_16x16x16_bf16 for the weight
+// gradient) with FP32 accumulation, FP32 master weights and Adam.  The network has no counterpart in the reference; its layer order,
 // layouts and gradients follow oracle/unet_oracle.py, which the tests hold it
 // against.
 //
@@ -44,8 +44,19 @@ struct ConvArgs {
 	int relu;
 };
 
+// K = 32 per MFMA (v_mfma_f32_16x16x32_bf16): the reduction runs over pairs
+// of taps of one 16-channel chunk, lanes 0-31 of a fragment holding the
+// chunk's channels at the first tap of the pair and lanes 32-63 at the
+// second (the 28th tap is a zero weight).  Lane l of an A fragment holds
+// A[l % 16][8 (l / 16) .. + 7], of a B fragment B[8 (l / 16) .. + 7][l % 16];
+// D as for the K = 16 instruction.
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef short s8v __attribute__((ext_vector_type(8)));
+
 __global__ __launch_bounds__(kUB) void kConv3(ConvArgs A)
 {
+	// input halo (18 x 6 x 6 voxels, 16 channels); the output tile (256
+	// voxels x 32 channels) reuses it once the reduction is done
 	__shared__ __align__(16) bf16 tin[HZ * HY * HX * 16];
 	__shared__ __align__(16) bf16 tw[27 * 32 * 16];
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -55,62 +66,121 @@ __global__ __launch_bounds__(kUB) void kConv3(ConvArgs A)
 	const int zt = blockIdx.z % nzt, grp = blockIdx.z / nzt;
 	const int z0 = zt * TZ, co0 = grp * 32;
 	const int NF = min(32, A.coutP - co0) / 16;
+	const int half = (kg & 1) * 8, second = kg >> 1;   // channel half and tap of the pair this lane holds
 	f4 acc[4][2];
 #pragma unroll
 	for (int f = 0; f < 4; f++)
 #pragma unroll
 		for (int q = 0; q < 2; q++) acc[f][q] = f4{0, 0, 0, 0};
 
+	// 16-byte staging (all loads of a thread in flight together) when every
+	// voxel's channels come in whole groups of eight; 8-byte otherwise
+	const bool wide = (A.inStride & 7) == 0 && (A.Cin & 7) == 0;
 	for (int ci0 = 0; ci0 < A.Cin; ci0 += 16) {
-		for (int q = tid; q < HZ * HY * HX * 4; q += kUB) {
-			const int quad = q & 3, v = q >> 2;
-			const int hx = v % HX, hy = (v / HX) % HY, hz = v / (HX * HY);
-			const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
-			const int ch = ci0 + quad * 4;
-			uint2 val = make_uint2(0, 0);
-			if (gx >= 0 && gx < A.nx && gy >= 0 && gy < A.ny && gz >= 0 && gz < A.nz && ch < A.Cin)
-				val = *reinterpret_cast<const uint2 *>(A.in + (size_t(gz * A.ny + gy) * A.nx + gx) * A.inStride + ch);
-			*reinterpret_cast<uint2 *>(tin + v * 16 + quad * 4) = val;
+		if (wide) {
+			constexpr int kVecs = HZ * HY * HX * 2, kIt = (kVecs + kUB - 1) / kUB;
+			uint4 hv[kIt];
+#pragma unroll
+			for (int r = 0; r < kIt; r++) {
+				const int q = tid + r * kUB, v = q >> 1, ch = ci0 + (q & 1) * 8;
+				const int hx = v % HX, hy = (v / HX) % HY, hz = v / (HX * HY);
+				const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+				hv[r] = make_uint4(0, 0, 0, 0);
+				if (q < kVecs && gx >= 0 && gx < A.nx && gy >= 0 && gy < A.ny && gz >= 0 && gz < A.nz && ch < A.Cin)
+					hv[r] = *reinterpret_cast<const uint4 *>(A.in + (size_t(gz * A.ny + gy) * A.nx + gx) * A.inStride + ch);
+			}
+#pragma unroll
+			for (int r = 0; r < kIt; r++) {
+				const int q = tid + r * kUB;
+				if (q < kVecs) *reinterpret_cast<uint4 *>(tin + q * 8) = hv[r];
+			}
+		} else {
+			for (int q = tid; q < HZ * HY * HX * 4; q += kUB) {
+				const int quad = q & 3, v = q >> 2;
+				const int hx = v % HX, hy = (v / HX) % HY, hz = v / (HX * HY);
+				const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+				const int ch = ci0 + quad * 4;
+				uint2 val = make_uint2(0, 0);
+				if (gx >= 0 && gx < A.nx && gy >= 0 && gy < A.ny && gz >= 0 && gz < A.nz && ch < A.Cin)
+					val = *reinterpret_cast<const uint2 *>(A.in + (size_t(gz * A.ny + gy) * A.nx + gx) * A.inStride + ch);
+				*reinterpret_cast<uint2 *>(tin + v * 16 + quad * 4) = val;
+			}
 		}
-		for (int q = tid; q < 27 * NF * 16 * 4; q += kUB) {
-			const int quad = q & 3, rr = q >> 2;
-			const int co = rr % (NF * 16), t = rr / (NF * 16);
-			const uint2 val = *reinterpret_cast<const uint2 *>(A.w + (size_t(t) * A.coutP + co0 + co) * A.cinP + ci0 + quad * 4);
-			*reinterpret_cast<uint2 *>(tw + (t * 32 + co) * 16 + quad * 4) = val;
+		{
+			// weights [27][NF * 16][16] of this chunk (cinP and coutP are multiples of 16)
+			constexpr int kMaxVecs = 27 * 32 * 2, kIt = (kMaxVecs + kUB - 1) / kUB;
+			const int vecs = 27 * NF * 16 * 2;
+			uint4 wv[kIt];
+#pragma unroll
+			for (int r = 0; r < kIt; r++) {
+				const int q = tid + r * kUB, rr = q >> 1, part = q & 1;
+				const int co = rr % (NF * 16), t = rr / (NF * 16);
+				wv[r] = q < vecs ? *reinterpret_cast<const uint4 *>(A.w + (size_t(t) * A.coutP + co0 + co) * A.cinP + ci0 + part * 8) : make_uint4(0, 0, 0, 0);
+			}
+#pragma unroll
+			for (int r = 0; r < kIt; r++) {
+				const int q = tid + r * kUB, rr = q >> 1, part = q & 1;
+				const int co = rr % (NF * 16), t = rr / (NF * 16);
+				if (q < vecs) *reinterpret_cast<uint4 *>(tw + (t * 32 + co) * 16 + part * 8) = wv[r];
+			}
 		}
 		__syncthreads();
-		for (int t = 0; t < 27; t++) {
+#pragma unroll 2
+		for (int p = 0; p < 14; p++) {
+			const int tl = 2 * p + second;                  // this lane's tap
+			const bool none = tl > 26;                      // the 28th tap: zero weight
+			const int t = none ? 26 : tl;                   // (a valid voxel for the A operand)
 			const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
-			s4 b[2];
-			b[0] = *reinterpret_cast<const s4 *>(tw + (t * 32 + li) * 16 + kg * 4);
-			b[1] = NF > 1 ? *reinterpret_cast<const s4 *>(tw + (t * 32 + 16 + li) * 16 + kg * 4) : b[0];
+			bf8v b[2];
+#pragma unroll
+			for (int q = 0; q < 2; q++) {
+				const s8v raw = *reinterpret_cast<const s8v *>(tw + (t * 32 + q * 16 + li) * 16 + half);
+				b[q] = __builtin_bit_cast(bf8v, none ? s8v{0, 0, 0, 0, 0, 0, 0, 0} : raw);
+			}
 #pragma unroll
 			for (int f = 0; f < 4; f++) {
-				const s4 a = *reinterpret_cast<const s4 *>(tin + (((wave + dz) * HY + f + dy) * HX + li + dx) * 16 + kg * 4);
-				acc[f][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[0], acc[f][0], 0, 0, 0);
-				if (NF > 1) acc[f][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[1], acc[f][1], 0, 0, 0);
+				const bf8v a = __builtin_bit_cast(bf8v, *reinterpret_cast<const s8v *>(tin + (((wave + dz) * HY + f + dy) * HX + li + dx) * 16 + half));
+				acc[f][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[0], acc[f][0], 0, 0, 0);
+				if (NF > 1) acc[f][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[1], acc[f][1], 0, 0, 0);
 			}
 		}
 		__syncthreads();
 	}
-	const int z = z0 + wave;
+	// bias, activation, BF16: the tile goes to LDS as [voxel][32 channels]
+	// and leaves as whole 16-byte words when the output layout allows
+	bf16 *tout = tin;   // 256 x 32 BF16 = 16 KB <= the halo buffer
 #pragma unroll
-	for (int f = 0; f < 4; f++) {
-		const int y = y0 + f;
+	for (int f = 0; f < 4; f++)
 #pragma unroll
 		for (int q = 0; q < 2; q++) {
 			if (q >= NF) continue;
 			const int co = co0 + q * 16 + li;
-			if (co >= A.Cout || y >= A.ny || z >= A.nz) continue;
-			const float bb = A.bias ? A.bias[co] : 0.0f;
+			const float bb = A.bias && co < A.Cout ? A.bias[co] : 0.0f;
 #pragma unroll
 			for (int i = 0; i < 4; i++) {
-				const int x = x0 + kg * 4 + i;
-				if (x >= A.nx) continue;
 				float v = acc[f][q][i] + bb;
 				if (A.relu) v = fmaxf(v, 0.0f);
-				A.out[(size_t(z * A.ny + y) * A.nx + x) * A.outStride + A.outOff + co] = f2bf(v);
+				const int vox = (wave * TY + f) * TX + kg * 4 + i;   // tile voxel (x fastest)
+				tout[vox * 32 + q * 16 + li] = f2bf(v);
 			}
+		}
+	__syncthreads();
+	const int nch = min(NF * 16, A.Cout - co0);
+	if (nch == NF * 16 && (A.outStride & 7) == 0 && ((A.outOff + co0) & 7) == 0) {
+		const int words = NF * 2;   // 16-byte words per voxel
+		for (int q = tid; q < TX * TY * TZ * words; q += kUB) {
+			const int vox = q / words, w8 = q % words;
+			const int x = x0 + vox % TX, y = y0 + (vox / TX) % TY, z = z0 + vox / (TX * TY);
+			if (x >= A.nx || y >= A.ny || z >= A.nz) continue;
+			*reinterpret_cast<uint4 *>(A.out + (size_t(z * A.ny + y) * A.nx + x) * A.outStride + A.outOff + co0 + w8 * 8) =
+				*reinterpret_cast<const uint4 *>(tout + vox * 32 + w8 * 8);
+		}
+	} else {
+		for (int q = tid; q < TX * TY * TZ * nch; q += kUB) {
+			const int vox = q / nch, c = q % nch;
+			const int x = x0 + vox % TX, y = y0 + (vox / TX) % TY, z = z0 + vox / (TX * TY);
+			if (x >= A.nx || y >= A.ny || z >= A.nz) continue;
+			A.out[(size_t(z * A.ny + y) * A.nx + x) * A.outStride + A.outOff + co0 + c] = tout[vox * 32 + c];
 		}
 	}
 }
@@ -222,64 +292,102 @@ __device__ __forceinline__ void coords(size_t v, int nx, int ny, int &x, int &y,
 }
 
 // 2x2x2 mean (child grid n = 2 * parent)
+// Eight channels per thread (16 bytes): every channel count, stride and
+// offset of these tensors is a multiple of eight.
+__device__ __forceinline__ void unpack8(const uint4 &u, float (&f)[8])
+{
+	const unsigned w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+	for (int i = 0; i < 4; i++) {
+		f[2 * i] = __uint_as_float(w[i] << 16);
+		f[2 * i + 1] = __uint_as_float(w[i] & 0xffff0000u);
+	}
+}
+
+__device__ __forceinline__ uint4 pack8(const float (&f)[8])
+{
+	unsigned w[4];
+#pragma unroll
+	for (int i = 0; i < 4; i++) w[i] = unsigned(f2bf(f[2 * i])) | unsigned(f2bf(f[2 * i + 1])) << 16;
+	return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+__device__ __forceinline__ const uint4 &vec8(const bf16 *p) { return *reinterpret_cast<const uint4 *>(p); }
+__device__ __forceinline__ uint4 &vec8(bf16 *p) { return *reinterpret_cast<uint4 *>(p); }
+
+// 2x2x2 mean pooling of channels [inOff, inOff + C) (child grid n)
 __global__ void kPool(const bf16 *in, int inStride, int inOff, int C, bf16 *out, int outStride, int nx, int ny, int nz)
 {
-	const int px = nx / 2, py = ny / 2, pz = nz / 2;
-	const size_t total = size_t(px) * py * pz * C;
+	const int px = nx / 2, py = ny / 2, pz = nz / 2, G = C / 8;
+	const size_t total = size_t(px) * py * pz * G;
 	for (size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x; e < total; e += size_t(gridDim.x) * blockDim.x) {
-		const int c = int(e % C);
+		const int c = int(e % G) * 8;
 		int x, y, z;
-		coords(e / C, px, py, x, y, z);
-		float s = 0;
+		coords(e / G, px, py, x, y, z);
+		float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 		for (int k = 0; k < 8; k++) {
 			const int cx = 2 * x + (k & 1), cy = 2 * y + (k >> 1 & 1), cz = 2 * z + (k >> 2);
-			s += bf2f(in[(size_t(cz * ny + cy) * nx + cx) * inStride + inOff + c]);
+			float f[8];
+			unpack8(vec8(in + (size_t(cz * ny + cy) * nx + cx) * inStride + inOff + c), f);
+#pragma unroll
+			for (int i = 0; i < 8; i++) s[i] += f[i];
 		}
-		out[(size_t(z * py + y) * px + x) * outStride + c] = f2bf(s * 0.125f);
+#pragma unroll
+		for (int i = 0; i < 8; i++) s[i] *= 0.125f;
+		vec8(out + (size_t(z * py + y) * px + x) * outStride + c) = pack8(s);
 	}
 }
 
 // gradient of the mean: child += parent / 8
 __global__ void kPoolBack(const bf16 *gp, int pStride, int C, bf16 *gc, int cStride, int cOff, int nx, int ny, int nz)
 {
-	const size_t total = size_t(nx) * ny * nz * C;
+	const int G = C / 8;
+	const size_t total = size_t(nx) * ny * nz * G;
 	for (size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x; e < total; e += size_t(gridDim.x) * blockDim.x) {
-		const int c = int(e % C);
+		const int c = int(e % G) * 8;
 		int x, y, z;
-		coords(e / C, nx, ny, x, y, z);
-		const float g = bf2f(gp[(size_t((z / 2) * (ny / 2) + y / 2) * (nx / 2) + x / 2) * pStride + c]) * 0.125f;
-		bf16 &dst = gc[(size_t(z * ny + y) * nx + x) * cStride + cOff + c];
-		dst = f2bf(bf2f(dst) + g);
+		coords(e / G, nx, ny, x, y, z);
+		float g[8], d[8];
+		unpack8(vec8(gp + (size_t((z / 2) * (ny / 2) + y / 2) * (nx / 2) + x / 2) * pStride + c), g);
+		bf16 *dst = gc + (size_t(z * ny + y) * nx + x) * cStride + cOff + c;
+		unpack8(vec8(dst), d);
+#pragma unroll
+		for (int i = 0; i < 8; i++) d[i] += g[i] * 0.125f;
+		vec8(dst) = pack8(d);
 	}
 }
 
 // nearest-neighbour x2 into channels [0, C) of a wider tensor (child grid n)
 __global__ void kUp(const bf16 *in, int inStride, int C, bf16 *out, int outStride, int nx, int ny, int nz)
 {
-	const size_t total = size_t(nx) * ny * nz * C;
+	const int G = C / 8;
+	const size_t total = size_t(nx) * ny * nz * G;
 	for (size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x; e < total; e += size_t(gridDim.x) * blockDim.x) {
-		const int c = int(e % C);
+		const int c = int(e % G) * 8;
 		int x, y, z;
-		coords(e / C, nx, ny, x, y, z);
-		out[(size_t(z * ny + y) * nx + x) * outStride + c] = in[(size_t((z / 2) * (ny / 2) + y / 2) * (nx / 2) + x / 2) * inStride + c];
+		coords(e / G, nx, ny, x, y, z);
+		vec8(out + (size_t(z * ny + y) * nx + x) * outStride + c) = vec8(in + (size_t((z / 2) * (ny / 2) + y / 2) * (nx / 2) + x / 2) * inStride + c);
 	}
 }
 
 // gradient of the copy: parent = sum of its eight children
 __global__ void kUpBack(const bf16 *gc, int cStride, int C, bf16 *gp, int pStride, int nx, int ny, int nz)
 {
-	const int px = nx / 2, py = ny / 2, pz = nz / 2;
-	const size_t total = size_t(px) * py * pz * C;
+	const int px = nx / 2, py = ny / 2, pz = nz / 2, G = C / 8;
+	const size_t total = size_t(px) * py * pz * G;
 	for (size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x; e < total; e += size_t(gridDim.x) * blockDim.x) {
-		const int c = int(e % C);
+		const int c = int(e % G) * 8;
 		int x, y, z;
-		coords(e / C, px, py, x, y, z);
-		float s = 0;
+		coords(e / G, px, py, x, y, z);
+		float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 		for (int k = 0; k < 8; k++) {
 			const int cx = 2 * x + (k & 1), cy = 2 * y + (k >> 1 & 1), cz = 2 * z + (k >> 2);
-			s += bf2f(gc[(size_t(cz * ny + cy) * nx + cx) * cStride + c]);
+			float f[8];
+			unpack8(vec8(gc + (size_t(cz * ny + cy) * nx + cx) * cStride + c), f);
+#pragma unroll
+			for (int i = 0; i < 8; i++) s[i] += f[i];
 		}
-		gp[(size_t(z * py + y) * px + x) * pStride + c] = f2bf(s);
+		vec8(gp + (size_t(z * py + y) * px + x) * pStride + c) = pack8(s);
 	}
 }
 
@@ -649,16 +757,16 @@ void unetForward(Unet &U, hipStream_t s)
 	const int *n0 = B.n[0], *n1 = B.n[1];
 	run(0);
 	run(1);
-	hipLaunchKernelGGL(kPool, dim3(blocksFor(B.vox[1] * 16)), dim3(kUB), 0, s, (const bf16 *)B.c1, 48, 32, 16, B.p1, 16, n0[0], n0[1], n0[2]);
+	hipLaunchKernelGGL(kPool, dim3(blocksFor(B.vox[1] * 2)), dim3(kUB), 0, s, (const bf16 *)B.c1, 48, 32, 16, B.p1, 16, n0[0], n0[1], n0[2]);
 	run(2);
 	run(3);
-	hipLaunchKernelGGL(kPool, dim3(blocksFor(B.vox[2] * 32)), dim3(kUB), 0, s, (const bf16 *)B.c2, 96, 64, 32, B.p2, 32, n1[0], n1[1], n1[2]);
+	hipLaunchKernelGGL(kPool, dim3(blocksFor(B.vox[2] * 4)), dim3(kUB), 0, s, (const bf16 *)B.c2, 96, 64, 32, B.p2, 32, n1[0], n1[1], n1[2]);
 	run(4);
 	run(5);
-	hipLaunchKernelGGL(kUp, dim3(blocksFor(B.vox[1] * 64)), dim3(kUB), 0, s, (const bf16 *)B.b3, 64, 64, B.c2, 96, n1[0], n1[1], n1[2]);
+	hipLaunchKernelGGL(kUp, dim3(blocksFor(B.vox[1] * 8)), dim3(kUB), 0, s, (const bf16 *)B.b3, 64, 64, B.c2, 96, n1[0], n1[1], n1[2]);
 	run(6);
 	run(7);
-	hipLaunchKernelGGL(kUp, dim3(blocksFor(B.vox[0] * 32)), dim3(kUB), 0, s, (const bf16 *)B.d2, 32, 32, B.c1, 48, n0[0], n0[1], n0[2]);
+	hipLaunchKernelGGL(kUp, dim3(blocksFor(B.vox[0] * 4)), dim3(kUB), 0, s, (const bf16 *)B.d2, 32, 32, B.c1, 48, n0[0], n0[1], n0[2]);
 	run(8);
 	run(9);
 	hipLaunchKernelGGL(kOut, dim3(blocksFor(B.vox[0])), dim3(kUB), 0, s, (const bf16 *)B.d1, (const bf16 *)B.in, (const float *)U.ow,
@@ -731,16 +839,16 @@ bool unetBackward(Unet &U, hipStream_t s, std::string &err)
 	};
 	back(9);
 	back(8);
-	hipLaunchKernelGGL(kUpBack, dim3(blocksFor(B.vox[1] * 32)), dim3(kUB), 0, s, (const bf16 *)B.g_c1, 48, 32, B.g_d2, 32, n0[0], n0[1], n0[2]);
+	hipLaunchKernelGGL(kUpBack, dim3(blocksFor(B.vox[1] * 4)), dim3(kUB), 0, s, (const bf16 *)B.g_c1, 48, 32, B.g_d2, 32, n0[0], n0[1], n0[2]);
 	back(7);
 	back(6);
-	hipLaunchKernelGGL(kUpBack, dim3(blocksFor(B.vox[2] * 64)), dim3(kUB), 0, s, (const bf16 *)B.g_c2, 96, 64, B.g_b3, 64, n1[0], n1[1], n1[2]);
+	hipLaunchKernelGGL(kUpBack, dim3(blocksFor(B.vox[2] * 8)), dim3(kUB), 0, s, (const bf16 *)B.g_c2, 96, 64, B.g_b3, 64, n1[0], n1[1], n1[2]);
 	back(5);
 	back(4);
-	hipLaunchKernelGGL(kPoolBack, dim3(blocksFor(B.vox[1] * 32)), dim3(kUB), 0, s, (const bf16 *)B.g_p2, 32, 32, B.g_c2, 96, 64, n1[0], n1[1], n1[2]);
+	hipLaunchKernelGGL(kPoolBack, dim3(blocksFor(B.vox[1] * 4)), dim3(kUB), 0, s, (const bf16 *)B.g_p2, 32, 32, B.g_c2, 96, 64, n1[0], n1[1], n1[2]);
 	back(3);
 	back(2);
-	hipLaunchKernelGGL(kPoolBack, dim3(blocksFor(B.vox[0] * 16)), dim3(kUB), 0, s, (const bf16 *)B.g_p1, 16, 16, B.g_c1, 48, 32, n0[0], n0[1], n0[2]);
+	hipLaunchKernelGGL(kPoolBack, dim3(blocksFor(B.vox[0] * 2)), dim3(kUB), 0, s, (const bf16 *)B.g_p1, 16, 16, B.g_c1, 48, 32, n0[0], n0[1], n0[2]);
 	back(1);
 	back(0);
 	return true;

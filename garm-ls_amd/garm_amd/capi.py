@@ -129,6 +129,7 @@ def lib():
             "garm_unet_forward": (i, [vp, ip, vp, vp, vp]),
             "garm_unet_loss_grad": (i, [vp, ip, vp, vp, vp, dp, vp, sz]),
             "garm_unet_train": (i, [vp, ip, i, vp, vp, vp, i, ctypes.c_float, ctypes.c_ulonglong, vp]),
+            "garm_unet_time_forward": (i, [vp, ip, i, dp, dp]),
             "garm_attach_preconditioner": (i, [vp, vp]),
         }
         for name, (res, args) in sig.items():
@@ -365,6 +366,12 @@ class Unet:
         y = np.empty_like(r)
         self._check(lib().garm_unet_forward(self.h, self._dims(r.shape), r.ctypes.data, m.ctypes.data, y.ctypes.data))
         return y
+
+    def time_forward(self, shape, reps=10):
+        """(average ms, FLOPs) of one forward pass on a (nz, ny, nx) grid of synthetic input."""
+        ms, fl = ctypes.c_double(), ctypes.c_double()
+        self._check(lib().garm_unet_time_forward(self.h, self._dims(shape), reps, ctypes.byref(ms), ctypes.byref(fl)))
+        return ms.value, fl.value
 
     def loss_grad(self, r, mask, target):
         r = np.ascontiguousarray(r, dtype=np.float32)

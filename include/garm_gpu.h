@@ -284,6 +284,11 @@ int garm_unet_loss_grad(garm_unet *net, const int dims[3], const float *r, const
  * (optional) receives `steps` losses. */
 int garm_unet_train(garm_unet *net, const int dims[3], int n_pairs, const float *rhs, const float *pressure, const uint8_t *mask,
 	int steps, float lr, unsigned long long seed, float *loss_history);
+/* Average time of `reps` forward passes on a dims grid of synthetic input,
+ * after one untimed pass, measured with HIP events on the network's stream,
+ * and the multiply-add FLOPs of one pass (2 * 27 * cin * cout per voxel of
+ * every layer's level, plus the 1x1 output). */
+int garm_unet_time_forward(garm_unet *net, const int dims[3], int reps, double *avg_ms, double *flops);
 /* Uses `net` for GARM_PCG_LEARNED solves of this context (NULL detaches).
  * The network must outlive the attachment and live on the same device. */
 int garm_attach_preconditioner(garm_ctx *ctx, garm_unet *net);
