@@ -44,6 +44,9 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0   # dense BF16 MFMA
 # (FETCH_SIZE, WRITE_SIZE, scaled by a copy kernel's reading):
 # profiles/dam_break_256_reinit_stage_traffic.md.  Used when --traffic is not given.
 MEASURED_TRAFFIC = {("reinit_stage", 256): 3.849e8}
+# Vector-ALU issue of the same kernel (SQ_INSTS_VALU against two wave64
+# instructions per CU-cycle): profiles/dam_break_256_valu_issue.md.
+MEASURED_VALU_ISSUE = {("reinit_stage", 256): 0.49}
 
 
 def log(*a):
@@ -199,6 +202,7 @@ def roofline(sim, stats, args, mode_name):
     return dom, kernels, {"kernel": dom, "bound": "hbm", "achieved": round(d["gbs"], 1), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(d["gbs"] / HBM_PEAK_GBS, 4),
                           "traffic": args.traffic if args.traffic is not None else MEASURED_TRAFFIC.get((dom, args.n)),
+                          "valu_issue_frac": MEASURED_VALU_ISSUE.get((dom, args.n)),
                           "avg_launch_ms": round(d["avg_ms"], 5), "bytes_per_launch": d["bytes"],
                           "copy_kernel": copy}
 

@@ -53,12 +53,15 @@ struct ConvArgs {
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 typedef short s8v __attribute__((ext_vector_type(8)));
 
+template <bool WIDE>
 __global__ __launch_bounds__(kUB) void kConv3(ConvArgs A)
 {
-	// input halo (18 x 6 x 6 voxels, 16 channels); the output tile (256
-	// voxels x 32 channels) reuses it once the reduction is done
-	__shared__ __align__(16) bf16 tin[HZ * HY * HX * 16];
-	__shared__ __align__(16) bf16 tw[27 * 32 * 16];
+	// input halo (18 x 6 x 6 voxels, 16 channels), then the chunk's weights
+	// [27][NF * 16][16] (the launch sizes the LDS for NF, so a 16-channel
+	// output takes 34 KB and four workgroups fit on a CU); the output tile
+	// (256 voxels x 32 channels) reuses the halo once the reduction is done
+	extern __shared__ __align__(16) bf16 lds[];
+	bf16 *tin = lds, *tw = lds + HZ * HY * HX * 16;
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const int kg = lane >> 4, li = lane & 15;
 	const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
@@ -66,6 +69,7 @@ __global__ __launch_bounds__(kUB) void kConv3(ConvArgs A)
 	const int zt = blockIdx.z % nzt, grp = blockIdx.z / nzt;
 	const int z0 = zt * TZ, co0 = grp * 32;
 	const int NF = min(32, A.coutP - co0) / 16;
+	const int WS = NF * 16;   // weight rows per tap in LDS
 	const int half = (kg & 1) * 8, second = kg >> 1;   // channel half and tap of the pair this lane holds
 	f4 acc[4][2];
 #pragma unroll
@@ -74,10 +78,9 @@ __global__ __launch_bounds__(kUB) void kConv3(ConvArgs A)
 		for (int q = 0; q < 2; q++) acc[f][q] = f4{0, 0, 0, 0};
 
 	// 16-byte staging (all loads of a thread in flight together) when every
-	// voxel's channels come in whole groups of eight; 8-byte otherwise
-	const bool wide = (A.inStride & 7) == 0 && (A.Cin & 7) == 0;
+	// voxel's channels come in whole groups of eight (WIDE); 8-byte otherwise
 	for (int ci0 = 0; ci0 < A.Cin; ci0 += 16) {
-		if (wide) {
+		if constexpr (WIDE) {
 			constexpr int kVecs = HZ * HY * HX * 2, kIt = (kVecs + kUB - 1) / kUB;
 			uint4 hv[kIt];
 #pragma unroll
@@ -95,15 +98,22 @@ __global__ __launch_bounds__(kUB) void kConv3(ConvArgs A)
 				if (q < kVecs) *reinterpret_cast<uint4 *>(tin + q * 8) = hv[r];
 			}
 		} else {
-			for (int q = tid; q < HZ * HY * HX * 4; q += kUB) {
-				const int quad = q & 3, v = q >> 2;
+			constexpr int kVecs = HZ * HY * HX * 4, kIt = (kVecs + kUB - 1) / kUB;
+			uint2 hv[kIt];
+#pragma unroll
+			for (int r = 0; r < kIt; r++) {
+				const int q = tid + r * kUB, quad = q & 3, v = q >> 2;
 				const int hx = v % HX, hy = (v / HX) % HY, hz = v / (HX * HY);
 				const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
 				const int ch = ci0 + quad * 4;
-				uint2 val = make_uint2(0, 0);
-				if (gx >= 0 && gx < A.nx && gy >= 0 && gy < A.ny && gz >= 0 && gz < A.nz && ch < A.Cin)
-					val = *reinterpret_cast<const uint2 *>(A.in + (size_t(gz * A.ny + gy) * A.nx + gx) * A.inStride + ch);
-				*reinterpret_cast<uint2 *>(tin + v * 16 + quad * 4) = val;
+				hv[r] = make_uint2(0, 0);
+				if (q < kVecs && gx >= 0 && gx < A.nx && gy >= 0 && gy < A.ny && gz >= 0 && gz < A.nz && ch < A.Cin)
+					hv[r] = *reinterpret_cast<const uint2 *>(A.in + (size_t(gz * A.ny + gy) * A.nx + gx) * A.inStride + ch);
+			}
+#pragma unroll
+			for (int r = 0; r < kIt; r++) {
+				const int q = tid + r * kUB;
+				if (q < kVecs) *reinterpret_cast<uint2 *>(tin + q * 4) = hv[r];
 			}
 		}
 		{
@@ -121,7 +131,7 @@ __global__ __launch_bounds__(kUB) void kConv3(ConvArgs A)
 			for (int r = 0; r < kIt; r++) {
 				const int q = tid + r * kUB, rr = q >> 1, part = q & 1;
 				const int co = rr % (NF * 16), t = rr / (NF * 16);
-				if (q < vecs) *reinterpret_cast<uint4 *>(tw + (t * 32 + co) * 16 + part * 8) = wv[r];
+				if (q < vecs) *reinterpret_cast<uint4 *>(tw + (t * WS + co) * 16 + part * 8) = wv[r];
 			}
 		}
 		__syncthreads();
@@ -134,7 +144,7 @@ __global__ __launch_bounds__(kUB) void kConv3(ConvArgs A)
 			bf8v b[2];
 #pragma unroll
 			for (int q = 0; q < 2; q++) {
-				const s8v raw = *reinterpret_cast<const s8v *>(tw + (t * 32 + q * 16 + li) * 16 + half);
+				const s8v raw = *reinterpret_cast<const s8v *>(tw + (t * WS + (q < NF ? q : 0) * 16 + li) * 16 + half);
 				b[q] = __builtin_bit_cast(bf8v, none ? s8v{0, 0, 0, 0, 0, 0, 0, 0} : raw);
 			}
 #pragma unroll
@@ -707,7 +717,9 @@ static void conv(const UnetBuffers &B, const UnetLayer &L, int level, const bf16
 	A.relu = relu ? 1 : 0;
 	const int groups = (coutP + 31) / 32;
 	const dim3 g((A.nx + TX - 1) / TX, (A.ny + TY - 1) / TY, ((A.nz + TZ - 1) / TZ) * groups);
-	hipLaunchKernelGGL(kConv3, g, dim3(kUB), 0, s, A);
+	const size_t lds = size_t(HZ * HY * HX * 16 + 27 * std::min(32, coutP) * 16) * sizeof(bf16);
+	if ((inStride & 7) == 0 && (cin & 7) == 0) hipLaunchKernelGGL(kConv3<true>, g, dim3(kUB), lds, s, A);
+	else hipLaunchKernelGGL(kConv3<false>, g, dim3(kUB), lds, s, A);
 	(void)L;
 }
 
