@@ -138,17 +138,20 @@ def run_gpu(args, mode_name, net=None):
     return g, sim, stats, elapsed, (v1 - v0) / v0
 
 
-def phase_roofline(stats, cells, mode_name):
+def phase_roofline(stats, cells, mode_name, unknown_fraction):
     """Algorithmic FP64 bytes per cell of each phase (one read and one write of
     every array it must touch, derived quantities on the fly) and the rate
     they imply at the measured phase times: advection a1-a7 433 B
     (216 + 48 + 169), WENO5 64 B per iteration (30 per reinitialisation),
-    glue a13 266 B, projection set-up and gradient 160 B plus 136 B (FP64
-    Jacobi PCG) or 68 B (FP32 inner) per iteration.  The flexible solvers'
-    iterations (V-cycles, network) have no per-cell byte count here."""
+    glue a13 266 B, projection set-up and gradient 160 B plus, per iteration,
+    136 B (FP64 Jacobi PCG) or 68 B (FP32 inner) per unknown, i.e. times the
+    fraction of cells that are unknowns.  The flexible solvers' iterations
+    (V-cycles, network) have no per-cell byte count here."""
     K = len(stats)
     it = sum(s["pcg"]["iterations"] + s["pcg"]["outer_iterations"] for s in stats) / K
     per_it = {"fp64": 136.0, "mixed": 68.0}.get(mode_name)
+    if per_it is not None:
+        per_it *= unknown_fraction
     reinit = 30 * 64.0 * sum(s["reinitialized"] + s["restarted"] * (1 - s["reinitialized"]) for s in stats) / K
     phases = {"advect": (433.0, "t_advect_ms"), "reinit_check": (reinit, "t_restart_ms"), "extrap_enforce": (266.0, "t_extrap_ms"),
               "project": (160.0 + per_it * it if per_it else None, "t_project_ms")}
@@ -161,11 +164,12 @@ def phase_roofline(stats, cells, mode_name):
     return out
 
 
-def summary(stats, elapsed, cells, drift, mode_name):
+def summary(stats, elapsed, cells, drift, mode_name, unknown_fraction):
     K = len(stats)
     it = [s["pcg"]["iterations"] for s in stats]
     return {"value": round(cells * K / elapsed, 1), "ms_per_step": round(1000 * elapsed / K, 3),
-            "phase_roofline": phase_roofline(stats, cells, mode_name),
+            "unknown_fraction": round(unknown_fraction, 4),
+            "phase_roofline": phase_roofline(stats, cells, mode_name, unknown_fraction),
             "phases_ms": {k: round(sum(s[k] for s in stats) / K, 3) for k in ("t_advect_ms", "t_restart_ms", "t_project_ms", "t_extrap_ms", "t_total_ms")},
             "pcg_iterations": {"mean": sum(it) / K, "min": min(it), "max": max(it),
                                "outer_mean": sum(s["pcg"]["outer_iterations"] for s in stats) / K,
@@ -304,7 +308,8 @@ def main():
         if learned and net is None:
             net, train_info = train_network(args)
         g, sim, stats, elapsed, drift = run_gpu(args, m, net if learned else None)
-        results[m] = summary(stats, elapsed, cells, drift, m)
+        unknown_fraction = float((sim.get(capi.UNKNOWNS) != 0).sum()) / cells   # at the last step
+        results[m] = summary(stats, elapsed, cells, drift, m, unknown_fraction)
         if learned and network is None:
             network = network_roofline(sim, net, g["res"])
         if m == args.mode:

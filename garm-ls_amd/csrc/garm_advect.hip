@@ -515,6 +515,107 @@ __global__ __launch_bounds__(kBlock) void kAdvectVelocity(FaceArrays<S> U, int a
 	out[gidx<D>(g, c.i, c.j, c.k)] = val;
 }
 
+// The float form with the cubic's input staged in LDS: the workgroup's 64 x 4
+// faces of one z-plane and three faces of halo on every side (70 x 10 x 7
+// floats in 3D, clamped into the grid as GridBasedData::at clamps), which
+// covers every cubic stencil of a backtrace shorter than one cell.  A face
+// whose stencil leaves the tile (a longer backtrace) reads the global copy;
+// the multilinear samples of the trace read the global copies (tiling them
+// too was measured slower: the larger fill and LDS footprint cost more than
+// the gathers).  Same operations in the same order as
+// kAdvectVelocity<D, float, float>, so the results are the same.
+constexpr int kVelHalo = 3;
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void kAdvectVelocityTile(FaceArrays<float> U, int axis, double *out, double hd)
+{
+	constexpr int TXT = 64 + 2 * kVelHalo, TYT = 4 + 2 * kVelHalo, TZT = D == 3 ? 1 + 2 * kVelHalo : 1;
+	constexpr int kTile = TZT * TYT * TXT, kFill = (kTile + kBlock - 1) / kBlock;
+	__shared__ float tile[kTile];
+	const GridDev &g = U.g[axis];
+	const float *ua = U.a[axis];
+	const int tx0 = int(blockIdx.x * blockDim.x) - kVelHalo, ty0 = int(blockIdx.y * blockDim.y) - kVelHalo;
+	const int tz0 = D == 3 ? int(blockIdx.z) - kVelHalo : 0;
+	const int tid = int(threadIdx.x + blockDim.x * threadIdx.y);
+	{
+		// all of a thread's fill loads in flight together
+		float v[kFill];
+#pragma unroll
+		for (int r = 0; r < kFill; r++) {
+			const int q = tid + r * kBlock;
+			const int tx = q % TXT, ty = (q / TXT) % TYT, tz = q / (TXT * TYT);
+			v[r] = q < kTile ? ua[gat<D>(g, tx0 + tx, ty0 + ty, tz0 + tz)] : 0.0f;
+		}
+#pragma unroll
+		for (int r = 0; r < kFill; r++) {
+			const int q = tid + r * kBlock;
+			if (q < kTile) tile[q] = v[r];
+		}
+	}
+	__syncthreads();
+	const Coord c = threadCoord();
+	if (!inGrid<D>(g, c)) return;
+	const float h = float(hd);
+	float x[3] = {float(g.o[0] + c.i * g.dx), float(g.o[1] + c.j * g.dx), float(g.o[2] + c.k * g.dx)};
+	float v0[3], p1[3], v1[3], X[3];
+	sampleVel<D, float, float>(U, x, v0);
+#pragma unroll
+	for (int a = 0; a < D; a++) p1[a] = x[a] + v0[a] * h;
+	sampleVel<D, float, float>(U, p1, v1);
+#pragma unroll
+	for (int a = 0; a < D; a++) X[a] = x[a] + (v0[a] + v1[a]) * h / 2;
+	int l[3] = {0, 0, 0}; float f[3];
+#pragma unroll
+	for (int a = 0; a < D; a++) {
+		const float o = float(g.o[a]), inv = float(g.inv);
+		l[a] = int(floor((X[a] - o) * inv)) - 1;   // getCubicLower (Grid.h:54)
+		f[a] = (X[a] - o - l[a] * float(g.dx)) * inv;
+	}
+	float w[4][3];
+#pragma unroll
+	for (int a = 0; a < D; a++) {
+		const float t = f[a];
+		w[0][a] = (t - 1) * (t - 2) * (t - 3) / -6;
+		w[1][a] = t * (t - 2) * (t - 3) / 2;
+		w[2][a] = t * (t - 1) * (t - 3) / -2;
+		w[3][a] = t * (t - 1) * (t - 2) / 6;
+	}
+	const int lx = l[0] - tx0, ly = l[1] - ty0, lz = D == 3 ? l[2] - tz0 : 0;
+	const bool inside = lx >= 0 && lx + 3 < TXT && ly >= 0 && ly + 3 < TYT && (D == 2 || (lz >= 0 && lz + 3 < TZT));
+	float val = 0;
+	if (inside) {
+#pragma unroll
+		for (int k = 0; k < (D == 3 ? 4 : 1); k++) {
+			float plane = 0;
+#pragma unroll
+			for (int j = 0; j < 4; j++) {
+				const float *row = tile + ((lz + k) * TYT + ly + j) * TXT + lx;
+				float r = 0;
+#pragma unroll
+				for (int i = 0; i < 4; i++) r += row[i] * w[i][0];
+				plane += r * w[j][1];
+			}
+			val += D == 3 ? plane * w[k][2] : plane;
+		}
+	} else {
+		unsigned ox[4], oy[4], oz[4];
+		cornerOffsets<D, 4>(g, l, ox, oy, oz);
+#pragma unroll
+		for (int k = 0; k < (D == 3 ? 4 : 1); k++) {
+			float plane = 0;
+#pragma unroll
+			for (int j = 0; j < 4; j++) {
+				float r = 0;
+#pragma unroll
+				for (int i = 0; i < 4; i++) r += ua[ox[i] + oy[j] + oz[k]] * w[i][0];
+				plane += r * w[j][1];
+			}
+			val += D == 3 ? plane * w[k][2] : plane;
+		}
+	}
+	out[gidx<D>(g, c.i, c.j, c.k)] = val;
+}
+
 // float copy of a face array (the FP32 velocity advection's input)
 template <typename S>
 __global__ void kFaceToFloat(const double *u, S *out, size_t n)
@@ -772,7 +873,7 @@ static void advectVelocityD(const Fields &F, double dt, hipStream_t s, bool copi
 			U.g[a] = F.face[a];
 		}
 		for (int a = 0; a < D; a++)
-			hipLaunchKernelGGL((kAdvectVelocity<D, float, float>), gridDims(F.face[a], D), gridBlock(), 0, s, U, a, F.u_new[a], -dt);
+			hipLaunchKernelGGL((kAdvectVelocityTile<D>), gridDims(F.face[a], D), gridBlock(), 0, s, U, a, F.u_new[a], -dt);
 	} else {
 		(void)copiesFresh;
 		FaceArrays<double> U{};

@@ -107,3 +107,17 @@ def test_fp32_mixed_trajectory_errors_and_volume(dim, n, steps):
     say(f"  volume drift FP64 {dr:.3e}, FP32+mixed {dl:.3e}; worst difference {worst_drift:.2e}")
     assert worst_drift <= 1e-3
     assert rel_linf(low.get(capi.PHI), ref.get(capi.PHI)) <= 1e-2
+
+
+@pytest.mark.parametrize("dim,n", CASES, ids=IDS)
+def test_fp32_velocity_advection_long_backtrace(dim, n):
+    """A step of five cells per step: most cubic stencils leave the LDS tile of
+    the FP32 velocity advection (three faces of halo) and read the global copy."""
+    say(f"fp32 velocity advection, 5-cell backtrace {dim}d n={n}")
+    g, o, d = _pair(dim, n)
+    dt = 5.0 * g["dx"] / max(o.max_speed(), 1e-12)
+    o.advect_velocity(dt)
+    d.advect_velocity(dt)
+    e = rel_linf(d.get(capi.VELOCITY), o.get(capi.VELOCITY))
+    say(f"  velocity rel Linf vs FP64 oracle {e:.2e}")
+    assert e <= FP32_OP_TOL
