@@ -27,6 +27,10 @@ typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int TX = 16, TY = 4, TZ = 4, HX = TX + 2, HY = TY + 2, HZ = TZ + 2;
+// forward convolution tile: CX x 4 x 4 voxels, one z-slice of CX / 16 x 4
+// 16-voxel fragments per wave (CX = 32 was measured slower: twice the
+// accumulators halve the occupancy)
+constexpr int CX = 16, CHX = CX + 2, kFrags = CX / 16 * TY;
 constexpr int kUB = 256;
 
 static int roundUp16(int c) { return (c + 15) / 16 * 16; }
@@ -61,19 +65,19 @@ __global__ __launch_bounds__(kUB) void kConv3(ConvArgs A)
 	// output takes 34 KB and four workgroups fit on a CU); the output tile
 	// (256 voxels x 32 channels) reuses the halo once the reduction is done
 	extern __shared__ __align__(16) bf16 lds[];
-	bf16 *tin = lds, *tw = lds + HZ * HY * HX * 16;
+	bf16 *tin = lds, *tw = lds + HZ * HY * CHX * 16;
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const int kg = lane >> 4, li = lane & 15;
-	const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+	const int x0 = blockIdx.x * CX, y0 = blockIdx.y * TY;
 	const int nzt = (A.nz + TZ - 1) / TZ;
 	const int zt = blockIdx.z % nzt, grp = blockIdx.z / nzt;
 	const int z0 = zt * TZ, co0 = grp * 32;
 	const int NF = min(32, A.coutP - co0) / 16;
 	const int WS = NF * 16;   // weight rows per tap in LDS
 	const int half = (kg & 1) * 8, second = kg >> 1;   // channel half and tap of the pair this lane holds
-	f4 acc[4][2];
+	f4 acc[kFrags][2];
 #pragma unroll
-	for (int f = 0; f < 4; f++)
+	for (int f = 0; f < kFrags; f++)
 #pragma unroll
 		for (int q = 0; q < 2; q++) acc[f][q] = f4{0, 0, 0, 0};
 
@@ -81,12 +85,12 @@ __global__ __launch_bounds__(kUB) void kConv3(ConvArgs A)
 	// voxel's channels come in whole groups of eight (WIDE); 8-byte otherwise
 	for (int ci0 = 0; ci0 < A.Cin; ci0 += 16) {
 		if constexpr (WIDE) {
-			constexpr int kVecs = HZ * HY * HX * 2, kIt = (kVecs + kUB - 1) / kUB;
+			constexpr int kVecs = HZ * HY * CHX * 2, kIt = (kVecs + kUB - 1) / kUB;
 			uint4 hv[kIt];
 #pragma unroll
 			for (int r = 0; r < kIt; r++) {
 				const int q = tid + r * kUB, v = q >> 1, ch = ci0 + (q & 1) * 8;
-				const int hx = v % HX, hy = (v / HX) % HY, hz = v / (HX * HY);
+				const int hx = v % CHX, hy = (v / CHX) % HY, hz = v / (CHX * HY);
 				const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
 				hv[r] = make_uint4(0, 0, 0, 0);
 				if (q < kVecs && gx >= 0 && gx < A.nx && gy >= 0 && gy < A.ny && gz >= 0 && gz < A.nz && ch < A.Cin)
@@ -98,12 +102,12 @@ __global__ __launch_bounds__(kUB) void kConv3(ConvArgs A)
 				if (q < kVecs) *reinterpret_cast<uint4 *>(tin + q * 8) = hv[r];
 			}
 		} else {
-			constexpr int kVecs = HZ * HY * HX * 4, kIt = (kVecs + kUB - 1) / kUB;
+			constexpr int kVecs = HZ * HY * CHX * 4, kIt = (kVecs + kUB - 1) / kUB;
 			uint2 hv[kIt];
 #pragma unroll
 			for (int r = 0; r < kIt; r++) {
 				const int q = tid + r * kUB, quad = q & 3, v = q >> 2;
-				const int hx = v % HX, hy = (v / HX) % HY, hz = v / (HX * HY);
+				const int hx = v % CHX, hy = (v / CHX) % HY, hz = v / (CHX * HY);
 				const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
 				const int ch = ci0 + quad * 4;
 				hv[r] = make_uint2(0, 0);
@@ -148,8 +152,9 @@ __global__ __launch_bounds__(kUB) void kConv3(ConvArgs A)
 				b[q] = __builtin_bit_cast(bf8v, none ? s8v{0, 0, 0, 0, 0, 0, 0, 0} : raw);
 			}
 #pragma unroll
-			for (int f = 0; f < 4; f++) {
-				const bf8v a = __builtin_bit_cast(bf8v, *reinterpret_cast<const s8v *>(tin + (((wave + dz) * HY + f + dy) * HX + li + dx) * 16 + half));
+			for (int f = 0; f < kFrags; f++) {
+				const int fy = f & 3, fx = f >> 2;   // row of the slice, 16-voxel half of the row
+				const bf8v a = __builtin_bit_cast(bf8v, *reinterpret_cast<const s8v *>(tin + (((wave + dz) * HY + fy + dy) * CHX + fx * 16 + li + dx) * 16 + half));
 				acc[f][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[0], acc[f][0], 0, 0, 0);
 				if (NF > 1) acc[f][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[1], acc[f][1], 0, 0, 0);
 			}
@@ -158,9 +163,9 @@ __global__ __launch_bounds__(kUB) void kConv3(ConvArgs A)
 	}
 	// bias, activation, BF16: the tile goes to LDS as [voxel][32 channels]
 	// and leaves as whole 16-byte words when the output layout allows
-	bf16 *tout = tin;   // 256 x 32 BF16 = 16 KB <= the halo buffer
+	bf16 *tout = tin;   // CX * 16 voxels x 32 BF16 <= the halo buffer
 #pragma unroll
-	for (int f = 0; f < 4; f++)
+	for (int f = 0; f < kFrags; f++)
 #pragma unroll
 		for (int q = 0; q < 2; q++) {
 			if (q >= NF) continue;
@@ -170,7 +175,7 @@ __global__ __launch_bounds__(kUB) void kConv3(ConvArgs A)
 			for (int i = 0; i < 4; i++) {
 				float v = acc[f][q][i] + bb;
 				if (A.relu) v = fmaxf(v, 0.0f);
-				const int vox = (wave * TY + f) * TX + kg * 4 + i;   // tile voxel (x fastest)
+				const int vox = (wave * TY + (f & 3)) * CX + (f >> 2) * 16 + kg * 4 + i;   // tile voxel (x fastest)
 				tout[vox * 32 + q * 16 + li] = f2bf(v);
 			}
 		}
@@ -178,17 +183,17 @@ __global__ __launch_bounds__(kUB) void kConv3(ConvArgs A)
 	const int nch = min(NF * 16, A.Cout - co0);
 	if (nch == NF * 16 && (A.outStride & 7) == 0 && ((A.outOff + co0) & 7) == 0) {
 		const int words = NF * 2;   // 16-byte words per voxel
-		for (int q = tid; q < TX * TY * TZ * words; q += kUB) {
+		for (int q = tid; q < CX * TY * TZ * words; q += kUB) {
 			const int vox = q / words, w8 = q % words;
-			const int x = x0 + vox % TX, y = y0 + (vox / TX) % TY, z = z0 + vox / (TX * TY);
+			const int x = x0 + vox % CX, y = y0 + (vox / CX) % TY, z = z0 + vox / (CX * TY);
 			if (x >= A.nx || y >= A.ny || z >= A.nz) continue;
 			*reinterpret_cast<uint4 *>(A.out + (size_t(z * A.ny + y) * A.nx + x) * A.outStride + A.outOff + co0 + w8 * 8) =
 				*reinterpret_cast<const uint4 *>(tout + vox * 32 + w8 * 8);
 		}
 	} else {
-		for (int q = tid; q < TX * TY * TZ * nch; q += kUB) {
+		for (int q = tid; q < CX * TY * TZ * nch; q += kUB) {
 			const int vox = q / nch, c = q % nch;
-			const int x = x0 + vox % TX, y = y0 + (vox / TX) % TY, z = z0 + vox / (TX * TY);
+			const int x = x0 + vox % CX, y = y0 + (vox / CX) % TY, z = z0 + vox / (CX * TY);
 			if (x >= A.nx || y >= A.ny || z >= A.nz) continue;
 			A.out[(size_t(z * A.ny + y) * A.nx + x) * A.outStride + A.outOff + co0 + c] = tout[vox * 32 + c];
 		}
@@ -716,8 +721,8 @@ static void conv(const UnetBuffers &B, const UnetLayer &L, int level, const bf16
 	A.nx = B.n[level][0]; A.ny = B.n[level][1]; A.nz = B.n[level][2];
 	A.relu = relu ? 1 : 0;
 	const int groups = (coutP + 31) / 32;
-	const dim3 g((A.nx + TX - 1) / TX, (A.ny + TY - 1) / TY, ((A.nz + TZ - 1) / TZ) * groups);
-	const size_t lds = size_t(HZ * HY * HX * 16 + 27 * std::min(32, coutP) * 16) * sizeof(bf16);
+	const dim3 g((A.nx + CX - 1) / CX, (A.ny + TY - 1) / TY, ((A.nz + TZ - 1) / TZ) * groups);
+	const size_t lds = size_t(HZ * HY * CHX * 16 + 27 * std::min(32, coutP) * 16) * sizeof(bf16);
 	if ((inStride & 7) == 0 && (cin & 7) == 0) hipLaunchKernelGGL(kConv3<true>, g, dim3(kUB), lds, s, A);
 	else hipLaunchKernelGGL(kConv3<false>, g, dim3(kUB), lds, s, A);
 	(void)L;

@@ -293,3 +293,38 @@ def test_project_multigrid_reaches_fp64_tolerance_in_few_iterations(dim, n):
     for f in (capi.PRESSURE, capi.VELOCITY):
         e = rel_linf(d.get(f), o.get(f))
         assert e <= 1e-7, f"{capi.FIELD_NAMES[f]}: {e:.2e}"
+
+
+@pytest.mark.parametrize("dim,n", [(2, 64), (3, 32)], ids=["2d-64", "3d-32"])
+def test_droplet_per_step_and_volume(dim, n):
+    """The droplet scene (pool plus a falling sphere with its own velocity,
+    the 384^3 configuration's scene at test size): 12 steps from the same
+    initial state, every step compared against the oracle's from identical
+    inputs, and the free-running volume drift of both."""
+    g = scenes.grid(n, dim)
+    o, d = make_oracle(g), make_gpu(g)
+    for s in (o, d):
+        scenes.droplet(s, g, seed=19260817)
+    assert o.initialize(tol=1e-12) == 0
+    d.initialize(capi.step_opts(tol=1e-12, max_iterations=50000))
+    free = make_gpu(g)
+    copy_state(d, free)
+    v0o, v0f = o.volume(), free.volume()
+    opts = capi.step_opts(reinit_interval=2, tol=1e-12, max_iterations=50000)
+    clk = FrameClock(g["dx"], 1.0)
+    worst = {}
+    for s in range(12):
+        copy_state(o, d)
+        dt = clk.next_dt(o.max_speed())
+        ro = o.step(dt, do_reinit=(s % 2 == 0), tol=1e-12, max_it=50000)
+        d.step(dt, s, opts)
+        free.step(dt, s, opts)
+        assert ro["rc"] == 0
+        for f in (capi.PHI, capi.PHI_GRAD, capi.VELOCITY, capi.PRESSURE):
+            worst[f] = max(worst.get(f, 0.0), rel_linf(d.get(f), o.get(f)))
+    dv = abs((free.volume() - v0f) / v0f - (o.volume() - v0o) / v0o)
+    say(f"droplet {dim}d n={n}: worst per-step rel Linf " + ", ".join(f"{capi.FIELD_NAMES[f]} {e:.1e}" for f, e in worst.items())
+        + f"; free-run volume drift difference {dv:.1e}")
+    for f, e in worst.items():
+        assert e <= TOL, f"{capi.FIELD_NAMES[f]}: {e:.3e}"
+    assert dv <= 1e-9
