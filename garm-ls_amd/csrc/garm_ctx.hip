@@ -187,10 +187,10 @@ int rebuildBoundaryFaces(garm_ctx *c)
 {
 	Fields &F = c->F;
 	int rc, n = 0;
-	if ((rc = hipCheck(c, hipMemsetAsync(F.front_cnt + 3, 0, sizeof(int), c->s), "boundary faces"))) return rc;
-	launchBoundaryFaces(F, F.bface, F.front_cnt + 3, c->s);
+	if ((rc = hipCheck(c, hipMemsetAsync(F.front_cnt + 9, 0, sizeof(int), c->s), "boundary faces"))) return rc;
+	launchBoundaryFaces(F, F.bface, F.front_cnt + 9, c->s);
 	if ((rc = launched(c, "boundary faces"))) return rc;
-	if ((rc = hipCheck(c, hipMemcpyAsync(&n, F.front_cnt + 3, sizeof(int), hipMemcpyDeviceToHost, c->s), "boundary faces"))) return rc;
+	if ((rc = hipCheck(c, hipMemcpyAsync(&n, F.front_cnt + 9, sizeof(int), hipMemcpyDeviceToHost, c->s), "boundary faces"))) return rc;
 	if ((rc = hipCheck(c, hipStreamSynchronize(c->s), "boundary faces"))) return rc;
 	if (size_t(n) > c->bface_rec_cap) {
 		if (F.bface_rec) (void)hipFree(F.bface_rec);
@@ -271,6 +271,7 @@ int garm_create(garm_ctx **out, int dim, const int res[3], double dx, const doub
 	const size_t N = F.N;
 	ok = ok && alloc(c, F.phi, N) && alloc(c, F.phi_tmp, N) && alloc(c, F.phi_ori, N) && alloc(c, F.phi_ref, N)
 		&& alloc(c, F.cell_dist, N) && alloc(c, F.node_dist, F.Nn) && alloc(c, F.adv_region, N) && alloc(c, F.layer, N + 8)
+		&& alloc(c, F.layer2, N + 8)
 		&& alloc(c, F.scratch[0], N) && alloc(c, F.scratch[1], N);
 	for (int a = 0; a < dim && ok; a++)
 		ok = alloc(c, F.phi_grad[a], N) && alloc(c, F.xi[a], N) && alloc(c, F.xi_new[a], N) && alloc(c, F.phi_ref_grad[a], N)
@@ -282,7 +283,9 @@ int garm_create(garm_ctx **out, int dim, const int res[3], double dx, const doub
 		size_t NFall = 0;
 		for (int a = 0; a < dim; a++) NFall += F.Nf[a];
 		const size_t cap = std::max(N, NFall);
-		ok = ok && alloc(c, F.front_list[0], cap) && alloc(c, F.front_list[1], cap) && alloc(c, F.front_cnt, 4)
+		// the air extrapolation's three fronts: cells (xi), cells (advRegion), faces
+		ok = ok && alloc(c, F.front_list[0], cap) && alloc(c, F.front_list[1], cap) && alloc(c, F.front_list[2], N)
+			&& alloc(c, F.front_list[3], N) && alloc(c, F.front_list[4], NFall) && alloc(c, F.front_list[5], NFall) && alloc(c, F.front_cnt, 16)
 			&& alloc(c, F.bface, NFall);
 	}
 	for (int e = 0; e < dim * dim && ok; e++) ok = alloc(c, F.J[e], N) && alloc(c, F.J_new[e], N);

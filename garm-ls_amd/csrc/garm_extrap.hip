@@ -167,56 +167,96 @@ __device__ __forceinline__ bool claimByte(uint8_t *base, size_t idx, uint8_t L)
 	}
 }
 
-// Layer L: claim the unreached neighbours of the layer L-1 front.  The
-// grid-stride loop runs the same trip count on every thread of a workgroup
-// (each workgroup's start and the stride are workgroup-uniform), as
+// Layer L of one front: one grid-stride element t of its layer L-1 list
+// claims each unreached neighbour, gives it the mean of its neighbours below
+// layer L (Extrapolation.h:101-124) and lists it for layer L+1.  Every
+// thread of the workgroup calls this for the same workgroup-uniform t0, as
 // blockAppendBase needs.
+template <int D>
+__device__ __forceinline__ void frontStepElements(const LayerSpace &S, int L, const int *listIn, int n, int t0, int *listOut, int *cntOut)
+{
+	const int t = t0 + int(threadIdx.x);
+	int a = 0, i = 0, j = 0, k = 0;
+	unsigned took = 0;
+	if (t < n) {
+		decode(S, listIn[t], a, i, j, k);
+		const GridDev &g = S.g[a];
+		uint8_t *lay = S.layer[a];
+		for (int o = 0; o < 2 * D; o++) {
+			const int ax = o >> 1, sd = (o & 1) ? 1 : -1;
+			const int qi = i + (ax == 0 ? sd : 0), qj = j + (ax == 1 ? sd : 0), qk = k + (ax == 2 ? sd : 0);
+			if (!validIn<D>(g, qi, qj, qk)) continue;
+			const size_t q = gidx<D>(g, qi, qj, qk);
+			if (lay[q] != kFar || !claimByte(lay, q, uint8_t(L))) continue;
+			took |= 1u << o;
+			int cnt = 0;
+			double s0 = 0, s1 = 0, s2 = 0;
+			for (int p = 0; p < 2 * D; p++) {
+				const int bx = p >> 1, bs = (p & 1) ? 1 : -1;
+				const int ri = qi + (bx == 0 ? bs : 0), rj = qj + (bx == 1 ? bs : 0), rk = qk + (bx == 2 ? bs : 0);
+				if (!validIn<D>(g, ri, rj, rk)) continue;
+				const size_t rr = gidx<D>(g, ri, rj, rk);
+				if (lay[rr] < L) {
+					if (S.nv > 0) s0 += S.v[a][0][rr];
+					if (S.nv > 1) s1 += S.v[a][1][rr];
+					if (S.nv > 2) s2 += S.v[a][2][rr];
+					cnt++;
+				}
+			}
+			if (S.nv > 0) S.v[a][0][q] = s0 / cnt;
+			if (S.nv > 1) S.v[a][1][q] = s1 / cnt;
+			if (S.nv > 2) S.v[a][2][q] = s2 / cnt;
+		}
+	}
+	int at = blockAppendBase(cntOut, __popc(took));
+	for (int o = 0; o < 2 * D; o++) {
+		if (!(took >> o & 1u)) continue;
+		const int ax = o >> 1, sd = (o & 1) ? 1 : -1;
+		listOut[at++] = S.off[a] + int(gidx<D>(S.g[a], i + (ax == 0 ? sd : 0), j + (ax == 1 ? sd : 0), k + (ax == 2 ? sd : 0)));
+	}
+}
+
+// Layer L: claim the unreached neighbours of the layer L-1 front.
 template <int D>
 __global__ __launch_bounds__(kBlock) void kFrontStep(LayerSpace S, int L, const int *listIn, const int *cntIn, int *listOut, int *cntOut, int *cntZero)
 {
 	if (blockIdx.x == 0 && threadIdx.x == 0) *cntZero = 0;
 	const int n = *cntIn;
-	for (int t0 = blockIdx.x * blockDim.x; t0 < n; t0 += gridDim.x * blockDim.x) {
-		const int t = t0 + int(threadIdx.x);
-		int a = 0, i = 0, j = 0, k = 0;
-		unsigned took = 0;
-		if (t < n) {
-			decode(S, listIn[t], a, i, j, k);
-			const GridDev &g = S.g[a];
-			uint8_t *lay = S.layer[a];
-			for (int o = 0; o < 2 * D; o++) {
-				const int ax = o >> 1, sd = (o & 1) ? 1 : -1;
-				const int qi = i + (ax == 0 ? sd : 0), qj = j + (ax == 1 ? sd : 0), qk = k + (ax == 2 ? sd : 0);
-				if (!validIn<D>(g, qi, qj, qk)) continue;
-				const size_t q = gidx<D>(g, qi, qj, qk);
-				if (lay[q] != kFar || !claimByte(lay, q, uint8_t(L))) continue;
-				took |= 1u << o;
-				// mean over q's neighbours reached before layer L (Extrapolation.h:101-124)
-				int cnt = 0;
-				double s0 = 0, s1 = 0, s2 = 0;
-				for (int p = 0; p < 2 * D; p++) {
-					const int bx = p >> 1, bs = (p & 1) ? 1 : -1;
-					const int ri = qi + (bx == 0 ? bs : 0), rj = qj + (bx == 1 ? bs : 0), rk = qk + (bx == 2 ? bs : 0);
-					if (!validIn<D>(g, ri, rj, rk)) continue;
-					const size_t rr = gidx<D>(g, ri, rj, rk);
-					if (lay[rr] < L) {
-						if (S.nv > 0) s0 += S.v[a][0][rr];
-						if (S.nv > 1) s1 += S.v[a][1][rr];
-						if (S.nv > 2) s2 += S.v[a][2][rr];
-						cnt++;
-					}
-				}
-				if (S.nv > 0) S.v[a][0][q] = s0 / cnt;
-				if (S.nv > 1) S.v[a][1][q] = s1 / cnt;
-				if (S.nv > 2) S.v[a][2][q] = s2 / cnt;
-			}
-		}
-		int at = blockAppendBase(cntOut, __popc(took));
-		for (int o = 0; o < 2 * D; o++) {
-			if (!(took >> o & 1u)) continue;
-			const int ax = o >> 1, sd = (o & 1) ? 1 : -1;
-			listOut[at++] = S.off[a] + int(gidx<D>(S.g[a], i + (ax == 0 ? sd : 0), j + (ax == 1 ? sd : 0), k + (ax == 2 ? sd : 0)));
-		}
+	for (int t0 = blockIdx.x * blockDim.x; t0 < n; t0 += gridDim.x * blockDim.x)
+		frontStepElements<D>(S, L, listIn, n, t0, listOut, cntOut);
+}
+
+// Several independent fronts (their own layer arrays, lists and counters)
+// advanced by one layer in one launch: each workgroup-sized chunk of the
+// concatenated lists belongs to one front.  Fronts past their last layer
+// take no work.
+struct FrontJob {
+	LayerSpace S;
+	const int *listIn, *cntIn;
+	int *listOut, *cntOut, *cntZero;
+	int active;
+};
+
+constexpr int kMaxJobs = 3;
+struct FrontJobs { FrontJob j[kMaxJobs]; int n; };
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void kFrontStepJobs(FrontJobs J, int L)
+{
+	int chunks[kMaxJobs], n[kMaxJobs], total = 0;
+#pragma unroll
+	for (int q = 0; q < kMaxJobs; q++) {
+		n[q] = q < J.n && J.j[q].active ? *J.j[q].cntIn : 0;
+		chunks[q] = (n[q] + kBlock - 1) / kBlock;
+		total += chunks[q];
+	}
+	if (blockIdx.x == 0 && threadIdx.x == 0)
+		for (int q = 0; q < J.n; q++) if (J.j[q].active) *J.j[q].cntZero = 0;
+	for (int c = blockIdx.x; c < total; c += gridDim.x) {
+		int q = 0, first = 0;
+		while (q + 1 < kMaxJobs && c >= first + chunks[q]) { first += chunks[q]; q++; }
+		const FrontJob &F = J.j[q];
+		frontStepElements<D>(F.S, L, F.listIn, n[q], (c - first) * kBlock, F.listOut, F.cntOut);
 	}
 }
 
@@ -325,27 +365,54 @@ void launchExtrapolateCollider(const Fields &F, hipStream_t s)
 // extrapolateFieldsToAir (LevelSetLiquid.cpp:1059-1084): xi from the old
 // advRegion (15 layers, unreached reset to the cell centre), advRegion
 // rebuilt from phi_ori <= 0 (15 layers), u on every face grid (17 layers,
-// unreached zeroed).
+// unreached zeroed).  The three fronts are independent (their own layer
+// arrays and lists), so each layer of all three is one launch.
 void launchExtrapolateAir(const Fields &F, hipStream_t s)
 {
-	runFront(F, cellSpace(F, F.D, F.xi[0], F.xi[1], F.D == 3 ? F.xi[2] : nullptr), fixArgs(F, kFixAdvRegion), 15, s);
-	if (F.D == 2) hipLaunchKernelGGL(kFrontFinishRefMap<2>, gridDims(F.cell, 2), gridBlock(), 0, s, F.cell, (const uint8_t *)F.layer, F.xi[0], F.xi[1], F.xi[2]);
-	else hipLaunchKernelGGL(kFrontFinishRefMap<3>, gridDims(F.cell, 3), gridBlock(), 0, s, F.cell, (const uint8_t *)F.layer, F.xi[0], F.xi[1], F.xi[2]);
-	runFront(F, cellSpace(F, 0, nullptr, nullptr, nullptr), fixArgs(F, kFixPhiOriNonPositive), 15, s);
-	hipLaunchKernelGGL(kFrontToMask, dim3(linBlocks(F.N)), dim3(kBlock), 0, s, (const uint8_t *)F.layer, F.adv_region, F.N);
-	LayerSpace S{};
-	S.ng = F.D;
-	S.nv = 1;
+	LayerSpace Sx = cellSpace(F, F.D, F.xi[0], F.xi[1], F.D == 3 ? F.xi[2] : nullptr);
+	LayerSpace Sa = cellSpace(F, 0, nullptr, nullptr, nullptr);
+	Sa.layer[0] = F.layer2;
+	LayerSpace Sf{};
+	Sf.ng = F.D;
+	Sf.nv = 1;
 	int off = 0;
 	for (int a = 0; a < F.D; a++) {
-		S.g[a] = F.face[a];
-		S.layer[a] = F.layer_f[a];
-		S.v[a][0] = F.u[a];
-		S.off[a] = off;
+		Sf.g[a] = F.face[a];
+		Sf.layer[a] = F.layer_f[a];
+		Sf.v[a][0] = F.u[a];
+		Sf.off[a] = off;
 		off += int(F.Nf[a]);
 	}
-	S.off[F.D] = off;
-	runFront(F, S, fixArgs(F, kFixLiquidFace), 17, s);
+	Sf.off[F.D] = off;
+	const LayerSpace *spaces[3] = {&Sx, &Sa, &Sf};
+	const int rules[3] = {kFixAdvRegion, kFixPhiOriNonPositive, kFixLiquidFace};
+	const int steps[3] = {15, 15, 17};
+	int *cnt = F.front_cnt;   // three counters per front
+	hipLaunchKernelGGL(kZeroInts, dim3(1), dim3(64), 0, s, cnt, 9);
+	for (int q = 0; q < 3; q++)
+		for (int a = 0; a < spaces[q]->ng; a++) {
+			if (F.D == 2) launchFrontInit<2>(*spaces[q], a, fixArgs(F, rules[q]), F.front_list[2 * q], cnt + 3 * q, s);
+			else launchFrontInit<3>(*spaces[q], a, fixArgs(F, rules[q]), F.front_list[2 * q], cnt + 3 * q, s);
+		}
+	for (int L = 1; L <= 17; L++) {
+		FrontJobs J{};
+		J.n = 3;
+		for (int q = 0; q < 3; q++) {
+			FrontJob &j = J.j[q];
+			j.S = *spaces[q];
+			j.listIn = F.front_list[2 * q + ((L - 1) & 1)];
+			j.listOut = F.front_list[2 * q + (L & 1)];
+			j.cntIn = cnt + 3 * q + (L - 1) % 3;
+			j.cntOut = cnt + 3 * q + L % 3;
+			j.cntZero = cnt + 3 * q + (L + 1) % 3;
+			j.active = L <= steps[q] ? 1 : 0;
+		}
+		if (F.D == 2) hipLaunchKernelGGL(kFrontStepJobs<2>, dim3(kFrontGrid), dim3(kBlock), 0, s, J, L);
+		else hipLaunchKernelGGL(kFrontStepJobs<3>, dim3(kFrontGrid), dim3(kBlock), 0, s, J, L);
+	}
+	if (F.D == 2) hipLaunchKernelGGL(kFrontFinishRefMap<2>, gridDims(F.cell, 2), gridBlock(), 0, s, F.cell, (const uint8_t *)F.layer, F.xi[0], F.xi[1], F.xi[2]);
+	else hipLaunchKernelGGL(kFrontFinishRefMap<3>, gridDims(F.cell, 3), gridBlock(), 0, s, F.cell, (const uint8_t *)F.layer, F.xi[0], F.xi[1], F.xi[2]);
+	hipLaunchKernelGGL(kFrontToMask, dim3(linBlocks(F.N)), dim3(kBlock), 0, s, (const uint8_t *)F.layer2, F.adv_region, F.N);
 	for (int a = 0; a < F.D; a++)
 		hipLaunchKernelGGL(kFrontFinishFaces, dim3(linBlocks(F.Nf[a])), dim3(kBlock), 0, s, (const uint8_t *)F.layer_f[a], F.u[a], F.Nf[a]);
 }

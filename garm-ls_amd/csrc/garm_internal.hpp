@@ -38,10 +38,10 @@ struct Fields {
 	double *cell_dist = nullptr, *node_dist = nullptr;// _boundary.cellDist / nodeDist
 	double *frac[3] = {}, *bvel[3] = {}, *bnormal[3] = {};   // _boundary.fraction / velocity / normal
 	double *scratch[2] = {};                          // two cell arrays (reinit stages)
-	uint8_t *layer = nullptr;                         // extrapolation layer index per cell
+	uint8_t *layer = nullptr, *layer2 = nullptr;      // extrapolation layer index per cell (two fronts)
 	uint8_t *layer_f[3] = {};                         // ... per face
-	int *front_list[2] = {};                          // extrapolation fronts (element ids)
-	int *front_cnt = nullptr;                         // three front counters
+	int *front_list[6] = {};                          // extrapolation fronts (element ids): two lists per front
+	int *front_cnt = nullptr;                         // three counters per front (9), then one more
 	int *bface = nullptr;                             // faces Boundary::enforce acts on (global face ids)
 	double *bface_rec = nullptr;                      // per listed face: unit normal, solid velocity, result
 	int nbface = 0;                                   // listed faces
