@@ -285,7 +285,8 @@ int garm_create(garm_ctx **out, int dim, const int res[3], double dx, const doub
 		const size_t cap = std::max(N, NFall);
 		// the air extrapolation's three fronts: cells (xi), cells (advRegion), faces
 		ok = ok && alloc(c, F.front_list[0], cap) && alloc(c, F.front_list[1], cap) && alloc(c, F.front_list[2], N)
-			&& alloc(c, F.front_list[3], N) && alloc(c, F.front_list[4], NFall) && alloc(c, F.front_list[5], NFall) && alloc(c, F.front_cnt, 16)
+			&& alloc(c, F.front_list[3], N) && alloc(c, F.front_list[4], NFall) && alloc(c, F.front_list[5], NFall) && alloc(c, F.front_cnt, 24)
+			&& alloc(c, F.coll_layer, N + 8) && alloc(c, F.coll_list, N)
 			&& alloc(c, F.bface, NFall);
 	}
 	for (int e = 0; e < dim * dim && ok; e++) ok = alloc(c, F.J[e], N) && alloc(c, F.J_new[e], N);
@@ -338,7 +339,7 @@ int garm_create(garm_ctx **out, int dim, const int res[3], double dx, const doub
 	launchRefLevelSetRecords(F, c->s);
 	launchFillU8(F.adv_region, 0, N, c->s);
 	ok = ok && hipGetLastError() == hipSuccess && hipStreamSynchronize(c->s) == hipSuccess;
-	ok = ok && rebuildBoundaryFaces(c) == GARM_OK;
+	ok = ok && rebuildBoundaryFaces(c) == GARM_OK && buildColliderLayers(F, c->s);
 	if (!ok) {
 		g_create_error = "initial upload failed";
 		garm_destroy(c);
@@ -399,6 +400,7 @@ int garm_upload(garm_ctx *c, int f, const void *host, size_t n)
 		if ((rc = launched(c, "reference level-set records"))) return rc;
 		return hipCheck(c, hipStreamSynchronize(c->s), "upload");
 	}
+	if (f == GARM_CELL_DIST && !buildColliderLayers(F, c->s)) return fail(c, GARM_EHIP, "collider extrapolation layers");
 	return GARM_OK;
 }
 

@@ -356,10 +356,64 @@ static FixArgs fixArgs(const Fields &F, int rule)
 
 // extrapolateFieldsToCollider (LevelSetLiquid.cpp:1086-1137): phi_ori and phi,
 // 6 layers (_boundaryExtrapolationRadius, LevelSetLiquid.h:92), fixed where
-// cellDist > 0; both share the fixed set, so one front carries both.
+// cellDist > 0; both share the fixed set, so one front carries both.  The
+// fixed set depends on cellDist alone, which steps do not change: the layers
+// are found once by the claiming fronts (buildColliderLayers, kept as one
+// list per layer) and each step only averages, layer by layer.
+template <int D>
+__global__ __launch_bounds__(kBlock) void kColliderLayer(GridDev g, const uint8_t *layer, const int *list, int n, int L, double *v0, double *v1)
+{
+	const int t = int(blockIdx.x * blockDim.x + threadIdx.x);
+	if (t >= n) return;
+	const int id = list[t];
+	const int qi = id % g.n[0], qj = (id / g.n[0]) % g.n[1], qk = D == 3 ? id / (g.n[0] * g.n[1]) : 0;
+	int cnt = 0;
+	double s0 = 0, s1 = 0;
+	for (int p = 0; p < 2 * D; p++) {
+		const int bx = p >> 1, bs = (p & 1) ? 1 : -1;
+		const int ri = qi + (bx == 0 ? bs : 0), rj = qj + (bx == 1 ? bs : 0), rk = qk + (bx == 2 ? bs : 0);
+		if (!validIn<D>(g, ri, rj, rk)) continue;
+		const size_t rr = gidx<D>(g, ri, rj, rk);
+		if (layer[rr] < L) {
+			s0 += v0[rr];
+			s1 += v1[rr];
+			cnt++;
+		}
+	}
+	v0[id] = s0 / cnt;
+	v1[id] = s1 / cnt;
+}
+
+bool buildColliderLayers(Fields &F, hipStream_t s)
+{
+	LayerSpace S = cellSpace(F, 0, nullptr, nullptr, nullptr);
+	S.layer[0] = F.coll_layer;
+	int *cnt = F.front_cnt + 10;   // [0] the fixed boundary, [L] layer L, [7] scratch
+	hipLaunchKernelGGL(kZeroInts, dim3(1), dim3(64), 0, s, cnt, 8);
+	if (F.D == 2) launchFrontInit<2>(S, 0, fixArgs(F, kFixCellDistPositive), F.front_list[0], cnt, s);
+	else launchFrontInit<3>(S, 0, fixArgs(F, kFixCellDistPositive), F.front_list[0], cnt, s);
+	F.coll_off[1] = 0;
+	for (int L = 1; L <= kColliderLayers; L++) {
+		const int *in = L == 1 ? F.front_list[0] : F.coll_list + F.coll_off[L - 1];
+		if (F.D == 2) hipLaunchKernelGGL(kFrontStep<2>, dim3(kFrontGrid), dim3(kBlock), 0, s, S, L, in, (const int *)cnt + L - 1, F.coll_list + F.coll_off[L], cnt + L, cnt + 7);
+		else hipLaunchKernelGGL(kFrontStep<3>, dim3(kFrontGrid), dim3(kBlock), 0, s, S, L, in, (const int *)cnt + L - 1, F.coll_list + F.coll_off[L], cnt + L, cnt + 7);
+		int n = 0;
+		if (hipMemcpyAsync(&n, cnt + L, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) return false;
+		F.coll_cnt[L] = n;
+		F.coll_off[L + 1] = F.coll_off[L] + n;
+	}
+	return hipGetLastError() == hipSuccess;
+}
+
 void launchExtrapolateCollider(const Fields &F, hipStream_t s)
 {
-	runFront(F, cellSpace(F, 2, F.phi_ori, F.phi, nullptr), fixArgs(F, kFixCellDistPositive), 6, s);
+	for (int L = 1; L <= kColliderLayers; L++) {
+		const int n = F.coll_cnt[L];
+		if (n == 0) break;
+		const dim3 gd((n + kBlock - 1) / kBlock);
+		if (F.D == 2) hipLaunchKernelGGL(kColliderLayer<2>, gd, dim3(kBlock), 0, s, F.cell, (const uint8_t *)F.coll_layer, (const int *)(F.coll_list + F.coll_off[L]), n, L, F.phi_ori, F.phi);
+		else hipLaunchKernelGGL(kColliderLayer<3>, gd, dim3(kBlock), 0, s, F.cell, (const uint8_t *)F.coll_layer, (const int *)(F.coll_list + F.coll_off[L]), n, L, F.phi_ori, F.phi);
+	}
 }
 
 // extrapolateFieldsToAir (LevelSetLiquid.cpp:1059-1084): xi from the old

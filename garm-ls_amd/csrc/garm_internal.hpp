@@ -16,6 +16,8 @@ struct FaceSet {
 	GridDev g[3];
 };
 
+constexpr int kColliderLayers = 6;   // _boundaryExtrapolationRadius (LevelSetLiquid.h:92)
+
 // Every device-resident field of one LevelSetLiquid<Dim>
 // (LevelSetLiquid.h:27-121), structure-of-arrays.
 struct Fields {
@@ -41,7 +43,10 @@ struct Fields {
 	uint8_t *layer = nullptr, *layer2 = nullptr;      // extrapolation layer index per cell (two fronts)
 	uint8_t *layer_f[3] = {};                         // ... per face
 	int *front_list[6] = {};                          // extrapolation fronts (element ids): two lists per front
-	int *front_cnt = nullptr;                         // three counters per front (9), then one more
+	int *front_cnt = nullptr;                         // three counters per front (9), one for the boundary faces, eight for the collider layers
+	uint8_t *coll_layer = nullptr;                    // collider extrapolation: layer per cell (fixed by cellDist)
+	int *coll_list = nullptr;                         // ... cells of layers 1..6, one run per layer
+	int coll_off[kColliderLayers + 2] = {}, coll_cnt[kColliderLayers + 1] = {};   // host: run offsets and lengths
 	int *bface = nullptr;                             // faces Boundary::enforce acts on (global face ids)
 	double *bface_rec = nullptr;                      // per listed face: unit normal, solid velocity, result
 	int nbface = 0;                                   // listed faces
@@ -76,6 +81,8 @@ void launchGravity(const Fields &F, double dt, hipStream_t s);
 // Enright deformation times coeff on the face arrays (3D)
 void launchEnrightVelocity(const Fields &F, double coeff, hipStream_t s);
 void launchExtrapolateCollider(const Fields &F, hipStream_t s);
+// finds the collider extrapolation's layers (after cellDist is set); synchronises
+bool buildColliderLayers(Fields &F, hipStream_t s);
 void launchExtrapolateAir(const Fields &F, hipStream_t s);
 // Boundary::enforce on the listed faces, in place on F.u
 void launchEnforceBoundary(const Fields &F, hipStream_t s);
