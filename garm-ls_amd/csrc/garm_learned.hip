@@ -397,7 +397,9 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 		B.gn[a] = cell.n[a];
 	}
 	const size_t boxVox = size_t(B.n[0]) * B.n[1] * B.n[2];
-	if (!U.buf.reserve(std::max(boxVox + boxVox / 4, U.buf.cap_vox), U.buf.has_grad, err) || !U.buf.setDims(B.n[0], B.n[1], B.n[2], err))
+	// inference buffers only (a box that outgrows them reallocates without the
+	// training gradients, with half again as much room)
+	if (!U.buf.reserve(std::max(boxVox + boxVox / 2, U.buf.cap_vox), false, err) || !U.buf.setDims(B.n[0], B.n[1], B.n[2], err))
 		return GARM_ENOMEM;
 	// z = M(r): two-level cycle, network on the pre-smoothed residual
 	auto precondition = [&]() {
