@@ -279,22 +279,36 @@ __global__ __launch_bounds__(kUB) void kConv3WGrad(WGradArgs A)
 }
 
 // Deterministic sum of the partials into the FP32 gradients [27][cin][cout].
-__global__ void kReduceWGrad(const float *partial, const float *partialB, int nblocks, int cin, int cout, int cinP, int coutP, float *gw, float *gb)
+// 64 outputs per workgroup of 1024 threads: 16 groups of 64 lanes split the
+// workgroups' partials (group g takes partials g, g + 16, ...), then the 16
+// group sums are added in group order -- a fixed order, so the gradients
+// repeat bit for bit, with 16x the parallelism of one thread per output.
+constexpr int kRedGroups = 16;
+
+__global__ __launch_bounds__(64 * kRedGroups) void kReduceWGrad(const float *partial, const float *partialB, int nblocks, int cin, int cout, int cinP, int coutP, float *gw, float *gb)
 {
+	__shared__ float part[kRedGroups][64];
 	const int n = 27 * cin * cout;
-	for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n + cout; e += gridDim.x * blockDim.x) {
-		if (e < n) {
-			const int co = e % cout, ci = (e / cout) % cin, t = e / (cout * cin);
-			const size_t off = (size_t(t) * cinP + ci) * coutP + co;
-			float s = 0;
-			for (int b = 0; b < nblocks; b++) s += partial[size_t(b) * 27 * cinP * coutP + off];
-			gw[e] = s;
-		} else {
-			const int co = e - n;
-			float s = 0;
-			for (int b = 0; b < nblocks; b++) s += partialB[size_t(b) * coutP + co];
-			gb[co] = s;
-		}
+	const int lane = int(threadIdx.x) & 63, grp = int(threadIdx.x) >> 6;
+	const int e = int(blockIdx.x) * 64 + lane;
+	float s = 0;
+	if (e < n) {
+		const int co = e % cout, ci = (e / cout) % cin, t = e / (cout * cin);
+		const size_t off = (size_t(t) * cinP + ci) * coutP + co, stride = size_t(27) * cinP * coutP;
+#pragma unroll 4
+		for (int b = grp; b < nblocks; b += kRedGroups) s += partial[size_t(b) * stride + off];
+	} else if (e < n + cout) {
+		const int co = e - n;
+#pragma unroll 4
+		for (int b = grp; b < nblocks; b += kRedGroups) s += partialB[size_t(b) * coutP + co];
+	}
+	part[grp][lane] = s;
+	__syncthreads();
+	if (grp == 0 && e < n + cout) {
+		float t = 0;
+		for (int g = 0; g < kRedGroups; g++) t += part[g][lane];
+		if (e < n) gw[e] = t;
+		else gb[e - n] = t;
 	}
 }
 
@@ -846,7 +860,7 @@ bool unetBackward(Unet &U, hipStream_t s, std::string &err)
 		W.partialB = B.partial + size_t(nblocks) * 27 * W.cinP * W.coutP;
 		const int groups = (W.coutP + 31) / 32;
 		hipLaunchKernelGGL(kConv3WGrad, dim3(tx, ty, tz * groups), dim3(kUB), 0, s, W);
-		hipLaunchKernelGGL(kReduceWGrad, dim3(blocksFor(size_t(27) * L.cin * L.cout + L.cout)), dim3(kUB), 0, s, (const float *)W.partial,
+		hipLaunchKernelGGL(kReduceWGrad, dim3(unsigned((size_t(27) * L.cin * L.cout + L.cout + 63) / 64)), dim3(64 * kRedGroups), 0, s, (const float *)W.partial,
 			(const float *)W.partialB, nblocks, L.cin, L.cout, W.cinP, W.coutP, L.gw, L.gb);
 		// input gradient: the transposed convolution (mirrored taps, swapped channels)
 		if (gIn) {
