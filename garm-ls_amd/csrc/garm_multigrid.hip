@@ -26,6 +26,7 @@
 namespace garm {
 
 constexpr int kMgSweeps = 2;            // red-black sweep pairs before and after the coarse correction
+constexpr int kMgSweeps0 = 2;           // ... on the finest level
 constexpr float kMgProlongScale = 1.9f; // over-correction of the coarse correction (aggregation underestimates it; 1.9 measured best at 256^3)
 constexpr int kMgCoarsestCells = 512;
 constexpr int kMgCoarsestSweeps = 30;
@@ -423,7 +424,7 @@ static void vcycle(Multigrid &M, const int *mat2grid, int n, const double *r, do
 	for (int l = 0; l < first; l++) {
 		const MgLevel L = level(M, l);
 		const unsigned g = mgBlocks((M.cells[l] + 1) / 2);
-		for (int v = 0; v < kMgSweeps; v++) {
+		for (int v = 0; v < (l == 0 ? kMgSweeps0 : kMgSweeps); v++) {
 			hipLaunchKernelGGL(kMgSmooth<D>, dim3(g), dim3(kBlock), 0, s, L, 0);
 			hipLaunchKernelGGL(kMgSmooth<D>, dim3(g), dim3(kBlock), 0, s, L, 1);
 		}
@@ -448,7 +449,7 @@ static void vcycle(Multigrid &M, const int *mat2grid, int n, const double *r, do
 		const MgLevel L = level(M, l);
 		hipLaunchKernelGGL(kMgProlong<D>, dim3(mgBlocks(M.cells[l])), dim3(kBlock), 0, s, L, level(M, l + 1));
 		const unsigned g = mgBlocks((M.cells[l] + 1) / 2);
-		for (int v = 0; v < kMgSweeps; v++) {
+		for (int v = 0; v < (l == 0 ? kMgSweeps0 : kMgSweeps); v++) {
 			hipLaunchKernelGGL(kMgSmooth<D>, dim3(g), dim3(kBlock), 0, s, L, 1);
 			hipLaunchKernelGGL(kMgSmooth<D>, dim3(g), dim3(kBlock), 0, s, L, 0);
 		}
