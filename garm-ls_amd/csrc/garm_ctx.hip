@@ -711,6 +711,7 @@ int garm_time_kernel(garm_ctx *c, int which, int reps, double *avg_ms, double *b
 		if (rc) return fail(c, rc, "solver kernels need a preceding projection (FP32 ones a mixed-precision one)");
 	} else {
 		const double dt = 0.5 * F.cell.dx;   // backtrace length of half a cell
+		if (which == GARM_K_REINIT_STAGE) launchReinitPrepare(F, c->s, c->fp32);
 		(void)hipEventRecord(c->ev[0], c->s);
 		for (int q = 0; q < reps; q++) {
 			if (which == GARM_K_ADVECT_REFMAP) launchAdvectRefMap(F, dt, c->s, c->fp32);           // writes xi_new, J_new only
@@ -725,7 +726,7 @@ int garm_time_kernel(garm_ctx *c, int which, int reps, double *avg_ms, double *b
 		if (which == GARM_K_ADVECT_REFMAP) *bytes = NF * 8 + N * (D + D * D) * 8 * 2;   // trace scratch not counted
 		else if (which == GARM_K_ADVECT_VELOCITY) *bytes = 2 * NF * 8;
 		else if (which == GARM_K_COMPOSE) *bytes = N * (8 * (D + D * D + 1 + D + M + 1) + 1 + 8 * (3 + D));
-		else *bytes = N * (8 + (c->fp32 ? 4 : 8));
+		else *bytes = N * (c->fp32 ? 4 + 4 : 8 + 8);   // stage input (phi or its float copy) in, stage value out
 	}
 	*avg_ms = double(ms) / reps;
 	return GARM_OK;

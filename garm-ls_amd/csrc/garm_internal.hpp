@@ -64,8 +64,10 @@ void launchCompose(const Fields &F, hipStream_t s, bool fp32 = false);
 void launchRefLevelSetRecords(const Fields &F, hipStream_t s);
 void launchGradient4(const Fields &F, const double *src, double *const *dst, hipStream_t s);
 void launchReinit(const Fields &F, int iterations, double cfl, hipStream_t s, bool fp32 = false);
-// one Runge-Kutta stage L(phi) -> scratch[0], for timing
+// one Runge-Kutta stage L(phi) -> scratch[0], for timing (the FP32 stage
+// reads the float copy of phi that launchReinitPrepare makes)
 void launchReinitStage(const Fields &F, double dtau, hipStream_t s, bool fp32 = false);
+void launchReinitPrepare(const Fields &F, hipStream_t s, bool fp32);
 
 void launchUnionBox(const Fields &F, const double *center, const double *half, hipStream_t s);
 void launchUnionSphere(const Fields &F, const double *center, double r, hipStream_t s);

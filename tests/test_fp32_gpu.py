@@ -65,7 +65,7 @@ def test_fp32_advection_and_composition(dim, n):
     assert max(worst.values()) <= FP32_OP_TOL, worst
 
 
-@pytest.mark.parametrize("dim,n", CASES, ids=IDS)
+@pytest.mark.parametrize("dim,n", CASES + [(2, 160)], ids=IDS + ["2d-160"])
 def test_fp32_reinit(dim, n):
     say(f"fp32 reinit {dim}d n={n}")
     g, o, d = _pair(dim, n)
