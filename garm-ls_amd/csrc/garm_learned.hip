@@ -419,9 +419,9 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 			dims[a] = live ? hi[a] - lo[a] + 1 : 1;
 			gn[a] = cell.n[a];
 		}
-		if (!M.layout(D, org, dims, gn, err)) return GARM_ENOMEM;
+		if (!M.layout(D, org, dims, gn, n, err)) return GARM_ENOMEM;
 		mgBuild(M, A32, n, mat2grid, s);
-		second = [&]() { mgApply(M, mat2grid, n, W.r, W.z, s); };
+		second = [&]() { mgApply(M, n, W.r, W.z, s); };
 	}
 	return flexibleCg<D>(A64, A32, b, n, o, K, W, s, st, err, precondition, true, std::min(o.max_iterations, 400), second, "learned solve");
 }
@@ -440,9 +440,9 @@ static int multigrid(const Rows<double> &A64, const Rows<float> &A32, const doub
 		dims[a] = live ? hi[a] - lo[a] + 1 : 1;
 		gn[a] = cell.n[a];
 	}
-	if (!M.layout(D, org, dims, gn, err)) return GARM_ENOMEM;
+	if (!M.layout(D, org, dims, gn, n, err)) return GARM_ENOMEM;
 	mgBuild(M, A32, n, mat2grid, s);
-	auto precondition = [&]() { mgApply(M, mat2grid, n, W.r, W.z, s); };
+	auto precondition = [&]() { mgApply(M, n, W.r, W.z, s); };
 	return flexibleCg<D>(A64, A32, b, n, o, K, W, s, st, err, precondition, false, o.max_iterations, nullptr, "multigrid solve");
 }
 

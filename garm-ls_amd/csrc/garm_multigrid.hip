@@ -14,7 +14,7 @@
 // weights) carry over to every level without special cases.  Smoothing is
 // red-black Gauss-Seidel, two sweeps down and two up in mirrored colour order
 // so the cycle is symmetric; the coarsest level (<= 512 cells) is relaxed
-// 30 times.  The coarse levels that fit in 128 KiB of LDS together (from a
+// 12 times.  The coarse levels that fit in 158 KiB of LDS together (from a
 // few thousand cells down) run as one workgroup-wide kernel on LDS copies:
 // each of their sweeps would otherwise be a launch of a few microseconds
 // over a grid too small to fill the chip.  Everything is FP32 on structured
@@ -29,7 +29,7 @@ constexpr int kMgSweeps = 2;            // red-black sweep pairs before and afte
 constexpr int kMgSweeps0 = 2;           // ... on the finest level
 constexpr float kMgProlongScale = 1.9f; // over-correction of the coarse correction (aggregation underestimates it; 1.9 measured best at 256^3)
 constexpr int kMgCoarsestCells = 512;
-constexpr int kMgCoarsestSweeps = 30;
+constexpr int kMgCoarsestSweeps = 12;
 
 struct MgLevel {
 	int n[3];
@@ -44,7 +44,14 @@ static MgLevel level(const Multigrid &M, int l)
 	return L;
 }
 
-__device__ __forceinline__ size_t cidx(const MgLevel &L, int i, int j, int k) { return (size_t(k) * L.n[1] + j) * L.n[0] + i; }
+// Level boxes lie inside the cell grid, which holds fewer than 2^31 cells
+// (checked at creation): 32-bit cell indices.
+__device__ __forceinline__ unsigned cidx(const MgLevel &L, int i, int j, int k) { return (unsigned(k) * unsigned(L.n[1]) + unsigned(j)) * unsigned(L.n[0]) + unsigned(i); }
+
+// The level kernels run on 3D grids (x over a row, y over rows, z over
+// planes), so no thread divides its way from a linear index to (i, j, k).
+struct Tid { int x, y, z; };
+__device__ __forceinline__ Tid tid3() { return Tid{int(blockIdx.x * blockDim.x + threadIdx.x), int(blockIdx.y * blockDim.y + threadIdx.y), int(blockIdx.z)}; }
 
 __global__ void kMgZero(float *p, size_t n)
 {
@@ -55,7 +62,7 @@ struct MgBox {
 	int org[3], gn[3];
 };
 
-__device__ __forceinline__ size_t boxCell(const MgBox &B, const MgLevel &L, int g)
+__device__ __forceinline__ unsigned boxCell(const MgBox &B, const MgLevel &L, int g)
 {
 	const int i = g % B.gn[0] - B.org[0];
 	const int j = (g / B.gn[0]) % B.gn[1] - B.org[1];
@@ -65,10 +72,11 @@ __device__ __forceinline__ size_t boxCell(const MgBox &B, const MgLevel &L, int 
 
 // level 0: the operator entries of every unknown at its box cell
 template <int D>
-__global__ __launch_bounds__(kBlock) void kMgBuild0(Rows<float> A, int n, const int *mat2grid, MgBox B, MgLevel L)
+__global__ __launch_bounds__(kBlock) void kMgBuild0(Rows<float> A, int n, const int *mat2grid, MgBox B, MgLevel L, unsigned *boxOf)
 {
 	for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
-		const size_t c = boxCell(B, L, mat2grid[r]);
+		const unsigned c = boxCell(B, L, mat2grid[r]);
+		boxOf[r] = c;
 		L.diag[c] = A.diag[r];
 		L.wx[c] = A.nb[0][r] >= 0 ? A.w[0][r] : 0.0f;
 		L.wy[c] = A.nb[2][r] >= 0 ? A.w[2][r] : 0.0f;
@@ -80,9 +88,11 @@ __global__ __launch_bounds__(kBlock) void kMgBuild0(Rows<float> A, int n, const 
 template <int D>
 __global__ __launch_bounds__(kBlock) void kMgCoarsen(MgLevel F, MgLevel C)
 {
-	const size_t cells = size_t(C.n[0]) * C.n[1] * C.n[2];
-	for (size_t q = blockIdx.x * size_t(blockDim.x) + threadIdx.x; q < cells; q += size_t(gridDim.x) * blockDim.x) {
-		const int I = int(q % C.n[0]), J = int((q / C.n[0]) % C.n[1]), K = int(q / (size_t(C.n[0]) * C.n[1]));
+	const Tid t = tid3();
+	const int I = t.x, J = t.y, K = t.z;
+	if (I >= C.n[0] || J >= C.n[1]) return;
+	const unsigned q = cidx(C, I, J, K);
+	{
 		float d = 0, inner = 0, sx = 0, sy = 0, sz = 0;
 #pragma unroll
 		for (int cz = 0; cz < (D == 3 ? 2 : 1); cz++)
@@ -92,7 +102,7 @@ __global__ __launch_bounds__(kBlock) void kMgCoarsen(MgLevel F, MgLevel C)
 				for (int cx = 0; cx < 2; cx++) {
 					const int i = 2 * I + cx, j = 2 * J + cy, k = 2 * K + cz;
 					if (i >= F.n[0] || j >= F.n[1] || k >= F.n[2]) continue;
-					const size_t f = cidx(F, i, j, k);
+					const unsigned f = cidx(F, i, j, k);
 					d += F.diag[f];
 					if (cx) inner += F.wx[f]; else sx += F.wx[f];
 					if (cy) inner += F.wy[f]; else sy += F.wy[f];
@@ -107,9 +117,9 @@ __global__ __launch_bounds__(kBlock) void kMgCoarsen(MgLevel F, MgLevel C)
 
 // (A x)_c without the diagonal: the sum of face weight times neighbour
 template <int D>
-__device__ __forceinline__ float offSum(const MgLevel &L, size_t c, int i, int j, int k)
+__device__ __forceinline__ float offSum(const MgLevel &L, unsigned c, int i, int j, int k)
 {
-	const size_t sy = L.n[0], sz = size_t(L.n[0]) * L.n[1];
+	const unsigned sy = unsigned(L.n[0]), sz = unsigned(L.n[0]) * unsigned(L.n[1]);
 	float s = 0;
 	if (i > 0) s += L.wx[c] * L.x[c - 1];
 	if (i + 1 < L.n[0]) s += L.wx[c + 1] * L.x[c + 1];
@@ -122,44 +132,46 @@ __device__ __forceinline__ float offSum(const MgLevel &L, size_t c, int i, int j
 	return s;
 }
 
-// one colour of red-black Gauss-Seidel; threads cover the cells of that colour only
+// one colour of red-black Gauss-Seidel; x threads cover the cells of that colour in a row
 template <int D>
 __global__ __launch_bounds__(kBlock) void kMgSmooth(MgLevel L, int color)
 {
-	const int hx = (L.n[0] + 1) / 2;
-	const size_t half = size_t(hx) * L.n[1] * L.n[2];
-	for (size_t q = blockIdx.x * size_t(blockDim.x) + threadIdx.x; q < half; q += size_t(gridDim.x) * blockDim.x) {
-		const int ii = int(q % hx), j = int((q / hx) % L.n[1]), k = int(q / (size_t(hx) * L.n[1]));
-		const int i = 2 * ii + ((j + k + color) & 1);
-		if (i >= L.n[0]) continue;
-		const size_t c = cidx(L, i, j, k);
-		const float d = L.diag[c];
-		if (!(d > 0)) continue;
-		L.x[c] = (L.b[c] + offSum<D>(L, c, i, j, k)) / d;
-	}
+	const Tid t = tid3();
+	const int j = t.y, k = t.z, i = 2 * t.x + ((j + k + color) & 1);
+	if (i >= L.n[0] || j >= L.n[1]) return;
+	const unsigned c = cidx(L, i, j, k);
+	const float d = L.diag[c];
+	if (!(d > 0)) return;
+	L.x[c] = (L.b[c] + offSum<D>(L, c, i, j, k)) / d;
 }
 
-// coarse right-hand side = sum of the fine residuals of the aggregate; coarse iterate = 0
+// coarse right-hand side = sum of the fine residuals of the aggregate; coarse
+// iterate = 0.  One lane per fine cell: the lanes of an aggregate are an
+// aligned group of 8 (4 in 2D) along x that sums its residuals across
+// lanes, so every lane issues its loads at once instead of one child after
+// another.  Block rows are a multiple of 8 lanes wide.
 template <int D>
 __global__ __launch_bounds__(kBlock) void kMgRestrict(MgLevel F, MgLevel C)
 {
-	const size_t cells = size_t(C.n[0]) * C.n[1] * C.n[2];
-	for (size_t q = blockIdx.x * size_t(blockDim.x) + threadIdx.x; q < cells; q += size_t(gridDim.x) * blockDim.x) {
-		const int I = int(q % C.n[0]), J = int((q / C.n[0]) % C.n[1]), K = int(q / (size_t(C.n[0]) * C.n[1]));
-		float s = 0;
-#pragma unroll
-		for (int cz = 0; cz < (D == 3 ? 2 : 1); cz++)
-#pragma unroll
-			for (int cy = 0; cy < 2; cy++)
-#pragma unroll
-				for (int cx = 0; cx < 2; cx++) {
-					const int i = 2 * I + cx, j = 2 * J + cy, k = 2 * K + cz;
-					if (i >= F.n[0] || j >= F.n[1] || k >= F.n[2]) continue;
-					const size_t f = cidx(F, i, j, k);
-					const float d = F.diag[f];
-					if (d > 0) s += F.b[f] - (d * F.x[f] - offSum<D>(F, f, i, j, k));
-				}
-		C.b[q] = s;
+	constexpr int kids = D == 3 ? 8 : 4;
+	const Tid t = tid3();
+	const int I = t.x / kids, ch = t.x % kids, J = t.y, K = t.z;
+	const bool live = I < C.n[0] && J < C.n[1];
+	float r = 0;
+	if (live) {
+		const int i = 2 * I + (ch & 1), j = 2 * J + ((ch >> 1) & 1), k = 2 * K + (ch >> 2);
+		if (i < F.n[0] && j < F.n[1] && k < F.n[2]) {
+			const unsigned f = cidx(F, i, j, k);
+			const float d = F.diag[f];
+			if (d > 0) r = F.b[f] - (d * F.x[f] - offSum<D>(F, f, i, j, k));
+		}
+	}
+	r += __shfl_xor(r, 1);
+	r += __shfl_xor(r, 2);
+	if constexpr (D == 3) r += __shfl_xor(r, 4);
+	if (live && ch == 0) {
+		const unsigned q = cidx(C, I, J, K);
+		C.b[q] = r;
 		C.x[q] = 0.0f;
 	}
 }
@@ -168,12 +180,12 @@ __global__ __launch_bounds__(kBlock) void kMgRestrict(MgLevel F, MgLevel C)
 template <int D>
 __global__ __launch_bounds__(kBlock) void kMgProlong(MgLevel F, MgLevel C)
 {
-	const size_t cells = size_t(F.n[0]) * F.n[1] * F.n[2];
-	for (size_t q = blockIdx.x * size_t(blockDim.x) + threadIdx.x; q < cells; q += size_t(gridDim.x) * blockDim.x) {
-		if (!(F.diag[q] > 0)) continue;
-		const int i = int(q % F.n[0]), j = int((q / F.n[0]) % F.n[1]), k = int(q / (size_t(F.n[0]) * F.n[1]));
-		F.x[q] += kMgProlongScale * C.x[cidx(C, i / 2, j / 2, D == 3 ? k / 2 : 0)];
-	}
+	const Tid t = tid3();
+	const int i = t.x, j = t.y, k = t.z;
+	if (i >= F.n[0] || j >= F.n[1]) return;
+	const unsigned q = cidx(F, i, j, k);
+	if (!(F.diag[q] > 0)) return;
+	F.x[q] += kMgProlongScale * C.x[cidx(C, i / 2, j / 2, D == 3 ? k / 2 : 0)];
 }
 
 // ---- the coarse levels in LDS -------------------------------------------
@@ -193,6 +205,9 @@ struct MgTail {
 	MgLevel g[kMgMaxLevels];        // their global arrays
 	int off[kMgMaxLevels];          // LDS offset (floats) of each level's padded arrays
 	int list[kMgMaxLevels];         // LDS offset (words) of each level's cell list
+	int list_words;                 // words of all the lists (contiguous from list[first])
+	unsigned *glist;                // the lists in global memory, built once per operator
+	int *gcount;                    // active cells per level and colour
 };
 
 // padded plane of level L
@@ -258,30 +273,11 @@ __global__ __launch_bounds__(1024) void kMgTail(MgTail T)
 		}
 	}
 	__syncthreads();
-	// count the active cells of each colour, then list them (red first)
-	for (int l = T.first; l < T.levels; l++) {
-		const MgLevel G = T.g[l];
-		for (int q = tid; q < mgCells(G); q += blockDim.x) {
-			if (!(G.diag[q] > 0)) continue;
-			const int i = q % G.n[0], j = (q / G.n[0]) % G.n[1], k = q / (G.n[0] * G.n[1]);
-			atomicAdd(&ncol[l][(i + j + k) & 1], 1);
-		}
-	}
-	__syncthreads();
-	__shared__ int fill[kMgMaxLevels][2];
-	if (tid < 2 * kMgMaxLevels) fill[tid / 2][tid % 2] = tid % 2 ? ncol[tid / 2][0] : 0;
-	__syncthreads();
-	for (int l = T.first; l < T.levels; l++) {
-		const MgLevel G = T.g[l];
-		unsigned *list = reinterpret_cast<unsigned *>(lds + T.list[l]);
-		const bool coarse = l + 1 < T.levels;
-		const MgLevel C = coarse ? T.g[l + 1] : G;
-		for (int q = tid; q < mgCells(G); q += blockDim.x) {
-			if (!(G.diag[q] > 0)) continue;
-			const int i = q % G.n[0], j = (q / G.n[0]) % G.n[1], k = q / (G.n[0] * G.n[1]);
-			const unsigned cq = coarse ? unsigned(cidx(C, i / 2, j / 2, D == 3 ? k / 2 : 0)) : 0u;
-			list[atomicAdd(&fill[l][(i + j + k) & 1], 1)] = unsigned(q) | cq << 16;
-		}
+	// the active cells of each colour (red first), listed when the operator was built
+	if (tid < 2 * kMgMaxLevels) ncol[tid / 2][tid % 2] = T.gcount[tid];
+	{
+		unsigned *list = reinterpret_cast<unsigned *>(lds + T.list[T.first]);
+		for (int q = tid; q < T.list_words; q += blockDim.x) list[q] = T.glist[q];
 	}
 	__syncthreads();
 	const int top = T.levels - 1;
@@ -334,26 +330,83 @@ __global__ __launch_bounds__(1024) void kMgTail(MgTail T)
 	for (int q = tid; q < mgCells(G); q += blockDim.x) G.x[q] = V.x[q];
 }
 
-// compact FP64 residual -> level-0 right-hand side (the level's b and x were zeroed)
-__global__ void kMgLoad(int n, const int *mat2grid, MgBox B, MgLevel L, const double *r)
+// Lists the active cells (diag > 0) of every tail level by colour, red
+// first, as 16-bit cell index | coarse cell index << 16; the order within a
+// colour is free (cells of one colour do not neighbour each other).
+template <int D>
+__global__ __launch_bounds__(1024) void kMgTailLists(MgTail T)
 {
-	for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) L.b[boxCell(B, L, mat2grid[q])] = float(r[q]);
+	__shared__ int ncol[kMgMaxLevels][2];
+	__shared__ int fill[kMgMaxLevels][2];
+	const int tid = int(threadIdx.x);
+	if (tid < 2 * kMgMaxLevels) ncol[tid / 2][tid % 2] = 0;
+	__syncthreads();
+	for (int l = T.first; l < T.levels; l++) {
+		const MgLevel G = T.g[l];
+		for (int q = tid; q < mgCells(G); q += blockDim.x) {
+			if (!(G.diag[q] > 0)) continue;
+			const int i = q % G.n[0], j = (q / G.n[0]) % G.n[1], k = q / (G.n[0] * G.n[1]);
+			atomicAdd(&ncol[l][(i + j + k) & 1], 1);
+		}
+	}
+	__syncthreads();
+	if (tid < 2 * kMgMaxLevels) {
+		fill[tid / 2][tid % 2] = tid % 2 ? ncol[tid / 2][0] : 0;
+		T.gcount[tid] = ncol[tid / 2][tid % 2];
+	}
+	__syncthreads();
+	for (int l = T.first; l < T.levels; l++) {
+		const MgLevel G = T.g[l];
+		unsigned *list = T.glist + (T.list[l] - T.list[T.first]);
+		const bool coarse = l + 1 < T.levels;
+		const MgLevel C = coarse ? T.g[l + 1] : G;
+		for (int q = tid; q < mgCells(G); q += blockDim.x) {
+			if (!(G.diag[q] > 0)) continue;
+			const int i = q % G.n[0], j = (q / G.n[0]) % G.n[1], k = q / (G.n[0] * G.n[1]);
+			const unsigned cq = coarse ? unsigned(cidx(C, i / 2, j / 2, D == 3 ? k / 2 : 0)) : 0u;
+			list[atomicAdd(&fill[l][(i + j + k) & 1], 1)] = unsigned(q) | cq << 16;
+		}
+	}
 }
 
-__global__ void kMgStore(int n, const int *mat2grid, MgBox B, MgLevel L, double *z)
+// compact FP64 residual -> level-0 right-hand side, zero iterate.  Only the
+// unknowns' cells: the others keep b = x = 0 from the build (no kernel
+// writes an inactive cell of level 0).
+__global__ void kMgLoad(int n, const unsigned *boxOf, MgLevel L, const double *r)
 {
-	for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) z[q] = double(L.x[boxCell(B, L, mat2grid[q])]);
+	for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+		const unsigned c = boxOf[q];
+		L.b[c] = float(r[q]);
+		L.x[c] = 0.0f;
+	}
+}
+
+__global__ void kMgStore(int n, const unsigned *boxOf, MgLevel L, double *z)
+{
+	for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) z[q] = double(L.x[boxOf[q]]);
 }
 
 // ------------------------------------------------------------------ host
 static unsigned mgBlocks(size_t n) { return unsigned(std::max<size_t>(1, std::min<size_t>((n + kBlock - 1) / kBlock, 4096))); }
+
+// 3D launch over nx x ny x nz threads: rows of bx threads (8 .. 64, the
+// smallest power of two that covers nx), kBlock / bx rows per block
+struct Launch3 { dim3 grid, block; };
+static Launch3 mgLaunch3(size_t nx, int ny, int nz)
+{
+	unsigned bx = 8;
+	while (bx < 64 && bx < nx) bx *= 2;
+	const unsigned by = kBlock / bx;
+	return Launch3{dim3(unsigned((nx + bx - 1) / bx), (unsigned(ny) + by - 1) / by, unsigned(nz)), dim3(bx, by, 1)};
+}
+static Launch3 mgLevelLaunch(const Multigrid &M, int l, size_t nx) { return mgLaunch3(nx, M.dims[l][1], M.dims[l][2]); }
 
 Multigrid::~Multigrid()
 {
 	if (pool) (void)hipFree(pool);
 }
 
-bool Multigrid::layout(int dim, const int box_org[3], const int box_n[3], const int grid_n[3], std::string &err)
+bool Multigrid::layout(int dim, const int box_org[3], const int box_n[3], const int grid_n[3], int unknowns, std::string &err)
 {
 	D = dim;
 	for (int a = 0; a < 3; a++) { org[a] = box_org[a]; gn[a] = grid_n[a]; dims[0][a] = box_n[a]; }
@@ -378,7 +431,9 @@ bool Multigrid::layout(int dim, const int box_org[3], const int box_n[3], const 
 		tail_first--;
 		words += tailWords(tail_first);
 	}
-	const size_t need = 6 * total + 64;
+	tail_words = 0;
+	for (int l = tail_first; l < levels; l++) tail_words += cells[l];
+	const size_t need = 6 * total + tail_words + 2 * kMgMaxLevels + size_t(unknowns) + 64;
 	if (need > pool_cap) {
 		if (pool) (void)hipFree(pool);
 		pool = nullptr;
@@ -391,7 +446,33 @@ bool Multigrid::layout(int dim, const int box_org[3], const int box_n[3], const 
 		float **arr[6] = {&diag[l], &wx[l], &wy[l], &wz[l], &b[l], &x[l]};
 		for (float **a : arr) { *a = p; p += cells[l]; }
 	}
+	tail_list = reinterpret_cast<unsigned *>(p);
+	tail_count = reinterpret_cast<int *>(p + tail_words);
+	box_of = reinterpret_cast<unsigned *>(p + tail_words + 2 * kMgMaxLevels);
 	return true;
+}
+
+// the coarse levels run in LDS: LDS offsets of their padded arrays, then of their cell lists
+static MgTail tailDesc(const Multigrid &M)
+{
+	MgTail T{};
+	T.first = M.tail_first;
+	T.levels = M.levels;
+	T.sweeps = kMgCoarsestSweeps;
+	int off = 0;
+	for (int l = T.first; l < M.levels; l++) {
+		T.g[l] = level(M, l);
+		T.off[l] = off;
+		off += int(6 * (M.cells[l] + 2 * size_t(M.dims[l][0]) * M.dims[l][1]));
+	}
+	for (int l = T.first; l < M.levels; l++) {
+		T.list[l] = off;
+		off += int(M.cells[l]);
+	}
+	T.list_words = int(M.tail_words);
+	T.glist = M.tail_list;
+	T.gcount = M.tail_count;
+	return T;
 }
 
 void mgBuild(Multigrid &M, const Rows<float> &A, int n, const int *mat2grid, hipStream_t s)
@@ -403,64 +484,54 @@ void mgBuild(Multigrid &M, const Rows<float> &A, int n, const int *mat2grid, hip
 	MgBox B;
 	for (int a = 0; a < 3; a++) { B.org[a] = M.org[a]; B.gn[a] = M.gn[a]; }
 	const MgLevel L0 = level(M, 0);
-	if (M.D == 2) hipLaunchKernelGGL(kMgBuild0<2>, dim3(mgBlocks(size_t(n))), dim3(kBlock), 0, s, A, n, mat2grid, B, L0);
-	else hipLaunchKernelGGL(kMgBuild0<3>, dim3(mgBlocks(size_t(n))), dim3(kBlock), 0, s, A, n, mat2grid, B, L0);
+	if (M.D == 2) hipLaunchKernelGGL(kMgBuild0<2>, dim3(mgBlocks(size_t(n))), dim3(kBlock), 0, s, A, n, mat2grid, B, L0, M.box_of);
+	else hipLaunchKernelGGL(kMgBuild0<3>, dim3(mgBlocks(size_t(n))), dim3(kBlock), 0, s, A, n, mat2grid, B, L0, M.box_of);
 	for (int l = 0; l + 1 < M.levels; l++) {
 		const MgLevel F = level(M, l), C = level(M, l + 1);
-		if (M.D == 2) hipLaunchKernelGGL(kMgCoarsen<2>, dim3(mgBlocks(M.cells[l + 1])), dim3(kBlock), 0, s, F, C);
-		else hipLaunchKernelGGL(kMgCoarsen<3>, dim3(mgBlocks(M.cells[l + 1])), dim3(kBlock), 0, s, F, C);
+		const Launch3 g = mgLevelLaunch(M, l + 1, size_t(M.dims[l + 1][0]));
+		if (M.D == 2) hipLaunchKernelGGL(kMgCoarsen<2>, g.grid, g.block, 0, s, F, C);
+		else hipLaunchKernelGGL(kMgCoarsen<3>, g.grid, g.block, 0, s, F, C);
 	}
+	const MgTail T = tailDesc(M);
+	if (M.D == 2) hipLaunchKernelGGL(kMgTailLists<2>, dim3(1), dim3(1024), 0, s, T);
+	else hipLaunchKernelGGL(kMgTailLists<3>, dim3(1), dim3(1024), 0, s, T);
 }
 
 template <int D>
-static void vcycle(Multigrid &M, const int *mat2grid, int n, const double *r, double *z, hipStream_t s)
+static void vcycle(Multigrid &M, int n, const double *r, double *z, hipStream_t s)
 {
-	MgBox B;
-	for (int a = 0; a < 3; a++) { B.org[a] = M.org[a]; B.gn[a] = M.gn[a]; }
 	const MgLevel L0 = level(M, 0);
-	hipLaunchKernelGGL(kMgZero, dim3(mgBlocks(2 * M.cells[0])), dim3(kBlock), 0, s, L0.b, 2 * M.cells[0]);   // b and x are adjacent
-	hipLaunchKernelGGL(kMgLoad, dim3(mgBlocks(size_t(n))), dim3(kBlock), 0, s, n, mat2grid, B, L0, r);
+	hipLaunchKernelGGL(kMgLoad, dim3(mgBlocks(size_t(n))), dim3(kBlock), 0, s, n, (const unsigned *)M.box_of, L0, r);
 	const int first = M.tail_first;
 	for (int l = 0; l < first; l++) {
 		const MgLevel L = level(M, l);
-		const unsigned g = mgBlocks((M.cells[l] + 1) / 2);
+		const Launch3 g = mgLevelLaunch(M, l, size_t(M.dims[l][0] + 1) / 2);
 		for (int v = 0; v < (l == 0 ? kMgSweeps0 : kMgSweeps); v++) {
-			hipLaunchKernelGGL(kMgSmooth<D>, dim3(g), dim3(kBlock), 0, s, L, 0);
-			hipLaunchKernelGGL(kMgSmooth<D>, dim3(g), dim3(kBlock), 0, s, L, 1);
+			hipLaunchKernelGGL(kMgSmooth<D>, g.grid, g.block, 0, s, L, 0);
+			hipLaunchKernelGGL(kMgSmooth<D>, g.grid, g.block, 0, s, L, 1);
 		}
-		hipLaunchKernelGGL(kMgRestrict<D>, dim3(mgBlocks(M.cells[l + 1])), dim3(kBlock), 0, s, L, level(M, l + 1));
+		const Launch3 gr = mgLevelLaunch(M, l + 1, size_t(M.dims[l + 1][0]) * (D == 3 ? 8 : 4));
+		hipLaunchKernelGGL(kMgRestrict<D>, gr.grid, gr.block, 0, s, L, level(M, l + 1));
 	}
-	MgTail T{};
-	T.first = first;
-	T.levels = M.levels;
-	T.sweeps = kMgCoarsestSweeps;
-	int off = 0;
-	for (int l = first; l < M.levels; l++) {
-		T.g[l] = level(M, l);
-		T.off[l] = off;
-		off += int(6 * (M.cells[l] + 2 * size_t(M.dims[l][0]) * M.dims[l][1]));
-	}
-	for (int l = first; l < M.levels; l++) {
-		T.list[l] = off;
-		off += int(M.cells[l]);
-	}
+	const MgTail T = tailDesc(M);
 	hipLaunchKernelGGL(kMgTail<D>, dim3(1), dim3(1024), 0, s, T);
 	for (int l = first - 1; l >= 0; l--) {
 		const MgLevel L = level(M, l);
-		hipLaunchKernelGGL(kMgProlong<D>, dim3(mgBlocks(M.cells[l])), dim3(kBlock), 0, s, L, level(M, l + 1));
-		const unsigned g = mgBlocks((M.cells[l] + 1) / 2);
+		const Launch3 gp = mgLevelLaunch(M, l, size_t(M.dims[l][0]));
+		hipLaunchKernelGGL(kMgProlong<D>, gp.grid, gp.block, 0, s, L, level(M, l + 1));
+		const Launch3 g = mgLevelLaunch(M, l, size_t(M.dims[l][0] + 1) / 2);
 		for (int v = 0; v < (l == 0 ? kMgSweeps0 : kMgSweeps); v++) {
-			hipLaunchKernelGGL(kMgSmooth<D>, dim3(g), dim3(kBlock), 0, s, L, 1);
-			hipLaunchKernelGGL(kMgSmooth<D>, dim3(g), dim3(kBlock), 0, s, L, 0);
+			hipLaunchKernelGGL(kMgSmooth<D>, g.grid, g.block, 0, s, L, 1);
+			hipLaunchKernelGGL(kMgSmooth<D>, g.grid, g.block, 0, s, L, 0);
 		}
 	}
-	hipLaunchKernelGGL(kMgStore, dim3(mgBlocks(size_t(n))), dim3(kBlock), 0, s, n, mat2grid, B, L0, z);
+	hipLaunchKernelGGL(kMgStore, dim3(mgBlocks(size_t(n))), dim3(kBlock), 0, s, n, (const unsigned *)M.box_of, L0, z);
 }
 
-void mgApply(Multigrid &M, const int *mat2grid, int n, const double *r, double *z, hipStream_t s)
+void mgApply(Multigrid &M, int n, const double *r, double *z, hipStream_t s)
 {
-	if (M.D == 2) vcycle<2>(M, mat2grid, n, r, z, s);
-	else vcycle<3>(M, mat2grid, n, r, z, s);
+	if (M.D == 2) vcycle<2>(M, n, r, z, s);
+	else vcycle<3>(M, n, r, z, s);
 }
 
 } // namespace garm

@@ -26,17 +26,22 @@ struct Multigrid {
 	int gn[3] = {};           // cell-grid size
 	int D = 3;
 	int tail_first = 0;       // first of the coarse levels run in one workgroup's LDS
-	float *pool = nullptr;    // one allocation for every level array
+	size_t tail_words = 0;    // cell-list words of those levels
+	unsigned *tail_list = nullptr;   // their active cells per colour, listed once per operator
+	int *tail_count = nullptr;       // active cells per level and colour
+	unsigned *box_of = nullptr;      // level-0 cell of every unknown
+	float *pool = nullptr;    // one allocation for every level array and the lists
 	size_t pool_cap = 0;      // floats
 	~Multigrid();
-	// lays out the levels for a box of the given size (multiple solves reuse the pool)
-	bool layout(int D, const int box_org[3], const int box_n[3], const int grid_n[3], std::string &err);
+	// lays out the levels for a box of the given size holding `unknowns`
+	// unknowns (multiple solves reuse the pool)
+	bool layout(int D, const int box_org[3], const int box_n[3], const int grid_n[3], int unknowns, std::string &err);
 };
 
 // Level 0 from the FP32 rows of the n unknowns, then every coarser level.
 void mgBuild(Multigrid &M, const Rows<float> &A, int n, const int *mat2grid, hipStream_t s);
 // z = V(r): one symmetric V-cycle (red-black Gauss-Seidel, two sweeps down
 // and up) applied to the compact residual r; z is compact, FP64.
-void mgApply(Multigrid &M, const int *mat2grid, int n, const double *r, double *z, hipStream_t s);
+void mgApply(Multigrid &M, int n, const double *r, double *z, hipStream_t s);
 
 } // namespace garm
