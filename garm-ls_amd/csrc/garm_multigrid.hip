@@ -36,6 +36,9 @@ struct MgLevel {
 	float *diag, *wx, *wy, *wz, *b, *x;
 };
 
+// sweep tiles: large levels, small levels, 2D
+constexpr int kTileL[3] = {32, 8, 8}, kTileS[3] = {16, 8, 4}, kTile2[3] = {32, 32, 1};
+
 static MgLevel level(const Multigrid &M, int l)
 {
 	MgLevel L;
@@ -132,17 +135,140 @@ __device__ __forceinline__ float offSum(const MgLevel &L, unsigned c, int i, int
 	return s;
 }
 
-// one colour of red-black Gauss-Seidel; x threads cover the cells of that colour in a row
+// The operator row of a cell: diagonal, right-hand side and its six face
+// weights (-x, +x, -y, +y, -z, +z).  The +x, +y, +z weights of the level's
+// last cells clamp to the last cell (they meet a 0 in the tile below).
+struct MgRow { float d, b, w[6]; };
+
 template <int D>
-__global__ __launch_bounds__(kBlock) void kMgSmooth(MgLevel L, int color)
+__device__ __forceinline__ MgRow loadRow(const MgLevel &L, unsigned c, unsigned last)
 {
-	const Tid t = tid3();
-	const int j = t.y, k = t.z, i = 2 * t.x + ((j + k + color) & 1);
-	if (i >= L.n[0] || j >= L.n[1]) return;
-	const unsigned c = cidx(L, i, j, k);
-	const float d = L.diag[c];
-	if (!(d > 0)) return;
-	L.x[c] = (L.b[c] + offSum<D>(L, c, i, j, k)) / d;
+	const unsigned sy = unsigned(L.n[0]), sz = unsigned(L.n[0]) * unsigned(L.n[1]);
+	MgRow r;
+	r.d = L.diag[c];
+	r.b = L.b[c];
+	r.w[0] = L.wx[c];
+	r.w[1] = L.wx[min(c + 1, last)];
+	r.w[2] = L.wy[c];
+	r.w[3] = L.wy[min(c + sy, last)];
+	r.w[4] = D == 3 ? L.wz[c] : 0.0f;
+	r.w[5] = D == 3 ? L.wz[min(c + sz, last)] : 0.0f;
+	return r;
+}
+
+// Relaxation of a cell (tile index l) against the iterate in an LDS tile
+// (row and plane strides SX, SXY).  The tile holds 0 outside the level box,
+// and every face on the box boundary has weight 0 (the box bounds the
+// unknowns; coarsening sums those zeros), so the six terms are added
+// unconditionally: a missing neighbour adds +0 exactly, and s, which starts
+// at +0 and adds products of non-negative weights, is never -0.  The result
+// is offSum's, term by term.  An inactive cell (diag <= 0) keeps `keep`.
+template <int D, int SX, int SXY>
+__device__ __forceinline__ float relaxTile(const MgRow &r, const float *xs, int l, float keep)
+{
+	float s = 0;
+	s += r.w[0] * xs[l - 1];
+	s += r.w[1] * xs[l + 1];
+	s += r.w[2] * xs[l - SX];
+	s += r.w[3] * xs[l + SX];
+	if constexpr (D == 3) {
+		s += r.w[4] * xs[l - SXY];
+		s += r.w[5] * xs[l + SXY];
+	}
+	return r.d > 0 ? (r.b + s) / r.d : keep;
+}
+
+// One red-black Gauss-Seidel sweep, colour `first` then the other, in one
+// launch.  A block owns a TX x TY x TZ tile of the level: it stages the old
+// iterate of the tile and two cells around it in LDS, relaxes the first
+// colour on the tile and the ring of cells around it (the values the tile's
+// second colour reads; the neighbouring blocks compute the same ring cells
+// from the same inputs), then relaxes the second colour on the tile and
+// writes the tile to the other iterate buffer.  Reading xin and writing xout
+// keeps the blocks independent, and every cell gets exactly the operations
+// of one single-colour pass after the other.  Each phase gives every thread
+// a fixed count of cells, predicated (indices outside the level clamp to a
+// valid cell, results are dropped), and issues all of their loads before it
+// uses any, so they are in flight together.
+template <int D, int TX, int TY, int TZ, int NTH>
+__global__ __launch_bounds__(NTH) void kMgSweep(MgLevel L, const float *xin, float *xout, int first)
+{
+	static_assert(TX % 2 == 0 && (D == 3 || TZ == 1), "tile shape");
+	constexpr int HZ = D == 3 ? 2 : 0;                    // halo of the staged tile along z
+	constexpr int SX = TX + 4, SY = TY + 4, SZ = TZ + 2 * HZ, SXY = SX * SY, NS = SXY * SZ;
+	constexpr int RY = TY + 2, RZ = D == 3 ? TZ + 2 : 1, HX = (TX + 2) / 2, NR = HX * RY * RZ, NT = TX * TY * TZ;
+	constexpr int P0 = (NS + NTH - 1) / NTH, P1 = (NR + NTH - 1) / NTH, P2 = (NT + NTH - 1) / NTH;
+	constexpr int kNone = SXY + SX + 1;                    // a tile index whose neighbours are all in the tile
+	__shared__ float xs[NS];
+	const int x0 = int(blockIdx.x) * TX, y0 = int(blockIdx.y) * TY, z0 = int(blockIdx.z) * TZ;
+	const int n0 = L.n[0], n1 = L.n[1], n2 = L.n[2];
+	const unsigned last = unsigned(n0) * unsigned(n1) * unsigned(n2) - 1;
+	auto inBox = [&](int i, int j, int k) { return i >= 0 && i < n0 && j >= 0 && j < n1 && k >= 0 && k < n2; };
+	{
+		float v[P0];
+		bool in[P0];
+#pragma unroll
+		for (int r = 0; r < P0; r++) {
+			const int t = int(threadIdx.x) + r * NTH;
+			const int i = x0 - 2 + t % SX, j = y0 - 2 + (t / SX) % SY, k = z0 - HZ + t / SXY;
+			in[r] = t < NS && inBox(i, j, k);
+			v[r] = xin[in[r] ? cidx(L, i, j, k) : 0u];
+		}
+#pragma unroll
+		for (int r = 0; r < P0; r++) {
+			const int t = int(threadIdx.x) + r * NTH;
+			if (t < NS) xs[t] = in[r] ? v[r] : 0.0f;
+		}
+	}
+	__syncthreads();
+	// the first colour on the tile and one cell around it
+	{
+		MgRow row[P1];
+		int l1[P1];
+#pragma unroll
+		for (int r = 0; r < P1; r++) {
+			const int t = int(threadIdx.x) + r * NTH;
+			const int j = y0 - 1 + (t / HX) % RY, k = z0 - (D == 3 ? 1 : 0) + t / (HX * RY);
+			const int lo = x0 - 1, i = lo + 2 * (t % HX) + ((lo + j + k + first) & 1);   // (i + j + k) & 1 == first
+			const bool in = t < NR && inBox(i, j, k);
+			l1[r] = in ? ((k - z0 + HZ) * SY + j - y0 + 2) * SX + i - x0 + 2 : -1;
+			row[r] = loadRow<D>(L, in ? cidx(L, i, j, k) : 0u, last);
+		}
+		float v1[P1];
+#pragma unroll
+		for (int r = 0; r < P1; r++) {
+			const int l = l1[r] >= 0 ? l1[r] : kNone;
+			v1[r] = relaxTile<D, SX, SXY>(row[r], xs, l, xs[l]);
+		}
+#pragma unroll
+		for (int r = 0; r < P1; r++)
+			if (l1[r] >= 0) xs[l1[r]] = v1[r];
+	}
+	__syncthreads();
+	// the second colour on the tile; the tile goes out
+	{
+		MgRow row[P2];
+		unsigned c2[P2];
+		int l2[P2];
+		bool in2[P2], upd[P2];
+#pragma unroll
+		for (int r = 0; r < P2; r++) {
+			const int t = int(threadIdx.x) + r * NTH;
+			const int ti = t % TX, tj = (t / TX) % TY, tk = t / (TX * TY);
+			const int i = x0 + ti, j = y0 + tj, k = z0 + tk;
+			in2[r] = t < NT && i < n0 && j < n1 && k < n2;
+			upd[r] = ((i + j + k) & 1) != first;
+			l2[r] = in2[r] ? ((tk + HZ) * SY + tj + 2) * SX + ti + 2 : kNone;
+			c2[r] = in2[r] ? cidx(L, i, j, k) : 0u;
+			row[r] = loadRow<D>(L, c2[r], last);
+		}
+#pragma unroll
+		for (int r = 0; r < P2; r++) {
+			const float v = xs[l2[r]];
+			const float nv = upd[r] ? relaxTile<D, SX, SXY>(row[r], xs, l2[r], v) : v;
+			if (in2[r]) xout[c2[r]] = nv;
+		}
+	}
 }
 
 // coarse right-hand side = sum of the fine residuals of the aggregate; coarse
@@ -433,7 +559,7 @@ bool Multigrid::layout(int dim, const int box_org[3], const int box_n[3], const 
 	}
 	tail_words = 0;
 	for (int l = tail_first; l < levels; l++) tail_words += cells[l];
-	const size_t need = 6 * total + tail_words + 2 * kMgMaxLevels + size_t(unknowns) + 64;
+	const size_t need = 7 * total + tail_words + 2 * kMgMaxLevels + size_t(unknowns) + 64;
 	if (need > pool_cap) {
 		if (pool) (void)hipFree(pool);
 		pool = nullptr;
@@ -443,7 +569,7 @@ bool Multigrid::layout(int dim, const int box_org[3], const int box_n[3], const 
 	}
 	float *p = pool;
 	for (int l = 0; l < levels; l++) {
-		float **arr[6] = {&diag[l], &wx[l], &wy[l], &wz[l], &b[l], &x[l]};
+		float **arr[7] = {&diag[l], &wx[l], &wy[l], &wz[l], &b[l], &x[l], &xalt[l]};
 		for (float **a : arr) { *a = p; p += cells[l]; }
 	}
 	tail_list = reinterpret_cast<unsigned *>(p);
@@ -479,7 +605,7 @@ void mgBuild(Multigrid &M, const Rows<float> &A, int n, const int *mat2grid, hip
 {
 	// every level array starts at zero (inactive cells keep zero entries)
 	size_t total = 0;
-	for (int l = 0; l < M.levels; l++) total += 6 * M.cells[l];
+	for (int l = 0; l < M.levels; l++) total += 7 * M.cells[l];
 	hipLaunchKernelGGL(kMgZero, dim3(mgBlocks(total)), dim3(kBlock), 0, s, M.pool, total);
 	MgBox B;
 	for (int a = 0; a < 3; a++) { B.org[a] = M.org[a]; B.gn[a] = M.gn[a]; }
@@ -497,35 +623,40 @@ void mgBuild(Multigrid &M, const Rows<float> &A, int n, const int *mat2grid, hip
 	else hipLaunchKernelGGL(kMgTailLists<3>, dim3(1), dim3(1024), 0, s, T);
 }
 
+// one sweep of level l (colour `first` first) from x into xalt; the two swap names
+template <int D>
+static void sweep(Multigrid &M, int l, int first, hipStream_t s)
+{
+	const MgLevel L = level(M, l);
+	const int *n = M.dims[l];
+	auto launch = [&](auto kernel, const int *tile, int threads) {
+		const dim3 g(unsigned((n[0] + tile[0] - 1) / tile[0]), unsigned((n[1] + tile[1] - 1) / tile[1]), unsigned((n[2] + tile[2] - 1) / tile[2]));
+		hipLaunchKernelGGL(kernel, g, dim3(unsigned(threads)), 0, s, L, (const float *)M.x[l], M.xalt[l], first);
+	};
+	if constexpr (D == 2) launch(kMgSweep<2, kTile2[0], kTile2[1], kTile2[2], 1024>, kTile2, 1024);
+	else if (M.cells[l] >= size_t(1) << 20) launch(kMgSweep<3, kTileL[0], kTileL[1], kTileL[2], 1024>, kTileL, 1024);
+	else launch(kMgSweep<3, kTileS[0], kTileS[1], kTileS[2], 256>, kTileS, 256);
+	std::swap(M.x[l], M.xalt[l]);
+}
+
 template <int D>
 static void vcycle(Multigrid &M, int n, const double *r, double *z, hipStream_t s)
 {
-	const MgLevel L0 = level(M, 0);
-	hipLaunchKernelGGL(kMgLoad, dim3(mgBlocks(size_t(n))), dim3(kBlock), 0, s, n, (const unsigned *)M.box_of, L0, r);
+	hipLaunchKernelGGL(kMgLoad, dim3(mgBlocks(size_t(n))), dim3(kBlock), 0, s, n, (const unsigned *)M.box_of, level(M, 0), r);
 	const int first = M.tail_first;
 	for (int l = 0; l < first; l++) {
-		const MgLevel L = level(M, l);
-		const Launch3 g = mgLevelLaunch(M, l, size_t(M.dims[l][0] + 1) / 2);
-		for (int v = 0; v < (l == 0 ? kMgSweeps0 : kMgSweeps); v++) {
-			hipLaunchKernelGGL(kMgSmooth<D>, g.grid, g.block, 0, s, L, 0);
-			hipLaunchKernelGGL(kMgSmooth<D>, g.grid, g.block, 0, s, L, 1);
-		}
+		for (int v = 0; v < (l == 0 ? kMgSweeps0 : kMgSweeps); v++) sweep<D>(M, l, 0, s);
 		const Launch3 gr = mgLevelLaunch(M, l + 1, size_t(M.dims[l + 1][0]) * (D == 3 ? 8 : 4));
-		hipLaunchKernelGGL(kMgRestrict<D>, gr.grid, gr.block, 0, s, L, level(M, l + 1));
+		hipLaunchKernelGGL(kMgRestrict<D>, gr.grid, gr.block, 0, s, level(M, l), level(M, l + 1));
 	}
 	const MgTail T = tailDesc(M);
 	hipLaunchKernelGGL(kMgTail<D>, dim3(1), dim3(1024), 0, s, T);
 	for (int l = first - 1; l >= 0; l--) {
-		const MgLevel L = level(M, l);
 		const Launch3 gp = mgLevelLaunch(M, l, size_t(M.dims[l][0]));
-		hipLaunchKernelGGL(kMgProlong<D>, gp.grid, gp.block, 0, s, L, level(M, l + 1));
-		const Launch3 g = mgLevelLaunch(M, l, size_t(M.dims[l][0] + 1) / 2);
-		for (int v = 0; v < (l == 0 ? kMgSweeps0 : kMgSweeps); v++) {
-			hipLaunchKernelGGL(kMgSmooth<D>, g.grid, g.block, 0, s, L, 1);
-			hipLaunchKernelGGL(kMgSmooth<D>, g.grid, g.block, 0, s, L, 0);
-		}
+		hipLaunchKernelGGL(kMgProlong<D>, gp.grid, gp.block, 0, s, level(M, l), level(M, l + 1));
+		for (int v = 0; v < (l == 0 ? kMgSweeps0 : kMgSweeps); v++) sweep<D>(M, l, 1, s);
 	}
-	hipLaunchKernelGGL(kMgStore, dim3(mgBlocks(size_t(n))), dim3(kBlock), 0, s, n, (const unsigned *)M.box_of, L0, z);
+	hipLaunchKernelGGL(kMgStore, dim3(mgBlocks(size_t(n))), dim3(kBlock), 0, s, n, (const unsigned *)M.box_of, level(M, 0), z);
 }
 
 void mgApply(Multigrid &M, int n, const double *r, double *z, hipStream_t s)

@@ -22,6 +22,7 @@ struct Multigrid {
 	size_t cells[kMgMaxLevels] = {};
 	float *diag[kMgMaxLevels] = {}, *wx[kMgMaxLevels] = {}, *wy[kMgMaxLevels] = {}, *wz[kMgMaxLevels] = {};
 	float *b[kMgMaxLevels] = {}, *x[kMgMaxLevels] = {};
+	float *xalt[kMgMaxLevels] = {};   // second iterate buffer: a sweep reads x, writes xalt, and the names swap
 	int org[3] = {};          // box origin in cell coordinates
 	int gn[3] = {};           // cell-grid size
 	int D = 3;
