@@ -37,7 +37,7 @@ struct MgLevel {
 };
 
 // sweep tiles: large levels, small levels, 2D
-constexpr int kTileL[3] = {32, 8, 8}, kTileS[3] = {16, 8, 4}, kTile2[3] = {32, 32, 1};
+constexpr int kTileL[3] = {32, 16, 4}, kTileS[3] = {16, 8, 4}, kTile2[3] = {32, 32, 1};
 
 static MgLevel level(const Multigrid &M, int l)
 {
