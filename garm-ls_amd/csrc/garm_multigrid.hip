@@ -272,33 +272,56 @@ __global__ __launch_bounds__(NTH) void kMgSweep(MgLevel L, const float *xin, flo
 }
 
 // coarse right-hand side = sum of the fine residuals of the aggregate; coarse
-// iterate = 0.  One lane per fine cell: the lanes of an aggregate are an
-// aligned group of 8 (4 in 2D) along x that sums its residuals across
-// lanes, so every lane issues its loads at once instead of one child after
-// another.  Block rows are a multiple of 8 lanes wide.
+// iterate = 0.  A block covers 64 x 2 x 2 fine cells (64 x 2 in 2D), one per
+// thread, so each wave reads 64 consecutive cells of a row.  The residual
+// takes the six neighbour terms unconditionally, in offSum's order: a face
+// on the box boundary has weight 0 and an index outside the level reads an
+// iterate of 0, so a missing neighbour adds +0 exactly, as in relaxTile.  The x pairs of an
+// aggregate meet across adjacent lanes, the y and z pairs in LDS; the sum is
+// ((r000 + r100) + (r010 + r110)) + ((r001 + r101) + (r011 + r111)).
+constexpr int kRestrictX = 64;
+
 template <int D>
 __global__ __launch_bounds__(kBlock) void kMgRestrict(MgLevel F, MgLevel C)
 {
-	constexpr int kids = D == 3 ? 8 : 4;
-	const Tid t = tid3();
-	const int I = t.x / kids, ch = t.x % kids, J = t.y, K = t.z;
-	const bool live = I < C.n[0] && J < C.n[1];
-	float r = 0;
-	if (live) {
-		const int i = 2 * I + (ch & 1), j = 2 * J + ((ch >> 1) & 1), k = 2 * K + (ch >> 2);
-		if (i < F.n[0] && j < F.n[1] && k < F.n[2]) {
-			const unsigned f = cidx(F, i, j, k);
-			const float d = F.diag[f];
-			if (d > 0) r = F.b[f] - (d * F.x[f] - offSum<D>(F, f, i, j, k));
-		}
+	constexpr int BZ = D == 3 ? 2 : 1;
+	__shared__ float pair[BZ][2][kRestrictX / 2];
+	const int i = int(blockIdx.x) * kRestrictX + int(threadIdx.x), j = int(blockIdx.y) * 2 + int(threadIdx.y), k = int(blockIdx.z) * BZ + int(threadIdx.z);
+	const int n0 = F.n[0], n1 = F.n[1], n2 = F.n[2];
+	const bool in = i < n0 && j < n1 && k < n2;
+	const unsigned last = unsigned(n0) * unsigned(n1) * unsigned(n2) - 1;
+	const unsigned sy = unsigned(n0), sz = unsigned(n0) * unsigned(n1);
+	const unsigned c = in ? cidx(F, i, j, k) : 0u;
+	const float d = F.diag[c], b = F.b[c], xc = F.x[c];
+	const float w0 = F.wx[c], w1 = F.wx[min(c + 1, last)], w2 = F.wy[c], w3 = F.wy[min(c + sy, last)];
+	// a clamped +x, +y or +z index (past the last cell) stands for a missing
+	// neighbour: its iterate reads as 0, so the clamped weight adds +0
+	auto xAt = [&](unsigned e, bool ok) { const float v = F.x[ok ? e : 0u]; return ok ? v : 0.0f; };
+	const float x0 = xAt(c - 1, c > 0), x1 = xAt(c + 1, c + 1 <= last), x2 = xAt(c - sy, c >= sy), x3 = xAt(c + sy, c + sy <= last);
+	float s = 0;
+	s += w0 * x0;
+	s += w1 * x1;
+	s += w2 * x2;
+	s += w3 * x3;
+	if constexpr (D == 3) {
+		const float w4 = F.wz[c], w5 = F.wz[min(c + sz, last)];
+		const float x4 = xAt(c - sz, c >= sz), x5 = xAt(c + sz, c + sz <= last);
+		s += w4 * x4;
+		s += w5 * x5;
 	}
+	float r = in && d > 0 ? b - (d * xc - s) : 0.0f;
 	r += __shfl_xor(r, 1);
-	r += __shfl_xor(r, 2);
-	if constexpr (D == 3) r += __shfl_xor(r, 4);
-	if (live && ch == 0) {
-		const unsigned q = cidx(C, I, J, K);
-		C.b[q] = r;
-		C.x[q] = 0.0f;
+	if ((threadIdx.x & 1) == 0) pair[threadIdx.z][threadIdx.y][threadIdx.x / 2] = r;
+	__syncthreads();
+	if (threadIdx.y == 0 && threadIdx.z == 0 && (threadIdx.x & 1) == 0) {
+		const int h = int(threadIdx.x) / 2, I = i / 2, J = j / 2, K = k / BZ;
+		float sum = pair[0][0][h] + pair[0][1][h];
+		if constexpr (D == 3) sum = sum + (pair[1][0][h] + pair[1][1][h]);
+		if (I < C.n[0] && J < C.n[1] && K < C.n[2]) {
+			const unsigned q = cidx(C, I, J, K);
+			C.b[q] = sum;
+			C.x[q] = 0.0f;
+		}
 	}
 }
 
@@ -646,8 +669,9 @@ static void vcycle(Multigrid &M, int n, const double *r, double *z, hipStream_t 
 	const int first = M.tail_first;
 	for (int l = 0; l < first; l++) {
 		for (int v = 0; v < (l == 0 ? kMgSweeps0 : kMgSweeps); v++) sweep<D>(M, l, 0, s);
-		const Launch3 gr = mgLevelLaunch(M, l + 1, size_t(M.dims[l + 1][0]) * (D == 3 ? 8 : 4));
-		hipLaunchKernelGGL(kMgRestrict<D>, gr.grid, gr.block, 0, s, level(M, l), level(M, l + 1));
+		const int *f = M.dims[l];
+		const dim3 gr(unsigned((f[0] + kRestrictX - 1) / kRestrictX), unsigned((f[1] + 1) / 2), unsigned(D == 3 ? (f[2] + 1) / 2 : 1));
+		hipLaunchKernelGGL(kMgRestrict<D>, gr, dim3(kRestrictX, 2, D == 3 ? 2 : 1), 0, s, level(M, l), level(M, l + 1));
 	}
 	const MgTail T = tailDesc(M);
 	hipLaunchKernelGGL(kMgTail<D>, dim3(1), dim3(1024), 0, s, T);
