@@ -616,6 +616,114 @@ __global__ __launch_bounds__(kBlock) void kAdvectVelocityTile(FaceArrays<float> 
 	out[gidx<D>(g, c.i, c.j, c.k)] = val;
 }
 
+// The 3D float form for a column of kVelColumn z-planes per workgroup: the
+// 64 x 4 faces of each plane in turn, with a window of the seven planes
+// k - 3 .. k + 3 (70 x 10 faces each, clamped into the grid) in LDS as a
+// ring.  Moving to the next plane loads one plane of the window instead of
+// seven, so each face costs about four staged loads instead of nineteen.
+// Stencils that leave the window read the global copy; every face performs
+// kAdvectVelocityTile's operations in the same order.
+constexpr int kVelColumn = 16;
+
+template <int D>   // 3 (a template so that each compile of this file has its own instance)
+__global__ __launch_bounds__(kBlock) void kAdvectVelocityColumn(FaceArrays<float> U, int axis, double *out, double hd)
+{
+	static_assert(D == 3, "3D only");
+	constexpr int TXT = 64 + 2 * kVelHalo, TYT = 4 + 2 * kVelHalo, W = 1 + 2 * kVelHalo, kPlane = TXT * TYT;
+	constexpr int kFill = (kPlane + kBlock - 1) / kBlock;
+	__shared__ float win[W][kPlane];
+	const GridDev &g = U.g[axis];
+	const float *ua = U.a[axis];
+	const int tx0 = int(blockIdx.x * blockDim.x) - kVelHalo, ty0 = int(blockIdx.y * blockDim.y) - kVelHalo;
+	const int z0 = int(blockIdx.z) * kVelColumn;
+	const int tid = int(threadIdx.x + blockDim.x * threadIdx.y);
+	const int i = int(blockIdx.x * blockDim.x + threadIdx.x), j = int(blockIdx.y * blockDim.y + threadIdx.y);
+	const bool live = i < g.n[0] && j < g.n[1];
+	const float h = float(hd);
+	// plane p of the grid (clamped) into its ring slot
+	auto fillPlane = [&](int p) {
+		float v[kFill];
+#pragma unroll
+		for (int r = 0; r < kFill; r++) {
+			const int q = tid + r * kBlock;
+			v[r] = q < kPlane ? ua[gat<3>(g, tx0 + q % TXT, ty0 + q / TXT, p)] : 0.0f;
+		}
+		float *slot = win[(p + W) % W];
+#pragma unroll
+		for (int r = 0; r < kFill; r++) {
+			const int q = tid + r * kBlock;
+			if (q < kPlane) slot[q] = v[r];
+		}
+	};
+	for (int p = z0 - kVelHalo; p < z0 + kVelHalo; p++) fillPlane(p);
+	for (int k = z0; k < z0 + kVelColumn && k < g.n[2]; k++) {
+		fillPlane(k + kVelHalo);
+		__syncthreads();
+		if (live) {
+			float x[3] = {float(g.o[0] + i * g.dx), float(g.o[1] + j * g.dx), float(g.o[2] + k * g.dx)};
+			float v0[3], p1[3], v1[3], X[3];
+			sampleVel<3, float, float>(U, x, v0);
+#pragma unroll
+			for (int a = 0; a < 3; a++) p1[a] = x[a] + v0[a] * h;
+			sampleVel<3, float, float>(U, p1, v1);
+#pragma unroll
+			for (int a = 0; a < 3; a++) X[a] = x[a] + (v0[a] + v1[a]) * h / 2;
+			int l[3]; float f[3];
+#pragma unroll
+			for (int a = 0; a < 3; a++) {
+				const float o = float(g.o[a]), inv = float(g.inv);
+				l[a] = int(floor((X[a] - o) * inv)) - 1;   // getCubicLower (Grid.h:54)
+				f[a] = (X[a] - o - l[a] * float(g.dx)) * inv;
+			}
+			float w[4][3];
+#pragma unroll
+			for (int a = 0; a < 3; a++) {
+				const float t = f[a];
+				w[0][a] = (t - 1) * (t - 2) * (t - 3) / -6;
+				w[1][a] = t * (t - 2) * (t - 3) / 2;
+				w[2][a] = t * (t - 1) * (t - 3) / -2;
+				w[3][a] = t * (t - 1) * (t - 2) / 6;
+			}
+			const int lx = l[0] - tx0, ly = l[1] - ty0;
+			const bool inside = lx >= 0 && lx + 3 < TXT && ly >= 0 && ly + 3 < TYT && l[2] >= k - kVelHalo && l[2] + 3 <= k + kVelHalo;
+			float val = 0;
+			if (inside) {
+#pragma unroll
+				for (int kk = 0; kk < 4; kk++) {
+					const float *pl = win[(l[2] + kk + W) % W];
+					float plane = 0;
+#pragma unroll
+					for (int jj = 0; jj < 4; jj++) {
+						const float *row = pl + (ly + jj) * TXT + lx;
+						float r = 0;
+#pragma unroll
+						for (int ii = 0; ii < 4; ii++) r += row[ii] * w[ii][0];
+						plane += r * w[jj][1];
+					}
+					val += plane * w[kk][2];
+				}
+			} else {
+				unsigned ox[4], oy[4], oz[4];
+				cornerOffsets<3, 4>(g, l, ox, oy, oz);
+#pragma unroll
+				for (int kk = 0; kk < 4; kk++) {
+					float plane = 0;
+#pragma unroll
+					for (int jj = 0; jj < 4; jj++) {
+						float r = 0;
+#pragma unroll
+						for (int ii = 0; ii < 4; ii++) r += ua[ox[ii] + oy[jj] + oz[kk]] * w[ii][0];
+						plane += r * w[jj][1];
+					}
+					val += plane * w[kk][2];
+				}
+			}
+			out[gidx<3>(g, i, j, k)] = val;
+		}
+		__syncthreads();
+	}
+}
+
 // float copy of a face array (the FP32 velocity advection's input)
 template <typename S>
 __global__ void kFaceToFloat(const double *u, S *out, size_t n)
@@ -872,8 +980,13 @@ static void advectVelocityD(const Fields &F, double dt, hipStream_t s, bool copi
 			U.a[a] = copy;
 			U.g[a] = F.face[a];
 		}
-		for (int a = 0; a < D; a++)
-			hipLaunchKernelGGL((kAdvectVelocityTile<D>), gridDims(F.face[a], D), gridBlock(), 0, s, U, a, F.u_new[a], -dt);
+		for (int a = 0; a < D; a++) {
+			if constexpr (D == 3) {
+				const GridDev &g = F.face[a];
+				const dim3 gd(unsigned((g.n[0] + 63) / 64), unsigned((g.n[1] + 3) / 4), unsigned((g.n[2] + kVelColumn - 1) / kVelColumn));
+				hipLaunchKernelGGL(kAdvectVelocityColumn<3>, gd, gridBlock(), 0, s, U, a, F.u_new[a], -dt);
+			} else hipLaunchKernelGGL((kAdvectVelocityTile<D>), gridDims(F.face[a], D), gridBlock(), 0, s, U, a, F.u_new[a], -dt);
+		}
 	} else {
 		(void)copiesFresh;
 		FaceArrays<double> U{};
