@@ -157,6 +157,25 @@ __device__ __forceinline__ void sampleVelGradPacked(const FaceSet &U, const T *c
 	}
 }
 
+// sampleVelGradPacked at a cell centre, in float: along axis a the centre
+// lies halfway between faces i and i + 1 of face grid a and on that grid's
+// points along the other axes, so the multilinear sample is the mean of those
+// two face records (the other corners carry weight 0).
+template <int D>
+__device__ __forceinline__ void sampleVelGradCentre(const FaceSet &U, const float *const *rec, const Coord &c, float *v, float *G)
+{
+#pragma unroll
+	for (int a = 0; a < D; a++) {
+		const GridDev &g = U.g[a];
+		float d0[4], d1[4];
+		loadRecord<float, 4>(rec[a], gidx<D>(g, c.i, c.j, c.k), d0);
+		loadRecord<float, 4>(rec[a], gidx<D>(g, c.i + (a == 0), c.j + (a == 1), c.k + (a == 2)), d1);
+		v[a] = (d0[0] + d1[0]) * 0.5f;
+#pragma unroll
+		for (int b = 0; b < D; b++) G[a * D + b] = (d0[1 + b] + d1[1 + b]) * 0.5f;
+	}
+}
+
 // ----------------------------------------------------------------- Hermite
 // 1-D cubic Hermite basis (HermiteIntrpl.h:89-125) for one axis:
 // B[vtx][ord] value weights, Bg[vtx][ord] their derivatives.
@@ -413,7 +432,8 @@ __global__ __launch_bounds__(kBlock) void kAdvectRefMap(RefMapArgs<D, T> A)
 	T x[3] = {T(A.cell.o[0] + c.i * A.cell.dx), T(A.cell.o[1] + c.j * A.cell.dx), T(A.cell.o[2] + c.k * A.cell.dx)};
 
 	T v0[3], G0[9], p1[3], R1[9], v1[3], G1[9], X[3], R[9];
-	sampleVelGradPacked<D, T>(A.U, A.vrec, x, v0, G0);
+	if constexpr (std::is_same<T, float>::value) sampleVelGradCentre<D>(A.U, A.vrec, c, v0, G0);
+	else sampleVelGradPacked<D, T>(A.U, A.vrec, x, v0, G0);
 #pragma unroll
 	for (int a = 0; a < D; a++) p1[a] = x[a] + v0[a] * h;
 #pragma unroll
@@ -621,8 +641,9 @@ __global__ __launch_bounds__(kBlock) void kAdvectVelocityTile(FaceArrays<float> 
 // k - 3 .. k + 3 (70 x 10 faces each, clamped into the grid) in LDS as a
 // ring.  Moving to the next plane loads one plane of the window instead of
 // seven, so each face costs about four staged loads instead of nineteen.
-// Stencils that leave the window read the global copy; every face performs
-// kAdvectVelocityTile's operations in the same order.
+// Stencils that leave the window read the global copy.  The first sample of
+// the trace, at the face itself, takes the two or four faces its multilinear
+// weights do not zero; the rest is kAdvectVelocityTile's arithmetic.
 constexpr int kVelColumn = 16;
 
 template <int D>   // 3 (a template so that each compile of this file has its own instance)
@@ -662,7 +683,22 @@ __global__ __launch_bounds__(kBlock) void kAdvectVelocityColumn(FaceArrays<float
 		if (live) {
 			float x[3] = {float(g.o[0] + i * g.dx), float(g.o[1] + j * g.dx), float(g.o[2] + k * g.dx)};
 			float v0[3], p1[3], v1[3], X[3];
-			sampleVel<3, float, float>(U, x, v0);
+			// the multilinear sample at the face itself: its own component is
+			// the face value; component a lies halfway between points
+			// idx - 1, idx along `axis` and idx, idx + 1 along a of face
+			// grid a (the other corners carry weight 0)
+#pragma unroll
+			for (int a = 0; a < 3; a++) {
+				if (a == axis) { v0[a] = win[(k + W) % W][(j - ty0) * TXT + i - tx0]; continue; }
+				const GridDev &ga = U.g[a];
+				const float *u = U.a[a];
+				const int e0[3] = {axis == 0, axis == 1, axis == 2}, e1[3] = {a == 0, a == 1, a == 2};
+				const int b0 = i - e0[0], b1 = j - e0[1], b2 = k - e0[2];
+				const float u00 = u[gat<3>(ga, b0, b1, b2)], u10 = u[gat<3>(ga, b0 + e0[0], b1 + e0[1], b2 + e0[2])];
+				const float u01 = u[gat<3>(ga, b0 + e1[0], b1 + e1[1], b2 + e1[2])];
+				const float u11 = u[gat<3>(ga, b0 + e0[0] + e1[0], b1 + e0[1] + e1[1], b2 + e0[2] + e1[2])];
+				v0[a] = ((u00 + u10) + (u01 + u11)) * 0.25f;
+			}
 #pragma unroll
 			for (int a = 0; a < 3; a++) p1[a] = x[a] + v0[a] * h;
 			sampleVel<3, float, float>(U, p1, v1);
