@@ -32,7 +32,11 @@ __device__ __forceinline__ float rsqrtT(float x) { return __builtin_amdgcn_rsqf(
 // which cancel in the normalisation: smoothness indicators and epsilon are
 // scaled by 4 (13/3 e^2 + f^2 instead of 13/12 e^2 + 1/4 f^2), the linear
 // weights 0.1 : 0.6 : 0.3 are 1 : 6 : 3 with the 6 and the 3 folded into the
-// candidate polynomials 2 and 3.
+// candidate polynomials 2 and 3.  The nonlinear weights 1 / (s_k + eps)^2 are
+// multiplied through by the product of all three, (s_k / eps + 1)^2 taken
+// with one reciprocal of eps: weight k becomes the product of the other two
+// (between 1 and 1e31, since eps is 4e-6 of the largest d^2 and s_k at most
+// about 140 times it), so a side takes two reciprocals instead of four.
 __device__ __forceinline__ void weno5f(const float *v, float &dm, float &dp)
 {
 	float d[6];
@@ -58,7 +62,8 @@ __device__ __forceinline__ void weno5f(const float *v, float &dm, float &dp)
 		const float p1 = d[0] * k3 - d[1] * k76 + d[2] * k116;       // candidate 1
 		const float p2 = 5 * d[2] - d[1] + 2 * d[3];                  // 6 x candidate 2
 		const float p3 = d[2] + 2.5f * d[3] - 0.5f * d[4];            // 3 x candidate 3
-		const float a1 = rcpT(sq(s1 + epsM)), a2 = rcpT(sq(s2 + epsM)), a3 = rcpT(sq(s3 + epsM));
+		const float r = rcpT(epsM), t1 = sq(s1 * r + 1), t2 = sq(s2 * r + 1), t3 = sq(s3 * r + 1);
+		const float a1 = t2 * t3, a2 = t1 * t3, a3 = t1 * t2;
 		dm = (a1 * p1 + a2 * p2 + a3 * p3) * rcpT(a1 + 6 * a2 + 3 * a3);
 	}
 	// plus side: the stencil d5, d4, d3, d2, d1
@@ -69,7 +74,8 @@ __device__ __forceinline__ void weno5f(const float *v, float &dm, float &dp)
 		const float p1 = d[5] * k3 - d[4] * k76 + d[3] * k116;
 		const float p2 = 5 * d[3] - d[4] + 2 * d[2];
 		const float p3 = d[3] + 2.5f * d[2] - 0.5f * d[1];
-		const float a1 = rcpT(sq(s1 + epsP)), a2 = rcpT(sq(s2 + epsP)), a3 = rcpT(sq(s3 + epsP));
+		const float r = rcpT(epsP), t1 = sq(s1 * r + 1), t2 = sq(s2 * r + 1), t3 = sq(s3 * r + 1);
+		const float a1 = t2 * t3, a2 = t1 * t3, a3 = t1 * t2;
 		dp = (a1 * p1 + a2 * p2 + a3 * p3) * rcpT(a1 + 6 * a2 + 3 * a3);
 	}
 }
