@@ -397,10 +397,10 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 		B.gn[a] = cell.n[a];
 	}
 	const size_t boxVox = size_t(B.n[0]) * B.n[1] * B.n[2];
-	// inference buffers only (a box that outgrows them reallocates without the
-	// training gradients, with half again as much room)
-	if (!U.buf.reserve(std::max(boxVox + boxVox / 2, U.buf.cap_vox), false, err) || !U.buf.setDims(B.n[0], B.n[1], B.n[2], err))
-		return GARM_ENOMEM;
+	// the buffers are reallocated only when the box outgrows them, then for
+	// inference only (no training gradients) and with half again as much room
+	if (boxVox > U.buf.cap_vox && !U.buf.reserve(boxVox + boxVox / 2, false, err)) return GARM_ENOMEM;
+	if (!U.buf.setDims(B.n[0], B.n[1], B.n[2], err)) return GARM_ENOMEM;
 	// z = M(r): two-level cycle, network on the pre-smoothed residual
 	auto precondition = [&]() {
 		hipLaunchKernelGGL(kJacobi, g1, b1, 0, s, (const double *)A64.invdiag, n, (const double *)W.r, W.z);
