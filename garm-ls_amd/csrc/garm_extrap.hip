@@ -4,13 +4,13 @@
 // element taking the mean of its reached neighbours.  Here each element
 // carries the layer at which it is reached (0 fixed, 255 not yet).  One
 // full-grid pass marks the fixed elements and lists those with an unfixed
-// neighbour; every later layer L walks only the list of layer L-1, claims
-// each unreached neighbour with a compare-and-swap on its byte, gives it the
-// mean of its neighbours below layer L (in the reference's ordinal order
-// -x, +x, -y, +y, -z, +z, so the value is the reference's exactly) and lists
-// it for layer L+1.  Work per layer is proportional to the front, not to the
-// grid.  A byte compare-and-swap retries at most once per byte of its word
-// that changes in the same launch, so every claim loop ends.  Lists are
+// neighbour; every later layer L walks only the list of layer L-1: each
+// unreached neighbour is taken by exactly one element of that list (its
+// first neighbour at layer L-1 in the reference's ordinal order -x, +x, -y,
+// +y, -z, +z), which gives it the mean of its neighbours below layer L (in
+// that order, so the value is the reference's exactly) and lists it for
+// layer L+1.  Work per layer is proportional to the front, not to the grid,
+// and takes no atomics but the list appends.  Lists are
 // appended a workgroup at a time (a prefix sum in LDS and one atomic on the
 // list's counter per workgroup and pass): atomics on one address serialise
 // in L2, and a front of 10^5 elements appended one by one costs more than
@@ -104,11 +104,13 @@ __device__ __forceinline__ void decode(const LayerSpace &S, int gid, int &a, int
 	k = rest / g.n[1];
 }
 
-// Full pass over grid `a`: layer 0 for fixed elements, 255 otherwise; fixed
-// elements with an unfixed neighbour start the front.  Each thread takes
-// kInitRun elements along the slowest axis (3D: k = blockIdx.z * kInitRun + r;
-// 2D: j = blockIdx.y * 4 * kInitRun + 4 r + threadIdx.y), so a workgroup
-// appends up to 2048 elements with one atomic.
+// Start of a front on grid `a`, in two passes: kFrontMark gives every element
+// layer 0 if fixed and 255 otherwise (the fixing rule's loads once per
+// element); kFrontEdges lists the fixed elements with an unfixed neighbour,
+// reading only layer bytes.  Each kFrontEdges thread takes kInitRun elements
+// along the slowest axis (3D: k = blockIdx.z * kInitRun + r; 2D:
+// j = blockIdx.y * 4 * kInitRun + 4 r + threadIdx.y), so a workgroup appends
+// up to 2048 elements with one atomic.
 constexpr int kInitRun = 8;
 
 template <int D>
@@ -119,26 +121,32 @@ static dim3 frontInitDims(const GridDev &g)
 }
 
 template <int D, int RULE>
-__global__ __launch_bounds__(kBlock) void kFrontInit(LayerSpace S, int a, FixArgs P, int *list, int *cnt)
+__global__ __launch_bounds__(kBlock) void kFrontMark(LayerSpace S, int a, FixArgs P)
 {
 	const GridDev &g = S.g[a];
+	const Coord c = threadCoord();
+	if (!inGrid<D>(g, c)) return;
+	S.layer[a][gidx<D>(g, c.i, c.j, c.k)] = isFixed<D, RULE>(P, a, g, c.i, c.j, c.k) ? 0 : kFar;
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void kFrontEdges(LayerSpace S, int a, int *list, int *cnt)
+{
+	const GridDev &g = S.g[a];
+	const uint8_t *lay = S.layer[a];
 	const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
 	unsigned edges = 0;
 #pragma unroll
 	for (int r = 0; r < kInitRun; r++) {
 		const int j = D == 3 ? int(blockIdx.y * blockDim.y + threadIdx.y) : int(blockIdx.y * blockDim.y * kInitRun + r * blockDim.y + threadIdx.y);
 		const int k = D == 3 ? int(blockIdx.z) * kInitRun + r : 0;
-		if (!validIn<D>(g, i, j, k)) continue;
-		const size_t id = gidx<D>(g, i, j, k);
-		const bool fixed = isFixed<D, RULE>(P, a, g, i, j, k);
-		S.layer[a][id] = fixed ? 0 : kFar;
-		if (!fixed) continue;
+		if (!validIn<D>(g, i, j, k) || lay[gidx<D>(g, i, j, k)] != 0) continue;
 		bool edge = false;
 #pragma unroll
 		for (int o = 0; o < 2 * D; o++) {
 			const int ax = o >> 1, sd = (o & 1) ? 1 : -1;
 			const int qi = i + (ax == 0 ? sd : 0), qj = j + (ax == 1 ? sd : 0), qk = k + (ax == 2 ? sd : 0);
-			if (validIn<D>(g, qi, qj, qk) && !isFixed<D, RULE>(P, a, g, qi, qj, qk)) edge = true;
+			if (validIn<D>(g, qi, qj, qk) && lay[gidx<D>(g, qi, qj, qk)] == kFar) edge = true;
 		}
 		if (edge) edges |= 1u << r;
 	}
@@ -152,26 +160,18 @@ __global__ __launch_bounds__(kBlock) void kFrontInit(LayerSpace S, int a, FixArg
 	}
 }
 
-__device__ __forceinline__ bool claimByte(uint8_t *base, size_t idx, uint8_t L)
-{
-	const uintptr_t addr = reinterpret_cast<uintptr_t>(base + idx);
-	unsigned int *word = reinterpret_cast<unsigned int *>(addr & ~uintptr_t(3));
-	const int shift = int(addr & 3) * 8;
-	unsigned int old = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-	for (;;) {
-		if (((old >> shift) & 0xffu) != kFar) return false;
-		const unsigned int nw = (old & ~(0xffu << shift)) | (unsigned(L) << shift);
-		const unsigned int prev = atomicCAS(word, old, nw);
-		if (prev == old) return true;
-		old = prev;
-	}
-}
-
 // Layer L of one front: one grid-stride element t of its layer L-1 list
-// claims each unreached neighbour, gives it the mean of its neighbours below
-// layer L (Extrapolation.h:101-124) and lists it for layer L+1.  Every
-// thread of the workgroup calls this for the same workgroup-uniform t0, as
-// blockAppendBase needs.
+// looks at each unreached neighbour q.  q is reached at layer L, and its
+// value is the mean of its neighbours below layer L (Extrapolation.h:101-124,
+// in the ordinal order, so the value is the reference's exactly); it is
+// taken by its owner, the first of its neighbours in the ordinal order that
+// is at layer L-1, which writes the value and the layer byte and lists q for
+// layer L+1.  Every other element that sees q stops at that test.  The test
+// and the mean read only bytes that are final (layers below L, written by
+// earlier launches) or move from 255 to L during this launch, which both
+// reads treat alike, so no atomics are needed and each q is listed once.
+// Every thread of the workgroup calls this for the same workgroup-uniform t0,
+// as blockAppendBase needs.
 template <int D>
 __device__ __forceinline__ void frontStepElements(const LayerSpace &S, int L, const int *listIn, int n, int t0, int *listOut, int *cntOut)
 {
@@ -187,8 +187,9 @@ __device__ __forceinline__ void frontStepElements(const LayerSpace &S, int L, co
 			const int qi = i + (ax == 0 ? sd : 0), qj = j + (ax == 1 ? sd : 0), qk = k + (ax == 2 ? sd : 0);
 			if (!validIn<D>(g, qi, qj, qk)) continue;
 			const size_t q = gidx<D>(g, qi, qj, qk);
-			if (lay[q] != kFar || !claimByte(lay, q, uint8_t(L))) continue;
-			took |= 1u << o;
+			if (lay[q] != kFar) continue;
+			// this element is q's neighbour in direction o ^ 1 (seen from q)
+			bool owner = true, seen = false;
 			int cnt = 0;
 			double s0 = 0, s1 = 0, s2 = 0;
 			for (int p = 0; p < 2 * D; p++) {
@@ -196,16 +197,24 @@ __device__ __forceinline__ void frontStepElements(const LayerSpace &S, int L, co
 				const int ri = qi + (bx == 0 ? bs : 0), rj = qj + (bx == 1 ? bs : 0), rk = qk + (bx == 2 ? bs : 0);
 				if (!validIn<D>(g, ri, rj, rk)) continue;
 				const size_t rr = gidx<D>(g, ri, rj, rk);
-				if (lay[rr] < L) {
+				const uint8_t lr = lay[rr];
+				if (!seen && lr == uint8_t(L - 1)) {
+					seen = true;
+					if (p != (o ^ 1)) { owner = false; break; }
+				}
+				if (lr < L) {
 					if (S.nv > 0) s0 += S.v[a][0][rr];
 					if (S.nv > 1) s1 += S.v[a][1][rr];
 					if (S.nv > 2) s2 += S.v[a][2][rr];
 					cnt++;
 				}
 			}
+			if (!owner) continue;
+			took |= 1u << o;
 			if (S.nv > 0) S.v[a][0][q] = s0 / cnt;
 			if (S.nv > 1) S.v[a][1][q] = s1 / cnt;
 			if (S.nv > 2) S.v[a][2][q] = s2 / cnt;
+			lay[q] = uint8_t(L);
 		}
 	}
 	int at = blockAppendBase(cntOut, __popc(took));
@@ -301,13 +310,14 @@ static unsigned linBlocks(size_t n)
 template <int D>
 static void launchFrontInit(const LayerSpace &S, int a, const FixArgs &P, int *list, int *cnt, hipStream_t s)
 {
-	const dim3 gd = frontInitDims<D>(S.g[a]), bd = gridBlock();
+	const dim3 gm = gridDims(S.g[a], D), bd = gridBlock();
 	switch (P.rule) {
-	case kFixCellDistPositive: hipLaunchKernelGGL((kFrontInit<D, kFixCellDistPositive>), gd, bd, 0, s, S, a, P, list, cnt); break;
-	case kFixAdvRegion: hipLaunchKernelGGL((kFrontInit<D, kFixAdvRegion>), gd, bd, 0, s, S, a, P, list, cnt); break;
-	case kFixPhiOriNonPositive: hipLaunchKernelGGL((kFrontInit<D, kFixPhiOriNonPositive>), gd, bd, 0, s, S, a, P, list, cnt); break;
-	default: hipLaunchKernelGGL((kFrontInit<D, kFixLiquidFace>), gd, bd, 0, s, S, a, P, list, cnt); break;
+	case kFixCellDistPositive: hipLaunchKernelGGL((kFrontMark<D, kFixCellDistPositive>), gm, bd, 0, s, S, a, P); break;
+	case kFixAdvRegion: hipLaunchKernelGGL((kFrontMark<D, kFixAdvRegion>), gm, bd, 0, s, S, a, P); break;
+	case kFixPhiOriNonPositive: hipLaunchKernelGGL((kFrontMark<D, kFixPhiOriNonPositive>), gm, bd, 0, s, S, a, P); break;
+	default: hipLaunchKernelGGL((kFrontMark<D, kFixLiquidFace>), gm, bd, 0, s, S, a, P); break;
 	}
+	hipLaunchKernelGGL(kFrontEdges<D>, frontInitDims<D>(S.g[a]), bd, 0, s, S, a, list, cnt);
 }
 
 // Run `steps` layers over S with fixed elements given by P.

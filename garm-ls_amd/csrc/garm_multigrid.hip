@@ -188,8 +188,9 @@ __device__ __forceinline__ float relaxTile(const MgRow &r, const float *xs, int 
 // keeps the blocks independent, and every cell gets exactly the operations
 // of one single-colour pass after the other.  Each phase gives every thread
 // a fixed count of cells, predicated (indices outside the level clamp to a
-// valid cell, results are dropped), and issues all of their loads before it
-// uses any, so they are in flight together.
+// valid cell, results are dropped), and a thread issues the global loads of
+// all three phases before its first LDS store, so they are in flight
+// together and the phases after the first barrier wait only on LDS.
 template <int D, int TX, int TY, int TZ, int NTH>
 __global__ __launch_bounds__(NTH) void kMgSweep(MgLevel L, const float *xin, float *xout, int first)
 {
@@ -204,41 +205,56 @@ __global__ __launch_bounds__(NTH) void kMgSweep(MgLevel L, const float *xin, flo
 	const int n0 = L.n[0], n1 = L.n[1], n2 = L.n[2];
 	const unsigned last = unsigned(n0) * unsigned(n1) * unsigned(n2) - 1;
 	auto inBox = [&](int i, int j, int k) { return i >= 0 && i < n0 && j >= 0 && j < n1 && k >= 0 && k < n2; };
-	{
-		float v[P0];
-		bool in[P0];
+	// every global load of the three phases first: the staged iterate and the
+	// operator rows of both colours' cells (the rows do not depend on the tile)
+	float v0[P0];
+	bool in0[P0];
 #pragma unroll
-		for (int r = 0; r < P0; r++) {
-			const int t = int(threadIdx.x) + r * NTH;
-			const int i = x0 - 2 + t % SX, j = y0 - 2 + (t / SX) % SY, k = z0 - HZ + t / SXY;
-			in[r] = t < NS && inBox(i, j, k);
-			v[r] = xin[in[r] ? cidx(L, i, j, k) : 0u];
-		}
+	for (int r = 0; r < P0; r++) {
+		const int t = int(threadIdx.x) + r * NTH;
+		const int i = x0 - 2 + t % SX, j = y0 - 2 + (t / SX) % SY, k = z0 - HZ + t / SXY;
+		in0[r] = t < NS && inBox(i, j, k);
+		v0[r] = xin[in0[r] ? cidx(L, i, j, k) : 0u];
+	}
+	MgRow row1[P1];
+	int l1[P1];
 #pragma unroll
-		for (int r = 0; r < P0; r++) {
-			const int t = int(threadIdx.x) + r * NTH;
-			if (t < NS) xs[t] = in[r] ? v[r] : 0.0f;
-		}
+	for (int r = 0; r < P1; r++) {
+		const int t = int(threadIdx.x) + r * NTH;
+		const int j = y0 - 1 + (t / HX) % RY, k = z0 - (D == 3 ? 1 : 0) + t / (HX * RY);
+		const int lo = x0 - 1, i = lo + 2 * (t % HX) + ((lo + j + k + first) & 1);   // (i + j + k) & 1 == first
+		const bool in = t < NR && inBox(i, j, k);
+		l1[r] = in ? ((k - z0 + HZ) * SY + j - y0 + 2) * SX + i - x0 + 2 : -1;
+		row1[r] = loadRow<D>(L, in ? cidx(L, i, j, k) : 0u, last);
+	}
+	MgRow row2[P2];
+	unsigned c2[P2];
+	int l2[P2];
+	bool in2[P2], upd[P2];
+#pragma unroll
+	for (int r = 0; r < P2; r++) {
+		const int t = int(threadIdx.x) + r * NTH;
+		const int ti = t % TX, tj = (t / TX) % TY, tk = t / (TX * TY);
+		const int i = x0 + ti, j = y0 + tj, k = z0 + tk;
+		in2[r] = t < NT && i < n0 && j < n1 && k < n2;
+		upd[r] = ((i + j + k) & 1) != first;
+		l2[r] = in2[r] ? ((tk + HZ) * SY + tj + 2) * SX + ti + 2 : kNone;
+		c2[r] = in2[r] ? cidx(L, i, j, k) : 0u;
+		row2[r] = loadRow<D>(L, upd[r] ? c2[r] : 0u, last);
+	}
+#pragma unroll
+	for (int r = 0; r < P0; r++) {
+		const int t = int(threadIdx.x) + r * NTH;
+		if (t < NS) xs[t] = in0[r] ? v0[r] : 0.0f;
 	}
 	__syncthreads();
 	// the first colour on the tile and one cell around it
 	{
-		MgRow row[P1];
-		int l1[P1];
-#pragma unroll
-		for (int r = 0; r < P1; r++) {
-			const int t = int(threadIdx.x) + r * NTH;
-			const int j = y0 - 1 + (t / HX) % RY, k = z0 - (D == 3 ? 1 : 0) + t / (HX * RY);
-			const int lo = x0 - 1, i = lo + 2 * (t % HX) + ((lo + j + k + first) & 1);   // (i + j + k) & 1 == first
-			const bool in = t < NR && inBox(i, j, k);
-			l1[r] = in ? ((k - z0 + HZ) * SY + j - y0 + 2) * SX + i - x0 + 2 : -1;
-			row[r] = loadRow<D>(L, in ? cidx(L, i, j, k) : 0u, last);
-		}
 		float v1[P1];
 #pragma unroll
 		for (int r = 0; r < P1; r++) {
 			const int l = l1[r] >= 0 ? l1[r] : kNone;
-			v1[r] = relaxTile<D, SX, SXY>(row[r], xs, l, xs[l]);
+			v1[r] = relaxTile<D, SX, SXY>(row1[r], xs, l, xs[l]);
 		}
 #pragma unroll
 		for (int r = 0; r < P1; r++)
@@ -246,28 +262,11 @@ __global__ __launch_bounds__(NTH) void kMgSweep(MgLevel L, const float *xin, flo
 	}
 	__syncthreads();
 	// the second colour on the tile; the tile goes out
-	{
-		MgRow row[P2];
-		unsigned c2[P2];
-		int l2[P2];
-		bool in2[P2], upd[P2];
 #pragma unroll
-		for (int r = 0; r < P2; r++) {
-			const int t = int(threadIdx.x) + r * NTH;
-			const int ti = t % TX, tj = (t / TX) % TY, tk = t / (TX * TY);
-			const int i = x0 + ti, j = y0 + tj, k = z0 + tk;
-			in2[r] = t < NT && i < n0 && j < n1 && k < n2;
-			upd[r] = ((i + j + k) & 1) != first;
-			l2[r] = in2[r] ? ((tk + HZ) * SY + tj + 2) * SX + ti + 2 : kNone;
-			c2[r] = in2[r] ? cidx(L, i, j, k) : 0u;
-			row[r] = loadRow<D>(L, c2[r], last);
-		}
-#pragma unroll
-		for (int r = 0; r < P2; r++) {
-			const float v = xs[l2[r]];
-			const float nv = upd[r] ? relaxTile<D, SX, SXY>(row[r], xs, l2[r], v) : v;
-			if (in2[r]) xout[c2[r]] = nv;
-		}
+	for (int r = 0; r < P2; r++) {
+		const float v = xs[l2[r]];
+		const float nv = upd[r] ? relaxTile<D, SX, SXY>(row2[r], xs, l2[r], v) : v;
+		if (in2[r]) xout[c2[r]] = nv;
 	}
 }
 
