@@ -238,6 +238,7 @@ LearnedWork::~LearnedWork()
 	auto fr = [](auto *&q) { if (q) { (void)hipFree(q); q = nullptr; } };
 	fr(x); fr(r); fr(p); fr(q); fr(z); fr(t); fr(b2); fr(pa); fr(pb); fr(ls); fr(box);
 	if (host) { (void)hipHostFree(host); host = nullptr; }
+	if (ready) { (void)hipEventDestroy(ready); ready = nullptr; }
 }
 
 bool LearnedWork::reserve(size_t n, std::string &err)
@@ -249,6 +250,7 @@ bool LearnedWork::reserve(size_t n, std::string &err)
 			&& hipHostMalloc(&host, 16 * sizeof(double)) == hipSuccess;
 		if (!ok) { err = "learned-solver scalar allocation failed"; return false; }
 	}
+	if (!ready && hipEventCreateWithFlags(&ready, hipEventDisableTiming) != hipSuccess) { err = "learned-solver event creation failed"; return false; }
 	if (n > cap) {
 		auto fr = [](auto *&q) { if (q) { (void)hipFree(q); q = nullptr; } };
 		fr(x); fr(r); fr(p); fr(q); fr(z); fr(t); fr(b2);
@@ -260,13 +262,32 @@ bool LearnedWork::reserve(size_t n, std::string &err)
 	return true;
 }
 
-static int readHost(LearnedWork &W, const double *src, int count, hipStream_t s, std::string &err)
+// The host polls an event behind the copy instead of blocking in a stream
+// synchronisation: the solver reads a few scalars every iteration, and a
+// blocking wait wakes the host tens of microseconds after the copy lands,
+// time the GPU spends idle before the next iteration is launched.
+static int readStart(LearnedWork &W, const double *src, int count, hipStream_t s, std::string &err)
 {
-	if (hipMemcpyAsync(W.host, src, count * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+	if (hipMemcpyAsync(W.host, src, count * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess || hipEventRecord(W.ready, s) != hipSuccess) {
 		err = "learned-solver readback failed";
 		return GARM_EHIP;
 	}
 	return GARM_OK;
+}
+
+static int readWait(LearnedWork &W, std::string &err)
+{
+	for (;;) {
+		const hipError_t e = hipEventQuery(W.ready);
+		if (e == hipSuccess) return GARM_OK;
+		if (e != hipErrorNotReady) { err = "learned-solver readback failed"; return GARM_EHIP; }
+	}
+}
+
+static int readHost(LearnedWork &W, const double *src, int count, hipStream_t s, std::string &err)
+{
+	int rc = readStart(W, src, count, s, err);
+	return rc ? rc : readWait(W, err);
 }
 
 // Bounding box of the unknowns' cells (inclusive corners).
@@ -283,6 +304,14 @@ static int unknownBox(const int *mat2grid, int n, const GridDev &cell, LearnedWo
 	return GARM_OK;
 }
 
+// A preconditioner application z = M(r) as an optional head and the rest:
+// the head may be enqueued before the solver knows whether the application
+// is needed (it then only writes scratch the next head overwrites).
+struct Precond {
+	std::function<void()> head, rest;
+	void full() const { if (head) head(); rest(); }
+};
+
 // Flexible CG on the FP64 system with preconditioner `precondition` (writes
 // W.z = M(W.r)).  Stall rule: with `perApplication` every application must
 // halve the residual, otherwise the residual must halve every kStallWindow
@@ -292,10 +321,10 @@ constexpr int kStallWindow = 10;
 
 template <int D>
 static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const garm_pcg_opts &o, Krylov &K,
-	LearnedWork &W, hipStream_t s, garm_pcg_stats *st, std::string &err, const std::function<void()> &first, bool perApplication,
-	int firstBudget, const std::function<void()> &second, const char *what)
+	LearnedWork &W, hipStream_t s, garm_pcg_stats *st, std::string &err, const Precond &first, bool perApplication,
+	int firstBudget, const Precond *second, const char *what)
 {
-	std::function<void()> precondition = first;
+	const Precond *precondition = &first;
 	const int budget = second ? o.max_iterations : firstBudget;
 	int switchedAt = -1;
 	const int nb = lblocks(n);
@@ -315,7 +344,7 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 	if (bb == 0) { st->relative_residual = 0; return GARM_OK; }
 	const double tol = std::max(o.tolerance, 2.220446049250313e-16 / std::sqrt(bb));
 	const double thr = tol * tol * bb;
-	precondition();
+	precondition->full();
 	hipLaunchKernelGGL(kCopyD, g1, b1, 0, s, W.p, (const double *)W.z, n);
 	std::vector<double> hist(1, bb);
 	int it = 0;
@@ -323,7 +352,11 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 		hipLaunchKernelGGL(kFcgSpmv<D>, g1, b1, 0, s, A64, n, (const double *)W.p, W.q, (const double *)W.r, W.pa);
 		hipLaunchKernelGGL(kFcgUpdate, g1, b1, 0, s, n, W.x, W.r, (const double *)W.p, (const double *)W.q, (const double *)W.pa, W.pb, W.ls);
 		hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nb, W.ls + kLsRr);
-		if ((rc = readHost(W, W.ls, 4, s, err))) return rc;
+		if ((rc = readStart(W, W.ls, 4, s, err))) return rc;
+		// the next application's head keeps the GPU busy while the host
+		// reads the residual and decides
+		if (precondition->head) precondition->head();
+		if ((rc = readWait(W, err))) return rc;
 		const double rr = W.host[kLsRr];
 		if (!std::isfinite(rr)) break;
 		hist.push_back(rr);
@@ -338,12 +371,13 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 		if (stalled || (onFirst && it + 1 >= firstBudget)) {
 			if (!onFirst || !second) { it++; break; }
 			// the first preconditioner has stopped paying: the same iteration
-			// continues with the second one
+			// continues with the second one (whole: the head already enqueued
+			// was the first one's)
 			precondition = second;
 			switchedAt = int(hist.size()) - 1;
 			st->fell_back = 1;
-		}
-		precondition();
+			precondition->full();
+		} else precondition->rest();
 		hipLaunchKernelGGL(kFcgDot, g1, b1, 0, s, n, (const double *)W.z, (const double *)W.q, W.pa);
 		hipLaunchKernelGGL(kFcgP, g1, b1, 0, s, n, W.p, (const double *)W.z, (const double *)W.pa, (const double *)W.ls);
 	}
@@ -402,7 +436,8 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 	if (boxVox > U.buf.cap_vox && !U.buf.reserve(boxVox + boxVox / 2, false, err)) return GARM_ENOMEM;
 	if (!U.buf.setDims(B.n[0], B.n[1], B.n[2], err)) return GARM_ENOMEM;
 	// z = M(r): two-level cycle, network on the pre-smoothed residual
-	auto precondition = [&]() {
+	Precond net;
+	net.rest = [&]() {
 		hipLaunchKernelGGL(kJacobi, g1, b1, 0, s, (const double *)A64.invdiag, n, (const double *)W.r, W.z);
 		hipLaunchKernelGGL(kTrueResidual<D>, g1, b1, 0, s, A64, n, (const double *)W.r, (const double *)W.z, W.t, W.pb);
 		hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nb, W.ls + kLsSmooth);
@@ -410,7 +445,8 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 		hipLaunchKernelGGL(kPostSmooth<D>, g1, b1, 0, s, A64, n, (const double *)W.r, (const double *)W.z, W.t);
 		std::swap(W.z, W.t);
 	};
-	std::function<void()> second;
+	Precond mg;
+	const Precond *second = nullptr;
 	if (o.fallback_mode == GARM_PCG_MULTIGRID) {
 		int org[3], dims[3], gn[3];
 		for (int a = 0; a < 3; a++) {
@@ -421,9 +457,11 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 		}
 		if (!M.layout(D, org, dims, gn, n, err)) return GARM_ENOMEM;
 		mgBuild(M, A32, n, mat2grid, s);
-		second = [&]() { mgApply(M, n, W.r, W.z, s); };
+		mg.head = [&]() { mgApplyHead(M, n, W.r, s); };
+		mg.rest = [&]() { mgApplyRest(M, n, W.z, s); };
+		second = &mg;
 	}
-	return flexibleCg<D>(A64, A32, b, n, o, K, W, s, st, err, precondition, true, std::min(o.max_iterations, 400), second, "learned solve");
+	return flexibleCg<D>(A64, A32, b, n, o, K, W, s, st, err, net, true, std::min(o.max_iterations, 400), second, "learned solve");
 }
 
 template <int D>
@@ -442,8 +480,10 @@ static int multigrid(const Rows<double> &A64, const Rows<float> &A32, const doub
 	}
 	if (!M.layout(D, org, dims, gn, n, err)) return GARM_ENOMEM;
 	mgBuild(M, A32, n, mat2grid, s);
-	auto precondition = [&]() { mgApply(M, n, W.r, W.z, s); };
-	return flexibleCg<D>(A64, A32, b, n, o, K, W, s, st, err, precondition, false, o.max_iterations, nullptr, "multigrid solve");
+	Precond mg;
+	mg.head = [&]() { mgApplyHead(M, n, W.r, s); };
+	mg.rest = [&]() { mgApplyRest(M, n, W.z, s); };
+	return flexibleCg<D>(A64, A32, b, n, o, K, W, s, st, err, mg, false, o.max_iterations, nullptr, "multigrid solve");
 }
 
 int solveLearned(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *grid2mat, const int *mat2grid,

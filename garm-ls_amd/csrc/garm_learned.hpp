@@ -17,6 +17,7 @@ struct LearnedWork {
 	double *pa = nullptr, *pb = nullptr, *ls = nullptr;
 	int *box = nullptr;
 	double *host = nullptr;   // pinned
+	hipEvent_t ready = nullptr;   // marks a readback's arrival (the host polls it)
 	size_t cap = 0;
 	~LearnedWork();
 	bool reserve(size_t n, std::string &err);

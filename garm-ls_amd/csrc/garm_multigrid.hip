@@ -26,7 +26,8 @@
 namespace garm {
 
 constexpr int kMgSweeps = 2;            // red-black sweep pairs before and after the coarse correction
-constexpr int kMgSweeps0 = 2;           // ... on the finest level
+constexpr int kMgSweeps0 = 2;           // ... on the finest level (at least 1: the cycle's head is one of them)
+static_assert(kMgSweeps0 >= 1, "the head of a cycle is a level-0 sweep");
 constexpr float kMgProlongScale = 1.9f; // over-correction of the coarse correction (aggregation underestimates it; 1.9 measured best at 256^3)
 constexpr int kMgCoarsestCells = 512;
 constexpr int kMgCoarsestSweeps = 12;
@@ -661,13 +662,21 @@ static void sweep(Multigrid &M, int l, int first, hipStream_t s)
 	std::swap(M.x[l], M.xalt[l]);
 }
 
+// The V-cycle in two parts: the head (load of the residual and the first
+// level-0 sweep) and the rest; together they are one cycle.
 template <int D>
-static void vcycle(Multigrid &M, int n, const double *r, double *z, hipStream_t s)
+static void vcycleHead(Multigrid &M, int n, const double *r, hipStream_t s)
 {
 	hipLaunchKernelGGL(kMgLoad, dim3(mgBlocks(size_t(n))), dim3(kBlock), 0, s, n, (const unsigned *)M.box_of, level(M, 0), r);
+	if (M.tail_first > 0) sweep<D>(M, 0, 0, s);
+}
+
+template <int D>
+static void vcycleRest(Multigrid &M, int n, double *z, hipStream_t s)
+{
 	const int first = M.tail_first;
 	for (int l = 0; l < first; l++) {
-		for (int v = 0; v < (l == 0 ? kMgSweeps0 : kMgSweeps); v++) sweep<D>(M, l, 0, s);
+		for (int v = (l == 0 ? 1 : 0); v < (l == 0 ? kMgSweeps0 : kMgSweeps); v++) sweep<D>(M, l, 0, s);
 		const int *f = M.dims[l];
 		const dim3 gr(unsigned((f[0] + kRestrictX - 1) / kRestrictX), unsigned((f[1] + 1) / 2), unsigned(D == 3 ? (f[2] + 1) / 2 : 1));
 		hipLaunchKernelGGL(kMgRestrict<D>, gr, dim3(kRestrictX, 2, D == 3 ? 2 : 1), 0, s, level(M, l), level(M, l + 1));
@@ -682,10 +691,22 @@ static void vcycle(Multigrid &M, int n, const double *r, double *z, hipStream_t 
 	hipLaunchKernelGGL(kMgStore, dim3(mgBlocks(size_t(n))), dim3(kBlock), 0, s, n, (const unsigned *)M.box_of, level(M, 0), z);
 }
 
+void mgApplyHead(Multigrid &M, int n, const double *r, hipStream_t s)
+{
+	if (M.D == 2) vcycleHead<2>(M, n, r, s);
+	else vcycleHead<3>(M, n, r, s);
+}
+
+void mgApplyRest(Multigrid &M, int n, double *z, hipStream_t s)
+{
+	if (M.D == 2) vcycleRest<2>(M, n, z, s);
+	else vcycleRest<3>(M, n, z, s);
+}
+
 void mgApply(Multigrid &M, int n, const double *r, double *z, hipStream_t s)
 {
-	if (M.D == 2) vcycle<2>(M, n, r, z, s);
-	else vcycle<3>(M, n, r, z, s);
+	mgApplyHead(M, n, r, s);
+	mgApplyRest(M, n, z, s);
 }
 
 } // namespace garm

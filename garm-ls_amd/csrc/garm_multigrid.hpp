@@ -44,5 +44,11 @@ void mgBuild(Multigrid &M, const Rows<float> &A, int n, const int *mat2grid, hip
 // z = V(r): one symmetric V-cycle (red-black Gauss-Seidel, two sweeps down
 // and up) applied to the compact residual r; z is compact, FP64.
 void mgApply(Multigrid &M, int n, const double *r, double *z, hipStream_t s);
+// The same cycle in two parts (the residual's load and the first level-0
+// sweep, then the rest), so a caller can enqueue the head before it knows
+// whether the cycle is needed; a head without its rest leaves nothing that
+// the next head does not overwrite.
+void mgApplyHead(Multigrid &M, int n, const double *r, hipStream_t s);
+void mgApplyRest(Multigrid &M, int n, double *z, hipStream_t s);
 
 } // namespace garm
