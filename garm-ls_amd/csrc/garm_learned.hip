@@ -16,7 +16,8 @@
 // residual stalls (an application that does not halve it) or the iteration
 // budget runs out, the solve falls back to the mixed-precision Jacobi PCG on
 // the current residual; the FP64 residual b - A x judges the result either
-// way.
+// way.  The Krylov vectors live on the bounding box of the unknowns (the
+// multigrid's level-0 layout) with the FP64 operator on the same box.
 #include "garm_learned.hpp"
 #include "garm_multigrid.hpp"
 
@@ -74,48 +75,6 @@ enum { kLsPq = 0, kLsRr = 1, kLsBb = 2, kLsTrue = 3, kLsSmooth = 4 };
 constexpr double kOmega = 2.0 / 3.0;
 // each network application must reduce the residual norm at least this much
 constexpr double kMinReduction = 0.5;
-
-// z = w D^-1 r
-__global__ void kJacobi(const double *invdiag, int n, const double *r, double *z)
-{
-	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) z[i] = kOmega * invdiag[i] * r[i];
-}
-
-// out = z + w D^-1 (r - A z)
-template <int D>
-__global__ __launch_bounds__(kBlock) void kPostSmooth(Rows<double> A, int n, const double *r, const double *z, double *out)
-{
-	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-		const double zi = z[i];
-		double s = A.diag[i] * zi;
-#pragma unroll
-		for (int o = 0; o < 2 * D; o++) {
-			const int c = A.nb[o][i];
-			if (c >= 0) s += -A.w[o][i] * z[c];
-		}
-		out[i] = zi + kOmega * A.invdiag[i] * (r[i] - s);
-	}
-}
-
-// q = A p; partials of p.q and r.p
-template <int D>
-__global__ __launch_bounds__(kBlock) void kFcgSpmv(Rows<double> A, int n, const double *p, double *q, const double *r, double *pa)
-{
-	double v[2] = {0, 0};
-	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-		const double pi = p[i];
-		double s = A.diag[i] * pi;
-#pragma unroll
-		for (int o = 0; o < 2 * D; o++) {
-			const int c = A.nb[o][i];
-			if (c >= 0) s += -A.w[o][i] * p[c];
-		}
-		q[i] = s;
-		v[0] += pi * s;
-		v[1] += r[i] * pi;
-	}
-	putPart<2>(v, pa);
-}
 
 // alpha = (r.p) / (p.q); x += alpha p; r -= alpha q; partial |r|^2
 __global__ __launch_bounds__(kBlock) void kFcgUpdate(int n, double *x, double *r, const double *p, const double *q, const double *pa, double *pb, double *ls)
@@ -232,6 +191,113 @@ __global__ void kBBoxInit(int *box)
 	else if (threadIdx.x < 6) box[threadIdx.x] = INT_MIN;
 }
 
+// ---------------------------------------------------------- box layout
+// The flexible solves keep their Krylov vectors on the bounding box of the
+// unknowns (cell order (k ny + j) nx + i, the multigrid's level-0 layout,
+// zero outside the unknowns) with the FP64 operator laid out on the same
+// box: diagonal, inverse diagonal and the weights of each cell's -x, -y, -z
+// faces.  A product then reads its neighbours at fixed offsets instead of
+// through six index arrays.  A face on the box boundary has weight 0, so a
+// missing neighbour contributes 0 x (a finite value); past either end of the
+// box the neighbour reads as 0.
+struct BoxOp {
+	int n, sy, sz;               // cells, row and plane strides
+	const double *diag, *idiag, *wx, *wy, *wz;
+};
+
+// box cell of every unknown, for a box with origin org and sizes bn
+__global__ void kBoxIndex(const int *mat2grid, int n, int gx, int gy, int ox, int oy, int oz, int bx, int by, unsigned *boxOf)
+{
+	for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+		const int g = mat2grid[r];
+		const int i = g % gx - ox, j = (g / gx) % gy - oy, k = g / (gx * gy) - oz;
+		boxOf[r] = (unsigned(k) * unsigned(by) + unsigned(j)) * unsigned(bx) + unsigned(i);
+	}
+}
+
+template <int D>
+__global__ void kBoxOpBuild(Rows<double> A, int n, const unsigned *boxOf, double *diag, double *idiag, double *wx, double *wy, double *wz)
+{
+	for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+		const unsigned c = boxOf[r];
+		diag[c] = A.diag[r];
+		idiag[c] = A.invdiag[r];
+		wx[c] = A.nb[0][r] >= 0 ? A.w[0][r] : 0.0;
+		wy[c] = A.nb[2][r] >= 0 ? A.w[2][r] : 0.0;
+		if constexpr (D == 3) wz[c] = A.nb[4][r] >= 0 ? A.w[4][r] : 0.0;
+	}
+}
+
+__global__ void kScatterToBox(int n, const unsigned *boxOf, const double *src, double *dst)
+{
+	for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) dst[boxOf[r]] = src[r];
+}
+
+__global__ void kGatherFromBox(int n, const unsigned *boxOf, const double *src, double *dst)
+{
+	for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) dst[r] = src[boxOf[r]];
+}
+
+// (A p)_c on the box: diag p_c minus the face terms, in the compact rows'
+// order -x, +x, -y, +y, -z, +z
+template <int D>
+__device__ __forceinline__ double boxApply(const BoxOp &O, const double *p, int c)
+{
+	const int last = O.n - 1;
+	auto P = [&](int e) { return e >= 0 && e <= last ? p[e] : 0.0; };
+	auto Wp = [&](const double *w, int e) { return e <= last ? w[e] : 0.0; };
+	double s = O.diag[c] * p[c];
+	s += -O.wx[c] * P(c - 1);
+	s += -Wp(O.wx, c + 1) * P(c + 1);
+	s += -O.wy[c] * P(c - O.sy);
+	s += -Wp(O.wy, c + O.sy) * P(c + O.sy);
+	if constexpr (D == 3) {
+		s += -O.wz[c] * P(c - O.sz);
+		s += -Wp(O.wz, c + O.sz) * P(c + O.sz);
+	}
+	return s;
+}
+
+// q = A p; partials of p.q and r.p
+template <int D>
+__global__ __launch_bounds__(kBlock) void kBoxSpmv(BoxOp O, const double *p, double *q, const double *r, double *pa)
+{
+	double v[2] = {0, 0};
+	for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < O.n; c += gridDim.x * blockDim.x) {
+		const double s = boxApply<D>(O, p, c);
+		q[c] = s;
+		v[0] += p[c] * s;
+		v[1] += r[c] * p[c];
+	}
+	putPart<2>(v, pa);
+}
+
+// rt = b - A x, partial |rt|^2
+template <int D>
+__global__ __launch_bounds__(kBlock) void kBoxResidual(BoxOp O, const double *b, const double *x, double *rt, double *pb)
+{
+	double v[1] = {0};
+	for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < O.n; c += gridDim.x * blockDim.x) {
+		const double ri = b[c] - boxApply<D>(O, x, c);
+		rt[c] = ri;
+		v[0] += ri * ri;
+	}
+	putPart<1>(v, pb);
+}
+
+__global__ void kBoxJacobi(BoxOp O, const double *r, double *z)
+{
+	for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < O.n; c += gridDim.x * blockDim.x) z[c] = kOmega * O.idiag[c] * r[c];
+}
+
+// out = z + w D^-1 (r - A z)
+template <int D>
+__global__ __launch_bounds__(kBlock) void kBoxPostSmooth(BoxOp O, const double *r, const double *z, double *out)
+{
+	for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < O.n; c += gridDim.x * blockDim.x)
+		out[c] = z[c] + kOmega * O.idiag[c] * (r[c] - boxApply<D>(O, z, c));
+}
+
 // ------------------------------------------------------------------ work
 LearnedWork::~LearnedWork()
 {
@@ -239,6 +305,28 @@ LearnedWork::~LearnedWork()
 	fr(x); fr(r); fr(p); fr(q); fr(z); fr(t); fr(b2); fr(pa); fr(pb); fr(ls); fr(box);
 	if (host) { (void)hipHostFree(host); host = nullptr; }
 	if (ready) { (void)hipEventDestroy(ready); ready = nullptr; }
+	if (bpool) { (void)hipFree(bpool); bpool = nullptr; }
+	if (boxOf) { (void)hipFree(boxOf); boxOf = nullptr; }
+}
+
+bool LearnedWork::reserveBox(size_t cells, size_t unknowns, std::string &err)
+{
+	const size_t need = 11 * cells;
+	if (need > bcap) {
+		if (bpool) (void)hipFree(bpool);
+		bpool = nullptr;
+		const size_t want = need + need / 4;
+		if (hipMalloc(&bpool, want * sizeof(double)) != hipSuccess) { err = "flexible-solver box allocation failed"; bcap = 0; return false; }
+		bcap = want;
+	}
+	if (unknowns > ocap) {
+		if (boxOf) (void)hipFree(boxOf);
+		boxOf = nullptr;
+		const size_t want = unknowns + unknowns / 4;
+		if (hipMalloc(&boxOf, want * sizeof(unsigned)) != hipSuccess) { err = "flexible-solver box allocation failed"; ocap = 0; return false; }
+		ocap = want;
+	}
+	return true;
 }
 
 bool LearnedWork::reserve(size_t n, std::string &err)
@@ -312,27 +400,69 @@ struct Precond {
 	void full() const { if (head) head(); rest(); }
 };
 
-// Flexible CG on the FP64 system with preconditioner `precondition` (writes
-// W.z = M(W.r)).  Stall rule: with `perApplication` every application must
-// halve the residual, otherwise the residual must halve every kStallWindow
-// iterations.  After the loop the FP64 residual b - A x judges the result and
-// what is left goes to the mixed-precision Jacobi solver.
+// The Krylov vectors of a flexible solve, on the box of the unknowns.
+struct BoxSpace {
+	BoxOp O;
+	double *x, *r, *p, *q, *z, *t;
+	const unsigned *boxOf;   // box cell of every unknown
+};
+
+// Lays out the box of the unknowns (origin lo, sizes hi - lo + 1) in W:
+// the operator from the FP64 rows and zeroed vectors.
+template <int D>
+static int boxSpace(const Rows<double> &A64, int n, const int *mat2grid, const GridDev &cell, const int lo[3], const int hi[3],
+	LearnedWork &W, hipStream_t s, BoxSpace &V, std::string &err)
+{
+	int bn[3];
+	for (int a = 0; a < 3; a++) bn[a] = a < D ? hi[a] - lo[a] + 1 : 1;
+	const size_t cells = size_t(bn[0]) * bn[1] * bn[2];
+	if (cells > size_t(INT_MAX)) { err = "flexible-solver box too large"; return GARM_EINVAL; }
+	if (!W.reserveBox(cells, size_t(n), err)) return GARM_ENOMEM;
+	const int N = int(cells);
+	double *q = W.bpool;
+	double *arr[11];
+	for (double *&e : arr) { e = q; q += cells; }
+	V.O.n = N;
+	V.O.sy = bn[0];
+	V.O.sz = bn[0] * bn[1];
+	V.O.diag = arr[0]; V.O.idiag = arr[1]; V.O.wx = arr[2]; V.O.wy = arr[3]; V.O.wz = arr[4];
+	V.x = arr[5]; V.r = arr[6]; V.p = arr[7]; V.q = arr[8]; V.z = arr[9]; V.t = arr[10];
+	V.boxOf = W.boxOf;
+	// the operator is zero outside the unknowns; the vectors are written
+	// whole before they are read
+	const size_t opWords = 5 * cells;
+	if (opWords > size_t(INT_MAX)) { err = "flexible-solver box too large"; return GARM_EINVAL; }
+	const dim3 gu(lblocks(n)), b1(kBlock);
+	hipLaunchKernelGGL(kZeroD, dim3(unsigned(std::min<size_t>((opWords + kBlock - 1) / kBlock, 8192))), b1, 0, s, W.bpool, int(opWords));
+	hipLaunchKernelGGL(kBoxIndex, gu, b1, 0, s, mat2grid, n, cell.n[0], cell.n[1], lo[0], lo[1], D == 3 ? lo[2] : 0, bn[0], bn[1], W.boxOf);
+	hipLaunchKernelGGL(kBoxOpBuild<D>, gu, b1, 0, s, A64, n, (const unsigned *)W.boxOf, arr[0], arr[1], arr[2], arr[3], arr[4]);
+	return GARM_OK;
+}
+
+// Flexible CG on the FP64 system, on the box space V, with preconditioner
+// `precondition` (writes V.z = M(V.r)).  Stall rule: with `perApplication`
+// every application must halve the residual, otherwise the residual must
+// halve every kStallWindow iterations.  After the loop the solution goes to
+// the compact W.x, the FP64 residual b - A x of the compact rows judges it,
+// and what is left goes to the mixed-precision Jacobi solver.
 constexpr int kStallWindow = 10;
 
 template <int D>
-static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const garm_pcg_opts &o, Krylov &K,
-	LearnedWork &W, hipStream_t s, garm_pcg_stats *st, std::string &err, const Precond &first, bool perApplication,
+static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, BoxSpace &V, const garm_pcg_opts &o,
+	Krylov &K, LearnedWork &W, hipStream_t s, garm_pcg_stats *st, std::string &err, const Precond &first, bool perApplication,
 	int firstBudget, const Precond *second, const char *what)
 {
 	const Precond *precondition = &first;
 	const int budget = second ? o.max_iterations : firstBudget;
 	int switchedAt = -1;
-	const int nb = lblocks(n);
-	const dim3 g1(nb), b1(kBlock);
+	const int N = V.O.n;
+	const int nb = lblocks(n), nbb = lblocks(N);
+	const dim3 g1(nb), gb(nbb), b1(kBlock);
 	int rc;
-	// x = 0, r = b, |b|^2
-	hipLaunchKernelGGL(kZeroD, g1, b1, 0, s, W.x, n);
-	hipLaunchKernelGGL(kCopyD, g1, b1, 0, s, W.r, b, n);
+	// x = 0, r = b (the box vectors were zeroed with the box), |b|^2
+	hipLaunchKernelGGL(kZeroD, gb, b1, 0, s, V.x, N);
+	hipLaunchKernelGGL(kZeroD, gb, b1, 0, s, V.r, N);
+	hipLaunchKernelGGL(kScatterToBox, g1, b1, 0, s, n, V.boxOf, b, V.r);
 	hipLaunchKernelGGL(kFcgDot, g1, b1, 0, s, n, b, b, W.pb);
 	hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nb, W.ls + kLsBb);
 	if ((rc = readHost(W, W.ls, 4, s, err))) return rc;
@@ -341,17 +471,21 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 	st->outer_iterations = 0;
 	st->fell_back = 0;
 	st->preconditioner_applications = 0;
-	if (bb == 0) { st->relative_residual = 0; return GARM_OK; }
+	if (bb == 0) {
+		hipLaunchKernelGGL(kZeroD, g1, b1, 0, s, W.x, n);
+		st->relative_residual = 0;
+		return GARM_OK;
+	}
 	const double tol = std::max(o.tolerance, 2.220446049250313e-16 / std::sqrt(bb));
 	const double thr = tol * tol * bb;
 	precondition->full();
-	hipLaunchKernelGGL(kCopyD, g1, b1, 0, s, W.p, (const double *)W.z, n);
+	hipLaunchKernelGGL(kCopyD, gb, b1, 0, s, V.p, (const double *)V.z, N);
 	std::vector<double> hist(1, bb);
 	int it = 0;
 	for (; it < budget; it++) {
-		hipLaunchKernelGGL(kFcgSpmv<D>, g1, b1, 0, s, A64, n, (const double *)W.p, W.q, (const double *)W.r, W.pa);
-		hipLaunchKernelGGL(kFcgUpdate, g1, b1, 0, s, n, W.x, W.r, (const double *)W.p, (const double *)W.q, (const double *)W.pa, W.pb, W.ls);
-		hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nb, W.ls + kLsRr);
+		hipLaunchKernelGGL(kBoxSpmv<D>, gb, b1, 0, s, V.O, (const double *)V.p, V.q, (const double *)V.r, W.pa);
+		hipLaunchKernelGGL(kFcgUpdate, gb, b1, 0, s, N, V.x, V.r, (const double *)V.p, (const double *)V.q, (const double *)W.pa, W.pb, W.ls);
+		hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nbb, W.ls + kLsRr);
 		if ((rc = readStart(W, W.ls, 4, s, err))) return rc;
 		// the next application's head keeps the GPU busy while the host
 		// reads the residual and decides
@@ -378,12 +512,14 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 			st->fell_back = 1;
 			precondition->full();
 		} else precondition->rest();
-		hipLaunchKernelGGL(kFcgDot, g1, b1, 0, s, n, (const double *)W.z, (const double *)W.q, W.pa);
-		hipLaunchKernelGGL(kFcgP, g1, b1, 0, s, n, W.p, (const double *)W.z, (const double *)W.pa, (const double *)W.ls);
+		hipLaunchKernelGGL(kFcgDot, gb, b1, 0, s, N, (const double *)V.z, (const double *)V.q, W.pa);
+		hipLaunchKernelGGL(kFcgP, gb, b1, 0, s, N, V.p, (const double *)V.z, (const double *)W.pa, (const double *)W.ls);
 	}
 	st->iterations = it;
 	st->preconditioner_applications = switchedAt < 0 ? it : switchedAt;
-	// the FP64 residual judges the result; fall back to plain PCG on what is left
+	// the solution to the compact unknowns; the FP64 residual judges it, and
+	// plain PCG takes what is left
+	hipLaunchKernelGGL(kGatherFromBox, g1, b1, 0, s, n, V.boxOf, (const double *)V.x, W.x);
 	if (!std::isfinite(W.host[kLsRr])) hipLaunchKernelGGL(kZeroD, g1, b1, 0, s, W.x, n);
 	hipLaunchKernelGGL(kTrueResidual<D>, g1, b1, 0, s, A64, n, b, (const double *)W.x, W.b2, W.pb);
 	hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nb, W.ls + kLsTrue);
@@ -417,8 +553,7 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 	std::string &err)
 {
 	if (!W.reserve(size_t(n), err)) return GARM_ENOMEM;
-	const int nb = lblocks(n);
-	const dim3 g1(nb), b1(kBlock);
+	const dim3 b1(kBlock);
 	int rc, lo[3], hi[3];
 	if ((rc = unknownBox(mat2grid, n, cell, W, s, lo, hi, err))) return rc;
 	// network box: bounding box of the unknowns, two cells of context, multiples of four
@@ -435,15 +570,24 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 	// inference only (no training gradients) and with half again as much room
 	if (boxVox > U.buf.cap_vox && !U.buf.reserve(boxVox + boxVox / 2, false, err)) return GARM_ENOMEM;
 	if (!U.buf.setDims(B.n[0], B.n[1], B.n[2], err)) return GARM_ENOMEM;
+	BoxSpace V{};
+	if ((rc = boxSpace<D>(A64, n, mat2grid, cell, lo, hi, W, s, V, err))) return rc;
+	VecBox VB{};
+	for (int a = 0; a < 3; a++) {
+		VB.org[a] = a < D ? lo[a] : 0;
+		VB.n[a] = a < D ? hi[a] - lo[a] + 1 : 1;
+	}
+	const dim3 gb(lblocks(V.O.n));
+	const int nbb = lblocks(V.O.n);
 	// z = M(r): two-level cycle, network on the pre-smoothed residual
 	Precond net;
 	net.rest = [&]() {
-		hipLaunchKernelGGL(kJacobi, g1, b1, 0, s, (const double *)A64.invdiag, n, (const double *)W.r, W.z);
-		hipLaunchKernelGGL(kTrueResidual<D>, g1, b1, 0, s, A64, n, (const double *)W.r, (const double *)W.z, W.t, W.pb);
-		hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nb, W.ls + kLsSmooth);
-		unetApplyCompact(U, B, grid2mat, W.t, W.ls + kLsSmooth, n, W.z, s);
-		hipLaunchKernelGGL(kPostSmooth<D>, g1, b1, 0, s, A64, n, (const double *)W.r, (const double *)W.z, W.t);
-		std::swap(W.z, W.t);
+		hipLaunchKernelGGL(kBoxJacobi, gb, b1, 0, s, V.O, (const double *)V.r, V.z);
+		hipLaunchKernelGGL(kBoxResidual<D>, gb, b1, 0, s, V.O, (const double *)V.r, (const double *)V.z, V.t, W.pb);
+		hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nbb, W.ls + kLsSmooth);
+		unetApplyVecBox(U, B, VB, grid2mat, V.t, W.ls + kLsSmooth, n, V.z, s);
+		hipLaunchKernelGGL(kBoxPostSmooth<D>, gb, b1, 0, s, V.O, (const double *)V.r, (const double *)V.z, V.t);
+		std::swap(V.z, V.t);
 	};
 	Precond mg;
 	const Precond *second = nullptr;
@@ -457,11 +601,11 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 		}
 		if (!M.layout(D, org, dims, gn, n, err)) return GARM_ENOMEM;
 		mgBuild(M, A32, n, mat2grid, s);
-		mg.head = [&]() { mgApplyHead(M, n, W.r, s); };
-		mg.rest = [&]() { mgApplyRest(M, n, W.z, s); };
+		mg.head = [&]() { mgApplyHead(M, n, V.r, s, true); };
+		mg.rest = [&]() { mgApplyRest(M, n, V.z, s, true); };
 		second = &mg;
 	}
-	return flexibleCg<D>(A64, A32, b, n, o, K, W, s, st, err, net, true, std::min(o.max_iterations, 400), second, "learned solve");
+	return flexibleCg<D>(A64, A32, b, n, V, o, K, W, s, st, err, net, true, std::min(o.max_iterations, 400), second, "learned solve");
 }
 
 template <int D>
@@ -480,10 +624,12 @@ static int multigrid(const Rows<double> &A64, const Rows<float> &A32, const doub
 	}
 	if (!M.layout(D, org, dims, gn, n, err)) return GARM_ENOMEM;
 	mgBuild(M, A32, n, mat2grid, s);
+	BoxSpace V{};
+	if ((rc = boxSpace<D>(A64, n, mat2grid, cell, lo, hi, W, s, V, err))) return rc;
 	Precond mg;
-	mg.head = [&]() { mgApplyHead(M, n, W.r, s); };
-	mg.rest = [&]() { mgApplyRest(M, n, W.z, s); };
-	return flexibleCg<D>(A64, A32, b, n, o, K, W, s, st, err, mg, false, o.max_iterations, nullptr, "multigrid solve");
+	mg.head = [&]() { mgApplyHead(M, n, V.r, s, true); };
+	mg.rest = [&]() { mgApplyRest(M, n, V.z, s, true); };
+	return flexibleCg<D>(A64, A32, b, n, V, o, K, W, s, st, err, mg, false, o.max_iterations, nullptr, "multigrid solve");
 }
 
 int solveLearned(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *grid2mat, const int *mat2grid,

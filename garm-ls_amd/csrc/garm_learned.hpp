@@ -19,8 +19,15 @@ struct LearnedWork {
 	double *host = nullptr;   // pinned
 	hipEvent_t ready = nullptr;   // marks a readback's arrival (the host polls it)
 	size_t cap = 0;
+	// the flexible solves' box layout: operator (diag, idiag, wx, wy, wz) and
+	// Krylov vectors (x, r, p, q, z, t) over the box's cells, box cell per unknown
+	double *bpool = nullptr;
+	size_t bcap = 0;              // doubles
+	unsigned *boxOf = nullptr;
+	size_t ocap = 0;
 	~LearnedWork();
 	bool reserve(size_t n, std::string &err);
+	bool reserveBox(size_t cells, size_t unknowns, std::string &err);
 };
 
 // Solves A x = b; the solution is left in W.x.  grid2mat / mat2grid map the

@@ -535,6 +535,20 @@ __global__ void kMgStore(int n, const unsigned *boxOf, MgLevel L, double *z)
 	for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) z[q] = double(L.x[boxOf[q]]);
 }
 
+// the same for residual and result laid out on level 0's box itself
+__global__ void kMgLoadBox(int cells, MgLevel L, const double *r)
+{
+	for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += gridDim.x * blockDim.x) {
+		L.b[c] = float(r[c]);
+		L.x[c] = 0.0f;
+	}
+}
+
+__global__ void kMgStoreBox(int cells, MgLevel L, double *z)
+{
+	for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += gridDim.x * blockDim.x) z[c] = double(L.x[c]);
+}
+
 // ------------------------------------------------------------------ host
 static unsigned mgBlocks(size_t n) { return unsigned(std::max<size_t>(1, std::min<size_t>((n + kBlock - 1) / kBlock, 4096))); }
 
@@ -665,14 +679,15 @@ static void sweep(Multigrid &M, int l, int first, hipStream_t s)
 // The V-cycle in two parts: the head (load of the residual and the first
 // level-0 sweep) and the rest; together they are one cycle.
 template <int D>
-static void vcycleHead(Multigrid &M, int n, const double *r, hipStream_t s)
+static void vcycleHead(Multigrid &M, int n, const double *r, bool box, hipStream_t s)
 {
-	hipLaunchKernelGGL(kMgLoad, dim3(mgBlocks(size_t(n))), dim3(kBlock), 0, s, n, (const unsigned *)M.box_of, level(M, 0), r);
+	if (box) hipLaunchKernelGGL(kMgLoadBox, dim3(mgBlocks(M.cells[0])), dim3(kBlock), 0, s, int(M.cells[0]), level(M, 0), r);
+	else hipLaunchKernelGGL(kMgLoad, dim3(mgBlocks(size_t(n))), dim3(kBlock), 0, s, n, (const unsigned *)M.box_of, level(M, 0), r);
 	if (M.tail_first > 0) sweep<D>(M, 0, 0, s);
 }
 
 template <int D>
-static void vcycleRest(Multigrid &M, int n, double *z, hipStream_t s)
+static void vcycleRest(Multigrid &M, int n, double *z, bool box, hipStream_t s)
 {
 	const int first = M.tail_first;
 	for (int l = 0; l < first; l++) {
@@ -688,19 +703,20 @@ static void vcycleRest(Multigrid &M, int n, double *z, hipStream_t s)
 		hipLaunchKernelGGL(kMgProlong<D>, gp.grid, gp.block, 0, s, level(M, l), level(M, l + 1));
 		for (int v = 0; v < (l == 0 ? kMgSweeps0 : kMgSweeps); v++) sweep<D>(M, l, 1, s);
 	}
-	hipLaunchKernelGGL(kMgStore, dim3(mgBlocks(size_t(n))), dim3(kBlock), 0, s, n, (const unsigned *)M.box_of, level(M, 0), z);
+	if (box) hipLaunchKernelGGL(kMgStoreBox, dim3(mgBlocks(M.cells[0])), dim3(kBlock), 0, s, int(M.cells[0]), level(M, 0), z);
+	else hipLaunchKernelGGL(kMgStore, dim3(mgBlocks(size_t(n))), dim3(kBlock), 0, s, n, (const unsigned *)M.box_of, level(M, 0), z);
 }
 
-void mgApplyHead(Multigrid &M, int n, const double *r, hipStream_t s)
+void mgApplyHead(Multigrid &M, int n, const double *r, hipStream_t s, bool box)
 {
-	if (M.D == 2) vcycleHead<2>(M, n, r, s);
-	else vcycleHead<3>(M, n, r, s);
+	if (M.D == 2) vcycleHead<2>(M, n, r, box, s);
+	else vcycleHead<3>(M, n, r, box, s);
 }
 
-void mgApplyRest(Multigrid &M, int n, double *z, hipStream_t s)
+void mgApplyRest(Multigrid &M, int n, double *z, hipStream_t s, bool box)
 {
-	if (M.D == 2) vcycleRest<2>(M, n, z, s);
-	else vcycleRest<3>(M, n, z, s);
+	if (M.D == 2) vcycleRest<2>(M, n, z, box, s);
+	else vcycleRest<3>(M, n, z, box, s);
 }
 
 void mgApply(Multigrid &M, int n, const double *r, double *z, hipStream_t s)

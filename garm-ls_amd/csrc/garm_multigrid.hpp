@@ -47,8 +47,9 @@ void mgApply(Multigrid &M, int n, const double *r, double *z, hipStream_t s);
 // The same cycle in two parts (the residual's load and the first level-0
 // sweep, then the rest), so a caller can enqueue the head before it knows
 // whether the cycle is needed; a head without its rest leaves nothing that
-// the next head does not overwrite.
-void mgApplyHead(Multigrid &M, int n, const double *r, hipStream_t s);
-void mgApplyRest(Multigrid &M, int n, double *z, hipStream_t s);
+// the next head does not overwrite.  With `box`, r and z are laid out on
+// level 0's box (its cells, zero outside the unknowns) instead of compact.
+void mgApplyHead(Multigrid &M, int n, const double *r, hipStream_t s, bool box = false);
+void mgApplyRest(Multigrid &M, int n, double *z, hipStream_t s, bool box = false);
 
 } // namespace garm

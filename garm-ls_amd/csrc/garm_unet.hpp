@@ -101,15 +101,21 @@ void unetPackWeights(Unet &U, hipStream_t s);
 // `box` selects the sub-block of a larger grid (offset, size).
 void unetLoadInput(Unet &U, const float *r, const uint8_t *mask, hipStream_t s);
 
-// Preconditioner application over a sub-box of a cell grid: gathers the
-// compact residual r (n unknowns, |r|^2 in device memory at rr) into the box
-// scaled to unit RMS, runs the network and adds the output, scaled back, to
-// the compact vector z.
+// Preconditioner application over a sub-box B of a cell grid: gathers the
+// residual r (n unknowns, |r|^2 in device memory at rr) into B scaled to unit
+// RMS, runs the network and adds the output, scaled back, to z.  r and z are
+// laid out on a box V of the cell grid that holds every unknown (cell order
+// (k ny + j) nx + i over V's sizes); grid2mat tells the unknowns.
 struct UnetBox {
 	int org[3];              // box origin in cell coordinates
 	int n[3];                // box size (multiples of 4)
 	int gn[3];               // full cell-grid size
 };
-void unetApplyCompact(Unet &U, const UnetBox &B, const int *grid2mat, const double *r, const double *rr, int n, double *z, hipStream_t s);
+struct VecBox {
+	int org[3];
+	int n[3];
+};
+void unetApplyVecBox(Unet &U, const UnetBox &B, const VecBox &V, const int *grid2mat, const double *r, const double *rr, int n, double *z,
+	hipStream_t s);
 
 } // namespace garm
