@@ -16,8 +16,9 @@
 // residual stalls (an application that does not halve it) or the iteration
 // budget runs out, the solve falls back to the mixed-precision Jacobi PCG on
 // the current residual; the FP64 residual b - A x judges the result either
-// way.  The Krylov vectors live on the bounding box of the unknowns (the
-// multigrid's level-0 layout) with the FP64 operator on the same box.
+// way.  While the bounding box of the unknowns is not much larger than their
+// count, the Krylov vectors live on that box (the multigrid's level-0
+// layout) with the FP64 operator on the same box; otherwise they are compact.
 #include "garm_learned.hpp"
 #include "garm_multigrid.hpp"
 
@@ -75,6 +76,48 @@ enum { kLsPq = 0, kLsRr = 1, kLsBb = 2, kLsTrue = 3, kLsSmooth = 4 };
 constexpr double kOmega = 2.0 / 3.0;
 // each network application must reduce the residual norm at least this much
 constexpr double kMinReduction = 0.5;
+
+// z = w D^-1 r
+__global__ void kJacobi(const double *invdiag, int n, const double *r, double *z)
+{
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) z[i] = kOmega * invdiag[i] * r[i];
+}
+
+// out = z + w D^-1 (r - A z)
+template <int D>
+__global__ __launch_bounds__(kBlock) void kPostSmooth(Rows<double> A, int n, const double *r, const double *z, double *out)
+{
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+		const double zi = z[i];
+		double s = A.diag[i] * zi;
+#pragma unroll
+		for (int o = 0; o < 2 * D; o++) {
+			const int c = A.nb[o][i];
+			if (c >= 0) s += -A.w[o][i] * z[c];
+		}
+		out[i] = zi + kOmega * A.invdiag[i] * (r[i] - s);
+	}
+}
+
+// q = A p; partials of p.q and r.p
+template <int D>
+__global__ __launch_bounds__(kBlock) void kFcgSpmv(Rows<double> A, int n, const double *p, double *q, const double *r, double *pa)
+{
+	double v[2] = {0, 0};
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+		const double pi = p[i];
+		double s = A.diag[i] * pi;
+#pragma unroll
+		for (int o = 0; o < 2 * D; o++) {
+			const int c = A.nb[o][i];
+			if (c >= 0) s += -A.w[o][i] * p[c];
+		}
+		q[i] = s;
+		v[0] += pi * s;
+		v[1] += r[i] * pi;
+	}
+	putPart<2>(v, pa);
+}
 
 // alpha = (r.p) / (p.q); x += alpha p; r -= alpha q; partial |r|^2
 __global__ __launch_bounds__(kBlock) void kFcgUpdate(int n, double *x, double *r, const double *p, const double *q, const double *pa, double *pb, double *ls)
@@ -400,18 +443,35 @@ struct Precond {
 	void full() const { if (head) head(); rest(); }
 };
 
-// The Krylov vectors of a flexible solve, on the box of the unknowns.
-struct BoxSpace {
-	BoxOp O;
-	double *x, *r, *p, *q, *z, *t;
-	const unsigned *boxOf;   // box cell of every unknown
+// The Krylov vectors of a flexible solve and how A acts on them: compact
+// (the unknowns' numbering, the FP64 rows), or on the bounding box of the
+// unknowns (the multigrid's level-0 layout, the FP64 operator on the box),
+// which a solve uses while the box holds at most kBoxMaxRatio cells per
+// unknown: its products read neighbours at fixed offsets instead of through
+// six index arrays, but every vector operation also covers the box's cells
+// that are not unknowns (a splash's bounding box holds mostly air).
+constexpr double kBoxMaxRatio = 1.5;
+
+struct FcgSpace {
+	bool box = false;
+	int nv = 0;                        // vector length
+	double *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *t = nullptr;
+	BoxOp O{};                         // box: the operator
+	const unsigned *boxOf = nullptr;   // box: box cell of every unknown
 };
+
+static void compactSpace(LearnedWork &W, int n, FcgSpace &V)
+{
+	V.box = false;
+	V.nv = n;
+	V.x = W.x; V.r = W.r; V.p = W.p; V.q = W.q; V.z = W.z; V.t = W.t;
+}
 
 // Lays out the box of the unknowns (origin lo, sizes hi - lo + 1) in W:
 // the operator from the FP64 rows and zeroed vectors.
 template <int D>
 static int boxSpace(const Rows<double> &A64, int n, const int *mat2grid, const GridDev &cell, const int lo[3], const int hi[3],
-	LearnedWork &W, hipStream_t s, BoxSpace &V, std::string &err)
+	LearnedWork &W, hipStream_t s, FcgSpace &V, std::string &err)
 {
 	int bn[3];
 	for (int a = 0; a < 3; a++) bn[a] = a < D ? hi[a] - lo[a] + 1 : 1;
@@ -419,6 +479,8 @@ static int boxSpace(const Rows<double> &A64, int n, const int *mat2grid, const G
 	if (cells > size_t(INT_MAX)) { err = "flexible-solver box too large"; return GARM_EINVAL; }
 	if (!W.reserveBox(cells, size_t(n), err)) return GARM_ENOMEM;
 	const int N = int(cells);
+	V.box = true;
+	V.nv = N;
 	double *q = W.bpool;
 	double *arr[11];
 	for (double *&e : arr) { e = q; q += cells; }
@@ -439,7 +501,7 @@ static int boxSpace(const Rows<double> &A64, int n, const int *mat2grid, const G
 	return GARM_OK;
 }
 
-// Flexible CG on the FP64 system, on the box space V, with preconditioner
+// Flexible CG on the FP64 system, on the vectors of V, with preconditioner
 // `precondition` (writes V.z = M(V.r)).  Stall rule: with `perApplication`
 // every application must halve the residual, otherwise the residual must
 // halve every kStallWindow iterations.  After the loop the solution goes to
@@ -448,21 +510,23 @@ static int boxSpace(const Rows<double> &A64, int n, const int *mat2grid, const G
 constexpr int kStallWindow = 10;
 
 template <int D>
-static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, BoxSpace &V, const garm_pcg_opts &o,
+static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, FcgSpace &V, const garm_pcg_opts &o,
 	Krylov &K, LearnedWork &W, hipStream_t s, garm_pcg_stats *st, std::string &err, const Precond &first, bool perApplication,
 	int firstBudget, const Precond *second, const char *what)
 {
 	const Precond *precondition = &first;
 	const int budget = second ? o.max_iterations : firstBudget;
 	int switchedAt = -1;
-	const int N = V.O.n;
+	const int N = V.nv;
 	const int nb = lblocks(n), nbb = lblocks(N);
 	const dim3 g1(nb), gb(nbb), b1(kBlock);
 	int rc;
-	// x = 0, r = b (the box vectors were zeroed with the box), |b|^2
+	// x = 0, r = b, |b|^2
 	hipLaunchKernelGGL(kZeroD, gb, b1, 0, s, V.x, N);
-	hipLaunchKernelGGL(kZeroD, gb, b1, 0, s, V.r, N);
-	hipLaunchKernelGGL(kScatterToBox, g1, b1, 0, s, n, V.boxOf, b, V.r);
+	if (V.box) {
+		hipLaunchKernelGGL(kZeroD, gb, b1, 0, s, V.r, N);
+		hipLaunchKernelGGL(kScatterToBox, g1, b1, 0, s, n, V.boxOf, b, V.r);
+	} else hipLaunchKernelGGL(kCopyD, g1, b1, 0, s, V.r, b, n);
 	hipLaunchKernelGGL(kFcgDot, g1, b1, 0, s, n, b, b, W.pb);
 	hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nb, W.ls + kLsBb);
 	if ((rc = readHost(W, W.ls, 4, s, err))) return rc;
@@ -483,7 +547,8 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 	std::vector<double> hist(1, bb);
 	int it = 0;
 	for (; it < budget; it++) {
-		hipLaunchKernelGGL(kBoxSpmv<D>, gb, b1, 0, s, V.O, (const double *)V.p, V.q, (const double *)V.r, W.pa);
+		if (V.box) hipLaunchKernelGGL(kBoxSpmv<D>, gb, b1, 0, s, V.O, (const double *)V.p, V.q, (const double *)V.r, W.pa);
+		else hipLaunchKernelGGL(kFcgSpmv<D>, gb, b1, 0, s, A64, n, (const double *)V.p, V.q, (const double *)V.r, W.pa);
 		hipLaunchKernelGGL(kFcgUpdate, gb, b1, 0, s, N, V.x, V.r, (const double *)V.p, (const double *)V.q, (const double *)W.pa, W.pb, W.ls);
 		hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nbb, W.ls + kLsRr);
 		if ((rc = readStart(W, W.ls, 4, s, err))) return rc;
@@ -517,9 +582,9 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 	}
 	st->iterations = it;
 	st->preconditioner_applications = switchedAt < 0 ? it : switchedAt;
-	// the solution to the compact unknowns; the FP64 residual judges it, and
+	// the solution in the compact unknowns; the FP64 residual judges it, and
 	// plain PCG takes what is left
-	hipLaunchKernelGGL(kGatherFromBox, g1, b1, 0, s, n, V.boxOf, (const double *)V.x, W.x);
+	if (V.box) hipLaunchKernelGGL(kGatherFromBox, g1, b1, 0, s, n, V.boxOf, (const double *)V.x, W.x);
 	if (!std::isfinite(W.host[kLsRr])) hipLaunchKernelGGL(kZeroD, g1, b1, 0, s, W.x, n);
 	hipLaunchKernelGGL(kTrueResidual<D>, g1, b1, 0, s, A64, n, b, (const double *)W.x, W.b2, W.pb);
 	hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nb, W.ls + kLsTrue);
@@ -570,23 +635,36 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 	// inference only (no training gradients) and with half again as much room
 	if (boxVox > U.buf.cap_vox && !U.buf.reserve(boxVox + boxVox / 2, false, err)) return GARM_ENOMEM;
 	if (!U.buf.setDims(B.n[0], B.n[1], B.n[2], err)) return GARM_ENOMEM;
-	BoxSpace V{};
-	if ((rc = boxSpace<D>(A64, n, mat2grid, cell, lo, hi, W, s, V, err))) return rc;
+	FcgSpace V;
+	const size_t boxCells = size_t(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (D == 3 ? hi[2] - lo[2] + 1 : 1);
+	if (double(boxCells) <= kBoxMaxRatio * n) {
+		if ((rc = boxSpace<D>(A64, n, mat2grid, cell, lo, hi, W, s, V, err))) return rc;
+	} else compactSpace(W, n, V);
 	VecBox VB{};
 	for (int a = 0; a < 3; a++) {
 		VB.org[a] = a < D ? lo[a] : 0;
 		VB.n[a] = a < D ? hi[a] - lo[a] + 1 : 1;
 	}
-	const dim3 gb(lblocks(V.O.n));
-	const int nbb = lblocks(V.O.n);
+	const dim3 gb(lblocks(V.nv));
+	const int nbb = lblocks(V.nv);
 	// z = M(r): two-level cycle, network on the pre-smoothed residual
 	Precond net;
 	net.rest = [&]() {
-		hipLaunchKernelGGL(kBoxJacobi, gb, b1, 0, s, V.O, (const double *)V.r, V.z);
-		hipLaunchKernelGGL(kBoxResidual<D>, gb, b1, 0, s, V.O, (const double *)V.r, (const double *)V.z, V.t, W.pb);
+		if (V.box) {
+			hipLaunchKernelGGL(kBoxJacobi, gb, b1, 0, s, V.O, (const double *)V.r, V.z);
+			hipLaunchKernelGGL(kBoxResidual<D>, gb, b1, 0, s, V.O, (const double *)V.r, (const double *)V.z, V.t, W.pb);
+		} else {
+			hipLaunchKernelGGL(kJacobi, gb, b1, 0, s, (const double *)A64.invdiag, n, (const double *)V.r, V.z);
+			hipLaunchKernelGGL(kTrueResidual<D>, gb, b1, 0, s, A64, n, (const double *)V.r, (const double *)V.z, V.t, W.pb);
+		}
 		hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nbb, W.ls + kLsSmooth);
-		unetApplyVecBox(U, B, VB, grid2mat, V.t, W.ls + kLsSmooth, n, V.z, s);
-		hipLaunchKernelGGL(kBoxPostSmooth<D>, gb, b1, 0, s, V.O, (const double *)V.r, (const double *)V.z, V.t);
+		if (V.box) {
+			unetApplyVecBox(U, B, VB, grid2mat, V.t, W.ls + kLsSmooth, n, V.z, s);
+			hipLaunchKernelGGL(kBoxPostSmooth<D>, gb, b1, 0, s, V.O, (const double *)V.r, (const double *)V.z, V.t);
+		} else {
+			unetApplyCompact(U, B, grid2mat, V.t, W.ls + kLsSmooth, n, V.z, s);
+			hipLaunchKernelGGL(kPostSmooth<D>, gb, b1, 0, s, A64, n, (const double *)V.r, (const double *)V.z, V.t);
+		}
 		std::swap(V.z, V.t);
 	};
 	Precond mg;
@@ -601,8 +679,8 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 		}
 		if (!M.layout(D, org, dims, gn, n, err)) return GARM_ENOMEM;
 		mgBuild(M, A32, n, mat2grid, s);
-		mg.head = [&]() { mgApplyHead(M, n, V.r, s, true); };
-		mg.rest = [&]() { mgApplyRest(M, n, V.z, s, true); };
+		mg.head = [&]() { mgApplyHead(M, n, V.r, s, V.box); };
+		mg.rest = [&]() { mgApplyRest(M, n, V.z, s, V.box); };
 		second = &mg;
 	}
 	return flexibleCg<D>(A64, A32, b, n, V, o, K, W, s, st, err, net, true, std::min(o.max_iterations, 400), second, "learned solve");
@@ -624,11 +702,13 @@ static int multigrid(const Rows<double> &A64, const Rows<float> &A32, const doub
 	}
 	if (!M.layout(D, org, dims, gn, n, err)) return GARM_ENOMEM;
 	mgBuild(M, A32, n, mat2grid, s);
-	BoxSpace V{};
-	if ((rc = boxSpace<D>(A64, n, mat2grid, cell, lo, hi, W, s, V, err))) return rc;
+	FcgSpace V;
+	if (double(M.cells[0]) <= kBoxMaxRatio * n) {
+		if ((rc = boxSpace<D>(A64, n, mat2grid, cell, lo, hi, W, s, V, err))) return rc;
+	} else compactSpace(W, n, V);
 	Precond mg;
-	mg.head = [&]() { mgApplyHead(M, n, V.r, s, true); };
-	mg.rest = [&]() { mgApplyRest(M, n, V.z, s, true); };
+	mg.head = [&]() { mgApplyHead(M, n, V.r, s, V.box); };
+	mg.rest = [&]() { mgApplyRest(M, n, V.z, s, V.box); };
 	return flexibleCg<D>(A64, A32, b, n, V, o, K, W, s, st, err, mg, false, o.max_iterations, nullptr, "multigrid solve");
 }
 

@@ -514,11 +514,44 @@ __global__ void kLoadInput(const float *r, const uint8_t *mask, bf16 *in, size_t
 	}
 }
 
-// Residual on the flexible solve's box V (origin, sizes, cell order
-// (k ny + j) nx + i; it holds every unknown) -> the network's input tensor on
-// its own box B, scaled to unit RMS over the n unknowns (rr = |r|^2 on the
-// device), and the network's output back: z += y / scale.  grid2mat tells
-// the unknowns.
+// compact residual -> box input tensor, scaled to unit RMS over the n
+// unknowns (rr = |r|^2 on the device)
+__global__ void kGatherBox(UnetBox B, const int *grid2mat, const double *r, const double *rr, int n, bf16 *in)
+{
+	const double scale = rr[0] > 0 ? sqrt(double(n) / rr[0]) : 1.0;
+	const size_t vox = size_t(B.n[0]) * B.n[1] * B.n[2];
+	for (size_t v = blockIdx.x * size_t(blockDim.x) + threadIdx.x; v < vox; v += size_t(gridDim.x) * blockDim.x) {
+		int x, y, z;
+		coords(v, B.n[0], B.n[1], x, y, z);
+		const int gx = B.org[0] + x, gy = B.org[1] + y, gz = B.org[2] + z;
+		int m = -1;
+		if (gx >= 0 && gx < B.gn[0] && gy >= 0 && gy < B.gn[1] && gz >= 0 && gz < B.gn[2])
+			m = grid2mat[(size_t(gz) * B.gn[1] + gy) * B.gn[0] + gx];
+		in[v * 4 + 0] = f2bf(m >= 0 ? float(r[m] * scale) : 0.0f);
+		in[v * 4 + 1] = f2bf(m >= 0 ? 1.0f : 0.0f);
+		in[v * 4 + 2] = 0;
+		in[v * 4 + 3] = 0;
+	}
+}
+
+// box output -> compact unknowns: z += y / scale
+__global__ void kScatterBox(UnetBox B, const int *grid2mat, const float *y, const double *rr, int n, double *z)
+{
+	const double inv = rr[0] > 0 ? sqrt(rr[0] / double(n)) : 1.0;
+	const size_t vox = size_t(B.n[0]) * B.n[1] * B.n[2];
+	for (size_t v = blockIdx.x * size_t(blockDim.x) + threadIdx.x; v < vox; v += size_t(gridDim.x) * blockDim.x) {
+		int x, yy, zz;
+		coords(v, B.n[0], B.n[1], x, yy, zz);
+		const int gx = B.org[0] + x, gy = B.org[1] + yy, gz = B.org[2] + zz;
+		if (gx < 0 || gx >= B.gn[0] || gy < 0 || gy >= B.gn[1] || gz < 0 || gz >= B.gn[2]) continue;
+		const int m = grid2mat[(size_t(gz) * B.gn[1] + gy) * B.gn[0] + gx];
+		if (m >= 0) z[m] += double(y[v]) * inv;
+	}
+}
+
+// The same two for r and z laid out on the flexible solve's box V (origin,
+// sizes, cell order (k ny + j) nx + i; it holds every unknown); grid2mat
+// still tells the unknowns.
 __device__ __forceinline__ size_t vecBoxIndex(const VecBox &V, int gx, int gy, int gz)
 {
 	return (size_t(gz - V.org[2]) * V.n[1] + size_t(gy - V.org[1])) * V.n[0] + size_t(gx - V.org[0]);
@@ -909,6 +942,14 @@ void unetAdam(Unet &U, float lr, float b1, float b2, float eps, hipStream_t s)
 void unetLoadInput(Unet &U, const float *r, const uint8_t *mask, hipStream_t s)
 {
 	hipLaunchKernelGGL(kLoadInput, dim3(blocksFor(U.buf.vox[0])), dim3(kUB), 0, s, r, mask, U.buf.in, U.buf.vox[0]);
+}
+
+void unetApplyCompact(Unet &U, const UnetBox &B, const int *grid2mat, const double *r, const double *rr, int n, double *z, hipStream_t s)
+{
+	const size_t vox = size_t(B.n[0]) * B.n[1] * B.n[2];
+	hipLaunchKernelGGL(kGatherBox, dim3(blocksFor(vox)), dim3(kUB), 0, s, B, grid2mat, r, rr, n, U.buf.in);
+	unetForward(U, s);
+	hipLaunchKernelGGL(kScatterBox, dim3(blocksFor(vox)), dim3(kUB), 0, s, B, grid2mat, (const float *)U.buf.y, rr, n, z);
 }
 
 void unetApplyVecBox(Unet &U, const UnetBox &B, const VecBox &V, const int *grid2mat, const double *r, const double *rr, int n, double *z,

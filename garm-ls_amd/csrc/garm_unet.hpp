@@ -104,8 +104,9 @@ void unetLoadInput(Unet &U, const float *r, const uint8_t *mask, hipStream_t s);
 // Preconditioner application over a sub-box B of a cell grid: gathers the
 // residual r (n unknowns, |r|^2 in device memory at rr) into B scaled to unit
 // RMS, runs the network and adds the output, scaled back, to z.  r and z are
-// laid out on a box V of the cell grid that holds every unknown (cell order
-// (k ny + j) nx + i over V's sizes); grid2mat tells the unknowns.
+// compact (unknowns in grid2mat's numbering), or, with unetApplyVecBox, laid
+// out on a box V of the cell grid that holds every unknown (cell order
+// (k ny + j) nx + i over V's sizes; grid2mat tells the unknowns).
 struct UnetBox {
 	int org[3];              // box origin in cell coordinates
 	int n[3];                // box size (multiples of 4)
@@ -115,6 +116,7 @@ struct VecBox {
 	int org[3];
 	int n[3];
 };
+void unetApplyCompact(Unet &U, const UnetBox &B, const int *grid2mat, const double *r, const double *rr, int n, double *z, hipStream_t s);
 void unetApplyVecBox(Unet &U, const UnetBox &B, const VecBox &V, const int *grid2mat, const double *r, const double *rr, int n, double *z,
 	hipStream_t s);
 
