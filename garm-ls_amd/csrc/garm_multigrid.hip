@@ -62,6 +62,11 @@ __global__ void kMgZero(float *p, size_t n)
 	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = 0.0f;
 }
 
+__global__ void kMgZeroBytes(uint8_t *p, size_t n)
+{
+	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = 0;
+}
+
 struct MgBox {
 	int org[3], gn[3];
 };
@@ -193,8 +198,10 @@ __device__ __forceinline__ float relaxTile(const MgRow &r, const float *xs, int 
 // all three phases before its first LDS store, so they are in flight
 // together and the phases after the first barrier wait only on LDS.
 template <int D, int TX, int TY, int TZ, int NTH>
-__global__ __launch_bounds__(NTH) void kMgSweep(MgLevel L, const float *xin, float *xout, int first)
+__global__ __launch_bounds__(NTH) void kMgSweep(MgLevel L, const float *xin, float *xout, int first, const uint8_t *active)
 {
+	// a tile without active cells keeps x = 0 in both buffers: nothing to do
+	if (!active[(size_t(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x]) return;
 	static_assert(TX % 2 == 0 && (D == 3 || TZ == 1), "tile shape");
 	constexpr int HZ = D == 3 ? 2 : 0;                    // halo of the staged tile along z
 	constexpr int SX = TX + 4, SY = TY + 4, SZ = TZ + 2 * HZ, SXY = SX * SY, NS = SXY * SZ;
@@ -269,6 +276,16 @@ __global__ __launch_bounds__(NTH) void kMgSweep(MgLevel L, const float *xin, flo
 		const float nv = upd[r] ? relaxTile<D, SX, SXY>(row2[r], xs, l2[r], v) : v;
 		if (in2[r]) xout[c2[r]] = nv;
 	}
+}
+
+// Marks the sweep tiles (tx x ty x tz cells, ntx x nty per plane of tiles)
+// that hold an active cell (diag > 0); every writer stores the same 1.
+template <int D>
+__global__ __launch_bounds__(kBlock) void kMgTileMask(MgLevel L, int tx, int ty, int tz, int ntx, int nty, uint8_t *mask)
+{
+	const Tid t = tid3();
+	if (t.x >= L.n[0] || t.y >= L.n[1]) return;
+	if (L.diag[cidx(L, t.x, t.y, t.z)] > 0) mask[(size_t(t.z / tz) * nty + t.y / ty) * ntx + t.x / tx] = 1;
 }
 
 // coarse right-hand side = sum of the fine residuals of the aggregate; coarse
@@ -564,9 +581,23 @@ static Launch3 mgLaunch3(size_t nx, int ny, int nz)
 }
 static Launch3 mgLevelLaunch(const Multigrid &M, int l, size_t nx) { return mgLaunch3(nx, M.dims[l][1], M.dims[l][2]); }
 
+// the sweep tile of level l
+static const int *sweepTile(const Multigrid &M, int l)
+{
+	return M.D == 2 ? kTile2 : M.cells[l] >= size_t(1) << 20 ? kTileL : kTileS;
+}
+
+static size_t tileCount(const Multigrid &M, int l)
+{
+	const int *t = sweepTile(M, l), *n = M.dims[l];
+	return size_t((n[0] + t[0] - 1) / t[0]) * ((n[1] + t[1] - 1) / t[1]) * ((n[2] + t[2] - 1) / t[2]);
+}
+
+
 Multigrid::~Multigrid()
 {
 	if (pool) (void)hipFree(pool);
+	if (tile_active) (void)hipFree(tile_active);
 }
 
 bool Multigrid::layout(int dim, const int box_org[3], const int box_n[3], const int grid_n[3], int unknowns, std::string &err)
@@ -612,6 +643,16 @@ bool Multigrid::layout(int dim, const int box_org[3], const int box_n[3], const 
 	tail_list = reinterpret_cast<unsigned *>(p);
 	tail_count = reinterpret_cast<int *>(p + tail_words);
 	box_of = reinterpret_cast<unsigned *>(p + tail_words + 2 * kMgMaxLevels);
+	// sweep-tile activity of the levels above the tail
+	size_t tiles = 0;
+	for (int l = 0; l < tail_first; l++) { tile_off[l] = tiles; tiles += tileCount(*this, l); }
+	if (tiles > tile_cap || !tile_active) {
+		if (tile_active) (void)hipFree(tile_active);
+		tile_active = nullptr;
+		const size_t want = std::max<size_t>(tiles + tiles / 4, 256);
+		if (hipMalloc(&tile_active, want) != hipSuccess) { err = "multigrid allocation failed"; tile_cap = 0; return false; }
+		tile_cap = want;
+	}
 	return true;
 }
 
@@ -658,6 +699,17 @@ void mgBuild(Multigrid &M, const Rows<float> &A, int n, const int *mat2grid, hip
 	const MgTail T = tailDesc(M);
 	if (M.D == 2) hipLaunchKernelGGL(kMgTailLists<2>, dim3(1), dim3(1024), 0, s, T);
 	else hipLaunchKernelGGL(kMgTailLists<3>, dim3(1), dim3(1024), 0, s, T);
+	if (M.tail_first > 0) {
+		const size_t tiles = M.tile_off[M.tail_first - 1] + tileCount(M, M.tail_first - 1);
+		hipLaunchKernelGGL(kMgZeroBytes, dim3(mgBlocks(tiles)), dim3(kBlock), 0, s, M.tile_active, tiles);
+	}
+	for (int l = 0; l < M.tail_first; l++) {
+		const int *t = sweepTile(M, l), *n = M.dims[l];
+		const int ntx = (n[0] + t[0] - 1) / t[0], nty = (n[1] + t[1] - 1) / t[1];
+		const Launch3 g = mgLevelLaunch(M, l, size_t(n[0]));
+		if (M.D == 2) hipLaunchKernelGGL(kMgTileMask<2>, g.grid, g.block, 0, s, level(M, l), t[0], t[1], t[2], ntx, nty, M.tile_active + M.tile_off[l]);
+		else hipLaunchKernelGGL(kMgTileMask<3>, g.grid, g.block, 0, s, level(M, l), t[0], t[1], t[2], ntx, nty, M.tile_active + M.tile_off[l]);
+	}
 }
 
 // one sweep of level l (colour `first` first) from x into xalt; the two swap names
@@ -666,9 +718,10 @@ static void sweep(Multigrid &M, int l, int first, hipStream_t s)
 {
 	const MgLevel L = level(M, l);
 	const int *n = M.dims[l];
+	const uint8_t *active = M.tile_active + M.tile_off[l];
 	auto launch = [&](auto kernel, const int *tile, int threads) {
 		const dim3 g(unsigned((n[0] + tile[0] - 1) / tile[0]), unsigned((n[1] + tile[1] - 1) / tile[1]), unsigned((n[2] + tile[2] - 1) / tile[2]));
-		hipLaunchKernelGGL(kernel, g, dim3(unsigned(threads)), 0, s, L, (const float *)M.x[l], M.xalt[l], first);
+		hipLaunchKernelGGL(kernel, g, dim3(unsigned(threads)), 0, s, L, (const float *)M.x[l], M.xalt[l], first, active);
 	};
 	if constexpr (D == 2) launch(kMgSweep<2, kTile2[0], kTile2[1], kTile2[2], 1024>, kTile2, 1024);
 	else if (M.cells[l] >= size_t(1) << 20) launch(kMgSweep<3, kTileL[0], kTileL[1], kTileL[2], 1024>, kTileL, 1024);

@@ -31,6 +31,9 @@ struct Multigrid {
 	unsigned *tail_list = nullptr;   // their active cells per colour, listed once per operator
 	int *tail_count = nullptr;       // active cells per level and colour
 	unsigned *box_of = nullptr;      // level-0 cell of every unknown
+	uint8_t *tile_active = nullptr;  // per level above the tail: 1 for each sweep tile holding an active cell
+	size_t tile_off[kMgMaxLevels] = {};
+	size_t tile_cap = 0;
 	float *pool = nullptr;    // one allocation for every level array and the lists
 	size_t pool_cap = 0;      // floats
 	~Multigrid();
