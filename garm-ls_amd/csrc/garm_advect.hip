@@ -9,7 +9,10 @@
 // map (value, first and mixed derivatives per component) per cell, and the
 // same for the reference level set.  A gather then costs one clamped index
 // and a few vector loads per corner instead of a dozen scalar loads with
-// their own clamped neighbour indices.  The packed values are the products
+// their own clamped neighbour indices.  The float reference-map records are
+// planar instead (six planes of one float4 per cell in 3D): a record is 96
+// bytes, and neighbouring lanes reading it whole would spread each wave's
+// 16-byte loads over 96-byte strides, six times the cache lines.  The packed values are the products
 // the reference's interpolants form (HermiteIntrpl.h:17-30, 62-87;
 // Derivatives.h:98-105), computed once per cell in the same operations.
 #include "garm_internal.hpp"
@@ -89,6 +92,20 @@ __device__ __forceinline__ void loadRecord(const T *base, size_t q, T (&v)[S])
 #pragma unroll
 	for (int i = 0; i < S / 4; i++) {
 		const V t = p[i];
+		v[4 * i] = t.x; v[4 * i + 1] = t.y; v[4 * i + 2] = t.z; v[4 * i + 3] = t.w;
+	}
+}
+
+// Record q of a planar float layout: S/4 planes of n float4, plane i holding
+// values 4i..4i+3 of every record.  Lanes whose records are neighbours then
+// read one contiguous run per plane.
+template <int S>
+__device__ __forceinline__ void loadRecordPlanar(const float *base, size_t n, size_t q, float (&v)[S])
+{
+	const float4 *p = reinterpret_cast<const float4 *>(base) + q;
+#pragma unroll
+	for (int i = 0; i < S / 4; i++) {
+		const float4 t = p[i * n];
 		v[4 * i] = t.x; v[4 * i + 1] = t.y; v[4 * i + 2] = t.z; v[4 * i + 3] = t.w;
 	}
 }
@@ -291,18 +308,28 @@ __global__ __launch_bounds__(kBlock) void kPackRefMap(GridDev g, RefMapFields<D>
 #pragma unroll
 			for (int ord = 0; ord < (1 << D); ord++) rec[ord * D + r] = v[ord];
 		}
+		if constexpr (kDisp) {
+			// planar: S/4 planes of one float4 per cell, each wave's stores one contiguous run
+			const size_t n = size_t(g.n[0]) * size_t(g.n[1]) * size_t(D == 3 ? g.n[2] : 1);
+			float4 *p = reinterpret_cast<float4 *>(out) + q;
 #pragma unroll
-		for (int e = 0; e < S; e++) stage[threadIdx.y][lane * S + e] = rec[e];
+			for (int e = 0; e < S / 4; e++) p[e * n] = make_float4(rec[4 * e], rec[4 * e + 1], rec[4 * e + 2], rec[4 * e + 3]);
+		} else {
+#pragma unroll
+			for (int e = 0; e < S; e++) stage[threadIdx.y][lane * S + e] = rec[e];
+		}
 	}
-	__syncthreads();
-	// the wave's records are one run in memory: write it as consecutive 16-byte words
-	const int i0 = int(blockIdx.x * blockDim.x);
-	const bool row = c.j < g.n[1] && (D == 2 || c.k < g.n[2]);
-	const int count = row ? min(kWave, g.n[0] - i0) : 0;
-	using V = typename Vec4<T>::type;
-	const V *src = reinterpret_cast<const V *>(stage[threadIdx.y]);
-	V *dst = reinterpret_cast<V *>(out + gidx<D>(g, row ? i0 : 0, row ? c.j : 0, row ? c.k : 0) * S);
-	for (int e = lane; e < count * S / 4; e += kWave) dst[e] = src[e];
+	if constexpr (!kDisp) {
+		__syncthreads();
+		// the wave's records are one run in memory: write it as consecutive 16-byte words
+		const int i0 = int(blockIdx.x * blockDim.x);
+		const bool row = c.j < g.n[1] && (D == 2 || c.k < g.n[2]);
+		const int count = row ? min(kWave, g.n[0] - i0) : 0;
+		using V = typename Vec4<T>::type;
+		const V *src = reinterpret_cast<const V *>(stage[threadIdx.y]);
+		V *dst = reinterpret_cast<V *>(out + gidx<D>(g, row ? i0 : 0, row ? c.j : 0, row ? c.k : 0) * S);
+		for (int e = lane; e < count * S / 4; e += kWave) dst[e] = src[e];
+	}
 }
 
 // HermiteIntrpl<D, VectorDd>::interpolate + interpolateGradient
@@ -380,6 +407,7 @@ template <int D>
 __device__ __forceinline__ void hermiteRefMapFast(const GridDev &g, const float *recs, const float *X, float *val, float *Gh)
 {
 	constexpr int S = (1 << D) * D;
+	const size_t n = size_t(g.n[0]) * size_t(g.n[1]) * size_t(D == 3 ? g.n[2] : 1);
 	int l[3] = {0, 0, 0}; float f[3];
 	lowerFracT<D, float>(g, X, l, f);
 	float B[3][2][2], Bg[3][2][2];
@@ -402,8 +430,8 @@ __device__ __forceinline__ void hermiteRefMapFast(const GridDev &g, const float 
 #pragma unroll
 		for (int vy = 0; vy < 2; vy++) {
 			float r0[S], r1[S];
-			loadRecord<float, S>(recs, ox[0] + oy[vy] + oz[vz], r0);
-			loadRecord<float, S>(recs, ox[1] + oy[vy] + oz[vz], r1);
+			loadRecordPlanar<S>(recs, n, ox[0] + oy[vy] + oz[vz], r0);
+			loadRecordPlanar<S>(recs, n, ox[1] + oy[vy] + oz[vz], r1);
 			r0[0] += cx[0]; r1[0] += cx[1];
 			r0[1] += cy[vy]; r1[1] += cy[vy];
 			if constexpr (D == 3) { r0[2] += cz[vz]; r1[2] += cz[vz]; }
