@@ -274,8 +274,16 @@ __global__ __launch_bounds__(kBlock) void kEnforceWrite(EnforceArgs<D> A, FaceOf
 // ------------------------------------------------------------- reductions
 __global__ __launch_bounds__(kBlock) void kMaxAbsPartial(const double *a, size_t n, double *partials, int slot0)
 {
-	double m = 0;
-	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) m = fmax(m, fabs(a[i]));
+	// four loads in flight per thread; the maximum does not depend on the order
+	const size_t st = size_t(gridDim.x) * blockDim.x;
+	size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
+	double m = 0, m1 = 0;
+	for (; i + 3 * st < n; i += 4 * st) {
+		m = fmax(m, fmax(fabs(a[i]), fabs(a[i + st])));
+		m1 = fmax(m1, fmax(fabs(a[i + 2 * st]), fabs(a[i + 3 * st])));
+	}
+	for (; i < n; i += st) m = fmax(m, fabs(a[i]));
+	m = fmax(m, m1);
 	__shared__ double red[kBlock / kWave];
 	m = waveMax(m);
 	const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
@@ -341,13 +349,32 @@ __global__ __launch_bounds__(kBlock) void kCheckRefMap(GridDev g, const double *
 	}
 }
 
-__global__ __launch_bounds__(kBlock) void kSumFinal2(const double *partials, int nblocks, double *out)
+// The two partials per workgroup are counts (whole numbers in double, far
+// below 2^53), so any summation order gives the same totals: one wide
+// workgroup with two loads in flight per thread.
+constexpr int kSumThreads = 1024;
+
+__global__ __launch_bounds__(kSumThreads) void kSumFinal2(const double *partials, int nblocks, double *out)
 {
-	double v[2] = {0, 0};
-	for (int i = threadIdx.x; i < nblocks; i += blockDim.x) { v[0] += partials[2 * i]; v[1] += partials[2 * i + 1]; }
-	double o[2];
-	blockSum<2>(v, o);
-	if (threadIdx.x == 0) { out[0] = o[0]; out[1] = o[1]; }
+	double a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+	int i = int(threadIdx.x);
+	for (; i + kSumThreads < nblocks; i += 2 * kSumThreads) {
+		a0 += partials[2 * i]; a1 += partials[2 * i + 1];
+		b0 += partials[2 * (i + kSumThreads)]; b1 += partials[2 * (i + kSumThreads) + 1];
+	}
+	if (i < nblocks) { a0 += partials[2 * i]; a1 += partials[2 * i + 1]; }
+	a0 = waveSum(a0 + b0);
+	a1 = waveSum(a1 + b1);
+	__shared__ double red[2][kSumThreads / kWave];
+	const int lane = int(threadIdx.x) & (kWave - 1), wid = int(threadIdx.x) / kWave;
+	if (lane == 0) { red[0][wid] = a0; red[1][wid] = a1; }
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double s0 = 0, s1 = 0;
+		for (int w = 0; w < kSumThreads / kWave; w++) { s0 += red[0][w]; s1 += red[1][w]; }
+		out[0] = s0;
+		out[1] = s1;
+	}
 }
 
 // --------------------------------------------------------------- launchers
@@ -469,7 +496,7 @@ void launchCheckRefMap(const Fields &F, double threshold, double band, hipStream
 	for (int e = 0; e < F.D * F.D; e++) J.p[e] = F.J[e];
 	launchCells(F.D, F.cell, s, kCheckRefMap<2>, kCheckRefMap<3>, F.cell, (const double *)F.phi_ori, (const double *)F.cell_dist,
 		J, threshold, band, F.partials);
-	hipLaunchKernelGGL(kSumFinal2, dim3(1), dim3(kBlock), 0, s, (const double *)F.partials, nb, F.dscalars + 1);
+	hipLaunchKernelGGL(kSumFinal2, dim3(1), dim3(kSumThreads), 0, s, (const double *)F.partials, nb, F.dscalars + 1);
 }
 
 } // namespace garm
