@@ -21,6 +21,18 @@ struct GridDev {
 };
 
 constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
+
+// Host wait for the work enqueued on s so far: records ev (created with
+// hipEventDisableTiming) and polls it, so the wait for a small readback ends
+// within microseconds of its copy, where a stream synchronisation can leave
+// the thread asleep for tens of microseconds.
+inline hipError_t spinWait(hipStream_t s, hipEvent_t ev)
+{
+	hipError_t e = hipEventRecord(ev, s);
+	if (e != hipSuccess) return e;
+	while ((e = hipEventQuery(ev)) == hipErrorNotReady) {}
+	return e;
+}
 constexpr int kWave = 64;
 
 template <int D>

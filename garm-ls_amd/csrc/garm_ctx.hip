@@ -131,7 +131,7 @@ int readScalars(garm_ctx *c, int n)
 {
 	int rc = hipCheck(c, hipMemcpyAsync(c->host_scalars, c->F.dscalars, n * sizeof(double), hipMemcpyDeviceToHost, c->s), "scalar readback");
 	if (rc) return rc;
-	return hipCheck(c, hipStreamSynchronize(c->s), "stream synchronize");
+	return hipCheck(c, spinWait(c->s, c->F.wait), "scalar readback");
 }
 
 int doProject(garm_ctx *c, const garm_pcg_opts &o, garm_pcg_stats *st)
@@ -306,7 +306,8 @@ int garm_create(garm_ctx **out, int dim, const int res[3], double dx, const doub
 		F.partials_cap = std::max<size_t>(size_t(2) * gd.x * gd.y * gd.z + 64, 4096);
 	}
 	ok = ok && alloc(c, F.partials, F.partials_cap) && alloc(c, F.dscalars, 64);
-	ok = ok && hipHostMalloc(&c->host_scalars, 64 * sizeof(double)) == hipSuccess;
+	ok = ok && hipHostMalloc(&c->host_scalars, 64 * sizeof(double)) == hipSuccess
+		&& hipHostMalloc(&F.host_ints, 16 * sizeof(int)) == hipSuccess && hipEventCreateWithFlags(&F.wait, hipEventDisableTiming) == hipSuccess;
 	for (int i = 0; i < 8 && ok; i++) ok = hipEventCreate(&c->ev[i]) == hipSuccess;
 	if (!ok) {
 		g_create_error = "device allocation failed";
@@ -358,6 +359,8 @@ void garm_destroy(garm_ctx *c)
 	if (c->F.bface_rec) (void)hipFree(c->F.bface_rec);
 	for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
 	if (c->host_scalars) (void)hipHostFree(c->host_scalars);
+	if (c->F.host_ints) (void)hipHostFree(c->F.host_ints);
+	if (c->F.wait) (void)hipEventDestroy(c->F.wait);
 	if (c->s) (void)hipStreamDestroy(c->s);
 	delete c;
 }
@@ -674,7 +677,10 @@ int garm_step(garm_ctx *c, double dt, int step_index, const garm_step_opts *o, g
 	doEnforceBoundary(c);
 	(void)hipEventRecord(ev[6], c->s);
 	if ((rc = launched(c, "step"))) return rc;
-	if ((rc = hipCheck(c, hipEventSynchronize(ev[6]), "step"))) return rc;
+	// polled, as spinWait: the step's end returns to the caller without a sleep
+	hipError_t e;
+	while ((e = hipEventQuery(ev[6])) == hipErrorNotReady) {}
+	if ((rc = hipCheck(c, e, "step"))) return rc;
 	float t[7] = {};
 	for (int i = 1; i <= 6; i++) (void)hipEventElapsedTime(&t[i], ev[i - 1], ev[i]);
 	st->t_advect_ms = t[1];

@@ -53,6 +53,8 @@ struct Fields {
 	double *partials = nullptr;                       // reduction partials
 	size_t partials_cap = 0;
 	double *dscalars = nullptr;                       // small device scalars
+	int *host_ints = nullptr;                         // pinned, 16 ints: small readbacks
+	hipEvent_t wait = nullptr;                        // spinWait's event on the context's stream
 };
 
 // ---- launchers (one per kernel family; all enqueue on `s`) ---------------

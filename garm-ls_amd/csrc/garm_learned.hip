@@ -426,11 +426,13 @@ static int unknownBox(const int *mat2grid, int n, const GridDev &cell, LearnedWo
 {
 	hipLaunchKernelGGL(kBBoxInit, dim3(1), dim3(64), 0, s, W.box);
 	hipLaunchKernelGGL(kBBox, dim3(lblocks(n)), dim3(kBlock), 0, s, mat2grid, n, cell.n[0], cell.n[1], W.box);
-	int hb[6];
-	if (hipMemcpyAsync(hb, W.box, sizeof(hb), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+	// six ints through the pinned readback buffer
+	const int *hb = reinterpret_cast<const int *>(W.host);
+	if (hipMemcpyAsync(W.host, W.box, 6 * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess || hipEventRecord(W.ready, s) != hipSuccess) {
 		err = "bounding-box readback failed";
 		return GARM_EHIP;
 	}
+	if (int rc = readWait(W, err)) return rc;
 	for (int a = 0; a < 3; a++) { lo[a] = hb[a]; hi[a] = hb[3 + a]; }
 	return GARM_OK;
 }

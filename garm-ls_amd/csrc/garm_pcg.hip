@@ -266,6 +266,7 @@ void Krylov::release()
 	fr(x); fr(r); fr(p); fr(y); fr(r32); fr(p32); fr(y32); fr(pa); fr(pb); fr(sc); fr(state);
 	if (host_state) { (void)hipHostFree(host_state); host_state = nullptr; }
 	if (host_sc) { (void)hipHostFree(host_sc); host_sc = nullptr; }
+	if (ready) { (void)hipEventDestroy(ready); ready = nullptr; }
 	cap = 0;
 }
 
@@ -275,6 +276,7 @@ bool Krylov::reserve(size_t n, bool mixed, std::string &err)
 	if (!sc) {
 		ok = hipMalloc(&sc, 32 * sizeof(double)) == hipSuccess && hipMalloc(&state, 16 * sizeof(int)) == hipSuccess
 			&& hipHostMalloc(&host_state, 16 * sizeof(int)) == hipSuccess && hipHostMalloc(&host_sc, 32 * sizeof(double)) == hipSuccess
+			&& hipEventCreateWithFlags(&ready, hipEventDisableTiming) == hipSuccess
 			&& hipMalloc(&pa, kGrid * sizeof(double)) == hipSuccess && hipMalloc(&pb, 2 * kGrid * sizeof(double)) == hipSuccess
 			&& hipMemset(state, 0, 16 * sizeof(int)) == hipSuccess && hipMemset(sc, 0, 32 * sizeof(double)) == hipSuccess;
 		if (!ok) { err = "solver scalar allocation failed"; return false; }
@@ -298,7 +300,7 @@ static int readState(Krylov &K, hipStream_t s, std::string &err)
 {
 	if (hipMemcpyAsync(K.host_state, K.state, 8 * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
 		hipMemcpyAsync(K.host_sc, K.sc, 16 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
-		hipStreamSynchronize(s) != hipSuccess) { err = "solver state readback failed"; return GARM_EHIP; }
+		spinWait(s, K.ready) != hipSuccess) { err = "solver state readback failed"; return GARM_EHIP; }
 	return GARM_OK;
 }
 

@@ -35,6 +35,7 @@ struct Krylov {
 	int *state = nullptr;
 	int *host_state = nullptr;   // pinned
 	double *host_sc = nullptr;   // pinned
+	hipEvent_t ready = nullptr;  // spinWait's event for the state readback
 
 	~Krylov();
 	void release();

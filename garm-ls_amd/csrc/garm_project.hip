@@ -222,10 +222,10 @@ static int numberUnknowns(const Fields &F, Projection &P, bool mixed, hipStream_
 	hipLaunchKernelGGL(kUnknownFlags<D>, gridDims(F.cell, D), gridBlock(), 0, s, F.cell, faceArrays(F, F.frac), (const double *)F.phi, P.flags);
 	size_t bytes = P.scan_bytes;
 	if (hipcub::DeviceScan::ExclusiveSum(P.scan_tmp, bytes, P.flags, P.grid2mat, int(F.N), s) != hipSuccess) { err = "prefix sum failed"; return -1; }
-	int last[2];
+	int *last = F.host_ints;
 	if (hipMemcpyAsync(&last[0], P.grid2mat + F.N - 1, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
 		hipMemcpyAsync(&last[1], P.flags + F.N - 1, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
-		hipStreamSynchronize(s) != hipSuccess) { err = "reading the unknown count failed"; return -1; }
+		spinWait(s, F.wait) != hipSuccess) { err = "reading the unknown count failed"; return -1; }
 	const int n = last[0] + last[1];
 	if (!P.reserveRows(D, size_t(n), mixed, err)) return -1;
 	hipLaunchKernelGGL(kCompactUnknowns, dim3(linBlocks(F.N)), dim3(kBlock), 0, s, (const int *)P.flags, P.grid2mat, P.mat2grid, F.N);
