@@ -46,6 +46,11 @@ struct ConvArgs {
 	int outStride, outOff, Cout;
 	int nx, ny, nz;
 	int relu;
+	// optional: input channels [0, upC) read from a grid of half the size
+	// (nearest-neighbour upsampling folded into the staging; upC a multiple
+	// of 8, of 4 for the narrow staging)
+	const bf16 *up;
+	int upStride, upC;
 };
 
 // K = 32 per MFMA (v_mfma_f32_16x16x32_bf16): the reduction runs over pairs
@@ -93,8 +98,12 @@ __global__ __launch_bounds__(kUB) void kConv3(ConvArgs A)
 				const int hx = v % CHX, hy = (v / CHX) % HY, hz = v / (CHX * HY);
 				const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
 				hv[r] = make_uint4(0, 0, 0, 0);
-				if (q < kVecs && gx >= 0 && gx < A.nx && gy >= 0 && gy < A.ny && gz >= 0 && gz < A.nz && ch < A.Cin)
-					hv[r] = *reinterpret_cast<const uint4 *>(A.in + (size_t(gz * A.ny + gy) * A.nx + gx) * A.inStride + ch);
+				if (q < kVecs && gx >= 0 && gx < A.nx && gy >= 0 && gy < A.ny && gz >= 0 && gz < A.nz && ch < A.Cin) {
+					const bf16 *src = A.up && ch < A.upC
+						? A.up + (size_t((gz >> 1) * (A.ny >> 1) + (gy >> 1)) * (A.nx >> 1) + (gx >> 1)) * A.upStride + ch
+						: A.in + (size_t(gz * A.ny + gy) * A.nx + gx) * A.inStride + ch;
+					hv[r] = *reinterpret_cast<const uint4 *>(src);
+				}
 			}
 #pragma unroll
 			for (int r = 0; r < kIt; r++) {
@@ -111,8 +120,12 @@ __global__ __launch_bounds__(kUB) void kConv3(ConvArgs A)
 				const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
 				const int ch = ci0 + quad * 4;
 				hv[r] = make_uint2(0, 0);
-				if (q < kVecs && gx >= 0 && gx < A.nx && gy >= 0 && gy < A.ny && gz >= 0 && gz < A.nz && ch < A.Cin)
-					hv[r] = *reinterpret_cast<const uint2 *>(A.in + (size_t(gz * A.ny + gy) * A.nx + gx) * A.inStride + ch);
+				if (q < kVecs && gx >= 0 && gx < A.nx && gy >= 0 && gy < A.ny && gz >= 0 && gz < A.nz && ch < A.Cin) {
+					const bf16 *src = A.up && ch < A.upC
+						? A.up + (size_t((gz >> 1) * (A.ny >> 1) + (gy >> 1)) * (A.nx >> 1) + (gx >> 1)) * A.upStride + ch
+						: A.in + (size_t(gz * A.ny + gy) * A.nx + gx) * A.inStride + ch;
+					hv[r] = *reinterpret_cast<const uint2 *>(src);
+				}
 			}
 #pragma unroll
 			for (int r = 0; r < kIt; r++) {
@@ -764,10 +777,11 @@ void unetPackWeights(Unet &U, hipStream_t s)
 }
 
 static void conv(const UnetBuffers &B, const UnetLayer &L, int level, const bf16 *in, int inStride, int cin, const bf16 *w, int cinP, int coutP,
-	const float *bias, bf16 *out, int outStride, int outOff, int cout, bool relu, hipStream_t s)
+	const float *bias, bf16 *out, int outStride, int outOff, int cout, bool relu, hipStream_t s, const bf16 *up = nullptr, int upStride = 0, int upC = 0)
 {
 	ConvArgs A{};
 	A.in = in; A.inStride = inStride; A.Cin = cin;
+	A.up = up; A.upStride = upStride; A.upC = upC;
 	A.w = w; A.cinP = cinP; A.coutP = coutP;
 	A.bias = bias;
 	A.out = out; A.outStride = outStride; A.outOff = outOff; A.Cout = cout;
@@ -776,7 +790,7 @@ static void conv(const UnetBuffers &B, const UnetLayer &L, int level, const bf16
 	const int groups = (coutP + 31) / 32;
 	const dim3 g((A.nx + CX - 1) / CX, (A.ny + TY - 1) / TY, ((A.nz + TZ - 1) / TZ) * groups);
 	const size_t lds = size_t(HZ * HY * CHX * 16 + 27 * std::min(32, coutP) * 16) * sizeof(bf16);
-	if ((inStride & 7) == 0 && (cin & 7) == 0) hipLaunchKernelGGL(kConv3<true>, g, dim3(kUB), lds, s, A);
+	if ((inStride & 7) == 0 && (cin & 7) == 0 && (!up || ((upStride & 7) == 0 && (upC & 7) == 0))) hipLaunchKernelGGL(kConv3<true>, g, dim3(kUB), lds, s, A);
 	else hipLaunchKernelGGL(kConv3<false>, g, dim3(kUB), lds, s, A);
 	(void)L;
 }
@@ -815,14 +829,26 @@ static void layerGrad(UnetBuffers &B, int i, bf16 *&gOut, bf16 *&gIn)
 	}
 }
 
+// The decoder's inputs c2 = [up(b3) | skip] and c1 = [up(d2) | skip]: the
+// forward pass reads the upsampled channels straight from b3 and d2 in the
+// convolutions' staging (the same values, so the same results), and
+// unetBackward writes them into c2 and c1 for the weight gradients.
+static void materializeUp(UnetBuffers &B, hipStream_t s)
+{
+	const int *n0 = B.n[0], *n1 = B.n[1];
+	hipLaunchKernelGGL(kUp, dim3(blocksFor(B.vox[1] * 8)), dim3(kUB), 0, s, (const bf16 *)B.b3, 64, 64, B.c2, 96, n1[0], n1[1], n1[2]);
+	hipLaunchKernelGGL(kUp, dim3(blocksFor(B.vox[0] * 4)), dim3(kUB), 0, s, (const bf16 *)B.d2, 32, 32, B.c1, 48, n0[0], n0[1], n0[2]);
+}
+
 void unetForward(Unet &U, hipStream_t s)
 {
 	UnetBuffers &B = U.buf;
-	auto run = [&](int i) {
+	auto run = [&](int i, const bf16 *up = nullptr, int upStride = 0, int upC = 0) {
 		LayerIO io;
 		layerIO(B, i, io);
 		const UnetLayer &L = U.L[i];
-		conv(B, L, io.level, io.in, io.inStride, L.cin, L.wf, roundUp16(L.cin), roundUp16(L.cout), L.b, io.out, io.outStride, io.outOff, L.cout, true, s);
+		conv(B, L, io.level, io.in, io.inStride, L.cin, L.wf, roundUp16(L.cin), roundUp16(L.cout), L.b, io.out, io.outStride, io.outOff, L.cout, true, s,
+			up, upStride, upC);
 	};
 	const int *n0 = B.n[0], *n1 = B.n[1];
 	run(0);
@@ -833,11 +859,9 @@ void unetForward(Unet &U, hipStream_t s)
 	hipLaunchKernelGGL(kPool, dim3(blocksFor(B.vox[2] * 4)), dim3(kUB), 0, s, (const bf16 *)B.c2, 96, 64, 32, B.p2, 32, n1[0], n1[1], n1[2]);
 	run(4);
 	run(5);
-	hipLaunchKernelGGL(kUp, dim3(blocksFor(B.vox[1] * 8)), dim3(kUB), 0, s, (const bf16 *)B.b3, 64, 64, B.c2, 96, n1[0], n1[1], n1[2]);
-	run(6);
+	run(6, B.b3, 64, 64);
 	run(7);
-	hipLaunchKernelGGL(kUp, dim3(blocksFor(B.vox[0] * 4)), dim3(kUB), 0, s, (const bf16 *)B.d2, 32, 32, B.c1, 48, n0[0], n0[1], n0[2]);
-	run(8);
+	run(8, B.d2, 32, 32);
 	run(9);
 	hipLaunchKernelGGL(kOut, dim3(blocksFor(B.vox[0])), dim3(kUB), 0, s, (const bf16 *)B.d1, (const bf16 *)B.in, (const float *)U.ow,
 		(const float *)U.ob, B.y, B.vox[0]);
@@ -871,6 +895,7 @@ bool unetBackward(Unet &U, hipStream_t s, std::string &err)
 		B.partial_cap = need;
 	}
 	const int *n0 = B.n[0], *n1 = B.n[1];
+	materializeUp(B, s);
 	// output layer
 	const int nbOut = int(std::min<size_t>((B.vox[0] + kUB - 1) / kUB, 1024));
 	hipLaunchKernelGGL(kOutBack, dim3(nbOut), dim3(kUB), 0, s, (const float *)B.gy, (const bf16 *)B.d1, (const bf16 *)B.in,
