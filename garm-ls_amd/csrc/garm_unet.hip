@@ -51,6 +51,11 @@ struct ConvArgs {
 	// of 8, of 4 for the narrow staging)
 	const bf16 *up;
 	int upStride, upC;
+	// optional: the 2x2x2 means of the BF16 output into a grid of half the
+	// size (channel co at pool[parent * poolStride + co]): the eight children
+	// summed in FP32, x fastest, then y, then z, times 1/8, rounded to BF16
+	bf16 *pool;
+	int poolStride;
 };
 
 // K = 32 per MFMA (v_mfma_f32_16x16x32_bf16): the reduction runs over pairs
@@ -194,6 +199,24 @@ __global__ __launch_bounds__(kUB) void kConv3(ConvArgs A)
 		}
 	__syncthreads();
 	const int nch = min(NF * 16, A.Cout - co0);
+	if (A.pool) {
+		// the tile's origin is even along every axis, so it holds whole 2x2x2 families
+		constexpr int PX = CX / 2, PY = TY / 2, PZ = TZ / 2;
+		const int pnx = A.nx >> 1, pny = A.ny >> 1, pnz = A.nz >> 1;
+		for (int q = tid; q < PX * PY * PZ * nch; q += kUB) {
+			const int c = q % nch, pv = q / nch;
+			const int px = pv % PX, py = (pv / PX) % PY, pz = pv / (PX * PY);
+			const int X = (x0 >> 1) + px, Y = (y0 >> 1) + py, Z = (z0 >> 1) + pz;
+			if (X >= pnx || Y >= pny || Z >= pnz) continue;
+			float sum = 0;
+#pragma unroll
+			for (int k = 0; k < 8; k++) {
+				const int vx = 2 * px + (k & 1), vy = 2 * py + (k >> 1 & 1), vz = 2 * pz + (k >> 2);
+				sum += bf2f(tout[((vz * TY + vy) * CX + vx) * 32 + c]);
+			}
+			A.pool[(size_t(Z * pny + Y) * pnx + X) * A.poolStride + co0 + c] = f2bf(sum * 0.125f);
+		}
+	}
 	if (nch == NF * 16 && (A.outStride & 7) == 0 && ((A.outOff + co0) & 7) == 0) {
 		const int words = NF * 2;   // 16-byte words per voxel
 		for (int q = tid; q < CX * TY * TZ * words; q += kUB) {
@@ -356,29 +379,6 @@ __device__ __forceinline__ uint4 pack8(const float (&f)[8])
 
 __device__ __forceinline__ const uint4 &vec8(const bf16 *p) { return *reinterpret_cast<const uint4 *>(p); }
 __device__ __forceinline__ uint4 &vec8(bf16 *p) { return *reinterpret_cast<uint4 *>(p); }
-
-// 2x2x2 mean pooling of channels [inOff, inOff + C) (child grid n)
-__global__ void kPool(const bf16 *in, int inStride, int inOff, int C, bf16 *out, int outStride, int nx, int ny, int nz)
-{
-	const int px = nx / 2, py = ny / 2, pz = nz / 2, G = C / 8;
-	const size_t total = size_t(px) * py * pz * G;
-	for (size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x; e < total; e += size_t(gridDim.x) * blockDim.x) {
-		const int c = int(e % G) * 8;
-		int x, y, z;
-		coords(e / G, px, py, x, y, z);
-		float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-		for (int k = 0; k < 8; k++) {
-			const int cx = 2 * x + (k & 1), cy = 2 * y + (k >> 1 & 1), cz = 2 * z + (k >> 2);
-			float f[8];
-			unpack8(vec8(in + (size_t(cz * ny + cy) * nx + cx) * inStride + inOff + c), f);
-#pragma unroll
-			for (int i = 0; i < 8; i++) s[i] += f[i];
-		}
-#pragma unroll
-		for (int i = 0; i < 8; i++) s[i] *= 0.125f;
-		vec8(out + (size_t(z * py + y) * px + x) * outStride + c) = pack8(s);
-	}
-}
 
 // gradient of the mean: child += parent / 8
 __global__ void kPoolBack(const bf16 *gp, int pStride, int C, bf16 *gc, int cStride, int cOff, int nx, int ny, int nz)
@@ -777,11 +777,13 @@ void unetPackWeights(Unet &U, hipStream_t s)
 }
 
 static void conv(const UnetBuffers &B, const UnetLayer &L, int level, const bf16 *in, int inStride, int cin, const bf16 *w, int cinP, int coutP,
-	const float *bias, bf16 *out, int outStride, int outOff, int cout, bool relu, hipStream_t s, const bf16 *up = nullptr, int upStride = 0, int upC = 0)
+	const float *bias, bf16 *out, int outStride, int outOff, int cout, bool relu, hipStream_t s, const bf16 *up = nullptr, int upStride = 0, int upC = 0,
+	bf16 *pool = nullptr, int poolStride = 0)
 {
 	ConvArgs A{};
 	A.in = in; A.inStride = inStride; A.Cin = cin;
 	A.up = up; A.upStride = upStride; A.upC = upC;
+	A.pool = pool; A.poolStride = poolStride;
 	A.w = w; A.cinP = cinP; A.coutP = coutP;
 	A.bias = bias;
 	A.out = out; A.outStride = outStride; A.outOff = outOff; A.Cout = cout;
@@ -843,20 +845,18 @@ static void materializeUp(UnetBuffers &B, hipStream_t s)
 void unetForward(Unet &U, hipStream_t s)
 {
 	UnetBuffers &B = U.buf;
-	auto run = [&](int i, const bf16 *up = nullptr, int upStride = 0, int upC = 0) {
+	auto run = [&](int i, const bf16 *up = nullptr, int upStride = 0, int upC = 0, bf16 *pool = nullptr, int poolStride = 0) {
 		LayerIO io;
 		layerIO(B, i, io);
 		const UnetLayer &L = U.L[i];
 		conv(B, L, io.level, io.in, io.inStride, L.cin, L.wf, roundUp16(L.cin), roundUp16(L.cout), L.b, io.out, io.outStride, io.outOff, L.cout, true, s,
-			up, upStride, upC);
+			up, upStride, upC, pool, poolStride);
 	};
-	const int *n0 = B.n[0], *n1 = B.n[1];
+	// the encoder's pooling happens in the epilogues of layers 1 and 3
 	run(0);
-	run(1);
-	hipLaunchKernelGGL(kPool, dim3(blocksFor(B.vox[1] * 2)), dim3(kUB), 0, s, (const bf16 *)B.c1, 48, 32, 16, B.p1, 16, n0[0], n0[1], n0[2]);
+	run(1, nullptr, 0, 0, B.p1, 16);
 	run(2);
-	run(3);
-	hipLaunchKernelGGL(kPool, dim3(blocksFor(B.vox[2] * 4)), dim3(kUB), 0, s, (const bf16 *)B.c2, 96, 64, 32, B.p2, 32, n1[0], n1[1], n1[2]);
+	run(3, nullptr, 0, 0, B.p2, 32);
 	run(4);
 	run(5);
 	run(6, B.b3, 64, 64);
