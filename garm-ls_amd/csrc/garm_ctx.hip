@@ -31,7 +31,7 @@ struct garm_ctx {
 
 namespace {
 
-thread_local std::string g_create_error;
+thread_local std::string g_create_error;   // also the message of garm_volume_device
 
 int fail(garm_ctx *c, int code, const std::string &msg)
 {
@@ -208,6 +208,8 @@ int rebuildBoundaryFaces(garm_ctx *c)
 }
 
 } // namespace
+
+void garm::setContextFreeError(const std::string &msg) { g_create_error = msg; }
 
 // =========================================================================
 extern "C" {

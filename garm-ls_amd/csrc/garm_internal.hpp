@@ -77,6 +77,8 @@ void launchUnionPlane(const Fields &F, const double *pos, const double *nrm, hip
 void launchResetRefMap(const Fields &F, hipStream_t s);
 // volume of phi_tmp into F.dscalars[3] (corrected: circular corner order)
 void launchVolume(const Fields &F, bool corrected, hipStream_t s);
+// message of the last failed call that has no context (garm_create_error)
+void setContextFreeError(const std::string &msg);
 void launchFill(double *p, double v, size_t n, hipStream_t s);
 void launchFillU8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
 void launchCopy(double *dst, const double *src, size_t n, hipStream_t s);

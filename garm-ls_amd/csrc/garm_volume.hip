@@ -14,8 +14,17 @@
 // The contour is planar, so its triangle sum is the exact area of the inside
 // polygons, computed here directly.
 #include "garm_internal.hpp"
+#include "../../include/garm_gpu.h"
+
+#include <mutex>
+#include <string>
 
 namespace garm {
+
+// Workgroups of the volume pass (one partial sum each), and the largest
+// resolution per axis garm_volume_device accepts.
+constexpr int kVolumeBlocks = 1024;
+constexpr int kVolumeDeviceMaxRes = 4096;
 
 // -------------------------------------------------- MC-style volume fraction
 // Corner order of the fraction: 0 (0,0,0) 1 (1,0,0) 2 (1,1,0) 3 (0,1,0)
@@ -295,18 +304,88 @@ __global__ __launch_bounds__(kBlock) void kVolumeFinal(const double *partials, i
 	if (threadIdx.x == 0) *out = o[0] * scale;
 }
 
-void launchVolume(const Fields &F, bool corrected, hipStream_t s)
+// Volume of phi on grid g (dim D) into *out: one pass writing kVolumeBlocks
+// partials, then one workgroup sums them in a fixed order.
+void launchVolumeInto(const GridDev &g, int D, const double *phi, bool corrected, double *partials, double *out, hipStream_t s)
 {
-	const GridDev &g = F.cell;
-	const int nb = 1024;
-	if (F.D == 3) {
-		if (corrected) hipLaunchKernelGGL(kVolume3<true>, dim3(nb), dim3(kBlock), 0, s, g, (const double *)F.phi_tmp, F.partials);
-		else hipLaunchKernelGGL(kVolume3<false>, dim3(nb), dim3(kBlock), 0, s, g, (const double *)F.phi_tmp, F.partials);
-		hipLaunchKernelGGL(kVolumeFinal, dim3(1), dim3(kBlock), 0, s, (const double *)F.partials, nb, g.dx * g.dx * g.dx, F.dscalars + 3);
+	const int nb = kVolumeBlocks;
+	if (D == 3) {
+		if (corrected) hipLaunchKernelGGL(kVolume3<true>, dim3(nb), dim3(kBlock), 0, s, g, phi, partials);
+		else hipLaunchKernelGGL(kVolume3<false>, dim3(nb), dim3(kBlock), 0, s, g, phi, partials);
+		hipLaunchKernelGGL(kVolumeFinal, dim3(1), dim3(kBlock), 0, s, (const double *)partials, nb, g.dx * g.dx * g.dx, out);
 	} else {
-		hipLaunchKernelGGL(kArea2, dim3(nb), dim3(kBlock), 0, s, g, (const double *)F.phi_tmp, F.partials);
-		hipLaunchKernelGGL(kVolumeFinal, dim3(1), dim3(kBlock), 0, s, (const double *)F.partials, nb, g.dx * g.dx, F.dscalars + 3);
+		hipLaunchKernelGGL(kArea2, dim3(nb), dim3(kBlock), 0, s, g, phi, partials);
+		hipLaunchKernelGGL(kVolumeFinal, dim3(1), dim3(kBlock), 0, s, (const double *)partials, nb, g.dx * g.dx, out);
 	}
 }
 
+void launchVolume(const Fields &F, bool corrected, hipStream_t s)
+{
+	launchVolumeInto(F.cell, F.D, F.phi_tmp, corrected, F.partials, F.dscalars + 3, s);
+}
+
 } // namespace garm
+
+// ------------------------------------------- allocation-free volume entry
+// garm_volume_device keeps its partial sums in this file-scope array, one
+// copy per device, instead of memory of a context.  In host code `scratch`
+// names a host-side stand-in, never the device copy: its device address is
+// taken once per device with hipGetSymbolAddress and only that address is
+// passed to kernels.
+static __device__ __align__(256) double scratch[garm::kVolumeBlocks];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[kMaxDevices] = {};
+// Device address of this device's copy of `scratch` (the device must be
+// current); nullptr with the HIP error in *e when the lookup fails.
+double *scratchOfDevice(int dev, hipError_t *e)
+{
+	std::lock_guard<std::mutex> lock(g_scratch_mutex);
+	*e = hipSuccess;
+	if (!g_scratch_ptr[dev]) {
+		void *ptr = nullptr;
+		*e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+		if (*e != hipSuccess) return nullptr;
+		g_scratch_ptr[dev] = static_cast<double *>(ptr);
+	}
+	return g_scratch_ptr[dev];
+}
+
+int deviceFail(int code, const std::string &msg)
+{
+	garm::setContextFreeError(msg);
+	return code;
+}
+
+} // namespace
+
+extern "C" int garm_volume_device(int dim, const int res[3], double dx, const double *phi_dev, int order, double *out_dev, void *stream)
+{
+	garm::setContextFreeError("");
+	if ((dim != 2 && dim != 3) || !res || !phi_dev || !out_dev || !(dx > 0))
+		return deviceFail(GARM_EINVAL, "garm_volume_device: invalid arguments");
+	if (order != GARM_VOLUME_REFERENCE && order != GARM_VOLUME_CORRECTED) return deviceFail(GARM_EINVAL, "garm_volume_device: unknown vertex order");
+	double cells = 1;
+	for (int a = 0; a < dim; a++) {
+		if (res[a] < 2 || res[a] > garm::kVolumeDeviceMaxRes)
+			return deviceFail(GARM_EINVAL, "garm_volume_device: resolution outside [2, " + std::to_string(garm::kVolumeDeviceMaxRes) + "]");
+		cells *= res[a];
+	}
+	if (cells >= 2147483647.0) return deviceFail(GARM_EINVAL, "garm_volume_device: grid holds 2^31 cells or more");
+	int dev = -1;
+	if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return deviceFail(GARM_ENODEV, "garm_volume_device: no usable device");
+	hipError_t e;
+	double *partials = scratchOfDevice(dev, &e);
+	if (!partials) return deviceFail(GARM_EHIP, std::string("garm_volume_device: scratch address: ") + hipGetErrorString(e));
+	garm::GridDev g{};
+	for (int a = 0; a < 3; a++) { g.n[a] = a < dim ? res[a] : 1; g.o[a] = 0; }
+	g.dx = dx;
+	g.inv = 1 / dx;
+	garm::launchVolumeInto(g, dim, phi_dev, order == GARM_VOLUME_CORRECTED, partials, out_dev, static_cast<hipStream_t>(stream));
+	e = hipGetLastError();
+	if (e != hipSuccess) return deviceFail(GARM_EHIP, std::string("garm_volume_device: launch: ") + hipGetErrorString(e));
+	return GARM_OK;
+}

@@ -115,6 +115,7 @@ def lib():
             "garm_max_speed": (i, [vp, dp]),
             "garm_apply_pressure_system": (i, [vp, vp, vp]),
             "garm_volume": (i, [vp, i, dp]),
+            "garm_volume_device": (i, [i, ip, d, vp, i, vp, vp]),
             "garm_set_precision": (i, [vp, i]),
             "garm_set_velocity_field": (i, [vp, i, d, d]),
             "garm_synchronize": (i, [vp]),
@@ -311,6 +312,16 @@ class Liquid:
         """Use `net` (a Unet, or None) for PCG_LEARNED solves."""
         self._check(lib().garm_attach_preconditioner(self.h, net.h if net is not None else None))
         self._net = net   # keep the network alive while attached
+
+
+def volume_device(dim, res, dx, phi_ptr, out_ptr, corrected=False, stream=None):
+    """garm_volume_device on device memory (phi_ptr, out_ptr: device addresses,
+    stream: a hipStream_t value or None); enqueues and returns without waiting."""
+    r = list(res) + [1] * (3 - len(res))
+    rc = lib().garm_volume_device(dim, (ctypes.c_int * 3)(*r), dx, ctypes.c_void_p(phi_ptr), 1 if corrected else 0,
+                                  ctypes.c_void_p(out_ptr), ctypes.c_void_p(stream) if stream else None)
+    if rc != GARM_OK:
+        raise GarmError(rc, lib().garm_create_error().decode())
 
 
 class Unet:

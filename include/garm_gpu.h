@@ -141,7 +141,8 @@ typedef struct garm_step_stats {
 int garm_create(garm_ctx **out, int dim, const int res[3], double dx, const double origin[3], int boundary_width);
 void garm_destroy(garm_ctx *ctx);
 const char *garm_last_error(const garm_ctx *ctx);
-/* Last error of a failed garm_create (no context exists then). */
+/* Last error of a failed garm_create or garm_volume_device (calls without a
+ * context). */
 const char *garm_create_error(void);
 
 /* Number of elements (not bytes) of a field. */
@@ -228,6 +229,21 @@ typedef enum garm_volume_order {
  * MC-style-vol-eval/fraction.hpp:510-771), 2D filled-contour area
  * (Contourer<2, true>, Contourer.cpp:9-96); `order` applies in 3D only. */
 int garm_volume(garm_ctx *ctx, int order, double *out);
+/* Allocation-free form of garm_volume for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It has no counterpart in the reference
+ * and needs no context: phi_dev is a device array of res[0]*res[1](*res[2])
+ * FP64 cell values (x fastest), *out_dev (device memory) receives the volume
+ * with cell size dx, computed exactly as garm_volume computes it (same
+ * kernels, same reduction order, same bits).  The work is enqueued on
+ * `stream` (a hipStream_t; NULL is the null stream) and the call returns
+ * without waiting.  Partial sums live in a file-scope device array of 1024
+ * doubles (8 KiB per device, held until the process ends); no call allocates
+ * or frees memory.  Every call on one device shares that array, so calls
+ * whose work overlaps on one device (two streams, two threads) race.
+ * Limits: dim 2 or 3, 2 <= res[a] <= 4096, fewer than 2^31 cells; anything
+ * else returns GARM_EINVAL with the reason in garm_create_error(). */
+int garm_volume_device(int dim, const int res[3], double dx, const double *phi_dev, int order, double *out_dev, void *stream);
 /* Blocks until all work queued on the context's stream is done. */
 int garm_synchronize(garm_ctx *ctx);
 
