@@ -400,6 +400,52 @@ struct Liquid {
 	// finish (Boundary.cpp:79-87) with getFaceFraction (Boundary.cpp:135-174).
 	void buildBoundary()
 	{
+		resetBoundary();
+		finishBoundary();
+	}
+
+	// The domain box with the solid half-space (x - pos).n < 0 added:
+	// Boundary::reset, unions(ImplicitPlane) and finish (Boundary.cpp:19-56,
+	// 79-87; ImplicitPlane, ImplicitSurface.h:111-126: normalised direction,
+	// signed distance (x - pos).n, closest normal n).
+	void solidPlane(const double *pos, const double *dir)
+	{
+		double n[3] = {0, 0, 0}, s = 0;
+		for (int a = 0; a < D; a++) s += dir[a] * dir[a];
+		s = std::sqrt(s);
+		for (int a = 0; a < D; a++) n[a] = s > 0 ? dir[a] / s : dir[a];
+		auto sd = [&](const double *p) {
+			double d = 0;
+			for (int a = 0; a < D; a++) d += (p[a] - pos[a]) * n[a];
+			return d;
+		};
+		resetBoundary();
+		parFor<D>(node, [&](const int *c) {
+			double p[3]; node.position(c, p);
+			const size_t id = node.index(c);
+			nodeDist[id] = std::min(nodeDist[id], sd(p));
+		});
+		parFor<D>(cell, [&](const int *c) {
+			double p[3]; cell.position(c, p);
+			const size_t id = cell.index(c);
+			cellDist[id] = std::min(cellDist[id], sd(p));
+		});
+		for (int ax = 0; ax < D; ax++) {
+			parFor<D>(face[ax], [&](const int *c) {
+				double p[3]; face[ax].position(c, p);
+				const size_t id = face[ax].index(c);
+				const double phi = sd(p);
+				if (phi < frac[ax][id]) {   // fraction holds a signed distance until finish
+					frac[ax][id] = phi;
+					bnormal[ax][id] = n[ax];
+				}
+			});
+		}
+		finishBoundary();
+	}
+
+	void resetBoundary()
+	{
 		double dorg[3], dlen[3], center[3], half[3];
 		for (int a = 0; a < D; a++) {
 			dorg[a] = origin[a] + bw * dx;                         // domainOrigin (StaggeredGrid.h:40)
@@ -424,6 +470,10 @@ struct Liquid {
 				bnormal[ax][id] = -nrm[ax];
 			});
 		}
+	}
+
+	void finishBoundary()
+	{
 		for (int ax = 0; ax < D; ax++) {
 			parFor<D>(face[ax], [&](const int *c) {
 				const size_t id = face[ax].index(c);
@@ -1326,6 +1376,7 @@ extern "C" {
 void ora_union_box(void *p, const double *mn, const double *len) { auto *h = (Handle *)p; DISPATCH(h, unionBox(L, mn, len)); }
 void ora_union_sphere(void *p, const double *c, double r) { auto *h = (Handle *)p; DISPATCH(h, unionSphere(L, c, r)); }
 void ora_union_plane(void *p, const double *pos, const double *n) { auto *h = (Handle *)p; DISPATCH(h, unionPlane(L, pos, n)); }
+void ora_solid_plane(void *p, const double *pos, const double *n) { auto *h = (Handle *)p; DISPATCH(h, L.solidPlane(pos, n)); }
 
 int ora_initialize(void *p, int reinitIters, double cfl, int precond, double tol, int maxIt)
 {

@@ -47,6 +47,7 @@ def lib():
             "ora_union_box": (None, [vp, dp, dp]),
             "ora_union_sphere": (None, [vp, dp, d]),
             "ora_union_plane": (None, [vp, dp, dp]),
+            "ora_solid_plane": (None, [vp, dp, dp]),
             "ora_initialize": (i, [vp, i, d, i, d, i]),
             "ora_advect_refmap": (None, [vp, d]),
             "ora_advect_velocity": (None, [vp, d]),
@@ -131,6 +132,11 @@ class Oracle:
 
     def union_plane(self, p, n):
         lib().ora_union_plane(self.h, _d3(p), _d3(n))
+
+    def solid_plane(self, p, n):
+        """Domain box plus the solid half-space (x - p).n < 0 (Boundary::unions
+        with an ImplicitPlane, then finish)."""
+        lib().ora_solid_plane(self.h, _d3(p), _d3(n))
 
     def initialize(self, reinit_iters=30, cfl=0.9, precond=PRECOND_JACOBI, tol=1e-10, max_it=20000):
         return lib().ora_initialize(self.h, reinit_iters, cfl, precond, tol, max_it)

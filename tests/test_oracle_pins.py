@@ -18,7 +18,10 @@ sign, index or transposition) fails:
   1e-3 clamp), discrete divergence after projection equals the solver
   residual, three preconditioners reach the same solution;
 * extrapolation: a single seed spreads as an L1 ball of the stated radius;
-* boundary enforcement: inflow removed, 0.2 of the tangential slip kept.
+* boundary enforcement: inflow removed, 0.2 of the tangential slip kept;
+* fourth-order gradient (Derivatives::computeFirst<4>, taken at restart and
+  initialisation): exact for polynomials of degree 4 per axis away from the
+  walls, and the clamped edge stencil by hand on a 6 x 5 grid.
 """
 import numpy as np
 import pytest
@@ -447,3 +450,162 @@ def test_check_refmap_detects_shear():
     o.set(REFMAP_GRAD, J)
     ok, tot2, cnt2 = o.check_refmap()
     assert not ok and cnt2 == tot2 == tot
+
+
+# ------------------------------------------------- computeFirst<4> ----
+def quartic_field(x):
+    """A polynomial of degree 4 in each coordinate and its gradient."""
+    X, Y = x[:, 0], x[:, 1]
+    Z = x[:, 2] if x.shape[1] == 3 else np.zeros_like(X)
+    f = X ** 4 - 2 * X ** 3 * Y + 3 * Y ** 4 + X ** 2 * Z ** 2 - Z ** 3 + 0.5 * Y * Z - 0.25 * Z ** 4 * X + 1.0
+    g = np.stack([4 * X ** 3 - 6 * X ** 2 * Y + 2 * X * Z ** 2 - 0.25 * Z ** 4,
+                  -2 * X ** 3 + 12 * Y ** 3 + 0.5 * Z,
+                  2 * X ** 2 * Z - 3 * Z ** 2 + 0.5 * Y - X * Z ** 3], axis=1)
+    return f, g[:, :x.shape[1]]
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_first4_exact_for_quartics(dim):
+    g, o = mk(14, dim)
+    x = scenes.cell_positions(g)
+    f, grad = quartic_field(x)
+    o.set(PHI, f)
+    o.restart()                     # phi_ref = phi, grad phi_ref = computeFirst<4>(phi_ref)
+    got = o.get(PHI_REF_GRAD).reshape(-1, dim)
+    m = interior_mask(g, 2)
+    assert np.max(np.abs(got[m] - grad[m])) <= 1e-10 * np.max(np.abs(grad[m]))
+    # next to a wall the clamped stencil is not exact any more
+    assert np.max(np.abs(got[~m] - grad[~m])) > 1e-3
+
+
+# values of (v0 - 8 v1 + 8 v3 - v4) / 12 with clamped indices, by hand, for
+# v = i^2 on a 6-cell x axis and v = 10 j on a 5-cell y axis (unit spacing)
+FIRST4_EDGE_X = [1 / 3, 23 / 12, 4.0, 6.0, 107 / 12, 14 / 3]
+FIRST4_EDGE_Y = [5.0, 130 / 12, 10.0, 130 / 12, 5.0]
+
+
+def test_first4_clamped_edges_by_hand():
+    o = O.Oracle(2, [6, 5], 1.0, [0.0, 0.0], 0)
+    i, j = np.meshgrid(np.arange(6), np.arange(5))      # x fastest after ravel
+    o.set(PHI, (i ** 2 + 10.0 * j).ravel())
+    o.restart()
+    got = o.get(PHI_REF_GRAD).reshape(5, 6, 2)
+    for jj in range(5):
+        assert np.allclose(got[jj, :, 0], FIRST4_EDGE_X, rtol=0, atol=1e-13)
+    for ii in range(6):
+        assert np.allclose(got[:, ii, 1], FIRST4_EDGE_Y, rtol=0, atol=1e-13)
+
+
+# ------------------------------------------- solid inside the domain ----
+def _square_solid_fraction(c0, c1, d):
+    """Fraction of the unit square {(s, t)} with c0 s + c1 t < d, exact."""
+    from test_volume_oracle import half_plane_area
+    return half_plane_area([c0, c1], d)
+
+
+def test_solid_plane_face_fractions_closed_form():
+    """Boundary::unions(ImplicitPlane) + finish: the node distances are
+    linear, so getFaceFraction's four triangles give the exact solid area of
+    each face (set to 1 above 0.9), and every face normal is the plane's."""
+    g, o = mk(12, 3)
+    p, n = np.array([0.0, 0.3, 0.0]), np.array([-0.3, 1.0, 0.2])
+    o.solid_plane(p, n)
+    nn = n / np.linalg.norm(n)
+    fr, nr = face_split(g, o.get(FRAC)), face_split(g, o.get(NORMAL))
+    dx, bw = g["dx"], g["bw"]
+    partial = 0
+    for a in range(3):
+        pos = scenes.face_positions(g, a)
+        cnt = [r + (1 if b == a else 0) for b, r in enumerate(g["res"])]
+        idx = np.indices(cnt[::-1]).reshape(3, -1)[::-1]
+        wall = (idx[a] <= bw) | (idx[a] >= g["res"][a] - bw)
+        for b in range(3):
+            wall |= (idx[b] < bw) | (idx[b] >= cnt[b] - bw)
+        u, v = (a + 1) % 3, (a + 2) % 3
+        for k in np.nonzero(~wall)[0]:
+            # face square: corner at pos - dx/2 along u and v
+            lo = pos[k].copy()
+            lo[u] -= dx / 2
+            lo[v] -= dx / 2
+            nodes = [lo + s * dx * np.eye(3)[u] + t * dx * np.eye(3)[v] for s in (0, 1) for t in (0, 1)]
+            if any(np.dot(q - p, nn) > np.min(np.minimum(q, 1 - q)) for q in nodes):
+                continue   # a wall is nearer than the plane at a node: the distance is not linear there
+            d = float(np.dot(p - lo, nn)) / dx       # solid: nn_u s + nn_v t < d (s, t in [0, 1])
+            want = _square_solid_fraction(nn[u], nn[v], d)
+            want = 1.0 if want > 0.9 else want
+            assert fr[a][k] == pytest.approx(want, abs=1e-12), (a, k)
+            partial += 0 < want < 1
+            if np.dot(pos[k] - p, nn) < np.min(np.minimum(pos[k], 1 - pos[k])):
+                assert nr[a][k] == pytest.approx(nn[a], abs=1e-15)
+    assert partial > 60
+
+
+def test_projection_partial_fractions_and_solid_velocity_by_hand():
+    """One liquid cell with faces of fraction 0.25, 0, 0.6 and 1 (-x, +x, -y,
+    +y), air around it at theta = 1/2, and a moving solid: Projector.cpp:30-68
+    by hand.  rhs = -0.75*1 - 0.25*0.4 + 1*2 - 0.4*(-0.5) - 0.6*(-2) + 1*0.8
+    = 3.35, diag = (0.75 + 1 + 0.4) * 2 = 4.3."""
+    o = O.Oracle(2, [5, 5], 1.0, [0.0, 0.0], 1)
+    p = hand_projection(o, "oracle")
+    assert p == pytest.approx(3.35 / 4.3, rel=1e-14)
+
+
+def hand_projection(sim, kind):
+    """Sets the single-cell scene of the hand projection on `sim` (oracle or
+    GPU), projects, checks the faces and returns the cell's pressure."""
+    phi = np.full(25, 0.5)
+    cid = 2 + 5 * 2
+    phi[cid] = -0.5
+    fx, fy = np.zeros(30), np.zeros(30)          # x faces 6 x 5, y faces 5 x 6
+    ux, uy = np.zeros(30), np.zeros(30)
+    sx, sy = np.zeros(30), np.zeros(30)
+    fx[14], ux[14], sx[14] = 0.25, 1.0, 0.4      # -x face of cell (2, 2)
+    fx[15], ux[15], sx[15] = 0.0, 2.0, 7.0       # +x (solid velocity unused: weight 1)
+    fy[12], uy[12], sy[12] = 0.6, -0.5, -2.0     # -y
+    fy[17], uy[17], sy[17] = 1.0, 3.0, 0.8       # +y (a wall: weight 0)
+    sim.set(PHI, phi)
+    sim.set(FRAC, np.concatenate([fx, fy]))
+    sim.set(BVEL, np.concatenate([sx, sy]))
+    sim.set(VEL, np.concatenate([ux, uy]))
+    if kind == "oracle":
+        r = sim.project(precond=O.PRECOND_JACOBI, tol=1e-14)
+    else:
+        r = sim.project(tol=1e-14)
+    assert r["unknowns"] == 1
+    p = sim.get(PRES)[cid]
+    u = sim.get(VEL)
+    nux, nuy = u[:30], u[30:]
+    # Projector.cpp:91-123: one unknown side, theta = 1/2 on every face
+    assert nux[15] == pytest.approx(2.0 - 2 * p, abs=1e-14)
+    assert nux[14] == pytest.approx(1.0 + 2 * p, abs=1e-14)
+    assert nuy[12] == pytest.approx(-0.5 + 2 * p, abs=1e-14)
+    assert nuy[17] == pytest.approx(3.0 - 2 * p, abs=1e-14)
+    return p
+
+
+def hand_enforce(sim):
+    """Boundary::enforce with uniform normal (0.6, 0.8), solid velocity
+    (0.5, 0.25) and fluid velocity (1, -2): vn = -1.5 < 0, vt = (1.4, -1.05),
+    so a face with 0.1 < fraction <= 1 becomes vel - vn n - 0.8 vt =
+    (0.78, 0.04) in its axis; fractions at or below 0.1 are left alone."""
+    fr = sim.get(FRAC).copy()
+    nxf = 17 * 16
+    fr[8 + 17 * 8] = 0.5          # interior x face, partial
+    fr[9 + 17 * 8] = 0.05         # interior x face, below 0.1
+    fr[nxf + 8 + 16 * 8] = 0.35   # interior y face, partial
+    sim.set(FRAC, fr)
+    sim.set(NORMAL, np.concatenate([np.full(nxf, 0.6), np.full(16 * 17, 0.8)]))
+    sim.set(BVEL, np.concatenate([np.full(nxf, 0.5), np.full(16 * 17, 0.25)]))
+    sim.set(VEL, np.concatenate([np.full(nxf, 1.0), np.full(16 * 17, -2.0)]))
+    sim.enforce_boundary()
+    u = sim.get(VEL)
+    hit = fr > 0.1
+    want = np.where(hit, np.concatenate([np.full(nxf, 0.78), np.full(16 * 17, 0.04)]),
+                    np.concatenate([np.full(nxf, 1.0), np.full(16 * 17, -2.0)]))
+    assert np.max(np.abs(u - want)) <= 1e-14
+    assert hit[8 + 17 * 8] and hit[nxf + 8 + 16 * 8] and not hit[9 + 17 * 8]
+
+
+def test_enforce_partial_fraction_branch_by_hand():
+    g, o = mk(16, 2)
+    hand_enforce(o)

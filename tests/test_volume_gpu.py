@@ -4,6 +4,10 @@ Random phi_tmp fields reach all 256 corner patterns of the 3D fraction and
 both diagonal rules of the 2D contour; the GPU sums in a different order,
 so the bar is 1e-12 relative.  A free trajectory of the plain FP64 path
 must keep the oracle's volume drift within 0.1% of the initial volume.
+The ambiguous corner configurations are also checked on the GPU directly
+against the closed forms of the marching-cubes geometry (mc_geometry.py),
+one cube at a time, since the GPU's case table and the oracle's are two
+transcriptions of the same data.
 """
 import numpy as np
 import pytest
@@ -11,6 +15,7 @@ import pytest
 import oracle as O
 from garm_amd import capi, scenes
 from garm_amd.driver import FrameClock
+import mc_geometry as G
 from helpers import copy_state, make_gpu, make_oracle, say
 
 pytestmark = pytest.mark.gpu
@@ -66,3 +71,48 @@ def test_volume_drift_of_free_trajectory_matches_oracle(dim, n, steps):
         worst = max(worst, abs(drift_d - drift_o))
     say(f"  final drift oracle {drift_o:.3e} gpu {drift_d:.3e}; worst difference {worst:.2e}")
     assert worst <= 1e-3
+
+
+def _cube_volume(d, v):
+    """GPU volume of one cube with circular-order corner values v (a 2x2x2
+    grid of cells, unit spacing, corrected order)."""
+    lex = np.empty(8)
+    for c, (x, y, z) in enumerate(G.CIRC):
+        lex[x + 2 * y + 4 * z] = v[c]
+    d.set(capi.PHI_TMP, lex)
+    return d.volume(corrected=True)
+
+
+def test_ambiguous_cases_closed_form_on_gpu():
+    say("volume: ambiguous corner configurations, closed forms, one cube per call")
+    d = capi.Liquid(3, [2, 2, 2], 1.0, [0.0, 0.0, 0.0])
+    rng = np.random.default_rng(21)
+    worst = 0.0
+    for m in range(256):
+        if not G.corners_and_edges(m):
+            continue
+        for _ in range(3):
+            alpha = rng.uniform(0.05, 5.0, 8)
+            v = [-alpha[i] if m >> i & 1 else 1.0 for i in range(8)]
+            worst = max(worst, abs(_cube_volume(d, v) - G.separated_volume(m, alpha)))
+    for a, b in G.BODY_DIAGONALS:
+        ba, bb = rng.uniform(0.05, 5.0, 2)
+        v = [-1.0] * 8
+        v[a], v[b] = ba, bb
+        worst = max(worst, abs(_cube_volume(d, v) - G.body_diagonal_complement_volume(ba, bb)))
+    say(f"  worst deviation {worst:.3e}")
+    assert worst <= 1e-14
+    # rotation invariance of the configurations that have it, on the GPU
+    R = G.rotations()
+    from test_volume_oracle import NOT_ROTATION_INVARIANT
+    for m in range(256):
+        if G.orbit_class(m) in NOT_ROTATION_INVARIANT:
+            continue
+        mag = rng.uniform(0.05, 1.0, 8)
+        v = np.where([m >> i & 1 for i in range(8)], -mag, mag)
+        want = _cube_volume(d, v)
+        for p in R[1::5]:
+            w = np.empty(8)
+            w[p] = v
+            assert abs(_cube_volume(d, w) - want) <= 1e-15, (m, p)
+    d.close()

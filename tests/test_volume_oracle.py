@@ -9,8 +9,16 @@ non-ambiguous case formulas that planes reach); the reference's vertex-order
 quirk (plane x = 1/2 gives 0.25, corrected 0.5); and convergence of a sphere
 to 4/3 pi r^3.  2D filled contour: half-planes against the closed form, the
 two diagonal configurations by hand, and a disk converging to pi r^2.
-The ambiguous 3D cases (those no plane produces) are pinned only by the
-restatement itself and by the GPU parity tests on random fields.
+Ambiguous 3D configurations (sign patterns no plane produces) are pinned by
+closed forms of the marching-cubes geometry (mc_geometry.py): separated
+corners and edges, a plane-cut component next to separated corners, two
+outside corners on a body diagonal; and by invariance under the 24 rotations
+of the cube.  Four classes stay pinned only by the restatement: the
+four-corner zigzag chains (the canonical cases with signed-volume fans that
+no plane reaches) and the two complement configurations whose outside
+corners the surface joins by a tunnel (an outside edge plus a corner, two
+outside corners on a face diagonal); their volumes depend on the fixed
+triangulation of a skew polygon, which no geometric identity fixes.
 """
 import itertools
 import math
@@ -18,6 +26,7 @@ import math
 import numpy as np
 import pytest
 
+import mc_geometry as G
 import oracle as O
 from garm_amd import scenes
 
@@ -146,3 +155,120 @@ def test_disk_area_converges():
         errs.append(abs(o.volume() - math.pi * r * r) / (math.pi * r * r))
     assert errs[1] < 2e-3
     assert errs[1] < errs[0] / 2.5
+
+
+# ------------------------------------------------- ambiguous 3D cases ----
+def _plane_for(pattern, rng, tries=200000):
+    """A plane n.x < d whose inside corners are exactly `pattern`, with every
+    corner value at least 0.05 away from 0 and no tiny normal component."""
+    P = np.array(CIRC, dtype=np.float64)
+    for _ in range(tries):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        if np.min(np.abs(n)) < 0.1:
+            continue
+        d = rng.uniform(-0.2, 1.9) * np.sum(np.abs(n)) - np.sum(np.abs(n) * (n < 0))
+        v = P @ n - d
+        if np.min(np.abs(v)) < 0.05:
+            continue
+        if sum(1 << i for i in range(8) if v[i] < 0) == pattern:
+            return n, d, v
+    return None
+
+
+def test_separated_corners_and_edges_closed_form():
+    """Every pattern of isolated inside corners and edges (two corners on a
+    face or body diagonal, edge + corner, three or four separated corners,
+    two separated edges): volume = sum of corner tetrahedra and edge wedges."""
+    rng = np.random.default_rng(11)
+    patterns = [m for m in range(256) if G.corners_and_edges(m)]
+    ambiguous = [m for m in patterns if len(G.components(G.inside_set(m))) > 1]
+    assert len(patterns) == 76 and len(ambiguous) == 56
+    worst = 0.0
+    for m in patterns:
+        for _ in range(20):
+            alpha = rng.uniform(0.05, 5.0, 8)
+            v = [-alpha[i] if m >> i & 1 else 1.0 for i in range(8)]
+            worst = max(worst, abs(O.mc_fraction(v) - G.separated_volume(m, alpha)))
+    assert worst <= 1e-14
+    # symmetric values by hand: every crossing at the middle of its edge
+    # (corner 1/48, edge 1/8): body diagonal {0, 6}, face diagonal {0, 5},
+    # edge + corner {0, 1, 6}, three corners {1, 3, 4}, four corners
+    # {0, 2, 5, 7}, two edges {0, 1} and {6, 7}
+    for ins, want in (((0, 6), 2 / 48), ((0, 5), 2 / 48), ((0, 1, 6), 1 / 8 + 1 / 48), ((1, 3, 4), 3 / 48),
+                      ((0, 2, 5, 7), 4 / 48), ((0, 1, 6, 7), 2 / 8)):
+        v = [-1.0 if i in ins else 1.0 for i in range(8)]
+        assert O.mc_fraction(v) == pytest.approx(want, abs=1e-15), ins
+
+
+def test_plane_component_next_to_separated_corners():
+    """An edge, a three-corner L or a face cut by a plane, plus inside corners
+    that touch it through no edge: volume = the plane's half-space volume plus
+    the corners' tetrahedra (legs from the actual edge values)."""
+    rng = np.random.default_rng(12)
+    seen = {}
+    for m in range(256):
+        cs = G.components(G.inside_set(m))
+        big = [c for c in cs if len(c) > 1]
+        iso = [c[0] for c in cs if len(c) == 1]
+        if len(cs) < 2 or len(big) != 1:
+            continue
+        K = sum(1 << i for i in big[0])
+        worst = 0.0
+        for _ in range(8):
+            plane = _plane_for(K, rng)
+            assert plane is not None, f"no plane reaches {K}"
+            n, d, v = plane
+            alpha = rng.uniform(0.1, 3.0, 8)
+            for c in iso:
+                v[c] = -alpha[c]
+            want = half_space_volume(n, d) + sum(np.prod([alpha[c] / (alpha[c] + v[o]) for o in G.ADJ[c]]) / 6.0 for c in iso)
+            worst = max(worst, abs(O.mc_fraction(list(v)) - want))
+        seen[G.orbit_class(m)] = max(seen.get(G.orbit_class(m), 0.0), worst)
+    # edge + corner (24 patterns), L + corner (24), face + corner (8)
+    assert sorted(k[:2] for k in seen) == [(3, (1, 2)), (4, (1, 3)), (5, (1, 4))]
+    assert max(seen.values()) <= 1e-13
+
+
+def test_two_outside_corners_on_a_body_diagonal():
+    rng = np.random.default_rng(13)
+    for a, b in G.BODY_DIAGONALS:
+        for _ in range(20):
+            ba, bb = rng.uniform(0.05, 5.0, 2)
+            v = [-1.0] * 8
+            v[a], v[b] = ba, bb
+            assert O.mc_fraction(v) == pytest.approx(G.body_diagonal_complement_volume(ba, bb), abs=1e-14)
+
+
+# orbit classes (orbit_class) whose canonical formula fixes a diagonal of a
+# skew quadrilateral or hexagon: a rotation of the data may pick the other
+# diagonal, so their volume is not rotation invariant for general values
+NOT_ROTATION_INVARIANT = {
+    (4, (2, 2), (2, 2), 0, 0),        # two separated edges
+    (4, (4,), (4,), 0, 0),            # four connected corners (face, star, zigzags)
+    (5, (1, 4), (1, 1, 1), 0, 0),     # face + separated corner
+    (6, (6,), (1, 1), 0, 2),          # two outside corners on a face diagonal
+}
+
+
+def test_rotation_invariance():
+    """The volume of every other configuration is unchanged when the cube is
+    rotated (24 rotations); the dispatch and rotation tables are checked for
+    all 256 patterns this way."""
+    rng = np.random.default_rng(14)
+    R = G.rotations()
+    assert len(R) == 24 and len({tuple(p) for p in R}) == 24
+    checked = 0
+    for m in range(256):
+        if G.orbit_class(m) in NOT_ROTATION_INVARIANT:
+            continue
+        checked += 1
+        for _ in range(4):
+            mag = rng.uniform(0.05, 1.0, 8)
+            v = np.where([m >> i & 1 for i in range(8)], -mag, mag)
+            want = O.mc_fraction(list(v))
+            for p in R:
+                w = np.empty(8)
+                w[p] = v
+                assert abs(O.mc_fraction(list(w)) - want) <= 1e-15, (m, p)
+    assert checked == 256 - 6 - 38 - 8 - 12
