@@ -1,15 +1,14 @@
 // Ghost-fluid pressure projection for gfx950 (Projector.cpp:13-163).
 //
 // Unknowns are numbered in the reference's lexicographic order
-// (Projector::setUnknowns, Projector.cpp:125-156) by a device prefix sum.
+// (Projector::setUnknowns, Projector.cpp:125-156) by a three-pass device
+// prefix sum over tiles of consecutive cells.
 // The 5/7-point operator is kept as one diagonal plus 2D (neighbour, weight)
 // pairs per unknown, structure-of-arrays, and applied matrix-free: no sparse
 // matrix is assembled and no factorisation is formed.  The conjugate-gradient
 // scalars (alpha, beta, r.z, |r|^2) never leave the device; the host only
 // polls a convergence flag every few iterations.
 #include "garm_project.hpp"
-
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cfloat>
@@ -19,30 +18,98 @@
 namespace garm {
 
 // ------------------------------------------------------------- unknowns
-template <int D>
-__global__ __launch_bounds__(kBlock) void kUnknownFlags(GridDev g, FaceFracs fr, const double *phi, int *flags)
+// Cells are numbered in tiles of kScanTile consecutive ids (x fastest), one
+// workgroup per tile: pass 1 flags the unknowns and counts them per tile,
+// pass 2 turns the tile counts into tile offsets (one workgroup), pass 3
+// numbers each tile's unknowns in order.  Tiles and cells within a tile are
+// taken in id order, so the numbering is Projector::setUnknowns' forEach order.
+constexpr int kScanSub = 16;                   // passes of kBlock cells per tile
+constexpr int kScanTile = kBlock * kScanSub;
+
+// Exclusive scan of one int per thread over a 256-thread workgroup; *total
+// receives the workgroup sum.
+__device__ __forceinline__ int blockExclusiveScan(int v, int *total)
 {
-	const Coord c = threadCoord();
-	if (!inGrid<D>(g, c)) return;
-	const size_t id = gidx<D>(g, c.i, c.j, c.k);
-	int flag = 0;
-	if (phi[id] <= 0) {
+	__shared__ int waveSum[kBlock / kWave];
+	const int tid = int(threadIdx.x), lane = tid & (kWave - 1), wid = tid / kWave;
+	int x = v;
 #pragma unroll
-		for (int o = 0; o < 2 * D; o++) {
-			const int ax = o >> 1, up = o & 1;
-			const GridDev &fg = fr.g[ax];
-			const size_t f = gidx<D>(fg, c.i + (ax == 0 ? up : 0), c.j + (ax == 1 ? up : 0), c.k + (ax == 2 ? up : 0));
-			if (fr.a[ax][f] < 1) { flag = 1; break; }
-		}
+	for (int off = 1; off < kWave; off <<= 1) {
+		const int y = __shfl_up(x, off, kWave);
+		if (lane >= off) x += y;
 	}
-	flags[id] = flag;
+	if (lane == kWave - 1) waveSum[wid] = x;
+	__syncthreads();
+	int before = 0, all = 0;
+#pragma unroll
+	for (int w = 0; w < kBlock / kWave; w++) {
+		const int t = waveSum[w];
+		before += w < wid ? t : 0;
+		all += t;
+	}
+	__syncthreads();   // waveSum is reused by the next call
+	*total = all;
+	return before + x - v;
 }
 
-__global__ void kCompactUnknowns(const int *flags, int *grid2mat, int *mat2grid, size_t n)
+template <int D>
+__global__ __launch_bounds__(kBlock) void kUnknownFlags(GridDev g, FaceFracs fr, const double *phi, uint8_t *flags, int *tileCount, size_t N)
 {
-	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
-		if (flags[i]) mat2grid[grid2mat[i]] = int(i);
-		else grid2mat[i] = -1;
+	const size_t base = size_t(blockIdx.x) * kScanTile;
+	int cnt = 0;
+	for (int sub = 0; sub < kScanSub; sub++) {
+		const size_t id = base + size_t(sub) * kBlock + threadIdx.x;
+		if (id >= N) break;
+		int flag = 0;
+		if (phi[id] <= 0) {
+			const int i = int(id % size_t(g.n[0]));
+			const int j = D == 2 ? int(id / size_t(g.n[0])) : int((id / size_t(g.n[0])) % size_t(g.n[1]));
+			const int k = D == 2 ? 0 : int(id / (size_t(g.n[0]) * g.n[1]));
+#pragma unroll
+			for (int o = 0; o < 2 * D; o++) {
+				const int ax = o >> 1, up = o & 1;
+				const GridDev &fg = fr.g[ax];
+				const size_t f = gidx<D>(fg, i + (ax == 0 ? up : 0), j + (ax == 1 ? up : 0), k + (ax == 2 ? up : 0));
+				if (fr.a[ax][f] < 1) { flag = 1; break; }
+			}
+		}
+		flags[id] = uint8_t(flag);
+		cnt += flag;
+	}
+	int total;
+	(void)blockExclusiveScan(cnt, &total);
+	if (threadIdx.x == 0) tileCount[blockIdx.x] = total;
+}
+
+// tileBase[t] = unknowns in tiles before t; *count = all unknowns
+__global__ __launch_bounds__(kBlock) void kScanTiles(const int *tileCount, int tiles, int *tileBase, int *count)
+{
+	int carry = 0;
+	for (int t0 = 0; t0 < tiles; t0 += kBlock) {
+		const int t = t0 + int(threadIdx.x);
+		const int v = t < tiles ? tileCount[t] : 0;
+		int total;
+		const int ex = blockExclusiveScan(v, &total);
+		if (t < tiles) tileBase[t] = carry + ex;
+		carry += total;
+	}
+	if (threadIdx.x == 0) *count = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void kNumberUnknowns(const uint8_t *flags, const int *tileBase, int *grid2mat, int *mat2grid, size_t N)
+{
+	const size_t base = size_t(blockIdx.x) * kScanTile;
+	int carry = tileBase[blockIdx.x];
+	for (int sub = 0; sub < kScanSub; sub++) {
+		const size_t id = base + size_t(sub) * kBlock + threadIdx.x;
+		const int flag = id < N ? flags[id] : 0;
+		int total;
+		const int idx = carry + blockExclusiveScan(flag, &total);
+		if (id < N) {
+			grid2mat[id] = flag ? idx : -1;
+			if (flag) mat2grid[idx] = int(id);
+		}
+		carry += total;
 	}
 }
 
@@ -162,7 +229,7 @@ Projection::~Projection() { release(); }
 void Projection::release()
 {
 	auto fr = [](auto *&p) { if (p) { (void)hipFree(p); p = nullptr; } };
-	fr(flags); fr(grid2mat); fr(mat2grid); fr(scan_tmp);
+	fr(flags); fr(grid2mat); fr(mat2grid); fr(tile_count); fr(tile_base);
 	for (int o = 0; o < 6; o++) { fr(rows.nb[o]); fr(rows.w[o]); fr(rows32.w[o]); }
 	fr(rows.diag); fr(rows.invdiag); fr(rhs); fr(rows32.diag); fr(rows32.invdiag);
 	for (int o = 0; o < 6; o++) rows32.nb[o] = nullptr;
@@ -173,14 +240,12 @@ void Projection::release()
 bool Projection::reserveGrid(size_t N, std::string &err)
 {
 	if (flags) return true;
-	if (hipMalloc(&flags, N * sizeof(int)) != hipSuccess || hipMalloc(&grid2mat, N * sizeof(int)) != hipSuccess) {
+	tiles = int((N + kScanTile - 1) / kScanTile);
+	if (hipMalloc(&flags, N) != hipSuccess || hipMalloc(&grid2mat, N * sizeof(int)) != hipSuccess ||
+		hipMalloc(&tile_count, size_t(tiles) * sizeof(int)) != hipSuccess || hipMalloc(&tile_base, size_t(tiles + 1) * sizeof(int)) != hipSuccess) {
 		err = "hipMalloc failed for unknown maps";
 		return false;
 	}
-	size_t bytes = 0;
-	if (hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, flags, grid2mat, int(N)) != hipSuccess) { err = "scan sizing failed"; return false; }
-	scan_bytes = bytes;
-	if (hipMalloc(&scan_tmp, bytes ? bytes : 16) != hipSuccess) { err = "hipMalloc failed for scan storage"; return false; }
 	return true;
 }
 
@@ -219,16 +284,18 @@ static FaceFracs faceArrays(const Fields &F, double *const *a)
 template <int D>
 static int numberUnknowns(const Fields &F, Projection &P, bool mixed, hipStream_t s, std::string &err)
 {
-	hipLaunchKernelGGL(kUnknownFlags<D>, gridDims(F.cell, D), gridBlock(), 0, s, F.cell, faceArrays(F, F.frac), (const double *)F.phi, P.flags);
-	size_t bytes = P.scan_bytes;
-	if (hipcub::DeviceScan::ExclusiveSum(P.scan_tmp, bytes, P.flags, P.grid2mat, int(F.N), s) != hipSuccess) { err = "prefix sum failed"; return -1; }
-	int *last = F.host_ints;
-	if (hipMemcpyAsync(&last[0], P.grid2mat + F.N - 1, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
-		hipMemcpyAsync(&last[1], P.flags + F.N - 1, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
-		spinWait(s, F.wait) != hipSuccess) { err = "reading the unknown count failed"; return -1; }
-	const int n = last[0] + last[1];
+	hipLaunchKernelGGL(kUnknownFlags<D>, dim3(P.tiles), dim3(kBlock), 0, s, F.cell, faceArrays(F, F.frac), (const double *)F.phi, P.flags,
+		P.tile_count, F.N);
+	hipLaunchKernelGGL(kScanTiles, dim3(1), dim3(kBlock), 0, s, (const int *)P.tile_count, P.tiles, P.tile_base, P.tile_base + P.tiles);
+	int *cnt = F.host_ints;
+	if (hipMemcpyAsync(cnt, P.tile_base + P.tiles, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess || spinWait(s, F.wait) != hipSuccess) {
+		err = "reading the unknown count failed";
+		return -1;
+	}
+	const int n = cnt[0];
 	if (!P.reserveRows(D, size_t(n), mixed, err)) return -1;
-	hipLaunchKernelGGL(kCompactUnknowns, dim3(linBlocks(F.N)), dim3(kBlock), 0, s, (const int *)P.flags, P.grid2mat, P.mat2grid, F.N);
+	hipLaunchKernelGGL(kNumberUnknowns, dim3(P.tiles), dim3(kBlock), 0, s, (const uint8_t *)P.flags, (const int *)P.tile_base, P.grid2mat,
+		P.mat2grid, F.N);
 	return n;
 }
 

@@ -17,9 +17,10 @@ struct FaceFracs {
 };
 
 struct Projection {
-	int *flags = nullptr, *grid2mat = nullptr, *mat2grid = nullptr;
-	void *scan_tmp = nullptr;
-	size_t scan_bytes = 0;
+	uint8_t *flags = nullptr;                  // 1 where a cell is an unknown
+	int *grid2mat = nullptr, *mat2grid = nullptr;
+	int *tile_count = nullptr, *tile_base = nullptr;   // unknown numbering: per-tile counts, offsets (+ total)
+	int tiles = 0;
 	Rows<double> rows{};
 	Rows<float> rows32{};
 	double *rhs = nullptr;
