@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <limits>
 #include <string>
 #include <vector>
@@ -572,6 +573,59 @@ int garm_apply_pressure_system(garm_ctx *c, const double *x, double *b)
 	applySystemCells(F, c->P, F.scratch[0], F.scratch[1], c->s);
 	if ((rc = launched(c, "pressure operator")) || (rc = hipCheck(c, hipStreamSynchronize(c->s), "synchronize"))) return rc;
 	return hipCheck(c, hipMemcpy(b, F.scratch[1], F.N * sizeof(double), hipMemcpyDeviceToHost), "download");
+}
+
+// The .sav files of LevelSetLiquid::saveFrame / loadFrame
+// (LevelSetLiquid.cpp:425-526) in the default build (reference map, gradient
+// augmentation, Newton): headerless raw arrays written by IO::writeArray
+// (IO.h:26) in the fields' array-of-structures layout (GridBasedData.h:102-103;
+// the face arrays of SGridBasedData one axis after the other,
+// SGridBasedData.h:54-55).
+namespace {
+struct SavFile {
+	const char *name;
+	int field;
+};
+constexpr SavFile kSavFiles[] = {
+	{"velocity.sav", GARM_VELOCITY}, {"refMap.sav", GARM_REFMAP}, {"refLevelSet.sav", GARM_PHI_REF},
+	{"advRegion.sav", GARM_ADV_REGION}, {"refMapGrad.sav", GARM_REFMAP_GRAD}, {"refLevelSetGrad.sav", GARM_PHI_REF_GRAD},
+};
+size_t elemBytes(int field) { return field == GARM_ADV_REGION ? 1 : sizeof(double); }
+} // namespace
+
+int garm_save_frame(garm_ctx *c, const char *frame_dir)
+{
+	if (!c || !frame_dir) return GARM_EINVAL;
+	for (const SavFile &f : kSavFiles) {
+		const size_t n = fieldCount(c->F, f.field);
+		std::vector<unsigned char> buf(n * elemBytes(f.field));
+		int rc = garm_download(c, f.field, buf.data(), n);
+		if (rc) return rc;
+		const std::string path = std::string(frame_dir) + "/" + f.name;
+		std::ofstream out(path, std::ios::binary);
+		if (!out || !out.write(reinterpret_cast<const char *>(buf.data()), std::streamsize(buf.size())))
+			return fail(c, GARM_EINVAL, "cannot write " + path);
+	}
+	return GARM_OK;
+}
+
+int garm_load_frame(garm_ctx *c, const char *frame_dir)
+{
+	if (!c || !frame_dir) return GARM_EINVAL;
+	for (const SavFile &f : kSavFiles) {
+		const size_t n = fieldCount(c->F, f.field);
+		std::vector<unsigned char> buf(n * elemBytes(f.field));
+		const std::string path = std::string(frame_dir) + "/" + f.name;
+		std::ifstream in(path, std::ios::binary | std::ios::ate);
+		if (!in) return fail(c, GARM_EINVAL, "cannot read " + path);
+		if (size_t(in.tellg()) != buf.size())
+			return fail(c, GARM_EINVAL, path + " holds " + std::to_string(size_t(in.tellg())) + " bytes, the grid needs " + std::to_string(buf.size()));
+		in.seekg(0);
+		if (!in.read(reinterpret_cast<char *>(buf.data()), std::streamsize(buf.size()))) return fail(c, GARM_EINVAL, "cannot read " + path);
+		int rc = garm_upload(c, f.field, buf.data(), n);
+		if (rc) return rc;
+	}
+	return GARM_OK;
 }
 
 int garm_set_velocity_field(garm_ctx *c, int field, double time, double period)

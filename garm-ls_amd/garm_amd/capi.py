@@ -118,6 +118,8 @@ def lib():
             "garm_volume_device": (i, [i, ip, d, vp, i, vp, vp]),
             "garm_set_precision": (i, [vp, i]),
             "garm_set_velocity_field": (i, [vp, i, d, d]),
+            "garm_save_frame": (i, [vp, ctypes.c_char_p]),
+            "garm_load_frame": (i, [vp, ctypes.c_char_p]),
             "garm_synchronize": (i, [vp]),
             "garm_time_kernel": (i, [vp, i, i, dp, dp]),
             "garm_unet_create": (i, [ctypes.POINTER(vp), ctypes.c_ulonglong]),
@@ -273,6 +275,14 @@ class Liquid:
     def set_velocity_field(self, field, time, period):
         """Analytic face velocity (VELOCITY_ENRIGHT) at time t with period T."""
         self._check(lib().garm_set_velocity_field(self.h, field, time, period))
+
+    def save_frame(self, frame_dir):
+        """Writes the reference's .sav files of this state into frame_dir."""
+        self._check(lib().garm_save_frame(self.h, os.fsencode(frame_dir)))
+
+    def load_frame(self, frame_dir):
+        """Reads the reference's .sav files from frame_dir into this state."""
+        self._check(lib().garm_load_frame(self.h, os.fsencode(frame_dir)))
 
     def set_precision(self, precision):
         """FP64 or FP32 compute for the per-operator advection / reinit calls."""

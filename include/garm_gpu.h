@@ -212,6 +212,19 @@ typedef enum garm_velocity_field {
 	                                w = -sin^2(pi z) sin(2 pi x) sin(2 pi y) cos(pi t / T) */
 } garm_velocity_field;
 
+/* LevelSetLiquid::saveFrame / loadFrame (LevelSetLiquid.cpp:425-526) of the
+ * default build: writes / reads velocity.sav, refMap.sav, refLevelSet.sav,
+ * advRegion.sav (uint8), refMapGrad.sav and refLevelSetGrad.sav in
+ * frame_dir, headerless raw arrays in the reference's layout (IO::writeArray,
+ * IO.h:26; GridBasedData.h:102-103, SGridBasedData.h:54-55), so files
+ * written by either program load in the other.  These fields determine the
+ * next step: every other field is recomputed from them within garm_step.
+ * Loading checks each file's size against the grid (GARM_EINVAL on a
+ * mismatch); the boundary is the context's own, as the reference keeps its
+ * Boundary and only calls finish (LevelSetLiquid.cpp:477). */
+int garm_save_frame(garm_ctx *ctx, const char *frame_dir);
+int garm_load_frame(garm_ctx *ctx, const char *frame_dir);
+
 /* Sets the face velocity of a 3D context to an analytic field at time t
  * with period T, on the device (the reference's tests evaluate it per
  * sample, TestAdvect3D.cpp:152-190, 390-397). */
