@@ -10,15 +10,22 @@ sphere r = 0.08 at a seeded (x, 0.7, z), falling at 2).
 A step is one LevelSetLiquid::advance (LevelSetLiquid.cpp:608-698) with every
 field resident in HBM.
 
-Three configurations of the same workload run one after the other:
-  fp64     plain path: FP64 advection / reinit, FP64 Jacobi PCG;
-  mixed    FP32 advection / reinit, FP32 Jacobi PCG refined in FP64;
-  learned  FP32 advection / reinit, flexible CG preconditioned by the U-Net
-           (falling back to the mixed solver), the network trained first,
-           untimed, on (divergence, pressure) pairs of 128^3 dam breaks with
-           seeded box extents.
+Configurations of the same workload, run one after the other:
+  fp64            plain path: FP64 advection / reinit, FP64 Jacobi PCG;
+  mixed           FP32 advection / reinit, FP32 Jacobi PCG refined in FP64;
+  multigrid       FP32 advection / reinit, flexible CG with an algebraic
+                  multigrid V-cycle (FP32 cycle, FP64 Krylov vectors);
+  learned-jacobi  FP32 advection / reinit, flexible CG preconditioned by the
+                  U-Net, the mixed Jacobi PCG taking over when it stalls;
+  learned         the same with the V-cycle taking over (the headline).
+The network is trained first, untimed, online on the device: 128^3 dam
+breaks with seeded box extents and drop positions U[0.2, 0.8], a few Adam
+steps after every step on pairs made from that step's own pressure system
+(its (divergence, pressure) pair and (A e, e) pairs of smooth random e).
 The headline value is --mode (default learned); the others are reported
 beside it with their speed-up over fp64 and their volume drift against it.
+After the timed steps each configuration runs on to --trajectory steps (the
+workload's 300), timed per step, and the whole trajectory is summarised.
 
 Prints one JSON line: cell-updates/s over the timed steps, per-phase times,
 the roofline position of the dominant kernel (timed live with HIP events on
@@ -59,6 +66,20 @@ MODES = {"fp64": (capi.PCG_FP64, capi.FP64, 0), "mixed": (capi.PCG_MIXED, capi.F
          "multigrid": (capi.PCG_MULTIGRID, capi.FP32, 0)}
 
 
+# what each configuration's pressure solve is, as the JSON line states it
+PRECONDITIONER = {
+    "fp64": "FP64 Jacobi PCG",
+    "mixed": "FP32 Jacobi PCG refined in FP64 (residual replacement, FP64 reductions)",
+    "multigrid": "flexible CG (FP64) with an FP32 algebraic-multigrid V-cycle",
+    "learned-jacobi": "flexible CG (FP64) with the U-Net cycle (Jacobi pre/post-smoothing, correction scaled to "
+                      "minimise the A-norm error); once the A-norm error stops halving per iteration the mixed "
+                      "Jacobi PCG finishes the solve",
+    "learned": "flexible CG (FP64) with the U-Net cycle until the A-norm error stops halving per iteration "
+               "(at least two applications), then the algebraic-multigrid V-cycle: the step's speed over FP64 "
+               "comes from the V-cycle and the FP32 kernels, the network does not shorten the solve",
+}
+
+
 def build_scene(sim, g, args, rng=None):
     """The benchmark scene, or with `rng` a randomised one for training:
     dam-break box extents U[0.2, 0.8]; droplet positions seeded."""
@@ -70,33 +91,39 @@ def build_scene(sim, g, args, rng=None):
 
 
 def train_network(args):
-    """Network trained on the simulator's own pressure solves (untimed)."""
+    """Network trained online on the simulator's own pressure systems
+    (untimed): train_runs dam breaks at train_n^3 with box extents and a drop
+    position U[0.2, 0.8] (seeded), train_pairs steps each; after every step
+    train_adam Adam steps on pairs of that step's system."""
     import numpy as np
-    from garm_amd import training
     rng = np.random.default_rng(args.train_seed)
-    t0 = time.time()
-    R, P, M = [], [], []
-    for run in range(args.train_runs):
-        g = scenes.grid(args.train_n, 3)
-        sim = capi.Liquid(3, g["res"], g["dx"], g["origin"], g["bw"])
-        ext = build_scene(sim, g, args, rng)
-        sim.initialize(capi.step_opts(mode=capi.PCG_FP64, tol=1e-8, max_iterations=args.max_iterations))
-        r, p, m = training.collect_pairs(sim, g, args.train_pairs, tol=1e-8, reinit_interval=args.reinit_interval)
-        R.append(r)
-        P.append(p)
-        M.append(m)
-        sim.close()
-        log(f"[bench] training run {run + 1}/{args.train_runs}: {args.scene} {ext}, {len(r)} pairs")
-    R, P, M = np.concatenate(R), np.concatenate(P), np.concatenate(M)
-    t_collect = time.time() - t0
     net = capi.Unet(seed=args.train_seed)
+    g = scenes.grid(args.train_n, 3)
+    opts = capi.step_opts(reinit_interval=args.reinit_interval, tol=args.tol, max_iterations=args.max_iterations,
+                          precision=capi.FP32, mode=capi.PCG_MULTIGRID)
     t0 = time.time()
-    hist = net.train(R, P, M, steps=args.train_steps, lr=args.train_lr, seed=args.train_seed)
-    t_train = time.time() - t0
-    k = max(1, len(hist) // 10)
-    info = {"pairs": int(len(R)), "grid": f"{args.train_n}^3", "runs": args.train_runs, "steps": args.train_steps,
-            "lr": args.train_lr, "loss_first": float(hist[:k].mean()), "loss_last": float(hist[-k:].mean()),
-            "collect_s": round(t_collect, 1), "train_s": round(t_train, 1)}
+    losses, pairs = [], 0
+    for run in range(args.train_runs):
+        sim = capi.Liquid(3, g["res"], g["dx"], g["origin"], g["bw"])
+        ext = tuple(float(x) for x in rng.uniform(0.2, 0.8, size=3))
+        drop = tuple(float(x) for x in rng.uniform(0.2, 0.8, size=3))
+        scenes.dam_break_with_drop(sim, g, ext, drop)
+        sim.initialize(capi.step_opts(reinit_interval=args.reinit_interval, tol=args.tol, max_iterations=args.max_iterations))
+        clk = FrameClock(g["dx"], args.cfl)
+        for s in range(args.train_pairs):
+            sim.step(clk.next_dt(sim.max_speed()), s, opts)
+            h = sim.train_network(net, args.train_adam, lr=args.train_lr, operator_fraction=args.train_operator_fraction,
+                                  seed=int(rng.integers(1 << 62)))
+            losses.append(float(h.mean()))
+            pairs += 1
+        sim.close()
+        if run % 5 == 4 or run + 1 == args.train_runs:
+            log(f"[bench] training run {run + 1}/{args.train_runs}: box {ext}, drop {drop}; loss {np.mean(losses[-args.train_pairs:]):.3f}")
+    k = max(1, len(losses) // 10)
+    info = {"pairs": pairs, "grid": f"{args.train_n}^3", "runs": args.train_runs, "adam_steps": pairs * args.train_adam,
+            "adam_steps_per_pair": args.train_adam, "operator_fraction": args.train_operator_fraction, "lr": args.train_lr,
+            "loss_first": round(float(np.mean(losses[:k])), 4), "loss_last": round(float(np.mean(losses[-k:])), 4),
+            "loss": "masked squared error of unit-RMS targets (1 = no better than 0)", "train_s": round(time.time() - t0, 1)}
     log(f"[bench] network trained: {info}")
     return net, info
 
@@ -134,27 +161,41 @@ def run_gpu(args, mode_name, net=None):
     sim.synchronize()
     elapsed = time.perf_counter() - t0
     v1 = sim.volume()
+    traj = None
+    if args.trajectory > step:
+        # on to the end of the workload's trajectory, each step timed on its own
+        # (a step returns once its work is done)
+        all_stats, wall = list(stats), []
+        while step < args.trajectory:
+            dt = clk.next_dt(sim.max_speed())
+            t1 = time.perf_counter()
+            all_stats.append(sim.step(dt, step, opts))
+            wall.append(time.perf_counter() - t1)
+            step += 1
+            if step % 50 == 0:
+                log(f"[bench] {mode_name}: trajectory step {step}/{args.trajectory}")
+        traj = (all_stats[len(stats):], wall, (sim.volume() - v0) / v0)
     sim.set_precision(precision)   # the kernels timed for the roofline are the ones this mode ran
-    return g, sim, stats, elapsed, (v1 - v0) / v0
+    return g, sim, stats, elapsed, (v1 - v0) / v0, traj
 
 
-def phase_roofline(stats, cells, mode_name, unknown_fraction):
+def phase_roofline(stats, cells):
     """Algorithmic FP64 bytes per cell of each phase (one read and one write of
     every array it must touch, derived quantities on the fly) and the rate
     they imply at the measured phase times: advection a1-a7 433 B
     (216 + 48 + 169), WENO5 64 B per iteration (30 per reinitialisation),
-    glue a13 266 B, projection set-up and gradient 160 B plus, per iteration,
-    136 B (FP64 Jacobi PCG) or 68 B (FP32 inner) per unknown, i.e. times the
-    fraction of cells that are unknowns.  The flexible solvers' iterations
-    (V-cycles, network) have no per-cell byte count here."""
+    glue a13 266 B, projection set-up and gradient 160 B plus the solver's
+    own count (hbm_bytes of the step's statistics: 136 B (FP64) or 68 B
+    (FP32 inner) per unknown per Jacobi-PCG iteration; for the flexible
+    solves their FCG vectors on the box or the compact rows, every V-cycle
+    level's sweeps, restriction and prolongation, and the vector work around
+    each network application, the network itself excluded (MFMA-bound,
+    reported under `network`))."""
     K = len(stats)
-    it = sum(s["pcg"]["iterations"] + s["pcg"]["outer_iterations"] for s in stats) / K
-    per_it = {"fp64": 136.0, "mixed": 68.0}.get(mode_name)
-    if per_it is not None:
-        per_it *= unknown_fraction
     reinit = 30 * 64.0 * sum(s["reinitialized"] + s["restarted"] * (1 - s["reinitialized"]) for s in stats) / K
+    solver = sum(s["pcg"]["hbm_bytes"] for s in stats) / K / cells
     phases = {"advect": (433.0, "t_advect_ms"), "reinit_check": (reinit, "t_restart_ms"), "extrap_enforce": (266.0, "t_extrap_ms"),
-              "project": (160.0 + per_it * it if per_it else None, "t_project_ms")}
+              "project": (160.0 + solver, "t_project_ms")}
     out = {}
     for name, (b, key) in phases.items():
         ms = sum(s[key] for s in stats) / K
@@ -164,20 +205,40 @@ def phase_roofline(stats, cells, mode_name, unknown_fraction):
     return out
 
 
-def summary(stats, elapsed, cells, drift, mode_name, unknown_fraction):
+def pcg_summary(stats):
     K = len(stats)
     it = [s["pcg"]["iterations"] for s in stats]
-    return {"value": round(cells * K / elapsed, 1), "ms_per_step": round(1000 * elapsed / K, 3),
-            "unknown_fraction": round(unknown_fraction, 4),
-            "phase_roofline": phase_roofline(stats, cells, mode_name, unknown_fraction),
-            "phases_ms": {k: round(sum(s[k] for s in stats) / K, 3) for k in ("t_advect_ms", "t_restart_ms", "t_project_ms", "t_extrap_ms", "t_total_ms")},
-            "pcg_iterations": {"mean": sum(it) / K, "min": min(it), "max": max(it),
-                               "outer_mean": sum(s["pcg"]["outer_iterations"] for s in stats) / K,
-                               "network_applications_mean": sum(s["pcg"]["preconditioner_applications"] for s in stats) / K,
-                               "fell_back": sum(s["pcg"]["fell_back"] for s in stats)},
-            "pcg_residual_max": max(s["pcg"]["relative_residual"] for s in stats),
-            "restarts": sum(s["restarted"] for s in stats),
-            "volume_drift": drift}
+    return {"mean": sum(it) / K, "min": min(it), "max": max(it),
+            "outer_mean": sum(s["pcg"]["outer_iterations"] for s in stats) / K,
+            "network_applications_mean": sum(s["pcg"]["preconditioner_applications"] for s in stats) / K,
+            "fell_back": sum(s["pcg"]["fell_back"] for s in stats)}
+
+
+def summary(stats, elapsed, cells, drift, unknown_fraction, traj):
+    K = len(stats)
+    out = {"value": round(cells * K / elapsed, 1), "ms_per_step": round(1000 * elapsed / K, 3),
+           "unknown_fraction": round(unknown_fraction, 4),
+           "phase_roofline": phase_roofline(stats, cells),
+           "phases_ms": {k: round(sum(s[k] for s in stats) / K, 3) for k in ("t_advect_ms", "t_restart_ms", "t_project_ms", "t_extrap_ms", "t_total_ms")},
+           "pcg_iterations": pcg_summary(stats),
+           "pcg_residual_max": max(s["pcg"]["relative_residual"] for s in stats),
+           "restarts": sum(s["restarted"] for s in stats),
+           "volume_drift": drift}
+    if traj:
+        tstats, wall, tdrift = traj
+        all_stats = stats + tstats
+        n_all = len(all_stats)
+        out["trajectory"] = {
+            "steps_run": n_all, "steps_after_timed": len(tstats),
+            "ms_per_step_after_timed": round(1000 * sum(wall) / len(wall), 3),
+            "ms_per_step_all": round((1000 * elapsed + 1000 * sum(wall)) / n_all, 3),
+            "phases_ms": {k: round(sum(s[k] for s in all_stats) / n_all, 3) for k in ("t_advect_ms", "t_restart_ms", "t_project_ms", "t_extrap_ms")},
+            "project_roofline": phase_roofline(all_stats, cells)["project"],
+            "pcg_iterations": pcg_summary(all_stats),
+            "pcg_residual_max": max(s["pcg"]["relative_residual"] for s in all_stats),
+            "restarts": sum(s["restarted"] for s in all_stats),
+            "volume_drift": tdrift}
+    return out
 
 
 def roofline(sim, stats, args, mode_name):
@@ -279,7 +340,8 @@ def main():
     ap.add_argument("--scene", choices=["dam_break", "droplet"], default="dam_break")
     ap.add_argument("--scene-seed", type=int, default=19260817, help="droplet position seed")
     ap.add_argument("--mode", choices=list(MODES), default="learned", help="headline configuration")
-    ap.add_argument("--also", default="fp64,mixed,multigrid", help="configurations run beside the headline one ('' for none)")
+    ap.add_argument("--also", default="fp64,mixed,multigrid,learned-jacobi", help="configurations run beside the headline one ('' for none)")
+    ap.add_argument("--trajectory", type=int, default=300, help="run every configuration on to this many steps after the timed ones (0: stop)")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iterations", type=int, default=20000)
     ap.add_argument("--check-interval", type=int, default=16)
@@ -287,9 +349,10 @@ def main():
     ap.add_argument("--cfl", type=float, default=1.0)
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--train-n", type=int, default=128)
-    ap.add_argument("--train-runs", type=int, default=2)
-    ap.add_argument("--train-pairs", type=int, default=24)
-    ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--train-runs", type=int, default=20)
+    ap.add_argument("--train-pairs", type=int, default=200, help="simulation steps (pressure systems) per training run")
+    ap.add_argument("--train-adam", type=int, default=2, help="Adam steps after every simulation step")
+    ap.add_argument("--train-operator-fraction", type=float, default=0.75)
     ap.add_argument("--train-lr", type=float, default=1e-3)
     ap.add_argument("--train-seed", type=int, default=19260817)
     ap.add_argument("--cpu-n", type=int, default=256)
@@ -311,9 +374,9 @@ def main():
         learned = MODES[m][0] == capi.PCG_LEARNED
         if learned and net is None:
             net, train_info = train_network(args)
-        g, sim, stats, elapsed, drift = run_gpu(args, m, net if learned else None)
+        g, sim, stats, elapsed, drift, traj = run_gpu(args, m, net if learned else None)
         unknown_fraction = float((sim.get(capi.UNKNOWNS) != 0).sum()) / cells   # at the last step
-        results[m] = summary(stats, elapsed, cells, drift, m, unknown_fraction)
+        results[m] = summary(stats, elapsed, cells, drift, unknown_fraction, traj)
         if learned and network is None:
             network = network_roofline(sim, net, g["res"])
         if m == args.mode:
@@ -325,6 +388,10 @@ def main():
         if plain and m != "fp64":
             r["speedup_vs_fp64"] = round(r["value"] / plain["value"], 3)
             r["volume_drift_difference_vs_fp64"] = abs(r["volume_drift"] - plain["volume_drift"])
+            if "trajectory" in r and "trajectory" in plain:
+                t, tp = r["trajectory"], plain["trajectory"]
+                t["speedup_vs_fp64"] = round(tp["ms_per_step_all"] / t["ms_per_step_all"], 3)
+                t["volume_drift_difference_vs_fp64"] = abs(t["volume_drift"] - tp["volume_drift"])
     out = {
         "metric": METRIC, "value": head["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
@@ -334,12 +401,14 @@ def main():
                                 if args.scene == "dam_break" else
                                 f"3D droplet into pool full step, {args.n}^3 MAC grid (pool y<0.3, sphere r=0.08 at seeded (x,0.7,z), "
                                 f"u=(0,-2,0) in the drop, gravity -9.8, CFL {args.cfl:g}, seed {args.scene_seed})"),
-                   "mode": args.mode, "tolerance": args.tol, "reinit_interval": args.reinit_interval, "cells": cells},
+                   "mode": args.mode, "preconditioner": PRECONDITIONER[args.mode], "tolerance": args.tol,
+                   "reinit_interval": args.reinit_interval, "cells": cells},
         "phases_ms": head["phases_ms"],
         "pcg_iterations": head["pcg_iterations"],
         "pcg_residual_max": head["pcg_residual_max"],
         "speedup_vs_fp64": head.get("speedup_vs_fp64"),
         "volume_drift_difference_vs_fp64": head.get("volume_drift_difference_vs_fp64"),
+        "trajectory": head.get("trajectory"),
         "modes": results,
         "network_training": train_info,
         "network": network,

@@ -657,6 +657,23 @@ int garm_attach_preconditioner(garm_ctx *c, garm_unet *net)
 	return hipCheck(c, hipStreamSynchronize(U->s), "network synchronize");
 }
 
+int garm_unet_train_on_system(garm_unet *net, garm_ctx *c, int steps, float lr, double operator_fraction, unsigned long long seed,
+	float *loss_history)
+{
+	if (!c || !net) return GARM_EINVAL;
+	if (steps < 0 || !(lr > 0) || !(operator_fraction >= 0 && operator_fraction <= 1)) return fail(c, GARM_EINVAL, "bad training arguments");
+	if (!c->P.rows.diag || c->P.n <= 0) return fail(c, GARM_EINVAL, "no pressure system with unknowns has been solved yet");
+	int dev = -1;
+	Unet *U = unetOf(net, &dev);
+	if (dev != c->device) return fail(c, GARM_EINVAL, "the network lives on another device");
+	int rc;
+	if ((rc = hipCheck(c, hipStreamSynchronize(U->s), "network synchronize"))) return rc;
+	std::string err;
+	rc = trainOnSystem(c->F.D, *U, c->P.rows, c->P.rhs, c->P.K.x, c->P.n, c->P.grid2mat, c->P.mat2grid, c->F.cell, c->P.LW, steps, lr,
+		operator_fraction, seed, loss_history, c->s, err);
+	return rc ? fail(c, rc, err) : GARM_OK;
+}
+
 int garm_synchronize(garm_ctx *c)
 {
 	if (!c) return GARM_EINVAL;

@@ -13,9 +13,10 @@
 // (two cells of context, sizes padded to multiples of four) to its input
 // scaled to unit RMS, the output scaled back.  The smoother removes the rough
 // error components a network trained on pressures does not resolve.  When the
-// residual stalls (an application that does not halve it) or the iteration
-// budget runs out, the solve falls back to the mixed-precision Jacobi PCG on
-// the current residual; the FP64 residual b - A x judges the result either
+// network stops paying (the A-norm error no longer halving per iteration)
+// or the iteration budget runs out, the solve falls back to the mixed-precision
+// Jacobi PCG on the current residual, or continues with the multigrid
+// V-cycle; the FP64 residual b - A x judges the result either
 // way.  While the bounding box of the unknowns is not much larger than their
 // count, the Krylov vectors live on that box (the multigrid's level-0
 // layout) with the FP64 operator on the same box; otherwise they are compact.
@@ -71,10 +72,12 @@ __device__ __forceinline__ void putPart(double (&v)[NV], double *partials)
 		for (int q = 0; q < NV; q++) partials[NV * blockIdx.x + q] = blk[q];
 }
 
-enum { kLsPq = 0, kLsRr = 1, kLsBb = 2, kLsTrue = 3, kLsSmooth = 4 };
+enum { kLsPq = 0, kLsRr = 1, kLsBb = 2, kLsTrue = 3, kLsEnergy = 4, kLsSmooth = 5, kLsTrainIn = 6, kLsGamma = 7, kLsTrainOut = 8,
+	kLsWords = 16 };
 
 constexpr double kOmega = 2.0 / 3.0;
-// each network application must reduce the residual norm at least this much
+// a preconditioner stalls when the residual norm has not fallen by this
+// factor over its last window of iterations
 constexpr double kMinReduction = 0.5;
 
 // z = w D^-1 r
@@ -127,7 +130,8 @@ __global__ __launch_bounds__(kBlock) void kFcgUpdate(int n, double *x, double *r
 	double a[1] = {0};
 	if (threadIdx.x == 0) a[0] = t[0] != 0 ? t[1] / t[0] : 0.0;
 	bcast<1>(a);
-	if (blockIdx.x == 0 && threadIdx.x == 0) ls[kLsPq] = t[0];
+	// the step's decrease of the A-norm error energy e^T A e: (r.p)^2 / (p.Ap)
+	if (blockIdx.x == 0 && threadIdx.x == 0) { ls[kLsPq] = t[0]; ls[kLsEnergy] = t[0] != 0 ? t[1] * t[1] / t[0] : 0.0; }
 	double v[1] = {0};
 	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
 		x[i] += a[0] * p[i];
@@ -341,11 +345,64 @@ __global__ __launch_bounds__(kBlock) void kBoxPostSmooth(BoxOp O, const double *
 		out[c] = z[c] + kOmega * O.idiag[c] * (r[c] - boxApply<D>(O, z, c));
 }
 
+// The network's correction u enters with the step length that minimises
+// the A-norm of the error left after the pre-smoothing: with t = r - A z0,
+// gamma = (u.t) / (u.A u).  w = A u and the partials of u.t, u.w:
+template <int D>
+__global__ __launch_bounds__(kBlock) void kBoxCorrectionDots(BoxOp O, const double *u, const double *t, double *w, double *pa)
+{
+	double v[2] = {0, 0};
+	for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < O.n; c += gridDim.x * blockDim.x) {
+		const double s = boxApply<D>(O, u, c);
+		w[c] = s;
+		v[0] += u[c] * t[c];
+		v[1] += u[c] * s;
+	}
+	putPart<2>(v, pa);
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void kRowsCorrectionDots(Rows<double> A, int n, const double *u, const double *t, double *w, double *pa)
+{
+	double v[2] = {0, 0};
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+		double s = A.diag[i] * u[i];
+#pragma unroll
+		for (int o = 0; o < 2 * D; o++) {
+			const int c = A.nb[o][i];
+			if (c >= 0) s += -A.w[o][i] * u[c];
+		}
+		w[i] = s;
+		v[0] += u[i] * t[i];
+		v[1] += u[i] * s;
+	}
+	putPart<2>(v, pa);
+}
+
+// ls[kLsGamma] = (u.t) / (u.A u), 0 when u.A u is not positive
+__global__ __launch_bounds__(kBlock) void kGamma(const double *pa, int nb, double *ls)
+{
+	double t[2];
+	sumPart<2>(pa, nb, t);
+	if (threadIdx.x == 0) ls[kLsGamma] = (t[1] > 0 && isfinite(t[0] / t[1])) ? t[0] / t[1] : 0.0;
+}
+
+// z = z0 + gamma u, then a Jacobi post-smoothing step, whose residual
+// r - A z = t - gamma w needs no further product:
+// out = z0 + gamma u + w D^-1 (t - gamma w)
+__global__ void kCombine(int n, const double *idiag, const double *z0, const double *u, const double *t, const double *w, const double *ls,
+	double *out)
+{
+	const double g = ls[kLsGamma];
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+		out[i] = z0[i] + g * u[i] + kOmega * idiag[i] * (t[i] - g * w[i]);
+}
+
 // ------------------------------------------------------------------ work
 LearnedWork::~LearnedWork()
 {
 	auto fr = [](auto *&q) { if (q) { (void)hipFree(q); q = nullptr; } };
-	fr(x); fr(r); fr(p); fr(q); fr(z); fr(t); fr(b2); fr(pa); fr(pb); fr(ls); fr(box);
+	fr(x); fr(r); fr(p); fr(q); fr(z); fr(t); fr(b2); fr(u); fr(w); fr(pa); fr(pb); fr(ls); fr(box);
 	if (host) { (void)hipHostFree(host); host = nullptr; }
 	if (ready) { (void)hipEventDestroy(ready); ready = nullptr; }
 	if (bpool) { (void)hipFree(bpool); bpool = nullptr; }
@@ -354,7 +411,7 @@ LearnedWork::~LearnedWork()
 
 bool LearnedWork::reserveBox(size_t cells, size_t unknowns, std::string &err)
 {
-	const size_t need = 11 * cells;
+	const size_t need = 13 * cells;
 	if (need > bcap) {
 		if (bpool) (void)hipFree(bpool);
 		bpool = nullptr;
@@ -377,16 +434,16 @@ bool LearnedWork::reserve(size_t n, std::string &err)
 	bool ok = true;
 	if (!ls) {
 		ok = hipMalloc(&pa, 2 * kLGrid * sizeof(double)) == hipSuccess && hipMalloc(&pb, kLGrid * sizeof(double)) == hipSuccess
-			&& hipMalloc(&ls, 8 * sizeof(double)) == hipSuccess && hipMalloc(&box, 8 * sizeof(int)) == hipSuccess
-			&& hipHostMalloc(&host, 16 * sizeof(double)) == hipSuccess;
+			&& hipMalloc(&ls, kLsWords * sizeof(double)) == hipSuccess && hipMalloc(&box, 8 * sizeof(int)) == hipSuccess
+			&& hipHostMalloc(&host, kLsWords * sizeof(double)) == hipSuccess;
 		if (!ok) { err = "learned-solver scalar allocation failed"; return false; }
 	}
 	if (!ready && hipEventCreateWithFlags(&ready, hipEventDisableTiming) != hipSuccess) { err = "learned-solver event creation failed"; return false; }
 	if (n > cap) {
 		auto fr = [](auto *&q) { if (q) { (void)hipFree(q); q = nullptr; } };
-		fr(x); fr(r); fr(p); fr(q); fr(z); fr(t); fr(b2);
+		fr(x); fr(r); fr(p); fr(q); fr(z); fr(t); fr(b2); fr(u); fr(w);
 		const size_t want = std::max<size_t>(n + n / 4, 1024);
-		for (double **v : {&x, &r, &p, &q, &z, &t, &b2}) ok = ok && hipMalloc(v, want * sizeof(double)) == hipSuccess;
+		for (double **v : {&x, &r, &p, &q, &z, &t, &b2, &u, &w}) ok = ok && hipMalloc(v, want * sizeof(double)) == hipSuccess;
 		if (!ok) { err = "learned-solver vector allocation failed"; cap = 0; return false; }
 		cap = want;
 	}
@@ -437,13 +494,41 @@ static int unknownBox(const int *mat2grid, int n, const GridDev &cell, LearnedWo
 	return GARM_OK;
 }
 
+// The network's grid for the unknowns in [lo, hi]: their bounding box with
+// two cells of context, sizes padded to multiples of four.
+static UnetBox networkBox(int D, const int lo[3], const int hi[3], const GridDev &cell)
+{
+	UnetBox B{};
+	for (int a = 0; a < 3; a++) {
+		const bool live = a < D;
+		const int l = live ? lo[a] - 2 : 0, h = live ? hi[a] + 2 : 0;
+		B.org[a] = l;
+		B.n[a] = live ? (h - l + 1 + 3) / 4 * 4 : 4;
+		B.gn[a] = cell.n[a];
+	}
+	return B;
+}
+
 // A preconditioner application z = M(r) as an optional head and the rest:
 // the head may be enqueued before the solver knows whether the application
 // is needed (it then only writes scratch the next head overwrites).
 struct Precond {
 	std::function<void()> head, rest;
+	double bytes = 0;   // algorithmic HBM bytes of one application
 	void full() const { if (head) head(); rest(); }
 };
+
+// Algorithmic HBM bytes of one flexible-CG iteration per vector element:
+// product (p and the operator in, the residual for r.p, q out), x/r update,
+// z.q, p update; on the box the operator is diag + three face weights (40 B),
+// on compact rows diag + six (index, weight) pairs (80 B).
+constexpr double kFcgBoxBytes = 56 + 48 + 16 + 24;
+constexpr double kFcgRowBytes = 96 + 48 + 16 + 24;
+// One network application's vector work per element (the network's own
+// activations excluded): Jacobi 24, pre-smoothed residual 56, zeroing the
+// correction 8, gather 16 and scatter 20, A u with two dots 56, combination 48.
+constexpr double kNetBoxBytes = 24 + 56 + 8 + 16 + 20 + 56 + 48;
+constexpr double kNetRowBytes = kNetBoxBytes + 2 * 40;
 
 // The Krylov vectors of a flexible solve and how A acts on them: compact
 // (the unknowns' numbering, the FP64 rows), or on the bounding box of the
@@ -458,6 +543,7 @@ struct FcgSpace {
 	bool box = false;
 	int nv = 0;                        // vector length
 	double *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *t = nullptr;
+	double *u = nullptr, *w = nullptr; // network correction and A times it
 	BoxOp O{};                         // box: the operator
 	const unsigned *boxOf = nullptr;   // box: box cell of every unknown
 };
@@ -467,6 +553,7 @@ static void compactSpace(LearnedWork &W, int n, FcgSpace &V)
 	V.box = false;
 	V.nv = n;
 	V.x = W.x; V.r = W.r; V.p = W.p; V.q = W.q; V.z = W.z; V.t = W.t;
+	V.u = W.u; V.w = W.w;
 }
 
 // Lays out the box of the unknowns (origin lo, sizes hi - lo + 1) in W:
@@ -484,13 +571,14 @@ static int boxSpace(const Rows<double> &A64, int n, const int *mat2grid, const G
 	V.box = true;
 	V.nv = N;
 	double *q = W.bpool;
-	double *arr[11];
+	double *arr[13];
 	for (double *&e : arr) { e = q; q += cells; }
 	V.O.n = N;
 	V.O.sy = bn[0];
 	V.O.sz = bn[0] * bn[1];
 	V.O.diag = arr[0]; V.O.idiag = arr[1]; V.O.wx = arr[2]; V.O.wy = arr[3]; V.O.wz = arr[4];
 	V.x = arr[5]; V.r = arr[6]; V.p = arr[7]; V.q = arr[8]; V.z = arr[9]; V.t = arr[10];
+	V.u = arr[11]; V.w = arr[12];
 	V.boxOf = W.boxOf;
 	// the operator is zero outside the unknowns; the vectors are written
 	// whole before they are read
@@ -504,16 +592,26 @@ static int boxSpace(const Rows<double> &A64, int n, const int *mat2grid, const G
 }
 
 // Flexible CG on the FP64 system, on the vectors of V, with preconditioner
-// `precondition` (writes V.z = M(V.r)).  Stall rule: with `perApplication`
-// every application must halve the residual, otherwise the residual must
-// halve every kStallWindow iterations.  After the loop the solution goes to
-// the compact W.x, the FP64 residual b - A x of the compact rows judges it,
-// and what is left goes to the mixed-precision Jacobi solver.
+// `precondition` (writes V.z = M(V.r)).  Stall rules (CG minimises the
+// error in the A-norm, not the residual, so a single iteration may leave
+// the residual where it was):
+//  - with `energyRule`, the first preconditioner is judged by the A-norm:
+//    each iteration lowers the error energy e^T A e by (r.p)^2 / (p.Ap),
+//    and the decrease of an iteration must be at most kEnergyRatio of the
+//    decrease of the iteration before (the A-norm error halving per
+//    iteration, as a V-cycle does and more), from the second iteration on;
+//  - otherwise, and for the second preconditioner, the residual norm must
+//    halve over every kStallWindow iterations.
+// After the loop the solution goes to the compact W.x, the FP64 residual
+// b - A x of the compact rows judges it, and what is left goes to the
+// mixed-precision Jacobi solver.
 constexpr int kStallWindow = 10;
+constexpr int kEnergyWindow = 1;
+constexpr double kEnergyRatio = 1.0 / 4.0;
 
 template <int D>
 static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, FcgSpace &V, const garm_pcg_opts &o,
-	Krylov &K, LearnedWork &W, hipStream_t s, garm_pcg_stats *st, std::string &err, const Precond &first, bool perApplication,
+	Krylov &K, LearnedWork &W, hipStream_t s, garm_pcg_stats *st, std::string &err, const Precond &first, bool energyRule,
 	int firstBudget, const Precond *second, const char *what)
 {
 	const Precond *precondition = &first;
@@ -537,6 +635,7 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 	st->outer_iterations = 0;
 	st->fell_back = 0;
 	st->preconditioner_applications = 0;
+	st->hbm_bytes = 0;
 	if (bb == 0) {
 		hipLaunchKernelGGL(kZeroD, g1, b1, 0, s, W.x, n);
 		st->relative_residual = 0;
@@ -546,14 +645,14 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 	const double thr = tol * tol * bb;
 	precondition->full();
 	hipLaunchKernelGGL(kCopyD, gb, b1, 0, s, V.p, (const double *)V.z, N);
-	std::vector<double> hist(1, bb);
+	std::vector<double> hist(1, bb), energy;
 	int it = 0;
 	for (; it < budget; it++) {
 		if (V.box) hipLaunchKernelGGL(kBoxSpmv<D>, gb, b1, 0, s, V.O, (const double *)V.p, V.q, (const double *)V.r, W.pa);
 		else hipLaunchKernelGGL(kFcgSpmv<D>, gb, b1, 0, s, A64, n, (const double *)V.p, V.q, (const double *)V.r, W.pa);
 		hipLaunchKernelGGL(kFcgUpdate, gb, b1, 0, s, N, V.x, V.r, (const double *)V.p, (const double *)V.q, (const double *)W.pa, W.pb, W.ls);
 		hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nbb, W.ls + kLsRr);
-		if ((rc = readStart(W, W.ls, 4, s, err))) return rc;
+		if ((rc = readStart(W, W.ls, kLsEnergy + 1, s, err))) return rc;
 		// the next application's head keeps the GPU busy while the host
 		// reads the residual and decides
 		if (precondition->head) precondition->head();
@@ -561,10 +660,12 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 		const double rr = W.host[kLsRr];
 		if (!std::isfinite(rr)) break;
 		hist.push_back(rr);
+		energy.push_back(W.host[kLsEnergy]);
 		if (rr < thr) { it++; break; }
 		const bool onFirst = switchedAt < 0;
 		bool stalled;
-		if (onFirst && perApplication) stalled = rr > kMinReduction * kMinReduction * hist[hist.size() - 2];   // did not halve
+		if (onFirst && energyRule)
+			stalled = energy.size() > size_t(kEnergyWindow) && energy.back() > kEnergyRatio * energy[energy.size() - 1 - kEnergyWindow];
 		else {
 			const size_t from = onFirst ? 0 : size_t(switchedAt) + 1;
 			stalled = hist.size() > from + kStallWindow && rr > kMinReduction * kMinReduction * hist[hist.size() - 1 - kStallWindow];
@@ -584,6 +685,13 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 	}
 	st->iterations = it;
 	st->preconditioner_applications = switchedAt < 0 ? it : switchedAt;
+	{
+		// the application before the loop and one per iteration but the last
+		const int apps = std::max(it, 1);
+		const int firstApps = switchedAt < 0 ? apps : switchedAt + 1;
+		st->hbm_bytes = double(it) * (V.box ? kFcgBoxBytes : kFcgRowBytes) * double(N) + firstApps * first.bytes +
+			(second ? double(apps - firstApps) * second->bytes : 0.0);
+	}
 	// the solution in the compact unknowns; the FP64 residual judges it, and
 	// plain PCG takes what is left
 	if (V.box) hipLaunchKernelGGL(kGatherFromBox, g1, b1, 0, s, n, V.boxOf, (const double *)V.x, W.x);
@@ -604,6 +712,7 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 		hipLaunchKernelGGL(kAddD, g1, b1, 0, s, W.x, (const double *)K.x, n);
 		st->iterations += fs.iterations;
 		st->outer_iterations = fs.outer_iterations;
+		st->hbm_bytes += fs.hbm_bytes;
 		hipLaunchKernelGGL(kTrueResidual<D>, g1, b1, 0, s, A64, n, b, (const double *)W.x, W.b2, W.pb);
 		hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nb, W.ls + kLsTrue);
 		if ((rc = readHost(W, W.ls, 4, s, err))) return rc;
@@ -623,15 +732,7 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 	const dim3 b1(kBlock);
 	int rc, lo[3], hi[3];
 	if ((rc = unknownBox(mat2grid, n, cell, W, s, lo, hi, err))) return rc;
-	// network box: bounding box of the unknowns, two cells of context, multiples of four
-	UnetBox B{};
-	for (int a = 0; a < 3; a++) {
-		const bool live = a < D;
-		const int l = live ? lo[a] - 2 : 0, h = live ? hi[a] + 2 : 0;
-		B.org[a] = l;
-		B.n[a] = live ? (h - l + 1 + 3) / 4 * 4 : 4;
-		B.gn[a] = cell.n[a];
-	}
+	const UnetBox B = networkBox(D, lo, hi, cell);
 	const size_t boxVox = size_t(B.n[0]) * B.n[1] * B.n[2];
 	// the buffers are reallocated only when the box outgrows them, then for
 	// inference only (no training gradients) and with half again as much room
@@ -649,7 +750,8 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 	}
 	const dim3 gb(lblocks(V.nv));
 	const int nbb = lblocks(V.nv);
-	// z = M(r): two-level cycle, network on the pre-smoothed residual
+	// z = M(r): two-level cycle, network on the pre-smoothed residual, its
+	// correction scaled to minimise the A-norm of the error
 	Precond net;
 	net.rest = [&]() {
 		if (V.box) {
@@ -660,15 +762,20 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 			hipLaunchKernelGGL(kTrueResidual<D>, gb, b1, 0, s, A64, n, (const double *)V.r, (const double *)V.z, V.t, W.pb);
 		}
 		hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nbb, W.ls + kLsSmooth);
+		hipLaunchKernelGGL(kZeroD, gb, b1, 0, s, V.u, V.nv);
 		if (V.box) {
-			unetApplyVecBox(U, B, VB, grid2mat, V.t, W.ls + kLsSmooth, n, V.z, s);
-			hipLaunchKernelGGL(kBoxPostSmooth<D>, gb, b1, 0, s, V.O, (const double *)V.r, (const double *)V.z, V.t);
+			unetApplyVecBox(U, B, VB, grid2mat, V.t, W.ls + kLsSmooth, n, V.u, s);
+			hipLaunchKernelGGL(kBoxCorrectionDots<D>, gb, b1, 0, s, V.O, (const double *)V.u, (const double *)V.t, V.w, W.pa);
 		} else {
-			unetApplyCompact(U, B, grid2mat, V.t, W.ls + kLsSmooth, n, V.z, s);
-			hipLaunchKernelGGL(kPostSmooth<D>, gb, b1, 0, s, A64, n, (const double *)V.r, (const double *)V.z, V.t);
+			unetApplyCompact(U, B, grid2mat, V.t, W.ls + kLsSmooth, n, V.u, s);
+			hipLaunchKernelGGL(kRowsCorrectionDots<D>, gb, b1, 0, s, A64, n, (const double *)V.u, (const double *)V.t, V.w, W.pa);
 		}
+		hipLaunchKernelGGL(kGamma, dim3(1), b1, 0, s, (const double *)W.pa, nbb, W.ls);
+		hipLaunchKernelGGL(kCombine, gb, b1, 0, s, V.nv, V.box ? V.O.idiag : (const double *)A64.invdiag, (const double *)V.z,
+			(const double *)V.u, (const double *)V.t, (const double *)V.w, (const double *)W.ls, V.t);
 		std::swap(V.z, V.t);
 	};
+	net.bytes = (V.box ? kNetBoxBytes : kNetRowBytes) * double(V.nv);
 	Precond mg;
 	const Precond *second = nullptr;
 	if (o.fallback_mode == GARM_PCG_MULTIGRID) {
@@ -683,6 +790,7 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 		mgBuild(M, A32, n, mat2grid, s);
 		mg.head = [&]() { mgApplyHead(M, n, V.r, s, V.box); };
 		mg.rest = [&]() { mgApplyRest(M, n, V.z, s, V.box); };
+		mg.bytes = mgCycleBytes(M);
 		second = &mg;
 	}
 	return flexibleCg<D>(A64, A32, b, n, V, o, K, W, s, st, err, net, true, std::min(o.max_iterations, 400), second, "learned solve");
@@ -711,6 +819,7 @@ static int multigrid(const Rows<double> &A64, const Rows<float> &A32, const doub
 	Precond mg;
 	mg.head = [&]() { mgApplyHead(M, n, V.r, s, V.box); };
 	mg.rest = [&]() { mgApplyRest(M, n, V.z, s, V.box); };
+	mg.bytes = mgCycleBytes(M);
 	return flexibleCg<D>(A64, A32, b, n, V, o, K, W, s, st, err, mg, false, o.max_iterations, nullptr, "multigrid solve");
 }
 
@@ -762,6 +871,180 @@ __global__ void kLoadScaled(const float *r, const uint8_t *mask, float scale, bf
 		in[v * 4 + 2] = 0;
 		in[v * 4 + 3] = 0;
 	}
+}
+
+// -------------------------------------------- training on a solve's system
+// Pairs drawn from the pressure system of the last projection, made on the
+// device: (A e, e) for random smooth e on the unknowns -- the residual and
+// error of the smooth error component the Jacobi smoothing of the
+// preconditioner leaves to the network -- and (b, x), the projection's own
+// right-hand side and solution, with a random sign.
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+	z += 0x9e3779b97f4a7c15ull;
+	z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+	z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+	return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ double unit01(uint64_t h) { return double(h >> 11) * 0x1.0p-53; }
+
+constexpr int kModes = 12;
+// The network sees about 20 cells around a voxel (two 3x3x3 convolutions per
+// level at spacings 1, 2 and 4, going down and up), so it cannot invert the
+// operator on wavelengths much longer than twice that; the operator pairs
+// stop there (with the unknowns' whole extent the loss stayed near 1 and the
+// trained network slowed the flexible solve down).
+constexpr double kMaxWavelength = 32.0;
+
+// modes[m] = (kx, ky, kz, phase, amplitude): wavelengths log-uniform in
+// [4, L] cells, directions uniform (on the sphere in 3D, the circle in 2D),
+// amplitude (lambda / L)^slope with slope uniform in [0.5, 2]
+__global__ void kSmoothModes(int D, uint64_t key, double L, double *modes)
+{
+	const int m = int(threadIdx.x);
+	if (m >= kModes) return;
+	const double slope = 0.5 + 1.5 * unit01(mix64(key));
+	const uint64_t h = mix64(key ^ (0x632be59bd9b4e019ull * uint64_t(m + 1)));
+	const double twoPi = 6.283185307179586;
+	const double lam = exp(log(4.0) + unit01(mix64(h + 1)) * (log(fmax(L, 4.0)) - log(4.0)));
+	double d[3];
+	const double az = twoPi * unit01(mix64(h + 2));
+	if (D == 3) {
+		const double z = 2.0 * unit01(mix64(h + 3)) - 1.0, rr = sqrt(fmax(0.0, 1.0 - z * z));
+		d[0] = rr * cos(az); d[1] = rr * sin(az); d[2] = z;
+	} else {
+		d[0] = cos(az); d[1] = sin(az); d[2] = 0;
+	}
+	const double k = twoPi / lam;
+	modes[5 * m + 0] = k * d[0];
+	modes[5 * m + 1] = k * d[1];
+	modes[5 * m + 2] = k * d[2];
+	modes[5 * m + 3] = twoPi * unit01(mix64(h + 4));
+	modes[5 * m + 4] = pow(lam / fmax(L, 4.0), slope);
+}
+
+__global__ __launch_bounds__(kBlock) void kSmoothField(const int *mat2grid, int n, int nx, int ny, const double *modes, double *e)
+{
+	__shared__ double md[5 * kModes];
+	for (int q = int(threadIdx.x); q < 5 * kModes; q += blockDim.x) md[q] = modes[q];
+	__syncthreads();
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+		const int c = mat2grid[i];
+		const double x = double(c % nx), y = double((c / nx) % ny), z = double(c / nx / ny);
+		double v = 0;
+#pragma unroll
+		for (int m = 0; m < kModes; m++) v += md[5 * m + 4] * sin(md[5 * m] * x + md[5 * m + 1] * y + md[5 * m + 2] * z + md[5 * m + 3]);
+		e[i] = v;
+	}
+}
+
+// r = A e; partials of |r|^2
+template <int D>
+__global__ __launch_bounds__(kBlock) void kOperatorPair(Rows<double> A, int n, const double *e, double *r, double *pb)
+{
+	double v[1] = {0};
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+		double t = A.diag[i] * e[i];
+#pragma unroll
+		for (int o = 0; o < 2 * D; o++) {
+			const int c = A.nb[o][i];
+			if (c >= 0) t += -A.w[o][i] * e[c];
+		}
+		r[i] = t;
+		v[0] += t * t;
+	}
+	putPart<1>(v, pb);
+}
+
+__global__ void kSignedCopies(int n, double sign, const double *b, const double *x, double *r, double *t)
+{
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+		r[i] = sign * b[i];
+		t[i] = sign * x[i];
+	}
+}
+
+// masked squared error of the network output y (box) against the target
+// (compact unknowns, scaled to unit RMS: the solver scales the network's
+// correction itself) and its gradient
+__global__ __launch_bounds__(kBlock) void kLossGradBox(UnetBox B, const int *grid2mat, const float *y, const double *target, const double *tt,
+	int n, float *gy, float *part)
+{
+	const double scale = tt[0] > 0 ? sqrt(double(n) / tt[0]) : 1.0;
+	const float inv = 1.0f / float(n);
+	const size_t vox = size_t(B.n[0]) * B.n[1] * B.n[2];
+	float acc = 0;
+	for (size_t v = blockIdx.x * size_t(blockDim.x) + threadIdx.x; v < vox; v += size_t(gridDim.x) * blockDim.x) {
+		const int x = int(v % B.n[0]), yy = int((v / B.n[0]) % B.n[1]), z = int(v / (size_t(B.n[0]) * B.n[1]));
+		const int gx = B.org[0] + x, gy_ = B.org[1] + yy, gz = B.org[2] + z;
+		int m = -1;
+		if (gx >= 0 && gx < B.gn[0] && gy_ >= 0 && gy_ < B.gn[1] && gz >= 0 && gz < B.gn[2])
+			m = grid2mat[(size_t(gz) * B.gn[1] + gy_) * B.gn[0] + gx];
+		const float d = m >= 0 ? y[v] - float(target[m] * scale) : 0.0f;
+		gy[v] = 2.0f * d * inv;
+		acc += d * d;
+	}
+	double a[1] = {acc};
+	double o[1];
+	blockSum<1>(a, o);
+	if (threadIdx.x == 0) part[blockIdx.x] = float(o[0]);
+}
+
+int trainOnSystem(int D, Unet &U, const Rows<double> &A64, const double *rhs, const double *solution, int n, const int *grid2mat,
+	const int *mat2grid, const GridDev &cell, LearnedWork &W, int steps, float lr, double operatorFraction, unsigned long long seed,
+	float *lossHistory, hipStream_t s, std::string &err)
+{
+	if (n <= 0) { err = "the last projection has no unknowns"; return GARM_EINVAL; }
+	if (!W.reserve(size_t(n), err)) return GARM_ENOMEM;
+	int rc, lo[3], hi[3];
+	if ((rc = unknownBox(mat2grid, n, cell, W, s, lo, hi, err))) return rc;
+	const UnetBox B = networkBox(D, lo, hi, cell);
+	const size_t vox = size_t(B.n[0]) * B.n[1] * B.n[2];
+	if (!U.buf.reserve(vox, true, err) || !U.buf.setDims(B.n[0], B.n[1], B.n[2], err)) return GARM_ENOMEM;
+	float *loss = nullptr, *part = nullptr;
+	const int nbl = 1024;
+	if (hipMalloc(&loss, size_t(std::max(steps, 1)) * sizeof(float)) != hipSuccess || hipMalloc(&part, nbl * sizeof(float)) != hipSuccess) {
+		if (loss) (void)hipFree(loss);
+		err = "training scratch allocation failed";
+		return GARM_ENOMEM;
+	}
+	// longest wavelength: the unknowns' extent, at most kMaxWavelength cells
+	double L = 0;
+	for (int a = 0; a < D; a++) L = std::max(L, double(hi[a] - lo[a] + 1));
+	L = std::min(L, kMaxWavelength);
+	const dim3 g1(lblocks(n)), b1(kBlock);
+	const int nb = lblocks(n);
+	double *rr = W.ls + kLsTrainIn, *tt = W.ls + kLsTrainOut, *modes = W.pa;
+	std::mt19937_64 rng(seed);
+	for (int k = 0; k < steps; k++) {
+		const uint64_t draw = rng();
+		if (double(draw >> 11) * 0x1.0p-53 < operatorFraction) {
+			hipLaunchKernelGGL(kSmoothModes, dim3(1), dim3(64), 0, s, D, uint64_t(rng()), L, modes);
+			hipLaunchKernelGGL(kSmoothField, g1, b1, 0, s, mat2grid, n, cell.n[0], cell.n[1], (const double *)modes, W.t);
+			if (D == 2) hipLaunchKernelGGL(kOperatorPair<2>, g1, b1, 0, s, A64, n, (const double *)W.t, W.r, W.pb);
+			else hipLaunchKernelGGL(kOperatorPair<3>, g1, b1, 0, s, A64, n, (const double *)W.t, W.r, W.pb);
+		} else {
+			hipLaunchKernelGGL(kSignedCopies, g1, b1, 0, s, n, (draw & 1) ? -1.0 : 1.0, rhs, solution, W.r, W.t);
+			hipLaunchKernelGGL(kFcgDot, g1, b1, 0, s, n, (const double *)W.r, (const double *)W.r, W.pb);
+		}
+		hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nb, rr);
+		hipLaunchKernelGGL(kFcgDot, g1, b1, 0, s, n, (const double *)W.t, (const double *)W.t, W.pb);
+		hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nb, tt);
+		unetGatherCompact(U, B, grid2mat, W.r, rr, n, s);
+		unetForward(U, s);
+		hipLaunchKernelGGL(kLossGradBox, dim3(nbl), b1, 0, s, B, grid2mat, (const float *)U.buf.y, (const double *)W.t, (const double *)tt, n,
+			U.buf.gy, part);
+		hipLaunchKernelGGL(kLossReduce, dim3(1), dim3(64), 0, s, (const float *)part, nbl, 1.0f / float(n), loss + k);
+		if (!unetBackward(U, s, err)) { (void)hipFree(loss); (void)hipFree(part); return GARM_ENOMEM; }
+		unetAdam(U, lr, 0.9f, 0.999f, 1e-8f, s);
+	}
+	bool ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess;
+	if (ok && lossHistory && steps > 0) ok = hipMemcpy(lossHistory, loss, size_t(steps) * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess;
+	(void)hipFree(loss);
+	(void)hipFree(part);
+	if (!ok) { err = "training on the pressure system failed"; return GARM_EHIP; }
+	return GARM_OK;
 }
 
 } // namespace garm

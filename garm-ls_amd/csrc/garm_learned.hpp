@@ -14,6 +14,7 @@ namespace garm {
 // FP64 vectors and scalars of the flexible solve.
 struct LearnedWork {
 	double *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *t = nullptr, *b2 = nullptr;
+	double *u = nullptr, *w = nullptr;   // the network's correction and A times it
 	double *pa = nullptr, *pb = nullptr, *ls = nullptr;
 	int *box = nullptr;
 	double *host = nullptr;   // pinned
@@ -40,6 +41,17 @@ int solveLearned(int D, const Rows<double> &A64, const Rows<float> &A32, const d
 // (FP32 cycle, FP64 Krylov vectors and residual); the solution is left in W.x.
 int solveMultigrid(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *mat2grid,
 	const GridDev &cell, const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Multigrid &M, hipStream_t s, garm_pcg_stats *st, std::string &err);
+
+// `steps` Adam steps of U on pairs made from the pressure system of a
+// projection (operator rows A64, right-hand side rhs, solution): with
+// probability operatorFraction an operator pair (A e, e) for a random smooth
+// e on the unknowns, otherwise (rhs, solution) with a random sign; input and
+// target each scaled to unit RMS over the unknowns (the solver scales the
+// correction), masked squared error.  The network
+// grid is the unknowns' bounding box with two cells of context.
+int trainOnSystem(int D, Unet &U, const Rows<double> &A64, const double *rhs, const double *solution, int n, const int *grid2mat,
+	const int *mat2grid, const GridDev &cell, LearnedWork &W, int steps, float lr, double operatorFraction, unsigned long long seed,
+	float *lossHistory, hipStream_t s, std::string &err);
 
 // The network behind a C-ABI handle and the device it was created on.
 Unet *unetOf(garm_unet *net, int *device);

@@ -772,6 +772,18 @@ void mgApplyRest(Multigrid &M, int n, double *z, hipStream_t s, bool box)
 	else vcycleRest<3>(M, n, z, box, s);
 }
 
+// Algorithmic HBM bytes of one V-cycle: the residual's load and the
+// correction's store on level 0 (FP64 vector, FP32 level arrays: 16 + 12 B
+// per cell), and per level above the LDS tail four sweeps (operator 16 B, b,
+// x in, x out: 28 B per cell each), the restriction (28 B per fine cell) and
+// the prolongation (8 B per fine cell).
+double mgCycleBytes(const Multigrid &M)
+{
+	double b = 28.0 * double(M.cells[0]);
+	for (int l = 0; l < M.tail_first; l++) b += (4 * 28.0 + 28.0 + 8.0) * double(M.cells[l]);
+	return b;
+}
+
 void mgApply(Multigrid &M, int n, const double *r, double *z, hipStream_t s)
 {
 	mgApplyHead(M, n, r, s);

@@ -54,5 +54,7 @@ void mgApply(Multigrid &M, int n, const double *r, double *z, hipStream_t s);
 // level 0's box (its cells, zero outside the unknowns) instead of compact.
 void mgApplyHead(Multigrid &M, int n, const double *r, hipStream_t s, bool box = false);
 void mgApplyRest(Multigrid &M, int n, double *z, hipStream_t s, bool box = false);
+// Algorithmic HBM bytes of one cycle (its arrays read or written once per launch).
+double mgCycleBytes(const Multigrid &M);
 
 } // namespace garm

@@ -379,16 +379,26 @@ static int mixed(const Rows<double> &A64, const Rows<float> &A32, const double *
 	return GARM_OK;
 }
 
+// Algorithmic HBM bytes per unknown: an FP64 Jacobi-PCG iteration (product:
+// p, diagonal and three face weights in, Ap out; update: x, r, p, Ap and the
+// inverse diagonal in, x, r, z out; p update: z, p in, p out), its FP32
+// counterpart, and an FP64 residual replacement (x, b, the rows in, r out).
+constexpr double kPcg64IterBytes = 136, kPcg32IterBytes = 68, kReplaceBytes = 104;
+
 int solvePcg64(int D, const Rows<double> &A, const double *b, int n, const garm_pcg_opts &o, Krylov &K, hipStream_t s,
 	garm_pcg_stats *st, std::string &err)
 {
-	return D == 2 ? pcg64<2>(A, b, n, o, K, s, st, err) : pcg64<3>(A, b, n, o, K, s, st, err);
+	const int rc = D == 2 ? pcg64<2>(A, b, n, o, K, s, st, err) : pcg64<3>(A, b, n, o, K, s, st, err);
+	st->hbm_bytes = kPcg64IterBytes * double(n) * st->iterations;
+	return rc;
 }
 
 int solveMixed(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const garm_pcg_opts &o, Krylov &K,
 	hipStream_t s, garm_pcg_stats *st, std::string &err)
 {
-	return D == 2 ? mixed<2>(A64, A32, b, n, o, K, s, st, err) : mixed<3>(A64, A32, b, n, o, K, s, st, err);
+	const int rc = D == 2 ? mixed<2>(A64, A32, b, n, o, K, s, st, err) : mixed<3>(A64, A32, b, n, o, K, s, st, err);
+	st->hbm_bytes = (kPcg32IterBytes * st->iterations + kReplaceBytes * st->outer_iterations) * double(n);
+	return rc;
 }
 
 int timeSolverKernel(int D, const Rows<double> &A64, const Rows<float> &A32, int n, int which, int reps, Krylov &K,

@@ -977,6 +977,12 @@ void unetApplyCompact(Unet &U, const UnetBox &B, const int *grid2mat, const doub
 	hipLaunchKernelGGL(kScatterBox, dim3(blocksFor(vox)), dim3(kUB), 0, s, B, grid2mat, (const float *)U.buf.y, rr, n, z);
 }
 
+void unetGatherCompact(Unet &U, const UnetBox &B, const int *grid2mat, const double *r, const double *rr, int n, hipStream_t s)
+{
+	const size_t vox = size_t(B.n[0]) * B.n[1] * B.n[2];
+	hipLaunchKernelGGL(kGatherBox, dim3(blocksFor(vox)), dim3(kUB), 0, s, B, grid2mat, r, rr, n, U.buf.in);
+}
+
 void unetApplyVecBox(Unet &U, const UnetBox &B, const VecBox &V, const int *grid2mat, const double *r, const double *rr, int n, double *z,
 	hipStream_t s)
 {

@@ -117,6 +117,8 @@ struct VecBox {
 	int n[3];
 };
 void unetApplyCompact(Unet &U, const UnetBox &B, const int *grid2mat, const double *r, const double *rr, int n, double *z, hipStream_t s);
+// The gather of unetApplyCompact alone: compact r scaled to unit RMS into U.buf.in.
+void unetGatherCompact(Unet &U, const UnetBox &B, const int *grid2mat, const double *r, const double *rr, int n, hipStream_t s);
 void unetApplyVecBox(Unet &U, const UnetBox &B, const VecBox &V, const int *grid2mat, const double *r, const double *rr, int n, double *z,
 	hipStream_t s);
 

@@ -48,7 +48,7 @@ class PcgOpts(ctypes.Structure):
 class PcgStats(ctypes.Structure):
     _fields_ = [("unknowns", ctypes.c_int), ("iterations", ctypes.c_int), ("outer_iterations", ctypes.c_int),
                 ("fell_back", ctypes.c_int), ("relative_residual", ctypes.c_double),
-                ("preconditioner_applications", ctypes.c_int)]
+                ("preconditioner_applications", ctypes.c_int), ("hbm_bytes", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -134,6 +134,7 @@ def lib():
             "garm_unet_train": (i, [vp, ip, i, vp, vp, vp, i, ctypes.c_float, ctypes.c_ulonglong, vp]),
             "garm_unet_time_forward": (i, [vp, ip, i, dp, dp]),
             "garm_attach_preconditioner": (i, [vp, vp]),
+            "garm_unet_train_on_system": (i, [vp, vp, i, ctypes.c_float, d, ctypes.c_ulonglong, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -317,6 +318,13 @@ class Liquid:
 
     def synchronize(self):
         self._check(lib().garm_synchronize(self.h))
+
+    def train_network(self, net, steps, lr=1e-3, operator_fraction=0.75, seed=0):
+        """Trains `net` on pairs made from this context's last pressure system
+        (garm_unet_train_on_system); returns the per-step losses."""
+        hist = np.zeros(max(steps, 1), dtype=np.float32)
+        self._check(lib().garm_unet_train_on_system(net.h, self.h, steps, lr, operator_fraction, seed, hist.ctypes.data))
+        return hist[:steps]
 
     def attach_preconditioner(self, net):
         """Use `net` (a Unet, or None) for PCG_LEARNED solves."""

@@ -60,3 +60,12 @@ def droplet(sim, g, seed=0, radius=0.08, pool=0.3, height=0.7, speed=-2.0):
         off += len(pos)
     sim.set(9, u)
     return c
+
+
+def dam_break_with_drop(sim, g, extents, drop, radius=0.08):
+    """A dam-break box (extents from the origin) and a spherical drop of
+    `radius` at `drop`, both at rest: the scenes of the network's training
+    runs (box extents and drop positions drawn by the caller)."""
+    dim = g["dim"]
+    sim.union_box([0.0] * dim, list(extents[:dim]))
+    sim.union_sphere(list(drop[:dim]), radius)

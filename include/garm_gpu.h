@@ -76,9 +76,12 @@ typedef enum garm_pcg_mode {
 	                            FP64, residual replaced by the FP64 b - Ax at every 1e3
 	                            reduction, FP64 reductions, FP64 convergence test */
 	GARM_PCG_LEARNED = 2,    /* flexible CG preconditioned by the U-Net attached with
-	                            garm_attach_preconditioner; when an application does not
-	                            halve the residual, fallback_mode takes over; judged by
-	                            the FP64 b - Ax */
+	                            garm_attach_preconditioner (Jacobi pre- and post-smoothing
+	                            around it, its correction scaled to minimise the A-norm
+	                            error); once the A-norm error stops halving per iteration
+	                            (an iteration's energy decrease (r.p)^2/(p.Ap) above a
+	                            quarter of the one before), fallback_mode takes over;
+	                            judged by the FP64 b - Ax */
 	GARM_PCG_MULTIGRID = 3   /* flexible CG preconditioned by an FP32 multigrid V-cycle
 	                            (Galerkin coarsening of the assembled operator on the
 	                            bounding box of the unknowns, red-black Gauss-Seidel),
@@ -105,6 +108,9 @@ typedef struct garm_pcg_stats {
 	double relative_residual;/* final ||b-Ax||/||b|| computed in FP64 */
 	int preconditioner_applications; /* learned: flexible-CG iterations with the network
 	                            (included in iterations) */
+	double hbm_bytes;        /* algorithmic HBM bytes of the solver's iterations (each array a
+	                            kernel must touch read or written once per launch; the
+	                            network's own activations are excluded: it is bound by MFMA) */
 } garm_pcg_stats;
 
 /* Precision of advection and reinitialisation. */
@@ -318,6 +324,21 @@ int garm_unet_train(garm_unet *net, const int dims[3], int n_pairs, const float 
  * and the multiply-add FLOPs of one pass (2 * 27 * cin * cout per voxel of
  * every layer's level, plus the 1x1 output). */
 int garm_unet_time_forward(garm_unet *net, const int dims[3], int reps, double *avg_ms, double *flops);
+/* `steps` Adam steps (beta 0.9 / 0.999) of the network on pairs made on the
+ * device from the pressure system of the context's last projection: with
+ * probability operator_fraction an operator pair (A e, e), A that
+ * projection's matrix (Projector::buildLinearSystem, Projector.cpp:30-68) and
+ * e a random smooth field on its unknowns (12 plane waves, wavelengths
+ * log-uniform from 4 cells to the unknowns' extent or 32 cells, whichever is
+ * smaller -- about the network's receptive field --, a random power-law
+ * spectrum) -- the error a Jacobi-smoothed flexible-CG residual leaves --,
+ * otherwise the projection's own (divergence, pressure) pair with a random
+ * sign.  Inputs are scaled to unit RMS over the unknowns, the loss is the
+ * masked squared error, the network grid is the unknowns' bounding box with
+ * two cells of context.  Nothing leaves the device but loss_history
+ * (optional, `steps` floats).  Seeded; no counterpart in the reference. */
+int garm_unet_train_on_system(garm_unet *net, garm_ctx *ctx, int steps, float lr, double operator_fraction,
+	unsigned long long seed, float *loss_history);
 /* Uses `net` for GARM_PCG_LEARNED solves of this context (NULL detaches).
  * The network must outlive the attachment and live on the same device. */
 int garm_attach_preconditioner(garm_ctx *ctx, garm_unet *net);

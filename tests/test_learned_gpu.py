@@ -140,3 +140,72 @@ def test_learned_with_multigrid_fallback_converges_fast(trained):
         f"mixed Jacobi {sj['iterations']}; rel residual {st['relative_residual']:.2e}")
     assert st["rc"] == 0 and st["relative_residual"] <= 1e-6
     assert st["iterations"] * 3 <= sj["iterations"]
+
+
+# ----------------------------------------------- training on the solve's own system
+@pytest.fixture(scope="module")
+def trained_on_system():
+    """A network trained online (garm_unet_train_on_system) on 24 steps of a
+    64^3 dam break with a drop: after every step, Adam steps on that step's
+    (divergence, pressure) pair and on operator pairs (A e, e)."""
+    g = scenes.grid(64, 3)
+    sim = capi.Liquid(3, g["res"], g["dx"], g["origin"], g["bw"])
+    scenes.dam_break_with_drop(sim, g, (0.5, 0.45, 0.6), (0.7, 0.7, 0.4))
+    opts = capi.step_opts(reinit_interval=5, precision=capi.FP32, mode=capi.PCG_MULTIGRID, tol=1e-8)
+    sim.initialize(capi.step_opts(tol=1e-8, max_iterations=20000))
+    net = capi.Unet(seed=5)
+    clk = FrameClock(g["dx"], 1.0)
+    losses = []
+    for s in range(24):
+        sim.step(clk.next_dt(sim.max_speed()), s, opts)
+        losses.append(sim.train_network(net, 40, lr=1e-3, operator_fraction=0.75, seed=100 + s))
+    losses = np.concatenate(losses)
+    k = len(losses) // 10
+    say(f"train on system: loss {losses[:k].mean():.3f} -> {losses[-k:].mean():.3f} over {len(losses)} Adam steps")
+    state = {f: sim.get(f) for f in STATE}
+    dt = clk.next_dt(sim.max_speed())
+    return g, net, state, dt, losses
+
+
+def test_training_on_the_system_learns(trained_on_system):
+    _, _, _, _, losses = trained_on_system
+    k = len(losses) // 10
+    # unit-RMS targets: 1 is the loss of predicting 0
+    assert losses[-k:].mean() < 0.7 * losses[:k].mean() and losses[-k:].mean() < 0.8
+
+
+def test_network_inside_the_solve_against_the_same_solve_without_it(trained_on_system):
+    """The same flexible CG with the trained network and with a network whose
+    output is zero (the cycle is then two damped Jacobi steps): the A-norm
+    error |p* - p|_A / |p*|_A (p* the FP64 solution to 1e-12) left by a
+    five-iteration budget (network iterations, then the mixed solver for the
+    rest, the same for both) -- the quantity conjugate gradients minimise --,
+    and the whole solve to 1e-6 with each fallback."""
+    g, net, state, dt, _ = trained_on_system
+    ref = _prepared(g, state, dt)
+    ref.project(mode=capi.PCG_FP64, tol=1e-12, max_iterations=20000)
+    pstar = ref.get(capi.PRESSURE)
+    a_pstar = np.dot(pstar, ref.apply_pressure_system(pstar))
+    zero = capi.Unet(seed=0)
+    zero.set_params(np.zeros(capi.Unet.param_count(), np.float32))
+    res = {}
+    for name, network in (("trained", net), ("zero", zero)):
+        d = _prepared(g, state, dt)
+        d.attach_preconditioner(network)
+        d.project(mode=capi.PCG_LEARNED, tol=1e-12, max_iterations=5, raise_on_noconv=False)
+        e = pstar - d.get(capi.PRESSURE)
+        err = np.sqrt(max(0.0, np.dot(e, ref.apply_pressure_system(e))) / a_pstar)
+        full = {}
+        for fb in (0, capi.PCG_MULTIGRID):
+            f = _prepared(g, state, dt)
+            f.attach_preconditioner(network)
+            full[fb] = f.project(mode=capi.PCG_LEARNED, tol=1e-6, max_iterations=5000, fallback_mode=fb)
+            assert full[fb]["rc"] == 0 and full[fb]["relative_residual"] <= 1e-6
+        res[name] = (err, full)
+        say(f"  {name}: A-norm error after 5 iterations {err:.3e}; "
+            f"to 1e-6 with Jacobi fallback {full[0]['iterations']} iterations ({full[0]['preconditioner_applications']} with the network), "
+            f"with V-cycle fallback {full[capi.PCG_MULTIGRID]['iterations']} ({full[capi.PCG_MULTIGRID]['preconditioner_applications']})")
+    # what the trained network gains inside the solve: less A-norm error after the same iterations
+    assert res["trained"][0] < 0.9 * res["zero"][0]
+    # the energy rule keeps the network for at least two applications
+    assert res["trained"][1][capi.PCG_MULTIGRID]["preconditioner_applications"] >= 2
