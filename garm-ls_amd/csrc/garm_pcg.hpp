@@ -53,6 +53,30 @@ int solvePcg64(int D, const Rows<double> &A, const double *b, int n, const garm_
 int solveMixed(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const garm_pcg_opts &o,
 	Krylov &K, hipStream_t s, garm_pcg_stats *st, std::string &err);
 
+// The bounding box of the unknowns: origin lo and sizes n in the cell grid
+// of sizes gn; mat2grid gives each unknown's cell.
+struct BoxLayout {
+	int lo[3], n[3], gn[3];
+	const int *mat2grid;
+};
+
+// Storage of the box solvers (grown as needed).
+struct BoxKrylov {
+	double *pool = nullptr;
+	size_t cap = 0;            // doubles
+	unsigned *boxOf = nullptr; // box cell of every unknown
+	size_t ocap = 0;
+	double *partials = nullptr;  // per-workgroup partials of the z-walking product
+	~BoxKrylov();
+};
+
+// Jacobi PCG on the box of the unknowns with the operator held per cell
+// (diagonal and -x, -y, -z face weights), two launches per iteration: FP64
+// (mixed = false, the plain path) or FP32 refined in FP64 (mixed = true, the
+// same contract as solveMixed).  The solution lands in K.x (compact).
+int solveBoxPcg(int D, bool mixed, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const BoxLayout &L,
+	const garm_pcg_opts &o, Krylov &K, BoxKrylov &B, hipStream_t s, garm_pcg_stats *st, std::string &err);
+
 // Launch one solver kernel `reps` times for timing (which: 0 FP64 SpMV,
 // 1 FP64 x/r update, 2 FP32 SpMV, 3 FP32 update); x and r are left unchanged.
 int timeSolverKernel(int D, const Rows<double> &A64, const Rows<float> &A32, int n, int which, int reps, Krylov &K,

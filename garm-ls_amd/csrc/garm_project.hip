@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <vector>
 
@@ -52,11 +53,19 @@ __device__ __forceinline__ int blockExclusiveScan(int v, int *total)
 	return before + x - v;
 }
 
+// box = (min i, min j, min k, -max i, -max j, -max k) of the unknowns
+__global__ void kBoxInit(int *box)
+{
+	if (threadIdx.x < 6) box[threadIdx.x] = INT_MAX;
+}
+
 template <int D>
-__global__ __launch_bounds__(kBlock) void kUnknownFlags(GridDev g, FaceFracs fr, const double *phi, uint8_t *flags, int *tileCount, size_t N)
+__global__ __launch_bounds__(kBlock) void kUnknownFlags(GridDev g, FaceFracs fr, const double *phi, uint8_t *flags, int *tileCount, size_t N,
+	int *box)
 {
 	const size_t base = size_t(blockIdx.x) * kScanTile;
 	int cnt = 0;
+	int bmin[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MAX, INT_MAX, INT_MAX};
 	for (int sub = 0; sub < kScanSub; sub++) {
 		const size_t id = base + size_t(sub) * kBlock + threadIdx.x;
 		if (id >= N) break;
@@ -75,10 +84,35 @@ __global__ __launch_bounds__(kBlock) void kUnknownFlags(GridDev g, FaceFracs fr,
 		}
 		flags[id] = uint8_t(flag);
 		cnt += flag;
+		if (flag) {
+			const int i = int(id % size_t(g.n[0]));
+			const int j = D == 2 ? int(id / size_t(g.n[0])) : int((id / size_t(g.n[0])) % size_t(g.n[1]));
+			const int k = D == 2 ? 0 : int(id / (size_t(g.n[0]) * g.n[1]));
+			bmin[0] = min(bmin[0], i); bmin[1] = min(bmin[1], j); bmin[2] = min(bmin[2], k);
+			bmin[3] = min(bmin[3], -i); bmin[4] = min(bmin[4], -j); bmin[5] = min(bmin[5], -k);
+		}
 	}
 	int total;
 	(void)blockExclusiveScan(cnt, &total);
 	if (threadIdx.x == 0) tileCount[blockIdx.x] = total;
+	// the tile's bounding box: wave minima, the workgroup's through LDS, then
+	// one atomic per bound and tile holding unknowns (integer minima: the
+	// result does not depend on the order)
+	__shared__ int wmin[6][kBlock / kWave];
+	const int lane = int(threadIdx.x) & (kWave - 1), wid = int(threadIdx.x) / kWave;
+#pragma unroll
+	for (int q = 0; q < 6; q++) {
+		int v = bmin[q];
+#pragma unroll
+		for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_down(v, off, kWave));
+		if (lane == 0) wmin[q][wid] = v;
+	}
+	__syncthreads();
+	if (threadIdx.x < 6 && total > 0) {
+		int v = INT_MAX;
+		for (int w = 0; w < kBlock / kWave; w++) v = min(v, wmin[threadIdx.x][w]);
+		atomicMin(&box[threadIdx.x], v);
+	}
 }
 
 // tileBase[t] = unknowns in tiles before t; *count = all unknowns
@@ -242,7 +276,7 @@ bool Projection::reserveGrid(size_t N, std::string &err)
 	if (flags) return true;
 	tiles = int((N + kScanTile - 1) / kScanTile);
 	if (hipMalloc(&flags, N) != hipSuccess || hipMalloc(&grid2mat, N * sizeof(int)) != hipSuccess ||
-		hipMalloc(&tile_count, size_t(tiles) * sizeof(int)) != hipSuccess || hipMalloc(&tile_base, size_t(tiles + 1) * sizeof(int)) != hipSuccess) {
+		hipMalloc(&tile_count, size_t(tiles) * sizeof(int)) != hipSuccess || hipMalloc(&tile_base, size_t(tiles + 8) * sizeof(int)) != hipSuccess) {
 		err = "hipMalloc failed for unknown maps";
 		return false;
 	}
@@ -284,15 +318,22 @@ static FaceFracs faceArrays(const Fields &F, double *const *a)
 template <int D>
 static int numberUnknowns(const Fields &F, Projection &P, bool mixed, hipStream_t s, std::string &err)
 {
+	// tile_base[tiles] receives the count, tile_base[tiles + 1 ..] the bounding box
+	int *box = P.tile_base + P.tiles + 1;
+	hipLaunchKernelGGL(kBoxInit, dim3(1), dim3(64), 0, s, box);
 	hipLaunchKernelGGL(kUnknownFlags<D>, dim3(P.tiles), dim3(kBlock), 0, s, F.cell, faceArrays(F, F.frac), (const double *)F.phi, P.flags,
-		P.tile_count, F.N);
+		P.tile_count, F.N, box);
 	hipLaunchKernelGGL(kScanTiles, dim3(1), dim3(kBlock), 0, s, (const int *)P.tile_count, P.tiles, P.tile_base, P.tile_base + P.tiles);
 	int *cnt = F.host_ints;
-	if (hipMemcpyAsync(cnt, P.tile_base + P.tiles, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess || spinWait(s, F.wait) != hipSuccess) {
+	if (hipMemcpyAsync(cnt, P.tile_base + P.tiles, 7 * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess || spinWait(s, F.wait) != hipSuccess) {
 		err = "reading the unknown count failed";
 		return -1;
 	}
 	const int n = cnt[0];
+	for (int a = 0; a < 3; a++) {
+		P.box_lo[a] = a < D ? cnt[1 + a] : 0;
+		P.box_n[a] = a < D ? -cnt[4 + a] - cnt[1 + a] + 1 : 1;
+	}
 	if (!P.reserveRows(D, size_t(n), mixed, err)) return -1;
 	hipLaunchKernelGGL(kNumberUnknowns, dim3(P.tiles), dim3(kBlock), 0, s, (const uint8_t *)P.flags, (const int *)P.tile_base, P.grid2mat,
 		P.mat2grid, F.N);
@@ -336,8 +377,20 @@ static int projectD(Fields &F, Projection &P, const garm_pcg_opts &o, garm_pcg_s
 				err = "copying the learned solution failed";
 				rc = GARM_EHIP;
 			}
-		} else if (mixed) rc = solveMixed(D, P.rows, P.rows32, P.rhs, n, o, P.K, s, st, err);
-		else rc = solvePcg64(D, P.rows, P.rhs, n, o, P.K, s, st, err);
+		} else {
+			// the box layout while the box holds at most kBoxSolverRatio cells
+			// per unknown; a sparser box (a splash's bounding box is mostly air,
+			// and every vector operation covers it) keeps the rows
+			const double boxCells = double(P.box_n[0]) * P.box_n[1] * P.box_n[2];
+			P.used_box = boxCells <= kBoxSolverRatio * n;
+			if (P.used_box) {
+				BoxLayout L{};
+				for (int a = 0; a < 3; a++) { L.lo[a] = P.box_lo[a]; L.n[a] = P.box_n[a]; L.gn[a] = F.cell.n[a]; }
+				L.mat2grid = P.mat2grid;
+				rc = solveBoxPcg(D, mixed, P.rows, P.rows32, P.rhs, n, L, o, P.K, P.BK, s, st, err);
+			} else if (mixed) rc = solveMixed(D, P.rows, P.rows32, P.rhs, n, o, P.K, s, st, err);
+			else rc = solvePcg64(D, P.rows, P.rhs, n, o, P.K, s, st, err);
+		}
 	}
 	// the velocity is updated with whatever the solver reached (the reference
 	// exits instead when its solver fails, LinearSystem.h:34-37)

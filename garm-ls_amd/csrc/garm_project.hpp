@@ -16,11 +16,18 @@ struct FaceFracs {
 	GridDev g[3];
 };
 
+// Largest ratio of box cells to unknowns at which the Jacobi PCG solvers
+// run on the box of the unknowns instead of on compact rows.
+constexpr double kBoxSolverRatio = 1.5;
+
 struct Projection {
 	uint8_t *flags = nullptr;                  // 1 where a cell is an unknown
 	int *grid2mat = nullptr, *mat2grid = nullptr;
-	int *tile_count = nullptr, *tile_base = nullptr;   // unknown numbering: per-tile counts, offsets (+ total)
+	int *tile_count = nullptr, *tile_base = nullptr;   // unknown numbering: per-tile counts, offsets (+ total, bounding box)
 	int tiles = 0;
+	int box_lo[3] = {0, 0, 0}, box_n[3] = {1, 1, 1};   // bounding box of the unknowns of the last numbering
+	bool used_box = false;                               // the last Jacobi-PCG solve ran on that box
+	BoxKrylov BK;
 	Rows<double> rows{};
 	Rows<float> rows32{};
 	double *rhs = nullptr;
