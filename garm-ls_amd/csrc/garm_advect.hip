@@ -1154,15 +1154,19 @@ void launchCompose(const Fields &F, hipStream_t s, bool fp32)
 	else composeD<3>(F, s);
 }
 
+void launchLevelSetRecords(const Fields &F, const double *phi, const double *const *grad, double *out64, float *out32, hipStream_t s)
+{
+	const double *g = grad[0];
+	if (F.D == 2)
+		hipLaunchKernelGGL(kPackRefLevelSet<2>, gridDims(F.cell, 2), gridBlock(), 0, s, F.cell, phi, g, (const double *)grad[1], g, out64, out32);
+	else
+		hipLaunchKernelGGL(kPackRefLevelSet<3>, gridDims(F.cell, 3), gridBlock(), 0, s, F.cell, phi, g, (const double *)grad[1],
+			(const double *)grad[2], out64, out32);
+}
+
 void launchRefLevelSetRecords(const Fields &F, hipStream_t s)
 {
-	const double *g = F.phi_ref_grad[0];
-	if (F.D == 2)
-		hipLaunchKernelGGL(kPackRefLevelSet<2>, gridDims(F.cell, 2), gridBlock(), 0, s, F.cell, (const double *)F.phi_ref, g,
-			(const double *)F.phi_ref_grad[1], g, F.ref_rec64, F.ref_rec32);
-	else
-		hipLaunchKernelGGL(kPackRefLevelSet<3>, gridDims(F.cell, 3), gridBlock(), 0, s, F.cell, (const double *)F.phi_ref, g,
-			(const double *)F.phi_ref_grad[1], (const double *)F.phi_ref_grad[2], F.ref_rec64, F.ref_rec32);
+	launchLevelSetRecords(F, F.phi_ref, F.phi_ref_grad, F.ref_rec64, F.ref_rec32, s);
 }
 
 void launchGradient4(const Fields &F, const double *src, double *const *dst, hipStream_t s)

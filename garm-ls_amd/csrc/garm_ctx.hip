@@ -546,6 +546,13 @@ int garm_restart(garm_ctx *c)
 	return launched(c, "restart");
 }
 
+int garm_restart_newton(garm_ctx *c)
+{
+	if (!c) return GARM_EINVAL;
+	launchRestartNewton(c->F, c->s);
+	return launched(c, "Newton restart");
+}
+
 int garm_project(garm_ctx *c, const garm_pcg_opts *o, garm_pcg_stats *st)
 {
 	if (!c || !o) return GARM_EINVAL;
@@ -715,6 +722,7 @@ int garm_step(garm_ctx *c, double dt, int step_index, const garm_step_opts *o, g
 	int rc;
 
 	if (o->advect_precision != GARM_FP64 && o->advect_precision != GARM_FP32) return fail(c, GARM_EINVAL, "unknown precision");
+	if (o->restart_mode != GARM_RESTART_PLAIN && o->restart_mode != GARM_RESTART_NEWTON) return fail(c, GARM_EINVAL, "unknown restart mode");
 	const bool fp32 = o->advect_precision == GARM_FP32;
 	(void)hipEventRecord(ev[0], c->s);
 	launchAdvectRefMap(F, dt, c->s, fp32);
@@ -735,7 +743,8 @@ int garm_step(garm_ctx *c, double dt, int step_index, const garm_step_opts *o, g
 		// restart always captures a reinitialised level set; with a longer
 		// cadence the restart step reinitialises first
 		if (!st->reinitialized && (rc = doReinit(c, o->reinit_iterations, o->reinit_cfl, fp32))) return rc;
-		doRestart(c);
+		if (o->restart_mode == GARM_RESTART_NEWTON) launchRestartNewton(F, c->s);
+		else doRestart(c);
 		launchCompose(F, c->s, fp32);
 		launchExtrapolateCollider(F, c->s);
 		st->restarted = 1;

@@ -64,6 +64,12 @@ void launchAdvectVelocity(const Fields &F, double dt, hipStream_t s, bool fp32 =
 void launchCompose(const Fields &F, hipStream_t s, bool fp32 = false);
 // Hermite records of the reference level set (after phi_ref or its gradient changes)
 void launchRefLevelSetRecords(const Fields &F, hipStream_t s);
+// the same records of any scalar field and its gradient (D arrays)
+void launchLevelSetRecords(const Fields &F, const double *phi, const double *const *grad, double *out64, float *out32, hipStream_t s);
+// LevelSetLiquid::restart of the default build: Hybrid Newton from
+// (phi_ori, phi_grad) with closest-point guesses from phi, characteristic
+// steps elsewhere; result in phi_ref / phi_ref_grad, reference map reset
+void launchRestartNewton(const Fields &F, hipStream_t s);
 void launchGradient4(const Fields &F, const double *src, double *const *dst, hipStream_t s);
 void launchReinit(const Fields &F, int iterations, double cfl, hipStream_t s, bool fp32 = false);
 // one Runge-Kutta stage L(phi) -> scratch[0], for timing (the FP32 stage

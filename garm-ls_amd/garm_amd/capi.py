@@ -33,6 +33,8 @@ VOLUME_REFERENCE, VOLUME_CORRECTED = 0, 1
 
 # garm_pcg_mode
 PCG_FP64, PCG_MIXED, PCG_LEARNED, PCG_MULTIGRID = 0, 1, 2, 3
+# garm_restart_mode
+RESTART_PLAIN, RESTART_NEWTON = 0, 1
 FP64, FP32 = 0, 1
 
 GARM_OK, GARM_EINVAL, GARM_EHIP, GARM_ENOCONV, GARM_ENOMEM, GARM_ENODEV = 0, -1, -2, -3, -4, -5
@@ -56,7 +58,7 @@ class PcgStats(ctypes.Structure):
 
 class StepOpts(ctypes.Structure):
     _fields_ = [("reinit_interval", ctypes.c_int), ("reinit_iterations", ctypes.c_int), ("reinit_cfl", ctypes.c_double),
-                ("advect_precision", ctypes.c_int), ("pcg", PcgOpts)]
+                ("advect_precision", ctypes.c_int), ("pcg", PcgOpts), ("restart_mode", ctypes.c_int)]
 
 
 class StepStats(ctypes.Structure):
@@ -110,6 +112,7 @@ def lib():
             "garm_apply_gravity": (i, [vp, d]),
             "garm_check_refmap": (i, [vp, d, d, d, ip]),
             "garm_restart": (i, [vp]),
+            "garm_restart_newton": (i, [vp]),
             "garm_project": (i, [vp, ctypes.POINTER(PcgOpts), ctypes.POINTER(PcgStats)]),
             "garm_step": (i, [vp, d, i, ctypes.POINTER(StepOpts), ctypes.POINTER(StepStats)]),
             "garm_max_speed": (i, [vp, dp]),
@@ -157,8 +160,9 @@ def _d3(v):
 
 
 def step_opts(reinit_interval=1, reinit_iterations=30, reinit_cfl=0.9, precision=FP64, mode=PCG_FP64, tol=1e-5,
-              max_iterations=2000, check_interval=8, fallback_mode=0):
+              max_iterations=2000, check_interval=8, fallback_mode=0, restart_mode=RESTART_PLAIN):
     o = StepOpts()
+    o.restart_mode = restart_mode
     o.reinit_interval = reinit_interval
     o.reinit_iterations = reinit_iterations
     o.reinit_cfl = reinit_cfl
@@ -256,6 +260,9 @@ class Liquid:
 
     def restart(self):
         self._check(lib().garm_restart(self.h))
+
+    def restart_newton(self):
+        self._check(lib().garm_restart_newton(self.h))
 
     def project(self, mode=PCG_FP64, tol=1e-5, max_iterations=2000, check_interval=8, raise_on_noconv=True,
                 fallback_mode=0):

@@ -125,7 +125,19 @@ typedef struct garm_step_opts {
 	                            (computed in FP32, stored in FP64; garm_initialize's
 	                            reinitialisation is always FP64) */
 	garm_pcg_opts pcg;
+	int restart_mode;        /* garm_restart_mode of a step whose reference map is too
+	                            distorted (0, the plain branch, unless set) */
 } garm_step_opts;
+
+/* How a step restarts the reference map (LevelSetLiquid::restart,
+ * LevelSetLiquid.cpp:1285-1352). */
+typedef enum garm_restart_mode {
+	GARM_RESTART_PLAIN = 0,      /* the branch without Hybrid Newton (:1331-1352): phi_ref =
+	                                phi, grad phi_ref by computeFirst<4> */
+	GARM_RESTART_NEWTON = 1      /* the default build (RM, GA, Newton) with enable-fmm=n
+	                                (:1290-1303): Hybrid Newton closest points and 20
+	                                characteristic steps */
+} garm_restart_mode;
 
 typedef struct garm_step_stats {
 	double t_advect_ms;      /* reference bucket tadvect  (LevelSetLiquid.h:103-109) */
@@ -196,6 +208,15 @@ int garm_apply_gravity(garm_ctx *ctx, double dt);
 int garm_check_refmap(garm_ctx *ctx, double threshold, double band_steps, double ratio, int *ok);
 /* restart without Hybrid Newton (LevelSetLiquid.cpp:1331-1352). */
 int garm_restart(garm_ctx *ctx);
+/* restart of the default build with enable-fmm=n (LevelSetLiquid.cpp:1290-1303):
+ * from x - phi(x) grad phi(x) (the cubic view of _levelSet) the modified Newton
+ * iteration finds the closest point on the Hermite interpolant of
+ * (_oriLevelSet, _levelSetGrad) for the cells next to a sign change and one
+ * layer more (HybridNewton.h:119-136, 147-467; HybridNewton.cpp:64-79,
+ * 103-123); the other cells take 20 characteristic RK3 steps of 0.9 dx
+ * (HybridNewton.cpp:30-49).  The result becomes _refLevelSet and its
+ * gradient; the reference map is reset. */
+int garm_restart_newton(garm_ctx *ctx);
 /* Projector::project (Projector.cpp:13-20) with matrix-free PCG in place of
  * Eigen's IC-preconditioned CG (LinearSystem.h:23-39). */
 int garm_project(garm_ctx *ctx, const garm_pcg_opts *opts, garm_pcg_stats *stats);

@@ -347,6 +347,8 @@ struct Liquid {
 	std::vector<double> phi, phiGrad, xi, J, phiTmp, phiOri, phiRef, phiRefGrad;
 	std::vector<u8> advRegion;
 	Hermite<D, 1> refView;   // _refLevelSetView's mixed derivatives
+	std::vector<uint8_t> newtonMark;   // HybridNewton::_mark of the last Newton restart (_reinitmark)
+	bool newtonRestart = false;        // step() restarts with Hybrid Newton (the default build) instead of the plain branch
 
 	std::vector<double> nodeDist, cellDist, frac[D], bvel[D], bnormal[D];
 
@@ -943,6 +945,181 @@ struct Liquid {
 		resetRefMap();
 	}
 
+	// ------------------------------------------ Hybrid-Newton restart ---
+	// LevelSetLiquid::restart of the default build (RM, GA, Newton) with
+	// enable-fmm=n (LevelSetLiquid.cpp:1290-1303): closest-point guesses from
+	// the cubic view of phi, then HybridNewton::perform on (phiOri, phiGrad)
+	// (HybridNewton.cpp:18-28): interface cells by the modified Newton
+	// iteration (HybridNewton.h:119-136, 147-467 with ENABLE_MODIFIED),
+	// every other cell by 20 characteristic steps (HybridNewton.cpp:30-49);
+	// the result becomes phiRef / phiRefGrad.  The restatement keeps the
+	// reference's operation order (Eigen's coefficient-wise expressions).
+	static void normalized(const double *v, double *out)
+	{
+		double s = 0;
+		for (int a = 0; a < D; a++) s += v[a] * v[a];
+		if (s > 0) { const double n = std::sqrt(s); for (int a = 0; a < D; a++) out[a] = v[a] / n; }
+		else for (int a = 0; a < D; a++) out[a] = v[a];
+	}
+
+	// ScalarFieldWrapper<CubicIntrpl>::gradient (ScalarField.h:41-47):
+	// linear weights of the corners times getFirst<2> at each (unclamped)
+	// corner coordinate, whose stencil is clamped (Derivatives.h:51-57)
+	void cubicViewGradient(const double *f, const double *pos, double *grad) const
+	{
+		int l[3] = {0, 0, 0}; double fr[3];
+		cell.lower(pos, l);
+		cell.frac(pos, l, fr);
+		double w[2][3];
+		for (int a = 0; a < D; a++) { w[0][a] = 1 - fr[a]; w[1][a] = fr[a]; }
+		for (int a = 0; a < D; a++) grad[a] = 0;
+		for (int k = 0; k < (D == 3 ? 2 : 1); k++)
+			for (int j = 0; j < 2; j++)
+				for (int i = 0; i < 2; i++) {
+					const int c[3] = {l[0] + i, l[1] + j, l[2] + k};
+					const double wt = D == 2 ? w[i][0] * w[j][1] : w[i][0] * w[j][1] * w[k][2];
+					for (int a = 0; a < D; a++) {
+						int cp[3] = {c[0], c[1], c[2]}, cm[3] = {c[0], c[1], c[2]};
+						cp[a] += 1;
+						cm[a] -= 1;
+						grad[a] += (f[cell.at(cp)] - f[cell.at(cm)]) / 2 * cell.inv * wt;
+					}
+				}
+	}
+
+	// HybridNewton::getClosestPosition (HybridNewton.h:147-467): ENABLE_MODIFIED,
+	// no FMM, epsl = 0; returns the position, *converged the flag
+	void closestPosition(const Hermite<D, 1> &H, const double *val, const double *drv, const double *start, const double *init, double *pos,
+		bool *converged) const
+	{
+		const double dx = cell.dx;
+		const double tolerance = 1e-3 * dx * dx * (D == 2 ? 1 : dx);
+		const double radius = .9 * dx;
+		for (int a = 0; a < D; a++) pos[a] = init[a];
+		*converged = false;
+		for (int iter = 0; iter < 20; iter++) {
+			double phi, psi[3];
+			H.interpolateBoth(cell, val, drv, pos, &phi, psi);
+			phi = phi - 0.0;
+			double sqn = 0, dot = 0, d[3], delta[3];
+			for (int a = 0; a < D; a++) sqn += psi[a] * psi[a];
+			for (int a = 0; a < D; a++) d[a] = start[a] - pos[a];
+			for (int a = 0; a < D; a++) dot += d[a] * psi[a];
+			for (int a = 0; a < D; a++) {
+				const double d1 = -phi * psi[a] / sqn;
+				const double d2 = d[a] - psi[a] * dot / sqn;
+				delta[a] = d1 + d2;
+			}
+			double n2 = 0;
+			for (int a = 0; a < D; a++) n2 += delta[a] * delta[a];
+			double norm = std::sqrt(n2);
+			if (norm > radius) {
+				const double sc = radius / norm;
+				for (int a = 0; a < D; a++) delta[a] *= sc;
+				norm = radius;
+			}
+			for (int a = 0; a < D; a++) pos[a] += delta[a];
+			if (norm < tolerance) { *converged = true; break; }
+		}
+	}
+
+	void restartNewton(int maxSteps = 20, double cfl = 0.9)
+	{
+		const size_t N = cell.count();
+		// LevelSetLiquid.cpp:1292-1296: points = pos - phi(pos) grad(pos)
+		std::vector<double> pts(N * D);
+		parFor<D>(cell, [&](const int *c) {
+			double p[3], g[3];
+			cell.position(c, p);
+			const double v = cubic<D>(cell, phi.data(), p);
+			cubicViewGradient(phi.data(), p, g);
+			const size_t id = cell.index(c);
+			for (int a = 0; a < D; a++) pts[id * D + a] = p[a] - v * g[a];
+		});
+		std::vector<double> f = phiOri, psi = phiGrad;
+		// prepareInterface + extrapolate(1) (HybridNewton.cpp:64-79, 103-123)
+		std::vector<uint8_t> mark0(N, 0), mark(N, 0);
+		parFor<D>(cell, [&](const int *c) {
+			const size_t id = cell.index(c);
+			for (int o = 0; o < 2 * D; o++) {
+				int q[3] = {c[0], c[1], c[2]};
+				q[o >> 1] += (o & 1) ? 1 : -1;
+				if (cell.valid(q) && f[id] * f[cell.index(q)] <= 0) { mark0[id] = 1; break; }
+			}
+		});
+		parFor<D>(cell, [&](const int *c) {
+			const size_t id = cell.index(c);
+			if (mark0[id]) { mark[id] = 1; return; }
+			for (int o = 0; o < 2 * D; o++) {
+				int q[3] = {c[0], c[1], c[2]};
+				q[o >> 1] += (o & 1) ? 1 : -1;
+				if (cell.valid(q) && mark0[cell.index(q)]) { mark[id] = 1; break; }
+			}
+		});
+		newtonMark = mark;
+		// initializeInterface with the guesses (HybridNewton.h:119-136)
+		{
+			Hermite<D, 1> H;
+			H.build(cell, psi.data());
+			std::vector<double> nf = f, npsi = psi;
+			parFor<D>(cell, [&](const int *c) {
+				const size_t id = cell.index(c);
+				if (!mark[id]) return;
+				double p[3], cp[3], g[3], v;
+				cell.position(c, p);
+				bool conv;
+				closestPosition(H, f.data(), psi.data(), p, &pts[id * D], cp, &conv);
+				double n2 = 0;
+				for (int a = 0; a < D; a++) n2 += (p[a] - cp[a]) * (p[a] - cp[a]);
+				nf[id] = std::sqrt(n2) * (f[id] < 0 ? -1 : 1);
+				H.interpolateBoth(cell, f.data(), psi.data(), cp, &v, g);
+				normalized(g, &npsi[id * D]);
+			});
+			f.swap(nf);
+			psi.swap(npsi);
+		}
+		// advect (HybridNewton.cpp:30-49): characteristic steps of length
+		// cfl dx along the normalised gradient, RK3 (Advection.h:43-49)
+		const double dt = cell.dx * cfl;
+		for (int iter = 0; iter < maxSteps; iter++) {
+			Hermite<D, 1> H;
+			H.build(cell, psi.data());
+			std::vector<double> nf = f, npsi = psi;
+			auto vel = [&](const double *x, double *v) {
+				double val, g[3];
+				H.interpolateBoth(cell, f.data(), psi.data(), x, &val, g);
+				bool any = false;
+				for (int a = 0; a < D; a++) any = any || g[a] != 0;
+				if (any) normalized(g, v);
+				else for (int a = 0; a < D; a++) v[a] = a == 0 ? 1 : 0;
+			};
+			parFor<D>(cell, [&](const int *c) {
+				const size_t id = cell.index(c);
+				if (mark[id]) return;
+				const int sgn = f[id] < 0 ? -1 : 1;
+				double p[3], v0[3], v1[3], v2[3], q[3], t[3];
+				cell.position(c, p);
+				const double h = -sgn * dt;
+				vel(p, v0);
+				for (int a = 0; a < D; a++) q[a] = p[a] + v0[a] * h;
+				vel(q, v1);
+				for (int a = 0; a < D; a++) q[a] = p[a] + (v0[a] + v1[a]) * h / 4;
+				vel(q, v2);
+				for (int a = 0; a < D; a++) t[a] = p[a] + (v0[a] + v1[a] + 4 * v2[a]) * h / 6;
+				double val, g[3];
+				H.interpolateBoth(cell, f.data(), psi.data(), t, &val, g);
+				nf[id] = val + sgn * dt;
+				normalized(g, &npsi[id * D]);
+			});
+			f.swap(nf);
+			psi.swap(npsi);
+		}
+		phiRef = f;
+		phiRefGrad = psi;
+		refView.build(cell, phiRefGrad.data());
+		resetRefMap();
+	}
+
 	void resetRefMap()
 	{
 		parFor<D>(cell, [&](const int *c) {
@@ -1210,7 +1387,8 @@ struct Liquid {
 			// captures a reinitialised level set; with a longer cadence the
 			// restart step reinitialises first
 			if (!doReinit) reinit(reinitIters, cfl);
-			restart();
+			if (newtonRestart) restartNewton();
+			else restart();
 			compose();
 			extrapolateCollider();
 			*restarted = 1;
@@ -1399,6 +1577,8 @@ int ora_check_refmap(void *p, double th, double band, double ratio, long long *t
 	return r;
 }
 void ora_restart(void *p) { auto *h = (Handle *)p; DISPATCH(h, L.restart()); }
+void ora_restart_newton(void *p) { auto *h = (Handle *)p; DISPATCH(h, L.restartNewton()); }
+void ora_set_restart_newton(void *p, int on) { auto *h = (Handle *)p; DISPATCH(h, L.newtonRestart = on != 0); }
 int ora_project(void *p, int precond, double tol, int maxIt, int *iters, double *err, int *unknowns)
 {
 	auto *h = (Handle *)p; int r = 0;

@@ -59,6 +59,8 @@ def lib():
             "ora_gravity": (None, [vp, d]),
             "ora_check_refmap": (i, [vp, d, d, d, llp, llp]),
             "ora_restart": (None, [vp]),
+            "ora_restart_newton": (None, [vp]),
+            "ora_set_restart_newton": (None, [vp, i]),
             "ora_project": (i, [vp, i, d, i, ip, dp, ip]),
             "ora_step": (i, [vp, d, i, i, d, i, d, i, ip, ip, dp, ip]),
             "ora_max_speed": (d, [vp]),
@@ -172,6 +174,15 @@ class Oracle:
 
     def restart(self):
         lib().ora_restart(self.h)
+
+    def restart_newton(self):
+        """LevelSetLiquid::restart of the default build: Hybrid Newton with
+        characteristic steps (enable-fmm=n)."""
+        lib().ora_restart_newton(self.h)
+
+    def set_restart_newton(self, on):
+        """step() restarts with Hybrid Newton when on, the plain branch otherwise."""
+        lib().ora_set_restart_newton(self.h, int(bool(on)))
 
     def project(self, precond=PRECOND_JACOBI, tol=1e-10, max_it=20000):
         it, unk, err = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()

@@ -609,3 +609,90 @@ def hand_enforce(sim):
 def test_enforce_partial_fraction_branch_by_hand():
     g, o = mk(16, 2)
     hand_enforce(o)
+
+
+# ------------------------------------------------- Hybrid-Newton restart ----
+def _newton_plane(dim, n, scale):
+    g = scenes.grid(n, dim)
+    o = O.Oracle(dim, g["res"], g["dx"], g["origin"], g["bw"])
+    x = scenes.cell_positions(g)
+    nrm = np.array([0.6, 0.8] if dim == 2 else [0.48, 0.64, 0.6])
+    sd = x @ nrm - 0.45
+    o.set(PHI, sd)                                        # the cubic view's guesses
+    o.set(PHI_ORI, scale * sd)                            # what Newton and the march work on
+    o.set(PHI_GRAD, np.tile(scale * nrm, (len(sd), 1)).ravel())
+    o.restart_newton()
+    return g, o, sd, nrm
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_newton_on_a_plane_returns_the_exact_closest_point(dim):
+    """Cubic Hermite reproduces a linear level set, so the modified Newton
+    iteration lands on the exact closest point: the interface band (cells
+    next to a sign change and one layer more) gets the exact signed
+    distance and the unit normal even from a level set scaled by 2."""
+    g, o, sd, nrm = _newton_plane(dim, 40 if dim == 2 else 24, 2.0)
+    pr, gr = o.get(PHI_REF), o.get(PHI_REF_GRAD).reshape(-1, dim)
+    # within 0.6 dx of the plane every cell has a neighbour across it (the
+    # largest normal component is 0.64 or 0.8): all Newton cells
+    band = (np.abs(sd) <= 0.6 * g["dx"]) & interior_mask(g, 4)
+    assert band.sum() > 10
+    assert np.max(np.abs(pr[band] - sd[band])) <= 1e-13
+    assert np.max(np.abs(gr[band] - nrm)) <= 1e-13
+    # the characteristic march carries the distance outward: two cells on
+    # the band is exact too
+    near = (np.abs(sd) <= 3.0 * g["dx"]) & interior_mask(g, 6)
+    assert np.max(np.abs(pr[near] - sd[near])) <= 1e-6 * g["dx"]
+    # the reference map is reset
+    assert np.allclose(o.get(REFMAP).reshape(-1, dim), scenes.cell_positions(g), atol=0, rtol=0)
+
+
+def test_newton_march_keeps_a_distance_field():
+    """A signed distance (|grad phi| = 1) is a fixed point of the
+    characteristic steps phi(x) <- phi(x - s dt n) + s dt, away from the walls
+    whose clamped data bends the interpolant."""
+    g, o, sd, nrm = _newton_plane(2, 64, 1.0)
+    pr = o.get(PHI_REF)
+    m = interior_mask(g, 24)
+    assert np.max(np.abs(pr[m] - sd[m])) <= 1e-12
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_newton_on_a_sphere_matches_the_distance(dim):
+    """Exact sphere distance and normal as phi_ori and grad phi: the Newton
+    band matches the true distance to the Hermite interpolant's accuracy."""
+    n = 48 if dim == 2 else 32
+    g = scenes.grid(n, dim)
+    o = O.Oracle(dim, g["res"], g["dx"], g["origin"], g["bw"])
+    x = scenes.cell_positions(g)
+    c = np.full(dim, 0.5)
+    r = np.linalg.norm(x - c, axis=1)
+    sd = r - 0.3
+    o.set(PHI, sd)
+    o.set(PHI_ORI, sd)
+    o.set(PHI_GRAD, ((x - c) / r[:, None]).ravel())
+    o.restart_newton()
+    pr, gr = o.get(PHI_REF), o.get(PHI_REF_GRAD).reshape(-1, dim)
+    band = np.abs(sd) <= 1.5 * g["dx"]
+    assert np.max(np.abs(pr[band] - sd[band])) <= 1e-3 * g["dx"]
+    assert np.max(np.abs(gr[band] - (x[band] - c) / r[band, None])) <= 1e-3
+    # a plain restart keeps phi as it is; Newton moves no band value by more
+    # than the interpolant's error
+    assert np.max(np.abs(pr[band] - sd[band])) < np.max(np.abs(sd[band])) * 1e-2
+
+
+def test_newton_marks_sign_changes_and_one_layer():
+    """prepareInterface + extrapolate(1) (HybridNewton.cpp:64-79, 103-123):
+    with phi_ori = x - 0.5 on a 10-cell row the cells next to the sign change
+    and their neighbours are Newton cells; the rest are marched."""
+    o = O.Oracle(2, [10, 6], 0.1, [0.0, 0.0], 0)
+    i, j = np.meshgrid(np.arange(10), np.arange(6))
+    xpos = (i + 0.5) * 0.1
+    phi = (xpos - 0.52).ravel()            # sign change between i = 4 and i = 5
+    o.set(PHI, phi)
+    o.set(PHI_ORI, phi)
+    o.set(PHI_GRAD, np.tile([1.0, 0.0], phi.size))
+    o.restart_newton()
+    pr = o.get(PHI_REF).reshape(6, 10)
+    # the band i = 3..6 is Newton-exact; the march reproduces the rest of a distance field as well
+    assert np.allclose(pr, (xpos - 0.52), atol=1e-13, rtol=0)
