@@ -152,6 +152,38 @@ __device__ __forceinline__ void blockSum(double (&v)[NV], double *out /* block's
 	}
 }
 
+// ---- tile scans --------------------------------------------------------------
+// Index spaces are numbered in tiles of kScanTile consecutive ids, one
+// workgroup per tile (kScanSub passes of kBlock ids).
+constexpr int kScanSub = 16;                   // passes of kBlock cells per tile
+constexpr int kScanTile = kBlock * kScanSub;
+
+// Exclusive scan of one int per thread over a 256-thread workgroup; *total
+// receives the workgroup sum.
+__device__ __forceinline__ int blockExclusiveScan(int v, int *total)
+{
+	__shared__ int waveSum[kBlock / kWave];
+	const int tid = int(threadIdx.x), lane = tid & (kWave - 1), wid = tid / kWave;
+	int x = v;
+#pragma unroll
+	for (int off = 1; off < kWave; off <<= 1) {
+		const int y = __shfl_up(x, off, kWave);
+		if (lane >= off) x += y;
+	}
+	if (lane == kWave - 1) waveSum[wid] = x;
+	__syncthreads();
+	int before = 0, all = 0;
+#pragma unroll
+	for (int w = 0; w < kBlock / kWave; w++) {
+		const int t = waveSum[w];
+		before += w < wid ? t : 0;
+		all += t;
+	}
+	__syncthreads();   // waveSum is reused by the next call
+	*total = all;
+	return before + x - v;
+}
+
 __device__ __forceinline__ int flatBlock() { return int(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)); }
 
 } // namespace garm

@@ -6,6 +6,7 @@
 // (LevelSetLiquid.cpp:608-698).
 #include "garm_internal.hpp"
 #include "garm_project.hpp"
+#include "garm_capillary.hpp"
 
 #include <cmath>
 #include <cstdio>
@@ -28,6 +29,8 @@ struct garm_ctx {
 	double *host_scalars = nullptr;   // pinned
 	size_t bface_rec_cap = 0;         // listed boundary faces F.bface_rec holds room for
 	bool fp32 = false;                // compute precision of the per-operator advection / reinit calls
+	Capillary C;                      // surface tension's capillary system
+	double elapsed = 0;               // LevelSetLiquid::_elapsedTime
 };
 
 namespace {
@@ -180,6 +183,29 @@ void doRestart(garm_ctx *c)
 void doEnforceBoundary(garm_ctx *c)
 {
 	launchEnforceBoundary(c->F, c->s);
+}
+
+// LevelSetLiquid::applySurfaceTension + trickLittleDrop (LevelSetLiquid.cpp:1026-1057)
+int doSurfaceTension(garm_ctx *c, double dt, const garm_surface_tension_opts &o, const garm_pcg_opts &pcg, garm_surface_tension_stats *st)
+{
+	garm_surface_tension_stats local{};
+	if (!st) st = &local;
+	*st = garm_surface_tension_stats{};
+	if (!(dt > 0) || !(o.coeff >= 0) || !(o.tolerance > 0) || o.max_iterations < 0 || !(o.delta_min > 0) || !(o.delta_max > 0) ||
+		!(o.delta_crit_time > 0))
+		return fail(c, GARM_EINVAL, "bad surface-tension settings");
+	Fields &F = c->F;
+	launchExtrapolateAir(F, c->s, true);
+	doEnforceBoundary(c);
+	st->band = o.delta_min + (o.delta_max - o.delta_min) * 2 / (1 + std::exp(c->elapsed / o.delta_crit_time));
+	CapillaryArgs a{o.coeff, st->band, dt, o.tolerance, o.max_iterations, std::max(1, pcg.check_interval)};
+	std::string err;
+	int rc = runCapillary(F, c->C, a, c->s, &st->capillary, err);
+	if (rc != GARM_OK) return fail(c, rc, "capillary solve: " + err);
+	if ((rc = launched(c, "capillary solve"))) return rc;
+	if ((rc = doProject(c, pcg, &st->reproject))) return rc;
+	if (o.little_drop) launchLittleDrop(F, dt, c->s);
+	return launched(c, "surface tension");
 }
 
 // Lists the faces Boundary::enforce acts on and their solid normal and
@@ -358,6 +384,7 @@ void garm_destroy(garm_ctx *c)
 	if (!c) return;
 	if (c->s) (void)hipStreamSynchronize(c->s);
 	c->P.release();
+	c->C.release();
 	for (void *p : c->allocs) (void)hipFree(p);
 	if (c->F.bface_rec) (void)hipFree(c->F.bface_rec);
 	for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
@@ -687,6 +714,27 @@ int garm_synchronize(garm_ctx *c)
 	return hipCheck(c, hipStreamSynchronize(c->s), "synchronize");
 }
 
+int garm_apply_surface_tension(garm_ctx *c, double dt, const garm_surface_tension_opts *o, const garm_pcg_opts *reproject,
+	garm_surface_tension_stats *st)
+{
+	if (!c || !o || !reproject) return GARM_EINVAL;
+	return doSurfaceTension(c, dt, *o, *reproject, st);
+}
+
+int garm_set_elapsed_time(garm_ctx *c, double t)
+{
+	if (!c) return GARM_EINVAL;
+	c->elapsed = t;
+	return GARM_OK;
+}
+
+int garm_get_elapsed_time(garm_ctx *c, double *t)
+{
+	if (!c || !t) return GARM_EINVAL;
+	*t = c->elapsed;
+	return GARM_OK;
+}
+
 int garm_initialize(garm_ctx *c, const garm_step_opts *o)
 {
 	if (!c || !o) return GARM_EINVAL;
@@ -730,6 +778,7 @@ int garm_step(garm_ctx *c, double dt, int step_index, const garm_step_opts *o, g
 	launchAdvectVelocity(F, dt, c->s, fp32, true);   // u is unchanged since the reference-map advection
 	swapFaces(F);
 	launchCompose(F, c->s, fp32);
+	c->elapsed += dt * 0.5 * 3.141592653589793;   // LevelSetLiquid.cpp:619
 	(void)hipEventRecord(ev[1], c->s);
 	st->reinitialized = step_index % k == 0;
 	if (st->reinitialized && (rc = doReinit(c, o->reinit_iterations, o->reinit_cfl, fp32))) return rc;
@@ -754,6 +803,7 @@ int garm_step(garm_ctx *c, double dt, int step_index, const garm_step_opts *o, g
 	if ((rc = launched(c, "step"))) return rc;
 	rc = doProject(c, o->pcg, &st->pcg);
 	if (rc) return rc;
+	if (o->surface_tension && (rc = doSurfaceTension(c, dt, *o->surface_tension, o->pcg, &st->surface_tension))) return rc;
 	(void)hipEventRecord(ev[5], c->s);
 	launchExtrapolateAir(F, c->s);
 	doEnforceBoundary(c);

@@ -431,7 +431,7 @@ void launchExtrapolateCollider(const Fields &F, hipStream_t s)
 // rebuilt from phi_ori <= 0 (15 layers), u on every face grid (17 layers,
 // unreached zeroed).  The three fronts are independent (their own layer
 // arrays and lists), so each layer of all three is one launch.
-void launchExtrapolateAir(const Fields &F, hipStream_t s)
+void launchExtrapolateAir(const Fields &F, hipStream_t s, bool velocityOnly)
 {
 	LayerSpace Sx = cellSpace(F, F.D, F.xi[0], F.xi[1], F.D == 3 ? F.xi[2] : nullptr);
 	LayerSpace Sa = cellSpace(F, 0, nullptr, nullptr, nullptr);
@@ -448,35 +448,44 @@ void launchExtrapolateAir(const Fields &F, hipStream_t s)
 		off += int(F.Nf[a]);
 	}
 	Sf.off[F.D] = off;
-	const LayerSpace *spaces[3] = {&Sx, &Sa, &Sf};
-	const int rules[3] = {kFixAdvRegion, kFixPhiOriNonPositive, kFixLiquidFace};
-	const int steps[3] = {15, 15, 17};
+	// the velocity-only form (extrapolateFieldsToAir(true)) runs the face front alone
+	const LayerSpace *all[3] = {&Sx, &Sa, &Sf};
+	const int allRules[3] = {kFixAdvRegion, kFixPhiOriNonPositive, kFixLiquidFace};
+	const int allSteps[3] = {15, 15, 17};
+	const int nq = velocityOnly ? 1 : 3, q0 = 3 - nq;
+	const LayerSpace *spaces[3] = {all[q0], nq > 1 ? all[q0 + 1] : nullptr, nq > 2 ? all[q0 + 2] : nullptr};
+	const int rules[3] = {allRules[q0], nq > 1 ? allRules[q0 + 1] : 0, nq > 2 ? allRules[q0 + 2] : 0};
+	const int steps[3] = {allSteps[q0], nq > 1 ? allSteps[q0 + 1] : 0, nq > 2 ? allSteps[q0 + 2] : 0};
 	int *cnt = F.front_cnt;   // three counters per front
 	hipLaunchKernelGGL(kZeroInts, dim3(1), dim3(64), 0, s, cnt, 9);
-	for (int q = 0; q < 3; q++)
+	for (int q = 0; q < nq; q++)
 		for (int a = 0; a < spaces[q]->ng; a++) {
-			if (F.D == 2) launchFrontInit<2>(*spaces[q], a, fixArgs(F, rules[q]), F.front_list[2 * q], cnt + 3 * q, s);
-			else launchFrontInit<3>(*spaces[q], a, fixArgs(F, rules[q]), F.front_list[2 * q], cnt + 3 * q, s);
+			const int f = q0 + q;   // the front's own lists and counters
+			if (F.D == 2) launchFrontInit<2>(*spaces[q], a, fixArgs(F, rules[q]), F.front_list[2 * f], cnt + 3 * f, s);
+			else launchFrontInit<3>(*spaces[q], a, fixArgs(F, rules[q]), F.front_list[2 * f], cnt + 3 * f, s);
 		}
 	for (int L = 1; L <= 17; L++) {
 		FrontJobs J{};
-		J.n = 3;
-		for (int q = 0; q < 3; q++) {
+		J.n = nq;
+		for (int q = 0; q < nq; q++) {
 			FrontJob &j = J.j[q];
+			const int f = q0 + q;
 			j.S = *spaces[q];
-			j.listIn = F.front_list[2 * q + ((L - 1) & 1)];
-			j.listOut = F.front_list[2 * q + (L & 1)];
-			j.cntIn = cnt + 3 * q + (L - 1) % 3;
-			j.cntOut = cnt + 3 * q + L % 3;
-			j.cntZero = cnt + 3 * q + (L + 1) % 3;
+			j.listIn = F.front_list[2 * f + ((L - 1) & 1)];
+			j.listOut = F.front_list[2 * f + (L & 1)];
+			j.cntIn = cnt + 3 * f + (L - 1) % 3;
+			j.cntOut = cnt + 3 * f + L % 3;
+			j.cntZero = cnt + 3 * f + (L + 1) % 3;
 			j.active = L <= steps[q] ? 1 : 0;
 		}
 		if (F.D == 2) hipLaunchKernelGGL(kFrontStepJobs<2>, dim3(kFrontGrid), dim3(kBlock), 0, s, J, L);
 		else hipLaunchKernelGGL(kFrontStepJobs<3>, dim3(kFrontGrid), dim3(kBlock), 0, s, J, L);
 	}
+	if (!velocityOnly) {
 	if (F.D == 2) hipLaunchKernelGGL(kFrontFinishRefMap<2>, gridDims(F.cell, 2), gridBlock(), 0, s, F.cell, (const uint8_t *)F.layer, F.xi[0], F.xi[1], F.xi[2]);
 	else hipLaunchKernelGGL(kFrontFinishRefMap<3>, gridDims(F.cell, 3), gridBlock(), 0, s, F.cell, (const uint8_t *)F.layer, F.xi[0], F.xi[1], F.xi[2]);
 	hipLaunchKernelGGL(kFrontToMask, dim3(linBlocks(F.N)), dim3(kBlock), 0, s, (const uint8_t *)F.layer2, F.adv_region, F.N);
+	}
 	for (int a = 0; a < F.D; a++)
 		hipLaunchKernelGGL(kFrontFinishFaces, dim3(linBlocks(F.Nf[a])), dim3(kBlock), 0, s, (const uint8_t *)F.layer_f[a], F.u[a], F.Nf[a]);
 }

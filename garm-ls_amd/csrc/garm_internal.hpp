@@ -95,7 +95,8 @@ void launchEnrightVelocity(const Fields &F, double coeff, hipStream_t s);
 void launchExtrapolateCollider(const Fields &F, hipStream_t s);
 // finds the collider extrapolation's layers (after cellDist is set); synchronises
 bool buildColliderLayers(Fields &F, hipStream_t s);
-void launchExtrapolateAir(const Fields &F, hipStream_t s);
+// velocityOnly: extrapolateFieldsToAir(true), the velocity alone
+void launchExtrapolateAir(const Fields &F, hipStream_t s, bool velocityOnly = false);
 // Boundary::enforce on the listed faces, in place on F.u
 void launchEnforceBoundary(const Fields &F, hipStream_t s);
 // lists the faces with 0.1 < fraction <= 1 into ids (count in *cnt, zeroed by the caller)

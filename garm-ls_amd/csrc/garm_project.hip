@@ -24,35 +24,6 @@ namespace garm {
 // pass 2 turns the tile counts into tile offsets (one workgroup), pass 3
 // numbers each tile's unknowns in order.  Tiles and cells within a tile are
 // taken in id order, so the numbering is Projector::setUnknowns' forEach order.
-constexpr int kScanSub = 16;                   // passes of kBlock cells per tile
-constexpr int kScanTile = kBlock * kScanSub;
-
-// Exclusive scan of one int per thread over a 256-thread workgroup; *total
-// receives the workgroup sum.
-__device__ __forceinline__ int blockExclusiveScan(int v, int *total)
-{
-	__shared__ int waveSum[kBlock / kWave];
-	const int tid = int(threadIdx.x), lane = tid & (kWave - 1), wid = tid / kWave;
-	int x = v;
-#pragma unroll
-	for (int off = 1; off < kWave; off <<= 1) {
-		const int y = __shfl_up(x, off, kWave);
-		if (lane >= off) x += y;
-	}
-	if (lane == kWave - 1) waveSum[wid] = x;
-	__syncthreads();
-	int before = 0, all = 0;
-#pragma unroll
-	for (int w = 0; w < kBlock / kWave; w++) {
-		const int t = waveSum[w];
-		before += w < wid ? t : 0;
-		all += t;
-	}
-	__syncthreads();   // waveSum is reused by the next call
-	*total = all;
-	return before + x - v;
-}
-
 // box = (min i, min j, min k, -max i, -max j, -max k) of the unknowns
 __global__ void kBoxInit(int *box)
 {
@@ -145,6 +116,16 @@ __global__ __launch_bounds__(kBlock) void kNumberUnknowns(const uint8_t *flags, 
 		}
 		carry += total;
 	}
+}
+
+void launchScanTiles(const int *tileCount, int tiles, int *tileBase, int *count, hipStream_t s)
+{
+	hipLaunchKernelGGL(kScanTiles, dim3(1), dim3(kBlock), 0, s, tileCount, tiles, tileBase, count);
+}
+
+void launchNumberFlags(const uint8_t *flags, const int *tileBase, int tiles, int *grid2mat, int *mat2grid, size_t N, hipStream_t s)
+{
+	hipLaunchKernelGGL(kNumberUnknowns, dim3(tiles), dim3(kBlock), 0, s, flags, tileBase, grid2mat, mat2grid, N);
 }
 
 // Projector::buildLinearSystem (Projector.cpp:30-68), one unknown per thread.

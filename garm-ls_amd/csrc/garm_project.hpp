@@ -44,6 +44,13 @@ struct Projection {
 	bool reserveRows(int D, size_t n, bool mixed, std::string &err);
 };
 
+// The numbering passes over any flag array (tiles of kScanTile ids):
+// tileBase[t] = flags set in tiles before t, *count = all of them; then
+// grid2mat[id] = the id's number in id order (-1 where unflagged) and
+// mat2grid its inverse.
+void launchScanTiles(const int *tileCount, int tiles, int *tileBase, int *count, hipStream_t s);
+void launchNumberFlags(const uint8_t *flags, const int *tileBase, int tiles, int *grid2mat, int *mat2grid, size_t N, hipStream_t s);
+
 int runProjection(Fields &F, Projection &P, const garm_pcg_opts &o, garm_pcg_stats *st, hipStream_t s, std::string &err);
 // What scatterCompact writes to every cell (0 where there is no unknown).
 enum CompactField { kCompactPressure = 0, kCompactRhs = 1, kCompactIndicator = 2 };

@@ -56,19 +56,34 @@ class PcgStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class SurfaceTensionOpts(ctypes.Structure):
+    _fields_ = [("coeff", ctypes.c_double), ("delta_min", ctypes.c_double), ("delta_max", ctypes.c_double),
+                ("delta_crit_time", ctypes.c_double), ("tolerance", ctypes.c_double), ("max_iterations", ctypes.c_int),
+                ("little_drop", ctypes.c_int)]
+
+
+class SurfaceTensionStats(ctypes.Structure):
+    _fields_ = [("band", ctypes.c_double), ("capillary", PcgStats), ("reproject", PcgStats)]
+
+    def as_dict(self):
+        return {"band": self.band, "capillary": self.capillary.as_dict(), "reproject": self.reproject.as_dict()}
+
+
 class StepOpts(ctypes.Structure):
     _fields_ = [("reinit_interval", ctypes.c_int), ("reinit_iterations", ctypes.c_int), ("reinit_cfl", ctypes.c_double),
-                ("advect_precision", ctypes.c_int), ("pcg", PcgOpts), ("restart_mode", ctypes.c_int)]
+                ("advect_precision", ctypes.c_int), ("pcg", PcgOpts), ("restart_mode", ctypes.c_int),
+                ("surface_tension", ctypes.POINTER(SurfaceTensionOpts))]
 
 
 class StepStats(ctypes.Structure):
     _fields_ = [("t_advect_ms", ctypes.c_double), ("t_restart_ms", ctypes.c_double), ("t_project_ms", ctypes.c_double),
                 ("t_extrap_ms", ctypes.c_double), ("t_total_ms", ctypes.c_double), ("restarted", ctypes.c_int),
-                ("reinitialized", ctypes.c_int), ("pcg", PcgStats)]
+                ("reinitialized", ctypes.c_int), ("pcg", PcgStats), ("surface_tension", SurfaceTensionStats)]
 
     def as_dict(self):
-        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "pcg"}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("pcg", "surface_tension")}
         d["pcg"] = self.pcg.as_dict()
+        d["surface_tension"] = self.surface_tension.as_dict()
         return d
 
 
@@ -115,6 +130,10 @@ def lib():
             "garm_restart_newton": (i, [vp]),
             "garm_project": (i, [vp, ctypes.POINTER(PcgOpts), ctypes.POINTER(PcgStats)]),
             "garm_step": (i, [vp, d, i, ctypes.POINTER(StepOpts), ctypes.POINTER(StepStats)]),
+            "garm_apply_surface_tension": (i, [vp, d, ctypes.POINTER(SurfaceTensionOpts), ctypes.POINTER(PcgOpts),
+                                               ctypes.POINTER(SurfaceTensionStats)]),
+            "garm_set_elapsed_time": (i, [vp, d]),
+            "garm_get_elapsed_time": (i, [vp, dp]),
             "garm_max_speed": (i, [vp, dp]),
             "garm_apply_pressure_system": (i, [vp, vp, vp]),
             "garm_volume": (i, [vp, i, dp]),
@@ -159,9 +178,19 @@ def _d3(v):
     return (ctypes.c_double * 3)(*v)
 
 
+def surface_tension_opts(coeff=7.28e-2 / 1e3, delta_min=2.0, delta_max=8.0, delta_crit_time=1.0, tolerance=1e-5,
+                         max_iterations=2000, little_drop=True):
+    """LevelSetLiquid's surface-tension settings (LevelSetLiquid.h:86-91
+    defaults; coeff is _surfaceTensionCoeff / _density)."""
+    return SurfaceTensionOpts(coeff, delta_min, delta_max, delta_crit_time, tolerance, max_iterations, int(bool(little_drop)))
+
+
 def step_opts(reinit_interval=1, reinit_iterations=30, reinit_cfl=0.9, precision=FP64, mode=PCG_FP64, tol=1e-5,
-              max_iterations=2000, check_interval=8, fallback_mode=0, restart_mode=RESTART_PLAIN):
+              max_iterations=2000, check_interval=8, fallback_mode=0, restart_mode=RESTART_PLAIN, surface_tension=None):
     o = StepOpts()
+    if surface_tension is not None:
+        o._surface_tension = surface_tension   # keeps the pointed-to settings alive with o
+        o.surface_tension = ctypes.pointer(surface_tension)
     o.restart_mode = restart_mode
     o.reinit_interval = reinit_interval
     o.reinit_iterations = reinit_iterations
@@ -263,6 +292,24 @@ class Liquid:
 
     def restart_newton(self):
         self._check(lib().garm_restart_newton(self.h))
+
+    def apply_surface_tension(self, dt, opts, mode=PCG_FP64, tol=1e-5, max_iterations=2000, check_interval=8, fallback_mode=0):
+        """garm_apply_surface_tension: opts from surface_tension_opts(), the
+        reprojection solved with the given PCG settings."""
+        o = PcgOpts(mode, tol, max_iterations, check_interval, fallback_mode)
+        st = SurfaceTensionStats()
+        self._check(lib().garm_apply_surface_tension(self.h, dt, ctypes.byref(opts), ctypes.byref(o), ctypes.byref(st)))
+        return st.as_dict()
+
+    @property
+    def elapsed_time(self):
+        t = ctypes.c_double()
+        self._check(lib().garm_get_elapsed_time(self.h, ctypes.byref(t)))
+        return t.value
+
+    @elapsed_time.setter
+    def elapsed_time(self, t):
+        self._check(lib().garm_set_elapsed_time(self.h, t))
 
     def project(self, mode=PCG_FP64, tol=1e-5, max_iterations=2000, check_interval=8, raise_on_noconv=True,
                 fallback_mode=0):

@@ -69,3 +69,28 @@ def dam_break_with_drop(sim, g, extents, drop, radius=0.08):
     dim = g["dim"]
     sim.union_box([0.0] * dim, list(extents[:dim]))
     sim.union_sphere(list(drop[:dim]), radius)
+
+
+def capillary_grid(scale, dim, bw=2):
+    """The grid of LevelSetLiquidBuilder::buildCase1 (LevelSetLiquidBuilder.cpp:67-76):
+    resolution scale (1, 1, 1) + scale e0 + int(0.2 scale) e1, spacing
+    1 / (scale - 2 bw), the domain's lower corner at the origin."""
+    dx = 1.0 / (scale - 2 * bw)
+    res = [2 * scale, scale + int(0.2 * scale), scale][:dim]
+    return dict(dim=dim, res=res, dx=dx, origin=[-bw * dx] * dim, bw=bw)
+
+
+def capillary_column(sim, g):
+    """buildCase1's liquid (LevelSetLiquidBuilder.cpp:79): the box from the
+    domain origin with lengths (0.4, 0.4, 1).  Its solid cylinder
+    (:82, base (0, -0.6, 0), radius 0.1, height 0.6) lies in y <= 0, below
+    the domain floor, where the domain walls are solid already: it changes no
+    node distance above the floor, so the domain box alone is its boundary."""
+    dim = g["dim"]
+    sim.union_box([0.0] * dim, [0.4, 0.4, 1.0][:dim])
+
+
+# buildCase1's surface-tension settings (LevelSetLiquidBuilder.cpp:76-78):
+# _surfaceTensionCoeff 30 over the default density 1e3, band 5.5 cells
+CAPILLARY_COEFF = 30.0 / 1e3
+CAPILLARY_BAND = 5.5

@@ -116,6 +116,22 @@ typedef struct garm_pcg_stats {
 /* Precision of advection and reinitialisation. */
 typedef enum garm_precision { GARM_FP64 = 0, GARM_FP32 = 1 } garm_precision;
 
+/* LevelSetLiquid's surface-tension settings (LevelSetLiquid.h:86-91). */
+typedef struct garm_surface_tension_opts {
+	double coeff;            /* _surfaceTensionCoeff / _density (reference 7.28e-2 / 1e3;
+	                            LevelSetLiquidBuilder::buildCase1: 30 / 1e3) */
+	double delta_min;        /* band half-width in cells: delta_min + (delta_max - delta_min)
+	                            * 2 / (1 + exp(elapsed / delta_crit_time)) (reference 2, 8, 1;
+	                            buildCase1 5.5, 5.5) */
+	double delta_max;
+	double delta_crit_time;
+	double tolerance;        /* capillary solve, relative residual (reference 1e-5,
+	                            SurfaceTension.h:93) */
+	int max_iterations;      /* reference 2000 */
+	int little_drop;         /* 1: trickLittleDrop after the reprojection (the reference
+	                            always runs it with surface tension) */
+} garm_surface_tension_opts;
+
 typedef struct garm_step_opts {
 	int reinit_interval;     /* WENO5 reinitialisation every k steps (reference: every step, k=1);
 	                            a step that restarts the reference map reinitialises first */
@@ -127,7 +143,17 @@ typedef struct garm_step_opts {
 	garm_pcg_opts pcg;
 	int restart_mode;        /* garm_restart_mode of a step whose reference map is too
 	                            distorted (0, the plain branch, unless set) */
+	const garm_surface_tension_opts *surface_tension;
+	                         /* NULL: off (the reference's _enableSurfaceTension = false);
+	                            otherwise applySurfaceTension + trickLittleDrop after the
+	                            projection (LevelSetLiquid.cpp:671-675) */
 } garm_step_opts;
+
+typedef struct garm_surface_tension_stats {
+	double band;             /* the surfaceTensionDelta used (cells) */
+	garm_pcg_stats capillary;/* unknowns = interface faces; FP64 Jacobi PCG */
+	garm_pcg_stats reproject;/* the second pressure solve */
+} garm_surface_tension_stats;
 
 /* How a step restarts the reference map (LevelSetLiquid::restart,
  * LevelSetLiquid.cpp:1285-1352). */
@@ -148,6 +174,7 @@ typedef struct garm_step_stats {
 	int restarted;           /* 1 if the reference map was restarted this step */
 	int reinitialized;
 	garm_pcg_stats pcg;
+	garm_surface_tension_stats surface_tension;   /* zero when off */
 } garm_step_stats;
 
 /* ---- context ---------------------------------------------------------- */
@@ -220,6 +247,24 @@ int garm_restart_newton(garm_ctx *ctx);
 /* Projector::project (Projector.cpp:13-20) with matrix-free PCG in place of
  * Eigen's IC-preconditioned CG (LinearSystem.h:23-39). */
 int garm_project(garm_ctx *ctx, const garm_pcg_opts *opts, garm_pcg_stats *stats);
+/* LevelSetLiquid::applySurfaceTension then trickLittleDrop
+ * (LevelSetLiquid.cpp:1026-1057), on the state a projection left:
+ * extrapolateFieldsToAir(true) (the velocity alone) and the boundary
+ * enforcement; SurfaceTension::solve (SurfaceTension.h:12-101) on the faces
+ * with |phi| < band dx under the cubic view of phi and fraction < 1, its
+ * implicit capillary system solved with FP64 Jacobi PCG in place of Eigen's
+ * IC-preconditioned CG; Projector::reproject (Projector.cpp:22-28) with
+ * `reproject` -- the same unknowns and operator as the projection, the
+ * right-hand side of the new velocity, solved from zero where the
+ * reference starts from the projection's pressure.  The band uses the
+ * context's elapsed time (advanced by garm_step as the reference's
+ * _elapsedTime, LevelSetLiquid.cpp:619). */
+int garm_apply_surface_tension(garm_ctx *ctx, double dt, const garm_surface_tension_opts *opts, const garm_pcg_opts *reproject,
+	garm_surface_tension_stats *stats);
+/* LevelSetLiquid::_elapsedTime (LevelSetLiquid.h:83): garm_step adds
+ * dt * pi / 2 after the advection (LevelSetLiquid.cpp:619); 0 at creation. */
+int garm_set_elapsed_time(garm_ctx *ctx, double t);
+int garm_get_elapsed_time(garm_ctx *ctx, double *t);
 /* LevelSetLiquid::advance (LevelSetLiquid.cpp:608-698); step_index selects
  * the reinitialisation cadence (reinit when step_index % reinit_interval == 0). */
 int garm_step(garm_ctx *ctx, double dt, int step_index, const garm_step_opts *opts, garm_step_stats *stats);

@@ -14,7 +14,7 @@
 //  - reinitialisation is WENO5 PDE (the --enable-fmm=n build), optionally
 //    every k steps (k = 1 is the reference);
 //  - the reference-map restart is the branch without Hybrid Newton
-//    (LevelSetLiquid.cpp:1331-1352);
+//    (LevelSetLiquid.cpp:1331-1352) unless Hybrid Newton is switched on;
 //  - the pressure solve is a restated conjugate gradient with IC(0) (natural
 //    ordering), Jacobi or no preconditioner in place of Eigen's
 //    IncompleteCholesky (LinearSystem.h:10-39); the tolerance is a parameter.
@@ -354,6 +354,19 @@ struct Liquid {
 
 	std::vector<int> grid2mat, mat2grid;
 	std::vector<double> pres;
+	std::vector<double> lastX;         // Projector::_rdP of the last solve (compact)
+
+	double elapsed = 0;                // _elapsedTime (LevelSetLiquid.h:83)
+	// surface tension (LevelSetLiquid.h:86-91); step() applies it when on
+	bool stOn = false, stLittleDrop = true;
+	double stCoeff = 7.28e-2 / 1e3, stDeltaMin = 2, stDeltaMax = 8, stCritTime = 1, stTol = 1e-5;
+	int stMaxIt = 2000;
+	// the solvers' start and preconditioner: the reference's (IC(0) from the
+	// velocity guess; the reprojection from the last pressure) unless changed
+	int stPrecond = 2;
+	bool stGuess = true, stWarm = true;
+	int stFaces = 0, stIters = 0;      // last capillary solve
+	double stErr = 0, stBand = 0;
 
 	// StaggeredGrid (StaggeredGrid.h:27-36, 104-120) with an explicit origin.
 	void setup(const int *r, double h, const double *org, int bw_)
@@ -858,6 +871,13 @@ struct Liquid {
 		const size_t N = cell.count();
 		for (size_t i = 0; i < N; i++) advRegion[i] = phiOri[i] <= 0 ? 1 : 0;
 		extrapolateMask(advRegion, radius, [&](const int *c) { return phiOri[cell.index(c)] <= 0; });
+		extrapolateAirVelocity();
+	}
+
+	// extrapolateFieldsToAir(true): the velocity alone (LevelSetLiquid.cpp:1075-1083)
+	void extrapolateAirVelocity()
+	{
+		const double radius = 15.0;
 		extrapolateFaces(radius + 2, [&](int ax, const int *c) {
 			const size_t id = face[ax].index(c);
 			if (!(frac[ax][id] < 1)) return false;
@@ -1253,14 +1273,20 @@ struct Liquid {
 	// (conjugate_gradient: residual test ||r||^2 < tol^2 ||b||^2, zero
 	// initial guess), called as in LinearSystem::solve (LinearSystem.h:23-39).
 	// precond: 0 identity, 1 Jacobi, 2 IC(0).  Returns 0 if converged.
-	int solveCG(const System &S, std::vector<double> &x, int precond, double tol, int maxIt, int *itersOut, double *errOut) const
+	// guess: Eigen's solveWithGuess start (residual b - A x0); zero when null.
+	int solveCG(const System &S, std::vector<double> &x, int precond, double tol, int maxIt, int *itersOut, double *errOut,
+		const std::vector<double> *guess = nullptr) const
 	{
 		const int n = S.n;
-		x.assign(n, 0.0);
 		std::vector<double> r(S.rhs), z(n), p(n), t(n), Ld;
+		if (guess) {
+			x = *guess;
+			spmv(S, x, t);
+			for (int i = 0; i < n; i++) r[i] = S.rhs[i] - t[i];
+		} else x.assign(n, 0.0);
 		double bnorm2 = 0;
 		for (int i = 0; i < n; i++) bnorm2 += S.rhs[i] * S.rhs[i];
-		if (bnorm2 == 0) { *itersOut = 0; *errOut = 0; return 0; }
+		if (bnorm2 == 0) { x.assign(n, 0.0); *itersOut = 0; *errOut = 0; return 0; }
 		tol = std::max(tol, std::numeric_limits<double>::epsilon() / std::sqrt(bnorm2));
 		const double threshold = std::max(tol * tol * bnorm2, std::numeric_limits<double>::min());
 		if (precond == 2) icFactor(S, Ld);
@@ -1272,7 +1298,8 @@ struct Liquid {
 			else if (precond == 2) icSolve(S, Ld, in, out);
 			else out = in;
 		};
-		double rn2 = bnorm2;
+		double rn2 = 0;
+		for (int i = 0; i < n; i++) rn2 += r[i] * r[i];
 		if (rn2 < threshold) { *itersOut = 0; *errOut = std::sqrt(rn2 / bnorm2); return 0; }
 		applyM(r, p);
 		double absNew = 0;
@@ -1337,7 +1364,182 @@ struct Liquid {
 		std::fill(pres.begin(), pres.end(), 0.0);
 		for (int r = 0; r < n; r++) pres[mat2grid[r]] = x[r];
 		if (unknowns) *unknowns = n;
+		lastX = x;
 		return rc;
+	}
+
+	// Projector::reproject (Projector.cpp:22-28): the unknowns and operator of
+	// the last projection (rebuilt from the unchanged phi and fractions), the
+	// right-hand side of the current velocity (setRightHandSide,
+	// Projector.cpp:70-89), solved from the last pressure.
+	int reproject(int precond, double tol, int maxIt, int *iters, double *err, int *unknowns)
+	{
+		const int n = setUnknowns();
+		System S;
+		buildSystem(S);
+		std::vector<double> x;
+		std::vector<double> guess = lastX;
+		guess.resize(n, 0.0);
+		const int rc = solveCG(S, x, precond, tol, maxIt, iters, err, stWarm ? &guess : nullptr);
+		applyPressureGradient(x);
+		std::fill(pres.begin(), pres.end(), 0.0);
+		for (int r = 0; r < n; r++) pres[mat2grid[r]] = x[r];
+		if (unknowns) *unknowns = n;
+		lastX = x;
+		return rc;
+	}
+
+	// SurfaceTension::getCurvature (SurfaceTension.h:12-23) with the cubic view
+	double curvature(const double *pos) const
+	{
+		const double invDx = 1. / dx;
+		double kappa = 0;
+		for (int a = 0; a < D; a++) {
+			double q[3] = {pos[0], pos[1], D == 3 ? pos[2] : 0}, gp[3], gm[3];
+			q[a] = pos[a] + dx;
+			cubicViewGradient(phi.data(), q, gp);
+			normalized(gp, gp);
+			q[a] = pos[a] - dx;
+			cubicViewGradient(phi.data(), q, gm);
+			normalized(gm, gm);
+			kappa += gp[a] - gm[a];
+		}
+		kappa *= invDx * .5;
+		return std::abs(kappa) > invDx ? (kappa < 0 ? -1 : 1) * invDx : kappa;
+	}
+
+	// Derivatives::getFirst<2> along b at the unclamped face coordinate q of
+	// face grid `ax`, stencil points clamped (Derivatives.h:47, 51-57)
+	double faceFirst(int ax, const int *q, int b) const
+	{
+		int hi[3] = {q[0], q[1], q[2]}, lo[3] = {q[0], q[1], q[2]};
+		hi[b] += 1;
+		lo[b] -= 1;
+		return (u[ax][face[ax].at(hi)] - u[ax][face[ax].at(lo)]) / 2 * face[ax].inv;
+	}
+
+	// SurfaceTension::solve (SurfaceTension.h:25-101): the capillary system
+	// on the faces with |phi| < maxSteps dx (cubic view) and fraction < 1,
+	// solved with the restated CG from the guess vel.
+	int capillary(double coeff, double maxSteps, double dt, double tol, int maxIt, int precond)
+	{
+		const double invDx = 1. / dx;
+		const double eps = maxSteps * dx;
+		std::vector<int> g2m[D];
+		std::vector<std::pair<int, size_t>> m2g;
+		for (int ax = 0; ax < D; ax++) {
+			g2m[ax].assign(face[ax].count(), -1);
+			serFor<D>(face[ax], [&](const int *c) {
+				double p[3] = {0, 0, 0};
+				face[ax].position(c, p);
+				const double v = cubic<D>(cell, phi.data(), p);
+				const size_t id = face[ax].index(c);
+				if (-eps < v && v < eps && frac[ax][id] < 1) {
+					g2m[ax][id] = int(m2g.size());
+					m2g.emplace_back(ax, id);
+				}
+			});
+		}
+		const int n = int(m2g.size());
+		System S;
+		S.n = n;
+		S.diag.assign(n, 0.0); S.rhs.assign(n, 0.0);
+		S.off.assign(size_t(n) * 2 * D, 0.0); S.nb.assign(size_t(n) * 2 * D, -1);
+		std::vector<double> vel(n);
+		const double weight = coeff * dt * dt / (dx * dx);
+		for (int r = 0; r < n; r++) {
+			const int ax = m2g[r].first;
+			const Grid<D> &fg = face[ax];
+			int c[3] = {0, 0, 0};
+			fg.coord(m2g[r].second, c);
+			double p[3] = {0, 0, 0};
+			fg.position(c, p);
+			double g[3] = {0, 0, 0}, nrm[3] = {0, 0, 0};
+			cubicViewGradient(phi.data(), p, g);
+			normalized(g, nrm);
+			const double kappa = curvature(p);
+			const double v = cubic<D>(cell, phi.data(), p);
+			const double dirac = (v < -eps || v > eps) ? 0 : 0.5 * (1.0 + std::cos(3.141592653589793 * v / eps)) / eps;
+			const double delta = std::max(dirac, 1e-7 * invDx);
+			double jac[3] = {0, 0, 0}, hes[3][3] = {};
+			for (int b = 0; b < D; b++) jac[b] = faceFirst(ax, c, b);
+			for (int a1 = 0; a1 < D; a1++)
+				for (int a2 = 0; a2 < D; a2++) {
+					int qp[3] = {c[0], c[1], c[2]}, qm[3] = {c[0], c[1], c[2]};
+					qp[a2] += 1;
+					qm[a2] -= 1;
+					hes[a1][a2] = (faceFirst(ax, qp, a1) - faceFirst(ax, qm, a1)) / 2 * fg.inv;
+				}
+			double nHn = 0, jn = 0;
+			for (int j = 0; j < D; j++) {
+				double t = 0;
+				for (int i = 0; i < D; i++) t += nrm[i] * hes[i][j];
+				nHn += t * nrm[j];
+			}
+			for (int i = 0; i < D; i++) jn += jac[i] * nrm[i];
+			const size_t fid = fg.index(c);
+			double diag = 1 / delta;
+			vel[r] = S.rhs[r] = u[ax][fid] / delta - coeff * dt * (kappa * nrm[ax] + dt * (nHn + kappa * jn));
+			for (int o = 0; o < 2 * D; o++) {
+				int q[3] = {c[0], c[1], c[2]};
+				q[o >> 1] += (o & 1) ? 1 : -1;
+				if (!fg.valid(q)) continue;
+				const size_t qf = fg.index(q);
+				if (frac[ax][qf] == 1) continue;
+				diag += weight;
+				const int m = g2m[ax][qf];
+				if (m != -1) {
+					S.nb[size_t(r) * 2 * D + o] = m;
+					S.off[size_t(r) * 2 * D + o] = -weight;
+				} else S.rhs[r] += weight * u[ax][qf];
+			}
+			S.diag[r] = diag;
+		}
+		std::vector<double> x;
+		const int rc = solveCG(S, x, precond, tol, maxIt, &stIters, &stErr, stGuess ? &vel : nullptr);
+		for (int r = 0; r < n; r++) u[m2g[r].first][m2g[r].second] = x[r];
+		stFaces = n;
+		stBand = maxSteps;
+		return rc;
+	}
+
+	// LevelSetLiquid::trickLittleDrop with checkDrop (LevelSetLiquid.cpp:1037-1057,
+	// LevelSetLiquid.h:178-190)
+	void littleDrop(double dt)
+	{
+		const double v0 = dx / dt;
+		const double vt = v0 * .15;
+		serFor<D>(cell, [&](const int *c) {
+			if (phi[cell.index(c)] > 0) return;
+			for (int o = 0; o < 2 * D; o++) {
+				const int ax = o >> 1;
+				int q[3] = {c[0], c[1], c[2]}, f[3] = {c[0], c[1], c[2]};
+				q[ax] += (o & 1) ? 1 : -1;
+				f[ax] += o & 1;
+				if (frac[ax][face[ax].index(f)] != 0 || phi[cell.at(q)] <= 0) return;
+			}
+			int f1[3] = {c[0], c[1] + 1, c[2]};
+			const size_t a = face[1].index(c), b = face[1].index(f1);
+			const double v = (u[1][a] + u[1][b]) * .5;
+			if (v < vt * 1.25 && v > -vt && u[1][a] > 0 && u[1][b] < 0) {
+				u[1][a] -= vt + v;
+				u[1][b] -= vt + v;
+			}
+		});
+	}
+
+	// LevelSetLiquid::applySurfaceTension + trickLittleDrop
+	// (LevelSetLiquid.cpp:1026-1057); the capillary and pressure solves use
+	// the given preconditioner (2: IC(0), as the reference's ICPCG)
+	int surfaceTension(double dt, int precond, double tol, int maxIt, int *iters, double *err, int *unknowns)
+	{
+		extrapolateAirVelocity();
+		enforceBoundary();
+		stBand = stDeltaMin + (stDeltaMax - stDeltaMin) * 2 / (1 + std::exp(elapsed / stCritTime));
+		int rc = capillary(stCoeff, stBand, dt, stTol, stMaxIt, stPrecond);
+		const int rp = reproject(precond, tol, maxIt, iters, err, unknowns);
+		if (stLittleDrop) littleDrop(dt);
+		return rc ? rc : rp;
 	}
 
 	double maxSpeed() const
@@ -1378,6 +1580,7 @@ struct Liquid {
 	{
 		advectRefMap(dt);
 		advectVelocity(dt);
+		elapsed += dt * 0.5 * 3.141592653589793;   // LevelSetLiquid.cpp:619
 		compose();
 		if (doReinit) reinit(reinitIters, cfl);
 		extrapolateCollider();
@@ -1394,7 +1597,13 @@ struct Liquid {
 			*restarted = 1;
 		}
 		gravity(dt);
-		const int rc = project(precond, tol, maxIt, iters, err, unknowns);
+		int rc = project(precond, tol, maxIt, iters, err, unknowns);
+		if (stOn) {
+			int it2, un2;
+			double e2;
+			const int r2 = surfaceTension(dt, precond, tol, maxIt, &it2, &e2, &un2);
+			rc = rc ? rc : r2;
+		}
 		extrapolateAir();
 		enforceBoundary();
 		return rc;
@@ -1579,6 +1788,41 @@ int ora_check_refmap(void *p, double th, double band, double ratio, long long *t
 void ora_restart(void *p) { auto *h = (Handle *)p; DISPATCH(h, L.restart()); }
 void ora_restart_newton(void *p) { auto *h = (Handle *)p; DISPATCH(h, L.restartNewton()); }
 void ora_set_restart_newton(void *p, int on) { auto *h = (Handle *)p; DISPATCH(h, L.newtonRestart = on != 0); }
+void ora_set_surface_tension(void *p, int on, double coeff, double dmin, double dmax, double crit, double tol, int maxIt, int littleDrop)
+{
+	auto *h = (Handle *)p;
+	DISPATCH(h, (L.stOn = on != 0, L.stCoeff = coeff, L.stDeltaMin = dmin, L.stDeltaMax = dmax, L.stCritTime = crit, L.stTol = tol,
+		L.stMaxIt = maxIt, L.stLittleDrop = littleDrop != 0));
+}
+void ora_set_surface_tension_solvers(void *p, int capillaryPrecond, int capillaryGuess, int reprojectWarm)
+{
+	auto *h = (Handle *)p;
+	DISPATCH(h, (L.stPrecond = capillaryPrecond, L.stGuess = capillaryGuess != 0, L.stWarm = reprojectWarm != 0));
+}
+int ora_apply_surface_tension(void *p, double dt, int precond, double tol, int maxIt, int *iters, double *err, int *unknowns)
+{
+	auto *h = (Handle *)p;
+	int r = 0;
+	DISPATCH(h, r = L.surfaceTension(dt, precond, tol, maxIt, iters, err, unknowns));
+	return r;
+}
+// last capillary solve: faces, iterations, relative residual, band
+void ora_capillary_stats(void *p, int *faces, int *iters, double *err, double *band)
+{
+	auto *h = (Handle *)p;
+	DISPATCH(h, (*faces = L.stFaces, *iters = L.stIters, *err = L.stErr, *band = L.stBand));
+}
+int ora_capillary(void *p, double coeff, double band, double dt, double tol, int maxIt, int precond)
+{
+	auto *h = (Handle *)p;
+	int r = 0;
+	DISPATCH(h, r = L.capillary(coeff, band, dt, tol, maxIt, precond));
+	return r;
+}
+double ora_curvature(void *p, const double *pos) { auto *h = (Handle *)p; double r = 0; DISPATCH(h, r = L.curvature(pos)); return r; }
+void ora_little_drop(void *p, double dt) { auto *h = (Handle *)p; DISPATCH(h, L.littleDrop(dt)); }
+void ora_set_elapsed_time(void *p, double t) { auto *h = (Handle *)p; DISPATCH(h, L.elapsed = t); }
+double ora_get_elapsed_time(void *p) { auto *h = (Handle *)p; double r = 0; DISPATCH(h, r = L.elapsed); return r; }
 int ora_project(void *p, int precond, double tol, int maxIt, int *iters, double *err, int *unknowns)
 {
 	auto *h = (Handle *)p; int r = 0;

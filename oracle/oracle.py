@@ -61,6 +61,15 @@ def lib():
             "ora_restart": (None, [vp]),
             "ora_restart_newton": (None, [vp]),
             "ora_set_restart_newton": (None, [vp, i]),
+            "ora_set_surface_tension": (None, [vp, i, d, d, d, d, d, i, i]),
+            "ora_apply_surface_tension": (i, [vp, d, i, d, i, ip, dp, ip]),
+            "ora_set_surface_tension_solvers": (None, [vp, i, i, i]),
+            "ora_capillary_stats": (None, [vp, ip, ip, dp, dp]),
+            "ora_set_elapsed_time": (None, [vp, d]),
+            "ora_capillary": (i, [vp, d, d, d, d, i, i]),
+            "ora_curvature": (d, [vp, dp]),
+            "ora_little_drop": (None, [vp, d]),
+            "ora_get_elapsed_time": (d, [vp]),
             "ora_project": (i, [vp, i, d, i, ip, dp, ip]),
             "ora_step": (i, [vp, d, i, i, d, i, d, i, ip, ip, dp, ip]),
             "ora_max_speed": (d, [vp]),
@@ -183,6 +192,50 @@ class Oracle:
     def set_restart_newton(self, on):
         """step() restarts with Hybrid Newton when on, the plain branch otherwise."""
         lib().ora_set_restart_newton(self.h, int(bool(on)))
+
+    def set_surface_tension(self, on, coeff=7.28e-2 / 1e3, delta_min=2.0, delta_max=8.0, crit_time=1.0, tol=1e-5, max_it=2000,
+                            little_drop=True):
+        """step() applies surface tension after the projection when on
+        (LevelSetLiquid.h:86-91 settings; the capillary solve uses IC(0))."""
+        lib().ora_set_surface_tension(self.h, int(bool(on)), coeff, delta_min, delta_max, crit_time, tol, max_it, int(bool(little_drop)))
+
+    def set_surface_tension_solvers(self, capillary_precond=PRECOND_IC0, capillary_guess=True, reproject_warm=True):
+        """How the capillary solve and the reprojection start: the reference's
+        by default (IC(0) from the velocity guess, the reprojection from the
+        last pressure); (PRECOND_JACOBI, False, False) is the GPU's."""
+        lib().ora_set_surface_tension_solvers(self.h, capillary_precond, int(bool(capillary_guess)), int(bool(reproject_warm)))
+
+    def apply_surface_tension(self, dt, precond=PRECOND_JACOBI, tol=1e-10, max_it=20000):
+        """applySurfaceTension + trickLittleDrop with the settings of
+        set_surface_tension; precond/tol/max_it are the reprojection's."""
+        it, unk, err = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        rc = lib().ora_apply_surface_tension(self.h, dt, precond, tol, max_it, ctypes.byref(it), ctypes.byref(err), ctypes.byref(unk))
+        return dict(rc=rc, iterations=it.value, residual=err.value, unknowns=unk.value, capillary=self.capillary_stats())
+
+    def capillary_stats(self):
+        f, it, err, band = ctypes.c_int(), ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        lib().ora_capillary_stats(self.h, ctypes.byref(f), ctypes.byref(it), ctypes.byref(err), ctypes.byref(band))
+        return dict(faces=f.value, iterations=it.value, residual=err.value, band=band.value)
+
+    def capillary(self, coeff, band, dt, tol=1e-12, max_it=20000, precond=PRECOND_IC0):
+        """SurfaceTension::solve alone on the current phi and velocity."""
+        rc = lib().ora_capillary(self.h, coeff, band, dt, tol, max_it, precond)
+        return dict(rc=rc, **self.capillary_stats())
+
+    def curvature(self, pos):
+        """SurfaceTension::getCurvature at pos with the cubic view of phi."""
+        return lib().ora_curvature(self.h, _d3(pos))
+
+    def little_drop(self, dt):
+        lib().ora_little_drop(self.h, dt)
+
+    @property
+    def elapsed_time(self):
+        return lib().ora_get_elapsed_time(self.h)
+
+    @elapsed_time.setter
+    def elapsed_time(self, t):
+        lib().ora_set_elapsed_time(self.h, t)
 
     def project(self, precond=PRECOND_JACOBI, tol=1e-10, max_it=20000):
         it, unk, err = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
