@@ -662,6 +662,47 @@ int garm_load_frame(garm_ctx *c, const char *frame_dir)
 	return GARM_OK;
 }
 
+int garm_contour(garm_ctx *c, float *positions, float *normals, size_t vertex_capacity, unsigned *indices, size_t index_capacity,
+	size_t *n_vertices, size_t *n_indices)
+{
+	if (!c || !n_vertices || !n_indices) return GARM_EINVAL;
+	std::vector<float> pos, nrm;
+	std::vector<unsigned> idx;
+	std::string err;
+	const int rc = contourMesh(c->F, c->F.phi_tmp, pos, nrm, idx, c->s, err);
+	if (rc) return fail(c, rc, "contour: " + err);
+	*n_vertices = pos.size() / 3;
+	*n_indices = idx.size();
+	if (*n_vertices > vertex_capacity || *n_indices > index_capacity || (*n_vertices && (!positions || !normals)) ||
+		(*n_indices && !indices))
+		return fail(c, GARM_EINVAL, "contour: the buffers are too small");
+	std::copy(pos.begin(), pos.end(), positions);
+	std::copy(nrm.begin(), nrm.end(), normals);
+	std::copy(idx.begin(), idx.end(), indices);
+	return GARM_OK;
+}
+
+int garm_write_mesh(garm_ctx *c, const char *path)
+{
+	if (!c || !path) return GARM_EINVAL;
+	std::vector<float> pos, nrm;
+	std::vector<unsigned> idx;
+	std::string err;
+	const int rc = contourMesh(c->F, c->F.phi_tmp, pos, nrm, idx, c->s, err);
+	if (rc) return fail(c, rc, "contour: " + err);
+	if (pos.size() / 3 > 0xffffffffull || idx.size() > 0xffffffffull) return fail(c, GARM_EINVAL, "mesh too large for its uint32 counts");
+	std::ofstream out(path, std::ios::binary);
+	if (!out) return fail(c, GARM_EINVAL, std::string("cannot open ") + path);
+	const uint32_t nv = uint32_t(pos.size() / 3), ni = uint32_t(idx.size());
+	out.write(reinterpret_cast<const char *>(&nv), sizeof(nv));
+	out.write(reinterpret_cast<const char *>(pos.data()), std::streamsize(pos.size() * sizeof(float)));
+	out.write(reinterpret_cast<const char *>(nrm.data()), std::streamsize(nrm.size() * sizeof(float)));
+	out.write(reinterpret_cast<const char *>(&ni), sizeof(ni));
+	out.write(reinterpret_cast<const char *>(idx.data()), std::streamsize(idx.size() * sizeof(unsigned)));
+	if (!out) return fail(c, GARM_EINVAL, std::string("writing ") + path + " failed");
+	return GARM_OK;
+}
+
 int garm_set_velocity_field(garm_ctx *c, int field, double time, double period)
 {
 	if (!c) return GARM_EINVAL;

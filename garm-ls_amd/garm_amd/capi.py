@@ -133,6 +133,8 @@ def lib():
             "garm_apply_surface_tension": (i, [vp, d, ctypes.POINTER(SurfaceTensionOpts), ctypes.POINTER(PcgOpts),
                                                ctypes.POINTER(SurfaceTensionStats)]),
             "garm_set_elapsed_time": (i, [vp, d]),
+            "garm_contour": (i, [vp, vp, vp, sz, vp, sz, ctypes.POINTER(sz), ctypes.POINTER(sz)]),
+            "garm_write_mesh": (i, [vp, ctypes.c_char_p]),
             "garm_get_elapsed_time": (i, [vp, dp]),
             "garm_max_speed": (i, [vp, dp]),
             "garm_apply_pressure_system": (i, [vp, vp, vp]),
@@ -300,6 +302,24 @@ class Liquid:
         st = SurfaceTensionStats()
         self._check(lib().garm_apply_surface_tension(self.h, dt, ctypes.byref(opts), ctypes.byref(o), ctypes.byref(st)))
         return st.as_dict()
+
+    def contour(self):
+        """garm_contour of phi_tmp: (positions (nv, 3) float32, normals
+        (nv, 3) float32, triangles (nt, 3) uint32)."""
+        nv, ni = ctypes.c_size_t(), ctypes.c_size_t()
+        rc = lib().garm_contour(self.h, None, None, 0, None, 0, ctypes.byref(nv), ctypes.byref(ni))
+        if rc not in (GARM_OK, GARM_EINVAL) or (rc == GARM_EINVAL and nv.value == 0 and ni.value == 0):
+            self._check(rc)
+        pos = np.zeros((nv.value, 3), np.float32)
+        nrm = np.zeros((nv.value, 3), np.float32)
+        idx = np.zeros(ni.value, np.uint32)
+        self._check(lib().garm_contour(self.h, pos.ctypes.data, nrm.ctypes.data, nv.value, idx.ctypes.data, ni.value,
+                                       ctypes.byref(nv), ctypes.byref(ni)))
+        return pos, nrm, idx.reshape(-1, 3)
+
+    def write_mesh(self, path):
+        """garm_write_mesh: phi_tmp's surface as a liquid.mesh file."""
+        self._check(lib().garm_write_mesh(self.h, os.fsencode(path)))
 
     @property
     def elapsed_time(self):
@@ -475,3 +495,21 @@ class Unet:
         self._check(lib().garm_unet_train(self.h, self._dims(r.shape[1:]), r.shape[0], r.ctypes.data, p.ctypes.data,
                                           m.ctypes.data, steps, lr, seed, hist.ctypes.data))
         return hist[:steps]
+
+
+def read_mesh(path):
+    """A liquid.mesh file (SurfaceMesh::write, SurfaceMesh.cpp:103-124, 3D):
+    (positions (nv, 3), normals (nv, 3), triangles (nt, 3))."""
+    raw = open(path, "rb").read()
+    nv = int(np.frombuffer(raw, np.uint32, 1, 0)[0])
+    off = 4
+    pos = np.frombuffer(raw, np.float32, nv * 3, off).reshape(nv, 3)
+    off += nv * 12
+    nrm = np.frombuffer(raw, np.float32, nv * 3, off).reshape(nv, 3)
+    off += nv * 12
+    ni = int(np.frombuffer(raw, np.uint32, 1, off)[0])
+    off += 4
+    idx = np.frombuffer(raw, np.uint32, ni, off)
+    if off + ni * 4 != len(raw):
+        raise ValueError(f"{path}: {len(raw) - off - ni * 4} trailing bytes")
+    return pos, nrm, idx.reshape(-1, 3)

@@ -297,6 +297,24 @@ typedef enum garm_velocity_field {
 int garm_save_frame(garm_ctx *ctx, const char *frame_dir);
 int garm_load_frame(garm_ctx *ctx, const char *frame_dir);
 
+/* Contourer<3>::contour of _tmpLevelSet at 0 (Contourer.cpp:9-97) with
+ * SurfaceMesh::computeNormals (SurfaceMesh.cpp:191-216), on the device: the
+ * reference's vertex numbering (first cube holding an edge, cube order then
+ * edge order) and positions, each cube's triangles from this library's
+ * table (crossing edges paired per cube face, an ambiguous face cutting off
+ * each inside corner; watertight), oriented from the liquid to the air.
+ * positions and normals: 3 floats per vertex; indices: 3 per triangle.
+ * The counts are always set; the arrays are filled when the capacities (in
+ * vertices and in indices) suffice, else GARM_EINVAL.  3D contexts only. */
+int garm_contour(garm_ctx *ctx, float *positions, float *normals, size_t vertex_capacity, unsigned *indices, size_t index_capacity,
+	size_t *n_vertices, size_t *n_indices);
+/* The same mesh written to `path` in SurfaceMesh::write's binary layout
+ * (SurfaceMesh.cpp:103-124), the reference's frame file liquid.mesh
+ * (LevelSetLiquid.cpp:306-345 without refinement): uint32 vertex count,
+ * float positions (x, y, z), float normals, uint32 index count, uint32
+ * indices. */
+int garm_write_mesh(garm_ctx *ctx, const char *path);
+
 /* Sets the face velocity of a 3D context to an analytic field at time t
  * with period T, on the device (the reference's tests evaluate it per
  * sample, TestAdvect3D.cpp:152-190, 390-397). */

@@ -1610,6 +1610,165 @@ struct Liquid {
 	}
 };
 
+// ------------------------------------------------------ surface mesh ----
+// Contourer<3>::contour at 0 (Contourer.cpp:9-97) and SurfaceMesh::
+// computeNormals (SurfaceMesh.cpp:191-216) over a node field, the
+// reference's numbering (StaggeredGrid.h:61-94: cube node k = (k&1, k>>1&1,
+// k>>2&1), edge i along axis i>>2 offset (i&1, i>>1&1) along the next two
+// axes) and vertex creation order.  The triangles of a cube type come from
+// polygons made of directed face segments: walking each cube face's corners
+// counter-clockwise as seen from outside the cube, a segment runs from each
+// crossing entered from outside to the next crossing left to outside (on an
+// ambiguous face the two around each inside corner); the segments chain
+// into loops whose normals point from the inside corners out.  Each loop is
+// started at its smallest edge ord and fanned; loops in the order of their
+// smallest edge.
+struct McPolys {
+	std::vector<int> tri[256];   // edge ords, three per triangle
+};
+
+static int mcEdgeCorner(int i, int end)
+{
+	const int a = i >> 2;
+	int p[3] = {0, 0, 0};
+	p[(a + 1) % 3] = i & 1;
+	p[(a + 2) % 3] = i >> 1 & 1;
+	p[a] = end;
+	return p[0] | p[1] << 1 | p[2] << 2;
+}
+
+static int mcEdgeBetween(int k0, int k1)
+{
+	for (int i = 0; i < 12; i++)
+		if ((mcEdgeCorner(i, 0) == k0 && mcEdgeCorner(i, 1) == k1) || (mcEdgeCorner(i, 0) == k1 && mcEdgeCorner(i, 1) == k0)) return i;
+	return -1;
+}
+
+static const McPolys &mcPolys()
+{
+	static McPolys P;
+	static bool built = false;
+	if (built) return P;
+	for (int type = 0; type < 256; type++) {
+		auto in = [&](int k) { return (type >> k & 1) != 0; };
+		int next[12];
+		for (int i = 0; i < 12; i++) next[i] = -1;
+		for (int f = 0; f < 3; f++)
+			for (int side = 0; side < 2; side++) {
+				// the face's corners counter-clockwise around its outward normal
+				const int u = (f + 1) % 3, v = (f + 2) % 3;   // (u, v, f) is right-handed
+				int ring[4];
+				const int uv[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+				for (int q = 0; q < 4; q++) {
+					int p[3] = {0, 0, 0};
+					p[f] = side;
+					p[u] = uv[q][0];
+					p[v] = uv[q][1];
+					ring[q] = p[0] | p[1] << 1 | p[2] << 2;
+				}
+				if (side == 0) std::swap(ring[1], ring[3]);   // outward normal -f: reverse the walk
+				// each crossing entered from outside (out -> in along the walk)
+				// is followed by the next crossing left to outside
+				for (int q = 0; q < 4; q++) {
+					const int a = ring[q], b = ring[(q + 1) % 4];
+					if (in(a) == in(b) || !in(b)) continue;
+					const int from = mcEdgeBetween(a, b);
+					for (int r = 1; r < 4; r++) {
+						const int c = ring[(q + r) % 4], d = ring[(q + r + 1) % 4];
+						if (in(c) && !in(d)) { next[from] = mcEdgeBetween(c, d); break; }
+					}
+				}
+			}
+		bool seen[12] = {};
+		for (int s0 = 0; s0 < 12; s0++) {
+			if (next[s0] < 0 || seen[s0]) continue;
+			std::vector<int> loop;
+			int cur = s0;
+			while (!seen[cur]) { seen[cur] = true; loop.push_back(cur); cur = next[cur]; }
+			std::rotate(loop.begin(), std::min_element(loop.begin(), loop.end()), loop.end());
+			for (size_t q = 1; q + 1 < loop.size(); q++) {
+				P.tri[type].push_back(loop[0]);
+				P.tri[type].push_back(loop[q]);
+				P.tri[type].push_back(loop[q + 1]);
+			}
+		}
+	}
+	built = true;
+	return P;
+}
+
+struct MeshOut {
+	std::vector<double> pos, nrm;
+	std::vector<unsigned> idx;
+};
+
+static void contour3(const int *n, const double *o, double dx, const double *f, MeshOut &M)
+{
+	const McPolys &P = mcPolys();
+	const int m[3] = {n[0] - 1, n[1] - 1, n[2] - 1};
+	auto node = [&](const int *c) { return size_t(c[0]) + size_t(n[0]) * (size_t(c[1]) + size_t(n[1]) * size_t(c[2])); };
+	// edgeMark per axis over the edge grids (node grid minus one along the axis)
+	std::vector<int> mark[3];
+	int en[3][3];
+	for (int a = 0; a < 3; a++) {
+		for (int q = 0; q < 3; q++) en[a][q] = n[q] - (q == a);
+		mark[a].assign(size_t(en[a][0]) * en[a][1] * en[a][2], -1);
+	}
+	auto edgeIndex = [&](int a, const int *e) { return size_t(e[0]) + size_t(en[a][0]) * (size_t(e[1]) + size_t(en[a][1]) * size_t(e[2])); };
+	M.pos.clear(); M.nrm.clear(); M.idx.clear();
+	if (m[0] < 1 || m[1] < 1 || m[2] < 1) return;
+	int c[3];
+	for (c[2] = 0; c[2] < m[2]; c[2]++)
+		for (c[1] = 0; c[1] < m[1]; c[1]++)
+			for (c[0] = 0; c[0] < m[0]; c[0]++) {
+				int type = 0;
+				for (int k = 0; k < 8; k++) {
+					const int q[3] = {c[0] + (k & 1), c[1] + (k >> 1 & 1), c[2] + (k >> 2 & 1)};
+					if (f[node(q)] <= 0) type |= 1 << k;
+				}
+				int eid[12];
+				for (int i = 0; i < 12; i++) {
+					const int a = i >> 2;
+					int e[3] = {c[0], c[1], c[2]};
+					e[(a + 1) % 3] += i & 1;
+					e[(a + 2) % 3] += i >> 1 & 1;
+					int e1[3] = {e[0], e[1], e[2]};
+					e1[a] += 1;
+					const bool cross = (f[node(e)] <= 0) != (f[node(e1)] <= 0);
+					int &mk = mark[a][edgeIndex(a, e)];
+					if (cross && mk < 0) {
+						const double f0 = f[node(e)], f1 = f[node(e1)];
+						const double theta = (f0 - 0) / (f0 - f1);
+						mk = int(M.pos.size() / 3);
+						for (int q = 0; q < 3; q++) {
+							const double p0 = o[q] + e[q] * dx, p1 = o[q] + e1[q] * dx;
+							M.pos.push_back((1 - theta) * p0 + theta * p1);
+						}
+					}
+					eid[i] = mk;
+				}
+				for (int ord : P.tri[type]) M.idx.push_back(unsigned(eid[ord]));
+			}
+	// computeNormals: unit face normals summed per vertex in triangle order
+	const size_t nv = M.pos.size() / 3;
+	M.nrm.assign(nv * 3, 0.0);
+	for (size_t t = 0; t < M.idx.size(); t += 3) {
+		const double *p0 = &M.pos[3 * M.idx[t]], *p1 = &M.pos[3 * M.idx[t + 1]], *p2 = &M.pos[3 * M.idx[t + 2]];
+		const double a[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+		const double b[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+		double nn[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+		const double s2 = nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2];
+		if (s2 > 0) { const double r = std::sqrt(s2); for (double &x : nn) x /= r; }
+		for (int j = 0; j < 3; j++)
+			for (int q = 0; q < 3; q++) M.nrm[3 * M.idx[t + j] + q] += nn[q];
+	}
+	for (size_t v = 0; v < nv; v++) {
+		double *x = &M.nrm[3 * v];
+		const double s2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+		if (s2 > 0) { const double r = std::sqrt(s2); for (int q = 0; q < 3; q++) x[q] /= r; }
+	}
+}
+
 } // namespace ora
 
 // ------------------------------------------------------------------ C API --
@@ -1846,6 +2005,35 @@ double ora_filled_area(const double *p);
 // summed serially in the reference's x-fastest cell order.  corrected = 0
 // passes the lexicographic stencil as the reference does; 1 reorders it to
 // the fraction's circular corner order first.
+// Contourer<3>::contour of a node field (n[0] x n[1] x n[2], x fastest,
+// origin o, spacing dx): counts always; arrays (float positions and normals,
+// 3 per vertex; uint indices) when the capacities suffice.  Returns 0, or -1
+// when a capacity is short.
+int ora_contour3(const int *n, const double *o, double dx, const double *f, float *pos, float *nrm, size_t vcap, unsigned *idx,
+	size_t icap, size_t *nv, size_t *ni)
+{
+	ora::MeshOut M;
+	ora::contour3(n, o, dx, f, M);
+	*nv = M.pos.size() / 3;
+	*ni = M.idx.size();
+	if (*nv > vcap || *ni > icap) return -1;
+	for (size_t i = 0; i < M.pos.size(); i++) { pos[i] = float(M.pos[i]); nrm[i] = float(M.nrm[i]); }
+	std::copy(M.idx.begin(), M.idx.end(), idx);
+	return 0;
+}
+// the triangle table: ntri[type] and up to 8 triangles (24 edge ords) per type
+int ora_mc_table(int *ntri, int *tri)
+{
+	const ora::McPolys &P = ora::mcPolys();
+	for (int t = 0; t < 256; t++) {
+		const size_t k = P.tri[t].size();
+		if (k > 24) return -1;
+		ntri[t] = int(k / 3);
+		for (int q = 0; q < 24; q++) tri[t * 24 + q] = q < int(k) ? P.tri[t][q] : -1;
+	}
+	return 0;
+}
+
 double ora_volume(void *p, int corrected)
 {
 	auto *h = (Handle *)p;
