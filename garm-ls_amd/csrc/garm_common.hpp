@@ -8,6 +8,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <chrono>
 #include <cstddef>
 #include <cstdint>
 
@@ -22,16 +23,30 @@ struct GridDev {
 
 constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
 
-// Host wait for the work enqueued on s so far: records ev (created with
-// hipEventDisableTiming) and polls it, so the wait for a small readback ends
-// within microseconds of its copy, where a stream synchronisation can leave
-// the thread asleep for tens of microseconds.
+// Host wait for a recorded event: polls it for up to kSpinMicros, so the
+// wait for a small readback ends within microseconds of its copy where a
+// blocking wait can leave the thread asleep for tens of microseconds, then
+// blocks in hipEventSynchronize so a long wait does not hold a host core.
+constexpr long kSpinMicros = 2000;
+inline hipError_t pollEvent(hipEvent_t ev)
+{
+	const auto t0 = std::chrono::steady_clock::now();
+	for (unsigned n = 1;; n++) {
+		const hipError_t e = hipEventQuery(ev);
+		if (e != hipErrorNotReady) return e;
+		if ((n & 255) == 0 &&
+			std::chrono::duration_cast<std::chrono::microseconds>(std::chrono::steady_clock::now() - t0).count() > kSpinMicros)
+			return hipEventSynchronize(ev);
+	}
+}
+
+// Host wait for the work enqueued on s so far (ev created with
+// hipEventDisableTiming).
 inline hipError_t spinWait(hipStream_t s, hipEvent_t ev)
 {
-	hipError_t e = hipEventRecord(ev, s);
+	const hipError_t e = hipEventRecord(ev, s);
 	if (e != hipSuccess) return e;
-	while ((e = hipEventQuery(ev)) == hipErrorNotReady) {}
-	return e;
+	return pollEvent(ev);
 }
 constexpr int kWave = 64;
 

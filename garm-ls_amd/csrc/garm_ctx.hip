@@ -850,10 +850,8 @@ int garm_step(garm_ctx *c, double dt, int step_index, const garm_step_opts *o, g
 	doEnforceBoundary(c);
 	(void)hipEventRecord(ev[6], c->s);
 	if ((rc = launched(c, "step"))) return rc;
-	// polled, as spinWait: the step's end returns to the caller without a sleep
-	hipError_t e;
-	while ((e = hipEventQuery(ev[6])) == hipErrorNotReady) {}
-	if ((rc = hipCheck(c, e, "step"))) return rc;
+	// polled first, as spinWait: the step's end returns to the caller without a sleep
+	if ((rc = hipCheck(c, pollEvent(ev[6]), "step"))) return rc;
 	float t[7] = {};
 	for (int i = 1; i <= 6; i++) (void)hipEventElapsedTime(&t[i], ev[i - 1], ev[i]);
 	st->t_advect_ms = t[1];

@@ -465,11 +465,8 @@ static int readStart(LearnedWork &W, const double *src, int count, hipStream_t s
 
 static int readWait(LearnedWork &W, std::string &err)
 {
-	for (;;) {
-		const hipError_t e = hipEventQuery(W.ready);
-		if (e == hipSuccess) return GARM_OK;
-		if (e != hipErrorNotReady) { err = "learned-solver readback failed"; return GARM_EHIP; }
-	}
+	if (pollEvent(W.ready) != hipSuccess) { err = "learned-solver readback failed"; return GARM_EHIP; }
+	return GARM_OK;
 }
 
 static int readHost(LearnedWork &W, const double *src, int count, hipStream_t s, std::string &err)
