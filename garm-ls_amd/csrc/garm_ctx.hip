@@ -666,7 +666,8 @@ int garm_contour(garm_ctx *c, float *positions, float *normals, size_t vertex_ca
 	size_t *n_vertices, size_t *n_indices)
 {
 	if (!c || !n_vertices || !n_indices) return GARM_EINVAL;
-	std::vector<float> pos, nrm;
+	std::vector<double> pos;
+	std::vector<float> nrm;
 	std::vector<unsigned> idx;
 	std::string err;
 	const int rc = contourMesh(c->F, c->F.phi_tmp, pos, nrm, idx, c->s, err);
@@ -676,30 +677,48 @@ int garm_contour(garm_ctx *c, float *positions, float *normals, size_t vertex_ca
 	if (*n_vertices > vertex_capacity || *n_indices > index_capacity || (*n_vertices && (!positions || !normals)) ||
 		(*n_indices && !indices))
 		return fail(c, GARM_EINVAL, "contour: the buffers are too small");
-	std::copy(pos.begin(), pos.end(), positions);
+	for (size_t i = 0; i < pos.size(); i++) positions[i] = float(pos[i]);
 	std::copy(nrm.begin(), nrm.end(), normals);
 	std::copy(idx.begin(), idx.end(), indices);
 	return GARM_OK;
 }
 
-int garm_write_mesh(garm_ctx *c, const char *path)
+int garm_write_mesh(garm_ctx *c, const char *mesh_path, const char *obj_path)
 {
-	if (!c || !path) return GARM_EINVAL;
-	std::vector<float> pos, nrm;
+	if (!c || (!mesh_path && !obj_path)) return GARM_EINVAL;
+	std::vector<double> pos;
+	std::vector<float> nrm;
 	std::vector<unsigned> idx;
 	std::string err;
 	const int rc = contourMesh(c->F, c->F.phi_tmp, pos, nrm, idx, c->s, err);
 	if (rc) return fail(c, rc, "contour: " + err);
 	if (pos.size() / 3 > 0xffffffffull || idx.size() > 0xffffffffull) return fail(c, GARM_EINVAL, "mesh too large for its uint32 counts");
-	std::ofstream out(path, std::ios::binary);
-	if (!out) return fail(c, GARM_EINVAL, std::string("cannot open ") + path);
-	const uint32_t nv = uint32_t(pos.size() / 3), ni = uint32_t(idx.size());
-	out.write(reinterpret_cast<const char *>(&nv), sizeof(nv));
-	out.write(reinterpret_cast<const char *>(pos.data()), std::streamsize(pos.size() * sizeof(float)));
-	out.write(reinterpret_cast<const char *>(nrm.data()), std::streamsize(nrm.size() * sizeof(float)));
-	out.write(reinterpret_cast<const char *>(&ni), sizeof(ni));
-	out.write(reinterpret_cast<const char *>(idx.data()), std::streamsize(idx.size() * sizeof(unsigned)));
-	if (!out) return fail(c, GARM_EINVAL, std::string("writing ") + path + " failed");
+	if (mesh_path) {
+		std::ofstream out(mesh_path, std::ios::binary);
+		if (!out) return fail(c, GARM_EINVAL, std::string("cannot open ") + mesh_path);
+		std::vector<float> pf(pos.size());
+		for (size_t i = 0; i < pos.size(); i++) pf[i] = float(pos[i]);
+		const uint32_t nv = uint32_t(pos.size() / 3), ni = uint32_t(idx.size());
+		out.write(reinterpret_cast<const char *>(&nv), sizeof(nv));
+		out.write(reinterpret_cast<const char *>(pf.data()), std::streamsize(pf.size() * sizeof(float)));
+		out.write(reinterpret_cast<const char *>(nrm.data()), std::streamsize(nrm.size() * sizeof(float)));
+		out.write(reinterpret_cast<const char *>(&ni), sizeof(ni));
+		out.write(reinterpret_cast<const char *>(idx.data()), std::streamsize(idx.size() * sizeof(unsigned)));
+		if (!out) return fail(c, GARM_EINVAL, std::string("writing ") + mesh_path + " failed");
+	}
+	if (obj_path) {
+		// SurfaceMesh::writeOBJ of a contoured mesh: its normals member is
+		// empty (write() computes them into a temporary), so no vn lines
+		std::FILE *fp = std::fopen(obj_path, "w");
+		if (!fp) return fail(c, GARM_EINVAL, std::string("cannot open ") + obj_path);
+		bool ok = true;
+		for (size_t v = 0; v < pos.size() / 3 && ok; v++)
+			ok = std::fprintf(fp, "v %.15e %.15e %.15e\n", pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]) > 0;
+		for (size_t t = 0; t + 2 < idx.size() && ok; t += 3)
+			ok = std::fprintf(fp, "f %u %u %u\n", idx[t] + 1, idx[t + 1] + 1, idx[t + 2] + 1) > 0;
+		ok = std::fclose(fp) == 0 && ok;
+		if (!ok) return fail(c, GARM_EINVAL, std::string("writing ") + obj_path + " failed");
+	}
 	return GARM_OK;
 }
 

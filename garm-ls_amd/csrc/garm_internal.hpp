@@ -97,9 +97,9 @@ void launchExtrapolateCollider(const Fields &F, hipStream_t s);
 // finds the collider extrapolation's layers (after cellDist is set); synchronises
 bool buildColliderLayers(Fields &F, hipStream_t s);
 // Contourer<3>::contour of `field` (a cell field) at 0 with SurfaceMesh
-// normals: float positions and normals (3 per vertex), uint indices
+// normals: FP64 positions and float normals (3 per vertex), uint indices
 // (3 per triangle) in the reference's vertex order.
-int contourMesh(const Fields &F, const double *field, std::vector<float> &pos, std::vector<float> &nrm, std::vector<unsigned> &idx,
+int contourMesh(const Fields &F, const double *field, std::vector<double> &pos, std::vector<float> &nrm, std::vector<unsigned> &idx,
 	hipStream_t s, std::string &err);
 // velocityOnly: extrapolateFieldsToAir(true), the velocity alone
 void launchExtrapolateAir(const Fields &F, hipStream_t s, bool velocityOnly = false);

@@ -1,8 +1,9 @@
 // Surface mesh of a cell field for gfx950: Contourer<3>::contour at 0
 // (Contourer.cpp:9-97) with SurfaceMesh::computeNormals (SurfaceMesh.cpp:
 // 191-216), FP64 with the reference's operations (compiled without
-// multiply-add contraction), positions and normals rounded to float as
-// SurfaceMesh::write stores them (SurfaceMesh.cpp:103-124).
+// multiply-add contraction): positions in FP64 (SurfaceMesh::writeOBJ
+// prints them, SurfaceMesh.cpp:126-144; SurfaceMesh::write rounds them to
+// float, :103-124), normals rounded to float on the device.
 //
 // The field's grid points are the marching cubes' nodes; a cube is a cell of
 // that node grid.  Numbering follows the reference exactly: node ord k of a
@@ -222,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void kMeshTileCounts(McGrid G, const double
 	if (threadIdx.x == 0) { tileV[blockIdx.x] = tv; tileT[blockIdx.x] = tt; }
 }
 
-__global__ __launch_bounds__(kBlock) void kMeshVertices(McGrid G, const double *f, const int *tileVBase, int *vbase, float *pos)
+__global__ __launch_bounds__(kBlock) void kMeshVertices(McGrid G, const double *f, const int *tileVBase, int *vbase, double *pos)
 {
 	const size_t base = size_t(blockIdx.x) * kScanTile;
 	int carry = tileVBase[blockIdx.x];
@@ -242,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void kMeshVertices(McGrid G, const double *
 				if (!(mine >> i & 1)) continue;
 				double p[3];
 				edgeVertex(G, f, c, i, p);
-				for (int q = 0; q < 3; q++) pos[size_t(v) * 3 + q] = float(p[q]);
+				for (int q = 0; q < 3; q++) pos[size_t(v) * 3 + q] = p[q];
 				v++;
 			}
 		}
@@ -374,7 +375,7 @@ static int ensureTable(std::string &err)
 	return GARM_OK;
 }
 
-int contourMesh(const Fields &F, const double *field, std::vector<float> &pos, std::vector<float> &nrm, std::vector<unsigned> &idx,
+int contourMesh(const Fields &F, const double *field, std::vector<double> &pos, std::vector<float> &nrm, std::vector<unsigned> &idx,
 	hipStream_t s, std::string &err)
 {
 	if (F.D != 3) { err = "the surface mesh is built for 3D grids"; return GARM_EINVAL; }
@@ -390,7 +391,8 @@ int contourMesh(const Fields &F, const double *field, std::vector<float> &pos, s
 	const size_t nI = size_t(tiles) * 4 + 4 + G.M;
 	if (hipMalloc(&ibuf, nI * sizeof(int)) != hipSuccess) { err = "hipMalloc failed for the mesh counts"; return GARM_ENOMEM; }
 	int *tileV = ibuf, *tileT = ibuf + tiles, *baseV = ibuf + 2 * tiles, *baseT = ibuf + 3 * tiles + 1, *vbase = ibuf + 4 * tiles + 4;
-	float *dpos = nullptr, *dnrm = nullptr;
+	double *dpos = nullptr;
+	float *dnrm = nullptr;
 	unsigned *didx = nullptr;
 	int counts[2] = {0, 0};
 	auto done = [&](int code) {
@@ -410,11 +412,11 @@ int contourMesh(const Fields &F, const double *field, std::vector<float> &pos, s
 		return done(GARM_EHIP);
 	}
 	const size_t nv = size_t(counts[0]), nt = size_t(counts[1]);
-	pos.assign(nv * 3, 0.f);
+	pos.assign(nv * 3, 0.0);
 	nrm.assign(nv * 3, 0.f);
 	idx.assign(nt * 3, 0u);
 	if (nv == 0) return done(GARM_OK);
-	if (hipMalloc(&dpos, nv * 3 * sizeof(float)) != hipSuccess || hipMalloc(&dnrm, nv * 3 * sizeof(float)) != hipSuccess ||
+	if (hipMalloc(&dpos, nv * 3 * sizeof(double)) != hipSuccess || hipMalloc(&dnrm, nv * 3 * sizeof(float)) != hipSuccess ||
 		(nt && hipMalloc(&didx, nt * 3 * sizeof(unsigned)) != hipSuccess)) {
 		err = "hipMalloc failed for the mesh";
 		return done(GARM_ENOMEM);
@@ -423,7 +425,7 @@ int contourMesh(const Fields &F, const double *field, std::vector<float> &pos, s
 	if (nt) hipLaunchKernelGGL(kMeshTriangles, dim3(tiles), dim3(kBlock), 0, s, G, field, (const int *)baseT, (const int *)vbase, didx);
 	hipLaunchKernelGGL(kMeshNormals, dim3(unsigned((G.M + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, G, field, (const int *)vbase, dnrm);
 	if (hipGetLastError() != hipSuccess ||
-		hipMemcpyAsync(pos.data(), dpos, nv * 3 * sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess ||
+		hipMemcpyAsync(pos.data(), dpos, nv * 3 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
 		hipMemcpyAsync(nrm.data(), dnrm, nv * 3 * sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess ||
 		(nt && hipMemcpyAsync(idx.data(), didx, nt * 3 * sizeof(unsigned), hipMemcpyDeviceToHost, s) != hipSuccess) ||
 		hipStreamSynchronize(s) != hipSuccess) {

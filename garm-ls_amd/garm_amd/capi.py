@@ -134,7 +134,7 @@ def lib():
                                                ctypes.POINTER(SurfaceTensionStats)]),
             "garm_set_elapsed_time": (i, [vp, d]),
             "garm_contour": (i, [vp, vp, vp, sz, vp, sz, ctypes.POINTER(sz), ctypes.POINTER(sz)]),
-            "garm_write_mesh": (i, [vp, ctypes.c_char_p]),
+            "garm_write_mesh": (i, [vp, ctypes.c_char_p, ctypes.c_char_p]),
             "garm_get_elapsed_time": (i, [vp, dp]),
             "garm_max_speed": (i, [vp, dp]),
             "garm_apply_pressure_system": (i, [vp, vp, vp]),
@@ -317,9 +317,11 @@ class Liquid:
                                        ctypes.byref(nv), ctypes.byref(ni)))
         return pos, nrm, idx.reshape(-1, 3)
 
-    def write_mesh(self, path):
-        """garm_write_mesh: phi_tmp's surface as a liquid.mesh file."""
-        self._check(lib().garm_write_mesh(self.h, os.fsencode(path)))
+    def write_mesh(self, mesh_path=None, obj_path=None):
+        """garm_write_mesh: phi_tmp's surface as a liquid.mesh file and/or an
+        OBJ file."""
+        self._check(lib().garm_write_mesh(self.h, os.fsencode(mesh_path) if mesh_path else None,
+                                          os.fsencode(obj_path) if obj_path else None))
 
     @property
     def elapsed_time(self):

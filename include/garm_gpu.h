@@ -308,12 +308,15 @@ int garm_load_frame(garm_ctx *ctx, const char *frame_dir);
  * vertices and in indices) suffice, else GARM_EINVAL.  3D contexts only. */
 int garm_contour(garm_ctx *ctx, float *positions, float *normals, size_t vertex_capacity, unsigned *indices, size_t index_capacity,
 	size_t *n_vertices, size_t *n_indices);
-/* The same mesh written to `path` in SurfaceMesh::write's binary layout
- * (SurfaceMesh.cpp:103-124), the reference's frame file liquid.mesh
- * (LevelSetLiquid.cpp:306-345 without refinement): uint32 vertex count,
- * float positions (x, y, z), float normals, uint32 index count, uint32
- * indices. */
-int garm_write_mesh(garm_ctx *ctx, const char *path);
+/* The same mesh written as the reference's frame files (LevelSetLiquid.cpp:
+ * 306-347 without refinement; either path may be NULL): mesh_path in
+ * SurfaceMesh::write's binary layout (SurfaceMesh.cpp:103-124, liquid.mesh:
+ * uint32 vertex count, float positions (x, y, z), float normals, uint32
+ * index count, uint32 indices), obj_path as SurfaceMesh::writeOBJ
+ * (SurfaceMesh.cpp:126-144, render/N.obj: "v" lines of the FP64 positions
+ * printed %.15e, no "vn" lines -- the contoured mesh's normals member is
+ * empty -- and 1-based "f" lines). */
+int garm_write_mesh(garm_ctx *ctx, const char *mesh_path, const char *obj_path);
 
 /* Sets the face velocity of a 3D context to an analytic field at time t
  * with period T, on the device (the reference's tests evaluate it per

@@ -2009,8 +2009,9 @@ double ora_filled_area(const double *p);
 // origin o, spacing dx): counts always; arrays (float positions and normals,
 // 3 per vertex; uint indices) when the capacities suffice.  Returns 0, or -1
 // when a capacity is short.
+// pos64 (optional): the FP64 positions SurfaceMesh::writeOBJ prints.
 int ora_contour3(const int *n, const double *o, double dx, const double *f, float *pos, float *nrm, size_t vcap, unsigned *idx,
-	size_t icap, size_t *nv, size_t *ni)
+	size_t icap, size_t *nv, size_t *ni, double *pos64)
 {
 	ora::MeshOut M;
 	ora::contour3(n, o, dx, f, M);
@@ -2018,6 +2019,7 @@ int ora_contour3(const int *n, const double *o, double dx, const double *f, floa
 	*ni = M.idx.size();
 	if (*nv > vcap || *ni > icap) return -1;
 	for (size_t i = 0; i < M.pos.size(); i++) { pos[i] = float(M.pos[i]); nrm[i] = float(M.nrm[i]); }
+	if (pos64) std::copy(M.pos.begin(), M.pos.end(), pos64);
 	std::copy(M.idx.begin(), M.idx.end(), idx);
 	return 0;
 }

@@ -67,7 +67,7 @@ def lib():
             "ora_capillary_stats": (None, [vp, ip, ip, dp, dp]),
             "ora_set_elapsed_time": (None, [vp, d]),
             "ora_capillary": (i, [vp, d, d, d, d, i, i]),
-            "ora_contour3": (i, [ip, dp, d, dp, vp, vp, sz, vp, sz, ctypes.POINTER(sz), ctypes.POINTER(sz)]),
+            "ora_contour3": (i, [ip, dp, d, dp, vp, vp, sz, vp, sz, ctypes.POINTER(sz), ctypes.POINTER(sz), vp]),
             "ora_mc_table": (i, [ip, ip]),
             "ora_curvature": (d, [vp, dp]),
             "ora_little_drop": (None, [vp, d]),
@@ -271,21 +271,23 @@ def filled_area(v):
     return lib().ora_filled_area(a)
 
 
-def contour3(field, origin, dx):
+def contour3(field, origin, dx, fp64_positions=False):
     """Contourer<3>::contour at 0 of a node field of shape (nz, ny, nx):
-    (positions (nv, 3) float32, normals (nv, 3) float32, triangles (nt, 3) uint32)."""
+    (positions (nv, 3) float32, normals (nv, 3) float32, triangles (nt, 3) uint32),
+    positions in float64 when fp64_positions."""
     f = np.ascontiguousarray(field, dtype=np.float64)
     n = (ctypes.c_int * 3)(f.shape[2], f.shape[1], f.shape[0])
     nv, ni = ctypes.c_size_t(), ctypes.c_size_t()
     args = (n, _d3(origin), dx, f.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
-    lib().ora_contour3(*args, None, None, 0, None, 0, ctypes.byref(nv), ctypes.byref(ni))
+    lib().ora_contour3(*args, None, None, 0, None, 0, ctypes.byref(nv), ctypes.byref(ni), None)
     pos = np.zeros((nv.value, 3), np.float32)
     nrm = np.zeros((nv.value, 3), np.float32)
     idx = np.zeros(ni.value, np.uint32)
+    p64 = np.zeros((nv.value, 3), np.float64)
     rc = lib().ora_contour3(*args, pos.ctypes.data, nrm.ctypes.data, nv.value, idx.ctypes.data, ni.value,
-                            ctypes.byref(nv), ctypes.byref(ni))
+                            ctypes.byref(nv), ctypes.byref(ni), p64.ctypes.data)
     assert rc == 0
-    return pos, nrm, idx.reshape(-1, 3)
+    return (p64 if fp64_positions else pos), nrm, idx.reshape(-1, 3)
 
 
 def mc_table():

@@ -1,7 +1,9 @@
 """Surface mesh on the GPU (garm_contour, garm_write_mesh) against the
 oracle's Contourer<3>::contour restatement on the same phi_tmp: vertex
 positions and triangle indices identical, normals within float rounding;
-the liquid.mesh file read back equal to the in-memory mesh.  Fields: a
+the liquid.mesh file read back equal to the in-memory mesh, and the OBJ
+file equal, character for character, to the oracle's FP64 positions printed
+as SurfaceMesh::writeOBJ prints them.  Fields: a
 random one (every cube type occurs), a sphere, and a dam break's phi_tmp
 after a few steps at 64^3.
 """
@@ -54,11 +56,15 @@ def test_sphere_and_file(tmp_path):
     d.set(capi.PHI_TMP, f)
     say("sphere 48^3")
     pos, nrm, tri = _compare(g, d, f)
-    path = os.path.join(tmp_path, "liquid.mesh")
-    d.write_mesh(path)
+    path, obj = os.path.join(tmp_path, "liquid.mesh"), os.path.join(tmp_path, "0.obj")
+    d.write_mesh(path, obj)
     fp, fn, ft = capi.read_mesh(path)
     assert np.array_equal(fp, pos) and np.array_equal(fn, nrm) and np.array_equal(ft, tri)
     assert os.path.getsize(path) == 4 + 24 * len(pos) + 4 + 12 * len(tri)
+    # SurfaceMesh::writeOBJ: FP64 positions printed %.15e, 1-based faces, no normals
+    p64, _, _ = O.contour3(f.reshape(g["res"][::-1]), _node_origin(g), g["dx"], fp64_positions=True)
+    want = "".join("v %.15e %.15e %.15e\n" % tuple(p) for p in p64) + "".join("f %d %d %d\n" % tuple(t + 1) for t in tri)
+    assert open(obj).read() == want
 
 
 def test_dam_break_phi_tmp():
