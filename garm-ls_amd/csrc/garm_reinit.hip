@@ -80,6 +80,39 @@ __device__ __forceinline__ void weno5f(const float *v, float &dm, float &dp)
 	}
 }
 
+// One WENO5 window of five differences w0..w4 (times dx) along an axis: the
+// minus-side derivative of the cell whose stencil starts the window
+// (forward) and the plus-side derivative of the cell before that one (the
+// window reversed).  The two share the window's epsilon, its three
+// smoothness indicators and the products behind the nonlinear weights --
+// reversed, the indicators swap ends, so the weights (a1, a2, a3) become
+// (a3, a2, a1) -- with the formulas of weno5f's minus side; only the
+// candidate polynomials and the normalisation are per side.  A thread that
+// walks a column evaluates one window per cell instead of two.
+__device__ __forceinline__ void weno5Window(const float *w, const float *q, float &fwd, float &rev)
+{
+	const float eps = fmaxf(fmaxf(fmaxf(q[1], q[2]), fmaxf(q[3], q[4])), q[0]) * 4e-6f + 4e-18f;
+	constexpr float k133 = 13.f / 3, k3 = 1.f / 3, k76 = 7.f / 6, k116 = 11.f / 6;
+	const float e1 = w[0] - 2 * w[1] + w[2], e2 = w[1] - 2 * w[2] + w[3], e3 = w[2] - 2 * w[3] + w[4];
+	const float sa = e1 * e1 * k133 + sq(w[0] - 4 * w[1] + 3 * w[2]);
+	const float sb = e2 * e2 * k133 + sq(w[1] - w[3]);
+	const float sc = e3 * e3 * k133 + sq(3 * w[2] - 4 * w[3] + w[4]);
+	const float r = rcpT(eps), ta = sq(sa * r + 1), tb = sq(sb * r + 1), tc = sq(sc * r + 1);
+	const float a1 = tb * tc, a2 = ta * tc, a3 = ta * tb;
+	{
+		const float p1 = w[0] * k3 - w[1] * k76 + w[2] * k116;
+		const float p2 = 5 * w[2] - w[1] + 2 * w[3];
+		const float p3 = w[2] + 2.5f * w[3] - 0.5f * w[4];
+		fwd = (a1 * p1 + a2 * p2 + a3 * p3) * rcpT(a1 + 6 * a2 + 3 * a3);
+	}
+	{
+		const float p1 = w[4] * k3 - w[3] * k76 + w[2] * k116;
+		const float p2 = 5 * w[2] - w[3] + 2 * w[1];
+		const float p3 = w[2] + 2.5f * w[1] - 0.5f * w[0];
+		rev = (a3 * p1 + a2 * p2 + a1 * p3) * rcpT(a3 + 6 * a2 + 3 * a1);
+	}
+}
+
 // The FP64 form evaluates the reference's expressions in the reference's
 // order (divisions by the constants, left to right) without contracting
 // a * b + c into fused multiply-adds, i.e. the IEEE operations of the CPU
@@ -258,6 +291,95 @@ __global__ __launch_bounds__(kBlock) void kReinitStage(GridDev g, const TI *in, 
 	}
 }
 
+// The FP32 3D stage with the z axis walked: a thread owns kReinitZ cells of
+// a z-column, loads the column's 7-point z stencils once (kReinitZ + 6
+// values) and evaluates one z window per cell (weno5Window: the minus side
+// of cell k and the plus side of cell k - 1 from the same window) instead of
+// two; x and y are as in kReinitStage.  Blocks are 64 x 4 threads over x, y
+// and kReinitZ cells in z, so a wavefront is one x row at fixed j.
+constexpr int kReinitZ = 8;
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void kReinitStageZ(GridDev g, const float *in, float *out, double *phi, float *phi32, double dtau)
+{
+	constexpr int KZ = kReinitZ;
+	const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+	const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+	const int j = __builtin_amdgcn_readfirstlane(int(blockIdx.y * blockDim.y + threadIdx.y));
+	const int k0 = int(blockIdx.z) * KZ;
+	if (i >= nx || j >= ny) return;
+	const unsigned ib = unsigned(i) * 4u;
+	float zv[KZ + 6];
+#pragma unroll
+	for (int t = 0; t < KZ + 6; t++) {
+		const unsigned kr = (unsigned(clampi(k0 - 3 + t, nz - 1)) * unsigned(ny) + unsigned(j)) * unsigned(nx);
+		zv[t] = ldOff(in + kr, ib);
+	}
+	float dz[KZ + 5], qz[KZ + 5];
+#pragma unroll
+	for (int t = 0; t < KZ + 5; t++) { dz[t] = zv[t + 1] - zv[t]; qz[t] = dz[t] * dz[t]; }
+	const float dt = float(dtau);
+	const int i0 = int(blockIdx.x * blockDim.x);   // the wave's first column
+	const bool interiorX = i0 >= 3 && i0 + 64 + 3 <= nx;
+	float dmPrev = 0;
+#pragma unroll
+	for (int w = 0; w <= KZ; w++) {
+		float fwd, rev;
+		weno5Window(dz + w, qz + w, fwd, rev);
+		if (w > 0) {
+			const int kk = w - 1, k = k0 + kk;
+			if (k < nz) {
+				const unsigned row = (unsigned(k) * unsigned(ny) + unsigned(j)) * unsigned(nx);
+				float v[2][7];
+				const float *rp = in + row;
+				if (interiorX) {
+#pragma unroll
+					for (int t = 0; t < 7; t++) v[0][t] = ldOff(rp + (t - 3), ib);
+				} else {
+#pragma unroll
+					for (int t = 0; t < 7; t++) v[0][t] = rp[clampi(i + t - 3, nx - 1)];
+				}
+#pragma unroll
+				for (int t = 0; t < 7; t++) {
+					if (t == 3) { v[1][3] = v[0][3]; continue; }
+					const unsigned jr = (unsigned(k) * unsigned(ny) + unsigned(clampi(j + t - 3, ny - 1))) * unsigned(nx);
+					v[1][t] = ldOff(in + jr, ib);
+				}
+				float dm[3], dp[3];
+				weno5f(v[0], dm[0], dp[0]);
+				weno5f(v[1], dm[1], dp[1]);
+				dm[2] = dmPrev;
+				dp[2] = rev;
+				const float val = zv[kk + 3];
+				float sum = 0;
+				if (val < 0) {
+#pragma unroll
+					for (int a = 0; a < 3; a++) { sum += sq(fminf(dm[a], 0.f)); sum += sq(fmaxf(dp[a], 0.f)); }
+				} else {
+#pragma unroll
+					for (int a = 0; a < 3; a++) { sum += sq(fmaxf(dm[a], 0.f)); sum += sq(fminf(dp[a], 0.f)); }
+				}
+				sum = __builtin_amdgcn_sqrtf(sum) * float(g.inv);
+				const float sp = float(g.dx) * sum;
+				const float l = val + dt * (val * rsqrtT(val * val + sp * sp)) * (1 - sum);
+				if constexpr (MODE == 0) (out + row)[i] = l;
+				else if constexpr (MODE == 1) (out + row)[i] = ((phi32 + row)[i] * 3 + l) / 4;
+				else {
+					const float x = ((phi32 + row)[i] + l * 2) / 3;
+					(phi + row)[i] = x;
+					(phi32 + row)[i] = x;
+				}
+			}
+		}
+		dmPrev = fwd;
+	}
+}
+
+static dim3 gridDimsZ(const GridDev &g)
+{
+	return dim3((g.n[0] + 63) / 64, (g.n[1] + 3) / 4, (g.n[2] + kReinitZ - 1) / kReinitZ);
+}
+
 __global__ void kPhiToFloat(const double *phi, float *out, size_t n)
 {
 	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) out[i] = float(phi[i]);
@@ -288,10 +410,19 @@ static void reinitD(const Fields &F, int iterations, double cfl, hipStream_t s)
 	} else {
 		float *s0 = reinterpret_cast<float *>(F.scratch[0]), *s1 = reinterpret_cast<float *>(F.scratch[1]), *p32 = phi32Of(F);
 		phiToFloat(F, s);
-		for (int it = 0; it < iterations; it++) {
-			hipLaunchKernelGGL((kReinitStage<D, 0, float, float, float>), gd, bd, 0, s, F.cell, (const float *)p32, s0, F.phi, p32, dtau);
-			hipLaunchKernelGGL((kReinitStage<D, 1, float, float, float>), gd, bd, 0, s, F.cell, (const float *)s0, s1, F.phi, p32, dtau);
-			hipLaunchKernelGGL((kReinitStage<D, 2, float, float, float>), gd, bd, 0, s, F.cell, (const float *)s1, (float *)nullptr, F.phi, p32, dtau);
+		if constexpr (D == 3) {
+			const dim3 gz = gridDimsZ(F.cell);
+			for (int it = 0; it < iterations; it++) {
+				hipLaunchKernelGGL(kReinitStageZ<0>, gz, bd, 0, s, F.cell, (const float *)p32, s0, F.phi, p32, dtau);
+				hipLaunchKernelGGL(kReinitStageZ<1>, gz, bd, 0, s, F.cell, (const float *)s0, s1, F.phi, p32, dtau);
+				hipLaunchKernelGGL(kReinitStageZ<2>, gz, bd, 0, s, F.cell, (const float *)s1, (float *)nullptr, F.phi, p32, dtau);
+			}
+		} else {
+			for (int it = 0; it < iterations; it++) {
+				hipLaunchKernelGGL((kReinitStage<D, 0, float, float, float>), gd, bd, 0, s, F.cell, (const float *)p32, s0, F.phi, p32, dtau);
+				hipLaunchKernelGGL((kReinitStage<D, 1, float, float, float>), gd, bd, 0, s, F.cell, (const float *)s0, s1, F.phi, p32, dtau);
+				hipLaunchKernelGGL((kReinitStage<D, 2, float, float, float>), gd, bd, 0, s, F.cell, (const float *)s1, (float *)nullptr, F.phi, p32, dtau);
+			}
 		}
 	}
 }
@@ -308,7 +439,7 @@ void launchReinitStage(const Fields &F, double dtau, hipStream_t s, bool fp32)
 	if (fp32) {
 		float *s0 = reinterpret_cast<float *>(F.scratch[0]), *p32 = phi32Of(F);
 		if (F.D == 2) hipLaunchKernelGGL((kReinitStage<2, 0, float, float, float>), gd, bd, 0, s, F.cell, (const float *)p32, s0, F.phi, p32, dtau);
-		else hipLaunchKernelGGL((kReinitStage<3, 0, float, float, float>), gd, bd, 0, s, F.cell, (const float *)p32, s0, F.phi, p32, dtau);
+		else hipLaunchKernelGGL(kReinitStageZ<0>, gridDimsZ(F.cell), bd, 0, s, F.cell, (const float *)p32, s0, F.phi, p32, dtau);
 	} else {
 		if (F.D == 2) hipLaunchKernelGGL((kReinitStage<2, 0, double, double, double>), gd, bd, 0, s, F.cell, phi, F.scratch[0], F.phi, (float *)nullptr, dtau);
 		else hipLaunchKernelGGL((kReinitStage<3, 0, double, double, double>), gd, bd, 0, s, F.cell, phi, F.scratch[0], F.phi, (float *)nullptr, dtau);
