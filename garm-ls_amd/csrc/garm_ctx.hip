@@ -672,9 +672,9 @@ int garm_contour(garm_ctx *c, float *positions, float *normals, size_t vertex_ca
 	std::string err;
 	const int rc = contourMesh(c->F, c->F.phi_tmp, pos, nrm, idx, c->s, err);
 	if (rc) return fail(c, rc, "contour: " + err);
-	*n_vertices = pos.size() / 3;
+	*n_vertices = pos.size() / size_t(c->F.D);
 	*n_indices = idx.size();
-	if (*n_vertices > vertex_capacity || *n_indices > index_capacity || (*n_vertices && (!positions || !normals)) ||
+	if (*n_vertices > vertex_capacity || *n_indices > index_capacity || (*n_vertices && (!positions || (c->F.D == 3 && !normals))) ||
 		(*n_indices && !indices))
 		return fail(c, GARM_EINVAL, "contour: the buffers are too small");
 	for (size_t i = 0; i < pos.size(); i++) positions[i] = float(pos[i]);
@@ -692,13 +692,14 @@ int garm_write_mesh(garm_ctx *c, const char *mesh_path, const char *obj_path)
 	std::string err;
 	const int rc = contourMesh(c->F, c->F.phi_tmp, pos, nrm, idx, c->s, err);
 	if (rc) return fail(c, rc, "contour: " + err);
-	if (pos.size() / 3 > 0xffffffffull || idx.size() > 0xffffffffull) return fail(c, GARM_EINVAL, "mesh too large for its uint32 counts");
+	const size_t D = size_t(c->F.D), nvert = pos.size() / D;
+	if (nvert > 0xffffffffull || idx.size() > 0xffffffffull) return fail(c, GARM_EINVAL, "mesh too large for its uint32 counts");
 	if (mesh_path) {
 		std::ofstream out(mesh_path, std::ios::binary);
 		if (!out) return fail(c, GARM_EINVAL, std::string("cannot open ") + mesh_path);
 		std::vector<float> pf(pos.size());
 		for (size_t i = 0; i < pos.size(); i++) pf[i] = float(pos[i]);
-		const uint32_t nv = uint32_t(pos.size() / 3), ni = uint32_t(idx.size());
+		const uint32_t nv = uint32_t(nvert), ni = uint32_t(idx.size());
 		out.write(reinterpret_cast<const char *>(&nv), sizeof(nv));
 		out.write(reinterpret_cast<const char *>(pf.data()), std::streamsize(pf.size() * sizeof(float)));
 		out.write(reinterpret_cast<const char *>(nrm.data()), std::streamsize(nrm.size() * sizeof(float)));
@@ -708,12 +709,14 @@ int garm_write_mesh(garm_ctx *c, const char *mesh_path, const char *obj_path)
 	}
 	if (obj_path) {
 		// SurfaceMesh::writeOBJ of a contoured mesh: its normals member is
-		// empty (write() computes them into a temporary), so no vn lines
+		// empty (write() computes them into a temporary), so no vn lines;
+		// 2D vertices get a z of 0
 		std::FILE *fp = std::fopen(obj_path, "w");
 		if (!fp) return fail(c, GARM_EINVAL, std::string("cannot open ") + obj_path);
 		bool ok = true;
-		for (size_t v = 0; v < pos.size() / 3 && ok; v++)
-			ok = std::fprintf(fp, "v %.15e %.15e %.15e\n", pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]) > 0;
+		for (size_t v = 0; v < nvert && ok; v++)
+			ok = (D == 3 ? std::fprintf(fp, "v %.15e %.15e %.15e\n", pos[3 * v], pos[3 * v + 1], pos[3 * v + 2])
+			             : std::fprintf(fp, "v %.15e %.15e 0\n", pos[2 * v], pos[2 * v + 1])) > 0;
 		for (size_t t = 0; t + 2 < idx.size() && ok; t += 3)
 			ok = std::fprintf(fp, "f %u %u %u\n", idx[t] + 1, idx[t + 1] + 1, idx[t + 2] + 1) > 0;
 		ok = std::fclose(fp) == 0 && ok;

@@ -375,10 +375,129 @@ static int ensureTable(std::string &err)
 	return GARM_OK;
 }
 
+// ------------------------------------------------------------- 2D, host
+// Contourer<2, true>::contour at 0 (Contourer.cpp:9-97), the filled contour
+// the reference writes for 2D frames: vertices are the inside nodes (field
+// <= 0) in node order, then the edge crossings in cell order and edge order
+// (first cell to meet a crossing edge creates it); cell types 6 and 9 (two
+// diagonal inside nodes) become 16 and 17 when the mean of the four nodes
+// is <= 0 (getCellType).  Triangles come from a table built here: walking a
+// cell's boundary counter-clockwise from node 0 lists its inside nodes and
+// crossings in order, which is the inside polygon (convex: a square less
+// corner triangles); a separated diagonal type gives each inside node its
+// own triangle with the crossings beside it, every other type one polygon
+// fanned from the first element of the walk, so triangles are
+// counter-clockwise.  Table entries: < 4 an edge ord, 4 + k node ord k (the
+// reference's encoding, Contourer.cpp:59-71).  Mesh export is off the step,
+// so this runs on the host on a copy of the field.
+struct FilledTable2 {
+	int8_t e[18][12];
+	uint8_t n[18];   // entries (three per triangle)
+};
+
+static void buildFilledTable2(FilledTable2 &T)
+{
+	const int ring[4] = {0, 1, 3, 2};       // nodes counter-clockwise
+	const int side[4] = {0, 3, 1, 2};       // the edge from ring[q] to ring[q + 1]
+	for (int t = 0; t < 18; t++) {
+		const int base = t < 16 ? t : (t == 16 ? 6 : 9);
+		auto inside = [&](int k) { return (base >> k & 1) != 0; };
+		int seq[8], m = 0;
+		for (int q = 0; q < 4; q++) {
+			const int c = ring[q], nx = ring[(q + 1) % 4];
+			if (inside(c)) seq[m++] = 4 + c;
+			if (inside(c) != inside(nx)) seq[m++] = side[q];
+		}
+		int k = 0;
+		if ((base == 6 || base == 9) && t < 16) {
+			for (int p = 0; p < m; p++)
+				if (seq[p] >= 4) {
+					T.e[t][k++] = int8_t(seq[(p + m - 1) % m]);
+					T.e[t][k++] = int8_t(seq[p]);
+					T.e[t][k++] = int8_t(seq[(p + 1) % m]);
+				}
+		} else {
+			for (int q = 1; q + 1 < m; q++) {
+				T.e[t][k++] = int8_t(seq[0]);
+				T.e[t][k++] = int8_t(seq[q]);
+				T.e[t][k++] = int8_t(seq[q + 1]);
+			}
+		}
+		T.n[t] = uint8_t(k);
+	}
+}
+
+static int contourFilled2(const Fields &F, const double *field, std::vector<double> &pos, std::vector<unsigned> &idx, hipStream_t s,
+	std::string &err)
+{
+	static FilledTable2 T;
+	static std::once_flag once;
+	std::call_once(once, [] { buildFilledTable2(T); });
+	const int nx = F.cell.n[0], ny = F.cell.n[1];
+	std::vector<double> f(size_t(nx) * ny);
+	if (hipMemcpyAsync(f.data(), field, f.size() * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+		hipStreamSynchronize(s) != hipSuccess) {
+		err = "reading the field failed";
+		return GARM_EHIP;
+	}
+	pos.clear();
+	idx.clear();
+	const double *o = F.cell.o, dx = F.cell.dx;
+	auto at = [&](int i, int j) { return f[size_t(j) * nx + i]; };
+	std::vector<int> nodeMark(f.size(), -1), mark[2];
+	mark[0].assign(size_t(nx - 1 > 0 ? nx - 1 : 0) * ny, -1);   // x edges: (nx - 1) x ny
+	mark[1].assign(size_t(nx) * (ny - 1 > 0 ? ny - 1 : 0), -1); // y edges: nx x (ny - 1)
+	for (int j = 0; j < ny; j++)
+		for (int i = 0; i < nx; i++)
+			if (at(i, j) <= 0) {
+				nodeMark[size_t(j) * nx + i] = int(pos.size() / 2);
+				pos.push_back(o[0] + i * dx);
+				pos.push_back(o[1] + j * dx);
+			}
+	for (int j = 0; j + 1 < ny; j++)
+		for (int i = 0; i + 1 < nx; i++) {
+			const double v[4] = {at(i, j), at(i + 1, j), at(i, j + 1), at(i + 1, j + 1)};
+			int type = 0;
+			for (int k = 0; k < 4; k++)
+				if (v[k] <= 0) type |= 1 << k;
+			if (type == 6 || type == 9) {
+				const double centre = (v[0] + v[1] + v[2] + v[3]) / 4;
+				if (centre <= 0) type = 16 + (type > 8);
+			}
+			int eid[4];
+			for (int e = 0; e < 4; e++) {
+				const int axis = e >> 1;
+				const int ei = i + (axis == 1 ? (e & 1) : 0), ej = j + (axis == 0 ? (e & 1) : 0);
+				const int fi = ei + (axis == 0), fj = ej + (axis == 1);
+				const double f0 = at(ei, ej), f1 = at(fi, fj);
+				int &mk = mark[axis][axis == 0 ? size_t(ej) * (nx - 1) + ei : size_t(ej) * nx + ei];
+				if ((f0 <= 0) != (f1 <= 0) && mk < 0) {
+					const double theta = (f0 - 0) / (f0 - f1);
+					mk = int(pos.size() / 2);
+					pos.push_back((1 - theta) * (o[0] + ei * dx) + theta * (o[0] + fi * dx));
+					pos.push_back((1 - theta) * (o[1] + ej * dx) + theta * (o[1] + fj * dx));
+				}
+				eid[e] = mk;
+			}
+			for (int q = 0; q < T.n[type]; q++) {
+				const int c = T.e[type][q];
+				if (c < 4) idx.push_back(unsigned(eid[c]));
+				else {
+					const int k = c & 3;
+					idx.push_back(unsigned(nodeMark[size_t(j + (k >> 1)) * nx + i + (k & 1)]));
+				}
+			}
+		}
+	return GARM_OK;
+}
+
 int contourMesh(const Fields &F, const double *field, std::vector<double> &pos, std::vector<float> &nrm, std::vector<unsigned> &idx,
 	hipStream_t s, std::string &err)
 {
-	if (F.D != 3) { err = "the surface mesh is built for 3D grids"; return GARM_EINVAL; }
+	if (F.D == 2) {
+		nrm.clear();
+		return contourFilled2(F, field, pos, idx, s, err);
+	}
 	int rc = ensureTable(err);
 	if (rc) return rc;
 	McGrid G{};

@@ -304,14 +304,14 @@ class Liquid:
         return st.as_dict()
 
     def contour(self):
-        """garm_contour of phi_tmp: (positions (nv, 3) float32, normals
-        (nv, 3) float32, triangles (nt, 3) uint32)."""
+        """garm_contour of phi_tmp: (positions (nv, dim) float32, normals
+        (nv, 3) float32 (3D; empty in 2D), triangles (nt, 3) uint32)."""
         nv, ni = ctypes.c_size_t(), ctypes.c_size_t()
         rc = lib().garm_contour(self.h, None, None, 0, None, 0, ctypes.byref(nv), ctypes.byref(ni))
         if rc not in (GARM_OK, GARM_EINVAL) or (rc == GARM_EINVAL and nv.value == 0 and ni.value == 0):
             self._check(rc)
-        pos = np.zeros((nv.value, 3), np.float32)
-        nrm = np.zeros((nv.value, 3), np.float32)
+        pos = np.zeros((nv.value, self.dim), np.float32)
+        nrm = np.zeros((nv.value if self.dim == 3 else 0, 3), np.float32)
         idx = np.zeros(ni.value, np.uint32)
         self._check(lib().garm_contour(self.h, pos.ctypes.data, nrm.ctypes.data, nv.value, idx.ctypes.data, ni.value,
                                        ctypes.byref(nv), ctypes.byref(ni)))
@@ -499,16 +499,19 @@ class Unet:
         return hist[:steps]
 
 
-def read_mesh(path):
-    """A liquid.mesh file (SurfaceMesh::write, SurfaceMesh.cpp:103-124, 3D):
-    (positions (nv, 3), normals (nv, 3), triangles (nt, 3))."""
+def read_mesh(path, dim=3):
+    """A liquid.mesh file (SurfaceMesh::write, SurfaceMesh.cpp:103-124):
+    (positions (nv, dim), normals (nv, 3) (empty in 2D), triangles (nt, 3))."""
     raw = open(path, "rb").read()
     nv = int(np.frombuffer(raw, np.uint32, 1, 0)[0])
     off = 4
-    pos = np.frombuffer(raw, np.float32, nv * 3, off).reshape(nv, 3)
-    off += nv * 12
-    nrm = np.frombuffer(raw, np.float32, nv * 3, off).reshape(nv, 3)
-    off += nv * 12
+    pos = np.frombuffer(raw, np.float32, nv * dim, off).reshape(nv, dim)
+    off += nv * 4 * dim
+    if dim == 3:
+        nrm = np.frombuffer(raw, np.float32, nv * 3, off).reshape(nv, 3)
+        off += nv * 12
+    else:
+        nrm = np.zeros((0, 3), np.float32)
     ni = int(np.frombuffer(raw, np.uint32, 1, off)[0])
     off += 4
     idx = np.frombuffer(raw, np.uint32, ni, off)

@@ -303,19 +303,24 @@ int garm_load_frame(garm_ctx *ctx, const char *frame_dir);
  * edge order) and positions, each cube's triangles from this library's
  * table (crossing edges paired per cube face, an ambiguous face cutting off
  * each inside corner; watertight), oriented from the liquid to the air.
- * positions and normals: 3 floats per vertex; indices: 3 per triangle.
- * The counts are always set; the arrays are filled when the capacities (in
- * vertices and in indices) suffice, else GARM_EINVAL.  3D contexts only. */
+ * 2D contexts: Contourer<2, true>::contour, the filled contour of the
+ * reference's 2D frames (inside nodes first, then edge crossings; cell types
+ * 6 and 9 joined when the cell mean is <= 0), on the host from a copy of
+ * _tmpLevelSet, its triangles (counter-clockwise) from this library's table;
+ * no normals.  positions: D floats per vertex; normals: 3 per vertex (3D
+ * only, may be NULL in 2D); indices: 3 per triangle.  The counts are always
+ * set; the arrays are filled when the capacities (in vertices and in
+ * indices) suffice, else GARM_EINVAL. */
 int garm_contour(garm_ctx *ctx, float *positions, float *normals, size_t vertex_capacity, unsigned *indices, size_t index_capacity,
 	size_t *n_vertices, size_t *n_indices);
 /* The same mesh written as the reference's frame files (LevelSetLiquid.cpp:
  * 306-347 without refinement; either path may be NULL): mesh_path in
  * SurfaceMesh::write's binary layout (SurfaceMesh.cpp:103-124, liquid.mesh:
- * uint32 vertex count, float positions (x, y, z), float normals, uint32
- * index count, uint32 indices), obj_path as SurfaceMesh::writeOBJ
- * (SurfaceMesh.cpp:126-144, render/N.obj: "v" lines of the FP64 positions
- * printed %.15e, no "vn" lines -- the contoured mesh's normals member is
- * empty -- and 1-based "f" lines). */
+ * uint32 vertex count, float positions (D per vertex), float normals (3D
+ * only), uint32 index count, uint32 indices), obj_path as
+ * SurfaceMesh::writeOBJ (SurfaceMesh.cpp:126-144, render/N.obj: "v" lines of
+ * the FP64 positions printed %.15e (z = 0 in 2D), no "vn" lines -- the
+ * contoured mesh's normals member is empty -- and 1-based "f" lines). */
 int garm_write_mesh(garm_ctx *ctx, const char *mesh_path, const char *obj_path);
 
 /* Sets the face velocity of a 3D context to an analytic field at time t

@@ -1702,6 +1702,79 @@ struct MeshOut {
 	std::vector<unsigned> idx;
 };
 
+// Contourer<2, true>::contour at 0 (Contourer.cpp:9-97, getCellType's
+// diagonal rule :85-95): inside nodes in node order, then edge crossings in
+// cell order and edge order (2D cell edge ord e: axis e >> 1, offset e & 1
+// along the other axis, StaggeredGrid.h:87-94).  A cell's triangles: its
+// boundary walked counter-clockwise from node 0 (nodes 0, 1, 3, 2 and the
+// edges 0, 3, 1, 2 between them) lists the inside nodes and crossings of
+// the inside polygon; the separated diagonal types (6, 9 with the cell
+// mean > 0) give each inside node a triangle with the crossings beside it,
+// every other type one polygon fanned from the walk's first element.
+static void contour2(const int *n, const double *o, double dx, const double *f, MeshOut &M)
+{
+	const int nx = n[0], ny = n[1];
+	M.pos.clear(); M.nrm.clear(); M.idx.clear();
+	auto at = [&](int i, int j) { return f[size_t(j) * nx + i]; };
+	std::vector<int> node(size_t(nx) * ny, -1);
+	std::vector<int> ex(size_t(std::max(nx - 1, 0)) * ny, -1), ey(size_t(nx) * std::max(ny - 1, 0), -1);
+	for (int j = 0; j < ny; j++)
+		for (int i = 0; i < nx; i++)
+			if (at(i, j) <= 0) {
+				node[size_t(j) * nx + i] = int(M.pos.size() / 2);
+				M.pos.push_back(o[0] + i * dx);
+				M.pos.push_back(o[1] + j * dx);
+			}
+	auto crossing = [&](int i0, int j0, int i1, int j1, int &mk) {
+		const double f0 = at(i0, j0), f1 = at(i1, j1);
+		if ((f0 <= 0) != (f1 <= 0) && mk < 0) {
+			const double theta = (f0 - 0) / (f0 - f1);
+			mk = int(M.pos.size() / 2);
+			M.pos.push_back((1 - theta) * (o[0] + i0 * dx) + theta * (o[0] + i1 * dx));
+			M.pos.push_back((1 - theta) * (o[1] + j0 * dx) + theta * (o[1] + j1 * dx));
+		}
+		return mk;
+	};
+	for (int j = 0; j + 1 < ny; j++)
+		for (int i = 0; i + 1 < nx; i++) {
+			// nodes counter-clockwise: (i,j), (i+1,j), (i+1,j+1), (i,j+1)
+			const int ci[4] = {i, i + 1, i + 1, i}, cj[4] = {j, j, j + 1, j + 1};
+			bool in[4];
+			for (int q = 0; q < 4; q++) in[q] = at(ci[q], cj[q]) <= 0;
+			// edges in the reference's order 0..3 first (vertex creation order)
+			int e[4];
+			e[0] = crossing(i, j, i + 1, j, ex[size_t(j) * (nx - 1) + i]);
+			e[1] = crossing(i, j + 1, i + 1, j + 1, ex[size_t(j + 1) * (nx - 1) + i]);
+			e[2] = crossing(i, j, i, j + 1, ey[size_t(j) * nx + i]);
+			e[3] = crossing(i + 1, j, i + 1, j + 1, ey[size_t(j) * nx + i + 1]);
+			// the walk: node q, then the edge to node q + 1
+			const int sideEdge[4] = {e[0], e[3], e[1], e[2]};
+			std::vector<int> seq;
+			std::vector<bool> isNode;
+			for (int q = 0; q < 4; q++) {
+				if (in[q]) { seq.push_back(node[size_t(cj[q]) * nx + ci[q]]); isNode.push_back(true); }
+				if (in[q] != in[(q + 1) % 4]) { seq.push_back(sideEdge[q]); isNode.push_back(false); }
+			}
+			const bool diag = (in[0] && in[2] && !in[1] && !in[3]) || (in[1] && in[3] && !in[0] && !in[2]);
+			const double centre = (at(i, j) + at(i + 1, j) + at(i, j + 1) + at(i + 1, j + 1)) / 4;
+			const int m = int(seq.size());
+			if (diag && !(centre <= 0)) {
+				for (int p = 0; p < m; p++)
+					if (isNode[p]) {
+						M.idx.push_back(unsigned(seq[(p + m - 1) % m]));
+						M.idx.push_back(unsigned(seq[p]));
+						M.idx.push_back(unsigned(seq[(p + 1) % m]));
+					}
+			} else {
+				for (int q = 1; q + 1 < m; q++) {
+					M.idx.push_back(unsigned(seq[0]));
+					M.idx.push_back(unsigned(seq[q]));
+					M.idx.push_back(unsigned(seq[q + 1]));
+				}
+			}
+		}
+}
+
 static void contour3(const int *n, const double *o, double dx, const double *f, MeshOut &M)
 {
 	const McPolys &P = mcPolys();
@@ -2020,6 +2093,21 @@ int ora_contour3(const int *n, const double *o, double dx, const double *f, floa
 	if (*nv > vcap || *ni > icap) return -1;
 	for (size_t i = 0; i < M.pos.size(); i++) { pos[i] = float(M.pos[i]); nrm[i] = float(M.nrm[i]); }
 	if (pos64) std::copy(M.pos.begin(), M.pos.end(), pos64);
+	std::copy(M.idx.begin(), M.idx.end(), idx);
+	return 0;
+}
+// Contourer<2, true>::contour of a node field (n[0] x n[1], x fastest):
+// FP64 positions (2 per vertex) and uint indices; counts always, arrays when
+// the capacities suffice (returns -1 otherwise).
+int ora_contour2(const int *n, const double *o, double dx, const double *f, double *pos, size_t vcap, unsigned *idx, size_t icap,
+	size_t *nv, size_t *ni)
+{
+	ora::MeshOut M;
+	ora::contour2(n, o, dx, f, M);
+	*nv = M.pos.size() / 2;
+	*ni = M.idx.size();
+	if (*nv > vcap || *ni > icap) return -1;
+	std::copy(M.pos.begin(), M.pos.end(), pos);
 	std::copy(M.idx.begin(), M.idx.end(), idx);
 	return 0;
 }

@@ -69,6 +69,7 @@ def lib():
             "ora_capillary": (i, [vp, d, d, d, d, i, i]),
             "ora_contour3": (i, [ip, dp, d, dp, vp, vp, sz, vp, sz, ctypes.POINTER(sz), ctypes.POINTER(sz), vp]),
             "ora_mc_table": (i, [ip, ip]),
+            "ora_contour2": (i, [ip, dp, d, dp, vp, sz, vp, sz, ctypes.POINTER(sz), ctypes.POINTER(sz)]),
             "ora_curvature": (d, [vp, dp]),
             "ora_little_drop": (None, [vp, d]),
             "ora_get_elapsed_time": (d, [vp]),
@@ -296,3 +297,17 @@ def mc_table():
     tri = np.zeros((256, 24), np.int32)
     assert lib().ora_mc_table(ntri.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), tri.ctypes.data_as(ctypes.POINTER(ctypes.c_int))) == 0
     return ntri, tri
+
+
+def contour2(field, origin, dx):
+    """Contourer<2, true>::contour at 0 of a node field of shape (ny, nx):
+    (positions (nv, 2) float64, triangles (nt, 3) uint32)."""
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    n = (ctypes.c_int * 3)(f.shape[1], f.shape[0], 1)
+    nv, ni = ctypes.c_size_t(), ctypes.c_size_t()
+    args = (n, _d3(origin), dx, f.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    lib().ora_contour2(*args, None, 0, None, 0, ctypes.byref(nv), ctypes.byref(ni))
+    pos = np.zeros((nv.value, 2), np.float64)
+    idx = np.zeros(ni.value, np.uint32)
+    assert lib().ora_contour2(*args, pos.ctypes.data, nv.value, idx.ctypes.data, ni.value, ctypes.byref(nv), ctypes.byref(ni)) == 0
+    return pos, idx.reshape(-1, 3)

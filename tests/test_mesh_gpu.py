@@ -5,7 +5,8 @@ the liquid.mesh file read back equal to the in-memory mesh, and the OBJ
 file equal, character for character, to the oracle's FP64 positions printed
 as SurfaceMesh::writeOBJ prints them.  Fields: a
 random one (every cube type occurs), a sphere, and a dam break's phi_tmp
-after a few steps at 64^3.
+after a few steps at 64^3; in 2D the filled contour on a random field and a
+dam break.
 """
 import os
 
@@ -80,8 +81,35 @@ def test_dam_break_phi_tmp():
     _compare(g, d, d.get(capi.PHI_TMP))
 
 
-def test_two_dimensional_context_is_refused():
-    g = scenes.grid(16, 2)
+@pytest.mark.parametrize("n", [24, 64])
+def test_filled_contour_2d(n, tmp_path):
+    """2D: the filled contour (Contourer<2, true>), from the host copy of
+    phi_tmp, against the oracle's, positions and indices identical; the
+    liquid.mesh (float2 positions, no normals) and OBJ files."""
+    g = scenes.grid(n, 2)
     d = make_gpu(g)
-    with pytest.raises(capi.GarmError):
-        d.contour()
+    if n == 24:
+        rng = np.random.default_rng(5)
+        f = rng.choice([-1.0, 1.0], size=int(np.prod(g["res"]))) * rng.uniform(0.2, 2.0, size=int(np.prod(g["res"])))
+        d.set(capi.PHI_TMP, f)
+    else:
+        scenes.dam_break(d, g)
+        d.initialize(capi.step_opts(tol=1e-8, max_iterations=20000))
+        clk = FrameClock(g["dx"], 1.0)
+        opts = capi.step_opts(reinit_interval=2, tol=1e-6, max_iterations=20000)
+        for s in range(10):
+            d.step(clk.next_dt(d.max_speed()), s, opts)
+        f = d.get(capi.PHI_TMP)
+    say(f"filled contour 2d n={n}")
+    pos, nrm, tri = d.contour()
+    o = [g["origin"][a] + g["dx"] / 2 for a in range(2)]
+    opos, otri = O.contour2(f.reshape(g["res"][::-1]), o, g["dx"])
+    say(f"  {len(pos)} vertices, {len(tri)} triangles")
+    assert nrm.shape == (0, 3)
+    assert np.array_equal(pos, opos.astype(np.float32)) and np.array_equal(tri, otri)
+    path, obj = os.path.join(tmp_path, "liquid.mesh"), os.path.join(tmp_path, "0.obj")
+    d.write_mesh(path, obj)
+    fp, _, ft = capi.read_mesh(path, dim=2)
+    assert np.array_equal(fp, pos) and np.array_equal(ft, tri)
+    want = "".join("v %.15e %.15e 0\n" % tuple(p) for p in opos) + "".join("f %d %d %d\n" % tuple(t + 1) for t in otri)
+    assert open(obj).read() == want

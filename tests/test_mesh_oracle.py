@@ -174,3 +174,70 @@ def test_table_uses_every_crossing_edge_and_points_out():
             assert N @ R > 0, (t, poly)
         assert sorted(seen) == sorted(cross)   # each crossing edge in one polygon
     assert ntri.max() <= 8
+
+
+# ------------------------------------------------------------------ 2D ----
+# Contourer<2, true>::contour, the filled contour of 2D frames.
+
+def _brute_vertices2(f, o, dx):
+    ny, nx = f.shape
+    out = [[o[0] + i * dx, o[1] + j * dx] for j in range(ny) for i in range(nx) if f[j, i] <= 0]
+    seen = set()
+    for j in range(ny - 1):
+        for i in range(nx - 1):
+            for e in range(4):
+                axis = e >> 1
+                p0 = [i + (e & 1 if axis == 1 else 0), j + (e & 1 if axis == 0 else 0)]
+                p1 = [p0[0] + (axis == 0), p0[1] + (axis == 1)]
+                f0, f1 = f[p0[1], p0[0]], f[p1[1], p1[0]]
+                if (f0 <= 0) == (f1 <= 0) or (axis, *p0) in seen:
+                    continue
+                seen.add((axis, *p0))
+                th = (f0 - 0) / (f0 - f1)
+                out.append([(1 - th) * (o[q] + p0[q] * dx) + th * (o[q] + p1[q] * dx) for q in range(2)])
+    return np.array(out)
+
+
+def test_filled_contour_vertex_numbering_by_brute_force():
+    rng = np.random.default_rng(4)
+    f = rng.normal(size=(7, 9))
+    pos, tri = O.contour2(f, [0.1, -0.3], 0.5)
+    assert np.array_equal(pos, _brute_vertices2(f, [0.1, -0.3], 0.5))
+    assert tri.max() < len(pos)
+
+
+def _signed_areas(pos, tri):
+    p = pos[tri]
+    return 0.5 * ((p[:, 1, 0] - p[:, 0, 0]) * (p[:, 2, 1] - p[:, 0, 1]) - (p[:, 1, 1] - p[:, 0, 1]) * (p[:, 2, 0] - p[:, 0, 0]))
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_filled_contour_area_per_cell_and_orientation(seed):
+    """Every triangle counter-clockwise, and the triangles of each cell
+    cover the filled area of that cell (the 2D volume's per-cell area,
+    itself pinned by half-planes and a disk) -- random signs, so every cell
+    type and both diagonal rules occur."""
+    rng = np.random.default_rng(seed)
+    f = rng.choice([-1.0, 1.0], size=(12, 14)) * rng.uniform(0.2, 2.0, size=(12, 14))
+    pos, tri = O.contour2(f, [0, 0], 1.0)
+    area = _signed_areas(pos, tri)
+    assert np.all(area > 0)
+    want = sum(O.filled_area([f[j, i], f[j, i + 1], f[j + 1, i], f[j + 1, i + 1]]) for j in range(11) for i in range(13))
+    assert area.sum() == pytest.approx(want, rel=1e-12)
+
+
+def test_filled_disk():
+    n = 81
+    ax = np.arange(n) / (n - 1)
+    y, x = np.meshgrid(ax, ax, indexing="ij")
+    R = 0.3
+    f = np.hypot(x - 0.5, y - 0.47) - R
+    pos, tri = O.contour2(f, [0, 0], 1.0 / (n - 1))
+    area = _signed_areas(pos, tri)
+    assert np.all(area > 0)
+    assert area.sum() == pytest.approx(np.pi * R * R, rel=2e-3)
+    # a disk's triangulation: Euler characteristic 1, boundary edges once
+    d = np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]])
+    und, cnt = np.unique(np.sort(d, axis=1), axis=0, return_counts=True)
+    assert np.all(cnt <= 2)
+    assert len(pos) - len(und) + len(tri) == 1
