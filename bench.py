@@ -50,10 +50,10 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0   # dense BF16 MFMA
 # HBM bytes per launch of the dominant kernel at 256^3 from two rocprofv3 --pmc passes
 # (FETCH_SIZE, WRITE_SIZE, scaled by a copy kernel's reading):
 # profiles/dam_break_256_reinit_stage_traffic.md.  Used when --traffic is not given.
-MEASURED_TRAFFIC = {("reinit_stage", 256): 2.512e8}
+MEASURED_TRAFFIC = {("reinit_stage", 256): 2.824e8}
 # Vector-ALU issue of the same kernel (SQ_INSTS_VALU against two wave64
 # instructions per CU-cycle): profiles/dam_break_256_valu_issue.md.
-MEASURED_VALU_ISSUE = {("reinit_stage", 256): 0.62}
+MEASURED_VALU_ISSUE = {("reinit_stage", 256): 0.57}
 
 
 def log(*a):
