@@ -256,8 +256,9 @@ __global__ __launch_bounds__(kColX * kColY) void kDirProductCol(BoxSys<T> O, int
 	const T beta = T(b[0]);
 	__shared__ T plane[kColY + 2][kColX + 2];
 	const int tx = int(threadIdx.x), ty = int(threadIdx.y);
-	const int i = int(blockIdx.x) * kColX + tx, j = int(blockIdx.y) * kColY + ty;
-	const int k0 = int(blockIdx.z) * zch, k1 = min(bz, k0 + zch);
+	const Tile3 tile = xcdTiles();
+	const int i = tile.x * kColX + tx, j = tile.y * kColY + ty;
+	const int k0 = tile.z * zch, k1 = min(bz, k0 + zch);
 	const bool inside = i < bx && j < by;
 	const int sy = O.sy, sz = O.sz;
 	auto dirAt = [&](int ii, int jj, int kk) -> T {
@@ -304,7 +305,7 @@ __global__ __launch_bounds__(kColX * kColY) void kDirProductCol(BoxSys<T> O, int
 	if (tid == 0) {
 		double tot = 0;
 		for (int q2 = 0; q2 < kColX * kColY / kWave; q2++) tot += red[q2];
-		pa[flat] = tot;
+		pa[tile.x + int(gridDim.x) * (tile.y + int(gridDim.y) * tile.z)] = tot;   // the tile's slot: the sum order of the partials is the launch's
 	}
 }
 

@@ -201,4 +201,21 @@ __device__ __forceinline__ int blockExclusiveScan(int v, int *total)
 
 __device__ __forceinline__ int flatBlock() { return int(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)); }
 
+// Workgroups are handed to the eight XCDs round-robin in launch order (x
+// fastest), so neighbouring tiles of a plain launch land on different XCDs,
+// each with its own L2, and the halo they share is fetched once per XCD.
+// xcdTiles renumbers a launch of nx x ny x nz tiles so that XCD x takes one
+// contiguous run of tiles in x-fastest order (runs differ in length by at
+// most one): neighbours along x and y, and along z inside a run, share an L2.
+struct Tile3 { int x, y, z; };
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ Tile3 xcdTiles()
+{
+	const unsigned nx = gridDim.x, ny = gridDim.y, total = nx * ny * gridDim.z;
+	const unsigned b = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+	const unsigned q = total / kXcds, r = total % kXcds, x = b % kXcds, k = b / kXcds;
+	const unsigned t = x * q + min(x, r) + k;
+	return Tile3{int(t % nx), int((t / nx) % ny), int(t / (nx * ny))};
+}
+
 } // namespace garm

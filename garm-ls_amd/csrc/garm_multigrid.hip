@@ -38,7 +38,7 @@ struct MgLevel {
 };
 
 // sweep tiles: large levels, small levels, 2D
-constexpr int kTileL[3] = {32, 16, 4}, kTileS[3] = {16, 8, 4}, kTile2[3] = {32, 32, 1};
+constexpr int kTileL[3] = {32, 8, 8}, kTileS[3] = {16, 8, 4}, kTile2[3] = {32, 32, 1};
 
 static MgLevel level(const Multigrid &M, int l)
 {
@@ -201,7 +201,8 @@ template <int D, int TX, int TY, int TZ, int NTH>
 __global__ __launch_bounds__(NTH) void kMgSweep(MgLevel L, const float *xin, float *xout, int first, const uint8_t *active)
 {
 	// a tile without active cells keeps x = 0 in both buffers: nothing to do
-	if (!active[(size_t(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x]) return;
+	const Tile3 tile = xcdTiles();
+	if (!active[(size_t(tile.z) * gridDim.y + tile.y) * gridDim.x + tile.x]) return;
 	static_assert(TX % 2 == 0 && (D == 3 || TZ == 1), "tile shape");
 	constexpr int HZ = D == 3 ? 2 : 0;                    // halo of the staged tile along z
 	constexpr int SX = TX + 4, SY = TY + 4, SZ = TZ + 2 * HZ, SXY = SX * SY, NS = SXY * SZ;
@@ -209,7 +210,7 @@ __global__ __launch_bounds__(NTH) void kMgSweep(MgLevel L, const float *xin, flo
 	constexpr int P0 = (NS + NTH - 1) / NTH, P1 = (NR + NTH - 1) / NTH, P2 = (NT + NTH - 1) / NTH;
 	constexpr int kNone = SXY + SX + 1;                    // a tile index whose neighbours are all in the tile
 	__shared__ float xs[NS];
-	const int x0 = int(blockIdx.x) * TX, y0 = int(blockIdx.y) * TY, z0 = int(blockIdx.z) * TZ;
+	const int x0 = tile.x * TX, y0 = tile.y * TY, z0 = tile.z * TZ;
 	const int n0 = L.n[0], n1 = L.n[1], n2 = L.n[2];
 	const unsigned last = unsigned(n0) * unsigned(n1) * unsigned(n2) - 1;
 	auto inBox = [&](int i, int j, int k) { return i >= 0 && i < n0 && j >= 0 && j < n1 && k >= 0 && k < n2; };

@@ -1,5 +1,5 @@
 """The multigrid V-cycle at a size where level 0 holds more than 2^20 cells,
-so its sweeps run on the large-tile kernel (32 x 16 x 4 cells per
+so its sweeps run on the large-tile kernel (32 x 8 x 8 cells per
 workgroup) that the small parity cases never reach.  A 128^3 dam break
 whose liquid spans the domain in x and z (level 0: 128 x 77 x 128 cells):
 flexible CG with the V-cycle reaches a 1e-12 FP64 residual and the pressure
