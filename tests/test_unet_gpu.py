@@ -5,7 +5,10 @@ rounding points (oracle q=bf16; the remaining differences are FP32
 accumulation order and the rare BF16 rounding that flips with it), and a
 looser one against the exact FP64 network, which bounds the BF16 error
 itself.  Grid sizes are deliberately unequal and not multiples of the
-16 x 4 x 4 convolution tile so that partial tiles and axis order are tested.
+convolution tiles (16 x 4 x 4 voxels for 32 and more output channels,
+16 x 8 x 8 for 16) so that partial tiles and axis order are tested; the
+larger shape holds whole 16 x 8 x 8 tiles beside partial ones on the finest
+level.
 """
 import numpy as np
 import pytest
@@ -17,6 +20,7 @@ from helpers import say
 pytestmark = pytest.mark.gpu
 
 SHAPE = (12, 8, 20)   # (nz, ny, nx)
+LARGE = (24, 20, 36)
 
 
 def _rel_l2(a, b):
@@ -57,10 +61,10 @@ def test_param_layout_round_trip():
     assert np.array_equal(net.get_params(), flat)
 
 
-@pytest.mark.parametrize("seed", [11, 12])
-def test_forward_matches_oracle(seed):
-    say(f"unet forward seed={seed} shape={SHAPE}")
-    _, r, mask = _case(seed)
+@pytest.mark.parametrize("seed,shape", [(11, SHAPE), (12, SHAPE), (13, LARGE)])
+def test_forward_matches_oracle(seed, shape):
+    say(f"unet forward seed={seed} shape={shape}")
+    _, r, mask = _case(seed, shape)
     P = _params(seed)
     net = capi.Unet(seed=0)
     net.set_params(UO.flatten(P))
@@ -75,10 +79,11 @@ def test_forward_matches_oracle(seed):
     assert e_e <= 4e-2
 
 
-def test_loss_gradient_matches_oracle():
-    say(f"unet loss gradient shape={SHAPE}")
-    rng, r, mask = _case(21)
-    target = (rng.normal(size=SHAPE) * mask).astype(np.float32)
+@pytest.mark.parametrize("shape", [SHAPE, LARGE])
+def test_loss_gradient_matches_oracle(shape):
+    say(f"unet loss gradient shape={shape}")
+    rng, r, mask = _case(21, shape)
+    target = (rng.normal(size=shape) * mask).astype(np.float32)
     P = _params(21)
     net = capi.Unet(seed=0)
     net.set_params(UO.flatten(P))
