@@ -34,7 +34,9 @@ constexpr int kMgCoarsestSweeps = 12;
 
 struct MgLevel {
 	int n[3];
-	float *diag, *wx, *wy, *wz, *b, *x;
+	float *diag;
+	mgw_t *wx, *wy, *wz;
+	float *b, *x;
 };
 
 // sweep tiles: large levels, small levels, 2D
@@ -87,9 +89,9 @@ __global__ __launch_bounds__(kBlock) void kMgBuild0(Rows<float> A, int n, const 
 		const unsigned c = boxCell(B, L, mat2grid[r]);
 		boxOf[r] = c;
 		L.diag[c] = A.diag[r];
-		L.wx[c] = A.nb[0][r] >= 0 ? A.w[0][r] : 0.0f;
-		L.wy[c] = A.nb[2][r] >= 0 ? A.w[2][r] : 0.0f;
-		if constexpr (D == 3) L.wz[c] = A.nb[4][r] >= 0 ? A.w[4][r] : 0.0f;
+		L.wx[c] = mgw_t(A.nb[0][r] >= 0 ? A.w[0][r] : 0.0f);
+		L.wy[c] = mgw_t(A.nb[2][r] >= 0 ? A.w[2][r] : 0.0f);
+		if constexpr (D == 3) L.wz[c] = mgw_t(A.nb[4][r] >= 0 ? A.w[4][r] : 0.0f);
 	}
 }
 
@@ -113,14 +115,15 @@ __global__ __launch_bounds__(kBlock) void kMgCoarsen(MgLevel F, MgLevel C)
 					if (i >= F.n[0] || j >= F.n[1] || k >= F.n[2]) continue;
 					const unsigned f = cidx(F, i, j, k);
 					d += F.diag[f];
-					if (cx) inner += F.wx[f]; else sx += F.wx[f];
-					if (cy) inner += F.wy[f]; else sy += F.wy[f];
-					if constexpr (D == 3) { if (cz) inner += F.wz[f]; else sz += F.wz[f]; }
+					const float wx = F.wx[f], wy = F.wy[f];
+					if (cx) inner += wx; else sx += wx;
+					if (cy) inner += wy; else sy += wy;
+					if constexpr (D == 3) { const float wz = F.wz[f]; if (cz) inner += wz; else sz += wz; }
 				}
 		C.diag[q] = d - 2 * inner;
-		C.wx[q] = sx;
-		C.wy[q] = sy;
-		if constexpr (D == 3) C.wz[q] = sz;
+		C.wx[q] = mgw_t(sx);
+		C.wy[q] = mgw_t(sy);
+		if constexpr (D == 3) C.wz[q] = mgw_t(sz);
 	}
 }
 
@@ -130,13 +133,13 @@ __device__ __forceinline__ float offSum(const MgLevel &L, unsigned c, int i, int
 {
 	const unsigned sy = unsigned(L.n[0]), sz = unsigned(L.n[0]) * unsigned(L.n[1]);
 	float s = 0;
-	if (i > 0) s += L.wx[c] * L.x[c - 1];
-	if (i + 1 < L.n[0]) s += L.wx[c + 1] * L.x[c + 1];
-	if (j > 0) s += L.wy[c] * L.x[c - sy];
-	if (j + 1 < L.n[1]) s += L.wy[c + sy] * L.x[c + sy];
+	if (i > 0) s += float(L.wx[c]) * L.x[c - 1];
+	if (i + 1 < L.n[0]) s += float(L.wx[c + 1]) * L.x[c + 1];
+	if (j > 0) s += float(L.wy[c]) * L.x[c - sy];
+	if (j + 1 < L.n[1]) s += float(L.wy[c + sy]) * L.x[c + sy];
 	if constexpr (D == 3) {
-		if (k > 0) s += L.wz[c] * L.x[c - sz];
-		if (k + 1 < L.n[2]) s += L.wz[c + sz] * L.x[c + sz];
+		if (k > 0) s += float(L.wz[c]) * L.x[c - sz];
+		if (k + 1 < L.n[2]) s += float(L.wz[c + sz]) * L.x[c + sz];
 	}
 	return s;
 }
@@ -157,8 +160,8 @@ __device__ __forceinline__ MgRow loadRow(const MgLevel &L, unsigned c, unsigned 
 	r.w[1] = L.wx[min(c + 1, last)];
 	r.w[2] = L.wy[c];
 	r.w[3] = L.wy[min(c + sy, last)];
-	r.w[4] = D == 3 ? L.wz[c] : 0.0f;
-	r.w[5] = D == 3 ? L.wz[min(c + sz, last)] : 0.0f;
+	r.w[4] = D == 3 ? float(L.wz[c]) : 0.0f;
+	r.w[5] = D == 3 ? float(L.wz[min(c + sz, last)]) : 0.0f;
 	return r;
 }
 
@@ -387,7 +390,13 @@ __device__ __forceinline__ MgLevel ldsLevel(float *lds, const MgTail &T, int l)
 	for (int a = 0; a < 3; a++) V.n[a] = T.g[l].n[a];
 	const int c = mgCells(V), pad = mgPad(V), span = c + 2 * pad;
 	float *p = lds + T.off[l] + pad;
-	V.diag = p; V.wx = p + span; V.wy = p + 2 * span; V.wz = p + 3 * span; V.b = p + 4 * span; V.x = p + 5 * span;
+	// the weights keep float-sized slots in LDS (the layout's accounting)
+	V.diag = p;
+	V.wx = reinterpret_cast<mgw_t *>(p + span);
+	V.wy = reinterpret_cast<mgw_t *>(p + 2 * span);
+	V.wz = reinterpret_cast<mgw_t *>(p + 3 * span);
+	V.b = p + 4 * span;
+	V.x = p + 5 * span;
 	return V;
 }
 
@@ -396,13 +405,13 @@ __device__ __forceinline__ float offSumPadded(const MgLevel &L, int c)
 {
 	const int sy = L.n[0], sz = L.n[0] * L.n[1];
 	float s = 0;
-	s += L.wx[c] * L.x[c - 1];
-	s += L.wx[c + 1] * L.x[c + 1];
-	s += L.wy[c] * L.x[c - sy];
-	s += L.wy[c + sy] * L.x[c + sy];
+	s += float(L.wx[c]) * L.x[c - 1];
+	s += float(L.wx[c + 1]) * L.x[c + 1];
+	s += float(L.wy[c]) * L.x[c - sy];
+	s += float(L.wy[c + sy]) * L.x[c + sy];
 	if constexpr (D == 3) {
-		s += L.wz[c] * L.x[c - sz];
-		s += L.wz[c + sz] * L.x[c + sz];
+		s += float(L.wz[c]) * L.x[c - sz];
+		s += float(L.wz[c + sz]) * L.x[c + sz];
 	}
 	return s;
 }
@@ -432,9 +441,9 @@ __global__ __launch_bounds__(1024) void kMgTail(MgTail T)
 		for (int q = tid - pad; q < c + pad; q += blockDim.x) {
 			const bool in = q >= 0 && q < c;
 			V.diag[q] = in ? G.diag[q] : 0.0f;
-			V.wx[q] = in ? G.wx[q] : 0.0f;
-			V.wy[q] = in ? G.wy[q] : 0.0f;
-			V.wz[q] = in ? G.wz[q] : 0.0f;
+			V.wx[q] = in ? G.wx[q] : mgw_t(0);
+			V.wy[q] = in ? G.wy[q] : mgw_t(0);
+			V.wz[q] = in ? G.wz[q] : mgw_t(0);
 			V.b[q] = in && l == T.first ? G.b[q] : 0.0f;
 			V.x[q] = 0.0f;
 		}
@@ -606,10 +615,8 @@ bool Multigrid::layout(int dim, const int box_org[3], const int box_n[3], const 
 	D = dim;
 	for (int a = 0; a < 3; a++) { org[a] = box_org[a]; gn[a] = grid_n[a]; dims[0][a] = box_n[a]; }
 	levels = 1;
-	size_t total = 0;
 	for (int l = 0;; l++) {
 		cells[l] = size_t(dims[l][0]) * dims[l][1] * dims[l][2];
-		total += cells[l];
 		if (cells[l] <= size_t(kMgCoarsestCells) || l + 1 == kMgMaxLevels) break;
 		for (int a = 0; a < 3; a++) dims[l + 1][a] = (a < D) ? (dims[l][a] + 1) / 2 : 1;
 		levels++;
@@ -628,7 +635,11 @@ bool Multigrid::layout(int dim, const int box_org[3], const int box_n[3], const 
 	}
 	tail_words = 0;
 	for (int l = tail_first; l < levels; l++) tail_words += cells[l];
-	const size_t need = 7 * total + tail_words + 2 * kMgMaxLevels + size_t(unknowns) + 64;
+	// per level: diag, b, x, xalt in floats, the three weight arrays in halves
+	auto levelFloats = [&](int l) { return 4 * cells[l] + 3 * ((cells[l] + 1) / 2); };
+	level_floats = 0;
+	for (int l = 0; l < levels; l++) level_floats += levelFloats(l);
+	const size_t need = level_floats + tail_words + 2 * kMgMaxLevels + size_t(unknowns) + 64;
 	if (need > pool_cap) {
 		if (pool) (void)hipFree(pool);
 		pool = nullptr;
@@ -638,8 +649,10 @@ bool Multigrid::layout(int dim, const int box_org[3], const int box_n[3], const 
 	}
 	float *p = pool;
 	for (int l = 0; l < levels; l++) {
-		float **arr[7] = {&diag[l], &wx[l], &wy[l], &wz[l], &b[l], &x[l], &xalt[l]};
+		float **arr[4] = {&diag[l], &b[l], &x[l], &xalt[l]};
 		for (float **a : arr) { *a = p; p += cells[l]; }
+		mgw_t **warr[3] = {&wx[l], &wy[l], &wz[l]};
+		for (mgw_t **a : warr) { *a = reinterpret_cast<mgw_t *>(p); p += (cells[l] + 1) / 2; }
 	}
 	tail_list = reinterpret_cast<unsigned *>(p);
 	tail_count = reinterpret_cast<int *>(p + tail_words);
@@ -683,9 +696,7 @@ static MgTail tailDesc(const Multigrid &M)
 void mgBuild(Multigrid &M, const Rows<float> &A, int n, const int *mat2grid, hipStream_t s)
 {
 	// every level array starts at zero (inactive cells keep zero entries)
-	size_t total = 0;
-	for (int l = 0; l < M.levels; l++) total += 7 * M.cells[l];
-	hipLaunchKernelGGL(kMgZero, dim3(mgBlocks(total)), dim3(kBlock), 0, s, M.pool, total);
+	hipLaunchKernelGGL(kMgZero, dim3(mgBlocks(M.level_floats)), dim3(kBlock), 0, s, M.pool, M.level_floats);
 	MgBox B;
 	for (int a = 0; a < 3; a++) { B.org[a] = M.org[a]; B.gn[a] = M.gn[a]; }
 	const MgLevel L0 = level(M, 0);
@@ -775,13 +786,14 @@ void mgApplyRest(Multigrid &M, int n, double *z, hipStream_t s, bool box)
 
 // Algorithmic HBM bytes of one V-cycle: the residual's load and the
 // correction's store on level 0 (FP64 vector, FP32 level arrays: 16 + 12 B
-// per cell), and per level above the LDS tail four sweeps (operator 16 B, b,
-// x in, x out: 28 B per cell each), the restriction (28 B per fine cell) and
-// the prolongation (8 B per fine cell).
+// per cell), and per level above the LDS tail four sweeps (operator 10 B --
+// FP32 diagonal, three half-precision weights --, b, x in, x out: 22 B per
+// cell each), the restriction (22 B per fine cell) and the prolongation
+// (8 B per fine cell).
 double mgCycleBytes(const Multigrid &M)
 {
 	double b = 28.0 * double(M.cells[0]);
-	for (int l = 0; l < M.tail_first; l++) b += (4 * 28.0 + 28.0 + 8.0) * double(M.cells[l]);
+	for (int l = 0; l < M.tail_first; l++) b += (4 * 22.0 + 22.0 + 8.0) * double(M.cells[l]);
 	return b;
 }
 
