@@ -236,36 +236,47 @@ __global__ __launch_bounds__(kUB) void kConv3(ConvArgs A)
 	}
 }
 
-// Convolutions with 16 output channels (the finest level's layers and the
-// data gradients of the 16-channel inputs) have one N-fragment, so kConv3
-// reads an A fragment from LDS for every MFMA.  kConv3Rows reuses each A
-// fragment instead: a wave owns 4 x 4 rows (y, z) of 16 voxels, and the
+// Row-reuse convolutions.  kConv3 reads an A fragment from LDS for every
+// one or two MFMAs (one per 16 output channels).  kConv3Rows<NF> reuses each
+// A fragment instead: a wave owns 4 (y) x WZ (z) rows of 16 voxels (WZ = 4
+// for NF = 1, 2 for NF = 2) and NF groups of 16 output channels, and the
 // fragment of one input row at one x shift feeds every output row of the
-// wave that the row is a (dy, dz) tap of -- up to nine MFMAs.  The 18 B
-// fragments of a 16-channel chunk (the nine (dy, dz) taps, each as the pair
-// dx = 0, 1 and as dx = 2 with a zero second half) stay in registers for the
-// chunk.  Per chunk and wave: 72 A reads for 288 MFMAs (kConv3: 70 reads
-// for 56 MFMAs of this size); the zero halves make it 288 instead of 216.
-// Workgroup tile 16 x 8 x 8 voxels (the four waves 2 x 2 over y and z), its
-// halo 18 x 10 x 10 voxels of the chunk and the chunk's weights in LDS.
-constexpr int RX = 16, RY = 8, RZ = 8, RHX = RX + 2, RHY = RY + 2, RHZ = RZ + 2;
-constexpr int kRowsHalo = RHX * RHY * RHZ;
+// wave that the row is a (dy, dz) tap of, for every output group -- up to
+// 9 NF MFMAs.  Per pair of the K dimension the 9 NF B fragments of the
+// (dy, dz) taps stay in registers (pair 0: the taps dx = 0 in K 0-15 and
+// dx = 1 in K 16-31; pair 1: the tap dx = 2 and zeros).  Per 16-channel
+// chunk and wave: 72 (NF = 1) or 48 (NF = 2) A reads for 288 MFMAs (kConv3:
+// 70 reads for 56 MFMAs, or 70 for 112); the zero halves make it 288
+// instead of 216.  Workgroup tile 16 x 8 x 2 WZ voxels (the four waves 2 x 2
+// over y and z) for 16 NF output channels (output group blockIdx.z / z-tiles
+// as in kConv3); its halo of the chunk and the chunk's weights in LDS.
+constexpr int RX = 16, RY = 8, RHX = RX + 2, RHY = RY + 2;
+template <int NF> struct RowsTile {
+	static constexpr int WZ = NF == 1 ? 4 : 2, RZ = 2 * WZ, RHZ = RZ + 2, HALO = RHX * RHY * RHZ, CO = 16 * NF;
+};
 
-template <bool WIDE>
+template <bool WIDE, int NF>
 __global__ __launch_bounds__(kUB, 2) void kConv3Rows(ConvArgs A)
 {
-	__shared__ __align__(16) bf16 tin[kRowsHalo * 16];   // the output tile [voxel][16] reuses it
-	__shared__ __align__(16) bf16 tw[27 * 16 * 16];      // [tap][co][ci] of the chunk
+	using RT = RowsTile<NF>;
+	constexpr int WZ = RT::WZ, RZ = RT::RZ, RHZ = RT::RHZ, kHalo = RT::HALO, CO = RT::CO;
+	static_assert(RX * RY * RZ * CO <= kHalo * 16, "the output tile fits in the halo buffer");
+	__shared__ __align__(16) bf16 tin[kHalo * 16];   // the output tile [voxel][CO] reuses it
+	__shared__ __align__(16) bf16 tw[27 * CO * 16];  // [tap][co][ci] of the chunk and group
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const int kg = lane >> 4, li = lane & 15;
 	const int half = (kg & 1) * 8, second = kg >> 1;   // channel half and member of the K pair this lane holds
-	const int x0 = blockIdx.x * RX, y0 = blockIdx.y * RY, z0 = blockIdx.z * RZ;
-	const int wy = (wave & 1) * 4, wz = (wave >> 1) * 4;   // the wave's rows in the tile
-	f4 acc[4][4];   // [oz][oy]
+	const int nzt = (A.nz + RZ - 1) / RZ;
+	const int zt = blockIdx.z % nzt, grp = blockIdx.z / nzt;
+	const int x0 = blockIdx.x * RX, y0 = blockIdx.y * RY, z0 = zt * RZ, co0 = grp * CO;
+	const int wy = (wave & 1) * 4, wz = (wave >> 1) * WZ;   // the wave's rows in the tile
+	f4 acc[WZ][4][NF];   // [oz][oy][output group]
 #pragma unroll
-	for (int a = 0; a < 4; a++)
+	for (int a = 0; a < WZ; a++)
 #pragma unroll
-		for (int b = 0; b < 4; b++) acc[a][b] = f4{0, 0, 0, 0};
+		for (int b = 0; b < 4; b++)
+#pragma unroll
+			for (int f = 0; f < NF; f++) acc[a][b][f] = f4{0, 0, 0, 0};
 
 	for (int ci0 = 0; ci0 < A.Cin; ci0 += 16) {
 		// the staging's indices are recomputed per chunk from an opaque copy of
@@ -273,17 +284,18 @@ __global__ __launch_bounds__(kUB, 2) void kConv3Rows(ConvArgs A)
 		// through the MFMA phase
 		int tid = threadIdx.x;
 		asm volatile("" : "+v"(tid));
-		// the chunk's weights [27][16][16] (cinP a multiple of 16, coutP = 16):
+		// the chunk's weights [27][CO][16] of the group (cinP a multiple of 16):
 		// loads issued first, stored after the halo's
-		constexpr int kWVecs = 27 * 16 * 2, kWIt = (kWVecs + kUB - 1) / kUB;
+		constexpr int kWVecs = 27 * CO * 2, kWIt = (kWVecs + kUB - 1) / kUB;
 		uint4 wv[kWIt];
 #pragma unroll
 		for (int r = 0; r < kWIt; r++) {
-			const int q = tid + r * kUB, row = q >> 1, part = q & 1;   // row = t * 16 + co
-			wv[r] = q < kWVecs ? *reinterpret_cast<const uint4 *>(A.w + size_t(row) * A.cinP + ci0 + part * 8) : make_uint4(0, 0, 0, 0);
+			const int q = tid + r * kUB, row = q >> 1, part = q & 1;   // row = t * CO + co
+			const int t = row / CO, co = row % CO;
+			wv[r] = q < kWVecs ? *reinterpret_cast<const uint4 *>(A.w + (size_t(t) * A.coutP + co0 + co) * A.cinP + ci0 + part * 8) : make_uint4(0, 0, 0, 0);
 		}
 		if constexpr (WIDE) {
-			constexpr int kVecs = kRowsHalo * 2, kIt = (kVecs + kUB - 1) / kUB;
+			constexpr int kVecs = kHalo * 2, kIt = (kVecs + kUB - 1) / kUB;
 			uint4 hv[kIt];   // every load of the halo in flight together
 #pragma unroll
 			for (int r = 0; r < kIt; r++) {
@@ -304,7 +316,7 @@ __global__ __launch_bounds__(kUB, 2) void kConv3Rows(ConvArgs A)
 				if (q < kVecs) *reinterpret_cast<uint4 *>(tin + q * 8) = hv[r];
 			}
 		} else {
-			constexpr int kVecs = kRowsHalo * 4, kIt = (kVecs + kUB - 1) / kUB;
+			constexpr int kVecs = kHalo * 4, kIt = (kVecs + kUB - 1) / kUB;
 			for (int r0 = 0; r0 < kIt; r0 += 8) {
 				uint2 hv[8];
 #pragma unroll
@@ -334,33 +346,36 @@ __global__ __launch_bounds__(kUB, 2) void kConv3Rows(ConvArgs A)
 			if (q < kWVecs) *reinterpret_cast<uint4 *>(tw + q * 8) = wv[r];
 		}
 		__syncthreads();
-		// per pair p: its nine B fragments [dz][dy] (p = 0: the taps dx = 0
-		// in K 0-15 and dx = 1 in K 16-31; p = 1: the tap dx = 2 and zeros),
-		// then every input row (hy, hz) of the wave at the pair's x shifts
+		// per pair p: its B fragments [dz][dy][group], then every input row
+		// (hy, hz) of the wave at the pair's x shifts
 #pragma unroll
 		for (int p = 0; p < 2; p++) {
-			bf8v bw[3][3];
+			bf8v bw[3][3][NF];
 #pragma unroll
 			for (int dz = 0; dz < 3; dz++)
 #pragma unroll
-				for (int dy = 0; dy < 3; dy++) {
-					const int t = dz * 9 + dy * 3 + (p == 0 ? second : 2);
-					const s8v raw = *reinterpret_cast<const s8v *>(tw + (t * 16 + li) * 16 + half);
-					bw[dz][dy] = __builtin_bit_cast(bf8v, p == 1 && second ? s8v{0, 0, 0, 0, 0, 0, 0, 0} : raw);
-				}
+				for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+					for (int f = 0; f < NF; f++) {
+						const int t = dz * 9 + dy * 3 + (p == 0 ? second : 2);
+						const s8v raw = *reinterpret_cast<const s8v *>(tw + (t * CO + f * 16 + li) * 16 + half);
+						bw[dz][dy][f] = __builtin_bit_cast(bf8v, p == 1 && second ? s8v{0, 0, 0, 0, 0, 0, 0, 0} : raw);
+					}
 			const int dx = p == 0 ? second : 2;
 #pragma unroll
-			for (int hz = 0; hz < 6; hz++) {
+			for (int hz = 0; hz < WZ + 2; hz++) {
 #pragma unroll
 				for (int hy = 0; hy < 6; hy++) {
 					const bf8v a = __builtin_bit_cast(bf8v, *reinterpret_cast<const s8v *>(tin + (((wz + hz) * RHY + wy + hy) * RHX + li + dx) * 16 + half));
 #pragma unroll
-					for (int oz = 0; oz < 4; oz++) {
+					for (int oz = 0; oz < WZ; oz++) {
 						if (hz - oz < 0 || hz - oz > 2) continue;
 #pragma unroll
 						for (int oy = 0; oy < 4; oy++) {
 							if (hy - oy < 0 || hy - oy > 2) continue;
-							acc[oz][oy] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[hz - oz][hy - oy], acc[oz][oy], 0, 0, 0);
+#pragma unroll
+							for (int f = 0; f < NF; f++)
+								acc[oz][oy][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[hz - oz][hy - oy][f], acc[oz][oy][f], 0, 0, 0);
 						}
 					}
 				}
@@ -370,24 +385,26 @@ __global__ __launch_bounds__(kUB, 2) void kConv3Rows(ConvArgs A)
 		}
 		__syncthreads();
 	}
-	// bias, activation, BF16 into the tile [voxel][16] (x fastest)
+	// bias, activation, BF16 into the tile [voxel][CO] (x fastest)
 	bf16 *tout = tin;
-	{
-		const float bb = A.bias && li < A.Cout ? A.bias[li] : 0.0f;
 #pragma unroll
-		for (int oz = 0; oz < 4; oz++)
+	for (int f = 0; f < NF; f++) {
+		const int co = co0 + f * 16 + li;
+		const float bb = A.bias && co < A.Cout ? A.bias[co] : 0.0f;
+#pragma unroll
+		for (int oz = 0; oz < WZ; oz++)
 #pragma unroll
 			for (int oy = 0; oy < 4; oy++)
 #pragma unroll
 				for (int i = 0; i < 4; i++) {
-					float v = acc[oz][oy][i] + bb;
+					float v = acc[oz][oy][f][i] + bb;
 					if (A.relu) v = fmaxf(v, 0.0f);
 					const int vox = ((wz + oz) * RY + wy + oy) * RX + kg * 4 + i;
-					tout[vox * 16 + li] = f2bf(v);
+					tout[vox * CO + f * 16 + li] = f2bf(v);
 				}
 	}
 	__syncthreads();
-	const int nch = min(16, A.Cout);
+	const int nch = min(CO, A.Cout - co0);
 	if (A.pool) {
 		// the tile's origin is even along every axis: whole 2 x 2 x 2 families
 		constexpr int PX = RX / 2, PY = RY / 2, PZ = RZ / 2;
@@ -401,25 +418,26 @@ __global__ __launch_bounds__(kUB, 2) void kConv3Rows(ConvArgs A)
 #pragma unroll
 			for (int k = 0; k < 8; k++) {
 				const int vx = 2 * px + (k & 1), vy = 2 * py + (k >> 1 & 1), vz = 2 * pz + (k >> 2);
-				sum += bf2f(tout[((vz * RY + vy) * RX + vx) * 16 + c]);
+				sum += bf2f(tout[((vz * RY + vy) * RX + vx) * CO + c]);
 			}
-			A.pool[(size_t(Z * pny + Y) * pnx + X) * A.poolStride + c] = f2bf(sum * 0.125f);
+			A.pool[(size_t(Z * pny + Y) * pnx + X) * A.poolStride + co0 + c] = f2bf(sum * 0.125f);
 		}
 	}
-	if (nch == 16 && (A.outStride & 7) == 0 && (A.outOff & 7) == 0) {
-		for (int q = tid; q < RX * RY * RZ * 2; q += kUB) {
-			const int vox = q >> 1, w8 = q & 1;
+	if (nch == CO && (A.outStride & 7) == 0 && ((A.outOff + co0) & 7) == 0) {
+		constexpr int words = CO / 8;   // 16-byte words per voxel
+		for (int q = tid; q < RX * RY * RZ * words; q += kUB) {
+			const int vox = q / words, w8 = q % words;
 			const int x = x0 + vox % RX, y = y0 + (vox / RX) % RY, z = z0 + vox / (RX * RY);
 			if (x >= A.nx || y >= A.ny || z >= A.nz) continue;
-			*reinterpret_cast<uint4 *>(A.out + (size_t(z * A.ny + y) * A.nx + x) * A.outStride + A.outOff + w8 * 8) =
-				*reinterpret_cast<const uint4 *>(tout + vox * 16 + w8 * 8);
+			*reinterpret_cast<uint4 *>(A.out + (size_t(z * A.ny + y) * A.nx + x) * A.outStride + A.outOff + co0 + w8 * 8) =
+				*reinterpret_cast<const uint4 *>(tout + vox * CO + w8 * 8);
 		}
 	} else {
 		for (int q = tid; q < RX * RY * RZ * nch; q += kUB) {
 			const int vox = q / nch, c = q % nch;
 			const int x = x0 + vox % RX, y = y0 + (vox / RX) % RY, z = z0 + vox / (RX * RY);
 			if (x >= A.nx || y >= A.ny || z >= A.nz) continue;
-			A.out[(size_t(z * A.ny + y) * A.nx + x) * A.outStride + A.outOff + c] = tout[vox * 16 + c];
+			A.out[(size_t(z * A.ny + y) * A.nx + x) * A.outStride + A.outOff + co0 + c] = tout[vox * CO + c];
 		}
 	}
 }
@@ -979,12 +997,18 @@ static void conv(const UnetBuffers &B, const UnetLayer &L, int level, const bf16
 	A.relu = relu ? 1 : 0;
 	const bool wide = (inStride & 7) == 0 && (cin & 7) == 0 && (!up || ((upStride & 7) == 0 && (upC & 7) == 0));
 	if (coutP == 16) {
-		const dim3 g((A.nx + RX - 1) / RX, (A.ny + RY - 1) / RY, (A.nz + RZ - 1) / RZ);
-		if (wide) hipLaunchKernelGGL(kConv3Rows<true>, g, dim3(kUB), 0, s, A);
-		else hipLaunchKernelGGL(kConv3Rows<false>, g, dim3(kUB), 0, s, A);
+		const dim3 g((A.nx + RX - 1) / RX, (A.ny + RY - 1) / RY, (A.nz + RowsTile<1>::RZ - 1) / RowsTile<1>::RZ);
+		if (wide) hipLaunchKernelGGL((kConv3Rows<true, 1>), g, dim3(kUB), 0, s, A);
+		else hipLaunchKernelGGL((kConv3Rows<false, 1>), g, dim3(kUB), 0, s, A);
 		return;
 	}
 	const int groups = (coutP + 31) / 32;
+	if (coutP % 32 == 0) {
+		const dim3 g((A.nx + RX - 1) / RX, (A.ny + RY - 1) / RY, ((A.nz + RowsTile<2>::RZ - 1) / RowsTile<2>::RZ) * groups);
+		if (wide) hipLaunchKernelGGL((kConv3Rows<true, 2>), g, dim3(kUB), 0, s, A);
+		else hipLaunchKernelGGL((kConv3Rows<false, 2>), g, dim3(kUB), 0, s, A);
+		return;
+	}
 	const dim3 g((A.nx + CX - 1) / CX, (A.ny + TY - 1) / TY, ((A.nz + TZ - 1) / TZ) * groups);
 	const size_t lds = size_t(HZ * HY * CHX * 16 + 27 * std::min(32, coutP) * 16) * sizeof(bf16);
 	if (wide) hipLaunchKernelGGL(kConv3<true>, g, dim3(kUB), lds, s, A);
