@@ -237,28 +237,33 @@ __global__ __launch_bounds__(kUB) void kConv3(ConvArgs A)
 }
 
 // Row-reuse convolutions.  kConv3 reads an A fragment from LDS for every
-// one or two MFMAs (one per 16 output channels).  kConv3Rows<NF> reuses each
-// A fragment instead: a wave owns 4 (y) x WZ (z) rows of 16 voxels (WZ = 4
-// for NF = 1, 2 for NF = 2) and NF groups of 16 output channels, and the
-// fragment of one input row at one x shift feeds every output row of the
-// wave that the row is a (dy, dz) tap of, for every output group -- up to
-// 9 NF MFMAs.  Per pair of the K dimension the 9 NF B fragments of the
-// (dy, dz) taps stay in registers (pair 0: the taps dx = 0 in K 0-15 and
-// dx = 1 in K 16-31; pair 1: the tap dx = 2 and zeros).  Per 16-channel
-// chunk and wave: 72 (NF = 1) or 48 (NF = 2) A reads for 288 MFMAs (kConv3:
-// 70 reads for 56 MFMAs, or 70 for 112); the zero halves make it 288
-// instead of 216.  Workgroup tile 16 x 8 x 2 WZ voxels (the four waves 2 x 2
-// over y and z) for 16 NF output channels (output group blockIdx.z / z-tiles
-// as in kConv3); its halo of the chunk and the chunk's weights in LDS.
+// one or two MFMAs (one per 16 output channels).  kConv3Rows<NF, WZ> reuses
+// each A fragment instead: a wave owns 4 (y) x WZ (z) rows of 16 voxels and
+// NF groups of 16 output channels, and the fragment of one input row at one
+// x shift feeds every output row of the wave that the row is a (dy, dz) tap
+// of, for every output group -- up to 3 min(WZ, 3) NF MFMAs.  Per pair of
+// the K dimension the 9 NF B fragments of the (dy, dz) taps stay in
+// registers (pair 0: the taps dx = 0 in K 0-15 and dx = 1 in K 16-31; pair
+// 1: the tap dx = 2 and zeros).  Per 16-channel chunk and wave, NF = 1 with
+// WZ = 2: 48 A reads for 144 MFMAs; NF = 2 with WZ = 1: 36 reads for 144
+// (kConv3: 70 reads for 56 or 112); the zero halves make it 144 instead of
+// 108.  Workgroup tile 16 x 8 x 2 WZ voxels (the four waves 2 x 2 over y and
+// z) for 16 NF output channels (output group blockIdx.z / z-tiles as in
+// kConv3); its halo of the chunk and the chunk's weights in LDS (48 and
+// 50 KB: three workgroups per CU).  Deeper tiles (WZ = 4 for NF = 1, 2 for
+// NF = 2) read fewer A fragments per MFMA but fit two workgroups per CU and
+// were measured 2-12% slower.
 constexpr int RX = 16, RY = 8, RHX = RX + 2, RHY = RY + 2;
-template <int NF> struct RowsTile {
-	static constexpr int WZ = NF == 1 ? 4 : 2, RZ = 2 * WZ, RHZ = RZ + 2, HALO = RHX * RHY * RHZ, CO = 16 * NF;
+// z rows per wave and workgroups per CU of the two forms
+constexpr int kRowsWZ1 = 2, kRowsOcc1 = 3, kRowsWZ2 = 1, kRowsOcc2 = 3;
+template <int NF, int WZ_> struct RowsTile {
+	static constexpr int WZ = WZ_, RZ = 2 * WZ, RHZ = RZ + 2, HALO = RHX * RHY * RHZ, CO = 16 * NF;
 };
 
-template <bool WIDE, int NF>
-__global__ __launch_bounds__(kUB, 2) void kConv3Rows(ConvArgs A)
+template <bool WIDE, int NF, int WZ_, int OCC>
+__global__ __launch_bounds__(kUB, OCC) void kConv3Rows(ConvArgs A)
 {
-	using RT = RowsTile<NF>;
+	using RT = RowsTile<NF, WZ_>;
 	constexpr int WZ = RT::WZ, RZ = RT::RZ, RHZ = RT::RHZ, kHalo = RT::HALO, CO = RT::CO;
 	static_assert(RX * RY * RZ * CO <= kHalo * 16, "the output tile fits in the halo buffer");
 	__shared__ __align__(16) bf16 tin[kHalo * 16];   // the output tile [voxel][CO] reuses it
@@ -997,16 +1002,18 @@ static void conv(const UnetBuffers &B, const UnetLayer &L, int level, const bf16
 	A.relu = relu ? 1 : 0;
 	const bool wide = (inStride & 7) == 0 && (cin & 7) == 0 && (!up || ((upStride & 7) == 0 && (upC & 7) == 0));
 	if (coutP == 16) {
-		const dim3 g((A.nx + RX - 1) / RX, (A.ny + RY - 1) / RY, (A.nz + RowsTile<1>::RZ - 1) / RowsTile<1>::RZ);
-		if (wide) hipLaunchKernelGGL((kConv3Rows<true, 1>), g, dim3(kUB), 0, s, A);
-		else hipLaunchKernelGGL((kConv3Rows<false, 1>), g, dim3(kUB), 0, s, A);
+		constexpr int RZ1 = RowsTile<1, kRowsWZ1>::RZ;
+		const dim3 g((A.nx + RX - 1) / RX, (A.ny + RY - 1) / RY, (A.nz + RZ1 - 1) / RZ1);
+		if (wide) hipLaunchKernelGGL((kConv3Rows<true, 1, kRowsWZ1, kRowsOcc1>), g, dim3(kUB), 0, s, A);
+		else hipLaunchKernelGGL((kConv3Rows<false, 1, kRowsWZ1, kRowsOcc1>), g, dim3(kUB), 0, s, A);
 		return;
 	}
 	const int groups = (coutP + 31) / 32;
 	if (coutP % 32 == 0) {
-		const dim3 g((A.nx + RX - 1) / RX, (A.ny + RY - 1) / RY, ((A.nz + RowsTile<2>::RZ - 1) / RowsTile<2>::RZ) * groups);
-		if (wide) hipLaunchKernelGGL((kConv3Rows<true, 2>), g, dim3(kUB), 0, s, A);
-		else hipLaunchKernelGGL((kConv3Rows<false, 2>), g, dim3(kUB), 0, s, A);
+		constexpr int RZ2 = RowsTile<2, kRowsWZ2>::RZ;
+		const dim3 g((A.nx + RX - 1) / RX, (A.ny + RY - 1) / RY, ((A.nz + RZ2 - 1) / RZ2) * groups);
+		if (wide) hipLaunchKernelGGL((kConv3Rows<true, 2, kRowsWZ2, kRowsOcc2>), g, dim3(kUB), 0, s, A);
+		else hipLaunchKernelGGL((kConv3Rows<false, 2, kRowsWZ2, kRowsOcc2>), g, dim3(kUB), 0, s, A);
 		return;
 	}
 	const dim3 g((A.nx + CX - 1) / CX, (A.ny + TY - 1) / TY, ((A.nz + TZ - 1) / TZ) * groups);
