@@ -1,19 +1,21 @@
 // Three-level 3D U-Net (16/32/64 channels) on gfx950: BF16 channels-last
 // activations and weights, 3x3x3 convolutions as implicit GEMM on
-// v_mfma_f32_16x16x32_bf16 (v_mfma_f32_16x16x16_bf16 for the weight
-// gradient) with FP32 accumulation, FP32 master weights and Adam.  The network has no counterpart in the reference; its layer order,
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation, FP32 master weights and
+// Adam.  The network has no counterpart in the reference; its layer order,
 // layouts and gradients follow oracle/unet_oracle.py, which the tests hold it
 // against.
 //
-// Convolution tile: 16 (x) x 4 (y) x 4 (z) output voxels per 256-thread
-// workgroup, each wave64 one z-slice of four 16-voxel M-fragments, output
-// channels in groups of 32 (two N-fragments).  The input tile with a one-voxel
-// halo (18 x 6 x 6 voxels, 16 channels at a time) and the matching weights are
-// staged in LDS (48 KB), so every input voxel is fetched from HBM once per
-// 16-channel chunk and group instead of 27 times.
+// Forward and data-gradient convolutions: kConv3Rows for output groups of 16
+// and 32 channels (an A fragment of one input row feeds every output row it
+// is a tap of), kConv3 for the rest (48-channel data gradients).  Both stage
+// the input tile with a one-voxel halo and the chunk's weights in LDS, 16
+// input channels at a time, so every input voxel is fetched from HBM once
+// per chunk and output group instead of 27 times.  Weight gradients:
+// kConv3WGrad, voxels on the K dimension, per-workgroup partials summed in a
+// fixed order by kReduceWGrad.
 //
-// MFMA 16x16x16 BF16 lane layout (wave64): lane l holds A[l%16][4(l/16)..+3],
-// B[4(l/16)..+3][l%16], and D[4(l/16)+i][l%16] for i = 0..3.
+// MFMA 16x16x32 BF16 lane layout (wave64): lane l holds A[l%16][8(l/16)..+7],
+// B[8(l/16)..+7][l%16], and D[4(l/16)+i][l%16] for i = 0..3.
 #include "garm_unet.hpp"
 
 #include <algorithm>
@@ -23,7 +25,6 @@
 
 namespace garm {
 
-typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int TX = 16, TY = 4, TZ = 4, HX = TX + 2, HY = TY + 2, HZ = TZ + 2;
@@ -264,7 +265,7 @@ template <bool WIDE, int NF, int WZ_, int OCC>
 __global__ __launch_bounds__(kUB, OCC) void kConv3Rows(ConvArgs A)
 {
 	using RT = RowsTile<NF, WZ_>;
-	constexpr int WZ = RT::WZ, RZ = RT::RZ, RHZ = RT::RHZ, kHalo = RT::HALO, CO = RT::CO;
+	constexpr int WZ = RT::WZ, RZ = RT::RZ, kHalo = RT::HALO, CO = RT::CO;
 	static_assert(RX * RY * RZ * CO <= kHalo * 16, "the output tile fits in the halo buffer");
 	__shared__ __align__(16) bf16 tin[kHalo * 16];   // the output tile [voxel][CO] reuses it
 	__shared__ __align__(16) bf16 tw[27 * CO * 16];  // [tap][co][ci] of the chunk and group
@@ -448,7 +449,18 @@ __global__ __launch_bounds__(kUB, OCC) void kConv3Rows(ConvArgs A)
 }
 
 // Weight gradient partials: for each tap t, dW_t = X_t^T Gz over this
-// workgroup's 256 voxels (K = voxels on the MFMA), and the bias partial.
+// workgroup's 256 voxels (16 x 4 x 4; K = voxels on the MFMA), and the bias
+// partial.  On v_mfma_f32_16x16x32_bf16 a lane holds eight consecutive K
+// values of one row (A: input channel, B: output channel), i.e. one channel
+// at eight voxels.  So both tiles are staged in LDS channel-major: the
+// output gradient as [channel][voxel], and the input as three copies
+// [channel][z][y][16 x] already shifted by the tap's dx = 0, 1, 2 (copy dx
+// holds halo column x + dx at x).  A K step of 32 voxels is two 16-voxel
+// rows, and every fragment is one aligned 16-byte LDS read: per tap and
+// chunk eight A reads for 8 NF MFMAs, the B fragments of the 256 voxels in
+// registers for all taps.  (Workgroups that sum a run of tiles along z in
+// registers, writing fewer partials, were measured 1.6x slower: the runs
+// serialise the staging that independent workgroups overlap.)
 struct WGradArgs {
 	const bf16 *x;
 	int xStride, Cin;
@@ -460,10 +472,13 @@ struct WGradArgs {
 	float *partialB;   // [block][coutP]
 };
 
+constexpr int kWVox = TX * TY * TZ;   // 256 voxels of a tile
+constexpr int kWRow = HZ * HY * TX;   // one channel of one shifted input copy
+
 __global__ __launch_bounds__(kUB) void kConv3WGrad(WGradArgs A)
 {
-	__shared__ __align__(16) bf16 tin[HZ * HY * HX * 16];
-	__shared__ __align__(16) bf16 tg[TZ * TY * TX * 32];
+	__shared__ __align__(16) bf16 tg[32 * kWVox];        // [co][voxel]
+	__shared__ __align__(16) bf16 tin[3 * 16 * kWRow];   // [dx][ci][z][y][x]
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const int kg = lane >> 4, li = lane & 15;
 	const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
@@ -472,50 +487,74 @@ __global__ __launch_bounds__(kUB) void kConv3WGrad(WGradArgs A)
 	const int z0 = zt * TZ, co0 = grp * 32;
 	const int NF = min(32, A.coutP - co0) / 16;
 	const int block = (zt * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-	// output-gradient tile (256 voxels x up to 32 channels), zero outside
-	for (int q = tid; q < TZ * TY * TX * 32; q += kUB) {
-		const int c = q & 31, v = q >> 5;
+	// output-gradient tile, transposed: 8 channels of a voxel per load
+	for (int q = tid; q < kWVox * 4; q += kUB) {
+		const int v = q >> 2, c8 = (q & 3) * 8;
 		const int vx = v % TX, vy = (v / TX) % TY, vz = v / (TX * TY);
 		const int gx = x0 + vx, gy = y0 + vy, gzz = z0 + vz;
-		bf16 val = 0;
-		if (c < NF * 16 && co0 + c < A.Cout && gx < A.nx && gy < A.ny && gzz < A.nz)
-			val = A.gz[(size_t(gzz * A.ny + gy) * A.nx + gx) * A.gStride + A.gOff + co0 + c];
-		tg[v * 32 + c] = val;
+		bf16 val[8];
+#pragma unroll
+		for (int j = 0; j < 8; j++) val[j] = 0;
+		if (gx < A.nx && gy < A.ny && gzz < A.nz) {
+			const bf16 *src = A.gz + (size_t(gzz * A.ny + gy) * A.nx + gx) * A.gStride + A.gOff + co0 + c8;
+			if (c8 + 8 <= NF * 16 && co0 + c8 + 8 <= A.Cout && ((A.gStride | (A.gOff + co0 + c8)) & 7) == 0) {
+				const uint4 w = *reinterpret_cast<const uint4 *>(src);
+				__builtin_memcpy(val, &w, 16);
+			} else {
+#pragma unroll
+				for (int j = 0; j < 8; j++)
+					if (c8 + j < NF * 16 && co0 + c8 + j < A.Cout) val[j] = src[j];
+			}
+		}
+#pragma unroll
+		for (int j = 0; j < 8; j++) tg[(c8 + j) * kWVox + v] = val[j];
 	}
 	__syncthreads();
 	if (tid < NF * 16) {
 		float sb = 0;
-		for (int v = 0; v < TZ * TY * TX; v++) sb += bf2f(tg[v * 32 + tid]);
+		for (int v = 0; v < kWVox; v++) sb += bf2f(tg[tid * kWVox + v]);
 		A.partialB[size_t(block) * A.coutP + co0 + tid] = sb;
 	}
+	// the B fragments of the tile: K step s covers voxels 32 s .. 32 s + 31
+	bf8v bq[kWVox / 32][2];
+#pragma unroll
+	for (int st = 0; st < kWVox / 32; st++)
+#pragma unroll
+		for (int f = 0; f < 2; f++)
+			bq[st][f] = __builtin_bit_cast(bf8v, *reinterpret_cast<const s8v *>(tg + (f * 16 + li) * kWVox + st * 32 + kg * 8));
 	for (int ci0 = 0; ci0 < A.cinP; ci0 += 16) {
 		__syncthreads();
+		// input halo (18 x 6 x 6 voxels, 4 channels per load) into the three
+		// shifted copies
 		for (int q = tid; q < HZ * HY * HX * 4; q += kUB) {
 			const int quad = q & 3, v = q >> 2;
 			const int hx = v % HX, hy = (v / HX) % HY, hz = v / (HX * HY);
 			const int gx = x0 + hx - 1, gy = y0 + hy - 1, gzz = z0 + hz - 1;
 			const int ch = ci0 + quad * 4;
-			uint2 val = make_uint2(0, 0);
-			if (gx >= 0 && gx < A.nx && gy >= 0 && gy < A.ny && gzz >= 0 && gzz < A.nz && ch < A.Cin)
-				val = *reinterpret_cast<const uint2 *>(A.x + (size_t(gzz * A.ny + gy) * A.nx + gx) * A.xStride + ch);
-			*reinterpret_cast<uint2 *>(tin + v * 16 + quad * 4) = val;
+			bf16 val[4] = {0, 0, 0, 0};
+			if (gx >= 0 && gx < A.nx && gy >= 0 && gy < A.ny && gzz >= 0 && gzz < A.nz && ch < A.Cin) {
+				const uint2 w = *reinterpret_cast<const uint2 *>(A.x + (size_t(gzz * A.ny + gy) * A.nx + gx) * A.xStride + ch);
+				__builtin_memcpy(val, &w, 8);
+			}
+#pragma unroll
+			for (int dx = 0; dx < 3; dx++) {
+				const int x = hx - dx;
+				if (x < 0 || x >= TX) continue;
+#pragma unroll
+				for (int j = 0; j < 4; j++) tin[((dx * 16 + quad * 4 + j) * HZ + hz) * HY * TX + hy * TX + x] = val[j];
+			}
 		}
 		__syncthreads();
 		for (int t = wave; t < 27; t += 4) {
 			const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
 			f4 acc[2] = {f4{0, 0, 0, 0}, f4{0, 0, 0, 0}};
-			for (int s = 0; s < TZ * TY * TX / 16; s++) {
-				s4 a, b0, b1;
 #pragma unroll
-				for (int j = 0; j < 4; j++) {
-					const int vk = s * 16 + kg * 4 + j;
-					const int vx = vk % TX, vy = (vk / TX) % TY, vz = vk / (TX * TY);
-					a[j] = short(tin[(((vz + dz) * HY + vy + dy) * HX + vx + dx) * 16 + li]);
-					b0[j] = short(tg[vk * 32 + li]);
-					b1[j] = short(tg[vk * 32 + 16 + li]);
-				}
-				acc[0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b0, acc[0], 0, 0, 0);
-				if (NF > 1) acc[1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b1, acc[1], 0, 0, 0);
+			for (int st = 0; st < kWVox / 32; st++) {
+				// voxels 32 st + 8 kg .. + 7: one x run of row (vy, vz)
+				const int v = st * 32 + kg * 8, vx = v % TX, vy = (v / TX) % TY, vz = v / (TX * TY);
+				const bf8v a = __builtin_bit_cast(bf8v, *reinterpret_cast<const s8v *>(tin + ((dx * 16 + li) * HZ + vz + dz) * HY * TX + (vy + dy) * TX + vx));
+				acc[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bq[st][0], acc[0], 0, 0, 0);
+				if (NF > 1) acc[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bq[st][1], acc[1], 0, 0, 0);
 			}
 			float *dst = A.partial + (size_t(block) * 27 + t) * A.cinP * A.coutP;
 			for (int q = 0; q < NF; q++)
