@@ -13,12 +13,15 @@ Ambiguous 3D configurations (sign patterns no plane produces) are pinned by
 closed forms of the marching-cubes geometry (mc_geometry.py): separated
 corners and edges, a plane-cut component next to separated corners, two
 outside corners on a body diagonal; and by invariance under the 24 rotations
-of the cube.  Four classes stay pinned only by the restatement: the
-four-corner zigzag chains (the canonical cases with signed-volume fans that
-no plane reaches) and the two complement configurations whose outside
-corners the surface joins by a tunnel (an outside edge plus a corner, two
-outside corners on a face diagonal); their volumes depend on the fixed
-triangulation of a skew polygon, which no geometric identity fixes.
+of the cube.  Every one of the 256 patterns is further pinned by the
+geometry alone (mc_geometry.candidate_volumes): its value must be the volume
+of an admissible marching-cubes surface -- some resolution of each ambiguous
+face and some triangulation of each (possibly skew) loop of edge crossings --
+computed by the divergence theorem with the field (x, 0, 0), which shares no
+formula with the case table.  This covers the classes whose value depends on
+the table's fixed triangulation of a skew polygon (the four-corner zigzag
+chains, an outside edge plus an outside corner joined by a tunnel, two
+outside corners on a face diagonal).
 """
 import itertools
 import math
@@ -272,3 +275,26 @@ def test_rotation_invariance():
                 w[p] = v
                 assert abs(O.mc_fraction(list(w)) - want) <= 1e-15, (m, p)
     assert checked == 256 - 6 - 38 - 8 - 12
+
+
+def test_every_pattern_is_the_volume_of_an_admissible_mc_surface():
+    """For all 256 sign patterns and random corner magnitudes, the oracle's
+    volume equals one of the candidate volumes of the marching-cubes geometry
+    (every face resolution and loop triangulation, divergence theorem); a
+    tetrahedron too many or too few, or an offset of 1e-4, matches none."""
+    rng = np.random.default_rng(15)
+    worst, planted, ambiguous = 0.0, 0, 0
+    for m in range(256):
+        for _ in range(3):
+            mag = rng.uniform(0.05, 1.0, 8)
+            v = np.where([m >> i & 1 for i in range(8)], -mag, mag)
+            cand = G.candidate_volumes(v)
+            got = O.mc_fraction(list(v))
+            worst = max(worst, float(np.min(np.abs(cand - got))))
+            ambiguous += len(np.unique(np.round(cand, 12))) > 1
+            tet = np.prod(rng.uniform(0.0, 1.0, 3)) / 6.0
+            for wrong in (got + tet, got - tet, got + 1e-4, got - 1e-4):
+                planted += bool(np.min(np.abs(cand - wrong)) < 1e-9)
+    assert worst <= 1e-13
+    assert planted == 0
+    assert ambiguous >= 300   # the check has a choice to make in most samples
