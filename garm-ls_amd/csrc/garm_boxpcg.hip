@@ -49,6 +49,10 @@ namespace garm {
 constexpr int kBoxGrid = 1024;
 constexpr double kBoxReplaceDrop = 1e-3;
 constexpr int kBoxReplaceMax = 200;
+// most cells of active column tiles per unknown for the box solver
+constexpr double kBoxActiveRatio64 = 1.6, kBoxActiveRatio32 = 2.0;
+
+constexpr int kColPartialsMax = 8192;   // most column tiles of one box
 
 // scalar slots of the single-reduction iteration (by iteration parity)
 enum { kScGam0 = 10, kScAlp0 = 12, kScPrevTrue = 14 };
@@ -140,6 +144,31 @@ __global__ void kBoxCells(const int *mat2grid, int nu, int gx, int gy, int ox, i
 		const int i = g % gx - ox, j = (g / gx) % gy - oy, k = g / (gx * gy) - oz;
 		boxOf[r] = (unsigned(k) * unsigned(by) + unsigned(j)) * unsigned(bx) + unsigned(i);
 	}
+}
+
+// Column tiles holding unknowns: a byte per tile (same-value stores), then
+// their ordered list, compacted by one workgroup (the list order fixes the
+// order of the partial sums, so it must not depend on timing); the count
+// follows the list.
+__global__ void kTileMark(const unsigned *boxOf, int nu, int bx, int by, int ty, int zch, int gx, int gy, unsigned char *mask)
+{
+	for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nu; r += gridDim.x * blockDim.x) {
+		const unsigned c = boxOf[r];
+		const int i = int(c % unsigned(bx)), j = int((c / unsigned(bx)) % unsigned(by)), k = int(c / (unsigned(bx) * unsigned(by)));
+		mask[i / 64 + gx * (j / ty + gy * (k / zch))] = 1;
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void kTileList(const unsigned char *mask, int nt, int *tiles)
+{
+	const int per = (nt + kBlock - 1) / kBlock, t0 = int(threadIdx.x) * per, t1 = min(nt, t0 + per);
+	int cnt = 0;
+	for (int t = t0; t < t1; t++) cnt += mask[t] != 0;
+	int total;
+	int at = blockExclusiveScan(cnt, &total);
+	for (int t = t0; t < t1; t++)
+		if (mask[t]) tiles[at++] = t;
+	if (threadIdx.x == 0) tiles[kColPartialsMax] = total;
 }
 
 __global__ void kZeroWords(double *p, size_t words)
@@ -245,6 +274,8 @@ struct CgArgs {
 	double *partOut;
 	double *sc;
 	int *st;
+	const int *tiles;            // column kernel: the tiles holding unknowns
+	int gx, gy;                  // column kernel: tiles per row and per plane
 };
 
 // alpha, beta of iteration k from the previous launch's partials (every
@@ -344,7 +375,7 @@ __global__ __launch_bounds__(kBlock) void kCgIter(CgArgs<T, TX> A, int k)
 // (48.6 against 54.8 us per FP64 iteration at 256^3).  Cells outside the box
 // read as 0 (their face weights are 0).
 constexpr int kColX = 64;
-constexpr int kColPartials = 8192;   // most workgroups of one launch
+constexpr int kColPartials = kColPartialsMax;   // most workgroups of one launch
 
 template <typename T, typename TX, bool RESTART, int TY>
 __global__ __launch_bounds__(kColX * TY) void kCgIterCol(CgArgs<T, TX> A, int k)
@@ -352,12 +383,16 @@ __global__ __launch_bounds__(kColX * TY) void kCgIterCol(CgArgs<T, TX> A, int k)
 	constexpr int NW = kColX * TY / kWave;
 	if (A.st[kStDone]) return;
 	double al, be;
-	if (!cgScalars<NW, RESTART>(A, k, flatBlock() == 0, al, be)) return;
+	if (!cgScalars<NW, RESTART>(A, k, blockIdx.x == 0, al, be)) return;
 	const T alpha = T(al), beta = T(be);
 	const TX alphaX = TX(al);
 	__shared__ T plane[TY + 2][kColX + 2];
 	const int tx = int(threadIdx.x), ty = int(threadIdx.y);
-	const Tile3 tile = xcdTiles();
+	// one workgroup per tile holding unknowns (the others stay zero in every
+	// vector), dealt to the XCDs in contiguous runs of the list
+	const int slot = xcdIndex();
+	const int tl = A.tiles[slot];
+	const Tile3 tile{tl % A.gx, (tl / A.gx) % A.gy, tl / (A.gx * A.gy)};
 	const int bx = A.bx, by = A.by, bz = A.bz;
 	const int i = tile.x * kColX + tx, j = tile.y * TY + ty;
 	const int k0 = tile.z * A.zch, k1 = min(bz, k0 + A.zch);
@@ -435,7 +470,7 @@ __global__ __launch_bounds__(kColX * TY) void kCgIterCol(CgArgs<T, TX> A, int k)
 		un0 = un1;
 		wz0 = wz1;
 	}
-	wgPutParts<NW, 3>(v, A.partOut, tile.x + int(gridDim.x) * (tile.y + int(gridDim.y) * tile.z));
+	wgPutParts<NW, 3>(v, A.partOut, slot);
 }
 
 // mixed form: x += e, e = 0
@@ -522,21 +557,55 @@ static int boxSolve(const Rows<double> &A64, const Rows<T> &A, const double *b, 
 	constexpr int nT = mixed ? 12 : 11;
 	const size_t tWords = (nT * span * sizeof(T) + 7) / 8;
 	const size_t words = tWords + 2 * span + (mixed ? 4 * span : 0);
-	if (words > B.cap) {
-		if (B.pool) (void)hipFree(B.pool);
-		B.pool = nullptr;
-		const size_t want = words + words / 4;
-		if (hipMalloc(&B.pool, want * sizeof(double)) != hipSuccess) { B.cap = 0; err = "solver box allocation failed"; return GARM_ENOMEM; }
-		B.cap = want;
-	}
 	constexpr size_t kPartWords = 6 * kColPartials;   // one parity's partials
 	if (!B.partials && hipMalloc(&B.partials, 2 * kPartWords * sizeof(double)) != hipSuccess) { err = "solver partials allocation failed"; return GARM_ENOMEM; }
+	if (!B.tiles && (hipMalloc(&B.tiles, (kColPartialsMax + 1) * sizeof(int)) != hipSuccess ||
+			hipMalloc(&B.tileMask, kColPartialsMax) != hipSuccess)) { err = "solver tile list allocation failed"; return GARM_ENOMEM; }
 	if (size_t(nu) > B.ocap) {
 		if (B.boxOf) (void)hipFree(B.boxOf);
 		B.boxOf = nullptr;
 		const size_t want = size_t(nu) + size_t(nu) / 4;
 		if (hipMalloc(&B.boxOf, want * sizeof(unsigned)) != hipSuccess) { B.ocap = 0; err = "solver box allocation failed"; return GARM_ENOMEM; }
 		B.ocap = want;
+	}
+	const int nbU = boxBlocks(size_t(nu)), nbC = boxBlocks(cells);
+	const dim3 b1(kBlock);
+	hipLaunchKernelGGL(kBoxCells, dim3(nbU), b1, 0, s, L.mat2grid, nu, L.gn[0], L.gn[1], L.lo[0], L.lo[1], D == 3 ? L.lo[2] : 0, bx, by, B.boxOf);
+	// 3D: the z-walking kernel over 64 x ty tiles and columns of zmin planes
+	// (more planes per column when that would exceed kColPartials
+	// workgroups); 2D and very wide boxes: the grid-stride one.  Measured at
+	// 256^3 over 64 x {4, 8, 16} tiles and {4, 8, 16}-plane columns: FP64
+	// 64 x 4 x 16 (48.6 us per iteration; 51.6-61.5 for the others), FP32
+	// 64 x 8 x 8 (28.7 us; 29.7-37.7)
+	constexpr int ty = mixed ? 8 : 4, zmin = mixed ? 8 : 16;
+	const int gx = (bx + kColX - 1) / kColX, gy = (by + ty - 1) / ty;
+	int zch = 0, gz = 0;
+	if (D == 3 && gx * gy <= kColPartials) {
+		zch = std::max(zmin, (bz + kColPartials / (gx * gy) - 1) / (kColPartials / (gx * gy)));
+		gz = (bz + zch - 1) / zch;
+	}
+	// the tiles holding unknowns, in order; only they are launched (a
+	// splash's box is mostly empty)
+	int ntiles = 0;
+	if (zch) {
+		const int nt = gx * gy * gz;
+		if (hipMemsetAsync(B.tileMask, 0, nt, s) != hipSuccess) { err = "solver tile mask clear failed"; return GARM_EHIP; }
+		hipLaunchKernelGGL(kTileMark, dim3(nbU), b1, 0, s, (const unsigned *)B.boxOf, nu, bx, by, ty, zch, gx, gy, B.tileMask);
+		hipLaunchKernelGGL(kTileList, dim3(1), b1, 0, s, (const unsigned char *)B.tileMask, nt, B.tiles);
+		if (hipMemcpyAsync(K.host_state + 8, B.tiles + kColPartialsMax, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+			spinWait(s, K.ready) != hipSuccess) { err = "solver tile count readback failed"; return GARM_EHIP; }
+		ntiles = K.host_state[8];
+		// too sparse for the box: the compact rows cost less per iteration
+		// (per cell of an active tile against per unknown, at 256^3: FP64
+		// 22.6 against 37 ns, FP32 13.2 against 27 ns)
+		if (double(ntiles) * kColX * ty * zch > (mixed ? kBoxActiveRatio32 : kBoxActiveRatio64) * nu) return kBoxTooSparse;
+	}
+	if (words > B.cap) {
+		if (B.pool) (void)hipFree(B.pool);
+		B.pool = nullptr;
+		const size_t want = words + words / 4;
+		if (hipMalloc(&B.pool, want * sizeof(double)) != hipSuccess) { B.cap = 0; err = "solver box allocation failed"; return GARM_ENOMEM; }
+		B.cap = want;
 	}
 	T *tp = reinterpret_cast<T *>(B.pool);
 	auto carveT = [&]() { T *q = tp + margin; tp += span; return q; };
@@ -558,11 +627,8 @@ static int boxSolve(const Rows<double> &A64, const Rows<T> &A, const double *b, 
 	} else xIt = x64;
 	double *part[2] = {B.partials, B.partials + kPartWords};
 
-	const int nbU = boxBlocks(size_t(nu)), nbC = boxBlocks(cells);
-	const dim3 b1(kBlock);
 	// everything zero (margins, non-unknown cells, p, s, e), then the operator and b
 	hipLaunchKernelGGL(kZeroWords, dim3(8192), b1, 0, s, B.pool, words);
-	hipLaunchKernelGGL(kBoxCells, dim3(nbU), b1, 0, s, L.mat2grid, nu, L.gn[0], L.gn[1], L.lo[0], L.lo[1], D == 3 ? L.lo[2] : 0, bx, by, B.boxOf);
 	hipLaunchKernelGGL((kBoxOperator<D, T>), dim3(nbU), b1, 0, s, A, nu, (const unsigned *)B.boxOf, diag, wx, wy, wz);
 	if (mixed)
 		hipLaunchKernelGGL((kBoxOperator<D, double>), dim3(nbU), b1, 0, s, A64, nu, (const unsigned *)B.boxOf, d64, w64[0], w64[1], w64[2]);
@@ -583,20 +649,7 @@ static int boxSolve(const Rows<double> &A64, const Rows<T> &A, const double *b, 
 	};
 	startRun(0, false);   // p = s = 0
 	const int check = std::max(1, o.check_interval);
-	// 3D: the z-walking kernel over 64 x ty tiles and columns of zmin planes
-	// (more planes per column when that would exceed kColPartials
-	// workgroups); 2D and very wide boxes: the grid-stride one.  Measured at
-	// 256^3 over 64 x {4, 8, 16} tiles and {4, 8, 16}-plane columns: FP64
-	// 64 x 4 x 16 (48.6 us per iteration; 51.6-61.5 for the others), FP32
-	// 64 x 8 x 8 (28.7 us; 29.7-37.7)
-	constexpr int ty = mixed ? 8 : 4, zmin = mixed ? 8 : 16;
-	const int gx = (bx + kColX - 1) / kColX, gy = (by + ty - 1) / ty;
-	int zch = 0, gz = 0;
-	if (D == 3 && gx * gy <= kColPartials) {
-		zch = std::max(zmin, (bz + kColPartials / (gx * gy) - 1) / (kColPartials / (gx * gy)));
-		gz = (bz + zch - 1) / zch;
-	}
-	const int nbIter = zch ? gx * gy * gz : nbC;
+	const int nbIter = zch ? ntiles : nbC;
 	int rc, replacements = 0;
 	bool restart = true;
 	for (;;) {
@@ -609,9 +662,9 @@ static int boxSolve(const Rows<double> &A64, const Rows<T> &A, const double *b, 
 				const int k = base + q, cur = k & 1;
 				const bool first = restart && q == 0;
 				CgArgs<T, TX> a{O, bx, by, bz, zch, rv[cur], wv[cur], sv[cur], rv[cur ^ 1], wv[cur ^ 1], sv[cur ^ 1], p, xIt,
-					part[cur ^ 1], first ? nbC : nbIter, part[cur], K.sc, K.state};
+					part[cur ^ 1], first ? nbC : nbIter, part[cur], K.sc, K.state, B.tiles, gx, gy};
 				if (zch) {
-					const dim3 g(gx, gy, gz), bk(kColX, ty);
+					const dim3 g(ntiles), bk(kColX, ty);
 					if (first) hipLaunchKernelGGL((kCgIterCol<T, TX, true, ty>), g, bk, 0, s, a, k);
 					else hipLaunchKernelGGL((kCgIterCol<T, TX, false, ty>), g, bk, 0, s, a, k);
 				} else {
@@ -642,11 +695,13 @@ static int boxSolve(const Rows<double> &A64, const Rows<T> &A, const double *b, 
 	// r, diag, s, p and writes w; a replacement reads x, b and the FP64
 	// operator and writes r (mixed: plus the fold, x and e read and written),
 	// then reads r, p and the operator and writes w and s
+	// (iterations: the cells of the active tiles)
 	const double ts = double(sizeof(T));
 	const double iterBytes = 14 * ts;
 	const double startBytes = 5 * ts;
 	const double replBytes = 48 + ts + (mixed ? 16 + 2 * ts : 0.0) + 8 * ts;
-	st->hbm_bytes = double(cells) * (iterBytes * st->iterations + startBytes + replBytes * replacements);
+	const double iterCells = zch ? std::min(double(cells), double(ntiles) * kColX * ty * zch) : double(cells);
+	st->hbm_bytes = iterCells * iterBytes * st->iterations + double(cells) * (startBytes + replBytes * replacements);
 	if (!K.host_state[kStOuterDone]) {
 		err = std::string(mixed ? "mixed-precision solve" : "PCG") + " did not reach the requested tolerance (" + std::to_string(st->iterations) +
 			" iterations, " + std::to_string(replacements) + " replacements, relative residual " + std::to_string(st->relative_residual) + ")";

@@ -218,4 +218,13 @@ __device__ __forceinline__ Tile3 xcdTiles()
 	return Tile3{int(t % nx), int((t / nx) % ny), int(t / (nx * ny))};
 }
 
+// The same for a one-dimensional launch over a list: the index in [0,
+// gridDim.x) this workgroup takes, contiguous runs per XCD.
+__device__ __forceinline__ int xcdIndex()
+{
+	const unsigned total = gridDim.x, b = blockIdx.x;
+	const unsigned q = total / kXcds, r = total % kXcds, x = b % kXcds, k = b / kXcds;
+	return int(x * q + min(x, r) + k);
+}
+
 } // namespace garm

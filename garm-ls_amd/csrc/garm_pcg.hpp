@@ -66,14 +66,19 @@ struct BoxKrylov {
 	size_t cap = 0;            // doubles
 	unsigned *boxOf = nullptr; // box cell of every unknown
 	size_t ocap = 0;
-	double *partials = nullptr;  // per-workgroup partials of the z-walking product
+	double *partials = nullptr;  // per-workgroup partials of the z-walking iteration
+	int *tiles = nullptr;        // active column tiles (and their count after them)
+	unsigned char *tileMask = nullptr;
 	~BoxKrylov();
 };
 
 // Jacobi PCG on the box of the unknowns with the operator held per cell
-// (diagonal and -x, -y, -z face weights), two launches per iteration: FP64
+// (diagonal and -x, -y, -z face weights), one launch per iteration: FP64
 // (mixed = false, the plain path) or FP32 refined in FP64 (mixed = true, the
 // same contract as solveMixed).  The solution lands in K.x (compact).
+// solveBoxPcg's answer when the tiles holding unknowns cover too much of
+// the box (the caller solves on compact rows instead)
+constexpr int kBoxTooSparse = 1;
 int solveBoxPcg(int D, bool mixed, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const BoxLayout &L,
 	const garm_pcg_opts &o, Krylov &K, BoxKrylov &B, hipStream_t s, garm_pcg_stats *st, std::string &err);
 

@@ -359,18 +359,19 @@ static int projectD(Fields &F, Projection &P, const garm_pcg_opts &o, garm_pcg_s
 				rc = GARM_EHIP;
 			}
 		} else {
-			// the box layout while the box holds at most kBoxSolverRatio cells
-			// per unknown; a sparser box (a splash's bounding box is mostly air,
-			// and every vector operation covers it) keeps the rows
+			// the box layout while the box holds at most kBoxSolverRatio2/3
+			// cells per unknown and (3D) the solver takes it; a sparser box
+			// keeps the compact rows
 			const double boxCells = double(P.box_n[0]) * P.box_n[1] * P.box_n[2];
-			P.used_box = boxCells <= kBoxSolverRatio * n;
+			P.used_box = boxCells <= (D == 3 ? kBoxSolverRatio3 : kBoxSolverRatio2) * n;
 			if (P.used_box) {
 				BoxLayout L{};
 				for (int a = 0; a < 3; a++) { L.lo[a] = P.box_lo[a]; L.n[a] = P.box_n[a]; L.gn[a] = F.cell.n[a]; }
 				L.mat2grid = P.mat2grid;
 				rc = solveBoxPcg(D, mixed, P.rows, P.rows32, P.rhs, n, L, o, P.K, P.BK, s, st, err);
-			} else if (mixed) rc = solveMixed(D, P.rows, P.rows32, P.rhs, n, o, P.K, s, st, err);
-			else rc = solvePcg64(D, P.rows, P.rhs, n, o, P.K, s, st, err);
+				if (rc == kBoxTooSparse) P.used_box = false;
+			}
+			if (!P.used_box) rc = mixed ? solveMixed(D, P.rows, P.rows32, P.rhs, n, o, P.K, s, st, err) : solvePcg64(D, P.rows, P.rhs, n, o, P.K, s, st, err);
 		}
 	}
 	// the velocity is updated with whatever the solver reached (the reference

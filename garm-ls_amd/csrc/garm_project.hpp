@@ -17,8 +17,10 @@ struct FaceFracs {
 };
 
 // Largest ratio of box cells to unknowns at which the Jacobi PCG solvers
-// run on the box of the unknowns instead of on compact rows.
-constexpr double kBoxSolverRatio = 1.5;
+// try the box of the unknowns instead of compact rows; in 3D the box solver
+// then launches only the column tiles holding unknowns and declines when
+// those cover too much more than the unknowns.
+constexpr double kBoxSolverRatio2 = 1.5, kBoxSolverRatio3 = 16.0;
 
 struct Projection {
 	uint8_t *flags = nullptr;                  // 1 where a cell is an unknown
