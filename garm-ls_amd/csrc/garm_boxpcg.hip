@@ -148,8 +148,9 @@ __global__ void kBoxCells(const int *mat2grid, int nu, int gx, int gy, int ox, i
 
 // Column tiles holding unknowns: a byte per tile (same-value stores), then
 // their ordered list, compacted by one workgroup (the list order fixes the
-// order of the partial sums, so it must not depend on timing); the count
-// follows the list.
+// order of the partial sums, so it must not depend on timing); the list's
+// length and the count of box cells its tiles cover follow it, at
+// kColPartialsMax and kColPartialsMax + 1.
 __global__ void kTileMark(const unsigned *boxOf, int nu, int bx, int by, int ty, int zch, int gx, int gy, unsigned char *mask)
 {
 	for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nu; r += gridDim.x * blockDim.x) {
@@ -159,16 +160,24 @@ __global__ void kTileMark(const unsigned *boxOf, int nu, int bx, int by, int ty,
 	}
 }
 
-__global__ __launch_bounds__(kBlock) void kTileList(const unsigned char *mask, int nt, int *tiles)
+__global__ __launch_bounds__(kBlock) void kTileList(const unsigned char *mask, int nt, int gx, int gy, int bx, int by, int bz, int ty, int zch,
+	int *tiles)
 {
 	const int per = (nt + kBlock - 1) / kBlock, t0 = int(threadIdx.x) * per, t1 = min(nt, t0 + per);
-	int cnt = 0;
-	for (int t = t0; t < t1; t++) cnt += mask[t] != 0;
-	int total;
+	int cnt = 0, cells = 0;
+	for (int t = t0; t < t1; t++)
+		if (mask[t]) {
+			cnt++;
+			// the tile's cells inside the box
+			const int x = t % gx, y = (t / gx) % gy, z = t / (gx * gy);
+			cells += min(64, bx - 64 * x) * min(ty, by - ty * y) * min(zch, bz - zch * z);
+		}
+	int total, allCells;
 	int at = blockExclusiveScan(cnt, &total);
+	(void)blockExclusiveScan(cells, &allCells);
 	for (int t = t0; t < t1; t++)
 		if (mask[t]) tiles[at++] = t;
-	if (threadIdx.x == 0) tiles[kColPartialsMax] = total;
+	if (threadIdx.x == 0) { tiles[kColPartialsMax] = total; tiles[kColPartialsMax + 1] = allCells; }
 }
 
 __global__ void kZeroWords(double *p, size_t words)
@@ -531,6 +540,8 @@ BoxKrylov::~BoxKrylov()
 	if (pool) (void)hipFree(pool);
 	if (partials) (void)hipFree(partials);
 	if (boxOf) (void)hipFree(boxOf);
+	if (tiles) (void)hipFree(tiles);
+	if (tileMask) (void)hipFree(tileMask);
 }
 
 static int readBoxState(Krylov &K, hipStream_t s, std::string &err)
@@ -559,7 +570,7 @@ static int boxSolve(const Rows<double> &A64, const Rows<T> &A, const double *b, 
 	const size_t words = tWords + 2 * span + (mixed ? 4 * span : 0);
 	constexpr size_t kPartWords = 6 * kColPartials;   // one parity's partials
 	if (!B.partials && hipMalloc(&B.partials, 2 * kPartWords * sizeof(double)) != hipSuccess) { err = "solver partials allocation failed"; return GARM_ENOMEM; }
-	if (!B.tiles && (hipMalloc(&B.tiles, (kColPartialsMax + 1) * sizeof(int)) != hipSuccess ||
+	if (!B.tiles && (hipMalloc(&B.tiles, (kColPartialsMax + 2) * sizeof(int)) != hipSuccess ||
 			hipMalloc(&B.tileMask, kColPartialsMax) != hipSuccess)) { err = "solver tile list allocation failed"; return GARM_ENOMEM; }
 	if (size_t(nu) > B.ocap) {
 		if (B.boxOf) (void)hipFree(B.boxOf);
@@ -586,19 +597,20 @@ static int boxSolve(const Rows<double> &A64, const Rows<T> &A, const double *b, 
 	}
 	// the tiles holding unknowns, in order; only they are launched (a
 	// splash's box is mostly empty)
-	int ntiles = 0;
+	int ntiles = 0, activeCells = int(cells);
 	if (zch) {
 		const int nt = gx * gy * gz;
 		if (hipMemsetAsync(B.tileMask, 0, nt, s) != hipSuccess) { err = "solver tile mask clear failed"; return GARM_EHIP; }
 		hipLaunchKernelGGL(kTileMark, dim3(nbU), b1, 0, s, (const unsigned *)B.boxOf, nu, bx, by, ty, zch, gx, gy, B.tileMask);
-		hipLaunchKernelGGL(kTileList, dim3(1), b1, 0, s, (const unsigned char *)B.tileMask, nt, B.tiles);
-		if (hipMemcpyAsync(K.host_state + 8, B.tiles + kColPartialsMax, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+		hipLaunchKernelGGL(kTileList, dim3(1), b1, 0, s, (const unsigned char *)B.tileMask, nt, gx, gy, bx, by, bz, ty, zch, B.tiles);
+		if (hipMemcpyAsync(K.host_state + 8, B.tiles + kColPartialsMax, 2 * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
 			spinWait(s, K.ready) != hipSuccess) { err = "solver tile count readback failed"; return GARM_EHIP; }
 		ntiles = K.host_state[8];
+		activeCells = K.host_state[9];
 		// too sparse for the box: the compact rows cost less per iteration
-		// (per cell of an active tile against per unknown, at 256^3: FP64
-		// 22.6 against 37 ns, FP32 13.2 against 27 ns)
-		if (double(ntiles) * kColX * ty * zch > (mixed ? kBoxActiveRatio32 : kBoxActiveRatio64) * nu) return kBoxTooSparse;
+		// (per box cell of an active tile against per unknown, at 256^3:
+		// FP64 22.6 against 37 ns, FP32 13.2 against 27 ns)
+		if (double(activeCells) > (mixed ? kBoxActiveRatio32 : kBoxActiveRatio64) * nu) return kBoxTooSparse;
 	}
 	if (words > B.cap) {
 		if (B.pool) (void)hipFree(B.pool);
@@ -700,8 +712,8 @@ static int boxSolve(const Rows<double> &A64, const Rows<T> &A, const double *b, 
 	const double iterBytes = 14 * ts;
 	const double startBytes = 5 * ts;
 	const double replBytes = 48 + ts + (mixed ? 16 + 2 * ts : 0.0) + 8 * ts;
-	const double iterCells = zch ? std::min(double(cells), double(ntiles) * kColX * ty * zch) : double(cells);
-	st->hbm_bytes = iterCells * iterBytes * st->iterations + double(cells) * (startBytes + replBytes * replacements);
+	st->hbm_bytes = double(activeCells) * iterBytes * st->iterations + double(cells) * (startBytes + replBytes * replacements);
+	st->solver_cells = activeCells;
 	if (!K.host_state[kStOuterDone]) {
 		err = std::string(mixed ? "mixed-precision solve" : "PCG") + " did not reach the requested tolerance (" + std::to_string(st->iterations) +
 			" iterations, " + std::to_string(replacements) + " replacements, relative residual " + std::to_string(st->relative_residual) + ")";

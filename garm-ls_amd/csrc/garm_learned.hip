@@ -633,6 +633,7 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 	st->fell_back = 0;
 	st->preconditioner_applications = 0;
 	st->hbm_bytes = 0;
+	st->solver_cells = N;
 	if (bb == 0) {
 		hipLaunchKernelGGL(kZeroD, g1, b1, 0, s, W.x, n);
 		st->relative_residual = 0;

@@ -390,6 +390,7 @@ int solvePcg64(int D, const Rows<double> &A, const double *b, int n, const garm_
 {
 	const int rc = D == 2 ? pcg64<2>(A, b, n, o, K, s, st, err) : pcg64<3>(A, b, n, o, K, s, st, err);
 	st->hbm_bytes = kPcg64IterBytes * double(n) * st->iterations;
+	st->solver_cells = n;
 	return rc;
 }
 
@@ -398,6 +399,7 @@ int solveMixed(int D, const Rows<double> &A64, const Rows<float> &A32, const dou
 {
 	const int rc = D == 2 ? mixed<2>(A64, A32, b, n, o, K, s, st, err) : mixed<3>(A64, A32, b, n, o, K, s, st, err);
 	st->hbm_bytes = (kPcg32IterBytes * st->iterations + kReplaceBytes * st->outer_iterations) * double(n);
+	st->solver_cells = n;
 	return rc;
 }
 

@@ -67,7 +67,7 @@ struct BoxKrylov {
 	unsigned *boxOf = nullptr; // box cell of every unknown
 	size_t ocap = 0;
 	double *partials = nullptr;  // per-workgroup partials of the z-walking iteration
-	int *tiles = nullptr;        // active column tiles (and their count after them)
+	int *tiles = nullptr;        // active column tiles, then their count and their box cells
 	unsigned char *tileMask = nullptr;
 	~BoxKrylov();
 };

@@ -348,6 +348,7 @@ static int projectD(Fields &F, Projection &P, const garm_pcg_opts &o, garm_pcg_s
 	st->preconditioner_applications = 0;
 	st->relative_residual = 0;
 	st->hbm_bytes = 0;
+	st->solver_cells = 0;
 	int rc = GARM_OK;
 	if (n > 0) {
 		assemble<D>(F, P, n, mixed, s);

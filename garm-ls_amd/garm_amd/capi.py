@@ -50,7 +50,8 @@ class PcgOpts(ctypes.Structure):
 class PcgStats(ctypes.Structure):
     _fields_ = [("unknowns", ctypes.c_int), ("iterations", ctypes.c_int), ("outer_iterations", ctypes.c_int),
                 ("fell_back", ctypes.c_int), ("relative_residual", ctypes.c_double),
-                ("preconditioner_applications", ctypes.c_int), ("hbm_bytes", ctypes.c_double)]
+                ("preconditioner_applications", ctypes.c_int), ("hbm_bytes", ctypes.c_double),
+                ("solver_cells", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -111,6 +111,9 @@ typedef struct garm_pcg_stats {
 	double hbm_bytes;        /* algorithmic HBM bytes of the solver's iterations (each array a
 	                            kernel must touch read or written once per launch; the
 	                            network's own activations are excluded: it is bound by MFMA) */
+	int solver_cells;        /* cells an iteration covers: the unknowns (compact rows), the
+	                            unknowns' bounding box, or (3D Jacobi PCG on the box) the
+	                            box cells of the column tiles that hold unknowns */
 } garm_pcg_stats;
 
 /* Precision of advection and reinitialisation. */
