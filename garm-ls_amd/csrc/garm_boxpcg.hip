@@ -23,7 +23,9 @@
 // for another workgroup), and writes its own partials.  r, w and s alternate
 // between two buffers (launch k reads set k & 1 and writes the other), p and
 // x are updated in place.  3D boxes are walked in z-columns with each plane
-// of u' shared through LDS.
+// of u' shared through LDS, one workgroup per column tile that holds
+// unknowns (a splash's bounding box is mostly empty; the other tiles stay
+// zero in every vector and are not launched).
 //
 // The recursive residual of this form can drift further from b - A x than
 // plain CG's, so a solve ends only on the true residual: when the recursive
