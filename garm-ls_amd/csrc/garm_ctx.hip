@@ -230,6 +230,7 @@ int rebuildBoundaryFaces(garm_ctx *c)
 	}
 	F.nbface = n;
 	launchBoundaryRecords(F, c->s);
+	launchOpenFaces(F, c->s);
 	if ((rc = launched(c, "boundary faces"))) return rc;
 	return hipCheck(c, hipStreamSynchronize(c->s), "boundary faces");
 }
@@ -305,7 +306,8 @@ int garm_create(garm_ctx **out, int dim, const int res[3], double dx, const doub
 	for (int a = 0; a < dim && ok; a++)
 		ok = alloc(c, F.phi_grad[a], N) && alloc(c, F.xi[a], N) && alloc(c, F.xi_new[a], N) && alloc(c, F.phi_ref_grad[a], N)
 			&& alloc(c, F.u[a], F.Nf[a]) && alloc(c, F.u_new[a], F.Nf[a]) && alloc(c, F.frac[a], F.Nf[a])
-			&& alloc(c, F.bvel[a], F.Nf[a]) && alloc(c, F.bnormal[a], F.Nf[a]) && alloc(c, F.layer_f[a], F.Nf[a] + 8);
+			&& alloc(c, F.bvel[a], F.Nf[a]) && alloc(c, F.bnormal[a], F.Nf[a]) && alloc(c, F.layer_f[a], F.Nf[a] + 8)
+			&& alloc(c, F.open_f[a], F.Nf[a] + 8);
 	{
 		// a front never lists an element twice, so the larger of the cell and
 		// face counts bounds every list

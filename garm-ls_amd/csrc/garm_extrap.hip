@@ -40,7 +40,8 @@ struct FixArgs {
 	const double *cellDist;
 	const uint8_t *adv;
 	const double *phiOri;
-	const double *frac[3];
+	const uint8_t *open[3];    // 1 where the face fraction is < 1
+	const uint8_t *liquid;     // if set: 0 exactly where phi_ori <= 0 (the marked advRegion front)
 };
 
 template <int D>
@@ -57,10 +58,12 @@ __device__ __forceinline__ bool isFixed(const FixArgs &P, int a, const GridDev &
 	else if constexpr (RULE == kFixAdvRegion) return P.adv[id] > 0;           // LevelSetLiquid.cpp:1062-1064
 	else if constexpr (RULE == kFixPhiOriNonPositive) return P.phiOri[id] <= 0;   // LevelSetLiquid.cpp:1068-1070
 	else {                                                                    // LevelSetLiquid.cpp:1072-1083
-		if (!(P.frac[a][id] < 1)) return false;
+		if (!P.open[a][id]) return false;   // fraction < 1
 		const int i0 = i - (a == 0), j0 = j - (a == 1), k0 = k - (a == 2);
 		if (!validIn<D>(P.cell, i0, j0, k0) || !validIn<D>(P.cell, i, j, k)) return false;
-		return P.phiOri[gidx<D>(P.cell, i0, j0, k0)] <= 0 || P.phiOri[gidx<D>(P.cell, i, j, k)] <= 0;
+		const size_t c0 = gidx<D>(P.cell, i0, j0, k0), c1 = gidx<D>(P.cell, i, j, k);
+		if (P.liquid) return P.liquid[c0] == 0 || P.liquid[c1] == 0;
+		return P.phiOri[c0] <= 0 || P.phiOri[c1] <= 0;
 	}
 }
 
@@ -360,7 +363,8 @@ static FixArgs fixArgs(const Fields &F, int rule)
 	P.cellDist = F.cell_dist;
 	P.adv = F.adv_region;
 	P.phiOri = F.phi_ori;
-	for (int a = 0; a < F.D; a++) P.frac[a] = F.frac[a];
+	for (int a = 0; a < F.D; a++) P.open[a] = F.open_f[a];
+	P.liquid = nullptr;
 	return P;
 }
 
@@ -461,8 +465,12 @@ void launchExtrapolateAir(const Fields &F, hipStream_t s, bool velocityOnly)
 	for (int q = 0; q < nq; q++)
 		for (int a = 0; a < spaces[q]->ng; a++) {
 			const int f = q0 + q;   // the front's own lists and counters
-			if (F.D == 2) launchFrontInit<2>(*spaces[q], a, fixArgs(F, rules[q]), F.front_list[2 * f], cnt + 3 * f, s);
-			else launchFrontInit<3>(*spaces[q], a, fixArgs(F, rules[q]), F.front_list[2 * f], cnt + 3 * f, s);
+			FixArgs P = fixArgs(F, rules[q]);
+			// the face front's rule reads the advRegion front's marks (marked
+			// just before, 0 exactly where phi_ori <= 0) instead of phi_ori
+			if (rules[q] == kFixLiquidFace && !velocityOnly) P.liquid = F.layer2;
+			if (F.D == 2) launchFrontInit<2>(*spaces[q], a, P, F.front_list[2 * f], cnt + 3 * f, s);
+			else launchFrontInit<3>(*spaces[q], a, P, F.front_list[2 * f], cnt + 3 * f, s);
 		}
 	for (int L = 1; L <= 17; L++) {
 		FrontJobs J{};

@@ -407,6 +407,17 @@ void launchResetRefMap(const Fields &F, hipStream_t s)
 
 void launchFill(double *p, double v, size_t n, hipStream_t s) { hipLaunchKernelGGL(kFill, dim3(linearBlocks(n)), dim3(kBlock), 0, s, p, v, n); }
 void launchFillU8(uint8_t *p, uint8_t v, size_t n, hipStream_t s) { hipLaunchKernelGGL(kFillU8, dim3(linearBlocks(n)), dim3(kBlock), 0, s, p, v, n); }
+
+__global__ void kOpenFaces(const double *frac, uint8_t *open, size_t n)
+{
+	for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) open[i] = frac[i] < 1 ? 1 : 0;
+}
+
+void launchOpenFaces(const Fields &F, hipStream_t s)
+{
+	for (int a = 0; a < F.D; a++)
+		hipLaunchKernelGGL(kOpenFaces, dim3(linearBlocks(F.Nf[a])), dim3(kBlock), 0, s, (const double *)F.frac[a], F.open_f[a], F.Nf[a]);
+}
 void launchCopy(double *d, const double *src, size_t n, hipStream_t s) { hipLaunchKernelGGL(kCopy, dim3(linearBlocks(n)), dim3(kBlock), 0, s, d, src, n); }
 
 void launchEnforceLevelSet(const Fields &F, double *sdf, double tol, hipStream_t s)

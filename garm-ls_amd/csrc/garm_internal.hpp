@@ -43,6 +43,7 @@ struct Fields {
 	double *scratch[2] = {};                          // two cell arrays (reinit stages)
 	uint8_t *layer = nullptr, *layer2 = nullptr;      // extrapolation layer index per cell (two fronts)
 	uint8_t *layer_f[3] = {};                         // ... per face
+	uint8_t *open_f[3] = {};                          // 1 where fraction < 1 (set with the boundary)
 	int *front_list[6] = {};                          // extrapolation fronts (element ids): two lists per front
 	int *front_cnt = nullptr;                         // three counters per front (9), one for the boundary faces, eight for the collider layers
 	uint8_t *coll_layer = nullptr;                    // collider extrapolation: layer per cell (fixed by cellDist)
@@ -88,6 +89,8 @@ void launchVolume(const Fields &F, bool corrected, hipStream_t s);
 void setContextFreeError(const std::string &msg);
 void launchFill(double *p, double v, size_t n, hipStream_t s);
 void launchFillU8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
+// open_f from the face fractions (after the boundary fields change)
+void launchOpenFaces(const Fields &F, hipStream_t s);
 void launchCopy(double *dst, const double *src, size_t n, hipStream_t s);
 void launchEnforceLevelSet(const Fields &F, double *sdf, double tol, hipStream_t s);
 void launchGravity(const Fields &F, double dt, hipStream_t s);
