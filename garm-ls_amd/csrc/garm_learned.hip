@@ -122,8 +122,11 @@ __global__ __launch_bounds__(kBlock) void kFcgSpmv(Rows<double> A, int n, const 
 	putPart<2>(v, pa);
 }
 
-// alpha = (r.p) / (p.q); x += alpha p; r -= alpha q; partial |r|^2
-__global__ __launch_bounds__(kBlock) void kFcgUpdate(int n, double *x, double *r, const double *p, const double *q, const double *pa, double *pb, double *ls)
+// alpha = (r.p) / (p.q); x += alpha p; r -= alpha q; partial |r|^2.  With
+// b32 (box layout, a V-cycle next), the new residual also goes to the
+// cycle's level-0 right-hand side and its iterate to 0 (the cycle's load).
+__global__ __launch_bounds__(kBlock) void kFcgUpdate(int n, double *x, double *r, const double *p, const double *q, const double *pa, double *pb, double *ls,
+	float *b32, float *x32)
 {
 	double t[2];
 	sumPart<2>(pa, gridDim.x, t);
@@ -137,6 +140,10 @@ __global__ __launch_bounds__(kBlock) void kFcgUpdate(int n, double *x, double *r
 		x[i] += a[0] * p[i];
 		const double ri = r[i] - a[0] * q[i];
 		r[i] = ri;
+		if (b32) {
+			b32[i] = float(ri);
+			x32[i] = 0.0f;
+		}
 		v[0] += ri * ri;
 	}
 	putPart<1>(v, pb);
@@ -511,6 +518,11 @@ static UnetBox networkBox(int D, const int lo[3], const int hi[3], const GridDev
 // is needed (it then only writes scratch the next head overwrites).
 struct Precond {
 	std::function<void()> head, rest;
+	// box layout, V-cycle: where the x/r update can write the cycle's
+	// load itself, the head that then only sweeps, and the rest whose
+	// store also forms the partials of z.q (each unset otherwise)
+	std::function<void(float **, float **)> loadTargets;
+	std::function<void()> headLoaded, restDot;
 	double bytes = 0;   // algorithmic HBM bytes of one application
 	void full() const { if (head) head(); rest(); }
 };
@@ -588,6 +600,15 @@ static int boxSpace(const Rows<double> &A64, int n, const int *mat2grid, const G
 	return GARM_OK;
 }
 
+// The V-cycle's hooks for the fused box-layout iteration (flexibleCg).
+static void fuseCycle(Precond &P, Multigrid &M, int n, FcgSpace &V, LearnedWork &W, hipStream_t s)
+{
+	if (!V.box) return;
+	P.loadTargets = [&M](float **b32, float **x32) { mgLevel0(M, b32, x32); };
+	P.headLoaded = [&M, n, &V, s]() { mgApplyHead(M, n, V.r, s, true, true); };
+	P.restDot = [&M, n, &V, &W, s]() { mgApplyRest(M, n, V.z, s, true, V.q, W.pa, lblocks(V.nv)); };
+}
+
 // Flexible CG on the FP64 system, on the vectors of V, with preconditioner
 // `precondition` (writes V.z = M(V.r)).  Stall rules (CG minimises the
 // error in the A-norm, not the residual, so a single iteration may leave
@@ -648,12 +669,19 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 	for (; it < budget; it++) {
 		if (V.box) hipLaunchKernelGGL(kBoxSpmv<D>, gb, b1, 0, s, V.O, (const double *)V.p, V.q, (const double *)V.r, W.pa);
 		else hipLaunchKernelGGL(kFcgSpmv<D>, gb, b1, 0, s, A64, n, (const double *)V.p, V.q, (const double *)V.r, W.pa);
-		hipLaunchKernelGGL(kFcgUpdate, gb, b1, 0, s, N, V.x, V.r, (const double *)V.p, (const double *)V.q, (const double *)W.pa, W.pb, W.ls);
+		// a V-cycle on the box takes its load from the update and forms z.q
+		// in its store (the same values as its own load and the dot product)
+		const Precond *fused = V.box && precondition->loadTargets ? precondition : nullptr;
+		float *b32 = nullptr, *x32 = nullptr;
+		if (fused) fused->loadTargets(&b32, &x32);
+		hipLaunchKernelGGL(kFcgUpdate, gb, b1, 0, s, N, V.x, V.r, (const double *)V.p, (const double *)V.q, (const double *)W.pa, W.pb, W.ls,
+			b32, x32);
 		hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nbb, W.ls + kLsRr);
 		if ((rc = readStart(W, W.ls, kLsEnergy + 1, s, err))) return rc;
 		// the next application's head keeps the GPU busy while the host
 		// reads the residual and decides
-		if (precondition->head) precondition->head();
+		if (fused) fused->headLoaded();
+		else if (precondition->head) precondition->head();
 		if ((rc = readWait(W, err))) return rc;
 		const double rr = W.host[kLsRr];
 		if (!std::isfinite(rr)) break;
@@ -677,8 +705,10 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 			switchedAt = int(hist.size()) - 1;
 			st->fell_back = 1;
 			precondition->full();
-		} else precondition->rest();
-		hipLaunchKernelGGL(kFcgDot, gb, b1, 0, s, N, (const double *)V.z, (const double *)V.q, W.pa);
+			fused = nullptr;
+		} else if (fused) fused->restDot();
+		else precondition->rest();
+		if (!fused) hipLaunchKernelGGL(kFcgDot, gb, b1, 0, s, N, (const double *)V.z, (const double *)V.q, W.pa);
 		hipLaunchKernelGGL(kFcgP, gb, b1, 0, s, N, V.p, (const double *)V.z, (const double *)W.pa, (const double *)W.ls);
 	}
 	st->iterations = it;
@@ -788,6 +818,7 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 		mgBuild(M, A32, n, mat2grid, s);
 		mg.head = [&]() { mgApplyHead(M, n, V.r, s, V.box); };
 		mg.rest = [&]() { mgApplyRest(M, n, V.z, s, V.box); };
+		fuseCycle(mg, M, n, V, W, s);
 		mg.bytes = mgCycleBytes(M);
 		second = &mg;
 	}
@@ -817,6 +848,7 @@ static int multigrid(const Rows<double> &A64, const Rows<float> &A32, const doub
 	Precond mg;
 	mg.head = [&]() { mgApplyHead(M, n, V.r, s, V.box); };
 	mg.rest = [&]() { mgApplyRest(M, n, V.z, s, V.box); };
+	fuseCycle(mg, M, n, V, W, s);
 	mg.bytes = mgCycleBytes(M);
 	return flexibleCg<D>(A64, A32, b, n, V, o, K, W, s, st, err, mg, false, o.max_iterations, nullptr, "multigrid solve");
 }

@@ -576,6 +576,22 @@ __global__ void kMgStoreBox(int cells, MgLevel L, double *z)
 	for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += gridDim.x * blockDim.x) z[c] = double(L.x[c]);
 }
 
+// The same store with the per-workgroup partials of z.q (the flexible CG's
+// next inner product): the launch's grid and each thread's order are those
+// of the CG's own dot-product kernel, so the partials are the same.
+__global__ __launch_bounds__(kBlock) void kMgStoreBoxDot(int cells, MgLevel L, double *z, const double *q, double *pa)
+{
+	double v[1] = {0};
+	for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += gridDim.x * blockDim.x) {
+		const double zc = double(L.x[c]);
+		z[c] = zc;
+		v[0] += zc * q[c];
+	}
+	double blk[1];
+	blockSum<1>(v, blk);
+	if (threadIdx.x == 0) pa[blockIdx.x] = blk[0];
+}
+
 // ------------------------------------------------------------------ host
 static unsigned mgBlocks(size_t n) { return unsigned(std::max<size_t>(1, std::min<size_t>((n + kBlock - 1) / kBlock, 4096))); }
 
@@ -744,15 +760,16 @@ static void sweep(Multigrid &M, int l, int first, hipStream_t s)
 // The V-cycle in two parts: the head (load of the residual and the first
 // level-0 sweep) and the rest; together they are one cycle.
 template <int D>
-static void vcycleHead(Multigrid &M, int n, const double *r, bool box, hipStream_t s)
+static void vcycleHead(Multigrid &M, int n, const double *r, bool box, bool loaded, hipStream_t s)
 {
-	if (box) hipLaunchKernelGGL(kMgLoadBox, dim3(mgBlocks(M.cells[0])), dim3(kBlock), 0, s, int(M.cells[0]), level(M, 0), r);
-	else hipLaunchKernelGGL(kMgLoad, dim3(mgBlocks(size_t(n))), dim3(kBlock), 0, s, n, (const unsigned *)M.box_of, level(M, 0), r);
+	if (box) {
+		if (!loaded) hipLaunchKernelGGL(kMgLoadBox, dim3(mgBlocks(M.cells[0])), dim3(kBlock), 0, s, int(M.cells[0]), level(M, 0), r);
+	} else hipLaunchKernelGGL(kMgLoad, dim3(mgBlocks(size_t(n))), dim3(kBlock), 0, s, n, (const unsigned *)M.box_of, level(M, 0), r);
 	if (M.tail_first > 0) sweep<D>(M, 0, 0, s);
 }
 
 template <int D>
-static void vcycleRest(Multigrid &M, int n, double *z, bool box, hipStream_t s)
+static void vcycleRest(Multigrid &M, int n, double *z, bool box, const double *q, double *zq, int zqBlocks, hipStream_t s)
 {
 	const int first = M.tail_first;
 	for (int l = 0; l < first; l++) {
@@ -768,20 +785,27 @@ static void vcycleRest(Multigrid &M, int n, double *z, bool box, hipStream_t s)
 		hipLaunchKernelGGL(kMgProlong<D>, gp.grid, gp.block, 0, s, level(M, l), level(M, l + 1));
 		for (int v = 0; v < (l == 0 ? kMgSweeps0 : kMgSweeps); v++) sweep<D>(M, l, 1, s);
 	}
-	if (box) hipLaunchKernelGGL(kMgStoreBox, dim3(mgBlocks(M.cells[0])), dim3(kBlock), 0, s, int(M.cells[0]), level(M, 0), z);
+	if (box && q) hipLaunchKernelGGL(kMgStoreBoxDot, dim3(unsigned(zqBlocks)), dim3(kBlock), 0, s, int(M.cells[0]), level(M, 0), z, q, zq);
+	else if (box) hipLaunchKernelGGL(kMgStoreBox, dim3(mgBlocks(M.cells[0])), dim3(kBlock), 0, s, int(M.cells[0]), level(M, 0), z);
 	else hipLaunchKernelGGL(kMgStore, dim3(mgBlocks(size_t(n))), dim3(kBlock), 0, s, n, (const unsigned *)M.box_of, level(M, 0), z);
 }
 
-void mgApplyHead(Multigrid &M, int n, const double *r, hipStream_t s, bool box)
+void mgApplyHead(Multigrid &M, int n, const double *r, hipStream_t s, bool box, bool loaded)
 {
-	if (M.D == 2) vcycleHead<2>(M, n, r, box, s);
-	else vcycleHead<3>(M, n, r, box, s);
+	if (M.D == 2) vcycleHead<2>(M, n, r, box, loaded, s);
+	else vcycleHead<3>(M, n, r, box, loaded, s);
 }
 
-void mgApplyRest(Multigrid &M, int n, double *z, hipStream_t s, bool box)
+void mgApplyRest(Multigrid &M, int n, double *z, hipStream_t s, bool box, const double *q, double *zq, int zqBlocks)
 {
-	if (M.D == 2) vcycleRest<2>(M, n, z, box, s);
-	else vcycleRest<3>(M, n, z, box, s);
+	if (M.D == 2) vcycleRest<2>(M, n, z, box, q, zq, zqBlocks, s);
+	else vcycleRest<3>(M, n, z, box, q, zq, zqBlocks, s);
+}
+
+void mgLevel0(const Multigrid &M, float **b, float **x)
+{
+	*b = M.b[0];
+	*x = M.x[0];
 }
 
 // Algorithmic HBM bytes of one V-cycle: the residual's load and the

@@ -63,8 +63,16 @@ void mgApply(Multigrid &M, int n, const double *r, double *z, hipStream_t s);
 // whether the cycle is needed; a head without its rest leaves nothing that
 // the next head does not overwrite.  With `box`, r and z are laid out on
 // level 0's box (its cells, zero outside the unknowns) instead of compact.
-void mgApplyHead(Multigrid &M, int n, const double *r, hipStream_t s, bool box = false);
-void mgApplyRest(Multigrid &M, int n, double *z, hipStream_t s, bool box = false);
+// With `loaded` (box only), level 0's right-hand side and iterate already
+// hold r and 0 (the caller wrote them where mgLevel0 says) and the head only
+// sweeps; with q (box only), the rest's store also writes the zqBlocks
+// per-workgroup partials of z.q into zq, in the order of a grid-stride dot
+// product over zqBlocks workgroups of kBlock threads.
+void mgApplyHead(Multigrid &M, int n, const double *r, hipStream_t s, bool box = false, bool loaded = false);
+void mgApplyRest(Multigrid &M, int n, double *z, hipStream_t s, bool box = false, const double *q = nullptr, double *zq = nullptr,
+	int zqBlocks = 0);
+// Where the next cycle reads its right-hand side and starts its iterate on level 0.
+void mgLevel0(const Multigrid &M, float **b, float **x);
 // Algorithmic HBM bytes of one cycle (its arrays read or written once per launch).
 double mgCycleBytes(const Multigrid &M);
 
