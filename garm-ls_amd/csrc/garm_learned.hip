@@ -647,24 +647,28 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 	} else hipLaunchKernelGGL(kCopyD, g1, b1, 0, s, V.r, b, n);
 	hipLaunchKernelGGL(kFcgDot, g1, b1, 0, s, n, b, b, W.pb);
 	hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nb, W.ls + kLsBb);
-	if ((rc = readHost(W, W.ls, 4, s, err))) return rc;
-	const double bb = W.host[kLsBb];
+	// |b|^2 is read once the first iteration is enqueued (everything before
+	// the first convergence check is finite for b = 0: the updates' step
+	// lengths are 0 where p.Ap is, and a zero right-hand side returns x = 0)
+	if ((rc = readStart(W, W.ls, 4, s, err))) return rc;
 	st->iterations = 0;
 	st->outer_iterations = 0;
 	st->fell_back = 0;
 	st->preconditioner_applications = 0;
 	st->hbm_bytes = 0;
 	st->solver_cells = N;
-	if (bb == 0) {
-		hipLaunchKernelGGL(kZeroD, g1, b1, 0, s, W.x, n);
-		st->relative_residual = 0;
+	double bb = -1, tol = 0, thr = 0;
+	std::vector<double> hist, energy;
+	auto readNorm = [&]() -> int {
+		if (int e = readWait(W, err)) return e;
+		bb = W.host[kLsBb];
+		tol = std::max(o.tolerance, 2.220446049250313e-16 / std::sqrt(bb));
+		thr = tol * tol * bb;
+		hist.assign(1, bb);
 		return GARM_OK;
-	}
-	const double tol = std::max(o.tolerance, 2.220446049250313e-16 / std::sqrt(bb));
-	const double thr = tol * tol * bb;
+	};
 	precondition->full();
 	hipLaunchKernelGGL(kCopyD, gb, b1, 0, s, V.p, (const double *)V.z, N);
-	std::vector<double> hist(1, bb), energy;
 	int it = 0;
 	for (; it < budget; it++) {
 		if (V.box) hipLaunchKernelGGL(kBoxSpmv<D>, gb, b1, 0, s, V.O, (const double *)V.p, V.q, (const double *)V.r, W.pa);
@@ -677,6 +681,11 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 		hipLaunchKernelGGL(kFcgUpdate, gb, b1, 0, s, N, V.x, V.r, (const double *)V.p, (const double *)V.q, (const double *)W.pa, W.pb, W.ls,
 			b32, x32);
 		hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nbb, W.ls + kLsRr);
+		if (bb < 0) {
+			// the norm's readback shares the slot with the iteration's
+			if ((rc = readNorm())) return rc;
+			if (bb == 0) break;
+		}
 		if ((rc = readStart(W, W.ls, kLsEnergy + 1, s, err))) return rc;
 		// the next application's head keeps the GPU busy while the host
 		// reads the residual and decides
@@ -710,6 +719,12 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 		else precondition->rest();
 		if (!fused) hipLaunchKernelGGL(kFcgDot, gb, b1, 0, s, N, (const double *)V.z, (const double *)V.q, W.pa);
 		hipLaunchKernelGGL(kFcgP, gb, b1, 0, s, N, V.p, (const double *)V.z, (const double *)W.pa, (const double *)W.ls);
+	}
+	if (bb < 0 && (rc = readNorm())) return rc;
+	if (bb == 0) {
+		hipLaunchKernelGGL(kZeroD, g1, b1, 0, s, W.x, n);
+		st->relative_residual = 0;
+		return GARM_OK;
 	}
 	st->iterations = it;
 	st->preconditioner_applications = switchedAt < 0 ? it : switchedAt;
@@ -753,13 +768,13 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 
 template <int D>
 static int learned(const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *grid2mat, const int *mat2grid,
-	const GridDev &cell, const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Unet &U, Multigrid &M, hipStream_t s, garm_pcg_stats *st,
-	std::string &err)
+	const GridDev &cell, const int boxLo[3], const int boxN[3], const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Unet &U, Multigrid &M,
+	hipStream_t s, garm_pcg_stats *st, std::string &err)
 {
 	if (!W.reserve(size_t(n), err)) return GARM_ENOMEM;
 	const dim3 b1(kBlock);
 	int rc, lo[3], hi[3];
-	if ((rc = unknownBox(mat2grid, n, cell, W, s, lo, hi, err))) return rc;
+	for (int a = 0; a < 3; a++) { lo[a] = boxLo[a]; hi[a] = boxLo[a] + boxN[a] - 1; }
 	const UnetBox B = networkBox(D, lo, hi, cell);
 	const size_t boxVox = size_t(B.n[0]) * B.n[1] * B.n[2];
 	// the buffers are reallocated only when the box outgrows them, then for
@@ -827,11 +842,12 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 
 template <int D>
 static int multigrid(const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *mat2grid, const GridDev &cell,
-	const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Multigrid &M, hipStream_t s, garm_pcg_stats *st, std::string &err)
+	const int boxLo[3], const int boxN[3], const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Multigrid &M, hipStream_t s,
+	garm_pcg_stats *st, std::string &err)
 {
 	if (!W.reserve(size_t(n), err)) return GARM_ENOMEM;
 	int rc, lo[3], hi[3];
-	if ((rc = unknownBox(mat2grid, n, cell, W, s, lo, hi, err))) return rc;
+	for (int a = 0; a < 3; a++) { lo[a] = boxLo[a]; hi[a] = boxLo[a] + boxN[a] - 1; }
 	int org[3], dims[3], gn[3];
 	for (int a = 0; a < 3; a++) {
 		const bool live = a < D;
@@ -854,18 +870,19 @@ static int multigrid(const Rows<double> &A64, const Rows<float> &A32, const doub
 }
 
 int solveLearned(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *grid2mat, const int *mat2grid,
-	const GridDev &cell, const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Unet &U, Multigrid &M, hipStream_t s, garm_pcg_stats *st,
-	std::string &err)
+	const GridDev &cell, const int boxLo[3], const int boxN[3], const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Unet &U, Multigrid &M,
+	hipStream_t s, garm_pcg_stats *st, std::string &err)
 {
-	return D == 2 ? learned<2>(A64, A32, b, n, grid2mat, mat2grid, cell, o, K, W, U, M, s, st, err)
-	              : learned<3>(A64, A32, b, n, grid2mat, mat2grid, cell, o, K, W, U, M, s, st, err);
+	return D == 2 ? learned<2>(A64, A32, b, n, grid2mat, mat2grid, cell, boxLo, boxN, o, K, W, U, M, s, st, err)
+	              : learned<3>(A64, A32, b, n, grid2mat, mat2grid, cell, boxLo, boxN, o, K, W, U, M, s, st, err);
 }
 
 int solveMultigrid(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *mat2grid,
-	const GridDev &cell, const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Multigrid &M, hipStream_t s, garm_pcg_stats *st, std::string &err)
+	const GridDev &cell, const int boxLo[3], const int boxN[3], const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Multigrid &M,
+	hipStream_t s, garm_pcg_stats *st, std::string &err)
 {
-	return D == 2 ? multigrid<2>(A64, A32, b, n, mat2grid, cell, o, K, W, M, s, st, err)
-	              : multigrid<3>(A64, A32, b, n, mat2grid, cell, o, K, W, M, s, st, err);
+	return D == 2 ? multigrid<2>(A64, A32, b, n, mat2grid, cell, boxLo, boxN, o, K, W, M, s, st, err)
+	              : multigrid<3>(A64, A32, b, n, mat2grid, cell, boxLo, boxN, o, K, W, M, s, st, err);
 }
 
 // ------------------------------------------------------------- training

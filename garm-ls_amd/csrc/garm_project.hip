@@ -30,37 +30,48 @@ __global__ void kBoxInit(int *box)
 	if (threadIdx.x < 6) box[threadIdx.x] = INT_MAX;
 }
 
+// open[a] holds 1 for each face of axis a whose fraction is below 1.
+struct OpenFaces {
+	GridDev g[3];
+	const uint8_t *open[3];
+};
+
 template <int D>
-__global__ __launch_bounds__(kBlock) void kUnknownFlags(GridDev g, FaceFracs fr, const double *phi, uint8_t *flags, int *tileCount, size_t N,
+__global__ __launch_bounds__(kBlock) void kUnknownFlags(GridDev g, OpenFaces fo, const double *phi, uint8_t *flags, int *tileCount, size_t N,
 	int *box)
 {
 	const size_t base = size_t(blockIdx.x) * kScanTile;
 	int cnt = 0;
 	int bmin[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MAX, INT_MAX, INT_MAX};
+	// the thread's cell coordinates, divided out once and then carried
+	// forward by kBlock ids per pass
+	const size_t id0 = base + threadIdx.x;
+	const size_t row = id0 / size_t(g.n[0]);
+	int i = int(id0 - row * size_t(g.n[0]));
+	int j = D == 2 ? int(row) : int(row % size_t(g.n[1]));
+	int k = D == 2 ? 0 : int(row / size_t(g.n[1]));
 	for (int sub = 0; sub < kScanSub; sub++) {
 		const size_t id = base + size_t(sub) * kBlock + threadIdx.x;
 		if (id >= N) break;
 		int flag = 0;
 		if (phi[id] <= 0) {
-			const int i = int(id % size_t(g.n[0]));
-			const int j = D == 2 ? int(id / size_t(g.n[0])) : int((id / size_t(g.n[0])) % size_t(g.n[1]));
-			const int k = D == 2 ? 0 : int(id / (size_t(g.n[0]) * g.n[1]));
 #pragma unroll
 			for (int o = 0; o < 2 * D; o++) {
 				const int ax = o >> 1, up = o & 1;
-				const GridDev &fg = fr.g[ax];
-				const size_t f = gidx<D>(fg, i + (ax == 0 ? up : 0), j + (ax == 1 ? up : 0), k + (ax == 2 ? up : 0));
-				if (fr.a[ax][f] < 1) { flag = 1; break; }
+				const size_t f = gidx<D>(fo.g[ax], i + (ax == 0 ? up : 0), j + (ax == 1 ? up : 0), k + (ax == 2 ? up : 0));
+				if (fo.open[ax][f]) { flag = 1; break; }
 			}
 		}
 		flags[id] = uint8_t(flag);
 		cnt += flag;
 		if (flag) {
-			const int i = int(id % size_t(g.n[0]));
-			const int j = D == 2 ? int(id / size_t(g.n[0])) : int((id / size_t(g.n[0])) % size_t(g.n[1]));
-			const int k = D == 2 ? 0 : int(id / (size_t(g.n[0]) * g.n[1]));
 			bmin[0] = min(bmin[0], i); bmin[1] = min(bmin[1], j); bmin[2] = min(bmin[2], k);
 			bmin[3] = min(bmin[3], -i); bmin[4] = min(bmin[4], -j); bmin[5] = min(bmin[5], -k);
+		}
+		i += kBlock;
+		while (i >= g.n[0]) {
+			i -= g.n[0];
+			if (++j == g.n[1] && D == 3) { j = 0; k++; }
 		}
 	}
 	int total;
@@ -302,8 +313,9 @@ static int numberUnknowns(const Fields &F, Projection &P, bool mixed, hipStream_
 	// tile_base[tiles] receives the count, tile_base[tiles + 1 ..] the bounding box
 	int *box = P.tile_base + P.tiles + 1;
 	hipLaunchKernelGGL(kBoxInit, dim3(1), dim3(64), 0, s, box);
-	hipLaunchKernelGGL(kUnknownFlags<D>, dim3(P.tiles), dim3(kBlock), 0, s, F.cell, faceArrays(F, F.frac), (const double *)F.phi, P.flags,
-		P.tile_count, F.N, box);
+	OpenFaces fo{};
+	for (int a = 0; a < D; a++) { fo.g[a] = F.face[a]; fo.open[a] = F.open_f[a]; }
+	hipLaunchKernelGGL(kUnknownFlags<D>, dim3(P.tiles), dim3(kBlock), 0, s, F.cell, fo, (const double *)F.phi, P.flags, P.tile_count, F.N, box);
 	hipLaunchKernelGGL(kScanTiles, dim3(1), dim3(kBlock), 0, s, (const int *)P.tile_count, P.tiles, P.tile_base, P.tile_base + P.tiles);
 	int *cnt = F.host_ints;
 	if (hipMemcpyAsync(cnt, P.tile_base + P.tiles, 7 * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess || spinWait(s, F.wait) != hipSuccess) {
@@ -353,8 +365,9 @@ static int projectD(Fields &F, Projection &P, const garm_pcg_opts &o, garm_pcg_s
 	if (n > 0) {
 		assemble<D>(F, P, n, mixed, s);
 		if (learned || mg) {
-			rc = learned ? solveLearned(D, P.rows, P.rows32, P.rhs, n, P.grid2mat, P.mat2grid, F.cell, o, P.K, P.LW, *P.net, P.MG, s, st, err)
-			             : solveMultigrid(D, P.rows, P.rows32, P.rhs, n, P.mat2grid, F.cell, o, P.K, P.LW, P.MG, s, st, err);
+			rc = learned ? solveLearned(D, P.rows, P.rows32, P.rhs, n, P.grid2mat, P.mat2grid, F.cell, P.box_lo, P.box_n, o, P.K, P.LW, *P.net,
+			                            P.MG, s, st, err)
+			             : solveMultigrid(D, P.rows, P.rows32, P.rhs, n, P.mat2grid, F.cell, P.box_lo, P.box_n, o, P.K, P.LW, P.MG, s, st, err);
 			if ((rc == GARM_OK || rc == GARM_ENOCONV) && hipMemcpyAsync(P.K.x, P.LW.x, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, s) != hipSuccess) {
 				err = "copying the learned solution failed";
 				rc = GARM_EHIP;
