@@ -6,6 +6,7 @@ flexible CG with the V-cycle reaches a 1e-12 FP64 residual and the pressure
 of the plain FP64 Jacobi PCG, in a tenth of its iterations or fewer.  The
 reference answer is the GPU's FP64 PCG (the parity tests pin it to the
 oracle; a CPU solve at this size takes minutes)."""
+import numpy as np
 import pytest
 
 from garm_amd import capi, scenes
@@ -66,3 +67,31 @@ def test_multigrid_sparse_box_matches_fp64_pcg():
     assert e <= 1e-7
     ev = rel_linf(mg.get(capi.VELOCITY), ref.get(capi.VELOCITY))
     assert ev <= 1e-7, ev
+
+
+@pytest.mark.parametrize("mode", ["multigrid", "learned"])
+def test_flexible_solvers_zero_right_hand_side(mode):
+    """A liquid at rest without gravity: the right-hand side is exactly zero.
+    The flexible solvers enqueue their first iteration before they read |b|^2
+    back, so that iteration runs on b = 0 (its step lengths are 0 where p.Ap
+    is) and the solve must still return p = 0, no iterations and a zero
+    residual, and leave the velocity as it was.  The learned form runs an
+    untrained network, whose output on a zero residual is not zero (its
+    second input channel is the mask of the unknowns)."""
+    g = scenes.grid(32, 3)
+    say(f"zero right-hand side, {mode}, 3d n=32")
+    d = make_gpu(g)
+    scenes.dam_break(d, g, extents=(1.0, 0.5, 1.0))
+    d.initialize(capi.step_opts(mode=capi.PCG_FP64, tol=1e-10, max_iterations=20000))
+    u0 = d.get(capi.VELOCITY)
+    assert not np.any(u0)
+    kw = {}
+    if mode == "learned":
+        net = capi.Unet(seed=3)
+        d.attach_preconditioner(net)
+        kw = dict(fallback_mode=capi.PCG_MULTIGRID)
+    r = d.project(mode=capi.PCG_MULTIGRID if mode == "multigrid" else capi.PCG_LEARNED, tol=1e-8, max_iterations=100, **kw)
+    say(f"  unknowns {r['unknowns']}, iterations {r['iterations']}, residual {r['relative_residual']}")
+    assert r["unknowns"] > 0 and r["iterations"] == 0 and r["relative_residual"] == 0 and r["fell_back"] == 0
+    assert not np.any(d.get(capi.PRESSURE))
+    assert np.array_equal(d.get(capi.VELOCITY), u0)
