@@ -731,4 +731,15 @@ int solveBoxPcg(int D, bool mixed, const Rows<double> &A64, const Rows<float> &A
 	return D == 2 ? boxSolve<2, double>(A64, A64, b, n, L, o, K, B, s, st, err) : boxSolve<3, double>(A64, A64, b, n, L, o, K, B, s, st, err);
 }
 
+int solveJacobi(int D, bool mixed, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const BoxLayout &L,
+	const garm_pcg_opts &o, Krylov &K, BoxKrylov &B, hipStream_t s, garm_pcg_stats *st, std::string &err, bool *usedBox)
+{
+	const double cells = double(L.n[0]) * L.n[1] * L.n[2];
+	int rc = kBoxTooSparse;
+	if (cells <= (D == 3 ? kBoxSolverRatio3 : kBoxSolverRatio2) * n) rc = solveBoxPcg(D, mixed, A64, A32, b, n, L, o, K, B, s, st, err);
+	*usedBox = rc != kBoxTooSparse;
+	if (*usedBox) return rc;
+	return mixed ? solveMixed(D, A64, A32, b, n, o, K, s, st, err) : solvePcg64(D, A64, b, n, o, K, s, st, err);
+}
+
 } // namespace garm

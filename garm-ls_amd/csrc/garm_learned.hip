@@ -500,7 +500,7 @@ static int unknownBox(const int *mat2grid, int n, const GridDev &cell, LearnedWo
 
 // The network's grid for the unknowns in [lo, hi]: their bounding box with
 // two cells of context, sizes padded to multiples of four.
-static UnetBox networkBox(int D, const int lo[3], const int hi[3], const GridDev &cell)
+static UnetBox networkBox(int D, const int lo[3], const int hi[3], const int gn[3])
 {
 	UnetBox B{};
 	for (int a = 0; a < 3; a++) {
@@ -508,7 +508,7 @@ static UnetBox networkBox(int D, const int lo[3], const int hi[3], const GridDev
 		const int l = live ? lo[a] - 2 : 0, h = live ? hi[a] + 2 : 0;
 		B.org[a] = l;
 		B.n[a] = live ? (h - l + 1 + 3) / 4 * 4 : 4;
-		B.gn[a] = cell.n[a];
+		B.gn[a] = gn[a];
 	}
 	return B;
 }
@@ -568,7 +568,7 @@ static void compactSpace(LearnedWork &W, int n, FcgSpace &V)
 // Lays out the box of the unknowns (origin lo, sizes hi - lo + 1) in W:
 // the operator from the FP64 rows and zeroed vectors.
 template <int D>
-static int boxSpace(const Rows<double> &A64, int n, const int *mat2grid, const GridDev &cell, const int lo[3], const int hi[3],
+static int boxSpace(const Rows<double> &A64, int n, const int *mat2grid, const int gn[3], const int lo[3], const int hi[3],
 	LearnedWork &W, hipStream_t s, FcgSpace &V, std::string &err)
 {
 	int bn[3];
@@ -595,7 +595,7 @@ static int boxSpace(const Rows<double> &A64, int n, const int *mat2grid, const G
 	if (opWords > size_t(INT_MAX)) { err = "flexible-solver box too large"; return GARM_EINVAL; }
 	const dim3 gu(lblocks(n)), b1(kBlock);
 	hipLaunchKernelGGL(kZeroD, dim3(unsigned(std::min<size_t>((opWords + kBlock - 1) / kBlock, 8192))), b1, 0, s, W.bpool, int(opWords));
-	hipLaunchKernelGGL(kBoxIndex, gu, b1, 0, s, mat2grid, n, cell.n[0], cell.n[1], lo[0], lo[1], D == 3 ? lo[2] : 0, bn[0], bn[1], W.boxOf);
+	hipLaunchKernelGGL(kBoxIndex, gu, b1, 0, s, mat2grid, n, gn[0], gn[1], lo[0], lo[1], D == 3 ? lo[2] : 0, bn[0], bn[1], W.boxOf);
 	hipLaunchKernelGGL(kBoxOpBuild<D>, gu, b1, 0, s, A64, n, (const unsigned *)W.boxOf, arr[0], arr[1], arr[2], arr[3], arr[4]);
 	return GARM_OK;
 }
@@ -622,14 +622,15 @@ static void fuseCycle(Precond &P, Multigrid &M, int n, FcgSpace &V, LearnedWork 
 //    halve over every kStallWindow iterations.
 // After the loop the solution goes to the compact W.x, the FP64 residual
 // b - A x of the compact rows judges it, and what is left goes to the
-// mixed-precision Jacobi solver.
+// mixed-precision Jacobi solver (solveJacobi: on the box of the unknowns
+// where that form applies).
 constexpr int kStallWindow = 10;
 constexpr int kEnergyWindow = 1;
 constexpr double kEnergyRatio = 1.0 / 4.0;
 
 template <int D>
-static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, FcgSpace &V, const garm_pcg_opts &o,
-	Krylov &K, LearnedWork &W, hipStream_t s, garm_pcg_stats *st, std::string &err, const Precond &first, bool energyRule,
+static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, FcgSpace &V, const BoxLayout &L,
+	const garm_pcg_opts &o, Krylov &K, BoxKrylov &BK, LearnedWork &W, hipStream_t s, garm_pcg_stats *st, std::string &err, const Precond &first, bool energyRule,
 	int firstBudget, const Precond *second, const char *what)
 {
 	const Precond *precondition = &first;
@@ -750,7 +751,8 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 		fo.tolerance = std::sqrt(thr / rt) * 0.999;
 		fo.max_iterations = std::max(1, o.max_iterations - it);
 		garm_pcg_stats fs{};
-		rc = solveMixed(D, A64, A32, W.b2, n, fo, K, s, &fs, err);
+		bool boxed;
+		rc = solveJacobi(D, true, A64, A32, W.b2, n, L, fo, K, BK, s, &fs, err, &boxed);
 		if (rc != GARM_OK && rc != GARM_ENOCONV) return rc;
 		hipLaunchKernelGGL(kAddD, g1, b1, 0, s, W.x, (const double *)K.x, n);
 		st->iterations += fs.iterations;
@@ -767,15 +769,16 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 }
 
 template <int D>
-static int learned(const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *grid2mat, const int *mat2grid,
-	const GridDev &cell, const int boxLo[3], const int boxN[3], const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Unet &U, Multigrid &M,
-	hipStream_t s, garm_pcg_stats *st, std::string &err)
+static int learned(const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *grid2mat, const BoxLayout &L,
+	const garm_pcg_opts &o, Krylov &K, BoxKrylov &BK, LearnedWork &W, Unet &U, Multigrid &M, hipStream_t s, garm_pcg_stats *st,
+	std::string &err)
 {
 	if (!W.reserve(size_t(n), err)) return GARM_ENOMEM;
 	const dim3 b1(kBlock);
+	const int *mat2grid = L.mat2grid;
 	int rc, lo[3], hi[3];
-	for (int a = 0; a < 3; a++) { lo[a] = boxLo[a]; hi[a] = boxLo[a] + boxN[a] - 1; }
-	const UnetBox B = networkBox(D, lo, hi, cell);
+	for (int a = 0; a < 3; a++) { lo[a] = L.lo[a]; hi[a] = L.lo[a] + L.n[a] - 1; }
+	const UnetBox B = networkBox(D, lo, hi, L.gn);
 	const size_t boxVox = size_t(B.n[0]) * B.n[1] * B.n[2];
 	// the buffers are reallocated only when the box outgrows them, then for
 	// inference only (no training gradients) and with half again as much room
@@ -784,7 +787,7 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 	FcgSpace V;
 	const size_t boxCells = size_t(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (D == 3 ? hi[2] - lo[2] + 1 : 1);
 	if (double(boxCells) <= kBoxMaxRatio * n) {
-		if ((rc = boxSpace<D>(A64, n, mat2grid, cell, lo, hi, W, s, V, err))) return rc;
+		if ((rc = boxSpace<D>(A64, n, mat2grid, L.gn, lo, hi, W, s, V, err))) return rc;
 	} else compactSpace(W, n, V);
 	VecBox VB{};
 	for (int a = 0; a < 3; a++) {
@@ -827,7 +830,7 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 			const bool live = a < D;
 			org[a] = live ? lo[a] : 0;
 			dims[a] = live ? hi[a] - lo[a] + 1 : 1;
-			gn[a] = cell.n[a];
+			gn[a] = L.gn[a];
 		}
 		if (!M.layout(D, org, dims, gn, n, err)) return GARM_ENOMEM;
 		mgBuild(M, A32, n, mat2grid, s);
@@ -837,52 +840,50 @@ static int learned(const Rows<double> &A64, const Rows<float> &A32, const double
 		mg.bytes = mgCycleBytes(M);
 		second = &mg;
 	}
-	return flexibleCg<D>(A64, A32, b, n, V, o, K, W, s, st, err, net, true, std::min(o.max_iterations, 400), second, "learned solve");
+	return flexibleCg<D>(A64, A32, b, n, V, L, o, K, BK, W, s, st, err, net, true, std::min(o.max_iterations, 400), second, "learned solve");
 }
 
 template <int D>
-static int multigrid(const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *mat2grid, const GridDev &cell,
-	const int boxLo[3], const int boxN[3], const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Multigrid &M, hipStream_t s,
-	garm_pcg_stats *st, std::string &err)
+static int multigrid(const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const BoxLayout &L, const garm_pcg_opts &o,
+	Krylov &K, BoxKrylov &BK, LearnedWork &W, Multigrid &M, hipStream_t s, garm_pcg_stats *st, std::string &err)
 {
 	if (!W.reserve(size_t(n), err)) return GARM_ENOMEM;
+	const int *mat2grid = L.mat2grid;
 	int rc, lo[3], hi[3];
-	for (int a = 0; a < 3; a++) { lo[a] = boxLo[a]; hi[a] = boxLo[a] + boxN[a] - 1; }
+	for (int a = 0; a < 3; a++) { lo[a] = L.lo[a]; hi[a] = L.lo[a] + L.n[a] - 1; }
 	int org[3], dims[3], gn[3];
 	for (int a = 0; a < 3; a++) {
 		const bool live = a < D;
 		org[a] = live ? lo[a] : 0;
 		dims[a] = live ? hi[a] - lo[a] + 1 : 1;
-		gn[a] = cell.n[a];
+		gn[a] = L.gn[a];
 	}
 	if (!M.layout(D, org, dims, gn, n, err)) return GARM_ENOMEM;
 	mgBuild(M, A32, n, mat2grid, s);
 	FcgSpace V;
 	if (double(M.cells[0]) <= kBoxMaxRatio * n) {
-		if ((rc = boxSpace<D>(A64, n, mat2grid, cell, lo, hi, W, s, V, err))) return rc;
+		if ((rc = boxSpace<D>(A64, n, mat2grid, L.gn, lo, hi, W, s, V, err))) return rc;
 	} else compactSpace(W, n, V);
 	Precond mg;
 	mg.head = [&]() { mgApplyHead(M, n, V.r, s, V.box); };
 	mg.rest = [&]() { mgApplyRest(M, n, V.z, s, V.box); };
 	fuseCycle(mg, M, n, V, W, s);
 	mg.bytes = mgCycleBytes(M);
-	return flexibleCg<D>(A64, A32, b, n, V, o, K, W, s, st, err, mg, false, o.max_iterations, nullptr, "multigrid solve");
+	return flexibleCg<D>(A64, A32, b, n, V, L, o, K, BK, W, s, st, err, mg, false, o.max_iterations, nullptr, "multigrid solve");
 }
 
-int solveLearned(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *grid2mat, const int *mat2grid,
-	const GridDev &cell, const int boxLo[3], const int boxN[3], const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Unet &U, Multigrid &M,
-	hipStream_t s, garm_pcg_stats *st, std::string &err)
+int solveLearned(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *grid2mat, const BoxLayout &L,
+	const garm_pcg_opts &o, Krylov &K, BoxKrylov &BK, LearnedWork &W, Unet &U, Multigrid &M, hipStream_t s, garm_pcg_stats *st,
+	std::string &err)
 {
-	return D == 2 ? learned<2>(A64, A32, b, n, grid2mat, mat2grid, cell, boxLo, boxN, o, K, W, U, M, s, st, err)
-	              : learned<3>(A64, A32, b, n, grid2mat, mat2grid, cell, boxLo, boxN, o, K, W, U, M, s, st, err);
+	return D == 2 ? learned<2>(A64, A32, b, n, grid2mat, L, o, K, BK, W, U, M, s, st, err)
+	              : learned<3>(A64, A32, b, n, grid2mat, L, o, K, BK, W, U, M, s, st, err);
 }
 
-int solveMultigrid(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *mat2grid,
-	const GridDev &cell, const int boxLo[3], const int boxN[3], const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Multigrid &M,
-	hipStream_t s, garm_pcg_stats *st, std::string &err)
+int solveMultigrid(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const BoxLayout &L,
+	const garm_pcg_opts &o, Krylov &K, BoxKrylov &BK, LearnedWork &W, Multigrid &M, hipStream_t s, garm_pcg_stats *st, std::string &err)
 {
-	return D == 2 ? multigrid<2>(A64, A32, b, n, mat2grid, cell, boxLo, boxN, o, K, W, M, s, st, err)
-	              : multigrid<3>(A64, A32, b, n, mat2grid, cell, boxLo, boxN, o, K, W, M, s, st, err);
+	return D == 2 ? multigrid<2>(A64, A32, b, n, L, o, K, BK, W, M, s, st, err) : multigrid<3>(A64, A32, b, n, L, o, K, BK, W, M, s, st, err);
 }
 
 // ------------------------------------------------------------- training
@@ -1046,7 +1047,7 @@ int trainOnSystem(int D, Unet &U, const Rows<double> &A64, const double *rhs, co
 	if (!W.reserve(size_t(n), err)) return GARM_ENOMEM;
 	int rc, lo[3], hi[3];
 	if ((rc = unknownBox(mat2grid, n, cell, W, s, lo, hi, err))) return rc;
-	const UnetBox B = networkBox(D, lo, hi, cell);
+	const UnetBox B = networkBox(D, lo, hi, cell.n);
 	const size_t vox = size_t(B.n[0]) * B.n[1] * B.n[2];
 	if (!U.buf.reserve(vox, true, err) || !U.buf.setDims(B.n[0], B.n[1], B.n[2], err)) return GARM_ENOMEM;
 	float *loss = nullptr, *part = nullptr;

@@ -31,18 +31,19 @@ struct LearnedWork {
 	bool reserveBox(size_t cells, size_t unknowns, std::string &err);
 };
 
-// Solves A x = b; the solution is left in W.x.  grid2mat / mat2grid map the
-// cell grid `cell` to the compact unknowns, whose bounding box has origin
-// boxLo and sizes boxN (cells; 0 and 1 along the axes past D).
-int solveLearned(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *grid2mat, const int *mat2grid,
-	const GridDev &cell, const int boxLo[3], const int boxN[3], const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Unet &U, Multigrid &M,
-	hipStream_t s, garm_pcg_stats *st, std::string &err);
+// Solves A x = b; the solution is left in W.x.  grid2mat maps the cell grid
+// to the compact unknowns; L is the unknowns' bounding box (origin, sizes:
+// 0 and 1 along the axes past D; the cell grid's sizes; mat2grid).  What
+// the flexible iteration leaves is solved by solveJacobi (mixed) with K and
+// BK.
+int solveLearned(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *grid2mat, const BoxLayout &L,
+	const garm_pcg_opts &o, Krylov &K, BoxKrylov &BK, LearnedWork &W, Unet &U, Multigrid &M, hipStream_t s, garm_pcg_stats *st,
+	std::string &err);
 
 // Solves A x = b by flexible CG with the multigrid V-cycle preconditioner
 // (FP32 cycle, FP64 Krylov vectors and residual); the solution is left in W.x.
-int solveMultigrid(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const int *mat2grid,
-	const GridDev &cell, const int boxLo[3], const int boxN[3], const garm_pcg_opts &o, Krylov &K, LearnedWork &W, Multigrid &M,
-	hipStream_t s, garm_pcg_stats *st, std::string &err);
+int solveMultigrid(int D, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const BoxLayout &L,
+	const garm_pcg_opts &o, Krylov &K, BoxKrylov &BK, LearnedWork &W, Multigrid &M, hipStream_t s, garm_pcg_stats *st, std::string &err);
 
 // `steps` Adam steps of U on pairs made from the pressure system of a
 // projection (operator rows A64, right-hand side rhs, solution): with

@@ -82,6 +82,16 @@ constexpr int kBoxTooSparse = 1;
 int solveBoxPcg(int D, bool mixed, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const BoxLayout &L,
 	const garm_pcg_opts &o, Krylov &K, BoxKrylov &B, hipStream_t s, garm_pcg_stats *st, std::string &err);
 
+// The Jacobi PCG (FP64, or mixed) in the faster of its two forms: on the
+// box of the unknowns while the box holds at most kBoxSolverRatio2 (2D) /
+// kBoxSolverRatio3 (3D) cells per unknown and the box solver takes it (in
+// 3D it launches only the column tiles holding unknowns and declines when
+// those cover too much more than the unknowns), else on the compact rows.
+// *usedBox tells which; the solution lands in K.x.
+constexpr double kBoxSolverRatio2 = 1.5, kBoxSolverRatio3 = 16.0;
+int solveJacobi(int D, bool mixed, const Rows<double> &A64, const Rows<float> &A32, const double *b, int n, const BoxLayout &L,
+	const garm_pcg_opts &o, Krylov &K, BoxKrylov &B, hipStream_t s, garm_pcg_stats *st, std::string &err, bool *usedBox);
+
 // Launch one solver kernel `reps` times for timing (which: 0 FP64 SpMV,
 // 1 FP64 x/r update, 2 FP32 SpMV, 3 FP32 update); x and r are left unchanged.
 int timeSolverKernel(int D, const Rows<double> &A64, const Rows<float> &A32, int n, int which, int reps, Krylov &K,

@@ -343,6 +343,15 @@ static void assemble(const Fields &F, Projection &P, int n, bool mixed, hipStrea
 		faceArrays(F, F.u), faceArrays(F, F.bvel), (const double *)F.phi, (const int *)P.grid2mat, (const int *)P.mat2grid, n, P.rows, r32, P.rhs);
 }
 
+// the unknowns' bounding box of the last numbering
+static BoxLayout boxLayout(const Fields &F, const Projection &P)
+{
+	BoxLayout L{};
+	for (int a = 0; a < 3; a++) { L.lo[a] = P.box_lo[a]; L.n[a] = P.box_n[a]; L.gn[a] = F.cell.n[a]; }
+	L.mat2grid = P.mat2grid;
+	return L;
+}
+
 template <int D>
 static int projectD(Fields &F, Projection &P, const garm_pcg_opts &o, garm_pcg_stats *st, hipStream_t s, std::string &err)
 {
@@ -365,27 +374,15 @@ static int projectD(Fields &F, Projection &P, const garm_pcg_opts &o, garm_pcg_s
 	if (n > 0) {
 		assemble<D>(F, P, n, mixed, s);
 		if (learned || mg) {
-			rc = learned ? solveLearned(D, P.rows, P.rows32, P.rhs, n, P.grid2mat, P.mat2grid, F.cell, P.box_lo, P.box_n, o, P.K, P.LW, *P.net,
-			                            P.MG, s, st, err)
-			             : solveMultigrid(D, P.rows, P.rows32, P.rhs, n, P.mat2grid, F.cell, P.box_lo, P.box_n, o, P.K, P.LW, P.MG, s, st, err);
+			const BoxLayout L = boxLayout(F, P);
+			rc = learned ? solveLearned(D, P.rows, P.rows32, P.rhs, n, P.grid2mat, L, o, P.K, P.BK, P.LW, *P.net, P.MG, s, st, err)
+			             : solveMultigrid(D, P.rows, P.rows32, P.rhs, n, L, o, P.K, P.BK, P.LW, P.MG, s, st, err);
 			if ((rc == GARM_OK || rc == GARM_ENOCONV) && hipMemcpyAsync(P.K.x, P.LW.x, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, s) != hipSuccess) {
 				err = "copying the learned solution failed";
 				rc = GARM_EHIP;
 			}
 		} else {
-			// the box layout while the box holds at most kBoxSolverRatio2/3
-			// cells per unknown and (3D) the solver takes it; a sparser box
-			// keeps the compact rows
-			const double boxCells = double(P.box_n[0]) * P.box_n[1] * P.box_n[2];
-			P.used_box = boxCells <= (D == 3 ? kBoxSolverRatio3 : kBoxSolverRatio2) * n;
-			if (P.used_box) {
-				BoxLayout L{};
-				for (int a = 0; a < 3; a++) { L.lo[a] = P.box_lo[a]; L.n[a] = P.box_n[a]; L.gn[a] = F.cell.n[a]; }
-				L.mat2grid = P.mat2grid;
-				rc = solveBoxPcg(D, mixed, P.rows, P.rows32, P.rhs, n, L, o, P.K, P.BK, s, st, err);
-				if (rc == kBoxTooSparse) P.used_box = false;
-			}
-			if (!P.used_box) rc = mixed ? solveMixed(D, P.rows, P.rows32, P.rhs, n, o, P.K, s, st, err) : solvePcg64(D, P.rows, P.rhs, n, o, P.K, s, st, err);
+			rc = solveJacobi(D, mixed, P.rows, P.rows32, P.rhs, n, boxLayout(F, P), o, P.K, P.BK, s, st, err, &P.used_box);
 		}
 	}
 	// the velocity is updated with whatever the solver reached (the reference

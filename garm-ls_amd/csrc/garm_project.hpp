@@ -16,12 +16,6 @@ struct FaceFracs {
 	GridDev g[3];
 };
 
-// Largest ratio of box cells to unknowns at which the Jacobi PCG solvers
-// try the box of the unknowns instead of compact rows; in 3D the box solver
-// then launches only the column tiles holding unknowns and declines when
-// those cover too much more than the unknowns.
-constexpr double kBoxSolverRatio2 = 1.5, kBoxSolverRatio3 = 16.0;
-
 struct Projection {
 	uint8_t *flags = nullptr;                  // 1 where a cell is an unknown
 	int *grid2mat = nullptr, *mat2grid = nullptr;
