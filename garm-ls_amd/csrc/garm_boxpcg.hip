@@ -393,10 +393,6 @@ __global__ __launch_bounds__(kColX * TY) void kCgIterCol(CgArgs<T, TX> A, int k)
 {
 	constexpr int NW = kColX * TY / kWave;
 	if (A.st[kStDone]) return;
-	double al, be;
-	if (!cgScalars<NW, RESTART>(A, k, blockIdx.x == 0, al, be)) return;
-	const T alpha = T(al), beta = T(be);
-	const TX alphaX = TX(al);
 	__shared__ T plane[TY + 2][kColX + 2];
 	const int tx = int(threadIdx.x), ty = int(threadIdx.y);
 	// one workgroup per tile holding unknowns (the others stay zero in every
@@ -410,6 +406,19 @@ __global__ __launch_bounds__(kColX * TY) void kCgIterCol(CgArgs<T, TX> A, int k)
 	const bool inside = i < bx && j < by;
 	const int sy = A.O.sy, sz = A.O.sz;
 	const T *__restrict__ diag = A.O.diag;
+	// the own column's loads of planes k0 - 1 and k0 do not depend on alpha
+	// and beta: issued before the previous launch's partials are summed, so
+	// their latency overlaps that reduction
+	int c = (k0 * by + j) * bx + i;
+	T rm = 0, wm = 0, sm = 0, dm = 0, r0 = 0, w0 = 0, s0 = 0, d0 = 0, wz0 = 0;
+	if (inside) {
+		rm = A.rIn[c - sz]; wm = A.wIn[c - sz]; sm = A.sIn[c - sz]; dm = diag[c - sz];   // k0 - 1 >= -1: the zero margin
+		r0 = A.rIn[c]; w0 = A.wIn[c]; s0 = A.sIn[c]; d0 = diag[c]; wz0 = A.O.wz[c];
+	}
+	double al, be;
+	if (!cgScalars<NW, RESTART>(A, k, blockIdx.x == 0, al, be)) return;
+	const T alpha = T(al), beta = T(be);
+	const TX alphaX = TX(al);
 	auto uAt = [&](int ii, int jj, int kk) -> T {
 		if (ii < 0 || ii >= bx || jj < 0 || jj >= by) return T(0);
 		const int c = (kk * by + jj) * bx + ii;   // kk in [-1, bz]: the zero margins
@@ -423,17 +432,13 @@ __global__ __launch_bounds__(kColX * TY) void kCgIterCol(CgArgs<T, TX> A, int k)
 	else { const int t = flat - 2 * kColX; lx = t < TY ? 0 : kColX + 1; ly = 1 + (t < TY ? t : t - TY); }
 	rx = tile.x * kColX + lx - 1;
 	ry = tile.y * TY + ly - 1;
-	// own cell: plane k0 - 1 (u' only) and plane k0
-	int c = (k0 * by + j) * bx + i;
-	T um = 0, r0 = 0, d0 = 0, sn0 = 0, rn0 = 0, un0 = 0, wz0 = 0;
+	// own cell: plane k0 - 1 (u' only) and plane k0, as uAt forms them
+	T um = 0, sn0 = 0, rn0 = 0, un0 = 0;
 	if (inside) {
-		um = uAt(i, j, k0 - 1);
-		r0 = A.rIn[c];
-		d0 = diag[c];
-		sn0 = A.wIn[c] + beta * A.sIn[c];
+		um = jacobi(rm - alpha * (wm + beta * sm), dm);
+		sn0 = w0 + beta * s0;
 		rn0 = r0 - alpha * sn0;
 		un0 = jacobi(rn0, d0);
-		wz0 = A.O.wz[c];
 	}
 	double v[3] = {0, 0, 0};
 	for (int kk = k0; kk < k1; kk++, c += sz) {
