@@ -326,6 +326,46 @@ __global__ __launch_bounds__(kBlock) void kBoxSpmv(BoxOp O, const double *p, dou
 	putPart<2>(v, pa);
 }
 
+// The V-cycle's direction update fused into the next product: p = z + beta p
+// with z the cycle's level-0 iterate (its FP64 value is what a store of the
+// cycle would write) and beta = -(z.q) / (p.q) from the cycle's nzq
+// partials of z.q and the previous p.q, then q = A p and the partials of p.q
+// and r.p -- kFcgP's and kBoxSpmv's values.  The old direction is read at the
+// neighbours, so the new one goes to the other buffer.
+template <int D>
+__global__ __launch_bounds__(kBlock) void kBoxSpmvP(BoxOp O, const float *z, const double *pIn, double *pOut, double *q, const double *r,
+	const double *zq, int nzq, const double *ls, double *pa)
+{
+	double t[1];
+	sumPart<1>(zq, nzq, t);
+	double b[1] = {0};
+	if (threadIdx.x == 0) b[0] = ls[kLsPq] != 0 ? -t[0] / ls[kLsPq] : 0.0;
+	bcast<1>(b);
+	const double beta = b[0];
+	auto pv = [&](int e) { return double(z[e]) + beta * pIn[e]; };
+	const int last = O.n - 1;
+	auto P = [&](int e) { return e >= 0 && e <= last ? pv(e) : 0.0; };
+	auto Wp = [&](const double *w, int e) { return e <= last ? w[e] : 0.0; };
+	double v[2] = {0, 0};
+	for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < O.n; c += gridDim.x * blockDim.x) {
+		const double pc = pv(c);
+		double s = O.diag[c] * pc;
+		s += -O.wx[c] * P(c - 1);
+		s += -Wp(O.wx, c + 1) * P(c + 1);
+		s += -O.wy[c] * P(c - O.sy);
+		s += -Wp(O.wy, c + O.sy) * P(c + O.sy);
+		if constexpr (D == 3) {
+			s += -O.wz[c] * P(c - O.sz);
+			s += -Wp(O.wz, c + O.sz) * P(c + O.sz);
+		}
+		pOut[c] = pc;
+		q[c] = s;
+		v[0] += pc * s;
+		v[1] += r[c] * pc;
+	}
+	putPart<2>(v, pa);
+}
+
 // rt = b - A x, partial |rt|^2
 template <int D>
 __global__ __launch_bounds__(kBlock) void kBoxResidual(BoxOp O, const double *b, const double *x, double *rt, double *pb)
@@ -519,10 +559,12 @@ static UnetBox networkBox(int D, const int lo[3], const int hi[3], const int gn[
 struct Precond {
 	std::function<void()> head, rest;
 	// box layout, V-cycle: where the x/r update can write the cycle's
-	// load itself, the head that then only sweeps, and the rest whose
-	// store also forms the partials of z.q (each unset otherwise)
+	// load itself, the head that then only sweeps, and the rest that forms
+	// the partials of z.q instead of storing z (the next product forms the
+	// direction from the level-0 iterate; each unset otherwise)
 	std::function<void(float **, float **)> loadTargets;
 	std::function<void()> headLoaded, restDot;
+	std::function<const float *()> level0;   // the level-0 iterate restDot leaves (z, FP32)
 	double bytes = 0;   // algorithmic HBM bytes of one application
 	void full() const { if (head) head(); rest(); }
 };
@@ -532,6 +574,9 @@ struct Precond {
 // z.q, p update; on the box the operator is diag + three face weights (40 B),
 // on compact rows diag + six (index, weight) pairs (80 B).
 constexpr double kFcgBoxBytes = 56 + 48 + 16 + 24;
+// with the V-cycle on the box: the direction formed inside the product
+// (z 4, p in and out 16, the operator 32, r 8, q 8), x/r update
+constexpr double kFcgBoxFusedBytes = 68 + 48;
 constexpr double kFcgRowBytes = 96 + 48 + 16 + 24;
 // One network application's vector work per element (the network's own
 // activations excluded): Jacobi 24, pre-smoothed residual 56, zeroing the
@@ -606,7 +651,8 @@ static void fuseCycle(Precond &P, Multigrid &M, int n, FcgSpace &V, LearnedWork 
 	if (!V.box) return;
 	P.loadTargets = [&M](float **b32, float **x32) { mgLevel0(M, b32, x32); };
 	P.headLoaded = [&M, n, &V, s]() { mgApplyHead(M, n, V.r, s, true, true); };
-	P.restDot = [&M, n, &V, &W, s]() { mgApplyRest(M, n, V.z, s, true, V.q, W.pa, lblocks(V.nv)); };
+	P.restDot = [&M, n, &V, &W, s]() { mgApplyRest(M, n, nullptr, s, true, V.q, W.pb, lblocks(V.nv)); };
+	P.level0 = [&M]() { float *b32, *x32; mgLevel0(M, &b32, &x32); return (const float *)x32; };
 }
 
 // Flexible CG on the FP64 system, on the vectors of V, with preconditioner
@@ -670,16 +716,29 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 	};
 	precondition->full();
 	hipLaunchKernelGGL(kCopyD, gb, b1, 0, s, V.p, (const double *)V.z, N);
-	int it = 0;
+	int it = 0, fusedIts = 0;
+	// p = z + beta p still to be formed, from the V-cycle's level-0 iterate
+	// and the z.q partials in W.pb, inside the next product
+	const float *zPending = nullptr;
+	double *p = V.p;
 	for (; it < budget; it++) {
-		if (V.box) hipLaunchKernelGGL(kBoxSpmv<D>, gb, b1, 0, s, V.O, (const double *)V.p, V.q, (const double *)V.r, W.pa);
-		else hipLaunchKernelGGL(kFcgSpmv<D>, gb, b1, 0, s, A64, n, (const double *)V.p, V.q, (const double *)V.r, W.pa);
+		if (zPending) {
+			// the old direction and the new one alternate between p and z
+			// (z itself is no longer stored once the fused cycle runs)
+			double *pNew = p == V.p ? V.z : V.p;
+			hipLaunchKernelGGL(kBoxSpmvP<D>, gb, b1, 0, s, V.O, zPending, (const double *)p, pNew, V.q, (const double *)V.r,
+				(const double *)W.pb, nbb, (const double *)W.ls, W.pa);
+			p = pNew;
+			zPending = nullptr;
+			fusedIts++;
+		} else if (V.box) hipLaunchKernelGGL(kBoxSpmv<D>, gb, b1, 0, s, V.O, (const double *)p, V.q, (const double *)V.r, W.pa);
+		else hipLaunchKernelGGL(kFcgSpmv<D>, gb, b1, 0, s, A64, n, (const double *)p, V.q, (const double *)V.r, W.pa);
 		// a V-cycle on the box takes its load from the update and forms z.q
 		// in its store (the same values as its own load and the dot product)
 		const Precond *fused = V.box && precondition->loadTargets ? precondition : nullptr;
 		float *b32 = nullptr, *x32 = nullptr;
 		if (fused) fused->loadTargets(&b32, &x32);
-		hipLaunchKernelGGL(kFcgUpdate, gb, b1, 0, s, N, V.x, V.r, (const double *)V.p, (const double *)V.q, (const double *)W.pa, W.pb, W.ls,
+		hipLaunchKernelGGL(kFcgUpdate, gb, b1, 0, s, N, V.x, V.r, (const double *)p, (const double *)V.q, (const double *)W.pa, W.pb, W.ls,
 			b32, x32);
 		hipLaunchKernelGGL(kFcgReduce1, dim3(1), b1, 0, s, (const double *)W.pb, nbb, W.ls + kLsRr);
 		if (bb < 0) {
@@ -718,8 +777,11 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 			fused = nullptr;
 		} else if (fused) fused->restDot();
 		else precondition->rest();
-		if (!fused) hipLaunchKernelGGL(kFcgDot, gb, b1, 0, s, N, (const double *)V.z, (const double *)V.q, W.pa);
-		hipLaunchKernelGGL(kFcgP, gb, b1, 0, s, N, V.p, (const double *)V.z, (const double *)W.pa, (const double *)W.ls);
+		if (fused) zPending = fused->level0();
+		else {
+			hipLaunchKernelGGL(kFcgDot, gb, b1, 0, s, N, (const double *)V.z, (const double *)V.q, W.pa);
+			hipLaunchKernelGGL(kFcgP, gb, b1, 0, s, N, p, (const double *)V.z, (const double *)W.pa, (const double *)W.ls);
+		}
 	}
 	if (bb < 0 && (rc = readNorm())) return rc;
 	if (bb == 0) {
@@ -733,7 +795,8 @@ static int flexibleCg(const Rows<double> &A64, const Rows<float> &A32, const dou
 		// the application before the loop and one per iteration but the last
 		const int apps = std::max(it, 1);
 		const int firstApps = switchedAt < 0 ? apps : switchedAt + 1;
-		st->hbm_bytes = double(it) * (V.box ? kFcgBoxBytes : kFcgRowBytes) * double(N) + firstApps * first.bytes +
+		st->hbm_bytes = (double(it - fusedIts) * (V.box ? kFcgBoxBytes : kFcgRowBytes) + double(fusedIts) * kFcgBoxFusedBytes) * double(N) +
+			firstApps * first.bytes +
 			(second ? double(apps - firstApps) * second->bytes : 0.0);
 	}
 	// the solution in the compact unknowns; the FP64 residual judges it, and

@@ -584,7 +584,7 @@ __global__ __launch_bounds__(kBlock) void kMgStoreBoxDot(int cells, MgLevel L, d
 	double v[1] = {0};
 	for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += gridDim.x * blockDim.x) {
 		const double zc = double(L.x[c]);
-		z[c] = zc;
+		if (z) z[c] = zc;
 		v[0] += zc * q[c];
 	}
 	double blk[1];

@@ -67,7 +67,8 @@ void mgApply(Multigrid &M, int n, const double *r, double *z, hipStream_t s);
 // hold r and 0 (the caller wrote them where mgLevel0 says) and the head only
 // sweeps; with q (box only), the rest's store also writes the zqBlocks
 // per-workgroup partials of z.q into zq, in the order of a grid-stride dot
-// product over zqBlocks workgroups of kBlock threads.
+// product over zqBlocks workgroups of kBlock threads, and z may be null (only
+// the partials are formed; z is then level 0's iterate, where mgLevel0 says).
 void mgApplyHead(Multigrid &M, int n, const double *r, hipStream_t s, bool box = false, bool loaded = false);
 void mgApplyRest(Multigrid &M, int n, double *z, hipStream_t s, bool box = false, const double *q = nullptr, double *zq = nullptr,
 	int zqBlocks = 0);
