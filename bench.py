@@ -145,16 +145,20 @@ def run_gpu(args, mode_name, net=None):
     log(f"[bench] {mode_name}: {args.n}^3 initialised in {time.time() - t0:.1f}s")
     clk = FrameClock(g["dx"], args.cfl)
     step = 0
+    # each step returns the largest speed of the velocity it leaves (the next
+    # step's CFL input), formed on the device at its end
+    speed = sim.max_speed()
     for _ in range(args.warmup):
-        sim.step(clk.next_dt(sim.max_speed()), step, opts)
+        speed = sim.step(clk.next_dt(speed), step, opts)["max_speed"]
         step += 1
     log(f"[bench] {args.warmup} warm-up steps done")
     sim.synchronize()
     stats = []
     t0 = time.perf_counter()
     for k in range(args.steps):
-        dt = clk.next_dt(sim.max_speed())
+        dt = clk.next_dt(speed)
         stats.append(sim.step(dt, step, opts))
+        speed = stats[-1]["max_speed"]
         step += 1
         if k % 5 == 4:
             log(f"[bench] timed step {k + 1}/{args.steps}")
@@ -167,10 +171,11 @@ def run_gpu(args, mode_name, net=None):
         # (a step returns once its work is done)
         all_stats, wall = list(stats), []
         while step < args.trajectory:
-            dt = clk.next_dt(sim.max_speed())
+            dt = clk.next_dt(speed)
             t1 = time.perf_counter()
             all_stats.append(sim.step(dt, step, opts))
             wall.append(time.perf_counter() - t1)
+            speed = all_stats[-1]["max_speed"]
             step += 1
             if step % 50 == 0:
                 log(f"[bench] {mode_name}: trajectory step {step}/{args.trajectory}")

@@ -873,9 +873,14 @@ int garm_step(garm_ctx *c, double dt, int step_index, const garm_step_opts *o, g
 	launchExtrapolateAir(F, c->s);
 	doEnforceBoundary(c);
 	(void)hipEventRecord(ev[6], c->s);
+	// the largest speed of the velocity left, read back with the step's end
+	launchMaxSpeed(F, c->s);
+	if ((rc = hipCheck(c, hipMemcpyAsync(c->host_scalars, F.dscalars, sizeof(double), hipMemcpyDeviceToHost, c->s), "step"))) return rc;
+	(void)hipEventRecord(ev[7], c->s);
 	if ((rc = launched(c, "step"))) return rc;
 	// polled first, as spinWait: the step's end returns to the caller without a sleep
-	if ((rc = hipCheck(c, pollEvent(ev[6]), "step"))) return rc;
+	if ((rc = hipCheck(c, pollEvent(ev[7]), "step"))) return rc;
+	st->max_speed = c->host_scalars[0];
 	float t[7] = {};
 	for (int i = 1; i <= 6; i++) (void)hipEventElapsedTime(&t[i], ev[i - 1], ev[i]);
 	st->t_advect_ms = t[1];

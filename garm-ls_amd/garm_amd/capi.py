@@ -79,7 +79,8 @@ class StepOpts(ctypes.Structure):
 class StepStats(ctypes.Structure):
     _fields_ = [("t_advect_ms", ctypes.c_double), ("t_restart_ms", ctypes.c_double), ("t_project_ms", ctypes.c_double),
                 ("t_extrap_ms", ctypes.c_double), ("t_total_ms", ctypes.c_double), ("restarted", ctypes.c_int),
-                ("reinitialized", ctypes.c_int), ("pcg", PcgStats), ("surface_tension", SurfaceTensionStats)]
+                ("reinitialized", ctypes.c_int), ("pcg", PcgStats), ("surface_tension", SurfaceTensionStats),
+                ("max_speed", ctypes.c_double)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("pcg", "surface_tension")}

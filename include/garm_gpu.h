@@ -178,6 +178,9 @@ typedef struct garm_step_stats {
 	int reinitialized;
 	garm_pcg_stats pcg;
 	garm_surface_tension_stats surface_tension;   /* zero when off */
+	double max_speed;        /* garm_max_speed of the velocity the step leaves: the next
+	                            step's CFL input (LevelSetLiquid.h:129), formed on the
+	                            device at the step's end so no separate call is needed */
 } garm_step_stats;
 
 /* ---- context ---------------------------------------------------------- */
